@@ -1,0 +1,174 @@
+/*
+ * mpgpu.h -- C-ABI of the MI355X-native normal-matrix build and global solve for
+ * Millepede-II pede.
+ *
+ * Flat ABI: plain pointers, explicit sizes, int status codes.  No C++ or torch types.
+ * Status codes are the pede exit codes the reference would have stopped with
+ * (doc/exitCodes.md); 0 = success.  The message of the last failure on a context is
+ * returned by mpgpu_last_error().  Nothing here ever exits the process.
+ *
+ * Indexing follows the reference (src/mpmod.f90): labels are positive int32; "total index"
+ * t = 1..ntot is the rank of a label in ascending label order (globalParLabelIndex(1,:),
+ * src/mppar.f90:99,203-209); "variable index" v = 1..nvar (globalParLabelIndex(2,:), >0 for
+ * variable parameters, -1 fixed by pre-sigma, -2 fixed by the entries cut, src/pede.f90:5956-5973);
+ * Lagrange multipliers follow as nvar+1..nvar+ncon.  Packed symmetric matrices use pede's
+ * lower-row-packed order, element (i,j), i>=j, at (i*i-i)/2+j (1-based; src/pede.f90:4204).
+ *
+ * Host buffers are owned by the caller; the library copies what it needs into device memory
+ * it owns and keeps no host pointer after a call returns.  One context = one HIP stream;
+ * calls on one context must be serialised by the caller; different contexts are independent.
+ */
+#ifndef MPGPU_H
+#define MPGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpgpu_ctx mpgpu_ctx;
+
+/* Solution methods: values of pede's metsol (src/mpmod.f90:35, src/pede.f90:10962-11016) */
+enum {
+    MPGPU_INVERSION = 1,        /* method inversion      -> minver  (src/pede.f90:7838)   */
+    MPGPU_DECOMPOSITION = 3,    /* method decomposition  -> mchdec  (src/pede.f90:7951)   */
+    MPGPU_MINRES = 4,           /* method (full|sparse)MINRES -> mminrs (src/pede.f90:8243) */
+    MPGPU_MINRES_QLP = 5,       /* method (full|sparse)MINRES-QLP (src/pede.f90:8341)       */
+    MPGPU_CHOLESKY_FP32_IR = 11 /* no pede counterpart: FP32-MFMA Cholesky + FP64 refinement */
+};
+
+/* Matrix storage: pede's matsto (src/mpmod.f90:36) */
+enum { MPGPU_FULL = 1, MPGPU_SPARSE = 2 };
+
+/* Steering options used by the path; defaults are pede's (src/mpmod.f90:33-126).
+ * Fill with mpgpu_default_options(). */
+typedef struct mpgpu_options {
+    int32_t metsol;          /* solution method (enum above)                          */
+    int32_t matsto;          /* MPGPU_FULL or MPGPU_SPARSE                            */
+    int32_t mbandw;          /* 'bandwidth': 0 diagonal, >0 band preconditioner, <0 none */
+    int32_t lprecm;          /* 'bandwidth m 1': skyline (rank preserving) band        */
+    int32_t lhuber;          /* 'outlierdownweighting n'                              */
+    int32_t lfitbb;          /* local fit: check bordered-band structure (1)          */
+    int32_t mreqenf;         /* 'entries f': min entries from binary files (25)       */
+    int32_t mreqena;         /* 'entries f a': min entries from accepted fits (10)    */
+    int32_t mcount;          /* 'countrecords': count per record (1) / equation (0)   */
+    int32_t itnlim;          /* MINRES iteration limit (2000)                         */
+    int32_t withmultipliers; /* constraints by Lagrange multipliers (1) or not (0)     */
+    int32_t mrmode;          /* MINRES-QLP mode                                       */
+    float chicut;            /* 'chisqcut' first factor (0 = none)                    */
+    float chirem;            /* 'chisqcut' second factor                              */
+    float chhuge;            /* 'hugecut' (50)                                        */
+    float dwcut;             /* 'dwfractioncut' (0)                                   */
+    float cndlmx;            /* 'maxlocalcond' (100)                                  */
+    float regula;            /* 'regularisation' factor (1)                           */
+    double default_presigma; /* 'presigma' (0)                                        */
+    double mrestl;           /* 'mrestol' (1e-6)                                      */
+    double mrtcnd;           /* 'mrestranscond' (1e7)                                 */
+    int32_t nregul;          /* regularisation flag                                   */
+    int32_t refine_steps;    /* FP32+IR: maximum FP64 refinement steps (5)            */
+    int32_t reserved[6];
+} mpgpu_options;
+
+/* Statistics of one data pass (loopn) */
+typedef struct mpgpu_pass_stats {
+    double chi2;         /* Sum(chi^2) of the pass (getsums fvalue, src/pede.f90:11699)   */
+    int64_t ndf;         /* Sum(ndf)                                                      */
+    double wndf;         /* weighted Sum(ndf)                                             */
+    int64_t nrej[6];     /* rejects: NaN, rank, cond(band), ndf<=0, huge chi2, chi2 cut   */
+    int64_t naccept;     /* accepted local fits                                           */
+    int64_t nrec;        /* records processed                                             */
+    double ms_kernel;    /* device time of the record kernel (HIP events)                 */
+    double ms_total;     /* device time of the whole pass                                 */
+} mpgpu_pass_stats;
+
+/* Statistics of one solve */
+typedef struct mpgpu_solve_stats {
+    int32_t method;      /* method actually used                                          */
+    int32_t nrank;       /* rank found (inversion / decomposition)                        */
+    int32_t istop;       /* MINRES stop code (src/minresModule.f90:591-600)               */
+    int32_t itn;         /* MINRES iterations / refinement steps                          */
+    double anorm, acond, rnorm, arnorm, ynorm;
+    double rel_residual; /* ||A x - b|| / ||b||, evaluated in FP64 on the device           */
+    double ms;           /* wall time of the solve (device-synchronised)                  */
+} mpgpu_solve_stats;
+
+/* ---- context ---------------------------------------------------------------------------- */
+void mpgpu_default_options(mpgpu_options *opt);
+/* create a context bound to HIP device `device`; no pede counterpart (pede's state is mpmod) */
+int mpgpu_create(const mpgpu_options *opt, int32_t device, mpgpu_ctx **out);
+int mpgpu_destroy(mpgpu_ctx *ctx);
+const char *mpgpu_last_error(const mpgpu_ctx *ctx);
+int mpgpu_set_options(mpgpu_ctx *ctx, const mpgpu_options *opt);
+
+/* ---- input: Mille records (replaces readFromBinary + read_mille, src/pede.f90:1714-2075) --
+ * Records are appended in order.  Record r occupies words [rec_ptr[r], rec_ptr[r+1]) of
+ * inder/glder exactly as they are stored in a Mille record (word 1 = (0.0, error count),
+ * src/mille.f90:27-59).  Exactly one of glder_f (FP32 payload) / glder_d (FP64) is non-NULL.
+ * Labels in global words stay raw labels. */
+int mpgpu_add_records(mpgpu_ctx *ctx, int64_t nrec, const int64_t *rec_ptr, const int32_t *inder,
+                      const float *glder_f, const double *glder_d);
+/* Read a whole Mille binary file (C binary, or Fortran sequential when fortran != 0; gzip
+ * and ROOT files are not supported) and append its records. */
+int mpgpu_read_mille_file(mpgpu_ctx *ctx, const char *path, int32_t fortran, int64_t *nrec_out);
+
+/* ---- parameters from steering (Parameter lists, src/pede.f90:5925-5950) --------------------
+ * labels[k], start value[k], presigma[k] (0 = none, <0 fixed, >0 pre-sigma). */
+int mpgpu_set_steering_parameters(mpgpu_ctx *ctx, int32_t n, const int32_t *labels, const double *values,
+                                  const double *presigma);
+/* Constraint or Measurement blocks (src/pede.f90:10916-11075).  Block c has rhs[c], sigma[c]
+ * and entries [ptr[c], ptr[c+1]) of (labels, factors).  kind: 0 constraint, 1 measurement. */
+int mpgpu_add_equations(mpgpu_ctx *ctx, int32_t kind, int32_t nblock, const int64_t *ptr, const int32_t *labels,
+                        const double *factors, const double *rhs, const double *sigma);
+
+/* ---- structure (loop1 + loop2, src/pede.f90:5777-7527; src/mppar.f90; src/mpbits.f90) ----
+ * Builds label -> index maps, entry counts, variable/fixed flags, parameter groups and the
+ * sparsity structure of the global matrix; compiles the records into device layout. */
+int mpgpu_build_structure(mpgpu_ctx *ctx, int32_t *ntot_out, int32_t *nvar_out, int32_t *ncon_out);
+/* globalParLabelIndex(1:2,:), globalParLabelCounter, group start labels (ntot each) */
+int mpgpu_get_labels(const mpgpu_ctx *ctx, int32_t *labels, int32_t *var_index, int64_t *counters,
+                     int32_t *group_first_label);
+/* stored elements of the global matrix (matrix words), number of group blocks */
+int mpgpu_matrix_info(const mpgpu_ctx *ctx, int64_t *stored_words, int64_t *nblocks, int32_t *ngroups);
+
+/* ---- state ------------------------------------------------------------------------------- */
+int mpgpu_set_parameters(mpgpu_ctx *ctx, const double *par_by_total_index);   /* globalParameter */
+int mpgpu_get_parameters(const mpgpu_ctx *ctx, double *par_by_total_index);
+
+/* ---- data pass: loopn + loopbf (src/pede.f90:2479-2885, 3387-4442) -------------------------
+ * icalcm = 1 builds matrix + vector + chi2, 0 vector + chi2 only.  nloopn is pede's pass
+ * counter (1 for the first pass: no down-weighting), chicut the cut factor in force. Adds the
+ * pre-sigma diagonal (src/pede.f90:2711-2718) and external measurements, then the Lagrange
+ * rows and constraint residuals (addcst, src/pede.f90:704-771). */
+int mpgpu_loopn(mpgpu_ctx *ctx, int32_t icalcm, int32_t nloopn, float chicut, mpgpu_pass_stats *st);
+/* globalVector (nall) and globalCounter (nvar) of the last pass */
+int mpgpu_get_vector(const mpgpu_ctx *ctx, double *vec, int64_t *counter);
+/* whole matrix as packed nall(nall+1)/2 FP64 (tests; small nall only) */
+int mpgpu_get_matrix_packed(const mpgpu_ctx *ctx, double *packed, int64_t len);
+/* mupdat: add v[k] to element (i[k], j[k]) (1-based, any order; src/pede.f90:3134-3204) */
+int mpgpu_mupdat(mpgpu_ctx *ctx, int32_t nel, const int32_t *i, const int32_t *j, const double *v);
+
+/* ---- global solve (xloopn metsol dispatch, src/pede.f90:8822-8855) -------------------------
+ * Solves A x = rhs for the matrix of the last icalcm=1 pass.  rhs may be NULL (use the
+ * globalVector of the last pass, as xloopn copies it into globalCorrections).  x: nall. */
+int mpgpu_solve(mpgpu_ctx *ctx, const double *rhs, double *x, mpgpu_solve_stats *st);
+/* y = A x (avprod, src/pede.f90:5331-5358), x,y host arrays of nall */
+int mpgpu_avprod(mpgpu_ctx *ctx, const double *x, double *y);
+/* parameter errors sqrt|A^-1_ii| and global correlations (prtglo, src/pede.f90:4538-4546);
+ * only after an inversion solve.  err, gcor: nvar (gcor may be NULL) */
+int mpgpu_get_errors(const mpgpu_ctx *ctx, double *err, double *gcor);
+
+/* ---- complete pede-like run: steering file -> millepede.res (src/pede.f90:33-528) -------- */
+typedef struct mpgpu_run_stats {
+    int32_t ntot, nvar, ncon, iterations;
+    int64_t nrec;
+    double final_chi2;
+    int64_t final_ndf;
+    double ms_structure, ms_passes, ms_solves;
+    int32_t exit_code;
+} mpgpu_run_stats;
+int mpgpu_run_steering(const char *steering_file, const char *output_dir, int32_t device, mpgpu_run_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,176 @@
+!> ISO_C_BINDING interface to libmpgpu (include/mpgpu.h) for a Fortran pede host.
+!!
+!! A pede-style host (src/pede.f90 with module mpmod) uses these interfaces to replace
+!! loopn/loopbf (mpgpu_loopn), mupdat (mpgpu_mupdat), avprod (mpgpu_avprod) and the metsol
+!! dispatch of xloopn (mpgpu_solve).  All arguments are passed as C pointers/values; no hidden
+!! string lengths and no assumed-shape arrays cross the boundary.
+MODULE mpgpu_mod
+    USE, INTRINSIC :: iso_c_binding
+    IMPLICIT NONE
+
+    TYPE, BIND(C) :: mpgpu_options
+        INTEGER(c_int32_t) :: metsol, matsto, mbandw, lprecm, lhuber, lfitbb, mreqenf, mreqena, &
+                              mcount, itnlim, withmultipliers, mrmode
+        REAL(c_float)      :: chicut, chirem, chhuge, dwcut, cndlmx, regula
+        REAL(c_double)     :: default_presigma, mrestl, mrtcnd
+        INTEGER(c_int32_t) :: nregul, refine_steps
+        INTEGER(c_int32_t) :: reserved(6)
+    END TYPE mpgpu_options
+
+    TYPE, BIND(C) :: mpgpu_pass_stats
+        REAL(c_double)     :: chi2
+        INTEGER(c_int64_t) :: ndf
+        REAL(c_double)     :: wndf
+        INTEGER(c_int64_t) :: nrej(6), naccept, nrec
+        REAL(c_double)     :: ms_kernel, ms_total
+    END TYPE mpgpu_pass_stats
+
+    TYPE, BIND(C) :: mpgpu_solve_stats
+        INTEGER(c_int32_t) :: method, nrank, istop, itn
+        REAL(c_double)     :: anorm, acond, rnorm, arnorm, ynorm, rel_residual, ms
+    END TYPE mpgpu_solve_stats
+
+    TYPE, BIND(C) :: mpgpu_run_stats
+        INTEGER(c_int32_t) :: ntot, nvar, ncon, iterations
+        INTEGER(c_int64_t) :: nrec
+        REAL(c_double)     :: final_chi2
+        INTEGER(c_int64_t) :: final_ndf
+        REAL(c_double)     :: ms_structure, ms_passes, ms_solves
+        INTEGER(c_int32_t) :: exit_code
+    END TYPE mpgpu_run_stats
+
+    INTERFACE
+        SUBROUTINE mpgpu_default_options(opt) BIND(C, name='mpgpu_default_options')
+            IMPORT :: mpgpu_options
+            TYPE(mpgpu_options), INTENT(OUT) :: opt
+        END SUBROUTINE
+        INTEGER(c_int) FUNCTION mpgpu_create(opt, device, ctx) BIND(C, name='mpgpu_create')
+            IMPORT :: c_int, c_int32_t, c_ptr, mpgpu_options
+            TYPE(mpgpu_options), INTENT(IN) :: opt
+            INTEGER(c_int32_t), VALUE :: device
+            TYPE(c_ptr), INTENT(OUT) :: ctx
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_destroy(ctx) BIND(C, name='mpgpu_destroy')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+        END FUNCTION
+        TYPE(c_ptr) FUNCTION mpgpu_last_error(ctx) BIND(C, name='mpgpu_last_error')
+            IMPORT :: c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_set_options(ctx, opt) BIND(C, name='mpgpu_set_options')
+            IMPORT :: c_int, c_ptr, mpgpu_options
+            TYPE(c_ptr), VALUE :: ctx
+            TYPE(mpgpu_options), INTENT(IN) :: opt
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_add_records(ctx, nrec, rec_ptr, inder, glder_f, glder_d) &
+                BIND(C, name='mpgpu_add_records')
+            IMPORT :: c_int, c_ptr, c_int64_t
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int64_t), VALUE :: nrec
+            TYPE(c_ptr), VALUE :: rec_ptr, inder, glder_f, glder_d
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_read_mille_file(ctx, path, fortran, nrec) BIND(C, name='mpgpu_read_mille_file')
+            IMPORT :: c_int, c_ptr, c_char, c_int32_t, c_int64_t
+            TYPE(c_ptr), VALUE :: ctx
+            CHARACTER(KIND=c_char), DIMENSION(*), INTENT(IN) :: path
+            INTEGER(c_int32_t), VALUE :: fortran
+            INTEGER(c_int64_t), INTENT(OUT) :: nrec
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_set_steering_parameters(ctx, n, labels, values, presigma) &
+                BIND(C, name='mpgpu_set_steering_parameters')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int32_t), VALUE :: n
+            INTEGER(c_int32_t), INTENT(IN) :: labels(*)
+            REAL(c_double), INTENT(IN) :: values(*), presigma(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_add_equations(ctx, kind, nblock, ptr, labels, factors, rhs, sigma) &
+                BIND(C, name='mpgpu_add_equations')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int32_t), VALUE :: kind, nblock
+            INTEGER(c_int64_t), INTENT(IN) :: ptr(*)
+            INTEGER(c_int32_t), INTENT(IN) :: labels(*)
+            REAL(c_double), INTENT(IN) :: factors(*), rhs(*), sigma(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_build_structure(ctx, ntot, nvar, ncon) BIND(C, name='mpgpu_build_structure')
+            IMPORT :: c_int, c_ptr, c_int32_t
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int32_t), INTENT(OUT) :: ntot, nvar, ncon
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_get_labels(ctx, labels, var_index, counters, group_first_label) &
+                BIND(C, name='mpgpu_get_labels')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_int64_t
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int32_t), INTENT(OUT) :: labels(*), var_index(*), group_first_label(*)
+            INTEGER(c_int64_t), INTENT(OUT) :: counters(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_matrix_info(ctx, stored, nblocks, ngroups) BIND(C, name='mpgpu_matrix_info')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_int64_t
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int64_t), INTENT(OUT) :: stored, nblocks
+            INTEGER(c_int32_t), INTENT(OUT) :: ngroups
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_set_parameters(ctx, par) BIND(C, name='mpgpu_set_parameters')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(IN) :: par(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_get_parameters(ctx, par) BIND(C, name='mpgpu_get_parameters')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(OUT) :: par(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_loopn(ctx, icalcm, nloopn, chicut, st) BIND(C, name='mpgpu_loopn')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_float, mpgpu_pass_stats
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int32_t), VALUE :: icalcm, nloopn
+            REAL(c_float), VALUE :: chicut
+            TYPE(mpgpu_pass_stats), INTENT(OUT) :: st
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_get_vector(ctx, vec, counter) BIND(C, name='mpgpu_get_vector')
+            IMPORT :: c_int, c_ptr, c_double, c_int64_t
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(OUT) :: vec(*)
+            INTEGER(c_int64_t), INTENT(OUT) :: counter(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_get_matrix_packed(ctx, packed, len) BIND(C, name='mpgpu_get_matrix_packed')
+            IMPORT :: c_int, c_ptr, c_double, c_int64_t
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(OUT) :: packed(*)
+            INTEGER(c_int64_t), VALUE :: len
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_mupdat(ctx, nel, i, j, v) BIND(C, name='mpgpu_mupdat')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int32_t), VALUE :: nel
+            INTEGER(c_int32_t), INTENT(IN) :: i(*), j(*)
+            REAL(c_double), INTENT(IN) :: v(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_solve(ctx, rhs, x, st) BIND(C, name='mpgpu_solve')
+            IMPORT :: c_int, c_ptr, c_double, mpgpu_solve_stats
+            TYPE(c_ptr), VALUE :: ctx
+            TYPE(c_ptr), VALUE :: rhs
+            REAL(c_double), INTENT(OUT) :: x(*)
+            TYPE(mpgpu_solve_stats), INTENT(OUT) :: st
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_avprod(ctx, x, y) BIND(C, name='mpgpu_avprod')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(IN) :: x(*)
+            REAL(c_double), INTENT(OUT) :: y(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_get_errors(ctx, err, gcor) BIND(C, name='mpgpu_get_errors')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(OUT) :: err(*), gcor(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_run_steering(steer, outdir, device, st) BIND(C, name='mpgpu_run_steering')
+            IMPORT :: c_int, c_char, c_int32_t, mpgpu_run_stats
+            CHARACTER(KIND=c_char), DIMENSION(*), INTENT(IN) :: steer, outdir
+            INTEGER(c_int32_t), VALUE :: device
+            TYPE(mpgpu_run_stats), INTENT(OUT) :: st
+        END FUNCTION
+    END INTERFACE
+END MODULE mpgpu_mod

@@ -1,0 +1,564 @@
+// Record-loop kernel: one 64-lane wavefront per Mille record.
+//
+// Restates loopbf (src/pede.f90:3387-4442) for records whose local fit takes the full
+// (non-banded) path: residuals after the current global parameters, local normal equations,
+// symmetric Gauss-Jordan inversion with diagonal pivoting (sqminv, src/mpnum.f90:97-213),
+// chi^2 / down-weighting / reject logic, then the "Millepede" update of the global system:
+//   globalVector += dw * w * r * g                       (src/pede.f90:4184-4186)
+//   globalMatrix += dw * w * g g^T  (per equation)       (src/pede.f90:4188-4201, 3336-3356)
+//   globalMatrix -= G Gamma^-1 G^T  (Schur complement)   (dbavat, src/mpnum.f90:1389-1447)
+// The per-element arithmetic of the local fit follows the reference expression by expression
+// (same operand order), so local-fit decisions (rank, chi^2 cuts) match pede; sums across
+// equations that pede forms sequentially are formed sequentially here as well, except the
+// chi^2 and weight sums, which are wavefront reductions.  Matrix contributions are scattered
+// with FP64 atomics, either into pede's packed full matrix or into the group-block layout.
+#include "mpgpu_device.h"
+#include <float.h>
+
+namespace mpgpu {
+
+#define WAVE 64
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+template <typename T> __device__ __forceinline__ double wval(const DevRecords &R, int64_t w);
+template <> __device__ __forceinline__ double wval<float>(const DevRecords &R, int64_t w) { return (double)R.vf[w]; }
+template <> __device__ __forceinline__ double wval<double>(const DevRecords &R, int64_t w) { return R.vd[w]; }
+
+// packed index helpers (0-based storage of 1-based (i,j), i>=j)
+__device__ __forceinline__ int pk(int i, int j) { return (i * i - i) / 2 + j - 1; }
+__device__ __forceinline__ int64_t pk64(int64_t i, int64_t j) { return (i * i - i) / 2 + j - 1; }
+
+// address of element (va >= vb) (1-based variable indices) in the block layout;
+// segbase: block id of the record-local segment pair, fa/fb first variable of the groups
+__device__ __forceinline__ int64_t block_addr(const DevModel &M, int32_t blk, int va, int vb, int ga, int gb)
+{
+    int fa = M.gfirst[ga], fb = M.gfirst[gb];
+    int nc = M.gfirst[gb + 1] - fb;
+    return M.boff[blk] + (int64_t)(va - fa) * nc + (vb - fb);
+}
+
+// binary search of column group gb in block row ga
+__device__ int32_t find_block(const DevModel &M, int ga, int gb)
+{
+    int64_t lo = M.brow[ga], hi = M.brow[ga + 1] - 1;
+    while (lo <= hi) {
+        int64_t mid = (lo + hi) >> 1;
+        int c = M.bcol[mid];
+        if (c == gb) return (int32_t)mid;
+        if (c < gb) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void mat_add(const DevModel &M, int va, int vb, double v, const int32_t *seg_of,
+                                        const int32_t *seg_grp, const int32_t *pblk, int pos_a, int pos_b)
+{
+    // va >= vb; pos_a/pos_b are the positions of va/vb in the record's variable list
+    if (!M.sparse) {
+        atomicAdd(&M.mat[pk64(va, vb)], v);
+    } else {
+        int sa = seg_of[pos_a], sb = seg_of[pos_b];
+        int blk = pblk[(sa * sa + sa) / 2 + sb];
+        if (blk < 0) return;
+        atomicAdd(&M.mat[block_addr(M, blk, va, vb, seg_grp[sa], seg_grp[sb])], v);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, PassOpts o, RecOut out, LdsPlan P,
+                                                  int64_t rec_begin, int64_t rec_end)
+{
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const double chuber = 1.345, cauchy = 2.3849;
+
+    // ---- carve LDS ----
+    double *gp = lds;                        // [max_gl] parameter value per distinct label
+    double *rm = gp + P.max_gl;              // [max_eq] corrected measurement / residual
+    double *re = rm + P.max_eq;              // [max_eq] sigma
+    double *lc = re + P.max_eq;              // [max_eq] local-fit prediction
+    double *lw = lc + P.max_eq;              // [max_eq] weight of the last local-fit pass
+    const int npk_max = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
+    double *cl = lw + P.max_eq;              // [npk_max] local matrix -> inverse
+    double *bl = cl + npk_max;               // [nalc_max] local vector -> solution
+    double *dg = bl + P.nalc_max;            // [nalc_max] |diag| for pivot test
+    double *cs = dg + P.nalc_max;            // [nalc_max] scaled pivot column
+    double *co = cs + P.nalc_max;            // [nalc_max] original pivot column
+    double *G = co + P.nalc_max;             // [max_var * nalc_max]
+    double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max]
+    int32_t *gv = (int32_t *)(Gh + (size_t)P.max_var * P.nalc_max); // [max_gl] var index
+    int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
+    int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
+    int32_t *segof = gvl + P.max_var;        // [max_var] segment of each record variable
+    int32_t *seggr = segof + P.max_var;      // [max_var] group of each segment
+    int32_t *pblk = seggr + P.max_var;       // [max_pair] block id per segment pair
+    int32_t *nxt = pblk + P.max_pair;        // [nalc_max+1] pivot list
+    int32_t *ibh = nxt + P.nalc_max + 1;     // [2*nalc_max+2] band-width histogram
+    int32_t *sh = ibh + 2 * P.nalc_max + 2;  // scalars
+
+    for (int64_t r = rec_begin + blockIdx.x; r < rec_end; r += gridDim.x) {
+        const int64_t e0 = R.rec_eq[r];
+        const int neq = (int)(R.rec_eq[r + 1] - e0);
+        const int64_t g0 = R.rec_gl[r];
+        const int ngl = (int)(R.rec_gl[r + 1] - g0);
+        const double dw1 = R.rec_w ? (double)R.rec_w[r] : 1.0;
+        const double dw2 = sqrt(dw1);
+        if (neq > P.max_eq || ngl > P.max_gl) {
+            if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            continue;
+        }
+        // ---- distinct labels: parameter value, variable index, position among variables ----
+        int nalg = 0;
+        for (int base = 0; base < ngl; base += WAVE) {
+            int k = base + lane;
+            int v = 0;
+            if (k < ngl) {
+                int t = R.gl_tix[g0 + k];
+                gp[k] = M.par[t];
+                v = M.var[t];
+                gv[k] = v;
+            }
+            bool isv = (k < ngl) && v > 0;
+            unsigned long long m = __ballot(isv);
+            int pos = nalg + __popcll(m & ((1ull << lane) - 1ull));
+            if (k < ngl) gpos[k] = isv ? pos : -1;
+            if (isv && pos < P.max_var) gvl[pos] = v;
+            nalg += __popcll(m);
+        }
+        if (nalg > P.max_var) {
+            if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            continue;
+        }
+        __syncthreads();
+        // ---- first loop: subtract global contribution, count local parameters (3655-3693) ----
+        int nalc = 0;
+        for (int i = lane; i < neq; i += WAVE) {
+            int64_t sl = R.eq_sl[e0 + i];
+            uint32_t en = R.eq_n[e0 + i];
+            int nl = en & 0xffff, ng = en >> 16;
+            int64_t sg = sl + nl + 1;
+            double rmeas = wval<T>(R, sl);
+            for (int j = 1; j <= ng; j++) rmeas = rmeas - wval<T>(R, sg + j) * gp[R.widx[sg + j]];
+            rm[i] = rmeas;
+            re[i] = wval<T>(R, sg);
+            lc[i] = 0.0;
+            for (int j = 1; j <= nl; j++) nalc = max(nalc, R.widx[sl + j]);
+        }
+        nalc = wave_max(nalc);
+        if (nalc <= 0) {
+            if (lane == 0) { out.status[r] = 7; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            __syncthreads();
+            continue;
+        }
+        if (nalc > P.nalc_max) {
+            if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            __syncthreads();
+            continue;
+        }
+        const int npk = (nalc * nalc + nalc) / 2;
+        // ---- bordered-band structure check (3815-3874), first iteration only ----
+        bool band = false;
+        if (nalc > 5 && o.lfitbb > 0) {
+            for (int i = lane; i < 2 * nalc + 2; i += WAVE) ibh[i] = 0;
+            __syncthreads();
+            for (int i = lane; i < neq; i += WAVE) {
+                int64_t sl = R.eq_sl[e0 + i];
+                int nl = R.eq_n[e0 + i] & 0xffff;
+                for (int j = 1; j <= nl; j++) {
+                    int ij = R.widx[sl + j];
+                    for (int k = 1; k <= j; k++) {
+                        int ik = R.widx[sl + k];
+                        int id = abs(ij - ik) + 1;
+                        atomicMax(&ibh[id], min(ij, ik));
+                        atomicMax(&ibh[nalc + id], min(nalc + 1 - ij, nalc + 1 - ik));
+                    }
+                }
+            }
+            __syncthreads();
+            if (lane == 0) {
+                int kx = -1;
+                long long icmn = (long long)nalc * nalc * nalc;
+                int kbdr = 0;
+                for (int k = nalc; k >= 2; k--) {
+                    kbdr = max(kbdr, ibh[k]);
+                    long long t = (k - 2) + kbdr + 1;
+                    long long icost = 6LL * (nalc - kbdr) * t * t + 2LL * kbdr * kbdr * kbdr;
+                    if (icost < icmn) { icmn = icost; kx = k; }
+                }
+                if (kx < 0) {
+                    kbdr = 0;
+                    for (int k = nalc; k >= 2; k--) {
+                        kbdr = max(kbdr, ibh[k + nalc]);
+                        long long t = (k - 2) + kbdr + 1;
+                        long long icost = 6LL * (nalc - kbdr) * t * t + 2LL * kbdr * kbdr * kbdr;
+                        if (icost < icmn) { icmn = icost; kx = k; }
+                    }
+                }
+                sh[0] = kx > 0 ? 1 : 0;
+            }
+            __syncthreads();
+            band = sh[0] != 0;
+        }
+        if (band) {
+            if (lane == 0) { out.status[r] = 8; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            __syncthreads();
+            continue;
+        }
+        // ---- outlier-suppression iterations (3759-4094) ----
+        const int nter = (o.nloopn != 1 && o.lhuber != 0) ? o.lhuber : 1;
+        int status = 0, nrank = 0, ndf = 0;
+        double summ = 0.0;
+        float resing = 0.0f;
+        for (int iter = 1; iter <= nter; iter++) {
+            // weights
+            for (int i = lane; i < neq; i += WAVE) {
+                double rerr = re[i];
+                double wght = 1.0 / (rerr * rerr);
+                double resid = rm[i] - lc[i];
+                if (o.nloopn != 1 && iter != 1 && o.lhuber != 0) {
+                    if (iter <= 3) {
+                        if (fabs(resid) > chuber * rerr) wght = wght * chuber * rerr / fabs(resid);
+                    } else {
+                        double t = resid / rerr / cauchy;
+                        wght = wght / (1.0 + t * t);
+                    }
+                }
+                lw[i] = wght;
+            }
+            __syncthreads();
+            // local normal equations, element-parallel, equations in order (3781-3836)
+            for (int t = lane; t < npk + nalc; t += WAVE) {
+                int J, K;
+                if (t < npk) {
+                    J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
+                    while ((J * J - J) / 2 > t) J--;
+                    while ((J * J + J) / 2 <= t) J++;
+                    K = t - (J * J - J) / 2 + 1;
+                } else {
+                    J = t - npk + 1;
+                    K = 0;
+                }
+                double acc = 0.0;
+                for (int i = 0; i < neq; i++) {
+                    int64_t sl = R.eq_sl[e0 + i];
+                    int nl = R.eq_n[e0 + i] & 0xffff;
+                    double dj = 0.0, dk = 0.0;
+                    bool hj = false, hk = false;
+                    for (int j = 1; j <= nl; j++) {
+                        int ij = R.widx[sl + j];
+                        if (ij == J) { dj = wval<T>(R, sl + j); hj = true; }
+                        if (K > 0 && ij == K) { dk = wval<T>(R, sl + j); hk = true; }
+                    }
+                    if (K == 0) {
+                        if (hj) acc = acc + lw[i] * rm[i] * dj;
+                    } else if (hj && hk) {
+                        acc = acc + lw[i] * dj * dk;
+                    }
+                }
+                if (K == 0) bl[J - 1] = acc; else cl[t] = acc;
+            }
+            __syncthreads();
+            // ---- sqminv (src/mpnum.f90:97-213), element-parallel Gauss-Jordan ----
+            const double eps = 16.0 * DBL_EPSILON;
+            for (int i = lane; i < nalc; i += WAVE) {
+                dg[i] = fabs(cl[pk(i + 1, i + 1)]);
+                nxt[i + 1] = i + 2;
+            }
+            __syncthreads();
+            if (lane == 0) { nxt[nalc] = -1; sh[1] = 1; /* next0 */ sh[2] = 0; /* nrank */ }
+            __syncthreads();
+            for (int step = 1; step <= nalc; step++) {
+                if (lane == 0) {
+                    int k = 0, l = 0, last = 0;
+                    double vkk = 0.0;
+                    int j = sh[1];
+                    while (j > 0) {
+                        double vjj = cl[pk(j, j)];
+                        double m = fmax(fabs(vkk), eps * dg[j - 1]);
+                        if (fabs(vjj) > m) { vkk = vjj; k = j; l = last; }
+                        last = j;
+                        j = nxt[last];
+                    }
+                    if (k != 0) {
+                        if (l == 0) sh[1] = nxt[k]; else nxt[l] = nxt[k];
+                        nxt[k] = 0;
+                        sh[2] = sh[2] + 1;
+                    }
+                    sh[3] = k;
+                }
+                __syncthreads();
+                const int k = sh[3];
+                if (k == 0) {
+                    for (int t = lane; t < npk; t += WAVE) {
+                        int J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
+                        while ((J * J - J) / 2 > t) J--;
+                        while ((J * J + J) / 2 <= t) J++;
+                        int K = t - (J * J - J) / 2 + 1;
+                        if (nxt[J] != 0 && nxt[K] != 0) cl[t] = 0.0;
+                    }
+                    for (int i = lane; i < nalc; i += WAVE) if (nxt[i + 1] != 0) bl[i] = 0.0;
+                    __syncthreads();
+                    break;
+                }
+                const int kk = pk(k, k);
+                const double vkk = 1.0 / cl[kk];
+                for (int j = lane + 1; j <= nalc; j += WAVE) {
+                    if (j != k) {
+                        double v = cl[j < k ? pk(k, j) : pk(j, k)];
+                        co[j - 1] = v;
+                        cs[j - 1] = vkk * v;
+                    }
+                }
+                __syncthreads();
+                const double bk = bl[k - 1] * vkk;
+                for (int t = lane; t < npk; t += WAVE) {
+                    int J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
+                    while ((J * J - J) / 2 > t) J--;
+                    while ((J * J + J) / 2 <= t) J++;
+                    int K = t - (J * J - J) / 2 + 1;
+                    if (J == k && K == k) cl[t] = -vkk;
+                    else if (J == k) cl[t] = cs[K - 1];
+                    else if (K == k) cl[t] = cs[J - 1];
+                    else cl[t] = cl[t] - cs[K - 1] * co[J - 1];
+                }
+                for (int j = lane + 1; j <= nalc; j += WAVE) {
+                    if (j == k) bl[j - 1] = bk;
+                    else bl[j - 1] = bl[j - 1] - bk * co[j - 1];
+                }
+                __syncthreads();
+            }
+            for (int t = lane; t < npk; t += WAVE) cl[t] = -cl[t];
+            nrank = sh[2];
+            __syncthreads();
+            // NaN check (3903-3906)
+            int nan = 0;
+            for (int i = lane; i < nalc; i += WAVE) {
+                double b = bl[i];
+                if (!(b <= 0.0) && !(b > 0.0)) nan++;
+            }
+            nan = wave_max(nan);
+            // ---- third loop: residuals, chi^2 (3919-4035) ----
+            double s_chi = 0.0, s_wt = 0.0;
+            for (int i = lane; i < neq; i += WAVE) {
+                int64_t sl = R.eq_sl[e0 + i];
+                int nl = R.eq_n[e0 + i] & 0xffff;
+                double rerr = re[i];
+                double wght = 1.0 / (rerr * rerr);
+                double rmloc = 0.0;
+                for (int j = 1; j <= nl; j++) rmloc = rmloc + wval<T>(R, sl + j) * bl[R.widx[sl + j] - 1];
+                lc[i] = rmloc;
+                double rmeas = rm[i] - rmloc;
+                double dchi2 = wght * rmeas * rmeas;
+                double resid = rmeas;
+                if (o.nloopn != 1 && iter != 1 && o.lhuber != 0) {
+                    if (iter <= 3) {
+                        if (fabs(resid) > chuber * rerr) {
+                            wght = wght * chuber * rerr / fabs(resid);
+                            dchi2 = 2.0 * chuber * (fabs(resid) / rerr - 0.5 * chuber);
+                        }
+                    } else {
+                        double t = resid / rerr / cauchy;
+                        wght = wght / (1.0 + t * t);
+                        dchi2 = log(1.0 + t * t) * cauchy * cauchy;
+                    }
+                }
+                double down = 1.0 / sqrt(wght);
+                s_wt += rerr / down;
+                s_chi += dchi2;
+                if (iter == nter) rm[i] = rmeas; // remaining residual
+            }
+            summ = wave_sum(s_chi);
+            double suwt = wave_sum(s_wt);
+            __syncthreads();
+            ndf = neq - nrank;
+            resing = ((float)neq - (float)suwt) / (float)neq;
+            if (nan > 0) { status = 1; break; }
+            if (nrank != nalc) { status = 2; break; }
+            if (ndf <= 0) { status = 4; break; }
+        }
+        double chi_add = 0.0;
+        int ndf_add = 0;
+        if (status == 0) {
+            // chi^2 cuts with REAL(4) thresholds (4106-4131)
+            float chichi;
+            {
+                // chindl(3, ndf) (src/mpnum.f90:2078-2115)
+                const float tab3[30] = {9.0000f, 5.9146f, 4.7184f, 4.0628f, 3.6410f, 3.3436f, 3.1209f, 2.9468f,
+                                        2.8063f, 2.6902f, 2.5922f, 2.5082f, 2.4352f, 2.3711f, 2.3143f, 2.2635f,
+                                        2.2178f, 2.1764f, 2.1386f, 2.1040f, 2.0722f, 2.0428f, 2.0155f, 1.9901f,
+                                        1.9665f, 1.9443f, 1.9235f, 1.9040f, 1.8855f, 1.8681f};
+                float c;
+                if (ndf < 1) c = 0.0f;
+                else if (ndf <= 30) c = tab3[ndf - 1];
+                else {
+                    float t = 2.782170f + sqrtf((float)(ndf + ndf - 1));
+                    c = t * t / (float)(ndf + ndf);
+                }
+                chichi = c * (float)ndf;
+            }
+            if (o.chicut >= 0.0f) {
+                if (summ > (double)(o.chhuge * chichi)) status = 5;
+                else if (o.chicut > 0.0f) {
+                    float chlimt = o.chicut * chichi;
+                    if (summ > (double)chlimt) { status = 6; chi_add = (double)chlimt; ndf_add = ndf; }
+                }
+            }
+            if (status == 0 && o.lhuber > 1 && o.dwcut != 0.0f && resing > o.dwcut) {
+                status = 6;
+                chi_add = summ;
+                ndf_add = ndf;
+            }
+        }
+        if (status != 0) {
+            if (lane == 0) { out.status[r] = (int8_t)status; out.chi2[r] = chi_add * dw1; out.ndf[r] = ndf_add; }
+            __syncthreads();
+            continue;
+        }
+        // ---- fourth loop: vector, counters, g g^T, G (4162-4229) ----
+        if (o.icalcm == 1) {
+            for (int t = lane; t < nalg * nalc; t += WAVE) G[t] = 0.0;
+            if (M.sparse) {
+                // segments: runs of record variables in the same parameter group
+                int nseg = 0;
+                for (int base = 0; base < nalg; base += WAVE) {
+                    int a = base + lane;
+                    int g = a < nalg ? M.group_of[gvl[a]] : -1;
+                    int gprev = (a > 0 && a < nalg) ? M.group_of[gvl[a - 1]] : -2;
+                    bool st = (a < nalg) && (a == 0 || g != gprev);
+                    unsigned long long m = __ballot(st);
+                    int s = nseg + __popcll(m & ((2ull << lane) - 1ull)) - 1;
+                    if (a < nalg) { segof[a] = s; if (st) seggr[s] = g; }
+                    nseg += __popcll(m);
+                }
+                int npair = (nseg * nseg + nseg) / 2;
+                const int64_t p0 = R.rec_pb[r];
+                for (int t = lane; t < npair; t += WAVE) pblk[t] = t < P.max_pair ? R.pair_blk[p0 + t] : -1;
+            }
+        }
+        __syncthreads();
+        double s_chi = 0.0;
+        const bool down = (o.nloopn != 1 && o.lhuber != 0);
+        for (int i = lane; i < neq; i += WAVE) {
+            int64_t sl = R.eq_sl[e0 + i];
+            uint32_t en = R.eq_n[e0 + i];
+            int nl = en & 0xffff, ng = en >> 16;
+            int64_t sg = sl + nl + 1;
+            double rmeas = rm[i], rerr = re[i];
+            double wght = 1.0 / (rerr * rerr);
+            double dchi2 = wght * rmeas * rmeas;
+            if (down) {
+                double resid = fabs(rmeas);
+                if (nter <= 3) {
+                    if (resid > chuber * rerr) dchi2 = 2.0 * chuber * (resid / rerr - 0.5 * chuber);
+                } else {
+                    double t = resid / rerr / cauchy;
+                    dchi2 = log(1.0 + t * t) * cauchy * cauchy;
+                }
+                wght = lw[i];
+            }
+            s_chi += dchi2;
+            lw[i] = wght; // weight used by the global update
+            for (int j = 1; j <= ng; j++) {
+                double gj = wval<T>(R, sg + j);
+                int pj = R.widx[sg + j];
+                int vj = gv[pj];
+                if (gj == 0.0) continue;
+                if (vj > 0) {
+                    atomicAdd(&M.vec[vj - 1], dw1 * wght * rmeas * gj);
+                    atomicAdd(&M.cnt[vj - 1], 1ull);
+                    if (o.icalcm == 1) {
+                        int aj = gpos[pj];
+                        for (int k = 1; k <= j; k++) {
+                            int pk_ = R.widx[sg + k];
+                            int vk = gv[pk_];
+                            if (vk > 0) {
+                                double val = dw1 * wght * gj * wval<T>(R, sg + k);
+                                int ak = gpos[pk_];
+                                if (vj >= vk) mat_add(M, vj, vk, val, segof, seggr, pblk, aj, ak);
+                                else mat_add(M, vk, vj, val, segof, seggr, pblk, ak, aj);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        summ = wave_sum(s_chi);
+        __syncthreads();
+        if (o.icalcm == 1) {
+            // G = sum over equations of sqrt(dw) w g l^T, equations in order (4206-4219)
+            for (int i = 0; i < neq; i++) {
+                int64_t sl = R.eq_sl[e0 + i];
+                uint32_t en = R.eq_n[e0 + i];
+                int nl = en & 0xffff, ng = en >> 16;
+                int64_t sg = sl + nl + 1;
+                double wght = lw[i];
+                for (int t = lane; t < ng * nl; t += WAVE) {
+                    int j = t / nl + 1, k = t % nl + 1;
+                    int pj = R.widx[sg + j];
+                    int a = gpos[pj];
+                    if (a >= 0) {
+                        int ik = R.widx[sl + k];
+                        G[a * nalc + ik - 1] = G[a * nalc + ik - 1] + dw2 * wght * wval<T>(R, sg + j) * wval<T>(R, sl + k);
+                    }
+                }
+                __syncthreads();
+            }
+            // Gh = G Gamma^-1 (row-wise, l ascending as dbavat)
+            for (int a = lane; a < nalg; a += WAVE) {
+                for (int k = 1; k <= nalc; k++) {
+                    double c = 0.0;
+                    for (int l = 1; l <= nalc; l++) c = c + G[a * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
+                    Gh[a * nalc + k - 1] = c;
+                }
+            }
+            __syncthreads();
+            // Schur complement: element (a>=b) -= sum_k Gh[a,k] G[b,k]
+            const int ngg = (nalg * nalg + nalg) / 2;
+            for (int t = lane; t < ngg; t += WAVE) {
+                int a = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+                while ((a * a + a) / 2 > t) a--;
+                while (((a + 1) * (a + 1) + (a + 1)) / 2 <= t) a++;
+                int b = t - (a * a + a) / 2;
+                double c = 0.0;
+                for (int k = 0; k < nalc; k++) c = c + Gh[a * nalc + k] * G[b * nalc + k];
+                mat_add(M, gvl[a], gvl[b], -c, segof, seggr, pblk, a, b);
+            }
+        }
+        if (lane == 0) { out.status[r] = 0; out.chi2[r] = summ * dw1; out.ndf[r] = ndf; }
+        __syncthreads();
+    }
+}
+
+// host launcher
+hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out,
+                          const LdsPlan &P, int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s)
+{
+    if (rec_end <= rec_begin) return hipSuccess;
+    if (R.vf)
+        hipLaunchKernelGGL(k_records<float>, dim3(grid), dim3(WAVE), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
+    else
+        hipLaunchKernelGGL(k_records<double>, dim3(grid), dim3(WAVE), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
+    return hipGetLastError();
+}
+
+int lds_bytes(LdsPlan &P)
+{
+    int npk = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
+    size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max + 2 * (size_t)P.max_var * P.nalc_max;
+    size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + P.max_pair + (P.nalc_max + 1) + (2 * P.nalc_max + 2) + 8;
+    size_t b = d * 8 + i * 4;
+    P.bytes = (int)b;
+    return P.bytes;
+}
+
+} // namespace mpgpu

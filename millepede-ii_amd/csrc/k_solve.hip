@@ -1,0 +1,800 @@
+// Global-solve kernels: packed-matrix utilities, Gauss-Jordan inversion, symmetric
+// matrix-vector products for packed and group-block storage, MINRES vector algebra and the
+// MINRES preconditioners.
+#include "mpgpu_device.h"
+#include <float.h>
+
+namespace mpgpu {
+
+#define TPB 256
+
+__device__ __forceinline__ int64_t pk64s(int64_t i, int64_t j) { return (i * i - i) / 2 + j - 1; }
+
+// ---------------------------------------------------------------------------------------------
+// element updates (mupdat, src/pede.f90:3134-3204); i,j 1-based in the active system
+// ---------------------------------------------------------------------------------------------
+__device__ int32_t find_blk(const int64_t *brow, const int32_t *bcol, int ga, int gb)
+{
+    int64_t lo = brow[ga], hi = brow[ga + 1] - 1;
+    while (lo <= hi) {
+        int64_t mid = (lo + hi) >> 1;
+        int c = bcol[mid];
+        if (c == gb) return (int32_t)mid;
+        if (c < gb) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+__global__ void k_mupdat(DevModel M, int32_t n, const int32_t *ii, const int32_t *jj, const double *vv)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = ii[t], j = jj[t];
+    double v = vv[t];
+    if (i <= 0 || j <= 0 || v == 0.0) return;
+    int a = max(i, j), b = min(i, j);
+    if (!M.sparse) { atomicAdd(&M.mat[pk64s(a, b)], v); return; }
+    if (a > M.nvar) return; // Lagrange rows live in the constraint CSR for sparse storage
+    int ga = M.group_of[a], gb = M.group_of[b];
+    int32_t blk = find_blk(M.brow, M.bcol, ga, gb);
+    if (blk < 0) return;
+    int fa = M.gfirst[ga], fb = M.gfirst[gb];
+    int nc = M.gfirst[gb + 1] - fb;
+    atomicAdd(&M.mat[M.boff[blk] + (int64_t)(a - fa) * nc + (b - fb)], v);
+}
+
+__global__ void k_get_diag(DevModel M, int32_t n, double *d)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (i > n) return;
+    if (!M.sparse) { d[i - 1] = M.mat[pk64s(i, i)]; return; }
+    int g = M.group_of[i];
+    int64_t blk = M.brow[g + 1] - 1; // diagonal block is last in its row
+    int f = M.gfirst[g];
+    int nc = M.gfirst[g + 1] - f;
+    d[i - 1] = M.mat[M.boff[blk] + (int64_t)(i - f) * nc + (i - f)];
+}
+
+// band of the parameter block into pede's variable-band layout (matPreCond, india pointers,
+// src/pede.f90:2638-2653): row i holds columns max(1,i-mbandw+1)..i ending at india(i)
+__global__ void k_get_band(DevModel M, int32_t n, int32_t mbandw, const int32_t *india, double *c)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (i > n) return;
+    int ia = india[i - 1];
+    for (int j = max(1, i - mbandw + 1); j <= i; j++) {
+        double v = 0.0;
+        if (!M.sparse) v = M.mat[pk64s(i, j)];
+        else {
+            int ga = M.group_of[i], gb = M.group_of[j];
+            int32_t blk = find_blk(M.brow, M.bcol, ga, gb);
+            if (blk >= 0) {
+                int fa = M.gfirst[ga], fb = M.gfirst[gb];
+                int nc = M.gfirst[gb + 1] - fb;
+                v = M.mat[M.boff[blk] + (int64_t)(i - fa) * nc + (j - fb)];
+            }
+        }
+        c[ia - i + j - 1] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// deterministic reductions
+// ---------------------------------------------------------------------------------------------
+__global__ void k_partial_dot(int64_t n, const double *a, const double *b, double *part)
+{
+    __shared__ double s[TPB];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+        acc += a[i] * (b ? b[i] : 1.0);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void k_final_sum(int n, const double *part, double *out)
+{
+    __shared__ double s[TPB];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += TPB) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// pass statistics: chi2 (double), ndf (int64), status histogram [0..20]
+__global__ void k_pass_stats(int64_t n, const double *chi2, const int32_t *ndf, const int8_t *status, double *part,
+                             long long *ndf_sum, unsigned long long *hist)
+{
+    __shared__ double s[TPB];
+    __shared__ unsigned long long h[24];
+    if (threadIdx.x < 24) h[threadIdx.x] = 0;
+    __syncthreads();
+    double acc = 0.0;
+    long long nd = 0;
+    for (int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+        acc += chi2[i];
+        nd += ndf[i];
+        int st = status[i];
+        atomicAdd(&h[st < 0 || st > 23 ? 23 : st], 1ull);
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+    atomicAdd((unsigned long long *)ndf_sum, (unsigned long long)nd);
+    __syncthreads();
+    if (threadIdx.x < 24 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Gauss-Jordan inversion of the packed matrix, restating sqminl (src/mpnum.f90:230-355):
+// per pivot step one single-block kernel picks the pivot exactly as the sequential scan does
+// (first unused j with |v_jj| > max(|v_kk|, eps*|diag_j|), i.e. the largest |v_jj| above its
+// threshold, ties to the lowest index) and stages the scaled pivot column; a grid-wide kernel
+// then applies the rank-1 elimination with the reference's operand order.
+// state: used[n] (0 free / 1 used), info[0]=k, info[1]=nrank, info[2]=stop
+// ---------------------------------------------------------------------------------------------
+__global__ void k_gj_init(int n, const double *v, double *diag, int *used, int *info)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        diag[i] = fabs(v[pk64s(i + 1, i + 1)]);
+        used[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { info[0] = 0; info[1] = 0; info[2] = 0; }
+}
+
+__global__ __launch_bounds__(1024) void k_gj_pivot(int n, double *v, const double *diag, int *used, int *info,
+                                                   double *vk, double *b, double eps)
+{
+    __shared__ double sv[1024];
+    __shared__ int si[1024];
+    if (info[2]) return;
+    double best = -1.0;
+    int bi = 0x7fffffff;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        if (used[j]) continue;
+        double a = fabs(v[pk64s(j + 1, j + 1)]);
+        if (a > eps * diag[j] && (a > best || (a == best && j < bi))) { best = a; bi = j; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            double a = sv[threadIdx.x + o];
+            int ai = si[threadIdx.x + o];
+            if (a > sv[threadIdx.x] || (a == sv[threadIdx.x] && ai < si[threadIdx.x])) { sv[threadIdx.x] = a; si[threadIdx.x] = ai; }
+        }
+        __syncthreads();
+    }
+    int k = si[0];
+    if (sv[0] < 0.0 || k == 0x7fffffff) { // no pivot: stop; clearing happens in k_gj_clear
+        if (threadIdx.x == 0) { info[0] = 0; info[2] = 1; }
+        return;
+    }
+    const int K = k + 1;
+    const int64_t kk = pk64s(K, K);
+    const double vkk = 1.0 / v[kk];
+    // scaled column k: vk(j) = v(jk)*vkk, vk(k) = 0  (sqminl lines 299-311)
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        int J = j + 1;
+        if (J == K) vk[j] = 0.0;
+        else {
+            int64_t jk = J < K ? pk64s(K, J) : pk64s(J, K);
+            vk[j] = v[jk] * vkk;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        used[k] = 1;
+        info[0] = K;
+        info[1] += 1;
+        b[k] = b[k] * vkk;
+        vk[n] = vkk;
+    }
+}
+
+// elimination (sqminl lines 312-319): v(jl) -= vk(l) * (vk(j)/vkk) for j,l != k;
+// column k takes vk, v(kk) = -vkk; b(j) -= b(k)*(vk(j)/vkk)
+__global__ void k_gj_update(int n, double *v, const double *vk, double *b, const int *info)
+{
+    if (info[2]) return;
+    const int K = info[0];
+    const double vkk = vk[n];
+    const int64_t total = ((int64_t)n * (n + 1)) / 2;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t J = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5) + 1;
+        while ((J * J - J) / 2 > t) J--;
+        while ((J * J + J) / 2 <= t) J++;
+        int64_t L = t - (J * J - J) / 2 + 1;
+        if (J == K && L == K) v[t] = -vkk;
+        else if (J == K) v[t] = vk[L - 1];
+        else if (L == K) v[t] = vk[J - 1];
+        else {
+            double vjk = vk[J - 1] / vkk;
+            v[t] = v[t] - vk[L - 1] * vjk;
+        }
+    }
+    if (blockIdx.x == 0) {
+        const double bk = b[K - 1];
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            if (j + 1 == K) continue;
+            double vjk = vk[j] / vkk;
+            b[j] = b[j] - bk * vjk;
+        }
+    }
+}
+
+// rank defect: clear rows/columns of unused pivots (sqminl lines 320-331), then negate all
+__global__ void k_gj_finish(int n, double *v, double *b, const int *used, const int *info)
+{
+    const int64_t total = ((int64_t)n * (n + 1)) / 2;
+    const bool clear = info[2] != 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t J = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5) + 1;
+        while ((J * J - J) / 2 > t) J--;
+        while ((J * J + J) / 2 <= t) J++;
+        int64_t L = t - (J * J - J) / 2 + 1;
+        double x = v[t];
+        if (clear && !used[J - 1] && !used[L - 1]) x = 0.0;
+        v[t] = -x;
+    }
+    if (clear && blockIdx.x == 0)
+        for (int j = threadIdx.x; j < n; j += blockDim.x) if (!used[j]) b[j] = 0.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// y = A x for the packed full matrix (avprd0 full storage, src/pede.f90:5069-5232):
+// 64x64 tiles of the lower triangle; each stored element serves both halves.
+// ---------------------------------------------------------------------------------------------
+#define ST 64
+__global__ __launch_bounds__(256) void k_packed_symv(int n, const double *a, const double *x, double *y)
+{
+    const int nt = (n + ST - 1) / ST;
+    int64_t tile = blockIdx.x;
+    // tile -> (ti >= tj)
+    int ti = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+    while ((int64_t)ti * (ti + 1) / 2 > tile) ti--;
+    while ((int64_t)(ti + 1) * (ti + 2) / 2 <= tile) ti++;
+    int tj = (int)(tile - (int64_t)ti * (ti + 1) / 2);
+    if (ti >= nt) return;
+    __shared__ double xr[ST], xc[ST], yr[ST], yc[ST];
+    int r0 = ti * ST, c0 = tj * ST;
+    if (threadIdx.x < ST) {
+        int i = r0 + threadIdx.x, j = c0 + threadIdx.x;
+        xr[threadIdx.x] = i < n ? x[i] : 0.0;
+        xc[threadIdx.x] = j < n ? x[j] : 0.0;
+        yr[threadIdx.x] = 0.0;
+        yc[threadIdx.x] = 0.0;
+    }
+    __syncthreads();
+    // each thread: rows r = threadIdx.x/4 (+64/4 stride)... simple scheme: thread handles one
+    // row segment of 16 columns
+    const int lr = threadIdx.x >> 2, q = threadIdx.x & 3;
+    for (int rr = lr; rr < ST; rr += 64) {
+        int i = r0 + rr;
+        if (i >= n) continue;
+        double sr = 0.0;
+        const int64_t rowoff = (int64_t)i * (i + 1) / 2; // 0-based row i start
+        for (int cc = q * 16; cc < q * 16 + 16; cc++) {
+            int j = c0 + cc;
+            if (j > i || j >= n) continue;
+            double aij = a[rowoff + j];
+            sr += aij * xc[cc];
+            if (j != i) atomicAdd(&yc[cc], aij * xr[rr]);
+        }
+        atomicAdd(&yr[rr], sr);
+    }
+    __syncthreads();
+    if (threadIdx.x < ST) {
+        int i = r0 + threadIdx.x, j = c0 + threadIdx.x;
+        if (i < n) atomicAdd(&y[i], yr[threadIdx.x]);
+        if (ti != tj && j < n) atomicAdd(&y[j], yc[threadIdx.x]);
+        else if (ti == tj && j < n) atomicAdd(&y[j], yc[threadIdx.x]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// y = A x for the group-block storage (avprd0 sparse storage, src/pede.f90:5069-5232).
+// One thread per (block, row) computes the row part and pushes the transposed part.
+// Diagonal blocks hold their lower triangle only.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff,
+                             const int32_t *gfirst, const int32_t *brg, int64_t nblock, const double *a,
+                             const double *x, double *y)
+{
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // t enumerates (block, row-in-block) with at most 8 rows per block slot
+    int64_t blk = t >> 3;
+    int rr = (int)(t & 7);
+    if (blk >= nblock) return;
+    int ga = brg[blk], gb = bcol[blk];
+    int fa = gfirst[ga] - 1, fb = gfirst[gb] - 1;
+    int nr = gfirst[ga + 1] - 1 - fa, nc = gfirst[gb + 1] - 1 - fb;
+    const double *B = a + boff[blk];
+    for (int r = rr; r < nr; r += 8) {
+        double s = 0.0;
+        const double xr = x[fa + r];
+        int cmax = (ga == gb) ? r + 1 : nc;
+        for (int c = 0; c < cmax; c++) {
+            double v = B[(int64_t)r * nc + c];
+            s += v * x[fb + c];
+            if (ga != gb || c != r) atomicAdd(&y[fb + c], v * xr);
+        }
+        atomicAdd(&y[fa + r], s);
+    }
+}
+
+// constraint rows (Lagrange multipliers) in CSR over variable indices:
+// y[nvar+j] += sum c_ji x_i ; y_i += c_ji x[nvar+j]
+__global__ void k_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx,
+                            const double *cval, const double *x, double *y)
+{
+    int j = blockIdx.x;
+    if (j >= ncon) return;
+    __shared__ double s[TPB];
+    double acc = 0.0;
+    const double lam = x[nvar + j];
+    for (int64_t p = cptr[j] + threadIdx.x; p < cptr[j + 1]; p += TPB) {
+        int i = cidx[p] - 1;
+        double c = cval[p];
+        acc += c * x[i];
+        atomicAdd(&y[i], c * lam);
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&y[nvar + j], s[0]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// MINRES vector algebra (src/minresModule.f90:439-520)
+// ---------------------------------------------------------------------------------------------
+__global__ void k_scale_copy(int64_t n, double s, const double *a, double *out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = s * a[i];
+}
+// y = y - c*r
+__global__ void k_axpy_neg(int64_t n, double c, const double *r, double *y)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = y[i] - c * r[i];
+}
+// w1 = w2; w2 = w; w = (v - oldeps*w1 - delta*w2)*denom; x = x + phi*w
+__global__ void k_minres_update(int64_t n, const double *v, double *w, double *w1, double *w2, double *x, double oldeps,
+                                double delta, double denom, double phi)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double a1 = w2[i];
+        double a2 = w[i];
+        w1[i] = a1;
+        w2[i] = a2;
+        double wn = (v[i] - oldeps * a1 - delta * a2) * denom;
+        w[i] = wn;
+        x[i] = x[i] + phi * wn;
+    }
+}
+__global__ void k_copy(int64_t n, const double *a, double *b)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+// r = b - A x  (A x given in ax)
+__global__ void k_sub(int64_t n, const double *b, const double *ax, double *r)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) r[i] = b[i] - ax[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Diagonal preconditioner with constraint blocks: precons/presols (src/mpnum.f90:2881-3069)
+// The constraint part (p constraints, CSR rows over variable indices, p small) is done by one
+// workgroup; the Schur block S = K K^T (K = C D^-1/2) is factorised once (k_precons_s).
+// ---------------------------------------------------------------------------------------------
+__global__ void k_precons_cu(int32_t n, const double *diag, double *cu, int *nrkd)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = diag[i];
+    if (d > 0.0) cu[i] = 1.0 / sqrt(d);
+    else { cu[i] = 0.0; atomicAdd(nrkd, 1); }
+}
+
+// S(jk) = sum_i (c_ji cu_i)(c_ki cu_i), packed p(p+1)/2, then H D H^T (precons lines 2960-2975)
+__global__ void k_precons_s(int32_t p, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *cu,
+                            int32_t nvar, double *s, double *scratch_row /* nvar */)
+{
+    // single workgroup: build S column by column using a dense scratch of constraint j
+    for (int j = 0; j < p; j++) {
+        for (int i = threadIdx.x; i < nvar; i += blockDim.x) scratch_row[i] = 0.0;
+        __syncthreads();
+        for (int64_t q = cptr[j] + threadIdx.x; q < cptr[j + 1]; q += blockDim.x) {
+            int i = cidx[q] - 1;
+            scratch_row[i] = cval[q] * cu[i];
+        }
+        __syncthreads();
+        for (int k = 0; k <= j; k++) {
+            __shared__ double red[TPB];
+            double acc = 0.0;
+            for (int64_t q = cptr[k] + threadIdx.x; q < cptr[k + 1]; q += blockDim.x) {
+                int i = cidx[q] - 1;
+                acc += scratch_row[i] * (cval[q] * cu[i]);
+            }
+            red[threadIdx.x] = acc;
+            __syncthreads();
+            for (int o = TPB / 2; o > 0; o >>= 1) {
+                if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) s[(j * j + j) / 2 + k] = red[0];
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        double *S = s - 1;
+        int ii = 0;
+        for (int i = 1; i <= p; i++) {
+            ii = ii + i;
+            if (S[ii] != 0.0) S[ii] = 1.0 / S[ii];
+            int jj = ii;
+            for (int j = i + 1; j <= p; j++) {
+                double ratio = S[i + jj] * S[ii];
+                int kk = jj;
+                for (int k = j; k <= p; k++) { S[kk + j] = S[kk + j] - S[kk + i] * ratio; kk = kk + k; }
+                S[i + jj] = ratio;
+                jj = jj + j;
+            }
+        }
+    }
+}
+
+// apply: x = M^-1 y (presols). n variables, p constraints.
+__global__ void k_presols_a(int64_t n, const double *cu, const double *y, double *x)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = y[i] * cu[i];
+}
+__global__ void k_presols_c(int32_t p, int32_t n, const int64_t *cptr, const int32_t *cidx, const double *cval,
+                            const double *cu, const double *s, const double *y, double *x)
+{
+    // single workgroup; x[0..n) holds u = C^-1/2 y on entry
+    __shared__ double red[TPB];
+    __shared__ double lam[1024];
+    for (int j = 0; j < p; j++) {
+        double acc = 0.0;
+        for (int64_t q = cptr[j] + threadIdx.x; q < cptr[j + 1]; q += blockDim.x) {
+            int i = cidx[q] - 1;
+            acc += (cval[q] * cu[i]) * x[i];
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = TPB / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) lam[j] = y[n + j] - red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double *S = s - 1;
+        double *X = lam - 1;
+        int jj = 0;
+        for (int j = 1; j <= p; j++) {
+            double dsum = X[j];
+            for (int k = 1; k <= j - 1; k++) dsum = dsum - S[k + jj] * X[k];
+            X[j] = dsum;
+            jj = jj + j;
+        }
+        for (int j = p; j >= 1; j--) {
+            double dsum = X[j] * S[jj];
+            int kk = jj;
+            for (int k = j + 1; k <= p; k++) { dsum = dsum + S[kk + j] * X[k]; kk = kk + k; }
+            X[j] = -dsum;
+            jj = jj - j;
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < p; j++) {
+        for (int64_t q = cptr[j] + threadIdx.x; q < cptr[j + 1]; q += blockDim.x) {
+            int i = cidx[q] - 1;
+            x[i] = x[i] - (cval[q] * cu[i]) * lam[j];
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < p; j += blockDim.x) x[n + j] = lam[j];
+}
+__global__ void k_presols_b(int64_t n, const double *cu, double *x)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = x[i] * cu[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Variable-band Cholesky preconditioner (lltdec/lltfwd/lltbwd, src/mpnum.f90:2155-2336).
+// The recurrences are sequential in the row index; they run on one lane.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_lltdec(int n, double *c, const int *india, int *nrkd, int iopt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double *C = c - 1;
+    const int *IND = india - 1;
+    const double eps = 16.0 * DBL_EPSILON;
+    int nr = 0;
+    double diag = 0.0;
+    if (C[IND[1]] > 0.0) C[IND[1]] = 1.0 / sqrt(C[IND[1]]);
+    else { C[IND[1]] = 0.0; nr = -1; }
+    for (int k = 2; k <= n; k++) {
+        int mk = k - IND[k] + IND[k - 1] + 1;
+        for (int j = mk; j <= k; j++) {
+            int mj = 1;
+            if (j > 1) mj = j - IND[j] + IND[j - 1] + 1;
+            int kj = IND[k] - k + j;
+            diag = C[IND[j]];
+            for (int i = max(mj, mk); i <= j - 1; i++) C[kj] = C[kj] - C[IND[k] - k + i] * C[IND[j] - j + i];
+            if (j != k) C[kj] = C[kj] * diag;
+        }
+        if (C[IND[k]] > eps * diag) C[IND[k]] = 1.0 / sqrt(C[IND[k]]);
+        else {
+            for (int j = mk; j <= k; j++) C[IND[k] - k + j] = 0.0;
+            if (iopt > 0 && diag > 0.0) C[IND[k]] = 1.0 / sqrt(diag);
+            else {
+                if (nr == 0) nr = -k;
+                else { if (nr < 0) nr = 1; nr++; }
+            }
+        }
+    }
+    *nrkd = nr;
+}
+
+__device__ void lltfwd_dev(int n, const double *c, const int *india, double *x)
+{
+    const double *C = c - 1;
+    const int *IND = india - 1;
+    double *X = x - 1;
+    X[1] = X[1] * C[IND[1]];
+    for (int k = 2; k <= n; k++) {
+        double xk = X[k];
+        for (int j = k - IND[k] + IND[k - 1] + 1; j <= k - 1; j++) xk = xk - C[IND[k] - k + j] * X[j];
+        X[k] = xk * C[IND[k]];
+    }
+}
+__device__ void lltbwd_dev(int n, const double *c, const int *india, double *x)
+{
+    const double *C = c - 1;
+    const int *IND = india - 1;
+    double *X = x - 1;
+    for (int k = n; k >= 2; k--) {
+        double xk = X[k] * C[IND[k]];
+        X[k] = xk;
+        for (int j = k - IND[k] + IND[k - 1] + 1; j <= k - 1; j++) X[j] = X[j] - C[IND[k] - k + j] * xk;
+    }
+    X[1] = X[1] * C[IND[1]];
+}
+__global__ void k_lltfwd(int n, const double *c, const int *india, double *x)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) lltfwd_dev(n, c, india, x);
+}
+__global__ void k_lltbwd(int n, const double *c, const int *india, double *x)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) lltbwd_dev(n, c, india, x);
+}
+
+// K rows: forward solve of each constraint column restricted to its block range (lltfwds,
+// src/mpnum.f90:2275-2306); one thread per constraint.  kmat: per constraint `in` values.
+__global__ void k_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb /* per constraint: i0, in */,
+                         const int64_t *koff, double *kmat)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ncon) return;
+    int i0 = cb[2 * j], ns = cb[2 * j + 1];
+    double *X = kmat + koff[j] - 1;
+    const double *C = c - 1;
+    const int *IND = india - 1;
+    int kmax = min(ns, n - i0);
+    for (int k = 1; k <= kmax; k++) {
+        int l = k + i0;
+        double xk = X[k];
+        if (l > 1) {
+            int j0 = max(1, k - IND[l] + IND[l - 1] + 1);
+            for (int jj = j0; jj <= k - 1; jj++) xk = xk - C[IND[l] - k + jj] * X[jj];
+        }
+        X[k] = xk * C[IND[l]];
+    }
+}
+
+// H = K K^T restricted to overlapping ranges (equdecs products), packed m(m+1)/2, plus lltdec
+__global__ void k_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h, const int *hind,
+                         int *nrkd2)
+{
+    // one workgroup; pairs (j>=k) by threads
+    const int np = (m * m + m) / 2;
+    for (int t = threadIdx.x; t < np; t += blockDim.x) {
+        int j = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((j * j + j) / 2 > t) j--;
+        while (((j + 1) * (j + 1) + (j + 1)) / 2 <= t) j++;
+        int k = t - (j * j + j) / 2;
+        int ij0 = cb[2 * j], ijn = cb[2 * j + 1];
+        int ik0 = cb[2 * k], ikn = cb[2 * k + 1];
+        int lo = max(ij0, ik0), hi = min(ij0 + ijn, ik0 + ikn);
+        double acc = 0.0;
+        for (int q = lo; q < hi; q++) acc += kmat[koff[j] + (q - ij0)] * kmat[koff[k] + (q - ik0)];
+        h[t] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // lltdec of the dense m x m lower-packed H (india = i(i+1)/2)
+        double *C = h - 1;
+        const int *IND = hind - 1;
+        const double eps = 16.0 * DBL_EPSILON;
+        int nr = 0;
+        double diag = 0.0;
+        if (C[IND[1]] > 0.0) C[IND[1]] = 1.0 / sqrt(C[IND[1]]);
+        else { C[IND[1]] = 0.0; nr = -1; }
+        for (int k = 2; k <= m; k++) {
+            int mk = k - IND[k] + IND[k - 1] + 1;
+            for (int j = mk; j <= k; j++) {
+                int mj = 1;
+                if (j > 1) mj = j - IND[j] + IND[j - 1] + 1;
+                int kj = IND[k] - k + j;
+                diag = C[IND[j]];
+                for (int i = max(mj, mk); i <= j - 1; i++) C[kj] = C[kj] - C[IND[k] - k + i] * C[IND[j] - j + i];
+                if (j != k) C[kj] = C[kj] * diag;
+            }
+            if (C[IND[k]] > eps * diag) C[IND[k]] = 1.0 / sqrt(C[IND[k]]);
+            else {
+                for (int j = mk; j <= k; j++) C[IND[k] - k + j] = 0.0;
+                if (nr == 0) nr = -k;
+                else { if (nr < 0) nr = 1; nr++; }
+            }
+        }
+        *nrkd2 = nr;
+    }
+}
+
+// equslvs apply (src/mpnum.f90:2613-2678) as one single-lane kernel: the band recurrences are
+// sequential; the constraint products are short.
+__global__ void k_equslvs(int n, int m, const double *c, const int *india, const int *cb, const int64_t *koff,
+                          const double *kmat, const double *h, const int *hind, double *x)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    lltfwd_dev(n, c, india, x);
+    if (m > 0) {
+        for (int j = 0; j < m; j++) {
+            int i0 = cb[2 * j], in = cb[2 * j + 1];
+            double s = x[n + j];
+            for (int i = 1; i <= in; i++) if (i0 + i <= n) s = s - kmat[koff[j] + i - 1] * x[i0 + i - 1];
+            x[n + j] = s;
+        }
+        lltfwd_dev(m, h, hind, x + n);
+        lltbwd_dev(m, h, hind, x + n);
+        for (int i = 0; i < m; i++) x[n + i] = -x[n + i];
+        for (int j = 0; j < m; j++) {
+            int i0 = cb[2 * j], in = cb[2 * j + 1];
+            double y = x[n + j];
+            for (int i = 1; i <= in; i++) if (i0 + i <= n) x[i0 + i - 1] = x[i0 + i - 1] - kmat[koff[j] + i - 1] * y;
+        }
+    }
+    lltbwd_dev(n, c, india, x);
+}
+
+// ---------------------------------------------------------------------------------------------
+// host launch helpers
+// ---------------------------------------------------------------------------------------------
+static inline int nblk(int64_t n, int t = TPB) { int64_t b = (n + t - 1) / t; return (int)(b < 1 ? 1 : (b > 65535 * 16 ? 65535 * 16 : b)); }
+
+void l_mupdat(const DevModel &M, int32_t n, const int32_t *i, const int32_t *j, const double *v, hipStream_t s)
+{ if (n > 0) hipLaunchKernelGGL(k_mupdat, dim3(nblk(n)), dim3(TPB), 0, s, M, n, i, j, v); }
+void l_get_diag(const DevModel &M, int32_t n, double *d, hipStream_t s)
+{ hipLaunchKernelGGL(k_get_diag, dim3(nblk(n)), dim3(TPB), 0, s, M, n, d); }
+void l_get_band(const DevModel &M, int32_t n, int32_t mbandw, const int32_t *india, double *c, hipStream_t s)
+{ hipLaunchKernelGGL(k_get_band, dim3(nblk(n)), dim3(TPB), 0, s, M, n, mbandw, india, c); }
+
+// dot product into *out (device), deterministic; part: >= 1024 doubles
+void l_dot(int64_t n, const double *a, const double *b, double *part, double *out, hipStream_t s)
+{
+    int g = (int)((n + TPB - 1) / TPB);
+    if (g > 1024) g = 1024;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_partial_dot, dim3(g), dim3(TPB), 0, s, n, a, b, part);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, g, part, out);
+}
+void l_pass_stats(int64_t n, const double *chi2, const int32_t *ndf, const int8_t *st, double *part, long long *ndfsum,
+                  unsigned long long *hist, double *chi_out, hipStream_t s)
+{
+    int g = (int)((n + TPB - 1) / TPB);
+    if (g > 1024) g = 1024;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_pass_stats, dim3(g), dim3(TPB), 0, s, n, chi2, ndf, st, part, ndfsum, hist);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, g, part, chi_out);
+}
+
+void l_gj_init(int n, const double *v, double *diag, int *used, int *info, hipStream_t s)
+{ hipLaunchKernelGGL(k_gj_init, dim3(nblk(n)), dim3(TPB), 0, s, n, v, diag, used, info); }
+void l_gj_step(int n, double *v, const double *diag, int *used, int *info, double *vk, double *b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, s, n, v, diag, used, info, vk, b, 1.0e-10);
+    int64_t tot = ((int64_t)n * (n + 1)) / 2;
+    int g = nblk(tot);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_gj_update, dim3(g), dim3(TPB), 0, s, n, v, vk, b, info);
+}
+void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, hipStream_t s)
+{
+    int64_t tot = ((int64_t)n * (n + 1)) / 2;
+    int g = nblk(tot);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_gj_finish, dim3(g), dim3(TPB), 0, s, n, v, b, used, info);
+}
+
+void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream_t s)
+{
+    hipMemsetAsync(y, 0, sizeof(double) * n, s);
+    int64_t nt = (n + ST - 1) / ST;
+    int64_t tiles = nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(k_packed_symv, dim3((unsigned)tiles), dim3(256), 0, s, n, a, x, y);
+}
+void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const int32_t *gfirst,
+                  const int32_t *brg, int64_t nblock, const double *a, const double *x, double *y, int64_t nall,
+                  hipStream_t s)
+{
+    hipMemsetAsync(y, 0, sizeof(double) * nall, s);
+    int64_t t = nblock * 8;
+    hipLaunchKernelGGL(k_block_symv, dim3((unsigned)((t + TPB - 1) / TPB)), dim3(TPB), 0, s, ngroup, brow, bcol, boff,
+                       gfirst, brg, nblock, a, x, y);
+}
+void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
+                 double *y, hipStream_t s)
+{ if (ncon > 0) hipLaunchKernelGGL(k_cons_symv, dim3(ncon), dim3(TPB), 0, s, ncon, nvar, cptr, cidx, cval, x, y); }
+
+void l_scale_copy(int64_t n, double sc, const double *a, double *o, hipStream_t s)
+{ hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n)), dim3(TPB), 0, s, n, sc, a, o); }
+void l_axpy_neg(int64_t n, double c, const double *r, double *y, hipStream_t s)
+{ hipLaunchKernelGGL(k_axpy_neg, dim3(nblk(n)), dim3(TPB), 0, s, n, c, r, y); }
+void l_minres_update(int64_t n, const double *v, double *w, double *w1, double *w2, double *x, double oldeps, double delta,
+                     double denom, double phi, hipStream_t s)
+{ hipLaunchKernelGGL(k_minres_update, dim3(nblk(n)), dim3(TPB), 0, s, n, v, w, w1, w2, x, oldeps, delta, denom, phi); }
+void l_copy(int64_t n, const double *a, double *b, hipStream_t s)
+{ hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(TPB), 0, s, n, a, b); }
+void l_sub(int64_t n, const double *b, const double *ax, double *r, hipStream_t s)
+{ hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(TPB), 0, s, n, b, ax, r); }
+
+void l_precons(int32_t n, int32_t p, const double *diag, double *cu, int *nrkd, const int64_t *cptr, const int32_t *cidx,
+               const double *cval, double *sm, double *scratch, hipStream_t s)
+{
+    hipMemsetAsync(nrkd, 0, sizeof(int), s);
+    hipLaunchKernelGGL(k_precons_cu, dim3(nblk(n)), dim3(TPB), 0, s, n, diag, cu, nrkd);
+    if (p > 0) hipLaunchKernelGGL(k_precons_s, dim3(1), dim3(TPB), 0, s, p, cptr, cidx, cval, cu, n, sm, scratch);
+}
+void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, const int32_t *cidx, const double *cval,
+               const double *sm, const double *y, double *x, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_presols_a, dim3(nblk(n)), dim3(TPB), 0, s, (int64_t)n, cu, y, x);
+    if (p > 0) hipLaunchKernelGGL(k_presols_c, dim3(1), dim3(TPB), 0, s, p, n, cptr, cidx, cval, cu, sm, y, x);
+    hipLaunchKernelGGL(k_presols_b, dim3(nblk(n)), dim3(TPB), 0, s, (int64_t)n, cu, x);
+}
+
+void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, hipStream_t s)
+{ hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); }
+void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
+              hipStream_t s)
+{ if (ncon > 0) hipLaunchKernelGGL(k_band_k, dim3((ncon + 63) / 64), dim3(64), 0, s, ncon, n, c, india, cb, koff, kmat); }
+void l_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h, const int *hind, int *nrkd2,
+              hipStream_t s)
+{ if (m > 0) hipLaunchKernelGGL(k_band_h, dim3(1), dim3(TPB), 0, s, m, cb, koff, kmat, h, hind, nrkd2); }
+void l_equslvs(int n, int m, const double *c, const int *india, const int *cb, const int64_t *koff, const double *kmat,
+               const double *h, const int *hind, double *x, hipStream_t s)
+{ hipLaunchKernelGGL(k_equslvs, dim3(1), dim3(64), 0, s, n, m, c, india, cb, koff, kmat, h, hind, x); }
+
+} // namespace mpgpu

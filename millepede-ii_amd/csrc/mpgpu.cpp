@@ -1,0 +1,1504 @@
+// libmpgpu host side: context, record compilation, structure build (loop1/loop2
+// equivalents), data passes (loopn) and the global solve dispatch (xloopn's metsol switch).
+#include "mpgpu_internal.h"
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <omp.h>
+
+using namespace mpgpu;
+
+struct mpgpu_ctx {
+    mpgpu_options opt{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // ---- host copy of the input records ----
+    std::vector<int64_t> rec_ptr{0};
+    std::vector<int32_t> inder;
+    std::vector<float> vf;
+    std::vector<double> vd;
+    bool dbl = false;
+
+    // ---- steering ----
+    std::vector<int32_t> sp_label;
+    std::vector<double> sp_value, sp_presig;
+    std::vector<EqBlock> cons, meas;
+
+    // ---- structure ----
+    bool built = false;
+    int32_t ntot = 0, nvar = 0, ncon = 0, nall = 0, ngroup = 0;
+    std::vector<int32_t> labels, var_index, group_start_label, var_to_tot;
+    std::vector<int64_t> counters;
+    std::vector<double> par, presig_tot, pre_weight;
+    std::vector<int32_t> gfirst, group_of; // variable groups
+    std::vector<int64_t> brow, boff;
+    std::vector<int32_t> bcol, brg;
+    int64_t nblock = 0, nstored = 0;
+    // constraints (CSR over variable indices) and residual bookkeeping
+    std::vector<int64_t> cptr;
+    std::vector<int32_t> cidx;
+    std::vector<double> cval;
+    std::vector<double> crhs;
+    std::vector<std::vector<std::pair<int32_t, double>>> con_tot; // (total index, factor) per constraint
+
+    // ---- device ----
+    DBuf<int64_t> d_rec_eq, d_rec_gl, d_rec_pb, d_eq_sl;
+    DBuf<uint32_t> d_eq_n;
+    DBuf<float> d_vf;
+    DBuf<double> d_vd;
+    DBuf<int32_t> d_widx, d_gl_tix, d_pair_blk;
+    DBuf<double> d_par, d_mat, d_vec, d_pre_weight;
+    DBuf<int32_t> d_var, d_group_of, d_gfirst, d_bcol, d_brg;
+    DBuf<int64_t> d_brow, d_boff;
+    DBuf<unsigned long long> d_cnt;
+    DBuf<double> d_chi2;
+    DBuf<int32_t> d_ndf;
+    DBuf<int8_t> d_status;
+    DBuf<double> d_part, d_scal;
+    DBuf<long long> d_ndfsum;
+    DBuf<unsigned long long> d_hist;
+    DBuf<int64_t> d_cptr;
+    DBuf<int32_t> d_cidx;
+    DBuf<double> d_cval;
+    LdsPlan plan{};
+    int64_t nrec = 0;
+    int64_t max_pair = 0;
+
+    // ---- solve state ----
+    bool have_inverse = false;
+    std::vector<double> diag_before; // workspaceDiag (matrix diagonal before inversion)
+    std::vector<double> last_vec;
+    std::vector<unsigned long long> last_cnt;
+};
+
+static int fail(mpgpu_ctx *c, int code, const std::string &m)
+{
+    if (c) c->err = m;
+    return code;
+}
+
+#define API_TRY try {
+#define API_CATCH(ctx)                                                                   \
+    }                                                                                    \
+    catch (const MpError &e) { return fail(ctx, e.code, e.msg); }                        \
+    catch (const std::bad_alloc &) { return fail(ctx, 30, "host allocation failed"); }   \
+    catch (const std::exception &e) { return fail(ctx, 30, e.what()); }
+
+static double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+extern "C" void mpgpu_default_options(mpgpu_options *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->metsol = MPGPU_INVERSION;
+    o->matsto = MPGPU_FULL;
+    o->mbandw = 0;
+    o->lprecm = 0;
+    o->lhuber = 0;
+    o->lfitbb = 1;
+    o->mreqenf = 25;
+    o->mreqena = 10;
+    o->mcount = 0;
+    o->itnlim = 2000;
+    o->withmultipliers = 1;
+    o->mrmode = 0;
+    o->chicut = 0.0f;
+    o->chirem = 0.0f;
+    o->chhuge = 50.0f;
+    o->dwcut = 0.0f;
+    o->cndlmx = 100.0f;
+    o->regula = 1.0f;
+    o->default_presigma = 0.0;
+    o->mrestl = 1.0e-6;
+    o->mrtcnd = 1.0e7;
+    o->nregul = 0;
+    o->refine_steps = 5;
+}
+
+extern "C" int mpgpu_create(const mpgpu_options *opt, int32_t device, mpgpu_ctx **out)
+{
+    if (!out) return 30;
+    mpgpu_ctx *c = nullptr;
+    API_TRY
+    c = new mpgpu_ctx();
+    if (opt) c->opt = *opt; else mpgpu_default_options(&c->opt);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw MpError(30, "no HIP device available");
+    if (device < 0 || device >= ndev) throw MpError(30, "bad device ordinal");
+    c->device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c;
+    return 0;
+    }
+    catch (const MpError &e) { std::fprintf(stderr, "mpgpu_create: %s\n", e.msg.c_str()); delete c; *out = nullptr; return e.code; }
+    catch (...) { delete c; *out = nullptr; return 30; }
+}
+
+extern "C" int mpgpu_destroy(mpgpu_ctx *c)
+{
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+    delete c;
+    return 0;
+}
+
+extern "C" const char *mpgpu_last_error(const mpgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int mpgpu_set_options(mpgpu_ctx *c, const mpgpu_options *o)
+{
+    if (!c || !o) return 30;
+    c->opt = *o;
+    return 0;
+}
+
+extern "C" int mpgpu_add_records(mpgpu_ctx *c, int64_t nrec, const int64_t *rec_ptr, const int32_t *inder,
+                                 const float *gf, const double *gd)
+{
+    if (!c) return 30;
+    API_TRY
+    if (c->built) throw MpError(24, "records cannot be added after the structure was built");
+    if (nrec < 0 || (nrec > 0 && (!rec_ptr || !inder || (!gf && !gd)))) throw MpError(24, "bad record arrays");
+    if (gf && gd) throw MpError(24, "give either FP32 or FP64 payload");
+    int64_t w0 = rec_ptr[0], w1 = rec_ptr[nrec];
+    if (w1 < w0) throw MpError(24, "record pointers not ascending");
+    for (int64_t r = 0; r < nrec; r++)
+        if (rec_ptr[r + 1] < rec_ptr[r]) throw MpError(24, "record pointers not ascending");
+    int64_t base = (int64_t)c->inder.size();
+    bool was_dbl = c->dbl;
+    if (gd && !was_dbl) {
+        // promote existing FP32 payload
+        c->vd.resize(c->vf.size());
+        for (size_t i = 0; i < c->vf.size(); i++) c->vd[i] = c->vf[i];
+        c->vf.clear();
+        c->vf.shrink_to_fit();
+        c->dbl = true;
+    }
+    c->inder.insert(c->inder.end(), inder + w0, inder + w1);
+    if (c->dbl) {
+        if (gd) c->vd.insert(c->vd.end(), gd + w0, gd + w1);
+        else for (int64_t i = w0; i < w1; i++) c->vd.push_back((double)gf[i]);
+    } else {
+        c->vf.insert(c->vf.end(), gf + w0, gf + w1);
+    }
+    for (int64_t r = 0; r < nrec; r++) c->rec_ptr.push_back(base + rec_ptr[r + 1] - w0);
+    return 0;
+    API_CATCH(c)
+}
+
+extern "C" int mpgpu_read_mille_file(mpgpu_ctx *c, const char *path, int32_t fortran, int64_t *nrec_out)
+{
+    if (!c) return 30;
+    API_TRY
+    FILE *f = std::fopen(path, "rb");
+    if (!f) throw MpError(18, std::string("cannot open binary file ") + path);
+    std::vector<int64_t> rp{0};
+    std::vector<int32_t> in;
+    std::vector<float> ff;
+    std::vector<double> fd;
+    bool any_dbl = false;
+    int64_t nrec = 0;
+    std::vector<float> tmpf;
+    for (;;) {
+        int32_t marker = 0, nr = 0;
+        if (fortran) {
+            if (std::fread(&marker, 4, 1, f) != 1) break;
+            if (std::fread(&nr, 4, 1, f) != 1) { std::fclose(f); throw MpError(18, "truncated Fortran record"); }
+        } else {
+            if (std::fread(&nr, 4, 1, f) != 1) break;
+        }
+        bool dbl = nr < 0;
+        int32_t nw = (dbl ? -nr : nr) / 2;
+        if (dbl && !any_dbl) {
+            any_dbl = true;
+            fd.resize(ff.size());
+            for (size_t i = 0; i < ff.size(); i++) fd[i] = ff[i];
+            ff.clear();
+        }
+        size_t off = in.size();
+        in.resize(off + nw);
+        if (dbl) {
+            fd.resize(off + nw);
+            if (std::fread(fd.data() + off, 8, nw, f) != (size_t)nw) { std::fclose(f); throw MpError(18, "truncated record"); }
+        } else {
+            tmpf.resize(nw);
+            if (std::fread(tmpf.data(), 4, nw, f) != (size_t)nw) { std::fclose(f); throw MpError(18, "truncated record"); }
+            if (any_dbl) { fd.resize(off + nw); for (int i = 0; i < nw; i++) fd[off + i] = tmpf[i]; }
+            else ff.insert(ff.end(), tmpf.begin(), tmpf.end());
+        }
+        if (std::fread(in.data() + off, 4, nw, f) != (size_t)nw) { std::fclose(f); throw MpError(18, "truncated record"); }
+        if (fortran) {
+            int32_t m2 = 0;
+            if (std::fread(&m2, 4, 1, f) != 1 || m2 != marker) { std::fclose(f); throw MpError(18, "bad Fortran record marker"); }
+        }
+        rp.push_back((int64_t)in.size());
+        nrec++;
+    }
+    std::fclose(f);
+    int rc = mpgpu_add_records(c, nrec, rp.data(), in.data(), any_dbl ? nullptr : ff.data(), any_dbl ? fd.data() : nullptr);
+    if (rc) return rc;
+    if (nrec_out) *nrec_out = nrec;
+    return 0;
+    API_CATCH(c)
+}
+
+extern "C" int mpgpu_set_steering_parameters(mpgpu_ctx *c, int32_t n, const int32_t *labels, const double *values,
+                                             const double *presigma)
+{
+    if (!c) return 30;
+    if (c->built) return fail(c, 24, "parameters must be given before the structure is built");
+    for (int32_t k = 0; k < n; k++) {
+        c->sp_label.push_back(labels[k]);
+        c->sp_value.push_back(values ? values[k] : 0.0);
+        c->sp_presig.push_back(presigma ? presigma[k] : 0.0);
+    }
+    return 0;
+}
+
+extern "C" int mpgpu_add_equations(mpgpu_ctx *c, int32_t kind, int32_t nblock, const int64_t *ptr, const int32_t *labels,
+                                   const double *factors, const double *rhs, const double *sigma)
+{
+    if (!c) return 30;
+    if (c->built) return fail(c, 24, "equations must be given before the structure is built");
+    for (int32_t b = 0; b < nblock; b++) {
+        EqBlock e;
+        e.kind = kind;
+        e.rhs = rhs ? rhs[b] : 0.0;
+        e.sigma = sigma ? sigma[b] : 0.0;
+        for (int64_t p = ptr[b]; p < ptr[b + 1]; p++) {
+            e.labels.push_back(labels[p]);
+            e.factors.push_back(factors[p]);
+        }
+        (kind == 0 ? c->cons : c->meas).push_back(std::move(e));
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// structure build
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct Decoded {
+    // per equation: sl (absolute word), nl, ng
+    std::vector<int64_t> sl;
+    std::vector<uint32_t> n;
+};
+
+// decodeNextMeasurement over one record (src/pede.f90:2428-2468); returns false on bad structure
+inline bool decode_record(const int32_t *I, const float *F, const double *D, int64_t w0, int64_t w1,
+                          std::vector<int64_t> &sl, std::vector<uint32_t> &nn)
+{
+    // words are 1-based within the record: word k at w0 + k - 1
+    auto val = [&](int64_t k) { return D ? D[w0 + k - 1] : (double)F[w0 + k - 1]; };
+    auto idx = [&](int64_t k) { return I[w0 + k - 1]; };
+    int64_t nw = w1 - w0;
+    int64_t last = 1, end = nw;
+    for (;;) {
+        if (last >= end) break;
+        do { last++; if (last > nw) return false; } while (idx(last) != 0);
+        int64_t sL = last;
+        do { last++; if (last > nw) return false; } while (idx(last) != 0);
+        int64_t sG = last;
+        if (sL + 1 == sG && val(sG) < 0.0) {
+            last = last + (int64_t)std::llround(-val(sG));
+            continue;
+        }
+        while (last < end && idx(last + 1) != 0) last++;
+        int64_t nl = sG - sL - 1, ng = last - sG;
+        if (nl > 0xffff || ng > 0xffff) return false;
+        sl.push_back(w0 + sL - 1);
+        nn.push_back((uint32_t)nl | ((uint32_t)ng << 16));
+    }
+    return true;
+}
+
+} // namespace
+
+extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *nvar_out, int32_t *ncon_out)
+{
+    if (!c) return 30;
+    API_TRY
+    HIPCHK(hipSetDevice(c->device));
+    const int64_t nrec = (int64_t)c->rec_ptr.size() - 1;
+    c->nrec = nrec;
+    const int32_t *I = c->inder.data();
+    const float *F = c->dbl ? nullptr : c->vf.data();
+    const double *D = c->dbl ? c->vd.data() : nullptr;
+    auto val = [&](int64_t w) { return D ? D[w] : (double)F[w]; };
+
+    // ---- decode all records (equation tables) ----
+    std::vector<int64_t> rec_eq(nrec + 1, 0);
+    std::vector<std::vector<int64_t>> t_sl(omp_get_max_threads());
+    std::vector<std::vector<uint32_t>> t_n(omp_get_max_threads());
+    std::vector<int64_t> neq_rec(nrec, 0);
+    std::atomic<int64_t> bad{-1};
+    // decode in record order per chunk so that concatenation keeps record order
+    const int64_t CH = 4096;
+    const int64_t nch = (nrec + CH - 1) / CH;
+    std::vector<std::vector<int64_t>> ch_sl(nch);
+    std::vector<std::vector<uint32_t>> ch_n(nch);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t ch = 0; ch < nch; ch++) {
+        for (int64_t r = ch * CH; r < std::min(nrec, (ch + 1) * CH); r++) {
+            size_t before = ch_sl[ch].size();
+            if (!decode_record(I, F, D, c->rec_ptr[r], c->rec_ptr[r + 1], ch_sl[ch], ch_n[ch])) {
+                int64_t exp = -1;
+                bad.compare_exchange_strong(exp, r);
+            }
+            neq_rec[r] = (int64_t)(ch_sl[ch].size() - before);
+        }
+    }
+    if (bad.load() >= 0) throw MpError(20, "bad record structure in record " + std::to_string(bad.load() + 1));
+    for (int64_t r = 0; r < nrec; r++) rec_eq[r + 1] = rec_eq[r] + neq_rec[r];
+    const int64_t neq = rec_eq[nrec];
+    std::vector<int64_t> eq_sl(neq);
+    std::vector<uint32_t> eq_n(neq);
+    {
+        std::vector<int64_t> choff(nch + 1, 0);
+        for (int64_t ch = 0; ch < nch; ch++) choff[ch + 1] = choff[ch] + (int64_t)ch_sl[ch].size();
+#pragma omp parallel for schedule(static)
+        for (int64_t ch = 0; ch < nch; ch++) {
+            std::copy(ch_sl[ch].begin(), ch_sl[ch].end(), eq_sl.begin() + choff[ch]);
+            std::copy(ch_n[ch].begin(), ch_n[ch].end(), eq_n.begin() + choff[ch]);
+            std::vector<int64_t>().swap(ch_sl[ch]);
+            std::vector<uint32_t>().swap(ch_n[ch]);
+        }
+    }
+
+    // ---- label table: all labels of global words, steering parameters and equations ----
+    int32_t lmin = INT32_MAX, lmax = 0;
+#pragma omp parallel for reduction(min : lmin) reduction(max : lmax) schedule(static)
+    for (int64_t e = 0; e < neq; e++) {
+        int64_t sg = eq_sl[e] + (eq_n[e] & 0xffff) + 1;
+        int ng = eq_n[e] >> 16;
+        for (int j = 1; j <= ng; j++) {
+            int32_t l = I[sg + j];
+            if (l < lmin) lmin = l;
+            if (l > lmax) lmax = l;
+        }
+    }
+    auto upd_mm = [&](int32_t l) { lmin = std::min(lmin, l); lmax = std::max(lmax, l); };
+    for (int32_t l : c->sp_label) upd_mm(l);
+    for (auto &b : c->cons) for (int32_t l : b.labels) upd_mm(l);
+    for (auto &b : c->meas) for (int32_t l : b.labels) upd_mm(l);
+    if (lmax == 0) throw MpError(22, "no global parameters");
+    if (lmin <= 0) throw MpError(20, "label <= 0 in input");
+    const int64_t span = (int64_t)lmax - lmin + 1;
+    const int64_t nwbits = (span + 63) / 64;
+    std::vector<std::atomic<uint64_t>> bits(nwbits);
+    for (auto &b : bits) b.store(0, std::memory_order_relaxed);
+    auto setbit = [&](int32_t l) {
+        int64_t o = (int64_t)l - lmin;
+        bits[o >> 6].fetch_or(1ull << (o & 63), std::memory_order_relaxed);
+    };
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < neq; e++) {
+        int64_t sg = eq_sl[e] + (eq_n[e] & 0xffff) + 1;
+        int ng = eq_n[e] >> 16;
+        for (int j = 1; j <= ng; j++) setbit(I[sg + j]);
+    }
+    for (int32_t l : c->sp_label) setbit(l);
+    for (auto &b : c->cons) for (int32_t l : b.labels) setbit(l);
+    for (auto &b : c->meas) for (int32_t l : b.labels) setbit(l);
+    std::vector<int32_t> rank_base(nwbits + 1, 0);
+    for (int64_t w = 0; w < nwbits; w++) rank_base[w + 1] = rank_base[w] + __builtin_popcountll(bits[w].load());
+    const int32_t ntot = rank_base[nwbits];
+    c->ntot = ntot;
+    c->labels.assign(ntot, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t w = 0; w < nwbits; w++) {
+        uint64_t b = bits[w].load();
+        int32_t k = rank_base[w];
+        while (b) {
+            int t = __builtin_ctzll(b);
+            c->labels[k++] = (int32_t)(lmin + w * 64 + t);
+            b &= b - 1;
+        }
+    }
+    auto tix_of = [&](int32_t l) -> int32_t { // 0-based total index
+        int64_t o = (int64_t)l - lmin;
+        uint64_t b = bits[o >> 6].load(std::memory_order_relaxed);
+        return rank_base[o >> 6] + __builtin_popcountll(b & ((1ull << (o & 63)) - 1ull));
+    };
+
+    // ---- per record: sorted distinct labels (all global words) -> gl lists, widx ----
+    std::vector<int64_t> rec_gl(nrec + 1, 0);
+    std::vector<int32_t> ngl_rec(nrec, 0);
+    std::vector<int32_t> widx(c->inder.size());
+    std::copy(c->inder.begin(), c->inder.end(), widx.begin());
+    // counters and group boundaries (counting per equation or per record, nonzero derivatives)
+    std::vector<std::atomic<int64_t>> cnt(ntot);
+    for (auto &x : cnt) x.store(0, std::memory_order_relaxed);
+    std::vector<std::atomic<uint8_t>> bnd(ntot + 1);
+    for (auto &x : bnd) x.store(0, std::memory_order_relaxed);
+    auto mark_list = [&](std::vector<int32_t> &lst) {
+        if (lst.empty()) return;
+        std::sort(lst.begin(), lst.end());
+        lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
+        bnd[lst[0]].store(1, std::memory_order_relaxed);
+        for (size_t q = 1; q < lst.size(); q++)
+            if (lst[q] != lst[q - 1] + 1) { bnd[lst[q - 1] + 1].store(1, std::memory_order_relaxed); bnd[lst[q]].store(1, std::memory_order_relaxed); }
+        bnd[lst.back() + 1].store(1, std::memory_order_relaxed);
+    };
+    std::vector<std::vector<int32_t>> rec_lists(nrec);
+#pragma omp parallel
+    {
+        std::vector<int32_t> all, eql;
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t r = 0; r < nrec; r++) {
+            all.clear();
+            std::vector<int32_t> recnz;
+            for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
+                int64_t sg = eq_sl[e] + (eq_n[e] & 0xffff) + 1;
+                int ng = eq_n[e] >> 16;
+                eql.clear();
+                for (int j = 1; j <= ng; j++) {
+                    int32_t t = tix_of(I[sg + j]);
+                    all.push_back(t);
+                    if (val(sg + j) != 0.0) {
+                        if (c->opt.mcount == 0) cnt[t].fetch_add(1, std::memory_order_relaxed);
+                        eql.push_back(t);
+                        recnz.push_back(t);
+                    }
+                }
+                if (c->opt.mcount == 0) mark_list(eql);
+            }
+            if (c->opt.mcount != 0) {
+                std::sort(recnz.begin(), recnz.end());
+                recnz.erase(std::unique(recnz.begin(), recnz.end()), recnz.end());
+                for (int32_t t : recnz) cnt[t].fetch_add(1, std::memory_order_relaxed);
+                mark_list(recnz);
+            }
+            std::sort(all.begin(), all.end());
+            all.erase(std::unique(all.begin(), all.end()), all.end());
+            ngl_rec[r] = (int32_t)all.size();
+            for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
+                int64_t sg = eq_sl[e] + (eq_n[e] & 0xffff) + 1;
+                int ng = eq_n[e] >> 16;
+                for (int j = 1; j <= ng; j++) {
+                    int32_t t = tix_of(I[sg + j]);
+                    widx[sg + j] = (int32_t)(std::lower_bound(all.begin(), all.end(), t) - all.begin());
+                }
+            }
+            rec_lists[r] = all;
+        }
+    }
+    for (int64_t r = 0; r < nrec; r++) rec_gl[r + 1] = rec_gl[r] + ngl_rec[r];
+    std::vector<int32_t> gl_tix(rec_gl[nrec]);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nrec; r++) {
+        std::copy(rec_lists[r].begin(), rec_lists[r].end(), gl_tix.begin() + rec_gl[r]);
+        std::vector<int32_t>().swap(rec_lists[r]);
+    }
+    bnd[0].store(1);
+    c->counters.assign(ntot, 0);
+    for (int32_t t = 0; t < ntot; t++) c->counters[t] = cnt[t].load();
+
+    // ---- parameters: start values, pre-sigmas, variable / fixed (loop1, src/pede.f90:5920-6062) ----
+    c->par.assign(ntot, 0.0);
+    c->presig_tot.assign(ntot, 0.0);
+    for (size_t k = 0; k < c->sp_label.size(); k++) {
+        int32_t t = tix_of(c->sp_label[k]);
+        c->par[t] = c->sp_value[k];
+        c->presig_tot[t] = c->sp_presig[k];
+    }
+    c->var_index.assign(ntot, 0);
+    c->var_to_tot.clear();
+    int32_t nv = 0;
+    for (int32_t t = 0; t < ntot; t++) {
+        if (c->presig_tot[t] < 0.0) c->var_index[t] = -1;
+        else if (c->counters[t] < c->opt.mreqenf) c->var_index[t] = -2;
+        else { c->var_index[t] = ++nv; c->var_to_tot.push_back(t); }
+    }
+    c->nvar = nv;
+    if (nv <= 0) throw MpError(22, "Aborted, no variable global parameters");
+    // group start labels over total indices
+    c->group_start_label.assign(ntot, 0);
+    {
+        int32_t start = 0;
+        for (int32_t t = 0; t < ntot; t++) {
+            if (bnd[t].load()) start = t;
+            c->group_start_label[t] = c->labels[start];
+        }
+    }
+    // variable groups: consecutive variables of the same total-index group
+    c->group_of.assign(nv + 1, 0);
+    c->gfirst.clear();
+    {
+        int32_t lastgrp_start = -1;
+        for (int32_t v = 1; v <= nv; v++) {
+            int32_t t = c->var_to_tot[v - 1];
+            int32_t gs = c->group_start_label[t];
+            if (gs != lastgrp_start || c->gfirst.empty()) {
+                c->gfirst.push_back(v);
+                lastgrp_start = gs;
+            }
+            c->group_of[v] = (int32_t)c->gfirst.size() - 1;
+        }
+        c->ngroup = (int32_t)c->gfirst.size();
+        c->gfirst.push_back(nv + 1);
+    }
+    // pre-weights (src/pede.f90:6050-6062)
+    c->pre_weight.assign(nv, 0.0);
+    for (int32_t v = 1; v <= nv; v++) {
+        double ps = c->presig_tot[c->var_to_tot[v - 1]];
+        double w = 0.0;
+        if (ps > 0.0) w = 1.0 / (ps * ps);
+        else if (ps == 0.0 && c->opt.default_presigma > 0.0) w = 1.0 / (c->opt.default_presigma * c->opt.default_presigma);
+        c->pre_weight[v - 1] = (double)c->opt.regula * w;
+    }
+
+    // ---- constraints: Lagrange rows over variable indices (addcst, src/pede.f90:704-771) ----
+    c->cptr.assign(1, 0);
+    c->cidx.clear();
+    c->cval.clear();
+    c->con_tot.clear();
+    c->crhs.clear();
+    for (auto &b : c->cons) {
+        std::vector<std::pair<int32_t, double>> tl;
+        for (size_t q = 0; q < b.labels.size(); q++) tl.push_back({tix_of(b.labels[q]), b.factors[q]});
+        // rows with no variable parameter are skipped (skipemptycons semantics)
+        bool any = false;
+        for (auto &p : tl) if (c->var_index[p.first] > 0) any = true;
+        if (!any) continue;
+        c->con_tot.push_back(tl);
+        c->crhs.push_back(b.rhs);
+        // accumulate factors per variable (duplicates add, as repeated mupdat calls do)
+        std::vector<std::pair<int32_t, double>> vl;
+        for (auto &p : tl) if (c->var_index[p.first] > 0) vl.push_back({c->var_index[p.first], p.second});
+        std::sort(vl.begin(), vl.end(), [](auto &a, auto &b2) { return a.first < b2.first; });
+        for (size_t q = 0; q < vl.size();) {
+            size_t q2 = q;
+            double s = 0.0;
+            while (q2 < vl.size() && vl[q2].first == vl[q].first) s += vl[q2++].second;
+            c->cidx.push_back(vl[q].first);
+            c->cval.push_back(s);
+            q = q2;
+        }
+        c->cptr.push_back((int64_t)c->cidx.size());
+    }
+    c->ncon = c->opt.withmultipliers ? (int32_t)c->con_tot.size() : 0;
+    if (!c->opt.withmultipliers && !c->con_tot.empty())
+        throw MpError(36, "constraints need 'withmultipliers' (elimination is not provided)");
+    c->nall = c->nvar + c->ncon;
+
+    // ---- matrix structure ----
+    const bool sparse = c->opt.matsto == MPGPU_SPARSE;
+    int32_t max_gl = 1, max_eq = 1, max_nalc = 1, max_var = 1, max_seg = 1;
+    std::vector<int64_t> rec_pb(nrec + 1, 0);
+    std::vector<int32_t> pair_blk;
+    {
+        // per-record maxima and (sparse) group pairs
+        std::vector<int32_t> nseg_rec(nrec, 0);
+        const int64_t ng = c->ngroup;
+        const int64_t npairs = ng * (ng + 1) / 2;
+        std::vector<std::atomic<uint64_t>> pbits(sparse ? (npairs + 63) / 64 : 0);
+        for (auto &b : pbits) b.store(0, std::memory_order_relaxed);
+#pragma omp parallel for schedule(dynamic, 256) reduction(max : max_gl, max_eq, max_nalc, max_var, max_seg)
+        for (int64_t r = 0; r < nrec; r++) {
+            int32_t ngl = ngl_rec[r];
+            max_gl = std::max(max_gl, ngl);
+            max_eq = std::max(max_eq, (int32_t)(rec_eq[r + 1] - rec_eq[r]));
+            int nalc = 0;
+            for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
+                int nl = eq_n[e] & 0xffff;
+                for (int j = 1; j <= nl; j++) nalc = std::max(nalc, c->inder[eq_sl[e] + j]);
+            }
+            max_nalc = std::max(max_nalc, nalc);
+            int nv_r = 0, nseg = 0, lastg = -1;
+            for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
+                int v = c->var_index[gl_tix[q]];
+                if (v > 0) {
+                    nv_r++;
+                    int g = c->group_of[v];
+                    if (g != lastg) { nseg++; lastg = g; }
+                }
+            }
+            max_var = std::max(max_var, nv_r);
+            max_seg = std::max(max_seg, nseg);
+            nseg_rec[r] = nseg;
+            if (sparse) {
+                std::vector<int32_t> segs;
+                lastg = -1;
+                for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
+                    int v = c->var_index[gl_tix[q]];
+                    if (v > 0 && c->group_of[v] != lastg) { lastg = c->group_of[v]; segs.push_back(lastg); }
+                }
+                for (size_t a = 0; a < segs.size(); a++)
+                    for (size_t b = 0; b <= a; b++) {
+                        int64_t ga = segs[a], gb = segs[b];
+                        int64_t o = ga * (ga + 1) / 2 + gb;
+                        pbits[o >> 6].fetch_or(1ull << (o & 63), std::memory_order_relaxed);
+                    }
+            }
+        }
+        if (sparse) {
+            // external measurements couple their parameters too
+            for (auto &b : c->meas) {
+                std::vector<int32_t> gs;
+                for (int32_t l : b.labels) { int v = c->var_index[tix_of(l)]; if (v > 0) gs.push_back(c->group_of[v]); }
+                for (int32_t a : gs) for (int32_t b2 : gs) if (a >= b2) {
+                    int64_t o = (int64_t)a * (a + 1) / 2 + b2;
+                    pbits[o >> 6].fetch_or(1ull << (o & 63));
+                }
+            }
+            // every diagonal block exists (pre-sigma, preconditioner)
+            for (int64_t g = 0; g < ng; g++) { int64_t o = g * (g + 1) / 2 + g; pbits[o >> 6].fetch_or(1ull << (o & 63)); }
+            // rows -> blocks
+            c->brow.assign(ng + 1, 0);
+            std::vector<int64_t> rowcnt(ng, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+            for (int64_t ga = 0; ga < ng; ga++) {
+                int64_t cntb = 0;
+                for (int64_t gb = 0; gb <= ga; gb++) {
+                    int64_t o = ga * (ga + 1) / 2 + gb;
+                    if (pbits[o >> 6].load(std::memory_order_relaxed) >> (o & 63) & 1) cntb++;
+                }
+                rowcnt[ga] = cntb;
+            }
+            for (int64_t ga = 0; ga < ng; ga++) c->brow[ga + 1] = c->brow[ga] + rowcnt[ga];
+            c->nblock = c->brow[ng];
+            c->bcol.assign(c->nblock, 0);
+            c->brg.assign(c->nblock, 0);
+            c->boff.assign(c->nblock, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+            for (int64_t ga = 0; ga < ng; ga++) {
+                int64_t k = c->brow[ga];
+                for (int64_t gb = 0; gb <= ga; gb++) {
+                    int64_t o = ga * (ga + 1) / 2 + gb;
+                    if (pbits[o >> 6].load(std::memory_order_relaxed) >> (o & 63) & 1) { c->bcol[k] = (int32_t)gb; c->brg[k] = (int32_t)ga; k++; }
+                }
+            }
+            int64_t off = 0;
+            for (int64_t b = 0; b < c->nblock; b++) {
+                int ga = c->brg[b], gb = c->bcol[b];
+                int64_t nr = c->gfirst[ga + 1] - c->gfirst[ga], nc = c->gfirst[gb + 1] - c->gfirst[gb];
+                c->boff[b] = off;
+                off += nr * nc;
+            }
+            c->nstored = off;
+            // per record: block id of each segment pair
+            for (int64_t r = 0; r < nrec; r++) rec_pb[r + 1] = rec_pb[r] + (int64_t)nseg_rec[r] * (nseg_rec[r] + 1) / 2;
+            pair_blk.assign(rec_pb[nrec], -1);
+#pragma omp parallel for schedule(dynamic, 256)
+            for (int64_t r = 0; r < nrec; r++) {
+                std::vector<int32_t> segs;
+                int lastg = -1;
+                for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
+                    int v = c->var_index[gl_tix[q]];
+                    if (v > 0 && c->group_of[v] != lastg) { lastg = c->group_of[v]; segs.push_back(lastg); }
+                }
+                int64_t p = rec_pb[r];
+                for (size_t a = 0; a < segs.size(); a++)
+                    for (size_t b = 0; b <= a; b++) {
+                        int32_t ga = segs[a], gb = segs[b];
+                        auto it = std::lower_bound(c->bcol.begin() + c->brow[ga], c->bcol.begin() + c->brow[ga + 1], gb);
+                        pair_blk[p++] = (int32_t)(it - c->bcol.begin());
+                    }
+            }
+        } else {
+            int64_t n = c->nall;
+            c->nstored = n * (n + 1) / 2;
+            c->nblock = 0;
+        }
+    }
+    c->max_pair = (int64_t)max_seg * (max_seg + 1) / 2;
+    c->plan.max_gl = max_gl;
+    c->plan.max_eq = max_eq;
+    c->plan.nalc_max = std::max(1, std::min(max_nalc, 64));
+    c->plan.max_var = max_var;
+    c->plan.max_pair = sparse ? (int32_t)c->max_pair : 1;
+    lds_bytes(c->plan);
+    if (c->plan.bytes > 160 * 1024)
+        throw MpError(30, "record too large for the record kernel (LDS " + std::to_string(c->plan.bytes) + " bytes)");
+
+    // ---- device upload ----
+    hipStream_t s = c->stream;
+    c->d_rec_eq.upload(rec_eq, s);
+    c->d_rec_gl.upload(rec_gl, s);
+    c->d_eq_sl.upload(eq_sl, s);
+    c->d_eq_n.upload(eq_n, s);
+    if (c->dbl) c->d_vd.upload(c->vd, s); else c->d_vf.upload(c->vf, s);
+    c->d_widx.upload(widx, s);
+    c->d_gl_tix.upload(gl_tix, s);
+    if (sparse) {
+        c->d_rec_pb.upload(rec_pb, s);
+        c->d_pair_blk.upload(pair_blk, s);
+        c->d_brow.upload(c->brow, s);
+        c->d_bcol.upload(c->bcol, s);
+        c->d_brg.upload(c->brg, s);
+        c->d_boff.upload(c->boff, s);
+    }
+    c->d_group_of.upload(c->group_of, s);
+    c->d_gfirst.upload(c->gfirst, s);
+    c->d_var.upload(c->var_index, s);
+    c->d_par.upload(c->par, s);
+    c->d_pre_weight.upload(c->pre_weight, s);
+    c->d_mat.alloc(std::max<int64_t>(1, c->nstored));
+    c->d_vec.alloc(c->nall);
+    c->d_cnt.alloc(c->nvar);
+    c->d_chi2.alloc(std::max<int64_t>(1, nrec));
+    c->d_ndf.alloc(std::max<int64_t>(1, nrec));
+    c->d_status.alloc(std::max<int64_t>(1, nrec));
+    c->d_part.alloc(2048);
+    c->d_scal.alloc(16);
+    c->d_ndfsum.alloc(1);
+    c->d_hist.alloc(24);
+    if (c->ncon > 0) {
+        c->d_cptr.upload(c->cptr, s);
+        c->d_cidx.upload(c->cidx, s);
+        c->d_cval.upload(c->cval, s);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    c->built = true;
+    if (ntot_out) *ntot_out = c->ntot;
+    if (nvar_out) *nvar_out = c->nvar;
+    if (ncon_out) *ncon_out = c->ncon;
+    return 0;
+    API_CATCH(c)
+}
+
+extern "C" int mpgpu_get_labels(const mpgpu_ctx *c, int32_t *labels, int32_t *var_index, int64_t *counters,
+                                int32_t *group_first_label)
+{
+    if (!c || !c->built) return 30;
+    for (int32_t t = 0; t < c->ntot; t++) {
+        if (labels) labels[t] = c->labels[t];
+        if (var_index) var_index[t] = c->var_index[t];
+        if (counters) counters[t] = c->counters[t];
+        if (group_first_label) group_first_label[t] = c->group_start_label[t];
+    }
+    return 0;
+}
+
+extern "C" int mpgpu_matrix_info(const mpgpu_ctx *c, int64_t *stored, int64_t *nblocks, int32_t *ngroups)
+{
+    if (!c || !c->built) return 30;
+    if (stored) *stored = c->nstored;
+    if (nblocks) *nblocks = c->nblock;
+    if (ngroups) *ngroups = c->ngroup;
+    return 0;
+}
+
+extern "C" int mpgpu_set_parameters(mpgpu_ctx *c, const double *p)
+{
+    if (!c || !c->built) return 30;
+    API_TRY
+    std::copy(p, p + c->ntot, c->par.begin());
+    c->d_par.upload(c->par, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+    API_CATCH(c)
+}
+
+extern "C" int mpgpu_get_parameters(const mpgpu_ctx *c, double *p)
+{
+    if (!c || !c->built) return 30;
+    std::copy(c->par.begin(), c->par.end(), p);
+    return 0;
+}
+
+static DevModel model_of(mpgpu_ctx *c)
+{
+    DevModel M{};
+    M.par = c->d_par.p;
+    M.var = c->d_var.p;
+    M.nvar = c->nvar;
+    M.sparse = c->opt.matsto == MPGPU_SPARSE;
+    M.mat = c->d_mat.p;
+    M.group_of = c->d_group_of.p;
+    M.gfirst = c->d_gfirst.p;
+    M.brow = c->d_brow.p;
+    M.bcol = c->d_bcol.p;
+    M.boff = c->d_boff.p;
+    M.vec = c->d_vec.p;
+    M.cnt = c->d_cnt.p;
+    return M;
+}
+
+static DevRecords records_of(mpgpu_ctx *c)
+{
+    DevRecords R{};
+    R.nrec = c->nrec;
+    R.rec_eq = c->d_rec_eq.p;
+    R.rec_gl = c->d_rec_gl.p;
+    R.rec_pb = c->d_rec_pb.p;
+    R.eq_sl = c->d_eq_sl.p;
+    R.eq_n = c->d_eq_n.p;
+    R.vf = c->dbl ? nullptr : c->d_vf.p;
+    R.vd = c->dbl ? c->d_vd.p : nullptr;
+    R.widx = c->d_widx.p;
+    R.gl_tix = c->d_gl_tix.p;
+    R.pair_blk = c->d_pair_blk.p;
+    R.rec_w = nullptr;
+    R.max_gl = c->plan.max_gl;
+    R.max_eq = c->plan.max_eq;
+    R.max_nalc = c->plan.nalc_max;
+    R.max_var = c->plan.max_var;
+    R.max_seg = 0;
+    return R;
+}
+
+// host-side element updates routed to the device matrix (mupdat semantics)
+static void device_mupdat(mpgpu_ctx *c, const std::vector<int32_t> &ii, const std::vector<int32_t> &jj,
+                          const std::vector<double> &vv)
+{
+    if (ii.empty()) return;
+    DBuf<int32_t> di, dj;
+    DBuf<double> dv;
+    di.upload(ii, c->stream);
+    dj.upload(jj, c->stream);
+    dv.upload(vv, c->stream);
+    l_mupdat(model_of(c), (int32_t)ii.size(), di.p, dj.p, dv.p, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float chicut, mpgpu_pass_stats *st)
+{
+    if (!c || !c->built) return fail(c, 30, "structure not built");
+    API_TRY
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    hipEvent_t e0, e1, e2;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    HIPCHK(hipEventRecord(e0, s));
+    c->d_vec.zero(s);
+    c->d_cnt.zero(s);
+    if (icalcm == 1) { c->d_mat.zero(s); c->have_inverse = false; }
+    PassOpts o{};
+    o.icalcm = icalcm;
+    o.nloopn = nloopn;
+    o.lhuber = c->opt.lhuber;
+    o.lfitbb = c->opt.lfitbb;
+    o.chicut = chicut;
+    o.chhuge = c->opt.chhuge;
+    o.dwcut = c->opt.dwcut;
+    o.cndlmx = c->opt.cndlmx;
+    RecOut out{c->d_chi2.p, c->d_ndf.p, c->d_status.p};
+    DevModel M = model_of(c);
+    DevRecords R = records_of(c);
+    int grid = (int)std::min<int64_t>(c->nrec, 256 * 40);
+    HIPCHK(hipEventRecord(e1, s));
+    if (c->nrec > 0) HIPCHK(launch_records(R, M, o, out, c->plan, 0, c->nrec, std::max(grid, 1), s));
+    HIPCHK(hipEventRecord(e2, s));
+    // pass statistics
+    c->d_ndfsum.zero(s);
+    c->d_hist.zero(s);
+    l_pass_stats(c->nrec, c->d_chi2.p, c->d_ndf.p, c->d_status.p, c->d_part.p, c->d_ndfsum.p, c->d_hist.p, c->d_scal.p, s);
+    double chi2 = 0.0;
+    long long ndfsum = 0;
+    unsigned long long hist[24];
+    c->d_scal.download(&chi2, 1, s);
+    c->d_ndfsum.download(&ndfsum, 1, s);
+    c->d_hist.download(hist, 24, s);
+    HIPCHK(hipStreamSynchronize(s));
+    if (hist[9] > 0) throw MpError(30, "record exceeds record-kernel limits (" + std::to_string(hist[9]) + " records)");
+    if (hist[8] > 0)
+        throw MpError(30, "local fits with bordered-band structure are not provided by this kernel (" +
+                              std::to_string(hist[8]) + " records)");
+    // pre-sigma diagonal (icalcm=1), regularisation and external measurements (loopn 2711-2798)
+    std::vector<int32_t> ii, jj;
+    std::vector<double> vv;
+    if (icalcm == 1)
+        for (int32_t v = 1; v <= c->nvar; v++)
+            if (c->pre_weight[v - 1] != 0.0) { ii.push_back(v); jj.push_back(v); vv.push_back(c->pre_weight[v - 1]); }
+    std::vector<double> vec_add(c->nall, 0.0);
+    std::vector<unsigned long long> cnt_add(c->nvar, 0);
+    double chi_extra = 0.0;
+    int64_t ndf_extra = 0;
+    if (c->opt.nregul != 0)
+        for (int32_t v = 1; v <= c->nvar; v++) {
+            int32_t t = c->var_to_tot[v - 1];
+            vec_add[v - 1] -= c->par[t] * c->pre_weight[v - 1];
+            chi_extra += c->pre_weight[v - 1] * c->par[t] * c->par[t];
+        }
+    for (auto &b : c->meas) {
+        double weight = b.sigma > 0.0 ? 1.0 / (b.sigma * b.sigma) : 0.0;
+        double dsum = -b.rhs;
+        std::vector<int32_t> vix(b.labels.size());
+        for (size_t q = 0; q < b.labels.size(); q++) {
+            auto it = std::lower_bound(c->labels.begin(), c->labels.end(), b.labels[q]);
+            int32_t t = (int32_t)(it - c->labels.begin());
+            dsum += b.factors[q] * c->par[t];
+            vix[q] = c->var_index[t];
+        }
+        for (size_t q = 0; q < b.labels.size(); q++) {
+            if (b.factors[q] == 0.0) continue;
+            if (vix[q] > 0) {
+                vec_add[vix[q] - 1] -= weight * dsum * b.factors[q];
+                cnt_add[vix[q] - 1] += 1;
+                if (icalcm == 1)
+                    for (size_t k = 0; k <= q; k++)
+                        if (vix[k] > 0) { ii.push_back(vix[q]); jj.push_back(vix[k]); vv.push_back(weight * b.factors[q] * b.factors[k]); }
+            }
+        }
+        chi_extra += weight * dsum * dsum;
+        ndf_extra += 1;
+    }
+    // constraints (addcst, src/pede.f90:704-771): Lagrange rows + residuals
+    for (int32_t j = 0; j < c->ncon; j++) {
+        double rhs = c->crhs[j];
+        for (auto &p : c->con_tot[j]) rhs = rhs - p.second * c->par[p.first];
+        vec_add[c->nvar + j] = rhs;
+        if (icalcm == 1 && c->opt.matsto != MPGPU_SPARSE)
+            for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) {
+                ii.push_back(c->nvar + j + 1);
+                jj.push_back(c->cidx[q]);
+                vv.push_back(c->cval[q]);
+            }
+    }
+    device_mupdat(c, ii, jj, vv);
+    // fold host contributions into the device vector / counter
+    std::vector<double> hv(c->nall);
+    std::vector<unsigned long long> hc(c->nvar);
+    c->d_vec.download(hv.data(), c->nall, s);
+    c->d_cnt.download(hc.data(), c->nvar, s);
+    HIPCHK(hipStreamSynchronize(s));
+    for (int32_t i = 0; i < c->nall; i++) hv[i] += vec_add[i];
+    for (int32_t i = 0; i < c->nvar; i++) hc[i] += cnt_add[i];
+    c->d_vec.upload(hv.data(), c->nall, s);
+    c->d_cnt.upload(hc.data(), c->nvar, s);
+    c->last_vec = hv;
+    c->last_cnt = hc;
+    HIPCHK(hipStreamSynchronize(s));
+    float k_ms = 0.f, t_ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&k_ms, e1, e2));
+    HIPCHK(hipEventElapsedTime(&t_ms, e0, e2));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->chi2 = chi2 + chi_extra;
+        st->ndf = ndfsum + ndf_extra;
+        st->wndf = (double)st->ndf;
+        for (int k = 0; k < 6; k++) st->nrej[k] = (int64_t)hist[k + 1];
+        st->naccept = (int64_t)hist[0];
+        st->nrec = c->nrec;
+        st->ms_kernel = k_ms;
+        st->ms_total = t_ms;
+    }
+    return 0;
+    API_CATCH(c)
+}
+
+extern "C" int mpgpu_get_vector(const mpgpu_ctx *c, double *vec, int64_t *counter)
+{
+    if (!c || !c->built) return 30;
+    if (vec) for (int32_t i = 0; i < c->nall; i++) vec[i] = i < (int32_t)c->last_vec.size() ? c->last_vec[i] : 0.0;
+    if (counter) for (int32_t i = 0; i < c->nvar; i++) counter[i] = i < (int32_t)c->last_cnt.size() ? (int64_t)c->last_cnt[i] : 0;
+    return 0;
+}
+
+extern "C" int mpgpu_get_matrix_packed(const mpgpu_ctx *c0, double *packed, int64_t len)
+{
+    mpgpu_ctx *c = const_cast<mpgpu_ctx *>(c0);
+    if (!c || !c->built) return 30;
+    API_TRY
+    int64_t n = c->nall;
+    if (len < n * (n + 1) / 2) throw MpError(24, "output too small");
+    if (c->opt.matsto != MPGPU_SPARSE) {
+        c->d_mat.download(packed, n * (n + 1) / 2, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    std::vector<double> vals(c->nstored);
+    c->d_mat.download(vals.data(), c->nstored, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::fill(packed, packed + n * (n + 1) / 2, 0.0);
+    for (int64_t b = 0; b < c->nblock; b++) {
+        int ga = c->brg[b], gb = c->bcol[b];
+        int fa = c->gfirst[ga], fb = c->gfirst[gb];
+        int nr = c->gfirst[ga + 1] - fa, nc = c->gfirst[gb + 1] - fb;
+        for (int r = 0; r < nr; r++)
+            for (int q = 0; q < nc; q++) {
+                int64_t i = fa + r, j = fb + q;
+                if (j > i) continue;
+                packed[(i * i - i) / 2 + j - 1] = vals[c->boff[b] + (int64_t)r * nc + q];
+            }
+    }
+    for (int32_t j = 0; j < c->ncon; j++)
+        for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) {
+            int64_t i = c->nvar + j + 1, k = c->cidx[q];
+            packed[(i * i - i) / 2 + k - 1] += c->cval[q];
+        }
+    return 0;
+    API_CATCH(c)
+}
+
+extern "C" int mpgpu_mupdat(mpgpu_ctx *c, int32_t nel, const int32_t *i, const int32_t *j, const double *v)
+{
+    if (!c || !c->built) return 30;
+    API_TRY
+    std::vector<int32_t> ii(i, i + nel), jj(j, j + nel);
+    std::vector<double> vv(v, v + nel);
+    for (int32_t k = 0; k < nel; k++)
+        if (std::max(ii[k], jj[k]) > c->nall) throw MpError(23, "Aborted, bad matrix index");
+    device_mupdat(c, ii, jj, vv);
+    return 0;
+    API_CATCH(c)
+}
+
+// ---------------------------------------------------------------------------------------------
+// solve
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct Work {
+    DBuf<double> a, b, c2, d, e, f, g, h, x, tmp, part, scal;
+};
+
+void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
+{
+    hipStream_t s = c->stream;
+    if (c->opt.matsto == MPGPU_SPARSE) {
+        l_block_symv(c->ngroup, c->d_brow.p, c->d_bcol.p, c->d_boff.p, c->d_gfirst.p, c->d_brg.p, c->nblock, c->d_mat.p, x, y,
+                     c->nall, s);
+        l_cons_symv(c->ncon, c->nvar, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, x, y, s);
+    } else {
+        l_packed_symv(c->nall, c->d_mat.p, x, y, s);
+    }
+}
+
+double dot_dev(mpgpu_ctx *c, int64_t n, const double *a, const double *b)
+{
+    l_dot(n, a, b, c->d_part.p, c->d_scal.p, c->stream);
+    double r = 0.0;
+    c->d_scal.download(&r, 1, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return r;
+}
+
+// preconditioner state for MINRES
+struct Precond {
+    int kind = 0; // 0 none, 1 diagonal (+constraints), 2 band (+constraints)
+    DBuf<double> cu, sm, scratch, band, kmat, h;
+    DBuf<int> nrkd, india, hind, cb;
+    DBuf<int64_t> koff;
+    int n = 0, m = 0;
+};
+
+void setup_precond(mpgpu_ctx *c, Precond &P)
+{
+    hipStream_t s = c->stream;
+    P.n = c->nvar;
+    P.m = c->ncon;
+    DevModel M = model_of(c);
+    if (c->opt.mbandw < 0) { P.kind = 0; return; }
+    if (c->opt.mbandw == 0) {
+        P.kind = 1;
+        DBuf<double> diag;
+        diag.alloc(P.n);
+        l_get_diag(M, P.n, diag.p, s);
+        P.cu.alloc(P.n);
+        P.nrkd.alloc(1);
+        P.sm.alloc(std::max(1, (P.m * P.m + P.m) / 2));
+        P.scratch.alloc(P.n);
+        l_precons(P.n, P.m, diag.p, P.cu.p, P.nrkd.p, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, P.sm.p, P.scratch.p, s);
+        HIPCHK(hipStreamSynchronize(s));
+        return;
+    }
+    // band preconditioner (vmprep 7690-7700, equdecs)
+    P.kind = 2;
+    const int mb = c->opt.mbandw;
+    std::vector<int> india(P.n);
+    for (int i = 1; i <= std::min(mb, P.n); i++) india[i - 1] = (i * i + i) / 2;
+    for (int i = std::min(mb, P.n) + 1; i <= P.n; i++) india[i - 1] = india[i - 2] + mb;
+    P.india.upload(india, s);
+    P.band.alloc(india[P.n - 1]);
+    l_get_band(M, P.n, mb, P.india.p, P.band.p, s);
+    P.nrkd.alloc(2);
+    l_lltdec(P.n, P.band.p, P.india.p, P.nrkd.p, c->opt.lprecm, s);
+    if (P.m > 0) {
+        // constraint blocks: merge overlapping variable ranges of constraints
+        std::vector<int> first(P.m), last(P.m);
+        for (int j = 0; j < P.m; j++) {
+            first[j] = c->cidx[c->cptr[j]];
+            last[j] = c->cidx[c->cptr[j + 1] - 1];
+        }
+        std::vector<int> cb(2 * P.m);
+        std::vector<int64_t> koff(P.m + 1, 0);
+        int j = 0;
+        const int nextra = std::max(0, mb - 1);
+        while (j < P.m) {
+            int lo = first[j], hi = last[j], k = j + 1;
+            while (k < P.m && first[k] <= hi) { hi = std::max(hi, last[k]); lo = std::min(lo, first[k]); k++; }
+            for (int q = j; q < k; q++) { cb[2 * q] = lo - 1; cb[2 * q + 1] = hi - lo + 1 + nextra; }
+            j = k;
+        }
+        for (int q = 0; q < P.m; q++) koff[q + 1] = koff[q] + cb[2 * q + 1];
+        std::vector<double> kmat(koff[P.m], 0.0);
+        for (int q = 0; q < P.m; q++)
+            for (int64_t p = c->cptr[q]; p < c->cptr[q + 1]; p++) kmat[koff[q] + (c->cidx[p] - 1 - cb[2 * q])] += c->cval[p];
+        P.cb.upload(cb, s);
+        P.koff.upload(koff, s);
+        P.kmat.upload(kmat, s);
+        std::vector<int> hind(P.m);
+        for (int i = 1; i <= P.m; i++) hind[i - 1] = (i * i + i) / 2;
+        P.hind.upload(hind, s);
+        P.h.alloc((P.m * P.m + P.m) / 2);
+        l_band_k(P.m, P.n, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, s);
+        l_band_h(P.m, P.cb.p, P.koff.p, P.kmat.p, P.h.p, P.hind.p, P.nrkd.p + 1, s);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+void apply_precond(mpgpu_ctx *c, Precond &P, const double *x, double *y)
+{
+    hipStream_t s = c->stream;
+    if (P.kind == 1) {
+        l_presols(P.n, P.m, P.cu.p, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, P.sm.p, x, y, s);
+    } else if (P.kind == 2) {
+        l_copy(P.n + P.m, x, y, s);
+        l_equslvs(P.n, P.m, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, P.h.p, P.hind.p, y, s);
+    } else {
+        l_copy(P.n + P.m, x, y, s);
+    }
+}
+
+// MINRES, restating src/minresModule.f90:294-660 with device vectors and host scalars
+void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats *st)
+{
+    const int64_t n = c->nall;
+    hipStream_t s = c->stream;
+    Precond P;
+    setup_precond(c, P);
+    const bool precon = P.kind != 0;
+    DBuf<double> r1, r2, v, w, w1, w2, y;
+    r1.alloc(n); r2.alloc(n); v.alloc(n); w.alloc(n); w1.alloc(n); w2.alloc(n); y.alloc(n);
+    const double eps = 2.220446049250313e-16;
+    const double rtol = c->opt.mrestl;
+    const int itnlim = c->opt.itnlim;
+    int istop = 0, itn = 0;
+    double Anorm = 0, Acond = 0, rnorm = 0, ynorm = 0, Arnorml = 0, gmin = 0, gmax = 0;
+    HIPCHK(hipMemsetAsync(x_dev, 0, sizeof(double) * n, s));
+    l_copy(n, b_dev, y.p, s);
+    l_copy(n, b_dev, r1.p, s);
+    if (precon) apply_precond(c, P, b_dev, y.p);
+    double beta1 = dot_dev(c, n, b_dev, y.p);
+    double alfa, beta, cs, dbar, delta, denom, epsa, epsln, epsr, epsx, gamma, gbar, oldb, oldeps, qrnorm, phi, phibar, rhs1,
+        rhs2, rnorml, rootl, relArnorml, z, tnorm2, ynorm2, sn;
+    if (beta1 < 0.0) { istop = 8; goto done; }
+    if (beta1 == 0.0) { istop = 0; goto done; }
+    beta1 = std::sqrt(beta1);
+    avprod_dev(c, y.p, w.p);
+    Arnorml = std::sqrt(dot_dev(c, n, w.p, w.p));
+    oldb = 0; beta = beta1; dbar = 0; epsln = 0; qrnorm = beta1; phibar = beta1; rhs1 = beta1; rhs2 = 0;
+    tnorm2 = 0; ynorm2 = 0; cs = -1; sn = 0;
+    w.zero(s);
+    w2.zero(s);
+    l_copy(n, r1.p, r2.p, s);
+    for (;;) {
+        itn++;
+        l_scale_copy(n, 1.0 / beta, y.p, v.p, s);
+        avprod_dev(c, v.p, y.p);
+        if (itn >= 2) l_axpy_neg(n, beta / oldb, r1.p, y.p, s);
+        alfa = dot_dev(c, n, v.p, y.p);
+        l_axpy_neg(n, alfa / beta, r2.p, y.p, s);
+        std::swap(r1.p, r2.p);          // r1 = r2 (old r2 buffer becomes r1)
+        l_copy(n, y.p, r2.p, s);        // r2 = y
+        if (precon) apply_precond(c, P, r2.p, y.p);
+        oldb = beta;
+        beta = dot_dev(c, n, r2.p, y.p);
+        if (beta < 0.0) { istop = 6; break; }
+        beta = std::sqrt(beta);
+        tnorm2 = tnorm2 + alfa * alfa + oldb * oldb + beta * beta;
+        if (itn == 1) {
+            if (beta / beta1 <= 10.0 * eps) istop = -1;
+            gmax = std::fabs(alfa);
+            gmin = gmax;
+        }
+        oldeps = epsln;
+        delta = cs * dbar + sn * alfa;
+        gbar = sn * dbar - cs * alfa;
+        epsln = sn * beta;
+        dbar = -cs * beta;
+        gamma = std::sqrt(gbar * gbar + beta * beta);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        phi = cs * phibar;
+        phibar = sn * phibar;
+        denom = 1.0 / gamma;
+        l_minres_update(n, v.p, w.p, w1.p, w2.p, x_dev, oldeps, delta, denom, phi, s);
+        gmax = std::max(gmax, gamma);
+        gmin = std::min(gmin, gamma);
+        z = rhs1 / gamma;
+        ynorm2 = z * z + ynorm2;
+        rhs1 = rhs2 - delta * z;
+        rhs2 = -epsln * z;
+        Anorm = std::sqrt(tnorm2);
+        ynorm = std::sqrt(ynorm2);
+        epsa = Anorm * eps;
+        epsx = Anorm * ynorm * eps;
+        epsr = Anorm * ynorm * rtol;
+        (void)epsa;
+        qrnorm = phibar;
+        rnorml = rnorm;
+        rnorm = qrnorm;
+        rootl = std::sqrt(gbar * gbar + dbar * dbar);
+        Arnorml = rnorml * rootl;
+        relArnorml = rootl / Anorm;
+        Acond = gmax / gmin;
+        if (istop == 0) {
+            if (itn >= itnlim) istop = 5;
+            if (Acond >= 0.1 / eps) istop = 4;
+            if (epsx >= beta1) istop = 3;
+            if (qrnorm <= epsx || relArnorml <= eps) istop = 2;
+            if (qrnorm <= epsr || relArnorml <= rtol) istop = 1;
+        }
+        if (istop != 0) break;
+    }
+done:
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        st->istop = istop;
+        st->itn = itn;
+        st->anorm = Anorm;
+        st->acond = Acond;
+        st->rnorm = rnorm;
+        st->arnorm = Arnorml;
+        st->ynorm = ynorm;
+    }
+}
+
+double rel_residual(mpgpu_ctx *c, const double *b_dev, const double *x_dev)
+{
+    const int64_t n = c->nall;
+    DBuf<double> ax, r;
+    ax.alloc(n);
+    r.alloc(n);
+    avprod_dev(c, x_dev, ax.p);
+    l_sub(n, b_dev, ax.p, r.p, c->stream);
+    double rr = dot_dev(c, n, r.p, r.p), bb = dot_dev(c, n, b_dev, b_dev);
+    return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+}
+
+} // namespace
+
+extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_solve_stats *st)
+{
+    if (!c || !c->built) return fail(c, 30, "structure not built");
+    API_TRY
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int64_t n = c->nall;
+    double t0 = now_ms();
+    mpgpu_solve_stats loc{};
+    loc.method = c->opt.metsol;
+    DBuf<double> b, xd;
+    b.alloc(n);
+    xd.alloc(n);
+    if (rhs) b.upload(rhs, n, s); else b.upload(c->last_vec.data(), n, s);
+    if (c->opt.metsol == MPGPU_INVERSION) {
+        if (c->opt.matsto == MPGPU_SPARSE) throw MpError(36, "method inversion needs full matrix storage");
+        // workspaceDiag = diagonal before inversion (minver, src/pede.f90:7859-7861)
+        DBuf<double> dg;
+        dg.alloc(n);
+        l_get_diag(model_of(c), (int32_t)n, dg.p, s);
+        c->diag_before.assign(n, 0.0);
+        dg.download(c->diag_before.data(), n, s);
+        // keep the matrix for the residual check, invert a copy in place of the stored one
+        DBuf<double> keep;
+        keep.alloc(c->nstored);
+        HIPCHK(hipMemcpyAsync(keep.p, c->d_mat.p, sizeof(double) * c->nstored, hipMemcpyDeviceToDevice, s));
+        DBuf<double> diag, vk;
+        DBuf<int> used, info;
+        diag.alloc(n);
+        vk.alloc(n + 1);
+        used.alloc(n);
+        info.alloc(4);
+        HIPCHK(hipMemcpyAsync(xd.p, b.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        l_gj_init((int)n, c->d_mat.p, diag.p, used.p, info.p, s);
+        for (int64_t step = 0; step < n; step++) l_gj_step((int)n, c->d_mat.p, diag.p, used.p, info.p, vk.p, xd.p, s);
+        l_gj_finish((int)n, c->d_mat.p, xd.p, used.p, info.p, s);
+        int hinfo[4];
+        info.download(hinfo, 4, s);
+        HIPCHK(hipStreamSynchronize(s));
+        loc.nrank = hinfo[1];
+        // residual with the kept matrix: swap buffers
+        std::swap(keep.p, c->d_mat.p);
+        loc.rel_residual = rel_residual(c, b.p, xd.p);
+        std::swap(keep.p, c->d_mat.p);
+        c->have_inverse = true;
+    } else if (c->opt.metsol == MPGPU_MINRES || c->opt.metsol == MPGPU_MINRES_QLP) {
+        minres_dev(c, b.p, xd.p, &loc);
+        loc.rel_residual = rel_residual(c, b.p, xd.p);
+    } else if (c->opt.metsol == MPGPU_DECOMPOSITION || c->opt.metsol == MPGPU_CHOLESKY_FP32_IR) {
+        if (c->opt.matsto == MPGPU_SPARSE) throw MpError(36, "method decomposition needs full matrix storage");
+        const int nv = c->nvar, m = c->ncon;
+        const int64_t lda = nv;
+        DBuf<double> A, work;
+        DBuf<int> info;
+        A.alloc((size_t)lda * nv);
+        work.alloc(1 << 20);
+        info.alloc(4);
+        // parameter block of the KKT matrix (leading nv x nv of the packed matrix)
+        l_packed_to_full(nv, c->d_mat.p, A.p, (int)lda, s);
+        const bool fp32 = c->opt.metsol == MPGPU_CHOLESKY_FP32_IR;
+        DBuf<float> A32;
+        DBuf<float> xf;
+        if (fp32) { A32.alloc((size_t)lda * nv); xf.alloc(nv); }
+        int rc = chol_factor(nv, A.p, (int)lda, work.p, info.p, s, fp32 ? 1 : 0, fp32 ? A32.p : nullptr);
+        if (rc > 0) throw MpError(29, "Aborted, matrix factorisation failed (not positive definite at " + std::to_string(rc) + ")");
+        loc.nrank = nv;
+        // solve with constraints via the Schur complement S = C A^-1 C^T (block elimination)
+        auto solveA = [&](double *v) {
+            if (!fp32) { chol_solve(nv, A.p, (int)lda, v, s); return; }
+            // FP32 factor + FP64 iterative refinement against the FP64 parameter block
+            DBuf<double> r, d, ax;
+            r.alloc(nv); d.alloc(nv); ax.alloc(nv);
+            l_copy(nv, v, r.p, s);                    // r = rhs
+            HIPCHK(hipMemsetAsync(d.p, 0, sizeof(double) * nv, s));
+            DBuf<double> xacc;
+            xacc.alloc(nv);
+            HIPCHK(hipMemsetAsync(xacc.p, 0, sizeof(double) * nv, s));
+            for (int it = 0; it <= c->opt.refine_steps; it++) {
+                l_copy(nv, r.p, d.p, s);
+                chol_solve32(nv, A32.p, (int)lda, d.p, xf.p, s);
+                l_axpy_neg(nv, -1.0, d.p, xacc.p, s); // xacc += d
+                // r = rhs - A_pp xacc, with the FP64 packed matrix restricted to the parameter block
+                l_packed_symv(nv, c->d_mat.p, xacc.p, ax.p, s);
+                l_sub(nv, v, ax.p, r.p, s);
+                double rr = dot_dev(c, nv, r.p, r.p), bb = dot_dev(c, nv, v, v);
+                loc.itn = it + 1;
+                if (bb == 0.0 || std::sqrt(rr / bb) < 1e-15) break;
+            }
+            l_copy(nv, xacc.p, v, s);
+        };
+        HIPCHK(hipMemcpyAsync(xd.p, b.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        if (m == 0) {
+            solveA(xd.p);
+        } else {
+            // Z = A^-1 C^T (m columns), S = C Z, lambda = S^-1 (C A^-1 b_p - b_c), x = A^-1 (b_p - C^T lambda)
+            std::vector<double> Ct((size_t)m * nv, 0.0);
+            for (int j = 0; j < m; j++)
+                for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) Ct[(size_t)j * nv + c->cidx[q] - 1] += c->cval[q];
+            std::vector<double> Z(Ct);
+            DBuf<double> zc;
+            zc.alloc(nv);
+            for (int j = 0; j < m; j++) {
+                zc.upload(Z.data() + (size_t)j * nv, nv, s);
+                solveA(zc.p);
+                zc.download(Z.data() + (size_t)j * nv, nv, s);
+            }
+            std::vector<double> bp(n);
+            b.download(bp.data(), n, s);
+            HIPCHK(hipStreamSynchronize(s));
+            zc.upload(bp.data(), nv, s);
+            solveA(zc.p);
+            std::vector<double> y(nv);
+            zc.download(y.data(), nv, s);
+            HIPCHK(hipStreamSynchronize(s));
+            // S (m x m) and rhs
+            std::vector<double> S((size_t)m * m), g(m);
+            for (int i = 0; i < m; i++) {
+                for (int j = 0; j < m; j++) {
+                    double acc = 0.0;
+                    for (int64_t q = c->cptr[i]; q < c->cptr[i + 1]; q++) acc += c->cval[q] * Z[(size_t)j * nv + c->cidx[q] - 1];
+                    S[(size_t)i * m + j] = acc;
+                }
+                double acc = 0.0;
+                for (int64_t q = c->cptr[i]; q < c->cptr[i + 1]; q++) acc += c->cval[q] * y[c->cidx[q] - 1];
+                g[i] = acc - bp[nv + i];
+            }
+            // dense symmetric solve of S lambda = g (Gaussian elimination with partial pivoting, m small)
+            std::vector<double> Aug(S);
+            std::vector<int> piv(m);
+            for (int i = 0; i < m; i++) piv[i] = i;
+            for (int k = 0; k < m; k++) {
+                int p = k;
+                for (int i = k + 1; i < m; i++) if (std::fabs(Aug[(size_t)i * m + k]) > std::fabs(Aug[(size_t)p * m + k])) p = i;
+                if (Aug[(size_t)p * m + k] == 0.0) throw MpError(29, "singular constraint system");
+                if (p != k) { for (int j = 0; j < m; j++) std::swap(Aug[(size_t)k * m + j], Aug[(size_t)p * m + j]); std::swap(g[k], g[p]); }
+                for (int i = k + 1; i < m; i++) {
+                    double f = Aug[(size_t)i * m + k] / Aug[(size_t)k * m + k];
+                    for (int j = k; j < m; j++) Aug[(size_t)i * m + j] -= f * Aug[(size_t)k * m + j];
+                    g[i] -= f * g[k];
+                }
+            }
+            std::vector<double> lam(m);
+            for (int i = m - 1; i >= 0; i--) {
+                double acc = g[i];
+                for (int j = i + 1; j < m; j++) acc -= Aug[(size_t)i * m + j] * lam[j];
+                lam[i] = acc / Aug[(size_t)i * m + i];
+            }
+            std::vector<double> xs(n);
+            for (int i = 0; i < nv; i++) {
+                double acc = y[i];
+                for (int j = 0; j < m; j++) acc -= Z[(size_t)j * nv + i] * lam[j];
+                xs[i] = acc;
+            }
+            for (int j = 0; j < m; j++) xs[nv + j] = lam[j];
+            xd.upload(xs.data(), n, s);
+        }
+        loc.rel_residual = rel_residual(c, b.p, xd.p);
+    } else {
+        throw MpError(36, "Unsupported solution method");
+    }
+    xd.download(x, n, s);
+    HIPCHK(hipStreamSynchronize(s));
+    loc.ms = now_ms() - t0;
+    if (st) *st = loc;
+    return 0;
+    API_CATCH(c)
+}
+
+extern "C" int mpgpu_avprod(mpgpu_ctx *c, const double *x, double *y)
+{
+    if (!c || !c->built) return 30;
+    API_TRY
+    if (c->have_inverse) throw MpError(30, "matrix holds its inverse after an inversion solve");
+    DBuf<double> xd, yd;
+    xd.upload(x, c->nall, c->stream);
+    yd.alloc(c->nall);
+    avprod_dev(c, xd.p, yd.p);
+    yd.download(y, c->nall, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+    API_CATCH(c)
+}
+
+extern "C" int mpgpu_get_errors(const mpgpu_ctx *c0, double *err, double *gcor)
+{
+    mpgpu_ctx *c = const_cast<mpgpu_ctx *>(c0);
+    if (!c || !c->built) return 30;
+    API_TRY
+    if (!c->have_inverse) throw MpError(30, "errors need an inversion solve");
+    DBuf<double> dg;
+    dg.alloc(c->nvar);
+    l_get_diag(model_of(c), c->nvar, dg.p, c->stream);
+    std::vector<double> d(c->nvar);
+    dg.download(d.data(), c->nvar, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // prtglo (src/pede.f90:4536-4546): err = sign * sqrt(|A^-1_ii|) in REAL(4)
+    for (int32_t v = 0; v < c->nvar; v++) {
+        double gmati = d[v];
+        float e = std::sqrt(std::fabs((float)gmati));
+        if (gmati < 0.0) e = -e;
+        err[v] = e;
+        if (gcor) {
+            float g = -1.0f;
+            double diag = c->diag_before[v];
+            if (gmati * diag > 0.0) {
+                double g2 = 1.0 - 1.0 / (gmati * diag);
+                if (g2 >= 0.0 && g2 <= 1.0) g = (float)std::sqrt(g2);
+            }
+            gcor[v] = g;
+        }
+    }
+    return 0;
+    API_CATCH(c)
+}

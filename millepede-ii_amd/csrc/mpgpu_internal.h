@@ -1,0 +1,113 @@
+// Host-side internals of libmpgpu (not part of the C-ABI).
+#pragma once
+#include "mpgpu_device.h"
+#include "../../include/mpgpu.h"
+#include <string>
+#include <vector>
+#include <cstring>
+
+namespace mpgpu {
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) throw MpError(30, std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                                    " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+struct MpError {
+    int code;
+    std::string msg;
+    MpError(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// owning device buffer
+template <typename T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        HIPCHK(hipMalloc((void **)&p, sizeof(T) * count));
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count) HIPCHK(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    void download(T *h, size_t count, hipStream_t s) const
+    {
+        if (count) HIPCHK(hipMemcpyAsync(h, p, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+    }
+    void zero(hipStream_t s)
+    {
+        if (n) HIPCHK(hipMemsetAsync(p, 0, sizeof(T) * n, s));
+    }
+};
+
+struct EqBlock {
+    int kind; // 0 constraint, 1 measurement
+    double rhs, sigma;
+    std::vector<int32_t> labels;
+    std::vector<double> factors;
+};
+
+// kernels (k_records.hip, k_solve.hip, k_chol.hip)
+hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out, const LdsPlan &P,
+                          int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s);
+int lds_bytes(LdsPlan &P);
+void l_mupdat(const DevModel &M, int32_t n, const int32_t *i, const int32_t *j, const double *v, hipStream_t s);
+void l_get_diag(const DevModel &M, int32_t n, double *d, hipStream_t s);
+void l_get_band(const DevModel &M, int32_t n, int32_t mbandw, const int32_t *india, double *c, hipStream_t s);
+void l_dot(int64_t n, const double *a, const double *b, double *part, double *out, hipStream_t s);
+void l_pass_stats(int64_t n, const double *chi2, const int32_t *ndf, const int8_t *st, double *part, long long *ndfsum,
+                  unsigned long long *hist, double *chi_out, hipStream_t s);
+void l_gj_init(int n, const double *v, double *diag, int *used, int *info, hipStream_t s);
+void l_gj_step(int n, double *v, const double *diag, int *used, int *info, double *vk, double *b, hipStream_t s);
+void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, hipStream_t s);
+void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream_t s);
+void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const int32_t *gfirst,
+                  const int32_t *brg, int64_t nblock, const double *a, const double *x, double *y, int64_t nall,
+                  hipStream_t s);
+void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
+                 double *y, hipStream_t s);
+void l_scale_copy(int64_t n, double sc, const double *a, double *o, hipStream_t s);
+void l_axpy_neg(int64_t n, double c, const double *r, double *y, hipStream_t s);
+void l_minres_update(int64_t n, const double *v, double *w, double *w1, double *w2, double *x, double oldeps, double delta,
+                     double denom, double phi, hipStream_t s);
+void l_copy(int64_t n, const double *a, double *b, hipStream_t s);
+void l_sub(int64_t n, const double *b, const double *ax, double *r, hipStream_t s);
+void l_precons(int32_t n, int32_t p, const double *diag, double *cu, int *nrkd, const int64_t *cptr, const int32_t *cidx,
+               const double *cval, double *sm, double *scratch, hipStream_t s);
+void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, const int32_t *cidx, const double *cval,
+               const double *sm, const double *y, double *x, hipStream_t s);
+void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, hipStream_t s);
+void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
+              hipStream_t s);
+void l_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h, const int *hind, int *nrkd2,
+              hipStream_t s);
+void l_equslvs(int n, int m, const double *c, const int *india, const int *cb, const int64_t *koff, const double *kmat,
+               const double *h, const int *hind, double *x, hipStream_t s);
+
+// dense Cholesky (k_chol.hip): full column-major n x n matrix (lda = n)
+int chol_factor(int n, double *a, int lda, double *work, int *info_dev, hipStream_t s, int fp32, float *a32);
+void chol_solve(int n, const double *a, int lda, double *x, hipStream_t s);
+void chol_solve32(int n, const float *a, int lda, double *x, float *xf, hipStream_t s);
+void l_packed_to_full(int n, const double *packed, double *full, int lda, hipStream_t s);
+void l_full_to_f32(int64_t nn, const double *a, float *b, hipStream_t s);
+
+} // namespace mpgpu

@@ -1,0 +1,106 @@
+"""Pure-Python restatement of pede's label bookkeeping for small inputs (loop1,
+src/pede.f90:5777-6146; decodeNextMeasurement 2428-2468; updateParGroups/pargrp 2157-2404).
+
+TEST INFRASTRUCTURE ONLY (checker for the library's index maps).
+
+Group partition: pede builds parameter groups incrementally (pargrp) while reading; the
+partition it reaches is "a boundary between adjacent labels k, k+1 (in ascending label
+order) wherever some equation (or record, with countrecords) holds exactly one of the two".
+That rule is what this module computes; it is pinned by brute force below (every
+equation's label set is a union of whole groups, and no two adjacent groups could be
+merged without breaking that).
+"""
+import numpy as np
+
+
+def decode(rec_ptr, inder, glder):
+    """Per record: list of equations (sL, sG, lG) as absolute word indices (0-based)."""
+    recs = []
+    for r in range(len(rec_ptr) - 1):
+        w0, w1 = int(rec_ptr[r]), int(rec_ptr[r + 1])
+        nw = w1 - w0
+        I = lambda k: int(inder[w0 + k - 1])
+        V = lambda k: float(glder[w0 + k - 1])
+        eqs = []
+        last, end = 1, nw
+        while last < end:
+            last += 1
+            while I(last) != 0:
+                last += 1
+            sL = last
+            last += 1
+            while I(last) != 0:
+                last += 1
+            sG = last
+            if sL + 1 == sG and V(sG) < 0.0:
+                last += int(round(-V(sG)))
+                continue
+            while last < end and I(last + 1) != 0:
+                last += 1
+            eqs.append((w0 + sL - 1, w0 + sG - 1, w0 + last - 1))
+        recs.append(eqs)
+    return recs
+
+
+def label_maps(rec_ptr, inder, glder, steer_labels=(), presigma=None, mreqenf=25, mcount=0, extra_labels=()):
+    """labels (sorted), var_index (>0 variable, -1 fixed by pre-sigma, -2 entries cut),
+    counters, group start label per label."""
+    recs = decode(rec_ptr, inder, glder)
+    labs = set(int(x) for x in steer_labels) | set(int(x) for x in extra_labels)
+    for eqs in recs:
+        for (sL, sG, lG) in eqs:
+            for w in range(sG + 1, lG + 1):
+                labs.add(int(inder[w]))
+    labels = np.array(sorted(labs), dtype=np.int32)
+    pos = {int(l): i for i, l in enumerate(labels)}
+    n = len(labels)
+    cnt = np.zeros(n, np.int64)
+    bnd = np.zeros(n + 1, bool)
+    bnd[0] = True
+
+    def mark(lst):
+        if not lst:
+            return
+        lst = sorted(set(lst))
+        bnd[lst[0]] = True
+        for a, b in zip(lst[:-1], lst[1:]):
+            if b != a + 1:
+                bnd[a + 1] = True
+                bnd[b] = True
+        bnd[lst[-1] + 1] = True
+
+    for eqs in recs:
+        recl = []
+        for (sL, sG, lG) in eqs:
+            el = [pos[int(inder[w])] for w in range(sG + 1, lG + 1) if glder[w] != 0.0]
+            if mcount == 0:
+                for t in el:
+                    cnt[t] += 1
+                mark(el)
+            recl += el
+        if mcount != 0:
+            u = sorted(set(recl))
+            for t in u:
+                cnt[t] += 1
+            mark(u)
+    ps = np.zeros(n)
+    if presigma is not None:
+        for l, s in zip(steer_labels, presigma):
+            ps[pos[int(l)]] = s
+    var = np.zeros(n, np.int32)
+    k = 0
+    for t in range(n):
+        if ps[t] < 0:
+            var[t] = -1
+        elif cnt[t] < mreqenf:
+            var[t] = -2
+        else:
+            k += 1
+            var[t] = k
+    gsl = np.zeros(n, np.int32)
+    start = 0
+    for t in range(n):
+        if bnd[t]:
+            start = t
+        gsl[t] = labels[start]
+    return labels, var, cnt, gsl

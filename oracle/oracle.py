@@ -1,0 +1,313 @@
+"""ctypes access to the CPU oracle (pede_oracle.c) and to the reference's own numeric
+routines compiled from src/*.f90 (oracle/_ref/libpedenum.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product library.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF = os.path.join(HERE, "_ref", "libpedenum.so")
+REF_GEN = os.path.join(HERE, "_ref", "mptest1_gen")
+
+D = C.POINTER(C.c_double)
+I32 = C.POINTER(C.c_int32)
+I64 = C.POINTER(C.c_int64)
+
+
+def build(reference_root="/root/reference"):
+    """Compile the oracle (gcc) and, when the reference sources are present, the reference's
+    numeric routines (flang) into oracle/_ref."""
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-o", LIB + ".tmp",
+                    os.path.join(HERE, "pede_oracle.c"), "-lm"], check=True)
+    os.replace(LIB + ".tmp", LIB)
+    if os.path.isdir(os.path.join(reference_root, "src")):
+        subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile.ref"), f"REF={reference_root}"],
+                       check=True, cwd=HERE)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class _RecStruct(C.Structure):
+    _fields_ = [("nrec", C.c_int64), ("rec_ptr", I64), ("inder", I32), ("glder", D),
+                ("ntot", C.c_int32), ("labels", I32), ("var_index", I32), ("global_par", D),
+                ("nvar", C.c_int32)]
+
+
+class _PassOpts(C.Structure):
+    _fields_ = [("icalcm", C.c_int32), ("nloopn", C.c_int32), ("lhuber", C.c_int32), ("lfitbb", C.c_int32),
+                ("chicut", C.c_float), ("chhuge", C.c_float), ("dwcut", C.c_float), ("cndlmx", C.c_float),
+                ("nthreads", C.c_int32)]
+
+
+class _PassStats(C.Structure):
+    _fields_ = [("chi2", C.c_double), ("ndf", C.c_int64), ("wndf", C.c_double), ("nrej", C.c_int64 * 6),
+                ("naccept", C.c_int64), ("nbandrec", C.c_int64)]
+
+
+OP = C.CFUNCTYPE(None, C.c_void_p, C.c_int, D, D)
+
+
+class Oracle:
+    def __init__(self):
+        if not os.path.exists(LIB):
+            build()
+        self.lib = C.CDLL(LIB)
+        L = self.lib
+        L.orc_chindl.restype = C.c_float
+        L.orc_chindl.argtypes = [C.c_int, C.c_int]
+        L.orc_sqminv.argtypes = [D, D, C.c_int, I32]
+        L.orc_sqminl.argtypes = [D, D, C.c_int, I32]
+        L.orc_chdec2.argtypes = [D, C.c_int, I32, D, D]
+        L.orc_chslv2.argtypes = [D, D, C.c_int]
+        L.orc_dbavat.argtypes = [D, D, D, C.c_int, C.c_int]
+        L.orc_lltdec.argtypes = [C.c_int, D, I32, I32, C.c_int]
+        L.orc_lltfwd.argtypes = [C.c_int, D, I32, D]
+        L.orc_lltbwd.argtypes = [C.c_int, D, I32, D]
+        L.orc_equdecs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, D, I32, I32, I32, I32]
+        L.orc_equslvs.argtypes = [C.c_int, C.c_int, C.c_int, D, I32, I32, D]
+        L.orc_precons.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D, I32, D, I32]
+        L.orc_presols.argtypes = [C.c_int, C.c_int, C.c_int, D, D, I32, D, D, D]
+        L.orc_minres.argtypes = [C.c_int, OP, OP, C.c_void_p, D, C.c_double, C.c_int, D, C.c_int, C.c_double,
+                                 I32, I32, D]
+        L.orc_packed_matvec.argtypes = [C.c_int, D, D, D]
+        L.orc_loopn.argtypes = [C.POINTER(_RecStruct), C.POINTER(_PassOpts), D, D, D, I64, C.POINTER(_PassStats)]
+        L.orc_loopn.restype = C.c_int
+        L.orc_read_mille.argtypes = [C.c_char_p, C.c_int, C.POINTER(I64), C.POINTER(I32), C.POINTER(D)]
+        L.orc_read_mille.restype = C.c_int64
+        L.orc_free.argtypes = [C.c_void_p]
+
+    # ---- dense numerics ------------------------------------------------------------------------
+    def chindl(self, n, nd):
+        return self.lib.orc_chindl(n, nd)
+
+    def sqminv(self, v, b):
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64).copy()
+        r = C.c_int32()
+        self.lib.orc_sqminv(_p(v, D), _p(b, D), len(b), C.byref(r))
+        return v, b, r.value
+
+    def sqminl(self, v, b):
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64).copy()
+        r = C.c_int32()
+        self.lib.orc_sqminl(_p(v, D), _p(b, D), len(b), C.byref(r))
+        return v, b, r.value
+
+    def chdec2(self, g, n):
+        g = np.ascontiguousarray(g, dtype=np.float64).copy()
+        r = C.c_int32()
+        e1, e2 = C.c_double(), C.c_double()
+        self.lib.orc_chdec2(_p(g, D), n, C.byref(r), C.byref(e1), C.byref(e2))
+        return g, r.value, e1.value, e2.value
+
+    def chslv2(self, g, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        self.lib.orc_chslv2(_p(g, D), _p(x, D), len(x))
+        return x
+
+    def dbavat(self, v, a, w, n, m):
+        w = np.ascontiguousarray(w, dtype=np.float64).copy()
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.lib.orc_dbavat(_p(v, D), _p(a, D), _p(w, D), n, m)
+        return w
+
+    def packed_matvec(self, a, x):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.lib.orc_packed_matvec(len(x), _p(a, D), _p(x, D), _p(y, D))
+        return y
+
+    def minres(self, n, aprod, msolve, b, rtol, itnlim=2000, precon=True):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(n)
+
+        def _ap(ctx, nn, xp, yp):
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] = aprod(xa)
+
+        def _ms(ctx, nn, xp, yp):
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] = msolve(xa) if msolve else xa
+
+        fa, fm = OP(_ap), OP(_ms)
+        istop, itn = C.c_int32(), C.c_int32()
+        out = np.zeros(5)
+        self.lib.orc_minres(n, fa, fm, None, _p(b, D), 0.0, 1 if precon else 0, _p(x, D), itnlim, rtol,
+                            C.byref(istop), C.byref(itn), _p(out, D))
+        return x, istop.value, itn.value, out
+
+    # ---- record loop ---------------------------------------------------------------------------
+    def loopn(self, rec_ptr, inder, glder, labels, var_index, par, nvar, icalcm=1, nloopn=1, lhuber=0,
+              chicut=0.0, chhuge=50.0, dwcut=0.0, cndlmx=100.0, lfitbb=1, pre_weight=None):
+        rec_ptr = np.ascontiguousarray(rec_ptr, dtype=np.int64)
+        inder = np.ascontiguousarray(inder, dtype=np.int32)
+        glder = np.ascontiguousarray(glder, dtype=np.float64)
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        var_index = np.ascontiguousarray(var_index, dtype=np.int32)
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        R = _RecStruct(len(rec_ptr) - 1, _p(rec_ptr, I64), _p(inder, I32), _p(glder, D), len(labels),
+                       _p(labels, I32), _p(var_index, I32), _p(par, D), nvar)
+        O = _PassOpts(icalcm, nloopn, lhuber, lfitbb, chicut, chhuge, dwcut, cndlmx, 1)
+        S = _PassStats()
+        mat = np.zeros(nvar * (nvar + 1) // 2 if icalcm == 1 else 1)
+        vec = np.zeros(nvar)
+        cnt = np.zeros(nvar, dtype=np.int64)
+        pw = None if pre_weight is None else np.ascontiguousarray(pre_weight, dtype=np.float64)
+        rc = self.lib.orc_loopn(C.byref(R), C.byref(O), _p(pw, D) if pw is not None else None,
+                                _p(mat, D) if icalcm == 1 else None, _p(vec, D), _p(cnt, I64), C.byref(S))
+        if rc != 0:
+            raise RuntimeError(f"oracle loopn failed with code {rc}")
+        stats = dict(chi2=S.chi2, ndf=S.ndf, nrej=list(S.nrej), naccept=S.naccept, nband=S.nbandrec)
+        return mat, vec, cnt, stats
+
+    def read_mille(self, path, fortran=False):
+        rp, ii, gg = I64(), I32(), D()
+        n = self.lib.orc_read_mille(path.encode(), 1 if fortran else 0, C.byref(rp), C.byref(ii), C.byref(gg))
+        if n < 0:
+            raise IOError(f"oracle read_mille failed ({n})")
+        rec_ptr = np.ctypeslib.as_array(rp, shape=(n + 1,)).copy()
+        nw = int(rec_ptr[-1])
+        inder = np.ctypeslib.as_array(ii, shape=(max(nw, 1),))[:nw].copy()
+        glder = np.ctypeslib.as_array(gg, shape=(max(nw, 1),))[:nw].copy()
+        for p in (rp, ii, gg):
+            self.lib.orc_free(C.cast(p, C.c_void_p))
+        return rec_ptr, inder, glder
+
+
+class Reference:
+    """The reference's Fortran numeric routines (src/mpnum.f90, src/Dbandmatrix.f90,
+    src/minresModule.f90) built with flang; all arguments by reference."""
+
+    def __init__(self):
+        if not os.path.exists(REF):
+            raise FileNotFoundError(REF)
+        self.lib = C.CDLL(REF)
+
+    @staticmethod
+    def _i(v):
+        return C.byref(C.c_int32(v))
+
+    def sqminv(self, v, b):
+        v = np.asfortranarray(v, dtype=np.float64).copy()
+        b = np.asfortranarray(b, dtype=np.float64).copy()
+        n = len(b)
+        nr = C.c_int32()
+        diag = np.zeros(n)
+        nxt = np.zeros(n, dtype=np.int32)
+        self.lib.sqminv_(_p(v, D), _p(b, D), self._i(n), C.byref(nr), _p(diag, D), _p(nxt, I32))
+        return v, b, nr.value
+
+    def sqminl(self, v, b):
+        v = np.asfortranarray(v, dtype=np.float64).copy()
+        b = np.asfortranarray(b, dtype=np.float64).copy()
+        n = len(b)
+        nr = C.c_int32()
+        diag, vk = np.zeros(n), np.zeros(n)
+        nxt = np.zeros(n, dtype=np.int32)
+        self.lib.sqminl_(_p(v, D), _p(b, D), self._i(n), C.byref(nr), _p(diag, D), _p(nxt, I32), _p(vk, D),
+                         self._i(0))
+        return v, b, nr.value
+
+    def chdec2(self, g, n):
+        g = np.asfortranarray(g, dtype=np.float64).copy()
+        nr = C.c_int32()
+        e1, e2 = C.c_double(), C.c_double()
+        self.lib.chdec2_(_p(g, D), self._i(n), C.byref(nr), C.byref(e1), C.byref(e2), self._i(0))
+        return g, nr.value, e1.value, e2.value
+
+    def chslv2(self, g, x):
+        x = np.asfortranarray(x, dtype=np.float64).copy()
+        g = np.asfortranarray(g, dtype=np.float64)
+        self.lib.chslv2_(_p(g, D), _p(x, D), self._i(len(x)))
+        return x
+
+    def dbavat(self, v, a, w, n, m):
+        w = np.asfortranarray(w, dtype=np.float64).copy()
+        self.lib.dbavat_(_p(np.asfortranarray(v, dtype=np.float64), D), _p(np.asfortranarray(a, dtype=np.float64), D),
+                         _p(w, D), self._i(n), self._i(m), self._i(1))
+        return w
+
+    def chindl(self, n, nd):
+        f = self.lib.chindl_
+        f.restype = C.c_float
+        return f(self._i(n), self._i(nd))
+
+    def lltdec(self, n, c, india, iopt=0):
+        c = np.asfortranarray(c, dtype=np.float64).copy()
+        india = np.asfortranarray(india, dtype=np.int32)
+        nr = C.c_int32()
+        self.lib.lltdec_(self._i(n), _p(c, D), _p(india, I32), C.byref(nr), self._i(iopt))
+        return c, nr.value
+
+    def equdecs(self, n, m, b, nm, ls, c, india, l):
+        c = np.asfortranarray(c, dtype=np.float64).copy()
+        india = np.asfortranarray(india, dtype=np.int32).copy()
+        l = np.asfortranarray(l, dtype=np.int32)
+        nr, nr2 = C.c_int32(), C.c_int32()
+        self.lib.equdecs_(self._i(n), self._i(m), self._i(b), C.byref(C.c_int64(nm)), self._i(ls), _p(c, D),
+                          _p(india, I32), _p(l, I32), C.byref(nr), C.byref(nr2))
+        return c, india, nr.value, nr2.value
+
+    def equslvs(self, n, m, b, nm, c, india, l, x):
+        x = np.asfortranarray(x, dtype=np.float64).copy()
+        self.lib.equslvs_(self._i(n), self._i(m), self._i(b), C.byref(C.c_int64(nm)),
+                          _p(np.asfortranarray(c, dtype=np.float64), D), _p(np.asfortranarray(india, dtype=np.int32), I32),
+                          _p(np.asfortranarray(l, dtype=np.int32), I32), _p(x, D))
+        return x
+
+    def precons(self, p, n, b, nm, c, a, l):
+        cu = np.zeros(n)
+        a = np.asfortranarray(a, dtype=np.float64).copy()
+        s = np.zeros(max(1, (p * p + p) // 2))
+        nr = C.c_int32()
+        c = np.asfortranarray(c, dtype=np.float64)
+        self.lib.precons_(self._i(p), self._i(n), self._i(b), C.byref(C.c_int64(nm)), _p(c, D), _p(cu, D), _p(a, D),
+                          _p(np.asfortranarray(l, dtype=np.int32), I32), _p(s, D), C.byref(nr))
+        return cu, a, s, nr.value
+
+    def presols(self, p, n, b, nm, cu, a, l, s, y):
+        x = np.zeros(n + p)
+        y = np.asfortranarray(y, dtype=np.float64)
+        self.lib.presols_(self._i(p), self._i(n), self._i(b), C.byref(C.c_int64(nm)), _p(cu, D), _p(a, D),
+                          _p(np.asfortranarray(l, dtype=np.int32), I32), _p(s, D), _p(x, D), _p(y, D))
+        return x
+
+    def minres(self, n, aprod, msolve, b, rtol, itnlim=2000, precon=True):
+        FOP = C.CFUNCTYPE(None, I32, D, D)
+
+        def _ap(np_, xp, yp):
+            nn = np_[0]
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] = aprod(xa)
+
+        def _ms(np_, xp, yp):
+            nn = np_[0]
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] = msolve(xa) if msolve else xa
+
+        fa, fm = FOP(_ap), FOP(_ms)
+        b = np.asfortranarray(b, dtype=np.float64)
+        x = np.zeros(n)
+        istop, itn = C.c_int32(), C.c_int32()
+        out = [C.c_double() for _ in range(5)]
+        f = self.lib._QMminresmodulePminres
+        f(self._i(n), fa, fm, _p(b, D), C.byref(C.c_double(0.0)), C.byref(C.c_int32(0)),
+          C.byref(C.c_int32(1 if precon else 0)), _p(x, D), self._i(itnlim), self._i(0), C.byref(C.c_double(rtol)),
+          C.byref(istop), C.byref(itn), *[C.byref(o) for o in out])
+        return x, istop.value, itn.value, np.array([o.value for o in out])

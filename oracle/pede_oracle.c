@@ -1,0 +1,1039 @@
+/*
+ * pede_oracle.c -- CPU restatement of Millepede-II pede's global normal-matrix build and
+ * global solve.  TEST INFRASTRUCTURE ONLY: this file is the checker the GPU product is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it; the product library never links or calls it.
+ *
+ * Every function names the reference routine (path:line, relative to the millepede-ii source
+ * tree) whose arithmetic it restates.  Matrices use pede's symmetric packed storage:
+ * element (i,j), i>=j, 1-based, lives at (i*i-i)/2+j (src/pede.f90:4204, src/mpnum.f90:97).
+ *
+ * Pinning (how each function is checked, see tests/test_oracle_*.py):
+ *   orc_sqminv, orc_sqminl, orc_chdec2/chslv2, orc_dbavat, orc_chindl, orc_lltdec/fwd/bwd,
+ *   orc_equdecs/equslvs, orc_precons/presols, orc_minres
+ *       -> bit-for-bit (or 1e-13) against the reference Fortran routines compiled from
+ *          src/mpnum.f90 / src/minresModule.f90 into oracle/_ref, plus numpy closed forms.
+ *   orc_loopn (record loop)
+ *       -> brute-force joint least-squares solve (globals + every record's locals) on tiny
+ *          inputs (Schur identity, doc/draftManual.md:291-330), per-record explicit inverse.
+ *   orc_read_mille -> round trip against the Python/C writers and the layout of
+ *          src/mille.f90:208-219 and python/readMilleBinary.py:128-216.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#define IDX(i, j) ((((int64_t)(i)) * (i) - (i)) / 2 + (j)) /* 1-based packed (i>=j) */
+
+/* ------------------------------------------------------------------------------------------ */
+/* chindl: chi^2/ndf cut values, REAL(4) table (src/mpnum.f90:2078-2115)                       */
+/* ------------------------------------------------------------------------------------------ */
+static const float chindl_sn[3] = {0.47523f, 1.690140f, 2.782170f};
+static const float chindl_table[3][30] = {
+    {1.0000f, 1.1479f, 1.1753f, 1.1798f, 1.1775f, 1.1730f, 1.1680f, 1.1630f, 1.1581f, 1.1536f,
+     1.1493f, 1.1454f, 1.1417f, 1.1383f, 1.1351f, 1.1321f, 1.1293f, 1.1266f, 1.1242f, 1.1218f,
+     1.1196f, 1.1175f, 1.1155f, 1.1136f, 1.1119f, 1.1101f, 1.1085f, 1.1070f, 1.1055f, 1.1040f},
+    {4.0000f, 3.0900f, 2.6750f, 2.4290f, 2.2628f, 2.1415f, 2.0481f, 1.9736f, 1.9124f, 1.8610f,
+     1.8171f, 1.7791f, 1.7457f, 1.7161f, 1.6897f, 1.6658f, 1.6442f, 1.6246f, 1.6065f, 1.5899f,
+     1.5745f, 1.5603f, 1.5470f, 1.5346f, 1.5230f, 1.5120f, 1.5017f, 1.4920f, 1.4829f, 1.4742f},
+    {9.0000f, 5.9146f, 4.7184f, 4.0628f, 3.6410f, 3.3436f, 3.1209f, 2.9468f, 2.8063f, 2.6902f,
+     2.5922f, 2.5082f, 2.4352f, 2.3711f, 2.3143f, 2.2635f, 2.2178f, 2.1764f, 2.1386f, 2.1040f,
+     2.0722f, 2.0428f, 2.0155f, 1.9901f, 1.9665f, 1.9443f, 1.9235f, 1.9040f, 1.8855f, 1.8681f}};
+
+float orc_chindl(int n, int nd)
+{
+    if (nd < 1) return 0.0f;
+    int m = n < 1 ? 1 : (n > 3 ? 3 : n);
+    if (nd <= 30) return chindl_table[m - 1][nd - 1];
+    float t = chindl_sn[m - 1] + sqrtf((float)(nd + nd - 1));
+    return t * t / (float)(nd + nd);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* sqminv: symmetric Gauss-Jordan inversion with diagonal pivoting (src/mpnum.f90:97-213)      */
+/* v packed (n(n+1)/2), b(n) -> inverse and solution in place                                 */
+/* ------------------------------------------------------------------------------------------ */
+void orc_sqminv(double *v, double *b, int n, int *nrank)
+{
+    double *diag = (double *)malloc(sizeof(double) * (n + 1));
+    int *next = (int *)malloc(sizeof(int) * (n + 1));
+    double eps = 16.0 * DBL_EPSILON;
+    int next0 = 1, l = 1;
+    v--; b--; /* 1-based */
+    for (int i = 1; i <= n; i++) {
+        next[i] = i + 1;
+        diag[i] = fabs(v[(i * i + i) / 2]);
+    }
+    next[n] = -1;
+    *nrank = 0;
+    for (int i = 1; i <= n; i++) {
+        int k = 0;
+        double vkk = 0.0;
+        int j = next0, last = 0;
+        while (j > 0) {
+            int jj = (j * j + j) / 2;
+            double m = fabs(vkk) > eps * diag[j] ? fabs(vkk) : eps * diag[j];
+            if (fabs(v[jj]) > m) { vkk = v[jj]; k = j; l = last; }
+            last = j;
+            j = next[last];
+        }
+        if (k != 0) {
+            int kk = (k * k + k) / 2;
+            if (l == 0) next0 = next[k]; else next[l] = next[k];
+            next[k] = 0;
+            (*nrank)++;
+            vkk = 1.0 / vkk;
+            v[kk] = -vkk;
+            b[k] = b[k] * vkk;
+            int jk = kk - k, jl = 0;
+            for (int jx = 1; jx <= n; jx++) {
+                if (jx == k) { jk = kk; jl = jl + jx; }
+                else {
+                    if (jx < k) jk = jk + 1; else jk = jk + jx - 1;
+                    double vjk = v[jk];
+                    v[jk] = vkk * vjk;
+                    b[jx] = b[jx] - b[k] * vjk;
+                    int lk = kk - k;
+                    for (int lx = 1; lx <= jx; lx++) {
+                        jl = jl + 1;
+                        if (lx == k) lk = kk;
+                        else {
+                            if (lx < k) lk = lk + 1; else lk = lk + lx - 1;
+                            v[jl] = v[jl] - v[lk] * vjk;
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int kx = 1; kx <= n; kx++) {
+                if (next[kx] != 0) {
+                    b[kx] = 0.0;
+                    for (int jx = 1; jx <= kx; jx++)
+                        if (next[jx] != 0) v[(kx * kx - kx) / 2 + jx] = 0.0;
+                }
+            }
+            break;
+        }
+    }
+    for (int ij = 1; ij <= (n * n + n) / 2; ij++) v[ij] = -v[ij];
+    free(diag);
+    free(next);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* sqminl: Gauss-Jordan for large matrices, relative pivot eps 1e-10 (src/mpnum.f90:230-355)   */
+/* ------------------------------------------------------------------------------------------ */
+void orc_sqminl(double *v, double *b, int n, int *nrank)
+{
+    double *diag = (double *)malloc(sizeof(double) * (n + 1));
+    double *vk = (double *)malloc(sizeof(double) * (n + 1));
+    int *next = (int *)malloc(sizeof(int) * (n + 1));
+    const double eps = 1.0e-10;
+    int next0 = 1, l = 1;
+    v--; b--;
+    for (int i = 1; i <= n; i++) {
+        int64_t i8 = i;
+        next[i] = i + 1;
+        diag[i] = fabs(v[(i8 * i8 + i8) / 2]);
+    }
+    next[n] = -1;
+    *nrank = 0;
+    for (int i = 1; i <= n; i++) {
+        int k = 0;
+        double vkk = 0.0;
+        int j = next0, last = 0;
+        while (j > 0) {
+            int64_t j8 = j, jj = (j8 * j8 + j8) / 2;
+            double m = fabs(vkk) > eps * diag[j] ? fabs(vkk) : eps * diag[j];
+            if (fabs(v[jj]) > m) { vkk = v[jj]; k = j; l = last; }
+            last = j;
+            j = next[last];
+        }
+        if (k != 0) {
+            int64_t k8 = k, kk = (k8 * k8 + k8) / 2, kkmk = kk - k8;
+            if (l == 0) next0 = next[k]; else next[l] = next[k];
+            next[k] = 0;
+            (*nrank)++;
+            vkk = 1.0 / vkk;
+            v[kk] = -vkk;
+            b[k] = b[k] * vkk;
+            int64_t jk = kkmk;
+            for (int jx = 1; jx <= n; jx++) {
+                if (jx == k) { jk = kk; vk[jx] = 0.0; }
+                else {
+                    if (jx < k) jk = jk + 1; else jk = jk + jx - 1;
+                    v[jk] = v[jk] * vkk;
+                    vk[jx] = v[jk];
+                }
+            }
+            for (int jx = n; jx >= 1; jx--) {
+                if (jx == k) continue;
+                int64_t j8 = jx, jl = j8 * (j8 - 1) / 2;
+                double vjk = vk[jx] / vkk;
+                b[jx] = b[jx] - b[k] * vjk;
+                for (int lx = 1; lx <= jx; lx++) v[jl + lx] = v[jl + lx] - vk[lx] * vjk;
+            }
+        } else {
+            for (int kx = 1; kx <= n; kx++) {
+                int64_t k8 = kx, kk = (k8 * k8 - k8) / 2;
+                if (next[kx] != 0) {
+                    b[kx] = 0.0;
+                    for (int jx = 1; jx <= kx; jx++)
+                        if (next[jx] != 0) v[kk + jx] = 0.0;
+                }
+            }
+            break;
+        }
+    }
+    int64_t nt = ((int64_t)n * (n + 1)) / 2;
+    for (int64_t jj = 1; jj <= nt; jj++) v[jj] = -v[jj];
+    free(diag);
+    free(vk);
+    free(next);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* chdec2 / chslv2: root-free LDL^T from the last row (src/mpnum.f90:891-987)                  */
+/* ------------------------------------------------------------------------------------------ */
+void orc_chdec2(double *g, int n, int *nrank, double *evmax, double *evmin)
+{
+    g--;
+    *nrank = 0;
+    *evmax = 0.0;
+    *evmin = 0.0;
+    int64_t ii = ((int64_t)n * (n + 1)) / 2;
+    for (int i = n; i >= 1; i--) {
+        if (g[ii] > 0.0) {
+            (*nrank)++;
+            if (*nrank == 1) { *evmax = g[ii]; *evmin = g[ii]; }
+            else {
+                if (g[ii] > *evmax) *evmax = g[ii];
+                if (g[ii] < *evmin) *evmin = g[ii];
+            }
+            g[ii] = 1.0 / g[ii];
+        }
+        ii = ii - i;
+        for (int j = 1; j <= i - 1; j++) {
+            double ratio = g[ii + j] * g[ii + i];
+            if (ratio == 0.0) continue;
+            int64_t jj = ((int64_t)(j - 1) * j) / 2;
+            for (int k = 1; k <= j; k++) g[jj + k] = g[jj + k] - g[ii + k] * ratio;
+        }
+        for (int k = 1; k <= i - 1; k++) g[ii + k] = g[ii + k] * g[ii + i];
+    }
+}
+
+void orc_chslv2(const double *g, double *x, int n)
+{
+    g--; x--;
+    int64_t ii = ((int64_t)n * (n + 1)) / 2;
+    for (int i = n; i >= 1; i--) {
+        double dsum = x[i];
+        int64_t kk = ii;
+        for (int k = i + 1; k <= n; k++) { dsum = dsum - g[kk + i] * x[k]; kk = kk + k; }
+        x[i] = dsum;
+        ii = ii - i;
+    }
+    for (int i = 1; i <= n; i++) {
+        double dsum = x[i] * g[ii + i];
+        for (int k = 1; k <= i - 1; k++) dsum = dsum - g[k + ii] * x[k];
+        x[i] = dsum;
+        ii = ii + i;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* dbavat (iopt=1): W += A V A^T, A (m x n) row-major = localGlobalMatrix (src/mpnum.f90:1389)  */
+/* ------------------------------------------------------------------------------------------ */
+void orc_dbavat(const double *v, const double *a, double *w, int n, int m)
+{
+    v--; a--; w--;
+    int il = -n, ijs = 0;
+    for (int i = 1; i <= m; i++) {
+        ijs = ijs + i - 1;
+        il = il + n;
+        int lkl = 0;
+        for (int k = 1; k <= n; k++) {
+            double cik = 0.0;
+            lkl = lkl + k - 1;
+            int lk = lkl;
+            for (int l = 1; l <= k; l++) { lk = lk + 1; cik = cik + a[il + l] * v[lk]; }
+            for (int l = k + 1; l <= n; l++) { lk = lk + l - 1; cik = cik + a[il + l] * v[lk]; }
+            int jk = k, ij = ijs;
+            for (int j = 1; j <= i; j++) { ij = ij + 1; w[ij] = w[ij] + cik * a[jk]; jk = jk + n; }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Variable-band Cholesky used as MINRES preconditioner (src/mpnum.f90:2155-2336)              */
+/* india(n) 1-based pointers to the diagonal elements                                         */
+/* ------------------------------------------------------------------------------------------ */
+void orc_lltdec(int n, double *c, const int *india, int *nrkd, int iopt)
+{
+    c--; india--;
+    double eps = 16.0 * DBL_EPSILON;
+    *nrkd = 0;
+    double diag = 0.0;
+    if (c[india[1]] > 0.0) c[india[1]] = 1.0 / sqrt(c[india[1]]);
+    else { c[india[1]] = 0.0; *nrkd = -1; }
+    for (int k = 2; k <= n; k++) {
+        int mk = k - india[k] + india[k - 1] + 1;
+        for (int j = mk; j <= k; j++) {
+            int mj = 1;
+            if (j > 1) mj = j - india[j] + india[j - 1] + 1;
+            int kj = india[k] - k + j;
+            diag = c[india[j]];
+            for (int i = (mj > mk ? mj : mk); i <= j - 1; i++)
+                c[kj] = c[kj] - c[india[k] - k + i] * c[india[j] - j + i];
+            if (j != k) c[kj] = c[kj] * diag;
+        }
+        if (c[india[k]] > eps * diag) c[india[k]] = 1.0 / sqrt(c[india[k]]);
+        else {
+            for (int j = mk; j <= k; j++) c[india[k] - k + j] = 0.0;
+            if (iopt > 0 && diag > 0.0) c[india[k]] = 1.0 / sqrt(diag);
+            else {
+                if (*nrkd == 0) *nrkd = -k;
+                else { if (*nrkd < 0) *nrkd = 1; (*nrkd)++; }
+            }
+        }
+    }
+}
+
+void orc_lltfwd(int n, const double *c, const int *india, double *x)
+{
+    c--; india--; x--;
+    x[1] = x[1] * c[india[1]];
+    for (int k = 2; k <= n; k++) {
+        for (int j = k - india[k] + india[k - 1] + 1; j <= k - 1; j++) x[k] = x[k] - c[india[k] - k + j] * x[j];
+        x[k] = x[k] * c[india[k]];
+    }
+}
+
+static void orc_lltfwds(int n, const double *c, const int *india, double *x, int i0, int ns)
+{
+    c--; india--; x--;
+    int kmax = ns < n - i0 ? ns : n - i0;
+    for (int k = 1; k <= kmax; k++) {
+        int l = k + i0;
+        if (l > 1) {
+            int j0 = k - india[l] + india[l - 1] + 1;
+            if (j0 < 1) j0 = 1;
+            for (int j = j0; j <= k - 1; j++) x[k] = x[k] - c[india[l] - k + j] * x[j];
+        }
+        x[k] = x[k] * c[india[l]];
+    }
+}
+
+void orc_lltbwd(int n, const double *c, const int *india, double *x)
+{
+    c--; india--; x--;
+    for (int k = n; k >= 2; k--) {
+        x[k] = x[k] * c[india[k]];
+        for (int j = k - india[k] + india[k - 1] + 1; j <= k - 1; j++) x[j] = x[j] - c[india[k] - k + j] * x[k];
+    }
+    x[1] = x[1] * c[india[1]];
+}
+
+/* equdecs: band decomposition + constraint Schur block (src/mpnum.f90:2486-2606).
+ * c layout: [band (india(n) words)] [K K^T packed m(m+1)/2] [constraint blocks]
+ * india has n+m entries; l(4*b) = (first con, last con, param offset, ncols) per block (1-based). */
+void orc_equdecs(int n, int m, int b, int ls, double *c, int *india, const int *l, int *nrkd, int *nrkd2)
+{
+    *nrkd = 0;
+    *nrkd2 = 0;
+    orc_lltdec(n, c, india, nrkd, ls);
+    if (m > 0) {
+        double *C = c - 1;
+        const int *L = l - 1;
+        int *IND = india - 1;
+        int64_t ij = IND[n] + (m * m + m) / 2;
+        for (int ib = 1; ib <= b; ib++) {
+            int p1 = L[ib * 4 - 3], p2 = L[ib * 4 - 2], i0 = L[ib * 4 - 1], in = L[ib * 4];
+            for (int j = p1; j <= p2; j++) { orc_lltfwds(n, c, india, &C[ij + 1], i0, in); ij += in; }
+        }
+        ij = IND[n] + (m * m + m) / 2;
+        for (int ib = 1; ib <= b; ib++) {
+            int p1 = L[ib * 4 - 3], p2 = L[ib * 4 - 2], i0 = L[ib * 4 - 1], in = L[ib * 4];
+            for (int i = 1; i <= in; i++) {
+                ij = ij + 1;
+                for (int j = p1; j <= p2; j++) {
+                    int64_t jk = IND[n] + (j * j - j) / 2 + p1;
+                    for (int k = p1; k <= j; k++) {
+                        C[jk] = C[jk] + C[ij + (int64_t)(j - p1) * in] * C[ij + (int64_t)(k - p1) * in];
+                        jk++;
+                    }
+                }
+            }
+            int64_t jj = ij + (int64_t)(p2 - p1) * in;
+            for (int jb = ib + 1; jb <= b; jb++) {
+                int q1 = L[jb * 4 - 3], q2 = L[jb * 4 - 2], j0 = L[jb * 4 - 1], jn = L[jb * 4];
+                int nc = in + i0 - j0 < jn ? in + i0 - j0 : jn;
+                if (nc < 1) break;
+                int64_t kk = ij + j0 - i0 - in;
+                for (int i = 1; i <= nc; i++) {
+                    kk = kk + 1;
+                    for (int j = q1; j <= q2; j++) {
+                        int64_t jk = IND[n] + (j * j - j) / 2 + p1;
+                        for (int k = p1; k <= p2; k++) {
+                            C[jk] = C[jk] + C[jj + (int64_t)(j - q1) * jn + i] * C[kk + (int64_t)(k - p1) * in];
+                            jk++;
+                        }
+                    }
+                }
+                jj = jj + (int64_t)(q2 + 1 - q1) * jn;
+            }
+            ij = ij + (int64_t)(p2 - p1) * in;
+        }
+        IND[n + 1] = 1;
+        for (int i = 2; i <= m; i++) IND[n + i] = IND[n + i - 1] + (i < m ? i : m);
+        orc_lltdec(m, &C[IND[n] + 1], &IND[n + 1], nrkd2, 0);
+    }
+}
+
+void orc_equslvs(int n, int m, int b, const double *c, const int *india, const int *l, double *x)
+{
+    orc_lltfwd(n, c, india, x);
+    if (m > 0) {
+        const double *C = c - 1;
+        const int *L = l - 1, *IND = india - 1;
+        double *X = x - 1;
+        int64_t ij = IND[n] + (m * m + m) / 2;
+        for (int ib = 1; ib <= b; ib++) {
+            int p1 = L[ib * 4 - 3], p2 = L[ib * 4 - 2], i0 = L[ib * 4 - 1], in = L[ib * 4];
+            for (int j = p1; j <= p2; j++)
+                for (int i = 1; i <= in; i++) { ij++; X[n + j] = X[n + j] - C[ij] * X[i + i0]; }
+        }
+        orc_lltfwd(m, &C[IND[n] + 1], &IND[n + 1], &X[n + 1]);
+        orc_lltbwd(m, &C[IND[n] + 1], &IND[n + 1], &X[n + 1]);
+        for (int i = 1; i <= m; i++) X[n + i] = -X[n + i];
+        ij = IND[n] + (m * m + m) / 2;
+        for (int ib = 1; ib <= b; ib++) {
+            int p1 = L[ib * 4 - 3], p2 = L[ib * 4 - 2], i0 = L[ib * 4 - 1], in = L[ib * 4];
+            for (int j = p1; j <= p2; j++)
+                for (int i = 1; i <= in; i++) { ij++; X[i + i0] = X[i + i0] - C[ij] * X[n + j]; }
+        }
+    }
+    orc_lltbwd(n, c, india, x);
+}
+
+/* precons / presols: diagonal preconditioner with constraint blocks (src/mpnum.f90:2881-3069) */
+void orc_precons(int p, int n, int b, const double *c, double *cu, double *a, const int *l, double *s, int *nrkd)
+{
+    const double *C = c - 1;
+    double *CU = cu - 1, *A = a - 1, *S = s - 1;
+    const int *L = l - 1;
+    *nrkd = 0;
+    for (int i = 1; i <= (p * p + p) / 2; i++) S[i] = 0.0;
+    for (int i = 1; i <= n; i++) {
+        double div = C[i];
+        if (div > 0.0) CU[i] = 1.0 / sqrt(div);
+        else { CU[i] = 0.0; (*nrkd)++; }
+    }
+    int64_t ij = 0;
+    for (int ib = 1; ib <= b; ib++) {
+        int p1 = L[ib * 4 - 3], p2 = L[ib * 4 - 2], i0 = L[ib * 4 - 1], np = L[ib * 4];
+        for (int i = 1; i <= np; i++) {
+            ij++;
+            for (int j = p1; j <= p2; j++) {
+                int jk = (j * j - j) / 2 + p1;
+                A[ij + (int64_t)(j - p1) * np] = A[ij + (int64_t)(j - p1) * np] * CU[i + i0];
+                for (int k = p1; k <= j; k++) {
+                    S[jk] = S[jk] + A[ij + (int64_t)(j - p1) * np] * A[ij + (int64_t)(k - p1) * np];
+                    jk++;
+                }
+            }
+        }
+        ij = ij + (int64_t)(p2 - p1) * np;
+    }
+    int ii = 0;
+    for (int i = 1; i <= p; i++) {
+        ii = ii + i;
+        if (S[ii] != 0.0) S[ii] = 1.0 / S[ii];
+        int jj = ii;
+        for (int j = i + 1; j <= p; j++) {
+            double ratio = S[i + jj] * S[ii];
+            int kk = jj;
+            for (int k = j; k <= p; k++) { S[kk + j] = S[kk + j] - S[kk + i] * ratio; kk = kk + k; }
+            S[i + jj] = ratio;
+            jj = jj + j;
+        }
+    }
+}
+
+void orc_presols(int p, int n, int b, const double *cu, const double *a, const int *l, const double *s,
+                 double *x, const double *y)
+{
+    const double *CU = cu - 1, *A = a - 1, *S = s - 1, *Y = y - 1;
+    const int *L = l - 1;
+    double *X = x - 1;
+    for (int i = 1; i <= n + p; i++) X[i] = Y[i];
+    for (int i = 1; i <= n; i++) X[i] = X[i] * CU[i];
+    int64_t ij = 0;
+    for (int ib = 1; ib <= b; ib++) {
+        int p1 = L[ib * 4 - 3], p2 = L[ib * 4 - 2], i0 = L[ib * 4 - 1], np = L[ib * 4];
+        for (int j = p1; j <= p2; j++)
+            for (int i = 1; i <= np; i++) { ij++; X[n + j] = X[n + j] - A[ij] * X[i + i0]; }
+    }
+    int jj = 0;
+    for (int j = 1; j <= p; j++) {
+        double dsum = X[n + j];
+        for (int k = 1; k <= j - 1; k++) dsum = dsum - S[k + jj] * X[n + k];
+        X[n + j] = dsum;
+        jj = jj + j;
+    }
+    for (int j = p; j >= 1; j--) {
+        double dsum = X[n + j] * S[jj];
+        int kk = jj;
+        for (int k = j + 1; k <= p; k++) { dsum = dsum + S[kk + j] * X[n + k]; kk = kk + k; }
+        X[n + j] = -dsum;
+        jj = jj - j;
+    }
+    ij = 0;
+    for (int ib = 1; ib <= b; ib++) {
+        int p1 = L[ib * 4 - 3], p2 = L[ib * 4 - 2], i0 = L[ib * 4 - 1], np = L[ib * 4];
+        for (int j = p1; j <= p2; j++)
+            for (int i = 1; i <= np; i++) { ij++; X[i + i0] = X[i + i0] - A[ij] * X[n + j]; }
+    }
+    for (int i = 1; i <= n; i++) X[i] = X[i] * CU[i];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* MINRES (Paige & Saunders), as in src/minresModule.f90:294-660 (checkA=.false.)              */
+/* ------------------------------------------------------------------------------------------ */
+typedef void (*orc_op)(void *ctx, int n, const double *x, double *y);
+
+static double dotp(int n, const double *a, const double *b)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+void orc_minres(int n, orc_op aprod, orc_op msolve, void *ctx, const double *b, double shift, int precon,
+                double *x, int itnlim, double rtol, int *istop_out, int *itn_out, double *out5 /* Anorm,Acond,rnorm,Arnorm,ynorm */)
+{
+    double *r1 = calloc(n, 8), *r2 = calloc(n, 8), *v = calloc(n, 8), *w = calloc(n, 8), *w1 = calloc(n, 8),
+           *w2 = calloc(n, 8), *y = calloc(n, 8);
+    double eps = DBL_EPSILON;
+    int istop = 0, itn = 0;
+    double Anorm = 0, Acond = 0, rnorm = 0, ynorm = 0, Arnorml = 0, gmin = 0, gmax = 0;
+    double alfa, beta, beta1, cs, dbar, delta, denom, diag, epsa, epsln, epsr, epsx, gamma, gbar, oldb, oldeps,
+        qrnorm, phi, phibar, rhs1, rhs2, rnorml, rootl, relArnorml, s, sn, tnorm2, ynorm2, z;
+    for (int i = 0; i < n; i++) { x[i] = 0.0; y[i] = b[i]; r1[i] = b[i]; }
+    if (precon) msolve(ctx, n, b, y);
+    beta1 = dotp(n, b, y);
+    if (beta1 < 0.0) { istop = 8; goto done; }
+    if (beta1 == 0.0) { istop = 0; goto done; }
+    beta1 = sqrt(beta1);
+    aprod(ctx, n, y, w);
+    Arnorml = sqrt(dotp(n, w, w));
+    oldb = 0; beta = beta1; dbar = 0; epsln = 0; qrnorm = beta1; phibar = beta1; rhs1 = beta1; rhs2 = 0;
+    tnorm2 = 0; ynorm2 = 0; cs = -1; sn = 0;
+    for (int i = 0; i < n; i++) { w[i] = 0; w2[i] = 0; r2[i] = r1[i]; }
+    for (;;) {
+        itn++;
+        s = 1.0 / beta;
+        for (int i = 0; i < n; i++) v[i] = s * y[i];
+        aprod(ctx, n, v, y);
+        for (int i = 0; i < n; i++) y[i] = y[i] - shift * v[i];
+        if (itn >= 2)
+            for (int i = 0; i < n; i++) y[i] = y[i] - (beta / oldb) * r1[i];
+        alfa = dotp(n, v, y);
+        for (int i = 0; i < n; i++) y[i] = y[i] - (alfa / beta) * r2[i];
+        for (int i = 0; i < n; i++) { r1[i] = r2[i]; r2[i] = y[i]; }
+        if (precon) msolve(ctx, n, r2, y);
+        oldb = beta;
+        beta = dotp(n, r2, y);
+        if (beta < 0.0) { istop = 6; goto done; }
+        beta = sqrt(beta);
+        tnorm2 = tnorm2 + alfa * alfa + oldb * oldb + beta * beta;
+        if (itn == 1) {
+            if (beta / beta1 <= 10.0 * eps) istop = -1;
+            gmax = fabs(alfa);
+            gmin = gmax;
+        }
+        oldeps = epsln;
+        delta = cs * dbar + sn * alfa;
+        gbar = sn * dbar - cs * alfa;
+        epsln = sn * beta;
+        dbar = -cs * beta;
+        gamma = sqrt(gbar * gbar + beta * beta);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        phi = cs * phibar;
+        phibar = sn * phibar;
+        denom = 1.0 / gamma;
+        for (int i = 0; i < n; i++) {
+            w1[i] = w2[i];
+            w2[i] = w[i];
+            w[i] = (v[i] - oldeps * w1[i] - delta * w2[i]) * denom;
+            x[i] = x[i] + phi * w[i];
+        }
+        gmax = gmax > gamma ? gmax : gamma;
+        gmin = gmin < gamma ? gmin : gamma;
+        z = rhs1 / gamma;
+        ynorm2 = z * z + ynorm2;
+        rhs1 = rhs2 - delta * z;
+        rhs2 = -epsln * z;
+        Anorm = sqrt(tnorm2);
+        ynorm = sqrt(ynorm2);
+        epsa = Anorm * eps;
+        epsx = Anorm * ynorm * eps;
+        epsr = Anorm * ynorm * rtol;
+        diag = gbar;
+        if (diag == 0.0) diag = epsa;
+        (void)diag;
+        qrnorm = phibar;
+        rnorml = rnorm;
+        rnorm = qrnorm;
+        rootl = sqrt(gbar * gbar + dbar * dbar);
+        Arnorml = rnorml * rootl;
+        relArnorml = rootl / Anorm;
+        Acond = gmax / gmin;
+        if (istop == 0) {
+            if (itn >= itnlim) istop = 5;
+            if (Acond >= 0.1 / eps) istop = 4;
+            if (epsx >= beta1) istop = 3;
+            if (qrnorm <= epsx || relArnorml <= eps) istop = 2;
+            if (qrnorm <= epsr || relArnorml <= rtol) istop = 1;
+        }
+        if (istop != 0) break;
+    }
+done:
+    if (out5) { out5[0] = Anorm; out5[1] = Acond; out5[2] = rnorm; out5[3] = Arnorml; out5[4] = ynorm; }
+    *istop_out = istop;
+    *itn_out = itn;
+    free(r1); free(r2); free(v); free(w); free(w1); free(w2); free(y);
+}
+
+/* Dense packed symmetric product y = A x for full storage (avprd0 matsto=1 path,
+ * src/pede.f90:5069-5232): every stored off-diagonal element serves both halves. */
+void orc_packed_matvec(int n, const double *a, const double *x, double *y)
+{
+    for (int i = 0; i < n; i++) y[i] = 0.0;
+    for (int i = 1; i <= n; i++) {
+        int64_t off = IDX(i, 0);
+        double s = 0.0;
+        for (int j = 1; j < i; j++) {
+            double aij = a[off + j - 1];
+            s += aij * x[j - 1];
+            y[j - 1] += aij * x[i - 1];
+        }
+        y[i - 1] += s + a[off + i - 1] * x[i - 1];
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Record loop: loopbf restated (src/pede.f90:3387-4442) with decodeNextMeasurement            */
+/* (src/pede.f90:2428-2468), addsums (11666-11689) and mgupdt's full-storage flush             */
+/* (3336-3356). Only the full (sqminv) local-fit path is restated; records whose local matrix  */
+/* qualifies for the bordered-band path (nalc>5 and cheaper by the cost model 3838-3874) are    */
+/* reported with status 2 so that a caller never compares against an unrestated branch.        */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int64_t nrec;
+    const int64_t *rec_ptr; /* record r occupies words [rec_ptr[r], rec_ptr[r+1]) (word 1 = header) */
+    const int32_t *inder;
+    const double *glder;
+    int32_t ntot;                /* number of labels (total parameters) */
+    const int32_t *labels;       /* sorted ascending, ntot */
+    const int32_t *var_index;    /* per total index (0-based array): >0 variable index (1-based) */
+    const double *global_par;    /* per total index */
+    int32_t nvar;
+} orc_records;
+
+typedef struct {
+    int32_t icalcm, nloopn, lhuber, lfitbb;
+    float chicut, chhuge, dwcut, cndlmx;
+    int32_t nthreads;
+} orc_pass_opts;
+
+typedef struct {
+    double chi2;        /* Sum(chi^2) as getsums (fvalue) */
+    int64_t ndf;
+    double wndf;
+    int64_t nrej[6];
+    int64_t naccept;
+    int64_t nbandrec;   /* records that would take the bordered-band path (not restated here) */
+} orc_pass_stats;
+
+static int label_to_index(const orc_records *R, int32_t label)
+{
+    int lo = 0, hi = R->ntot - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        int32_t v = R->labels[mid];
+        if (v == label) return mid;
+        if (v < label) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+typedef struct { double chi_frac; int64_t chi_int; int64_t ndf; double wndf; } orc_sums;
+
+static void addsums(orc_sums *s, double chi2, int ndf, double dw)
+{
+    double add = chi2 * dw;
+    int64_t nadd = (int64_t)add;
+    s->chi_int += nadd;
+    s->chi_frac += add - (double)nadd;
+    if (s->chi_frac > 16.0) { s->chi_frac -= 16.0; s->chi_int += 16; }
+    s->ndf += ndf;
+    s->wndf += (double)ndf * dw;
+}
+
+/* one record; returns 0 ok (accepted or rejected, see stats), 20 bad record structure */
+static int loop_record(const orc_records *R, int64_t r, const orc_pass_opts *o, double *mat, double *vec,
+                       int64_t *counter, orc_sums *sums, int64_t nrej[6], int64_t *naccept, int64_t *nband,
+                       int *back /* nvar+1 scratch, zero on entry/exit */)
+{
+    const double chuber = 1.345, cauchy = 2.3849;
+    int64_t base = R->rec_ptr[r] - 1; /* so that word w (1-based) is at base+w */
+    int nw = (int)(R->rec_ptr[r + 1] - R->rec_ptr[r]);
+    const int32_t *I = R->inder + base;
+    const double *Din = R->glder + base;
+    double *D = (double *)malloc(sizeof(double) * (nw + 1));
+    for (int w = 1; w <= nw; w++) D[w] = Din[w];
+    int *tix = (int *)malloc(sizeof(int) * (nw + 1)); /* total index per global word (0-based), -1 unknown */
+    int cap = 64, neq = 0;
+    int *eqs = (int *)malloc(sizeof(int) * 3 * cap);
+    int nalc = 0, nalg = 0, rc = 0;
+    int *gidx = (int *)malloc(sizeof(int) * (nw + 1));
+    int lastGlobal = 1, endOfEntry = nw;
+    /* ---- first loop: decode, subtract global contribution, collect globals ---- */
+    for (;;) {
+        int startLocal = 0, startGlobal = 0;
+        if (lastGlobal >= endOfEntry) break;
+        do { lastGlobal++; if (lastGlobal > nw) { rc = 20; goto out; } } while (I[lastGlobal] != 0);
+        startLocal = lastGlobal;
+        do { lastGlobal++; if (lastGlobal > nw) { rc = 20; goto out; } } while (I[lastGlobal] != 0);
+        startGlobal = lastGlobal;
+        if (startLocal + 1 == startGlobal && D[startGlobal] < 0.0) { /* special data */
+            lastGlobal = lastGlobal + (int)lrint(-D[startGlobal]);
+            continue;
+        }
+        while (lastGlobal < endOfEntry && I[lastGlobal + 1] != 0) lastGlobal++;
+        if (neq == cap) { cap *= 2; eqs = (int *)realloc(eqs, sizeof(int) * 3 * cap); }
+        eqs[3 * neq] = startLocal; eqs[3 * neq + 1] = startGlobal; eqs[3 * neq + 2] = lastGlobal;
+        neq++;
+        double rmeas = D[startLocal];
+        for (int j = 1; j <= lastGlobal - startGlobal; j++) {
+            int w = startGlobal + j;
+            int ti = label_to_index(R, I[w]);
+            if (ti < 0) { rc = 20; goto out; }
+            tix[w] = ti;
+            rmeas = rmeas - D[w] * R->global_par[ti];
+            if (o->icalcm == 1) {
+                int ij = R->var_index[ti];
+                if (ij > 0 && back[ij] == 0) { nalg++; gidx[nalg] = ij; back[ij] = nalg; }
+            }
+        }
+        D[startLocal] = rmeas;
+        for (int j = 1; j <= startGlobal - startLocal - 1; j++) if (I[startLocal + j] > nalc) nalc = I[startLocal + j];
+    }
+    if (nalc <= 0) goto out;
+    {
+        double dw1 = 1.0, dw2 = 1.0; /* file weight (fileLevelWeight), 1 unless steered */
+        int ngg = (nalg * nalg + nalg) / 2;
+        double *upd = NULL, *glm = NULL;
+        if (o->icalcm == 1) {
+            /* sort1k of the record's variable indices, update back index (3699-3741) */
+            for (int a = 2; a <= nalg; a++) { int t = gidx[a], b = a - 1; while (b >= 1 && gidx[b] > t) { gidx[b + 1] = gidx[b]; b--; } gidx[b + 1] = t; }
+            for (int k = 1; k <= nalg; k++) back[gidx[k]] = k;
+            upd = (double *)calloc((size_t)ngg + 1, sizeof(double));
+            glm = (double *)calloc((size_t)nalg * nalc + 1, sizeof(double));
+        }
+        int nter = 1;
+        if (o->nloopn != 1 && o->lhuber != 0) nter = o->lhuber;
+        double *lcorr = (double *)calloc(neq + 1, sizeof(double));
+        double *lwgt = (double *)calloc(neq + 1, sizeof(double));
+        int npk = (nalc * nalc + nalc) / 2;
+        double *clmat = (double *)calloc(npk + 1, sizeof(double));
+        double *blvec = (double *)calloc(nalc + 1, sizeof(double));
+        int *ibandh = (int *)calloc(2 * nalc + 2, sizeof(int));
+        int mbnd = -1, nrank = 0, ndf = 0, nan = 0;
+        double summ = 0.0, suwt = 0.0;
+        float resing = 0.0f, cndl10 = 0.0f;
+        int rejected = 0;
+        for (int iter = 1; iter <= nter; iter++) {
+            for (int i = 1; i <= nalc; i++) blvec[i] = 0.0;
+            for (int i = 1; i <= npk; i++) clmat[i] = 0.0;
+            int nweig = 0;
+            cndl10 = 0.0f;
+            for (int e = 0; e < neq; e++) {
+                int sL = eqs[3 * e], sG = eqs[3 * e + 1];
+                double rmeas = D[sL], rerr = D[sG];
+                double wght = 1.0 / (rerr * rerr);
+                nweig++;
+                double resid = rmeas - lcorr[e + 1];
+                if (o->nloopn != 1 && iter != 1 && o->lhuber != 0) {
+                    if (iter <= 3) {
+                        if (fabs(resid) > chuber * rerr) wght = wght * chuber * rerr / fabs(resid);
+                    } else {
+                        double t = resid / rerr / cauchy;
+                        wght = wght / (1.0 + t * t);
+                    }
+                }
+                lwgt[e + 1] = wght;
+                for (int j = 1; j <= sG - sL - 1; j++) {
+                    int ij = I[sL + j];
+                    blvec[ij] = blvec[ij] + wght * rmeas * D[sL + j];
+                    for (int k = 1; k <= j; k++) {
+                        int ik = I[sL + k];
+                        int jk = (ij * ij - ij) / 2 + ik;
+                        clmat[jk] = clmat[jk] + wght * D[sL + j] * D[sL + k];
+                        if (iter == 1) {
+                            int id = abs(ij - ik) + 1;
+                            int im = ij < ik ? ij : ik;
+                            if (im > ibandh[id]) ibandh[id] = im;
+                            im = (nalc + 1 - ij) < (nalc + 1 - ik) ? (nalc + 1 - ij) : (nalc + 1 - ik);
+                            if (im > ibandh[nalc + id]) ibandh[nalc + id] = im;
+                        }
+                    }
+                }
+            }
+            if (iter == 1 && nalc > 5 && o->lfitbb > 0) {
+                int kx = -1;
+                int64_t icmn = (int64_t)nalc * nalc * nalc;
+                int kbdr = 0;
+                for (int k = nalc; k >= 2; k--) {
+                    int kbnd = k - 2;
+                    if (ibandh[k] > kbdr) kbdr = ibandh[k];
+                    int64_t t = kbnd + kbdr + 1;
+                    int64_t icost = 6 * (int64_t)(nalc - kbdr) * t * t + 2 * (int64_t)kbdr * kbdr * kbdr;
+                    if (icost < icmn) { icmn = icost; kx = k; }
+                }
+                if (kx < 0) {
+                    kbdr = 0;
+                    for (int k = nalc; k >= 2; k--) {
+                        int kbnd = k - 2;
+                        if (ibandh[k + nalc] > kbdr) kbdr = ibandh[k + nalc];
+                        int64_t t = kbnd + kbdr + 1;
+                        int64_t icost = 6 * (int64_t)(nalc - kbdr) * t * t + 2 * (int64_t)kbdr * kbdr * kbdr;
+                        if (icost < icmn) { icmn = icost; kx = k; }
+                    }
+                }
+                if (kx > 0) mbnd = kx - 2;
+            }
+            if (mbnd >= 0) { (*nband)++; rejected = 1; rc = 2; break; }
+            orc_sqminv(clmat + 1, blvec + 1, nalc, &nrank);
+            nan = 0;
+            for (int k = 1; k <= nalc; k++) if (!(blvec[k] <= 0.0) && !(blvec[k] > 0.0)) nan++;
+            summ = 0.0;
+            suwt = 0.0;
+            for (int e = 0; e < neq; e++) {
+                int sL = eqs[3 * e], sG = eqs[3 * e + 1];
+                double rmeas = D[sL], rerr = D[sG];
+                double wght = 1.0 / (rerr * rerr);
+                double rmloc = 0.0;
+                for (int j = 1; j <= sG - sL - 1; j++) rmloc = rmloc + D[sL + j] * blvec[I[sL + j]];
+                lcorr[e + 1] = rmloc;
+                rmeas = rmeas - rmloc;
+                double dchi2 = wght * rmeas * rmeas;
+                double resid = rmeas;
+                if (o->nloopn != 1 && iter != 1 && o->lhuber != 0) {
+                    if (iter <= 3) {
+                        if (fabs(resid) > chuber * rerr) {
+                            wght = wght * chuber * rerr / fabs(resid);
+                            dchi2 = 2.0 * chuber * (fabs(resid) / rerr - 0.5 * chuber);
+                        }
+                    } else {
+                        double t = resid / rerr / cauchy;
+                        wght = wght / (1.0 + t * t);
+                        dchi2 = log(1.0 + t * t) * cauchy * cauchy;
+                    }
+                }
+                double down = 1.0 / sqrt(wght);
+                suwt = suwt + rerr / down;
+                if (iter == nter) D[sL] = rmeas;
+                summ = summ + dchi2;
+            }
+            ndf = neq - nrank;
+            resing = ((float)nweig - (float)suwt) / (float)nweig;
+            if (nan > 0) { nrej[0]++; rejected = 1; break; }
+            if (nrank != nalc) { nrej[1]++; rejected = 1; break; }
+            if (cndl10 > o->cndlmx) { nrej[2]++; rejected = 1; break; }
+            if (ndf <= 0) { nrej[3]++; rejected = 1; break; }
+        }
+        if (!rejected) {
+            float chichi = orc_chindl(3, ndf) * (float)ndf;
+            if (o->chicut >= 0.0f) {
+                if (summ > (double)(o->chhuge * chichi)) { nrej[4]++; rejected = 1; }
+                else if (o->chicut > 0.0f) {
+                    float chlimt = o->chicut * chichi;
+                    if (summ > (double)chlimt) { addsums(sums, (double)chlimt, ndf, dw1); nrej[5]++; rejected = 1; }
+                }
+            }
+        }
+        if (!rejected && o->lhuber > 1 && o->dwcut != 0.0f && resing > o->dwcut) {
+            addsums(sums, summ, ndf, dw1);
+            nrej[5]++;
+            rejected = 1;
+        }
+        if (!rejected) {
+            (*naccept)++;
+            summ = 0.0;
+            for (int e = 0; e < neq; e++) {
+                int sL = eqs[3 * e], sG = eqs[3 * e + 1], lG = eqs[3 * e + 2];
+                double rmeas = D[sL], rerr = D[sG];
+                double wght = 1.0 / (rerr * rerr);
+                double dchi2 = wght * rmeas * rmeas;
+                if (o->nloopn != 1 && o->lhuber != 0) {
+                    double resid = fabs(rmeas);
+                    if (nter <= 3) {
+                        if (resid > chuber * rerr) dchi2 = 2.0 * chuber * (resid / rerr - 0.5 * chuber);
+                    } else {
+                        double t = resid / rerr / cauchy;
+                        dchi2 = log(1.0 + t * t) * cauchy * cauchy;
+                    }
+                    wght = lwgt[e + 1];
+                }
+                summ = summ + dchi2;
+                for (int j = 1; j <= lG - sG; j++) {
+                    int wj = sG + j;
+                    int ivgbj = R->var_index[tix[wj]];
+                    if (D[wj] == 0.0) continue;
+                    if (ivgbj > 0) {
+                        vec[ivgbj - 1] += dw1 * wght * rmeas * D[wj];
+                        counter[ivgbj - 1] += 1;
+                        if (o->icalcm == 1) {
+                            int ije = back[ivgbj];
+                            for (int k = 1; k <= j; k++) {
+                                int ivgbk = R->var_index[tix[sG + k]];
+                                if (ivgbk > 0) {
+                                    int ike = back[ivgbk];
+                                    int ia = ije > ike ? ije : ike, ib = ije < ike ? ije : ike;
+                                    int ij = ib + (ia * ia - ia) / 2;
+                                    upd[ij] = upd[ij] - dw1 * wght * D[wj] * D[sG + k];
+                                }
+                            }
+                        }
+                    }
+                }
+                if (o->icalcm != 1) continue;
+                for (int j = 1; j <= lG - sG; j++) {
+                    int ivgbj = R->var_index[tix[sG + j]];
+                    if (ivgbj > 0) {
+                        int ije = back[ivgbj];
+                        for (int k = 1; k <= sG - sL - 1; k++) {
+                            int ik = I[sL + k];
+                            int jk = ik + (ije - 1) * nalc;
+                            glm[jk] = glm[jk] + dw2 * wght * D[sG + j] * D[sL + k];
+                        }
+                    }
+                }
+            }
+            addsums(sums, summ, ndf, dw1);
+            if (o->icalcm == 1) {
+                orc_dbavat(clmat + 1, glm + 1, upd + 1, nalc, nalg);
+                /* mgupdt, full storage: globalMat(row offsets + col) -= sub */
+                for (int a = 1; a <= nalg; a++) {
+                    int va = gidx[a];
+                    for (int b = 1; b <= a; b++) {
+                        int vb = gidx[b];
+                        mat[IDX(va, vb) - 1] -= upd[(a * a - a) / 2 + b];
+                    }
+                }
+            }
+        }
+        free(lcorr); free(lwgt); free(clmat); free(blvec); free(ibandh);
+        if (upd) free(upd);
+        if (glm) free(glm);
+        if (o->icalcm == 1) for (int k = 1; k <= nalg; k++) back[gidx[k]] = 0;
+        nalg = 0;
+    }
+out:
+    if (rc == 20 && o->icalcm == 1) for (int k = 1; k <= nalg; k++) back[gidx[k]] = 0;
+    free(D); free(tix); free(eqs); free(gidx);
+    return rc == 2 ? 0 : rc;
+}
+
+/* One data pass (loopn, src/pede.f90:2479-2885, record part + pre-sigma diagonal 2711-2718).
+ * mat: packed nvar(nvar+1)/2 (zeroed here when icalcm==1), vec/counter: nvar.
+ * Records are processed in order on one thread, so sums are deterministic. */
+int orc_loopn(const orc_records *R, const orc_pass_opts *o, const double *pre_weight, double *mat, double *vec,
+              int64_t *counter, orc_pass_stats *st)
+{
+    int nvar = R->nvar;
+    int64_t npk = ((int64_t)nvar * (nvar + 1)) / 2;
+    for (int i = 0; i < nvar; i++) { vec[i] = 0.0; counter[i] = 0; }
+    if (o->icalcm == 1 && mat) memset(mat, 0, sizeof(double) * npk);
+    orc_sums sums = {0.0, 0, 0, 0.0};
+    memset(st, 0, sizeof(*st));
+    int *back = (int *)calloc(nvar + 2, sizeof(int));
+    int rc = 0;
+    for (int64_t r = 0; r < R->nrec; r++) {
+        rc = loop_record(R, r, o, mat, vec, counter, &sums, st->nrej, &st->naccept, &st->nbandrec, back);
+        if (rc) break;
+    }
+    free(back);
+    if (o->icalcm == 1 && mat && pre_weight)
+        for (int i = 1; i <= nvar; i++)
+            if (pre_weight[i - 1] != 0.0) mat[IDX(i, i) - 1] += pre_weight[i - 1];
+    st->chi2 = sums.chi_frac + (double)sums.chi_int;
+    st->ndf = sums.ndf;
+    st->wndf = sums.wndf;
+    return rc;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Mille binary reader (C binary files: python/readMilleBinary.py:186-213; Fortran sequential: */
+/* src/mille.f90:214-216, src/pede.f90:1827-1829).  Returns number of records or <0 on error.   */
+/* Arrays are malloc'ed; values converted to double as pede does (src/pede.f90:1831-1834).     */
+/* ------------------------------------------------------------------------------------------ */
+int64_t orc_read_mille(const char *path, int fortran, int64_t **rec_ptr_out, int32_t **inder_out,
+                       double **glder_out)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    int64_t cap_r = 1024, cap_w = 1 << 16, nrec = 0, nwords = 0;
+    int64_t *rp = malloc(sizeof(int64_t) * (cap_r + 1));
+    int32_t *in = malloc(sizeof(int32_t) * cap_w);
+    double *gd = malloc(sizeof(double) * cap_w);
+    rp[0] = 0;
+    for (;;) {
+        int32_t marker = 0, nr = 0;
+        if (fortran) {
+            if (fread(&marker, 4, 1, f) != 1) break;
+            if (fread(&nr, 4, 1, f) != 1) { nrec = -2; break; }
+        } else {
+            if (fread(&nr, 4, 1, f) != 1) break;
+        }
+        int dbl = nr < 0;
+        int32_t nw = (dbl ? -nr : nr) / 2;
+        if (nwords + nw > cap_w) {
+            while (nwords + nw > cap_w) cap_w *= 2;
+            in = realloc(in, sizeof(int32_t) * cap_w);
+            gd = realloc(gd, sizeof(double) * cap_w);
+        }
+        if (dbl) {
+            if (fread(gd + nwords, 8, nw, f) != (size_t)nw) { nrec = -2; break; }
+        } else {
+            float *tmp = malloc(sizeof(float) * (nw > 0 ? nw : 1));
+            if (fread(tmp, 4, nw, f) != (size_t)nw) { free(tmp); nrec = -2; break; }
+            for (int i = 0; i < nw; i++) gd[nwords + i] = (double)tmp[i];
+            free(tmp);
+        }
+        if (fread(in + nwords, 4, nw, f) != (size_t)nw) { nrec = -2; break; }
+        if (fortran) {
+            int32_t m2;
+            if (fread(&m2, 4, 1, f) != 1 || m2 != marker) { nrec = -2; break; }
+        }
+        nwords += nw;
+        nrec++;
+        if (nrec >= cap_r) { cap_r *= 2; rp = realloc(rp, sizeof(int64_t) * (cap_r + 1)); }
+        rp[nrec] = nwords;
+    }
+    fclose(f);
+    if (nrec < 0) { free(rp); free(in); free(gd); return nrec; }
+    *rec_ptr_out = rp;
+    *inder_out = in;
+    *glder_out = gd;
+    return nrec;
+}
+
+void orc_free(void *p) { free(p); }

@@ -1,0 +1,183 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on the same inputs.
+
+Tolerances: matrix, vector, chi^2 and corrections agree to 1e-10 relative (same FP64
+arithmetic, only the summation order of FP64 atomics differs); counters, reject counts,
+ndf and index maps are exact."""
+import numpy as np
+import pytest
+
+import labels as LB
+import mpgpu
+from helpers import packed_to_dense, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOY_FIXED = [100 * p + 10 * m + k for p in (1, 10) for m in range(5) for k in (1, 2)]
+
+
+def _pede(rp, ii, gg, steer=None, presig=None, cons=None, **opt):
+    P = mpgpu.Pede(**opt)
+    P.add_records(rp, ii, gg)
+    if steer is not None:
+        P.set_steering_parameters(steer, np.zeros(len(steer)), presig)
+    if cons:
+        P.add_constraints(cons)
+    P.build_structure()
+    return P
+
+
+def _check_maps(P, rp, ii, gg, steer=(), presig=None, mreqenf=25, mcount=0):
+    lab, var, cnt, gsl = P.labels()
+    l2, v2, c2, g2 = LB.label_maps(rp, ii, gg.astype(float), steer_labels=steer, presigma=presig, mreqenf=mreqenf,
+                                   mcount=mcount)
+    assert np.array_equal(lab, l2)
+    assert np.array_equal(var, v2)
+    assert np.array_equal(cnt, c2)
+    assert np.array_equal(gsl, g2)
+    return lab, var
+
+
+def _compare_pass(P, orc, rp, ii, gg, lab, var, pre_weight=None, **kw):
+    nv = P.nvar
+    s_gpu = P.loopn(icalcm=kw.get("icalcm", 1), nloopn=kw.get("nloopn", 1), chicut=kw.get("chicut", 0.0))
+    mat, vec, cnt, s_orc = orc.loopn(rp, ii, gg.astype(np.float64), lab, var, P.parameters(), nv,
+                                     pre_weight=pre_weight, **kw)
+    v_gpu, c_gpu = P.vector()
+    assert s_gpu["nrej"] == s_orc["nrej"]
+    assert s_gpu["ndf"] == s_orc["ndf"]
+    assert abs(s_gpu["chi2"] - s_orc["chi2"]) <= 1e-11 * abs(s_orc["chi2"])
+    assert np.array_equal(c_gpu, cnt)
+    assert rel_err(v_gpu[:nv], vec) < 1e-10
+    if kw.get("icalcm", 1) == 1:
+        m_gpu = P.matrix_packed()
+        n = P.nall
+        A = packed_to_dense(m_gpu, n)[:nv, :nv]
+        assert rel_err(A, packed_to_dense(mat, nv)) < 1e-10
+    return s_gpu, mat, vec
+
+
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_toy_pass(orc, matsto):
+    rp, ii, gg, truth = mpgpu.toy_records(42, 10000)
+    pres = [-1.0] * len(TOY_FIXED)
+    P = _pede(rp, ii, gg, TOY_FIXED, pres, matsto=matsto, mreqenf=10, metsol=1 if matsto == 1 else 4)
+    lab, var = _check_maps(P, rp, ii, gg, TOY_FIXED, pres, mreqenf=10)
+    _compare_pass(P, orc, rp, ii, gg, lab, var)
+
+
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_pixel_pass(orc, matsto):
+    rp, ii, gg, _ = mpgpu.pixel_records(11, 4, 6, 3, 3000, 3, 6)
+    P = _pede(rp, ii, gg, matsto=matsto, mreqenf=5, mcount=1, metsol=1 if matsto == 1 else 4)
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=5, mcount=1)
+    _compare_pass(P, orc, rp, ii, gg, lab, var)
+
+
+def test_downweighting_pass(orc):
+    rp, ii, gg, _ = mpgpu.toy_records(5, 4000)
+    gg = gg.copy()
+    recs = LB.decode(rp, ii, gg)
+    for r in range(0, 4000, 17):
+        gg[recs[r][min(3, len(recs[r]) - 1)][0]] += 0.2
+    P = _pede(rp, ii, gg, matsto=mpgpu.FULL, mreqenf=1, lhuber=3, dwcut=0.2)
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=1)
+    par = np.zeros(P.ntot)
+    par[::3] = 1e-3
+    P.set_parameters(par)
+    for nloopn, chicut in ((1, 0.0), (2, 4.0), (3, 1.5)):
+        _compare_pass(P, orc, rp, ii, gg, lab, var, nloopn=nloopn, lhuber=3, chicut=chicut, dwcut=0.2)
+
+
+def test_inversion_matches_sqminl(orc):
+    rp, ii, gg, truth = mpgpu.toy_records(42, 10000)
+    pres = [-1.0] * len(TOY_FIXED)
+    P = _pede(rp, ii, gg, TOY_FIXED, pres, matsto=mpgpu.FULL, mreqenf=10, metsol=mpgpu.INVERSION)
+    lab, var = _check_maps(P, rp, ii, gg, TOY_FIXED, pres, mreqenf=10)
+    P.loopn()
+    A = P.matrix_packed()
+    b, _ = P.vector()
+    x, st = P.solve()
+    inv, xo, nr = orc.sqminl(A, b)
+    assert st["nrank"] == nr == P.nall
+    assert rel_err(x, xo) < 1e-10
+    err, gcor = P.errors()
+    e_or = np.sqrt(np.abs(np.array([inv[(i * i + i) // 2 - 1] for i in range(1, P.nvar + 1)], dtype=np.float32)))
+    assert np.allclose(err, e_or, rtol=1e-6)
+    assert st["rel_residual"] < 1e-10
+
+
+def _pixel_cons_problem(seed=21, nl=4, nx=5, ny=3, ntr=4000):
+    rp, ii, gg, _ = mpgpu.pixel_records(seed, nl, nx, ny, ntr, 3, 6)
+    cons = mpgpu.pixel_constraints(nl, nx, ny)
+    # loose pre-sigma on the rotations about the beam axis removes the global-rotation null mode
+    gam = [(l + 1) * 100000 + (m + 1) * 10 + 6 for l in range(nl) for m in range(nx * ny)]
+    return rp, ii, gg, cons, gam
+
+
+def _kkt_reference(P):
+    A = packed_to_dense(P.matrix_packed(), P.nall)
+    b, _ = P.vector()
+    return np.linalg.solve(A, b), A, b
+
+
+@pytest.mark.parametrize("mbandw", [0, 6])
+def test_minres_sparse_constraints(mbandw):
+    rp, ii, gg, cons, gam = _pixel_cons_problem()
+    P = _pede(rp, ii, gg, gam, [1e-2] * len(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=5,
+              mcount=1, mbandw=mbandw, mrestl=1e-12)
+    assert P.ncon == 12
+    P.loopn()
+    x, st = P.solve()
+    xr, A, b = _kkt_reference(P)
+    assert st["istop"] in (1, 2)
+    assert rel_err(x[:P.nvar], xr[:P.nvar]) < 1e-8
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-9
+
+
+def test_inversion_with_multipliers():
+    rp, ii, gg, cons, gam = _pixel_cons_problem(seed=22, ntr=3000)
+    P = _pede(rp, ii, gg, gam, [1e-2] * len(gam), cons, matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=5, mcount=1)
+    P.loopn()
+    xr, A, b = _kkt_reference(P)
+    x, st = P.solve()
+    assert st["nrank"] == P.nall
+    assert rel_err(x, xr) < 1e-9
+
+
+@pytest.mark.parametrize("metsol", [mpgpu.DECOMPOSITION, mpgpu.CHOLESKY_FP32_IR])
+def test_cholesky(metsol):
+    rp, ii, gg, cons, gam = _pixel_cons_problem(seed=23, nl=6, nx=8, ny=4, ntr=20000)
+    P = _pede(rp, ii, gg, gam, [1e-2] * len(gam), cons, matsto=mpgpu.FULL, metsol=metsol, mreqenf=5, mcount=1)
+    P.loopn()
+    xr, A, b = _kkt_reference(P)
+    x, st = P.solve()
+    tol = 1e-9 if metsol == mpgpu.DECOMPOSITION else 1e-8
+    assert rel_err(x[:P.nvar], xr[:P.nvar]) < tol
+    # max |dp| / sigma_p for the mixed-precision factorisation
+    Ainv = np.linalg.inv(A)
+    sig = np.sqrt(np.abs(np.diag(Ainv)[:P.nvar]))
+    assert np.max(np.abs(x[:P.nvar] - xr[:P.nvar]) / sig) < 1e-6
+
+
+def test_avprod_matches_dense():
+    rp, ii, gg, cons, gam = _pixel_cons_problem(seed=24, ntr=2000)
+    for matsto in (mpgpu.FULL, mpgpu.SPARSE):
+        P = _pede(rp, ii, gg, gam, [1e-2] * len(gam), cons, matsto=matsto, metsol=4, mreqenf=5, mcount=1)
+        P.loopn()
+        A = packed_to_dense(P.matrix_packed(), P.nall)
+        x = np.random.default_rng(1).normal(size=P.nall)
+        assert rel_err(P.avprod(x), A @ x) < 1e-12
+
+
+def test_mupdat_and_errors_codes():
+    rp, ii, gg, _ = mpgpu.toy_records(3, 500)
+    P = _pede(rp, ii, gg, matsto=mpgpu.FULL, mreqenf=1)
+    P.loopn()
+    before = P.matrix_packed().copy()
+    P.mupdat([1, 3], [1, 2], [2.5, -1.0])
+    after = P.matrix_packed()
+    d = after - before
+    assert abs(d[0] - 2.5) < 1e-12 and abs(d[4] + 1.0) < 1e-12
+    with pytest.raises(mpgpu.MpgpuError) as e:
+        P.mupdat([P.nall + 5], [1], [1.0])
+    assert e.value.code == 23
