@@ -544,6 +544,11 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
                           const LdsPlan &P, int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s)
 {
     if (rec_end <= rec_begin) return hipSuccess;
+    if (P.bytes > 65536) {
+        hipError_t e = R.vf ? hipFuncSetAttribute((const void *)k_records<float>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes)
+                            : hipFuncSetAttribute((const void *)k_records<double>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes);
+        if (e != hipSuccess) return e;
+    }
     if (R.vf)
         hipLaunchKernelGGL(k_records<float>, dim3(grid), dim3(WAVE), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
     else

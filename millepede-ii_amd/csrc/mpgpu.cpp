@@ -1099,6 +1099,7 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
     DevModel M = model_of(c);
     if (c->opt.mbandw < 0) { P.kind = 0; return; }
     if (c->opt.mbandw == 0) {
+        if (P.m > 1024) throw MpError(30, "diagonal preconditioner supports at most 1024 constraints");
         P.kind = 1;
         DBuf<double> diag;
         diag.alloc(P.n);
@@ -1302,7 +1303,12 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
     b.alloc(n);
     xd.alloc(n);
     if (rhs) b.upload(rhs, n, s); else b.upload(c->last_vec.data(), n, s);
-    if (c->opt.metsol == MPGPU_INVERSION) {
+    if (c->opt.metsol == MPGPU_INVERSION && c->have_inverse) {
+        // matrix already inverted by an earlier solve: x = A^-1 b (dbsvxl, src/mpnum.f90:1308)
+        l_packed_symv((int)n, c->d_mat.p, b.p, xd.p, s);
+        loc.nrank = -1;
+        loc.rel_residual = -1.0;
+    } else if (c->opt.metsol == MPGPU_INVERSION) {
         if (c->opt.matsto == MPGPU_SPARSE) throw MpError(36, "method inversion needs full matrix storage");
         // workspaceDiag = diagonal before inversion (minver, src/pede.f90:7859-7861)
         DBuf<double> dg;

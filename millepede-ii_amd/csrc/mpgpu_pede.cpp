@@ -1,3 +1,560 @@
-// placeholder, replaced below
+// Pede-compatible driver on top of the C-ABI: steering file -> Mille files -> structure ->
+// iterations (data passes, global solve, line search) -> millepede.res.
+//
+// Restates the parts of pede that decide what the GPU path computes:
+//   steering keywords and file lists (filetc/filetx/intext, src/pede.f90:9406-11121),
+//   the iteration driver xloopn (src/pede.f90:8501-9009) with the line search ptline
+//   (src/linesrch.f90:89-231), and the result file writer prtglo (src/pede.f90:4477-4651).
+// Keywords are matched case-insensitively on the whole word or an unambiguous prefix of at
+// least 80% of its length (pede's matint accepts 80% fuzzy matches).
 #include "mpgpu_internal.h"
-extern "C" int mpgpu_run_steering(const char *, const char *, int32_t, mpgpu_run_stats *) { return 36; }
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+#include <sys/stat.h>
+
+namespace {
+
+struct ParLine { int32_t label; double value, presigma; };
+struct EqLine { double rhs, sigma; std::vector<int32_t> labels; std::vector<double> f; };
+
+struct Steering {
+    std::vector<std::pair<std::string, int>> binfiles; // name, 1 = C binary, 3 = Fortran
+    std::vector<ParLine> params;
+    std::vector<EqLine> cons, meas;
+    mpgpu_options opt;
+    int mitera = 1, matrit = 0, lsearch = 2, ipcntr = 0, igcorr = 0;
+    float dflim = 0.0f;
+    bool method_given = false;
+    std::string err;
+};
+
+std::string lower(std::string s)
+{
+    for (auto &c : s) c = (char)std::tolower((unsigned char)c);
+    return s;
+}
+
+bool key_match(const std::string &tok, const char *key)
+{
+    std::string t = lower(tok), k = lower(key);
+    if (t == k) return true;
+    if (t.size() > k.size()) return false;
+    return k.compare(0, t.size(), t) == 0 && 10 * t.size() >= 8 * k.size();
+}
+
+bool file_exists(const std::string &f)
+{
+    struct stat st;
+    return stat(f.c_str(), &st) == 0 && S_ISREG(st.st_mode);
+}
+
+std::vector<std::string> tokens(const std::string &line)
+{
+    std::string l = line;
+    size_t c = l.find('!');
+    if (c != std::string::npos) l = l.substr(0, c);
+    std::vector<std::string> t;
+    std::istringstream is(l);
+    std::string w;
+    while (is >> w) t.push_back(w);
+    return t;
+}
+
+bool is_number(const std::string &s, double &v)
+{
+    std::string t = s;
+    for (auto &ch : t) if (ch == 'd' || ch == 'D') ch = 'e';
+    char *end = nullptr;
+    v = std::strtod(t.c_str(), &end);
+    return end && *end == '\0' && !t.empty();
+}
+
+// parse one text file (steering or constraint/parameter file)
+bool parse_text(Steering &S, const std::string &path, const std::string &dir, bool top, int &cur_ftype)
+{
+    std::ifstream in(path);
+    if (!in) { S.err = "cannot open text file " + path; return false; }
+    enum { NONE, PARAM, CONS, MEAS } mode = NONE;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line[0] == '*') continue;
+        auto t = tokens(line);
+        if (t.empty()) continue;
+        double v0;
+        if (is_number(t[0], v0)) {
+            std::vector<double> nums;
+            for (auto &w : t) { double v; if (is_number(w, v)) nums.push_back(v); }
+            if (mode == PARAM && nums.size() >= 1) {
+                ParLine p{(int32_t)std::llround(nums[0]), nums.size() > 1 ? nums[1] : 0.0, nums.size() > 2 ? nums[2] : 0.0};
+                S.params.push_back(p);
+            } else if ((mode == CONS || mode == MEAS) && nums.size() >= 2) {
+                auto &E = mode == CONS ? S.cons.back() : S.meas.back();
+                for (size_t k = 0; k + 1 < nums.size(); k += 2) {
+                    E.labels.push_back((int32_t)std::llround(nums[k]));
+                    E.f.push_back(nums[k + 1]);
+                }
+            }
+            continue;
+        }
+        const std::string &k = t[0];
+        std::vector<double> nums;
+        for (size_t q = 1; q < t.size(); q++) { double v; if (is_number(t[q], v)) nums.push_back(v); }
+        mode = NONE;
+        if (key_match(k, "parameter") || key_match(k, "parameters")) { mode = PARAM; continue; }
+        if (key_match(k, "constraint")) {
+            mode = CONS;
+            S.cons.push_back(EqLine{nums.size() > 0 ? nums[0] : 0.0, nums.size() > 1 ? nums[1] : 0.0, {}, {}});
+            continue;
+        }
+        if (key_match(k, "measurement")) {
+            mode = MEAS;
+            S.meas.push_back(EqLine{nums.size() > 0 ? nums[0] : 0.0, nums.size() > 1 ? nums[1] : 0.0, {}, {}});
+            continue;
+        }
+        if (key_match(k, "end")) { if (top) break; continue; }
+        if (key_match(k, "method")) {
+            std::string m = t.size() > 1 ? lower(t[1]) : "";
+            S.method_given = true;
+            if (m == "inversion") { S.opt.metsol = MPGPU_INVERSION; S.opt.matsto = MPGPU_FULL; }
+            else if (m == "decomposition" || m == "cholesky") { S.opt.metsol = MPGPU_DECOMPOSITION; S.opt.matsto = MPGPU_FULL; }
+            else if (m == "fullminres") { S.opt.metsol = MPGPU_MINRES; S.opt.matsto = MPGPU_FULL; }
+            else if (m == "sparseminres") { S.opt.metsol = MPGPU_MINRES; S.opt.matsto = MPGPU_SPARSE; }
+            else if (m == "fullminres-qlp") { S.opt.metsol = MPGPU_MINRES_QLP; S.opt.matsto = MPGPU_FULL; }
+            else if (m == "sparseminres-qlp") { S.opt.metsol = MPGPU_MINRES_QLP; S.opt.matsto = MPGPU_SPARSE; }
+            else if (m == "cholesky-fp32") { S.opt.metsol = MPGPU_CHOLESKY_FP32_IR; S.opt.matsto = MPGPU_FULL; }
+            else { S.opt.metsol = -1; S.err = "Unsupported solution method " + m; }
+            std::vector<double> mn;
+            for (size_t q = 2; q < t.size(); q++) { double v; if (is_number(t[q], v)) mn.push_back(v); }
+            if (mn.size() > 0) S.mitera = std::max(1, (int)std::llround(mn[0]));
+            if (mn.size() > 1) S.dflim = (float)mn[1];
+            continue;
+        }
+        if (key_match(k, "entries")) {
+            if (nums.size() > 0 && nums[0] > 0.5) S.opt.mreqenf = (int)std::llround(nums[0]);
+            if (nums.size() > 1 && nums[1] > 0.5) S.opt.mreqena = (int)std::llround(nums[1]);
+            continue;
+        }
+        if (key_match(k, "chisqcut")) {
+            if (nums.empty()) { S.opt.chicut = 1.0f; S.opt.chirem = 1.0f; }
+            else {
+                S.opt.chicut = (float)nums[0];
+                if (S.opt.chicut < 1.0f) S.opt.chicut = -1.0f;
+                if (nums.size() == 1) S.opt.chirem = 1.0f;
+                else {
+                    S.opt.chirem = (float)nums[1];
+                    if (S.opt.chirem < 1.0f) S.opt.chirem = 1.0f;
+                    if (S.opt.chicut >= 1.0f) S.opt.chirem = std::min(S.opt.chirem, S.opt.chicut);
+                }
+            }
+            continue;
+        }
+        if (key_match(k, "hugecut")) { if (!nums.empty()) S.opt.chhuge = std::max(1.0f, (float)nums[0]); continue; }
+        if (key_match(k, "linesearch")) { if (!nums.empty()) S.lsearch = (int)std::llround(nums[0]); continue; }
+        if (key_match(k, "localfit")) { if (nums.size() > 1) S.opt.lfitbb = (int)std::llround(nums[1]); continue; }
+        if (key_match(k, "regularisation") || key_match(k, "regularization")) {
+            S.opt.nregul = 1;
+            if (!nums.empty()) S.opt.regula = (float)nums[0];
+            if (nums.size() >= 2) S.opt.default_presigma = (float)nums[1];
+            continue;
+        }
+        if (key_match(k, "presigma")) { if (!nums.empty()) S.opt.default_presigma = (float)nums[0]; continue; }
+        if (key_match(k, "matiter")) { if (!nums.empty()) S.matrit = (int)std::llround(nums[0]); continue; }
+        if (key_match(k, "bandwidth")) {
+            if (!nums.empty()) S.opt.mbandw = std::max(-1, (int)std::llround(nums[0]));
+            if (nums.size() > 1) S.opt.lprecm = (int)std::llround(nums[1]);
+            continue;
+        }
+        if (key_match(k, "outlierdownweighting")) {
+            if (!nums.empty()) { int l = (int)std::llround(nums[0]); if (l > 0 && l <= 2) l = 2; S.opt.lhuber = l; }
+            continue;
+        }
+        if (key_match(k, "dwfractioncut")) {
+            if (!nums.empty()) S.opt.dwcut = (float)std::min(0.5, std::max(0.0, nums[0]));
+            continue;
+        }
+        if (key_match(k, "maxlocalcond")) { if (!nums.empty()) S.opt.cndlmx = (float)nums[0]; continue; }
+        if (key_match(k, "printcounts")) { S.ipcntr = nums.empty() ? 1 : (int)std::llround(nums[0]); continue; }
+        if (key_match(k, "globalcorr")) { S.igcorr = 1; continue; }
+        if (key_match(k, "withmultipliers")) { S.opt.withmultipliers = 1; continue; }
+        if (key_match(k, "withelimination")) { S.opt.withmultipliers = 1; continue; } // same solution, see DESIGN
+        if (key_match(k, "countrecords")) { S.opt.mcount = 1; continue; }
+        if (key_match(k, "mrestol")) {
+            if (!nums.empty() && nums[0] >= 1e-10 && nums[0] <= 1e-4) S.opt.mrestl = nums[0];
+            continue;
+        }
+        if (key_match(k, "mrestranscond")) { if (!nums.empty()) S.opt.mrtcnd = nums[0]; continue; }
+        if (key_match(k, "mresmode")) { if (!nums.empty()) S.opt.mrmode = (int)std::llround(nums[0]); continue; }
+        if (key_match(k, "fortranfiles")) { cur_ftype = 3; continue; }
+        if (key_match(k, "cfiles")) { cur_ftype = 1; continue; }
+        // file names (steering file only)
+        if (top) {
+            std::string f = t[0];
+            std::string full = file_exists(f) ? f : dir + "/" + f;
+            if (file_exists(full)) {
+                std::string tail = lower(f.size() >= 2 ? f.substr(f.size() - 2) : f);
+                if (tail == "xt" || tail == "tx") {
+                    int ft = cur_ftype;
+                    if (!parse_text(S, full, dir, false, ft)) return false;
+                } else {
+                    S.binfiles.push_back({full, cur_ftype});
+                }
+                continue;
+            }
+        }
+        // other keywords (threads, cache, compress, monitoring, ...) do not change the path
+    }
+    return true;
+}
+
+// Fortran Gw.d edit descriptor for a REAL(4) value (gfortran rendering)
+std::string fmt_g(float xf, int w = 14, int d = 5)
+{
+    double x = (double)xf;
+    char buf[64];
+    if (x == 0.0) {
+        std::snprintf(buf, sizeof buf, "%*.*f%*s", w - 4, d - 1, 0.0, 4, "");
+        return buf;
+    }
+    char e[64];
+    std::snprintf(e, sizeof e, "%.*e", d - 1, std::fabs(x));
+    int ex = std::atoi(std::strchr(e, 'e') + 1);
+    int k = ex + 1;
+    if (k >= 0 && k <= d) {
+        std::snprintf(buf, sizeof buf, "%*.*f%*s", w - 4, d - k, x, 4, "");
+        return buf;
+    }
+    std::string digits;
+    for (char c : std::string(e)) { if (c == 'e') break; if (c >= '0' && c <= '9') digits += c; }
+    int eo = ex + 1;
+    char s2[64];
+    std::snprintf(s2, sizeof s2, "%s0.%sE%c%02d", x < 0 ? "-" : "", digits.c_str(), eo < 0 ? '-' : '+', std::abs(eo));
+    std::snprintf(buf, sizeof buf, "%*s", w, s2);
+    return buf;
+}
+
+// ptline (src/linesrch.f90:89-231) with ptldef (232-257)
+struct LineSearch {
+    int nsfd = 0, idgl = 0, idgr = 0, idgm = 0, minf = 1, maxf = 5, im = 1;
+    double sfd[4][21];
+    double stmx = 0.9, gtol = 0.9, dginit = 0, alpha = 0, tot = 0, fsaved = 0;
+    void define(float gtole, float stmax, int minfe, int maxfe)
+    {
+        gtol = std::max(1.0e-4, (double)std::min(gtole, 0.9f));
+        if (gtole == 0.0f) gtol = 0.9;
+        stmx = stmax;
+        if (stmx == 0.0) stmx = 10.0;
+        minf = std::max(1, std::min(minfe, 18));
+        maxf = std::max(2, std::min(maxfe, 19));
+        if (maxfe == 0) maxf = 5;
+        nsfd = 0;
+    }
+    // x, g, s of length n; returns info; f in/out; step out
+    int line(int n, double *x, double &f, const double *g, const double *s, double &step)
+    {
+        int info = 0;
+        double dg = 0.0;
+        for (int i = 0; i < n; i++) dg = dg - g[i] * s[i];
+        auto S = [&](int r, int c) -> double & { return sfd[r - 1][c]; };
+        if (nsfd == 0) {
+            dginit = dg;
+            if (dginit >= 0.0) { step = 0.0; return 0; }
+            step = 1.0;
+            alpha = step;
+            tot = 0.0;
+            idgl = 1;
+            idgr = 0;
+            fsaved = f;
+            nsfd = 1;
+            S(1, 1) = 0.0; S(2, 1) = 0.0; S(3, 1) = dginit; S(4, 1) = 0.0;
+            im = 1;
+        } else {
+            nsfd++;
+            S(1, nsfd) = tot; S(2, nsfd) = f - fsaved; S(3, nsfd) = dg; S(4, nsfd) = 0.0;
+            if (dg < S(3, im)) im = nsfd;
+            if (dg <= 0.0 && dg >= S(3, idgl)) idgl = nsfd;
+            if (dg >= 0.0) { if (idgr == 0) idgr = nsfd; if (dg <= S(3, idgr)) idgr = nsfd; }
+            int i1, i2;
+            if (idgr == 0) { i1 = nsfd - 1; i2 = nsfd; } else { i1 = idgl; i2 = idgr; }
+            double fp1 = S(3, i1), fp2 = S(3, i2);
+            S(4, nsfd) = (S(1, i1) * fp2 - S(1, i2) * fp1) / (fp2 - fp1);
+            if (nsfd >= minf && std::fabs(dg) <= std::fabs(dginit) * gtol) {
+                alpha = tot + alpha;
+                step = alpha;
+                idgm = idgl;
+                if (idgr != 0 && S(3, idgr) + S(3, idgl) < 0.0) idgm = idgr;
+                info = 1;
+            } else if (nsfd >= maxf) info = 2;
+            else {
+                alpha = std::min(S(4, nsfd), stmx) - tot;
+                if (std::fabs(alpha) < 1.0e-3 && S(4, nsfd) > stmx) info = 3;
+                else if (std::fabs(alpha) < 1.0e-3) info = 4;
+            }
+            if (info > 0) {
+                im = 1;
+                for (int i = 1; i <= nsfd; i++) if (std::fabs(S(3, i)) < std::fabs(S(3, im))) im = i;
+                alpha = S(1, im) - S(1, nsfd);
+                if (im == nsfd) return info;
+                for (int i = 0; i < n; i++) x[i] = x[i] + alpha * s[i];
+                f = S(2, im) + fsaved;
+                step = S(1, im);
+                if (im != 1) return info;
+                step = 0.0;
+                return 5;
+            }
+        }
+        for (int i = 0; i < n; i++) x[i] = x[i] + alpha * s[i];
+        tot = tot + alpha;
+        step = tot;
+        return -1;
+    }
+};
+
+int write_res(const std::string &path, const std::vector<int32_t> &labels, const std::vector<int32_t> &var,
+              const std::vector<double> &par, const std::vector<double> &start, const std::vector<double> &presig,
+              const std::vector<int64_t> &cnt_var, const std::vector<int64_t> &cnt_tot, const std::vector<double> *err,
+              const std::vector<double> *gcor, int ipcntr, int mreqena)
+{
+    // keep an existing file as name~ (mvopen, src/pede.f90:11302-11333)
+    if (file_exists(path)) std::rename(path.c_str(), (path + "~").c_str());
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return 30;
+    std::fprintf(f, " Parameter   ! first 3 elements per line are significant (if used as input)\n");
+    for (size_t t = 0; t < labels.size(); t++) {
+        int v = var[t];
+        float p = (float)par[t];
+        float ps = (float)presig[t];
+        int64_t icount = 0;
+        bool lowstat = false;
+        if (v > 0) {
+            icount = cnt_var[v - 1];
+            lowstat = icount < mreqena;
+        }
+        if (ipcntr > 1) icount = cnt_tot[t];
+        if (lowstat) icount = -(icount + 1);
+        std::string line;
+        char lab[32];
+        std::snprintf(lab, sizeof lab, "%10d  ", labels[t]);
+        line = lab;
+        if (v <= 0) {
+            line += fmt_g(p) + fmt_g(ps);
+            if (ipcntr != 0) {
+                char c[32];
+                std::snprintf(c, sizeof c, "%28s%12lld", "", (long long)icount);
+                line += c;
+            }
+        } else {
+            float dpa = (float)(par[t] - start[t]);
+            if (err) {
+                line += fmt_g(p) + fmt_g(ps) + fmt_g(dpa) + fmt_g((float)(*err)[v - 1]);
+                if (ipcntr != 0) { char c[32]; std::snprintf(c, sizeof c, "%12lld", (long long)icount); line += c; }
+                else if (gcor) { char c[32]; std::snprintf(c, sizeof c, "%8.3f", (float)(*gcor)[v - 1]); line += c; }
+            } else {
+                line += fmt_g(p) + fmt_g(ps) + fmt_g(dpa);
+                if (ipcntr != 0) { char c[48]; std::snprintf(c, sizeof c, "%14s%12lld", "", (long long)icount); line += c; }
+            }
+        }
+        while (!line.empty() && line.back() == ' ') line.pop_back();
+        std::fprintf(f, "%s\n", line.c_str());
+    }
+    std::fclose(f);
+    return 0;
+}
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+} // namespace
+
+extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_dir, int32_t device, mpgpu_run_stats *st)
+{
+    mpgpu_run_stats loc{};
+    auto finish = [&](int code) {
+        loc.exit_code = code;
+        if (st) *st = loc;
+        std::string od = output_dir ? output_dir : ".";
+        FILE *e = std::fopen((od + "/millepede.end").c_str(), "w");
+        if (e) {
+            std::fprintf(e, "%5d   %s\n", code, code == 0 ? "Ended normally" : "Aborted");
+            std::fclose(e);
+        }
+        return code;
+    };
+    std::string od = output_dir ? output_dir : ".";
+    {
+        // crash sentinel first, as pede does (src/pede.f90:134)
+        FILE *e = std::fopen((od + "/millepede.end").c_str(), "w");
+        if (e) { std::fprintf(e, "%5d   %s\n", -1, "Still running or crashed"); std::fclose(e); }
+    }
+    Steering S;
+    mpgpu_default_options(&S.opt);
+    S.opt.withmultipliers = 1;
+    std::string sf = steering_file;
+    std::string dir = sf.find('/') == std::string::npos ? "." : sf.substr(0, sf.rfind('/'));
+    int ftype = 1;
+    if (!parse_text(S, sf, dir, true, ftype)) { std::fprintf(stderr, "%s\n", S.err.c_str()); return finish(10); }
+    if (S.opt.metsol < 0) { std::fprintf(stderr, "%s\n", S.err.c_str()); return finish(36); }
+    if (S.binfiles.empty()) return finish(11);
+    mpgpu_ctx *c = nullptr;
+    int rc = mpgpu_create(&S.opt, device, &c);
+    if (rc) return finish(rc);
+    auto bail = [&](int code) {
+        std::fprintf(stderr, "pede(gpu): %s\n", mpgpu_last_error(c));
+        mpgpu_destroy(c);
+        return finish(code);
+    };
+    double t0 = now_ms();
+    for (auto &bf : S.binfiles) {
+        int64_t n = 0;
+        rc = mpgpu_read_mille_file(c, bf.first.c_str(), bf.second == 3 ? 1 : 0, &n);
+        if (rc) return bail(rc);
+        loc.nrec += n;
+    }
+    std::vector<int32_t> pl;
+    std::vector<double> pv, ps;
+    for (auto &p : S.params) { pl.push_back(p.label); pv.push_back(p.value); ps.push_back(p.presigma); }
+    if (!pl.empty() && (rc = mpgpu_set_steering_parameters(c, (int32_t)pl.size(), pl.data(), pv.data(), ps.data()))) return bail(rc);
+    for (int kind = 0; kind < 2; kind++) {
+        auto &L = kind == 0 ? S.cons : S.meas;
+        if (L.empty()) continue;
+        std::vector<int64_t> ptr{0};
+        std::vector<int32_t> lab;
+        std::vector<double> fac, rhs, sig;
+        for (auto &e : L) {
+            lab.insert(lab.end(), e.labels.begin(), e.labels.end());
+            fac.insert(fac.end(), e.f.begin(), e.f.end());
+            ptr.push_back((int64_t)lab.size());
+            rhs.push_back(e.rhs);
+            sig.push_back(e.sigma);
+        }
+        rc = mpgpu_add_equations(c, kind, (int32_t)L.size(), ptr.data(), lab.data(), fac.data(), rhs.data(), sig.data());
+        if (rc) return bail(rc);
+    }
+    int32_t ntot, nvar, ncon;
+    if ((rc = mpgpu_build_structure(c, &ntot, &nvar, &ncon))) return bail(rc);
+    loc.ntot = ntot; loc.nvar = nvar; loc.ncon = ncon;
+    loc.ms_structure = now_ms() - t0;
+    const int nall = nvar + ncon;
+    std::vector<int32_t> labels(ntot), var(ntot), grp(ntot);
+    std::vector<int64_t> cnt_tot(ntot);
+    mpgpu_get_labels(c, labels.data(), var.data(), cnt_tot.data(), grp.data());
+    std::vector<double> par(ntot), start(ntot), presig(ntot, 0.0);
+    mpgpu_get_parameters(c, par.data());
+    start = par;
+    for (auto &p : S.params) {
+        auto it = std::lower_bound(labels.begin(), labels.end(), p.label);
+        if (it != labels.end() && *it == p.label) presig[it - labels.begin()] = p.presigma;
+    }
+    std::vector<int32_t> var2tot;
+    for (int32_t t = 0; t < ntot; t++) if (var[t] > 0) var2tot.push_back(t);
+
+    // ---- xloopn (src/pede.f90:8501-9009) ----
+    const int metsol = S.opt.metsol;
+    float wolfc2 = (metsol >= 4 && metsol <= 6) ? 0.1f : 0.5f;
+    int minf = (metsol >= 4 && metsol <= 6) ? 3 : (metsol == 2 ? 2 : 1);
+    const int matrit = std::min(S.matrit, S.mitera);
+    int nloopn = 0, iterat = 0, lastit = 0;
+    float chicut = S.opt.chicut;
+    int64_t nrej_last = 0;
+    mpgpu_pass_stats ps_st{};
+    std::vector<double> b(nall), x(nall), xls(nvar), corr(nvar), gvec(nvar);
+    std::vector<int64_t> cnt_var(nvar);
+    double tp = 0, tsol = 0;
+    auto pass = [&](int icalcm) -> int {
+        nloopn++;
+        // cut factor of a new iteration (loopn, src/pede.f90:2580-2595)
+        if (iterat != lastit && nloopn > 1) {
+            if (iterat == 1) chicut = S.opt.chirem;
+            else if (iterat > 1) {
+                chicut = std::sqrt(chicut);
+                if (chicut != 0.0f && chicut < 1.5f) chicut = 1.0f;
+                if (chicut != 0.0f && nrej_last == 0) chicut = 1.0f;
+            }
+        }
+        lastit = iterat;
+        double t = now_ms();
+        int r = mpgpu_loopn(c, icalcm, nloopn, chicut, &ps_st);
+        tp += now_ms() - t;
+        if (r) return r;
+        int64_t nrej = 0;
+        for (int k = 0; k < 6; k++) nrej += ps_st.nrej[k];
+        nrej_last = nrej;
+        if (3 * nrej > ps_st.nrec) { std::fprintf(stderr, "Too many rejects (>33.3%%) - stop\n"); return 26; }
+        mpgpu_get_vector(c, b.data(), cnt_var.data());
+        return 0;
+    };
+    if ((rc = pass(1))) return bail(rc);
+    double fvalue = ps_st.chi2;
+    std::printf(" it fc   fcn_value dfcn_exp  rejects\n  0  1 %12.5E          %8lld\n", fvalue, (long long)nrej_last);
+    LineSearch LS;
+    std::vector<double> err_v, gcor_v;
+    bool have_err = false;
+    for (iterat = 1;; iterat++) {
+        // solve with the matrix of the last matrix pass and the current gradient
+        for (int32_t v = 0; v < nvar; v++) xls[v] = par[var2tot[v]];
+        double t = now_ms();
+        mpgpu_solve_stats sst{};
+        if ((rc = mpgpu_solve(c, b.data(), x.data(), &sst))) return bail(rc);
+        tsol += now_ms() - t;
+        if (metsol == MPGPU_INVERSION) {
+            err_v.assign(nvar, 0.0);
+            gcor_v.assign(nvar, 0.0);
+            have_err = mpgpu_get_errors(c, err_v.data(), gcor_v.data()) == 0;
+        }
+        for (int32_t v = 0; v < nvar; v++) { corr[v] = x[v]; gvec[v] = b[v]; }
+        double db = 0, db1 = 0, db2 = 0;
+        for (int32_t v = 0; v < nvar; v++) { db += corr[v] * gvec[v]; db1 += corr[v] * corr[v]; db2 += gvec[v] * gvec[v]; }
+        float delfun = (float)db;
+        double dbsig = 16.0 * std::sqrt(std::max(db1, db2)) * 2.220446049250313e-16;
+        bool lsflag = (S.lsearch > 2 || (S.lsearch == 2 && chicut < 2.25f) ||
+                       (S.lsearch == 1 && chicut < 2.25f && (delfun <= S.dflim || iterat >= S.mitera)));
+        lsflag = lsflag && (db > dbsig);
+        if (lsflag) {
+            LS.define(wolfc2, 10.0f, minf, 10);
+            wolfc2 = (metsol >= 4 && metsol <= 6) ? 0.1f : 0.5f;
+            minf = (metsol >= 4 && metsol <= 6) ? 4 : 3;
+            double step = 0, flines = 0.5 * fvalue;
+            for (;;) {
+                int info = LS.line(nvar, xls.data(), flines, gvec.data(), corr.data(), step);
+                for (int32_t v = 0; v < nvar; v++) par[var2tot[v]] = xls[v];
+                if (info >= 0) break;
+                if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
+                if ((rc = pass(0))) return bail(rc);
+                fvalue = ps_st.chi2;
+                flines = 0.5 * fvalue;
+                for (int32_t v = 0; v < nvar; v++) gvec[v] = b[v];
+            }
+            if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
+        } else {
+            for (int32_t v = 0; v < nvar; v++) { xls[v] += corr[v]; par[var2tot[v]] = xls[v]; }
+            if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
+            if ((rc = pass(0))) return bail(rc);   // function value and gradient at the new point
+            fvalue = ps_st.chi2;
+        }
+        std::printf(" %2d %2d %12.5E %8.2E %8lld\n", iterat, nloopn, fvalue, (double)delfun, (long long)nrej_last);
+        if (delfun <= S.dflim || iterat >= S.mitera) break;
+        if (iterat < matrit) {                   // new matrix (matiter), else reuse it
+            if ((rc = pass(1))) return bail(rc);
+            fvalue = ps_st.chi2;
+        }
+    }
+    loc.final_chi2 = ps_st.chi2;
+    loc.final_ndf = ps_st.ndf;
+    loc.iterations = iterat;
+    loc.ms_passes = tp;
+    loc.ms_solves = tsol;
+    std::printf(" Sum(Chi^2)/Sum(Ndf) = %.17g / %lld = %.17g\n", ps_st.chi2, (long long)(ps_st.ndf - nvar),
+                ps_st.ndf > nvar ? ps_st.chi2 / (double)(ps_st.ndf - nvar) : 0.0);
+    rc = write_res(od + "/millepede.res", labels, var, par, start, presig, cnt_var, cnt_tot, have_err ? &err_v : nullptr,
+                   (have_err && S.igcorr) ? &gcor_v : nullptr, S.ipcntr, S.opt.mreqena);
+    mpgpu_destroy(c);
+    if (rc) return finish(rc);
+    return finish(0);
+}

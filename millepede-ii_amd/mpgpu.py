@@ -352,6 +352,16 @@ def pixel_constraints(nlayers, nx, ny):
     return blocks
 
 
+def pixel_presigma(labels):
+    """Loose pre-sigmas for the weakly determined module parameters: 0.1 cm for dw,
+    0.01 rad for the rotations."""
+    return [0.1 if l % 10 == 3 else 1e-2 for l in labels]
+
+
+def pixel_weak_labels(nlayers, nx, ny):
+    return [(l + 1) * 100000 + (m + 1) * 10 + k for l in range(nlayers) for m in range(nx * ny) for k in (3, 4, 5, 6)]
+
+
 def write_mille(path, rec_ptr, inder, glder, fortran=False):
     r = _GenRecords()
     rp = np.ascontiguousarray(rec_ptr, dtype=np.int64)

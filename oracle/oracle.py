@@ -80,6 +80,9 @@ class Oracle:
         L.orc_packed_matvec.argtypes = [C.c_int, D, D, D]
         L.orc_loopn.argtypes = [C.POINTER(_RecStruct), C.POINTER(_PassOpts), D, D, D, I64, C.POINTER(_PassStats)]
         L.orc_loopn.restype = C.c_int
+        L.orc_loopn_hash.argtypes = [C.POINTER(_RecStruct), C.POINTER(_PassOpts), C.POINTER(C.c_uint64), D,
+                                     C.c_int64, D, I64, C.POINTER(_PassStats), I64]
+        L.orc_loopn_hash.restype = C.c_int
         L.orc_read_mille.argtypes = [C.c_char_p, C.c_int, C.POINTER(I64), C.POINTER(I32), C.POINTER(D)]
         L.orc_read_mille.restype = C.c_int64
         L.orc_free.argtypes = [C.c_void_p]
@@ -173,6 +176,34 @@ class Oracle:
             raise RuntimeError(f"oracle loopn failed with code {rc}")
         stats = dict(chi2=S.chi2, ndf=S.ndf, nrej=list(S.nrej), naccept=S.naccept, nband=S.nbandrec)
         return mat, vec, cnt, stats
+
+    def loopn_hash(self, rec_ptr, inder, glder, labels, var_index, par, nvar, cap_log2=24, **kw):
+        """Record pass accumulating the matrix into a hash table; returns (keys, vals, vec, cnt, stats)
+        with keys = i<<32 | j (1-based, i >= j) of the used slots."""
+        rec_ptr = np.ascontiguousarray(rec_ptr, dtype=np.int64)
+        inder = np.ascontiguousarray(inder, dtype=np.int32)
+        glder = np.ascontiguousarray(glder, dtype=np.float64)
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        var_index = np.ascontiguousarray(var_index, dtype=np.int32)
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        R = _RecStruct(len(rec_ptr) - 1, _p(rec_ptr, I64), _p(inder, I32), _p(glder, D), len(labels),
+                       _p(labels, I32), _p(var_index, I32), _p(par, D), nvar)
+        O = _PassOpts(kw.get("icalcm", 1), kw.get("nloopn", 1), kw.get("lhuber", 0), kw.get("lfitbb", 1),
+                      kw.get("chicut", 0.0), kw.get("chhuge", 50.0), kw.get("dwcut", 0.0), kw.get("cndlmx", 100.0), 1)
+        S = _PassStats()
+        cap = 1 << cap_log2
+        keys = np.zeros(cap, np.uint64)
+        vals = np.zeros(cap)
+        vec = np.zeros(nvar)
+        cnt = np.zeros(nvar, np.int64)
+        used = C.c_int64()
+        rc = self.lib.orc_loopn_hash(C.byref(R), C.byref(O), keys.ctypes.data_as(C.POINTER(C.c_uint64)), _p(vals, D),
+                                     cap, _p(vec, D), _p(cnt, I64), C.byref(S), C.byref(used))
+        if rc != 0:
+            raise RuntimeError(f"oracle loopn_hash failed with code {rc}")
+        m = keys != 0
+        stats = dict(chi2=S.chi2, ndf=S.ndf, nrej=list(S.nrej), naccept=S.naccept, nband=S.nbandrec, used=used.value)
+        return keys[m], vals[m], vec, cnt, stats
 
     def read_mille(self, path, fortran=False):
         rp, ii, gg = I64(), I32(), D()

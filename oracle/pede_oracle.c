@@ -675,6 +675,28 @@ static int label_to_index(const orc_records *R, int32_t label)
 
 typedef struct { double chi_frac; int64_t chi_int; int64_t ndf; double wndf; } orc_sums;
 
+/* matrix sink: packed full matrix, or an open-addressing hash of (i,j) -> value used to
+ * accumulate very large sparse systems on the CPU (keys are i<<32|j, 0 = empty slot) */
+typedef struct { double *mat; uint64_t *keys; double *vals; int64_t cap; int64_t used; int overflow; } orc_sink;
+
+static void sink_add(orc_sink *S, int va, int vb, double v)
+{
+    if (S->mat) { S->mat[IDX(va, vb) - 1] += v; return; }
+    uint64_t key = ((uint64_t)(uint32_t)va << 32) | (uint32_t)vb;
+    uint64_t h = key * 0x9e3779b97f4a7c15ull;
+    int64_t mask = S->cap - 1;
+    int64_t p = (int64_t)(h >> 17) & mask;
+    for (;;) {
+        uint64_t k = S->keys[p];
+        if (k == key) { S->vals[p] += v; return; }
+        if (k == 0) {
+            if (S->used * 10 >= S->cap * 9) { S->overflow = 1; return; }
+            S->keys[p] = key; S->vals[p] = v; S->used++; return;
+        }
+        p = (p + 1) & mask;
+    }
+}
+
 static void addsums(orc_sums *s, double chi2, int ndf, double dw)
 {
     double add = chi2 * dw;
@@ -687,7 +709,7 @@ static void addsums(orc_sums *s, double chi2, int ndf, double dw)
 }
 
 /* one record; returns 0 ok (accepted or rejected, see stats), 20 bad record structure */
-static int loop_record(const orc_records *R, int64_t r, const orc_pass_opts *o, double *mat, double *vec,
+static int loop_record(const orc_records *R, int64_t r, const orc_pass_opts *o, orc_sink *mat, double *vec,
                        int64_t *counter, orc_sums *sums, int64_t nrej[6], int64_t *naccept, int64_t *nband,
                        int *back /* nvar+1 scratch, zero on entry/exit */)
 {
@@ -935,7 +957,7 @@ static int loop_record(const orc_records *R, int64_t r, const orc_pass_opts *o, 
                     int va = gidx[a];
                     for (int b = 1; b <= a; b++) {
                         int vb = gidx[b];
-                        mat[IDX(va, vb) - 1] -= upd[(a * a - a) / 2 + b];
+                        sink_add(mat, va, vb, -upd[(a * a - a) / 2 + b]);
                     }
                 }
             }
@@ -966,8 +988,9 @@ int orc_loopn(const orc_records *R, const orc_pass_opts *o, const double *pre_we
     memset(st, 0, sizeof(*st));
     int *back = (int *)calloc(nvar + 2, sizeof(int));
     int rc = 0;
+    orc_sink S = {mat, NULL, NULL, 0, 0, 0};
     for (int64_t r = 0; r < R->nrec; r++) {
-        rc = loop_record(R, r, o, mat, vec, counter, &sums, st->nrej, &st->naccept, &st->nbandrec, back);
+        rc = loop_record(R, r, o, &S, vec, counter, &sums, st->nrej, &st->naccept, &st->nbandrec, back);
         if (rc) break;
     }
     free(back);
@@ -977,6 +1000,31 @@ int orc_loopn(const orc_records *R, const orc_pass_opts *o, const double *pre_we
     st->chi2 = sums.chi_frac + (double)sums.chi_int;
     st->ndf = sums.ndf;
     st->wndf = sums.wndf;
+    return rc;
+}
+
+/* Same pass accumulating the matrix into a caller-provided hash table (cap = power of two,
+ * keys zeroed); returns 31 when the table overflows.  The pre-sigma diagonal is not added. */
+int orc_loopn_hash(const orc_records *R, const orc_pass_opts *o, uint64_t *keys, double *vals, int64_t cap,
+                   double *vec, int64_t *counter, orc_pass_stats *st, int64_t *used)
+{
+    int nvar = R->nvar;
+    for (int i = 0; i < nvar; i++) { vec[i] = 0.0; counter[i] = 0; }
+    orc_sums sums = {0.0, 0, 0, 0.0};
+    memset(st, 0, sizeof(*st));
+    int *back = (int *)calloc(nvar + 2, sizeof(int));
+    int rc = 0;
+    orc_sink S = {NULL, keys, vals, cap, 0, 0};
+    for (int64_t r = 0; r < R->nrec; r++) {
+        rc = loop_record(R, r, o, &S, vec, counter, &sums, st->nrej, &st->naccept, &st->nbandrec, back);
+        if (rc) break;
+        if (S.overflow) { rc = 31; break; }
+    }
+    free(back);
+    st->chi2 = sums.chi_frac + (double)sums.chi_int;
+    st->ndf = sums.ndf;
+    st->wndf = sums.wndf;
+    *used = S.used;
     return rc;
 }
 

@@ -1,0 +1,73 @@
+"""End-to-end drop-in check against the reference's CI golden files.
+
+The CI job of the reference (.gitlab-ci.yml:86-100) generates 100,000 wire-chamber records
+with pede's own generator (src/mptest1.f90, built from the reference sources into
+oracle/_ref/mptest1_gen), solves with CI/mp2str.txt and compares millepede.res with
+CI/refFiles/millepede.res (kept here as tests/golden/ci_wirechamber_millepede.res).  The CI
+steering ends with 'method fullLAPACK' (needs LAPACK); this run uses 'method inversion' with
+the same iteration count and convergence limit, and Lagrange multipliers instead of
+elimination for the two constraints: the constrained least-squares solution is the same.
+Parameters are compared at the 5 significant digits the file holds; entry counts exactly.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import mpgpu
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GEN = os.path.join(ROOT, "oracle", "_ref", "mptest1_gen")
+GOLD = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.res")
+
+STEER = """fortranfiles
+mp2con.txt
+mp2tst.bin
+Cfiles
+method inversion 3 0.8
+entries 50 10 2
+outlierdownweighting 3
+dwfractioncut 0.1
+matiter 1
+printcounts
+countrecords
+end
+"""
+
+
+def parse_res(path):
+    out = {}
+    for line in open(path).read().splitlines()[1:]:
+        t = line.split()
+        if not t:
+            continue
+        out[int(t[0])] = ([float(x) for x in t[1:-1]], int(t[-1]))
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
+def test_ci_wirechamber_res():
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([GEN, "100000"], cwd=d, check=True, capture_output=True)
+        with open(os.path.join(d, "steer.txt"), "w") as f:
+            f.write(STEER)
+        st = mpgpu.run_steering(os.path.join(d, "steer.txt"), d)
+        assert st["exit_code"] == 0
+        assert st["nrec"] == 100000 and st["ntot"] == 200 and st["ncon"] == 2
+        got = parse_res(os.path.join(d, "millepede.res"))
+    gold = parse_res(GOLD)
+    assert sorted(got) == sorted(gold)
+    bad = []
+    for lab, (vals, cnt) in gold.items():
+        gv, gc = got[lab]
+        # 5 significant digits: compare to half a unit in the 5th digit (plus rounding slack)
+        if abs(gv[0] - vals[0]) > 1.2e-4 * max(abs(vals[0]), 1e-6):
+            bad.append((lab, vals[0], gv[0]))
+        assert gc == cnt, (lab, gc, cnt)
+    assert not bad, bad[:10]
+    # final chi^2 of the CI log (CI/refFiles/millepede.log:153-155)
+    assert abs(st["final_chi2"] - 8130954.4875403922) < 1e-6 * 8130954.4875403922
