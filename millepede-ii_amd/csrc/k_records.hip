@@ -106,7 +106,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     int32_t *ibh = nxt + P.nalc_max + 1;     // [2*nalc_max+2] band-width histogram
     int32_t *sh = ibh + 2 * P.nalc_max + 2;  // scalars
 
-    for (int64_t r = rec_begin + blockIdx.x; r < rec_end; r += gridDim.x) {
+    for (int64_t ri = rec_begin + blockIdx.x; ri < rec_end; ri += gridDim.x) {
+        const int64_t r = R.order ? R.order[ri] : ri;
         const int64_t e0 = R.rec_eq[r];
         const int neq = (int)(R.rec_eq[r + 1] - e0);
         const int64_t g0 = R.rec_gl[r];

@@ -797,4 +797,203 @@ void l_equslvs(int n, int m, const double *c, const int *india, const int *cb, c
                const double *h, const int *hind, double *x, hipStream_t s)
 { hipLaunchKernelGGL(k_equslvs, dim3(1), dim3(64), 0, s, n, m, c, india, cb, koff, kmat, h, hind, x); }
 
+
+// ---------------------------------------------------------------------------------------------
+// Band preconditioner apply, chunk-parallel ("spike" decomposition of the band recurrences of
+// lltfwd/lltbwd, src/mpnum.f90:2237-2336).  Rows are split into chunks of CH rows; each chunk
+// solves its rows with a zero boundary, the boundary dependence (w = mbandw-1 values of the
+// neighbouring chunk) is linear and precomputed per chunk as an CH x w response matrix, and
+// a short carry over the chunks plus a row-parallel correction completes the solve.  Valid
+// for every coefficient set of the factor (rows reset by lltdec included), since the
+// recurrences are linear in their inputs.
+// Layout: row k (1-based) of the factor holds columns k-len+1..k at c[india(k)-k+j-1],
+// len = india(k)-india(k-1) (india(0)=0).
+// ---------------------------------------------------------------------------------------------
+#define SPK_CH 128
+#define SPK_WMAX 16
+
+__device__ __forceinline__ int spk_len(const int *india, int k) { return india[k - 1] - (k > 1 ? india[k - 2] : 0); }
+
+// forward: chunk-local solve of rows [k0, k1] (1-based) with boundary values bnd[0..w) holding
+// x_{k0-w} .. x_{k0-1}; x holds the right-hand side on entry.
+__device__ void spk_fwd_chunk(int k0, int k1, int w, const double *c, const int *india, double *x, const double *bnd)
+{
+    for (int k = k0; k <= k1; k++) {
+        double xk = x[k - 1];
+        int len = spk_len(india, k);
+        for (int j = k - len + 1; j <= k - 1; j++) {
+            double xj = j >= k0 ? x[j - 1] : (bnd ? bnd[j - (k0 - w)] : 0.0);
+            xk = xk - c[india[k - 1] - k + j - 1] * xj;
+        }
+        x[k - 1] = xk * c[india[k - 1] - 1];
+    }
+}
+// backward: chunk-local solve of rows [k1 down to k0] with boundary x_{k1+1}..x_{k1+w} in bnd
+__device__ void spk_bwd_chunk(int k0, int k1, int w, int n, const double *c, const int *india, double *x, const double *bnd)
+{
+    // contributions of the boundary rows k in (k1, k1+w] to x_j, j <= k1
+    if (bnd) {
+        for (int k = k1 + 1; k <= min(n, k1 + w); k++) {
+            double xk = bnd[k - k1 - 1];
+            int len = spk_len(india, k);
+            for (int j = k - len + 1; j <= k - 1; j++)
+                if (j >= k0 && j <= k1) x[j - 1] = x[j - 1] - c[india[k - 1] - k + j - 1] * xk;
+        }
+    }
+    for (int k = k1; k >= k0; k--) {
+        double xk = x[k - 1] * (k > 1 ? c[india[k - 1] - 1] : c[india[0] - 1]);
+        x[k - 1] = xk;
+        if (k == 1) break;
+        int len = spk_len(india, k);
+        for (int j = k - len + 1; j <= k - 1; j++)
+            if (j >= k0) x[j - 1] = x[j - 1] - c[india[k - 1] - k + j - 1] * xk;
+    }
+}
+
+// response matrices: Sf[chunk][row][q] = forward response of the chunk's rows to boundary e_q,
+// Sb likewise for the backward sweep.  One thread per (chunk, q); uses a scratch row buffer.
+__global__ void k_spk_build(int n, int w, int nch, const double *c, const int *india, double *Sf, double *Sb, double *scr)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nch * w) return;
+    int ch = t / w, q = t % w;
+    int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1), L = k1 - k0 + 1;
+    double *x = scr + (size_t)t * SPK_CH - (k0 - 1);   // x[k-1] for k in [k0,k1]
+    double bnd[SPK_WMAX];
+    for (int i = 0; i < w; i++) bnd[i] = (i == q) ? 1.0 : 0.0;
+    for (int k = k0; k <= k1; k++) x[k - 1] = 0.0;
+    if (ch > 0) spk_fwd_chunk(k0, k1, w, c, india, x, bnd);
+    for (int i = 0; i < L; i++) Sf[((size_t)(k0 - 1 + i)) * w + q] = ch > 0 ? x[k0 - 1 + i] : 0.0;
+    for (int k = k0; k <= k1; k++) x[k - 1] = 0.0;
+    if (k1 < n) spk_bwd_chunk(k0, k1, w, n, c, india, x, bnd);
+    for (int i = 0; i < L; i++) Sb[((size_t)(k0 - 1 + i)) * w + q] = k1 < n ? x[k0 - 1 + i] : 0.0;
+}
+
+__global__ void k_spk_local(int n, int w, int nch, const double *c, const int *india, double *x, int backward)
+{
+    int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ch >= nch) return;
+    int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
+    if (!backward) spk_fwd_chunk(k0, k1, w, c, india, x, nullptr);
+    else spk_bwd_chunk(k0, k1, w, n, c, india, x, nullptr);
+}
+
+// carry: boundary values of every chunk, sequential over chunks (w x w products)
+__global__ void k_spk_carry(int n, int w, int nch, const double *x, const double *S, double *t, int backward)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double cur[SPK_WMAX], nxt[SPK_WMAX];
+    if (!backward) {
+        // t[ch] = true last w values of chunk ch-1
+        for (int i = 0; i < w; i++) cur[i] = 0.0;
+        for (int ch = 0; ch < nch; ch++) {
+            int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
+            for (int i = 0; i < w; i++) t[(size_t)ch * w + i] = cur[i];
+            for (int i = 0; i < w; i++) {
+                int k = k1 - w + 1 + i;              // 1-based row
+                double v = 0.0;
+                if (k >= k0) {
+                    v = x[k - 1];
+                    if (ch > 0) for (int q = 0; q < w; q++) v += S[(size_t)(k - 1) * w + q] * cur[q];
+                } else {
+                    v = cur[i + (k1 - k0 + 1)]; // chunk shorter than w: carry through
+                }
+                nxt[i] = v;
+            }
+            for (int i = 0; i < w; i++) cur[i] = nxt[i];
+        }
+    } else {
+        for (int i = 0; i < w; i++) cur[i] = 0.0;
+        for (int ch = nch - 1; ch >= 0; ch--) {
+            int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
+            for (int i = 0; i < w; i++) t[(size_t)ch * w + i] = cur[i];
+            for (int i = 0; i < w; i++) {
+                int k = k0 + i;
+                double v = 0.0;
+                if (k <= k1) {
+                    v = x[k - 1];
+                    if (k1 < n) for (int q = 0; q < w; q++) v += S[(size_t)(k - 1) * w + q] * cur[q];
+                } else {
+                    v = cur[i - (k1 - k0 + 1)];
+                }
+                nxt[i] = v;
+            }
+            for (int i = 0; i < w; i++) cur[i] = nxt[i];
+        }
+    }
+}
+
+__global__ void k_spk_fix(int n, int w, const double *S, const double *t, double *x, int backward)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (k > n) return;
+    int ch = (k - 1) / SPK_CH;
+    if (!backward && ch == 0) return;
+    if (backward && min(n, ch * SPK_CH + SPK_CH) >= n) return;
+    double v = x[k - 1];
+    for (int q = 0; q < w; q++) v += S[(size_t)(k - 1) * w + q] * t[(size_t)ch * w + q];
+    x[k - 1] = v;
+}
+
+// constraint part of equslvs, parallel: d_j = x[n+j] - K_j . x ; small solve ; x -= K^T lambda
+__global__ void k_eqs_cdot(int n, const int *cb, const int64_t *koff, const double *kmat, const double *x, double *d)
+{
+    int j = blockIdx.x;
+    __shared__ double red[TPB];
+    int i0 = cb[2 * j], in = cb[2 * j + 1];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < in; i += TPB) if (i0 + i < n) acc += kmat[koff[j] + i] * x[i0 + i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d[j] = x[n + j] - red[0];
+}
+__global__ void k_eqs_small(int n, int m, const double *h, const int *hind, const double *d, double *x)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int j = 0; j < m; j++) x[n + j] = d[j];
+    lltfwd_dev(m, h, hind, x + n);
+    lltbwd_dev(m, h, hind, x + n);
+    for (int i = 0; i < m; i++) x[n + i] = -x[n + i];
+}
+__global__ void k_eqs_upd(int n, const int *cb, const int64_t *koff, const double *kmat, double *x)
+{
+    int j = blockIdx.y;
+    int i0 = cb[2 * j], in = cb[2 * j + 1];
+    double lam = x[n + j];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < in; i += gridDim.x * blockDim.x)
+        if (i0 + i < n) atomicAdd(&x[i0 + i], -kmat[koff[j] + i] * lam);
+}
+
+namespace {
+inline int g1(int64_t n, int t = TPB) { return (int)((n + t - 1) / t); }
+}
+
+void l_spk_build(int n, int w, const double *c, const int *india, double *Sf, double *Sb, double *scr, hipStream_t s)
+{
+    int nch = (n + SPK_CH - 1) / SPK_CH;
+    hipLaunchKernelGGL(k_spk_build, dim3(g1((int64_t)nch * w, 64)), dim3(64), 0, s, n, w, nch, c, india, Sf, Sb, scr);
+}
+
+void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, const double *Sf, const double *Sb,
+                     double *tbuf, const int *cb, const int64_t *koff, const double *kmat, const double *h, const int *hind,
+                     double *dtmp, double *x, hipStream_t s)
+{
+    int nch = (n + SPK_CH - 1) / SPK_CH;
+    hipLaunchKernelGGL(k_spk_local, dim3(g1(nch, 64)), dim3(64), 0, s, n, w, nch, c, india, x, 0);
+    hipLaunchKernelGGL(k_spk_carry, dim3(1), dim3(64), 0, s, n, w, nch, x, Sf, tbuf, 0);
+    hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sf, tbuf, x, 0);
+    if (m > 0) {
+        hipLaunchKernelGGL(k_eqs_cdot, dim3(m), dim3(TPB), 0, s, n, cb, koff, kmat, x, dtmp);
+        hipLaunchKernelGGL(k_eqs_small, dim3(1), dim3(64), 0, s, n, m, h, hind, dtmp, x);
+        hipLaunchKernelGGL(k_eqs_upd, dim3(8, m), dim3(TPB), 0, s, n, cb, koff, kmat, x);
+    }
+    hipLaunchKernelGGL(k_spk_local, dim3(g1(nch, 64)), dim3(64), 0, s, n, w, nch, c, india, x, 1);
+    hipLaunchKernelGGL(k_spk_carry, dim3(1), dim3(64), 0, s, n, w, nch, x, Sb, tbuf, 1);
+    hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sb, tbuf, x, 1);
+}
+
 } // namespace mpgpu

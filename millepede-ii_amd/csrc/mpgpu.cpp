@@ -47,7 +47,7 @@ struct mpgpu_ctx {
     std::vector<std::vector<std::pair<int32_t, double>>> con_tot; // (total index, factor) per constraint
 
     // ---- device ----
-    DBuf<int64_t> d_rec_eq, d_rec_gl, d_rec_pb, d_eq_sl;
+    DBuf<int64_t> d_rec_eq, d_rec_gl, d_rec_pb, d_eq_sl, d_order;
     DBuf<uint32_t> d_eq_n;
     DBuf<float> d_vf;
     DBuf<double> d_vd;
@@ -723,8 +723,24 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     if (c->plan.bytes > 160 * 1024)
         throw MpError(30, "record too large for the record kernel (LDS " + std::to_string(c->plan.bytes) + " bytes)");
 
+    // ---- processing order: records sorted by the first parameter group they touch, so that the
+    // records in flight at the same time update a compact set of matrix blocks (cache reuse of
+    // the FP64 atomics); results do not depend on the order beyond FP64 summation order
+    std::vector<int64_t> order(nrec);
+    {
+        std::vector<int32_t> key(nrec, INT32_MAX);
+#pragma omp parallel for schedule(static)
+        for (int64_t r = 0; r < nrec; r++)
+            for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
+                int v = c->var_index[gl_tix[q]];
+                if (v > 0) { key[r] = c->group_of[v]; break; }
+            }
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    }
     // ---- device upload ----
     hipStream_t s = c->stream;
+    c->d_order.upload(order, s);
     c->d_rec_eq.upload(rec_eq, s);
     c->d_rec_gl.upload(rec_gl, s);
     c->d_eq_sl.upload(eq_sl, s);
@@ -842,6 +858,7 @@ static DevRecords records_of(mpgpu_ctx *c)
     R.gl_tix = c->d_gl_tix.p;
     R.pair_blk = c->d_pair_blk.p;
     R.rec_w = nullptr;
+    R.order = c->d_order.p;
     R.max_gl = c->plan.max_gl;
     R.max_eq = c->plan.max_eq;
     R.max_nalc = c->plan.nalc_max;
@@ -1085,10 +1102,11 @@ double dot_dev(mpgpu_ctx *c, int64_t n, const double *a, const double *b)
 // preconditioner state for MINRES
 struct Precond {
     int kind = 0; // 0 none, 1 diagonal (+constraints), 2 band (+constraints)
-    DBuf<double> cu, sm, scratch, band, kmat, h;
+    DBuf<double> cu, sm, scratch, band, kmat, h, Sf, Sb, tbuf, dtmp;
     DBuf<int> nrkd, india, hind, cb;
     DBuf<int64_t> koff;
-    int n = 0, m = 0;
+    int n = 0, m = 0, w = 0;
+    bool spike = false;
 };
 
 void setup_precond(mpgpu_ctx *c, Precond &P)
@@ -1154,6 +1172,20 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
         l_band_k(P.m, P.n, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, s);
         l_band_h(P.m, P.cb.p, P.koff.p, P.kmat.p, P.h.p, P.hind.p, P.nrkd.p + 1, s);
     }
+    // chunk-parallel band solves (response matrices of 128-row chunks)
+    P.w = mb - 1;
+    P.spike = P.w >= 1 && P.w <= 16;
+    if (P.spike) {
+        const int nch = (P.n + 127) / 128;
+        P.Sf.alloc((size_t)P.n * P.w);
+        P.Sb.alloc((size_t)P.n * P.w);
+        P.tbuf.alloc((size_t)nch * P.w);
+        P.dtmp.alloc(std::max(1, P.m));
+        DBuf<double> scr;
+        scr.alloc((size_t)nch * P.w * 128);
+        l_spk_build(P.n, P.w, P.band.p, P.india.p, P.Sf.p, P.Sb.p, scr.p, s);
+        HIPCHK(hipStreamSynchronize(s));
+    }
     HIPCHK(hipStreamSynchronize(s));
 }
 
@@ -1164,7 +1196,11 @@ void apply_precond(mpgpu_ctx *c, Precond &P, const double *x, double *y)
         l_presols(P.n, P.m, P.cu.p, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, P.sm.p, x, y, s);
     } else if (P.kind == 2) {
         l_copy(P.n + P.m, x, y, s);
-        l_equslvs(P.n, P.m, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, P.h.p, P.hind.p, y, s);
+        if (P.spike)
+            l_equslvs_spike(P.n, P.m, P.w, P.band.p, P.india.p, P.Sf.p, P.Sb.p, P.tbuf.p, P.cb.p, P.koff.p, P.kmat.p,
+                            P.h.p, P.hind.p, P.dtmp.p, y, s);
+        else
+            l_equslvs(P.n, P.m, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, P.h.p, P.hind.p, y, s);
     } else {
         l_copy(P.n + P.m, x, y, s);
     }

@@ -23,6 +23,7 @@ struct DevRecords {
     const int32_t *gl_tix;   // total index (0-based) of each distinct label of a record
     const int32_t *pair_blk; // block id per (segment_i >= segment_j) pair of a record
     const float *rec_w;      // per-record (file) weight, nullptr = 1
+    const int64_t *order;    // processing order of the records (nullptr = file order)
     int32_t max_gl, max_eq, max_nalc, max_var, max_seg;
 };
 

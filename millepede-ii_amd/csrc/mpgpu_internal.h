@@ -103,6 +103,11 @@ void l_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat,
 void l_equslvs(int n, int m, const double *c, const int *india, const int *cb, const int64_t *koff, const double *kmat,
                const double *h, const int *hind, double *x, hipStream_t s);
 
+void l_spk_build(int n, int w, const double *c, const int *india, double *Sf, double *Sb, double *scr, hipStream_t s);
+void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, const double *Sf, const double *Sb,
+                     double *tbuf, const int *cb, const int64_t *koff, const double *kmat, const double *h, const int *hind,
+                     double *dtmp, double *x, hipStream_t s);
+
 // dense Cholesky (k_chol.hip): full column-major n x n matrix (lda = n)
 int chol_factor(int n, double *a, int lda, double *work, int *info_dev, hipStream_t s, int fp32, float *a32);
 void chol_solve(int n, const double *a, int lda, double *x, hipStream_t s);

@@ -83,6 +83,7 @@ class Oracle:
         L.orc_loopn_hash.argtypes = [C.POINTER(_RecStruct), C.POINTER(_PassOpts), C.POINTER(C.c_uint64), D,
                                      C.c_int64, D, I64, C.POINTER(_PassStats), I64]
         L.orc_loopn_hash.restype = C.c_int
+        L.orc_sqmibb.argtypes = [D, D, C.c_int, C.c_int, C.c_int, C.c_int, I32, D, D, C.c_int]
         L.orc_read_mille.argtypes = [C.c_char_p, C.c_int, C.POINTER(I64), C.POINTER(I32), C.POINTER(D)]
         L.orc_read_mille.restype = C.c_int64
         L.orc_free.argtypes = [C.c_void_p]
@@ -124,6 +125,14 @@ class Oracle:
         a = np.ascontiguousarray(a, dtype=np.float64)
         self.lib.orc_dbavat(_p(v, D), _p(a, D), _p(w, D), n, m)
         return w
+
+    def sqmibb(self, v, b, nbdr, nbnd, inv, side=1):
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64).copy()
+        r = C.c_int32()
+        e1, e2 = C.c_double(), C.c_double()
+        self.lib.orc_sqmibb(_p(v, D), _p(b, D), len(b), nbdr, nbnd, inv, C.byref(r), C.byref(e1), C.byref(e2), side)
+        return v, b, r.value, e1.value, e2.value
 
     def packed_matvec(self, a, x):
         a = np.ascontiguousarray(a, dtype=np.float64)
@@ -271,6 +280,24 @@ class Reference:
         self.lib.dbavat_(_p(np.asfortranarray(v, dtype=np.float64), D), _p(np.asfortranarray(a, dtype=np.float64), D),
                          _p(w, D), self._i(n), self._i(m), self._i(1))
         return w
+
+    def sqmibb(self, v, b, nbdr, nbnd, inv, side=1):
+        v = np.asfortranarray(v, dtype=np.float64).copy()
+        b = np.asfortranarray(b, dtype=np.float64).copy()
+        n = len(b)
+        nr = C.c_int32()
+        e1, e2 = C.c_double(), C.c_double()
+        vbnd = np.zeros(n * (nbnd + 1))
+        vbdr = np.zeros(max(1, n * nbdr))
+        aux = np.zeros(max(1, n * nbdr) + n)
+        vbk = np.zeros(max(1, (nbdr * nbdr + nbdr) // 2))
+        vzru = np.zeros(max(1, nbdr))
+        scd = np.zeros(max(1, nbdr))
+        scf = np.zeros(max(1, nbdr), np.int32)
+        f = self.lib.sqmibb_ if side == 1 else self.lib.sqmibb2_
+        f(_p(v, D), _p(b, D), self._i(n), self._i(nbdr), self._i(nbnd), self._i(inv), C.byref(nr), _p(vbnd, D),
+          _p(vbdr, D), _p(aux, D), _p(vbk, D), _p(vzru, D), _p(scd, D), _p(scf, I32), C.byref(e1), C.byref(e2))
+        return v, b, nr.value, e1.value, e2.value
 
     def chindl(self, n, nd):
         f = self.lib.chindl_

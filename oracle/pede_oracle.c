@@ -627,12 +627,292 @@ void orc_packed_matvec(int n, const double *a, const double *x, double *y)
     }
 }
 
+
+/* ------------------------------------------------------------------------------------------ */
+/* Root-free band Cholesky and bordered-band local fit (src/Dbandmatrix.f90:459-638,          */
+/* src/mpnum.f90:3118-3628).  w is column-major w(mp1,n): W(r,c) = w[(c-1)*mp1 + r-1].        */
+/* ------------------------------------------------------------------------------------------ */
+#define W_(r, c) w[((int64_t)(c) - 1) * mp1 + (r) - 1]
+void orc_dbcdec(double *w, int mp1, int n, double *aux)
+{
+    aux--;
+    for (int i = 1; i <= n; i++) aux[i] = 16.0 * W_(1, i);
+    for (int i = 1; i <= n; i++) {
+        if (W_(1, i) + aux[i] != aux[i]) W_(1, i) = 1.0 / W_(1, i);
+        else W_(1, i) = 0.0;
+        int jm = mp1 - 1 < n - i ? mp1 - 1 : n - i;
+        for (int j = 1; j <= jm; j++) {
+            double rxw = W_(j + 1, i) * W_(1, i);
+            for (int k = 1; k <= jm + 1 - j; k++) W_(k, i + j) = W_(k, i + j) - W_(k + j, i) * rxw;
+            W_(j + 1, i) = rxw;
+        }
+    }
+}
+
+void orc_dbcslv(const double *w, int mp1, int n, const double *b, double *x)
+{
+    b--; x--;
+    for (int i = 1; i <= n; i++) x[i] = b[i];
+    for (int i = 1; i <= n; i++) {
+        int jm = mp1 - 1 < n - i ? mp1 - 1 : n - i;
+        for (int j = 1; j <= jm; j++) x[j + i] = x[j + i] - W_(j + 1, i) * x[i];
+    }
+    for (int i = n; i >= 1; i--) {
+        double rxw = x[i] * W_(1, i);
+        int jm = mp1 - 1 < n - i ? mp1 - 1 : n - i;
+        for (int j = 1; j <= jm; j++) rxw = rxw - W_(j + 1, i) * x[j + i];
+        x[i] = rxw;
+    }
+}
+
+/* band part (inv=1, dbcinb) or complete (inv=2, dbcinv) inverse into packed vs */
+static void dbc_inverse(const double *w, int mp1, int n, double *vs, int full)
+{
+    vs--;
+    for (int i = n; i >= 1; i--) {
+        double rxw = W_(1, i);
+        int jlo = full ? 1 : (i - mp1 + 1 > 1 ? i - mp1 + 1 : 1);
+        for (int j = i; j >= jlo; j--) {
+            int kmax = n < j + mp1 - 1 ? n : j + mp1 - 1;
+            for (int k = j + 1; k <= kmax; k++) {
+                int64_t mx = i > k ? i : k, mn = i < k ? i : k;
+                rxw = rxw - vs[(mx * (mx - 1)) / 2 + mn] * W_(1 + k - j, j);
+            }
+            vs[((int64_t)i * i - i) / 2 + j] = rxw;
+            rxw = 0.0;
+        }
+    }
+}
+void orc_dbcinb(const double *w, int mp1, int n, double *vs) { dbc_inverse(w, mp1, n, vs, 0); }
+void orc_dbcinv(const double *w, int mp1, int n, double *vs) { dbc_inverse(w, mp1, n, vs, 1); }
+#undef W_
+
+/* sqmibb (border first, side 1) and sqmibb2 (border last, side 2).  v packed (n), b(n);
+ * returns nrank, evdmin, evdmax. */
+void orc_sqmibb(double *v, double *b, int n, int nbdr, int nbnd, int inv, int *nrank, double *evdmin,
+                double *evdmax, int side)
+{
+    const int mp1 = nbnd + 1, nmb = n - nbdr;
+    double *vbnd = (double *)calloc((size_t)n * (nbnd + 1) + 1, sizeof(double));
+    double *vbdr = (double *)calloc((size_t)n * (nbdr > 0 ? nbdr : 1) + 1, sizeof(double));
+    double *aux = (double *)calloc((size_t)n * (nbdr > 0 ? nbdr : 1) + n + 1, sizeof(double));
+    double *vbk = (double *)calloc((size_t)(nbdr * nbdr + nbdr) / 2 + 1, sizeof(double));
+    double *vzru = (double *)calloc((size_t)nbdr + 1, sizeof(double));
+    double *V = v - 1, *B = b - 1;
+    double *VBND = vbnd - 1, *VBDR = vbdr - 1, *AUX = aux - 1, *VBK = vbk - 1, *VZRU = vzru - 1;
+    *nrank = 0;
+    const int nb1 = side == 1 ? nbdr + 1 : nmb + 1;
+    if (side == 1) {
+        for (int i = nb1; i <= n; i++) {
+            int ip = (i * (i + 1)) / 2, is = 0;
+            for (int j = i; j <= (n < i + nbnd ? n : i + nbnd); j++) {
+                ip = ip + is; is = j;
+                VBND[(j - i + 1) + (i - nb1) * mp1] = V[ip];
+            }
+        }
+        if (nbdr > 0) {
+            int ioff = 0;
+            for (int i = 1; i <= nbdr; i++) {
+                int ip = (i * (i + 1)) / 2, is = 0;
+                for (int j = i; j <= n; j++) { ip = ip + is; is = j; VBDR[ioff + j] = V[ip]; }
+                ioff = ioff + n;
+            }
+        }
+    } else {
+        for (int i = 1; i <= nmb; i++) {
+            int ip = (i * (i + 1)) / 2, is = 0;
+            for (int j = i; j <= (nmb < i + nbnd ? nmb : i + nbnd); j++) {
+                ip = ip + is; is = j;
+                VBND[(j - i + 1) + (i - 1) * mp1] = V[ip];
+            }
+        }
+        if (nbdr > 0) {
+            int ioff = 0;
+            for (int i = nb1; i <= n; i++) {
+                int ip = (i * (i - 1)) / 2;
+                for (int j = 1; j <= i; j++) VBDR[ioff + j] = V[ip + j];
+                ioff = ioff + n;
+            }
+        }
+    }
+    orc_dbcdec(vbnd, mp1, nmb, aux);
+    int ip = 1;
+    *evdmin = 0.0;
+    *evdmax = 0.0;
+    for (int i = 1; i <= nmb; i++) {
+        if (VBND[ip] <= 0.0) {
+            for (int q = 1; q <= n; q++) B[q] = 0.0;
+            for (int q = 1; q <= (n * n + n) / 2; q++) V[q] = 0.0;
+            *nrank = 0;
+            goto out;
+        }
+        (*nrank)++;
+        if (*nrank == 1) { *evdmin = VBND[ip]; *evdmax = VBND[ip]; }
+        else {
+            if (VBND[ip] < *evdmin) *evdmin = VBND[ip];
+            if (VBND[ip] > *evdmax) *evdmax = VBND[ip];
+        }
+        ip = ip + mp1;
+    }
+    if (nbdr == 0) {
+        orc_dbcslv(vbnd, mp1, nmb, b, b);
+        if (inv > 0) { if (inv > 1) orc_dbcinv(vbnd, mp1, nmb, v); else orc_dbcinb(vbnd, mp1, nmb, v); }
+        goto out;
+    }
+    if (side == 1) {
+        int ioff = nb1;
+        for (int ib = 1; ib <= nbdr; ib++) {
+            orc_dbcslv(vbnd, mp1, nmb, &VBDR[ioff], &AUX[ioff]);
+            VZRU[ib] = B[ib];
+            for (int i = 0; i <= nmb - 1; i++) VZRU[ib] = VZRU[ib] - B[nb1 + i] * AUX[ioff + i];
+            ioff = ioff + n;
+        }
+        orc_dbcslv(vbnd, mp1, nmb, &B[nb1], &B[nb1]);
+        int q = 0;
+        ioff = nb1;
+        for (int ib = 1; ib <= nbdr; ib++) {
+            int joff = nb1;
+            for (int jb = 1; jb <= ib; jb++) {
+                q = q + 1;
+                VBK[q] = V[q];
+                for (int i = 0; i <= nmb - 1; i++) VBK[q] = VBK[q] - VBDR[ioff + i] * AUX[joff + i];
+                joff = joff + n;
+            }
+            ioff = ioff + n;
+        }
+    } else {
+        int ioff = 0;
+        for (int ib = 1; ib <= nbdr; ib++) {
+            orc_dbcslv(vbnd, mp1, nmb, &VBDR[ioff + 1], &AUX[ioff + 1]);
+            VZRU[ib] = B[nmb + ib];
+            for (int i = 1; i <= nmb; i++) VZRU[ib] = VZRU[ib] - B[i] * AUX[ioff + i];
+            ioff = ioff + n;
+        }
+        orc_dbcslv(vbnd, mp1, nmb, b, b);
+        int q = 0, koff = nmb;
+        ioff = 0;
+        for (int ib = 1; ib <= nbdr; ib++) {
+            int joff = 0;
+            for (int jb = 1; jb <= ib; jb++) {
+                q = q + 1;
+                VBK[q] = VBDR[koff + jb];
+                for (int i = 1; i <= nmb; i++) VBK[q] = VBK[q] - VBDR[ioff + i] * AUX[joff + i];
+                joff = joff + n;
+            }
+            ioff = ioff + n;
+            koff = koff + n;
+        }
+    }
+    {
+        int nrankb = 0;
+        orc_sqminv(vbk, vzru, nbdr, &nrankb);
+        if (nrankb == nbdr) *nrank = *nrank + nbdr;
+        else {
+            for (int ib = 1; ib <= nbdr; ib++) VZRU[ib] = 0.0;
+            for (int q = (nbdr * nbdr + nbdr) / 2; q >= 1; q--) VBK[q] = 0.0;
+        }
+    }
+    if (side == 1) {
+        int ioff = nb1;
+        for (int ib = 1; ib <= nbdr; ib++) {
+            B[ib] = VZRU[ib];
+            for (int i = 0; i <= nmb - 1; i++) B[nb1 + i] = B[nb1 + i] - B[ib] * AUX[ioff + i];
+            ioff = ioff + n;
+        }
+        if (inv > 0) {
+            if (inv > 1) orc_dbcinv(vbnd, mp1, nmb, v); else orc_dbcinb(vbnd, mp1, nmb, v);
+            int ip1 = (nmb * nmb + nmb) / 2, ip2 = (n * n + n) / 2;
+            for (int i = nmb - 1; i >= 0; i--) {
+                int j0 = 0;
+                if (inv == 1) j0 = i - nbnd > 0 ? i - nbnd : 0;
+                for (int j = i; j >= j0; j--) {
+                    V[ip2] = V[ip1];
+                    int io = nb1;
+                    for (int ib = 1; ib <= nbdr; ib++) {
+                        int jo = nb1;
+                        for (int jb = 1; jb <= nbdr; jb++) {
+                            int ij = ib > jb ? ib : jb;
+                            ij = (ij * ij - ij) / 2 + (ib < jb ? ib : jb);
+                            V[ip2] = V[ip2] + VBK[ij] * AUX[io + i] * AUX[jo + j];
+                            jo = jo + n;
+                        }
+                        io = io + n;
+                    }
+                    ip1 = ip1 - 1;
+                    ip2 = ip2 - 1;
+                }
+                ip1 = ip1 - j0;
+                ip2 = ip2 - j0;
+                for (int ib = nbdr; ib >= 1; ib--) {
+                    V[ip2] = 0.0;
+                    int jo = nb1;
+                    for (int jb = 1; jb <= nbdr; jb++) {
+                        int ij = ib > jb ? ib : jb;
+                        ij = (ij * ij - ij) / 2 + (ib < jb ? ib : jb);
+                        V[ip2] = V[ip2] - VBK[ij] * AUX[i + jo];
+                        jo = jo + n;
+                    }
+                    ip2 = ip2 - 1;
+                }
+            }
+            for (int q = (nbdr * nbdr + nbdr) / 2; q >= 1; q--) { V[ip2] = VBK[q]; ip2 = ip2 - 1; }
+        }
+    } else {
+        int ioff = 0;
+        for (int ib = 1; ib <= nbdr; ib++) {
+            for (int i = 1; i <= nmb; i++) B[i] = B[i] - VZRU[ib] * AUX[ioff + i];
+            ioff = ioff + n;
+            B[nmb + ib] = VZRU[ib];
+        }
+        if (inv > 0) {
+            if (inv > 1) orc_dbcinv(vbnd, mp1, nmb, v); else orc_dbcinb(vbnd, mp1, nmb, v);
+            int ip1 = (nmb * nmb + nmb) / 2;
+            for (int i = nmb - 1; i >= 0; i--) {
+                int j0 = 0;
+                if (inv == 1) j0 = i - nbnd > 0 ? i - nbnd : 0;
+                for (int j = i; j >= j0; j--) {
+                    int io = 1;
+                    for (int ib = 1; ib <= nbdr; ib++) {
+                        int jo = 1;
+                        for (int jb = 1; jb <= nbdr; jb++) {
+                            int ij = ib > jb ? ib : jb;
+                            ij = (ij * ij - ij) / 2 + (ib < jb ? ib : jb);
+                            V[ip1] = V[ip1] + VBK[ij] * AUX[io + i] * AUX[jo + j];
+                            jo = jo + n;
+                        }
+                        io = io + n;
+                    }
+                    ip1 = ip1 - 1;
+                }
+                ip1 = ip1 - j0;
+            }
+            ip1 = (nmb * nmb + nmb) / 2;
+            int q = 0;
+            for (int ib = 1; ib <= nbdr; ib++) {
+                for (int i = 1; i <= nmb; i++) {
+                    ip1 = ip1 + 1;
+                    V[ip1] = 0.0;
+                    int jo = 0;
+                    for (int jb = 1; jb <= nbdr; jb++) {
+                        int ij = ib > jb ? ib : jb;
+                        ij = (ij * ij - ij) / 2 + (ib < jb ? ib : jb);
+                        V[ip1] = V[ip1] - VBK[ij] * AUX[i + jo];
+                        jo = jo + n;
+                    }
+                }
+                for (int jb = 1; jb <= ib; jb++) { ip1 = ip1 + 1; q = q + 1; V[ip1] = VBK[q]; }
+            }
+        }
+    }
+out:
+    free(vbnd); free(vbdr); free(aux); free(vbk); free(vzru);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Record loop: loopbf restated (src/pede.f90:3387-4442) with decodeNextMeasurement            */
-/* (src/pede.f90:2428-2468), addsums (11666-11689) and mgupdt's full-storage flush             */
-/* (3336-3356). Only the full (sqminv) local-fit path is restated; records whose local matrix  */
-/* qualifies for the bordered-band path (nalc>5 and cheaper by the cost model 3838-3874) are    */
-/* reported with status 2 so that a caller never compares against an unrestated branch.        */
+/* (src/pede.f90:2428-2468), addsums (11666-11689) and mgupdt full-storage flush              */
+/* (3336-3356).  Local fits take the full (sqminv) or the bordered-band (sqmibb/sqmibb2) path  */
+/* by the cost model of 3838-3874.                                                             */
 /* ------------------------------------------------------------------------------------------ */
 typedef struct {
     int64_t nrec;
@@ -658,7 +938,7 @@ typedef struct {
     double wndf;
     int64_t nrej[6];
     int64_t naccept;
-    int64_t nbandrec;   /* records that would take the bordered-band path (not restated here) */
+    int64_t nbandrec;   /* records whose local fit took the bordered-band path */
 } orc_pass_stats;
 
 static int label_to_index(const orc_records *R, int32_t label)
@@ -777,7 +1057,7 @@ static int loop_record(const orc_records *R, int64_t r, const orc_pass_opts *o, 
         double *clmat = (double *)calloc(npk + 1, sizeof(double));
         double *blvec = (double *)calloc(nalc + 1, sizeof(double));
         int *ibandh = (int *)calloc(2 * nalc + 2, sizeof(int));
-        int mbnd = -1, nrank = 0, ndf = 0, nan = 0;
+        int mbnd = -1, mbdr = nalc, mside = -1, nrank = 0, ndf = 0, nan = 0;
         double summ = 0.0, suwt = 0.0;
         float resing = 0.0f, cndl10 = 0.0f;
         int rejected = 0;
@@ -819,7 +1099,7 @@ static int loop_record(const orc_records *R, int64_t r, const orc_pass_opts *o, 
                 }
             }
             if (iter == 1 && nalc > 5 && o->lfitbb > 0) {
-                int kx = -1;
+                int kx = -1, kbdrx = 0;
                 int64_t icmn = (int64_t)nalc * nalc * nalc;
                 int kbdr = 0;
                 for (int k = nalc; k >= 2; k--) {
@@ -827,7 +1107,7 @@ static int loop_record(const orc_records *R, int64_t r, const orc_pass_opts *o, 
                     if (ibandh[k] > kbdr) kbdr = ibandh[k];
                     int64_t t = kbnd + kbdr + 1;
                     int64_t icost = 6 * (int64_t)(nalc - kbdr) * t * t + 2 * (int64_t)kbdr * kbdr * kbdr;
-                    if (icost < icmn) { icmn = icost; kx = k; }
+                    if (icost < icmn) { icmn = icost; kx = k; kbdrx = kbdr; mside = 1; }
                 }
                 if (kx < 0) {
                     kbdr = 0;
@@ -836,13 +1116,23 @@ static int loop_record(const orc_records *R, int64_t r, const orc_pass_opts *o, 
                         if (ibandh[k + nalc] > kbdr) kbdr = ibandh[k + nalc];
                         int64_t t = kbnd + kbdr + 1;
                         int64_t icost = 6 * (int64_t)(nalc - kbdr) * t * t + 2 * (int64_t)kbdr * kbdr * kbdr;
-                        if (icost < icmn) { icmn = icost; kx = k; }
+                        if (icost < icmn) { icmn = icost; kx = k; kbdrx = kbdr; mside = 2; }
                     }
                 }
-                if (kx > 0) mbnd = kx - 2;
+                if (kx > 0) { mbnd = kx - 2; mbdr = kbdrx; }
             }
-            if (mbnd >= 0) { (*nband)++; rejected = 1; rc = 2; break; }
-            orc_sqminv(clmat + 1, blvec + 1, nalc, &nrank);
+            if (mbnd >= 0) {
+                /* bordered-band solution (3876-3897); band part of the inverse for pulls in the
+                 * first iteration, complete inverse for the matrix update */
+                int inv = 0;
+                if (iter == 1) inv = 1;
+                if (o->icalcm == 1) inv = 2;
+                double evdmin = 0.0, evdmax = 0.0;
+                orc_sqmibb(clmat + 1, blvec + 1, nalc, mbdr, mbnd, inv, &nrank, &evdmin, &evdmax, mside);
+                if (evdmin > 0.0) cndl10 = log10f((float)(evdmax / evdmin));
+                if (iter == 1) (*nband)++;
+            } else
+                orc_sqminv(clmat + 1, blvec + 1, nalc, &nrank);
             nan = 0;
             for (int k = 1; k <= nalc; k++) if (!(blvec[k] <= 0.0) && !(blvec[k] > 0.0)) nan++;
             summ = 0.0;

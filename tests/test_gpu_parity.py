@@ -139,6 +139,15 @@ def test_minres_sparse_constraints(orc, mbandw):
     assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-9
     sig = np.sqrt(np.abs(np.diag(np.linalg.inv(A))[:P.nvar]))
     assert np.max(np.abs(x[:P.nvar] - xr[:P.nvar]) / sig) < 1e-4
+    if mbandw > 0:
+        # the oracle's MINRES with the reference's band preconditioner (lltdec/equdecs/equslvs,
+        # pinned bitwise to src/mpnum.f90) on the same matrix: same stop code, iteration count
+        # within a few iterations, so the GPU preconditioner is the same operator
+        msolve = _oracle_band_precond(orc, A, P.nvar, P.ncon, mbandw)
+        xo, so, io, _ = orc.minres(P.nall, lambda v: A @ v, msolve, b, 1e-12, precon=True)
+        assert so == st["istop"]
+        assert abs(io - st["itn"]) <= max(3, io // 20), (io, st["itn"])
+        assert np.max(np.abs(x[:P.nvar] - xo[:P.nvar]) / sig) < 1e-4
     if mbandw == 0:
         # the same matrix through the oracle's MINRES restatement with the same stop rule
         d = np.diag(A).copy()
@@ -150,6 +159,49 @@ def test_minres_sparse_constraints(orc, mbandw):
         assert st2["istop"] == so
         assert abs(st2["itn"] - io) <= max(3, io // 50)
         assert np.max(np.abs(x2[:P.nvar] - xo[:P.nvar]) / sig) < 1e-4
+
+
+def _oracle_band_precond(orc, A, n, m, mb):
+    """pede's band preconditioner (vmprep india layout, constraint blocks of overlapping ranges,
+    equdecs/equslvs) built by the oracle for the dense KKT matrix A."""
+    import ctypes as C
+    india = np.zeros(n, np.int32)
+    for i in range(1, min(mb, n) + 1):
+        india[i - 1] = (i * i + i) // 2
+    for i in range(min(mb, n) + 1, n + 1):
+        india[i - 1] = india[i - 2] + mb
+    band = np.zeros(india[-1])
+    for i in range(1, n + 1):
+        for j in range(max(1, i - mb + 1), i + 1):
+            band[india[i - 1] - i + j - 1] = A[i - 1, j - 1]
+    rows = [np.nonzero(A[n + j, :n])[0] for j in range(m)]
+    first = [int(r[0]) + 1 for r in rows]
+    last = [int(r[-1]) + 1 for r in rows]
+    l, kpart, j = [], [], 0
+    while j < m:
+        lo, hi, k = first[j], last[j], j + 1
+        while k < m and first[k] <= hi:
+            hi, lo, k = max(hi, last[k]), min(lo, first[k]), k + 1
+        nin = hi - lo + 1 + mb - 1
+        l += [j + 1, k, lo - 1, nin]
+        for q in range(j, k):
+            row = np.zeros(nin)
+            row[: hi - lo + 1] = A[n + q, lo - 1:hi]
+            kpart.append(row)
+        j = k
+    l = np.array(l, np.int32)
+    nb = len(l) // 4
+    cc = np.concatenate([band, np.zeros(m * (m + 1) // 2), np.concatenate(kpart)])
+    ind = np.concatenate([india, np.zeros(m, np.int32)])
+    n1, n2 = C.c_int32(), C.c_int32()
+    P = lambda a, t=C.c_double: a.ctypes.data_as(C.POINTER(t))
+    orc.lib.orc_equdecs(n, m, nb, 0, P(cc), P(ind, C.c_int32), P(l, C.c_int32), C.byref(n1), C.byref(n2))
+
+    def msolve(v):
+        y = np.ascontiguousarray(v, dtype=np.float64).copy()
+        orc.lib.orc_equslvs(n, m, nb, P(cc), P(ind, C.c_int32), P(l, C.c_int32), P(y))
+        return y
+    return msolve
 
 
 def test_inversion_with_multipliers():

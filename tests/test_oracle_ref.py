@@ -204,3 +204,37 @@ def test_packed_matvec(orc):
     A = spd(rng, 25)
     x = rng.normal(size=25)
     assert np.allclose(orc.packed_matvec(dense_to_packed(A), x), A @ x, rtol=1e-12)
+
+
+def bordered_band(rng, n, nbdr, nbnd, side):
+    """SPD matrix with a band of half-width nbnd plus nbdr dense border rows/columns."""
+    A = np.zeros((n, n))
+    nmb = n - nbdr
+    b0 = nbdr if side == 1 else 0
+    for i in range(nmb):
+        for j in range(max(0, i - nbnd), i + 1):
+            A[b0 + i, b0 + j] = A[b0 + j, b0 + i] = rng.normal()
+    border = list(range(nbdr)) if side == 1 else list(range(nmb, n))
+    for i in border:
+        A[i, :] = rng.normal(size=n)
+        A[:, i] = A[i, :]
+    return A @ A.T * 0 + (A + A.T) / 2 + n * np.eye(n)
+
+
+@pytest.mark.parametrize("side", [1, 2])
+@pytest.mark.parametrize("inv", [0, 1, 2])
+@pytest.mark.parametrize("n,nbdr,nbnd", [(10, 0, 2), (24, 4, 5), (60, 4, 5), (13, 2, 1)])
+def test_sqmibb_bitwise(orc, ref, side, inv, n, nbdr, nbnd):
+    rng = np.random.default_rng(n * 100 + nbdr * 10 + nbnd + side)
+    A = bordered_band(rng, n, nbdr, nbnd, side)
+    v, b = dense_to_packed(A), rng.normal(size=n)
+    o = orc.sqmibb(v, b, nbdr, nbnd, inv, side)
+    r = ref.sqmibb(v, b, nbdr, nbnd, inv, side)
+    assert o[2] == r[2] == n
+    assert np.array_equal(o[1], r[1])
+    assert (o[3], o[4]) == (r[3], r[4])
+    if inv > 0:
+        assert np.array_equal(o[0], r[0])
+    assert np.allclose(o[1], np.linalg.solve(A, b), rtol=1e-9)
+    if inv == 2:
+        assert np.allclose(packed_to_dense(o[0], n), np.linalg.inv(A), rtol=1e-8, atol=1e-12)
