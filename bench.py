@@ -40,18 +40,17 @@ def workload_bytes(rp):
     return float(np.sum(4 + 8 * nw + 16 * (nalg * (nalg + 1) // 2 + nalg))), float(np.mean(nalg))
 
 
-def cpu_baseline(P, rp, ii, gg, target_s=12.0):
+def cpu_baseline(lab_var, rp, ii, gg, target_s=12.0):
     """The oracle (single-threaded C restatement of the record loop) on a bounded sample of
     the same records, accumulating into a hash table (pede's sparse layout is not rebuilt)."""
     import oracle
     orc = oracle.Oracle()
-    lab, var, cnt, gsl = P.labels()
-    par = P.parameters()
+    lab, var, par, nvar = lab_var
 
     def run(n):
         sl = slice(0, int(rp[n]))
         t = time.perf_counter()
-        orc.loopn_hash(rp[: n + 1], ii[sl], gg[sl].astype(np.float64), lab, var, par, P.nvar, cap_log2=27)
+        orc.loopn_hash(rp[: n + 1], ii[sl], gg[sl].astype(np.float64), lab, var, par, nvar, cap_log2=27)
         return time.perf_counter() - t
 
     t_cal = run(400)
@@ -62,6 +61,54 @@ def cpu_baseline(P, rp, ii, gg, target_s=12.0):
                 sample=f"first {n} records of the same workload, one thread, FP64, matrix into a hash table")
 
 
+def serving_run(njobs, rate=200.0, seed=7):
+    """Serving: Poisson arrivals at `rate` jobs/s of detector subsets with 500-20,000 globals
+    and 1,000-50,000 tracks (log-uniform); Cholesky (FP32 MFMA + FP64 refinement) up to 5,000
+    parameters, band-preconditioned MINRES above; 200 GB admission budget."""
+    import mpgpu
+    rng = np.random.default_rng(seed)
+    jobs = []
+    t0 = time.perf_counter()
+    for k in range(njobs):
+        G = int(np.exp(rng.uniform(np.log(500), np.log(20000))))
+        T = int(np.exp(rng.uniform(np.log(1000), np.log(50000))))
+        nl = 10
+        nmod = max(1, int(round(G / (6 * nl))))
+        ny = max(1, int(round(np.sqrt(nmod / 2.5))))
+        nx = max(1, int(np.ceil(nmod / ny)))
+        rec = mpgpu.pixel_records(seed * 1000 + k, nl, nx, ny, T, 8, 30, wx=100.0 / nx, wy=100.0 / ny, truth=False)[:3]
+        weak = mpgpu.pixel_weak_labels(nl, nx, ny)
+        par = (np.array(weak), np.zeros(len(weak)), np.array(mpgpu.pixel_presigma(weak)))
+        cons = mpgpu.pixel_constraints(nl, nx, ny)
+        if 6 * nl * nx * ny <= 5000:
+            opt = dict(matsto=mpgpu.FULL, metsol=mpgpu.CHOLESKY_FP32_IR, mreqenf=5, mcount=1)
+        else:
+            opt = dict(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mbandw=6, mrestl=1e-10, mreqenf=5, mcount=1)
+        jobs.append((rec, par, cons, opt, 6 * nl * nx * ny))
+    log(f"serving: generated {njobs} jobs in {time.perf_counter() - t0:.1f}s")
+    S = mpgpu.Server(nworkers=8, budget_bytes=200 << 30)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, njobs))
+    ids = []
+    t0 = time.perf_counter()
+    for k, (rec, par, cons, opt, g) in enumerate(jobs):
+        delay = t0 + arrivals[k] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(S.submit(rec, par, cons, **opt))
+    lat, bad = [], 0
+    for i in ids:
+        rc, st, _, _ = S.wait(i)
+        lat.append(st["latency_ms"])
+        bad += rc != 0
+    t_end = time.perf_counter()
+    S.close()
+    lat = np.array(lat)
+    out = dict(jobs=njobs, arrival_rate=rate, jobs_per_s=njobs / (t_end - t0), p50_ms=float(np.percentile(lat, 50)),
+               p99_ms=float(np.percentile(lat, 99)), failed=int(bad), workers=8, budget_gb=200)
+    log(f"serving: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +117,7 @@ def main():
     ap.add_argument("--records", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=120)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X, "
@@ -124,6 +172,8 @@ def main():
                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                        "kernel": "k_records<float>", "kernel_ms": kavg * 1e3,
                        "algorithmic_bytes_per_launch": abytes, "mean_nalg": mean_nalg}
+    lab, var, cnt, gsl = P.labels()
+    lab_var = (lab, var, P.parameters(), P.nvar)
     out["pass_chi2"] = st["chi2"]
     out["pass_ndf"] = st["ndf"]
 
@@ -136,14 +186,21 @@ def main():
         out["minres_istop"] = sst["istop"]
         log(f"solve: {out['solve_s']:.3f}s itn={sst['itn']} istop={sst['istop']} rel.res={sst['rel_residual']:.2e}")
 
+    P.close()
+    if args.serving_jobs > 0:
+        try:
+            out["serving"] = serving_run(args.serving_jobs)
+            out["p50_job_latency_ms"] = out["serving"]["p50_ms"]
+            out["p99_job_latency_ms"] = out["serving"]["p99_ms"]
+        except Exception as e:
+            out["serving"] = {"error": str(e)}
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(P, rp, ii, gg)
+            out["cpu_baseline"] = cpu_baseline(lab_var, rp, ii, gg)
             log(f"cpu baseline: {out['cpu_baseline']['value']:.1f} records/s")
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(out), flush=True)
-    P.close()
 
 
 if __name__ == "__main__":

@@ -168,6 +168,32 @@ typedef struct mpgpu_run_stats {
 } mpgpu_run_stats;
 int mpgpu_run_steering(const char *steering_file, const char *output_dir, int32_t device, mpgpu_run_stats *st);
 
+/* ---- serving: concurrent jobs on one GPU (no pede counterpart; pede runs one job) ---------
+ * Jobs are admitted by their estimated device-memory footprint under budget_bytes and run by
+ * nworkers workers, each with its own context and HIP stream, so the kernels of different
+ * jobs overlap on the device.  A job is: options, FP32 Mille records, optional Parameter
+ * lines (label, start value, pre-sigma) and Lagrange constraints (CSR: cptr[ncon+1]). */
+typedef struct mpgpu_server mpgpu_server;
+typedef struct mpgpu_job_stats {
+    int32_t status, nvar;
+    double latency_ms, queue_ms, run_ms; /* arrival -> result, arrival -> start, start -> result */
+    double chi2;
+    int64_t ndf;
+    int32_t istop, itn;
+    double rel_residual;
+} mpgpu_job_stats;
+int mpgpu_server_create(int32_t device, int32_t nworkers, int64_t budget_bytes, int32_t batch_max_params,
+                        mpgpu_server **out);
+int mpgpu_server_submit(mpgpu_server *s, const mpgpu_options *opt, int64_t nrec, const int64_t *rec_ptr,
+                        const int32_t *inder, const float *glder, int32_t npar, const int32_t *plab, const double *pval,
+                        const double *psig, int32_t ncon, const int64_t *cptr, const int32_t *clab, const double *cfac,
+                        const double *crhs, int64_t *job_id);
+/* blocks until the job is done; returns its status; fills up to capacity (label, correction)
+ * pairs of its variable parameters */
+int mpgpu_server_wait(mpgpu_server *s, int64_t job_id, int32_t capacity, int32_t *labels, double *corr,
+                      mpgpu_job_stats *st);
+int mpgpu_server_destroy(mpgpu_server *s);
+
 #ifdef __cplusplus
 }
 #endif

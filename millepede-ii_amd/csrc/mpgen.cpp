@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <omp.h>
 #include <vector>
 
@@ -173,7 +174,7 @@ int mpgen_pixel(uint64_t seed, int nlayers, int nx, int ny, double wx, double wy
         Rng r(seed, (uint64_t)t + 1);
         int nh = r.range(hmin, hmax);
         int nl = nh < nlayers ? nh : nlayers;
-        int extra = nh - nl;
+        int extra = std::min(nh - nl, nl);
         int l0 = r.range(0, nlayers - nl);
         // track through the middle of the run of layers
         double zmid = z0 + dz * (l0 + 0.5 * (nl - 1));
@@ -208,6 +209,87 @@ int mpgen_pixel(uint64_t seed, int nlayers, int nx, int ny, double wx, double wy
             rec.meas((float)mu, dlu, ilu, 2, (float)sigma, dgu, lab, 6);
             rec.meas((float)mv, dlv, ilv, 2, (float)sigma, dgv, lab, 6);
         }
+    });
+}
+
+
+// ---- broken-lines tracks through a stack of planes (bordered-band local fits) -------------
+// nlayers planes at z = z0 + dz*l, each a grid of nx*ny modules with 6 rigid-body globals
+// (labels as mpgen_pixel).  A track crosses np consecutive planes (np uniform in
+// [pmin, pmax]); its local parameters are 4 border parameters (quadratic and cubic bending
+// in x and y: derivatives (z-zc)^2/2 and (z-zc)^3/6) followed by 2 offsets (u, v) per plane.
+// Per plane two hit measurements (offset + border terms + 6 globals), and per interior plane
+// two kink pseudo-measurements with value 0 on the offsets of the plane and its neighbours
+// (the broken-lines curvature terms, src/mptest2.f90:379-395), sigma theta0.  Local indices
+// are ordered border first, so the local matrix is a band of half-width 4 with a border of 4.
+int mpgen_brokenlines(uint64_t seed, int nlayers, int nx, int ny, double wx, double wy, double z0, double dz,
+                      int64_t ntracks, int pmin, int pmax, double slope_sigma, double sigma, double theta0,
+                      mpgen_records *out)
+{
+    const int nmod = nx * ny;
+    const double xmin = -0.5 * nx * wx, ymin = -0.5 * ny * wy;
+    std::vector<double> tr((size_t)nlayers * nmod * 6);
+    {
+        Rng rt(seed, 0xfffffffffull);
+        for (size_t k = 0; k < tr.size(); k++) tr[k] = ((k % 6) < 3 ? 0.01 : 1e-4) * rt.gauss();
+    }
+    return make_records(ntracks, out, [&](int64_t t, Rec &rec) {
+        Rng r(seed, (uint64_t)t + 1);
+        int np = r.range(pmin, std::min(pmax, nlayers));
+        int l0 = r.range(0, nlayers - np);
+        double zc = z0 + dz * (l0 + 0.5 * (np - 1));
+        double tx = slope_sigma * r.gauss(), ty = slope_sigma * r.gauss();
+        double xm = xmin + (nx * wx) * (0.1 + 0.8 * r.uni()), ym = ymin + (ny * wy) * (0.1 + 0.8 * r.uni());
+        // true trajectory: straight line plus random kinks (offsets accumulate)
+        std::vector<double> ux(np), uy(np);
+        double ax = tx, ay = ty, x = xm - tx * (zc - z0 - dz * l0), y = ym - ty * (zc - z0 - dz * l0);
+        for (int q = 0; q < np; q++) {
+            ux[q] = x; uy[q] = y;
+            ax += theta0 * r.gauss(); ay += theta0 * r.gauss();
+            x += ax * dz; y += ay * dz;
+        }
+        const int nb = 4;
+        int nalc = nb + 2 * np;
+        std::vector<float> dl(nalc);
+        std::vector<int> il(nalc);
+        for (int k = 0; k < nalc; k++) il[k] = k + 1;
+        for (int q = 0; q < np; q++) {
+            int l = l0 + q;
+            double z = z0 + dz * l, s = z - zc;
+            int ix = (int)std::floor((ux[q] - xmin) / wx), iy = (int)std::floor((uy[q] - ymin) / wy);
+            if (ix < 0 || ix >= nx || iy < 0 || iy >= ny) continue;
+            int m = iy * nx + ix;
+            double xcen = xmin + (ix + 0.5) * wx, ycen = ymin + (iy + 0.5) * wy;
+            double u = ux[q] - xcen, v = uy[q] - ycen;
+            const double *p = &tr[((size_t)l * nmod + m) * 6];
+            double gu[6] = {1.0, 0.0, -tx, -tx * v, tx * u, -v};
+            double gv[6] = {0.0, 1.0, -ty, -ty * v, ty * u, u};
+            double mu = ux[q], mv = uy[q];
+            for (int k = 0; k < 6; k++) { mu += gu[k] * p[k]; mv += gv[k] * p[k]; }
+            mu += sigma * r.gauss();
+            mv += sigma * r.gauss();
+            int lab[6];
+            for (int k = 0; k < 6; k++) lab[k] = (l + 1) * 100000 + (m + 1) * 10 + k + 1;
+            float dgu[6], dgv[6];
+            for (int k = 0; k < 6; k++) { dgu[k] = (float)gu[k]; dgv[k] = (float)gv[k]; }
+            // u: border (s^2/2, 0, s^3/6, 0), offset u_q
+            std::fill(dl.begin(), dl.end(), 0.0f);
+            dl[0] = (float)(0.5 * s * s); dl[2] = (float)(s * s * s / 6.0); dl[nb + 2 * q] = 1.0f;
+            rec.meas((float)mu, dl.data(), il.data(), nalc, (float)sigma, dgu, lab, 6);
+            std::fill(dl.begin(), dl.end(), 0.0f);
+            dl[1] = (float)(0.5 * s * s); dl[3] = (float)(s * s * s / 6.0); dl[nb + 2 * q + 1] = 1.0f;
+            rec.meas((float)mv, dl.data(), il.data(), nalc, (float)sigma, dgv, lab, 6);
+        }
+        // kinks at interior planes (value 0, no globals)
+        const float dn = (float)(1.0 / dz), dp = (float)(1.0 / dz);
+        for (int q = 1; q < np - 1; q++)
+            for (int c = 0; c < 2; c++) {
+                std::fill(dl.begin(), dl.end(), 0.0f);
+                dl[nb + 2 * (q - 1) + c] = dp;
+                dl[nb + 2 * q + c] = -dp - dn;
+                dl[nb + 2 * (q + 1) + c] = dn;
+                rec.meas(0.0f, dl.data(), il.data(), nalc, (float)theta0, nullptr, nullptr, 0);
+            }
     });
 }
 

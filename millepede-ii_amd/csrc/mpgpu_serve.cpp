@@ -1,0 +1,263 @@
+// Serving loop: alignment/calibration jobs of different sizes arrive over time, are admitted
+// by their device-memory footprint under a budget, run concurrently on one GPU (one context
+// and HIP stream per worker, so the kernels of different jobs overlap on the device) and
+// report their latency from arrival to result.  Small jobs (full storage) are grouped: a
+// worker that takes a small job also takes the small jobs queued behind it (up to 8, while
+// they fit the budget) and runs them back to back on its stream.
+// No pede counterpart: pede runs one job per process (src/pede.f90:33-528).
+#include "mpgpu_internal.h"
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Job {
+    int64_t id = 0;
+    mpgpu_options opt{};
+    std::vector<int64_t> rec_ptr;
+    std::vector<int32_t> inder;
+    std::vector<float> glder;
+    std::vector<int32_t> plab;
+    std::vector<double> pval, psig;
+    std::vector<int64_t> cptr;
+    std::vector<int32_t> clab;
+    std::vector<double> cfac, crhs;
+    int64_t footprint = 0;
+    clk::time_point t_arrival, t_start, t_end;
+    // results
+    int status = -1;
+    std::string err;
+    std::vector<int32_t> labels;
+    std::vector<double> corr;
+    mpgpu_solve_stats sst{};
+    double chi2 = 0;
+    int64_t ndf = 0;
+    bool done = false;
+};
+
+// device bytes a job needs: records (8 B/word + compiled tables), the matrix (full packed or
+// an upper estimate of the group-block storage), vectors and solver work space
+int64_t estimate_footprint(const Job &j)
+{
+    int64_t nw = j.rec_ptr.empty() ? 0 : j.rec_ptr.back();
+    int64_t nrec = (int64_t)j.rec_ptr.size() - 1;
+    int64_t rec = 12 * nw + 64 * nrec;
+    // distinct labels of the global words: after the second zero-index word of a measurement
+    // (measurement, locals, sigma, globals; src/mille.f90:27-59)
+    std::vector<int32_t> lab;
+    for (int64_t r = 0; r < nrec; r++) {
+        int zeros = 0;
+        for (int64_t w = j.rec_ptr[r] + 1; w < j.rec_ptr[r + 1]; w++) {
+            if (j.inder[w] == 0) { zeros++; continue; }
+            if (zeros > 0 && zeros % 2 == 0) lab.push_back(j.inder[w]);
+        }
+    }
+    std::sort(lab.begin(), lab.end());
+    int64_t nlab = std::unique(lab.begin(), lab.end()) - lab.begin();
+    int64_t nall = nlab + (int64_t)j.crhs.size();
+    int64_t mat;
+    if (j.opt.matsto == MPGPU_FULL) mat = 8 * nall * (nall + 1) / 2 * (j.opt.metsol == MPGPU_DECOMPOSITION || j.opt.metsol == MPGPU_CHOLESKY_FP32_IR ? 3 : 2);
+    else mat = std::min<int64_t>(8 * nall * (nall + 1) / 2, 8 * nw * 24);
+    return rec + mat + 200 * nall + (64 << 20);
+}
+
+} // namespace
+
+struct mpgpu_server {
+    int device = 0;
+    int64_t budget = 0, in_use = 0;
+    int batch_max_params = 5000;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Job>> queue;
+    std::map<int64_t, std::shared_ptr<Job>> jobs;
+    std::vector<std::thread> workers;
+    bool stop = false;
+    int64_t next_id = 1;
+
+    void run_job(Job &j, mpgpu_ctx *&ctx)
+    {
+        j.t_start = clk::now();
+        int rc = 0;
+        mpgpu_ctx *c = nullptr;
+        rc = mpgpu_create(&j.opt, device, &c);
+        if (!rc) rc = mpgpu_add_records(c, (int64_t)j.rec_ptr.size() - 1, j.rec_ptr.data(), j.inder.data(), j.glder.data(), nullptr);
+        if (!rc && !j.plab.empty())
+            rc = mpgpu_set_steering_parameters(c, (int32_t)j.plab.size(), j.plab.data(), j.pval.data(), j.psig.data());
+        if (!rc && !j.crhs.empty()) {
+            std::vector<double> sig(j.crhs.size(), 0.0);
+            rc = mpgpu_add_equations(c, 0, (int32_t)j.crhs.size(), j.cptr.data(), j.clab.data(), j.cfac.data(), j.crhs.data(), sig.data());
+        }
+        int32_t ntot = 0, nvar = 0, ncon = 0;
+        if (!rc) rc = mpgpu_build_structure(c, &ntot, &nvar, &ncon);
+        mpgpu_pass_stats ps{};
+        if (!rc) rc = mpgpu_loopn(c, 1, 1, 0.0f, &ps);
+        std::vector<double> x((size_t)nvar + ncon);
+        if (!rc) rc = mpgpu_solve(c, nullptr, x.data(), &j.sst);
+        if (!rc) {
+            std::vector<int32_t> lab(ntot), var(ntot), grp(ntot);
+            std::vector<int64_t> cnt(ntot);
+            mpgpu_get_labels(c, lab.data(), var.data(), cnt.data(), grp.data());
+            j.labels.clear();
+            j.corr.clear();
+            for (int32_t t = 0; t < ntot; t++)
+                if (var[t] > 0) { j.labels.push_back(lab[t]); j.corr.push_back(x[var[t] - 1]); }
+            j.chi2 = ps.chi2;
+            j.ndf = ps.ndf;
+        } else if (c) {
+            j.err = mpgpu_last_error(c);
+        }
+        if (c) mpgpu_destroy(c);
+        (void)ctx;
+        j.status = rc;
+        j.t_end = clk::now();
+    }
+
+    void worker()
+    {
+        mpgpu_ctx *ctx = nullptr;
+        (void)hipSetDevice(device);
+        for (;;) {
+            std::vector<std::shared_ptr<Job>> batch;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] {
+                    if (stop) return true;
+                    return !queue.empty() && (in_use + queue.front()->footprint <= budget || in_use == 0);
+                });
+                if (stop && queue.empty()) return;
+                if (queue.empty()) continue;
+                // admit the head job; group further small jobs behind it while they fit
+                auto head = queue.front();
+                queue.pop_front();
+                in_use += head->footprint;
+                batch.push_back(head);
+                bool small = head->opt.matsto == MPGPU_FULL;
+                while (small && !queue.empty() && batch.size() < 8 && queue.front()->opt.matsto == MPGPU_FULL &&
+                       in_use + queue.front()->footprint <= budget) {
+                    auto j = queue.front();
+                    queue.pop_front();
+                    in_use += j->footprint;
+                    batch.push_back(j);
+                }
+            }
+            for (auto &j : batch) run_job(*j, ctx);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (auto &j : batch) { in_use -= j->footprint; j->done = true; }
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int mpgpu_server_create(int32_t device, int32_t nworkers, int64_t budget_bytes, int32_t batch_max_params, mpgpu_server **out)
+{
+    if (!out || nworkers <= 0) return 30;
+    auto *s = new mpgpu_server();
+    s->device = device;
+    s->budget = budget_bytes;
+    s->batch_max_params = batch_max_params;
+    for (int i = 0; i < nworkers; i++) s->workers.emplace_back([s] { s->worker(); });
+    *out = s;
+    return 0;
+}
+
+int mpgpu_server_submit(mpgpu_server *s, const mpgpu_options *opt, int64_t nrec, const int64_t *rec_ptr, const int32_t *inder,
+                        const float *glder, int32_t npar, const int32_t *plab, const double *pval, const double *psig,
+                        int32_t ncon, const int64_t *cptr, const int32_t *clab, const double *cfac, const double *crhs,
+                        int64_t *job_id)
+{
+    if (!s || !opt || nrec < 0) return 30;
+    auto j = std::make_shared<Job>();
+    j->opt = *opt;
+    int64_t w0 = rec_ptr[0], w1 = rec_ptr[nrec];
+    j->rec_ptr.resize(nrec + 1);
+    for (int64_t r = 0; r <= nrec; r++) j->rec_ptr[r] = rec_ptr[r] - w0;
+    j->inder.assign(inder + w0, inder + w1);
+    j->glder.assign(glder + w0, glder + w1);
+    if (npar > 0) {
+        j->plab.assign(plab, plab + npar);
+        j->pval.assign(pval, pval + npar);
+        j->psig.assign(psig, psig + npar);
+    }
+    if (ncon > 0) {
+        j->cptr.assign(cptr, cptr + ncon + 1);
+        j->clab.assign(clab + cptr[0], clab + cptr[ncon]);
+        j->cfac.assign(cfac + cptr[0], cfac + cptr[ncon]);
+        for (auto &p : j->cptr) p -= cptr[0];
+        j->crhs.assign(crhs, crhs + ncon);
+    }
+    j->footprint = estimate_footprint(*j);
+    j->t_arrival = clk::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (j->footprint > s->budget) return 30;
+        j->id = s->next_id++;
+        s->jobs[j->id] = j;
+        s->queue.push_back(j);
+    }
+    s->cv_work.notify_one();
+    if (job_id) *job_id = j->id;
+    return 0;
+}
+
+// waits for a job; corr/labels may be NULL (then only statistics); capacity = size of arrays
+int mpgpu_server_wait(mpgpu_server *s, int64_t job_id, int32_t capacity, int32_t *labels, double *corr, mpgpu_job_stats *st)
+{
+    if (!s) return 30;
+    std::shared_ptr<Job> j;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(job_id);
+        if (it == s->jobs.end()) return 30;
+        j = it->second;
+        s->cv_done.wait(lk, [&] { return j->done; });
+        s->jobs.erase(it);
+    }
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->status = j->status;
+        st->nvar = (int32_t)j->labels.size();
+        st->latency_ms = std::chrono::duration<double, std::milli>(j->t_end - j->t_arrival).count();
+        st->queue_ms = std::chrono::duration<double, std::milli>(j->t_start - j->t_arrival).count();
+        st->run_ms = std::chrono::duration<double, std::milli>(j->t_end - j->t_start).count();
+        st->chi2 = j->chi2;
+        st->ndf = j->ndf;
+        st->istop = j->sst.istop;
+        st->itn = j->sst.itn;
+        st->rel_residual = j->sst.rel_residual;
+    }
+    if (labels && corr) {
+        int32_t n = std::min<int32_t>(capacity, (int32_t)j->labels.size());
+        std::copy(j->labels.begin(), j->labels.begin() + n, labels);
+        std::copy(j->corr.begin(), j->corr.begin() + n, corr);
+    }
+    return j->status;
+}
+
+int mpgpu_server_destroy(mpgpu_server *s)
+{
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    delete s;
+    return 0;
+}
+
+} // extern "C"

@@ -30,6 +30,7 @@ EXPORTS = [
     "mpgpu_build_structure", "mpgpu_get_labels", "mpgpu_matrix_info", "mpgpu_set_parameters",
     "mpgpu_get_parameters", "mpgpu_loopn", "mpgpu_get_vector", "mpgpu_get_matrix_packed", "mpgpu_mupdat",
     "mpgpu_solve", "mpgpu_avprod", "mpgpu_get_errors", "mpgpu_run_steering",
+    "mpgpu_server_create", "mpgpu_server_submit", "mpgpu_server_wait", "mpgpu_server_destroy",
 ]
 
 
@@ -59,6 +60,12 @@ class RunStats(C.Structure):
                 ("nrec", C.c_int64), ("final_chi2", C.c_double), ("final_ndf", C.c_int64),
                 ("ms_structure", C.c_double), ("ms_passes", C.c_double), ("ms_solves", C.c_double),
                 ("exit_code", C.c_int32)]
+
+
+class JobStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("nvar", C.c_int32), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
+                ("run_ms", C.c_double), ("chi2", C.c_double), ("ndf", C.c_int64), ("istop", C.c_int32),
+                ("itn", C.c_int32), ("rel_residual", C.c_double)]
 
 
 class MpgpuError(RuntimeError):
@@ -100,6 +107,11 @@ def lib():
         L.mpgpu_avprod.argtypes = [C.c_void_p, D, D]
         L.mpgpu_get_errors.argtypes = [C.c_void_p, D, D]
         L.mpgpu_run_steering.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(RunStats)]
+        L.mpgpu_server_create.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]
+        L.mpgpu_server_submit.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int64, I64, I32, F, C.c_int32, I32, D, D,
+                                          C.c_int32, I64, I32, D, D, I64]
+        L.mpgpu_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_int32, I32, D, C.POINTER(JobStats)]
+        L.mpgpu_server_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -267,6 +279,75 @@ class Pede:
         return e, g
 
 
+class Server:
+    """Serving engine: jobs admitted by device-memory footprint, run concurrently (one context
+    and stream per worker) on one GPU."""
+
+    def __init__(self, device=0, nworkers=8, budget_bytes=200 << 30, batch_max_params=5000):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.mpgpu_server_create(device, nworkers, budget_bytes, batch_max_params, C.byref(h))
+        if rc:
+            raise MpgpuError(rc, "server create failed")
+        self.h = h
+
+    def submit(self, records, params=None, constraints=None, **options):
+        rp, ii, gg = records
+        rp = np.ascontiguousarray(rp, dtype=np.int64)
+        ii = np.ascontiguousarray(ii, dtype=np.int32)
+        gg = np.ascontiguousarray(gg, dtype=np.float32)
+        o = default_options(**options)
+        pl = pv = ps = None
+        npar = 0
+        if params is not None:
+            pl = np.ascontiguousarray(params[0], dtype=np.int32)
+            pv = np.ascontiguousarray(params[1], dtype=np.float64)
+            ps = np.ascontiguousarray(params[2], dtype=np.float64)
+            npar = len(pl)
+        cp = cl = cf = cr = None
+        ncon = 0
+        if constraints:
+            ptr, labs, facs, rhs = [0], [], [], []
+            for (r, s, l, f) in constraints:
+                labs += list(l)
+                facs += list(f)
+                ptr.append(len(labs))
+                rhs.append(r)
+            cp = np.array(ptr, np.int64)
+            cl = np.array(labs, np.int32)
+            cf = np.array(facs, np.float64)
+            cr = np.array(rhs, np.float64)
+            ncon = len(rhs)
+        jid = C.c_int64()
+        rc = self.L.mpgpu_server_submit(self.h, C.byref(o), len(rp) - 1, _p(rp, I64), _p(ii, I32), _p(gg, F), npar,
+                                        _p(pl, I32), _p(pv, D), _p(ps, D), ncon, _p(cp, I64), _p(cl, I32), _p(cf, D),
+                                        _p(cr, D), C.byref(jid))
+        if rc:
+            raise MpgpuError(rc, "submit failed (job larger than the memory budget?)")
+        return jid.value
+
+    def wait(self, job_id, capacity=0):
+        st = JobStats()
+        lab = np.zeros(max(capacity, 1), np.int32)
+        cor = np.zeros(max(capacity, 1))
+        rc = self.L.mpgpu_server_wait(self.h, job_id, capacity, _p(lab, I32) if capacity else None,
+                                      _p(cor, D) if capacity else None, C.byref(st))
+        out = {f: getattr(st, f) for f, _ in JobStats._fields_}
+        n = min(capacity, st.nvar)
+        return rc, out, lab[:n], cor[:n]
+
+    def close(self):
+        if self.h:
+            self.L.mpgpu_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def run_steering(steering_file, output_dir=".", device=0):
     st = RunStats()
     rc = lib().mpgpu_run_steering(steering_file.encode(), output_dir.encode(), device, C.byref(st))
@@ -296,6 +377,9 @@ def gen():
         G.mpgen_pixel.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double,
                                   C.POINTER(_GenRecords), D]
+        G.mpgen_brokenlines.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.POINTER(_GenRecords)]
         G.mpgen_write.argtypes = [C.c_char_p, C.POINTER(_GenRecords), C.c_int]
         G.mpgen_free.argtypes = [C.POINTER(_GenRecords)]
         _gen = G
@@ -335,6 +419,18 @@ def pixel_records(seed, nlayers, nx, ny, ntracks, hmin, hmax, wx=2.0, wy=5.0, z0
     if rc:
         raise MpgpuError(rc, "generator failed")
     return _take(r) + (tr,)
+
+
+def brokenlines_records(seed, nlayers, nx, ny, ntracks, pmin, pmax, wx=2.0, wy=5.0, z0=10.0, dz=1.0,
+                        slope_sigma=0.05, sigma=0.002, theta0=2e-4):
+    """Broken-lines tracks: 4 border + 2 offsets per plane as local parameters (bordered-band
+    local fits), 6 rigid-body globals per hit module."""
+    r = _GenRecords()
+    rc = gen().mpgen_brokenlines(seed, nlayers, nx, ny, wx, wy, z0, dz, ntracks, pmin, pmax, slope_sigma, sigma,
+                                 theta0, C.byref(r))
+    if rc:
+        raise MpgpuError(rc, "generator failed")
+    return _take(r)
 
 
 def pixel_labels(nlayers, nx, ny):

@@ -1,0 +1,48 @@
+"""Serving engine: concurrent jobs on one GPU give the same result as the job run alone, and
+every job reports its latency."""
+import numpy as np
+import pytest
+
+import mpgpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _job(seed, nl, nx, ny, ntr):
+    rp, ii, gg, _ = mpgpu.pixel_records(seed, nl, nx, ny, ntr, 4, 8, wx=100.0 / nx, wy=100.0 / ny)
+    weak = mpgpu.pixel_weak_labels(nl, nx, ny)
+    params = (np.array(weak), np.zeros(len(weak)), np.array(mpgpu.pixel_presigma(weak)))
+    return (rp, ii, gg), params, mpgpu.pixel_constraints(nl, nx, ny)
+
+
+def test_alone_vs_in_the_mix():
+    rec, par, cons = _job(1, 6, 4, 3, 3000)
+    opt = dict(matsto=mpgpu.FULL, metsol=mpgpu.DECOMPOSITION, mreqenf=5, mcount=1)
+    S = mpgpu.Server(nworkers=4, budget_bytes=8 << 30)
+    j = S.submit(rec, par, cons, **opt)
+    rc, st, lab0, x0 = S.wait(j, capacity=100000)
+    assert rc == 0 and st["nvar"] > 0 and st["latency_ms"] > 0
+    others = [_job(10 + k, 5 + k % 3, 3 + k % 2, 3, 1500 + 200 * k) for k in range(6)]
+    ids = []
+    for k, (r2, p2, c2) in enumerate(others):
+        o2 = dict(opt) if k % 2 == 0 else dict(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mbandw=6, mreqenf=5, mcount=1,
+                                                mrestl=1e-10)
+        ids.append(S.submit(r2, p2, c2, **o2))
+        if k == 2:
+            jm = S.submit(rec, par, cons, **opt)
+    for i in ids:
+        rc2, st2, _, _ = S.wait(i)
+        assert rc2 == 0, st2
+    rc, st, lab1, x1 = S.wait(jm, capacity=100000)
+    assert rc == 0
+    assert np.array_equal(lab0, lab1)
+    assert np.max(np.abs(x1 - x0)) <= 1e-9 * np.max(np.abs(x0))
+    S.close()
+
+
+def test_budget_rejects_oversized_job():
+    rec, par, cons = _job(2, 4, 3, 3, 500)
+    S = mpgpu.Server(nworkers=1, budget_bytes=1 << 20)
+    with pytest.raises(mpgpu.MpgpuError):
+        S.submit(rec, par, cons, matsto=mpgpu.FULL, metsol=mpgpu.DECOMPOSITION, mreqenf=5, mcount=1)
+    S.close()
