@@ -388,6 +388,58 @@ __global__ void k_minres_update(int64_t n, const double *v, double *w, double *w
         x[i] = x[i] + phi * wn;
     }
 }
+// MINRES-QLP vector updates (src/minresqlpModule.f90:970-1034).
+// MINRES step: wl2 = wl; wl = w; w = (v - epln*wl2 - dlta*wl)*s (if upd_w); x += tau*w (if upd_x)
+__global__ void k_qlp_minres(int64_t n, const double *v, double *w, double *wl, double *wl2, double *x, double epln,
+                             double dlta, double s, double tau, int upd_w, int upd_x)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double a2 = wl[i], a1 = w[i];
+        wl2[i] = a2;
+        wl[i] = a1;
+        double wn = upd_w ? (v[i] - epln * a2 - dlta * a1) * s : a1;
+        w[i] = wn;
+        if (upd_x) x[i] = x[i] + tau * wn;
+    }
+}
+// QLP step: on the first QLP iteration the MINRES directions are converted (w_{k-3..k-1})
+// and xl2 rebuilt from x; then the right reflections rotate (wl2, wl, w, v) and
+// xl2 += ul2*wl2, x = xl2 + ul*wl + u*w
+__global__ void k_qlp_step(int64_t n, double *v, double *w, double *wl, double *wl2, double *x, double *xl2, QlpStep c)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double V = v[i], W = w[i], WL = wl[i], WL2 = wl2[i], X = x[i], XL2 = xl2[i];
+        if (c.first) {
+            XL2 = 0.0;
+            if (c.itn > 1) {
+                if (c.itn > 3) WL2 = c.gamal3 * WL2 + c.veplnl2 * WL + c.etal * W;
+                if (c.itn > 2) WL = c.gamal_qlp * WL + c.vepln_qlp * W;
+                W = c.gama_qlp * W;
+                XL2 = X - c.ul_qlp * WL - c.u_qlp * W;
+            }
+        }
+        if (c.itn == 1) {
+            WL2 = WL;
+            WL = c.sr1 * V;
+            W = -c.cr1 * V;
+        } else if (c.itn == 2) {
+            WL2 = WL;
+            double nwl = c.cr1 * W + c.sr1 * V;
+            W = c.sr1 * W - c.cr1 * V;
+            WL = nwl;
+        } else {
+            double a2 = WL, a1 = W;
+            double nw = c.sr2 * a2 - c.cr2 * V;
+            WL2 = c.cr2 * a2 + c.sr2 * V;
+            V = c.cr1 * a1 + c.sr1 * nw;
+            W = c.sr1 * a1 - c.cr1 * nw;
+            WL = V;
+        }
+        XL2 = XL2 + c.ul2 * WL2;
+        X = XL2 + c.ul * WL + c.u * W;
+        v[i] = V; w[i] = W; wl[i] = WL; wl2[i] = WL2; x[i] = X; xl2[i] = XL2;
+    }
+}
 __global__ void k_copy(int64_t n, const double *a, double *b)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
@@ -765,6 +817,12 @@ void l_axpy_neg(int64_t n, double c, const double *r, double *y, hipStream_t s)
 void l_minres_update(int64_t n, const double *v, double *w, double *w1, double *w2, double *x, double oldeps, double delta,
                      double denom, double phi, hipStream_t s)
 { hipLaunchKernelGGL(k_minres_update, dim3(nblk(n)), dim3(TPB), 0, s, n, v, w, w1, w2, x, oldeps, delta, denom, phi); }
+void l_qlp_minres(int64_t n, const double *v, double *w, double *wl, double *wl2, double *x, double epln, double dlta,
+                  double sc, double tau, int upd_w, int upd_x, hipStream_t s)
+{ hipLaunchKernelGGL(k_qlp_minres, dim3(nblk(n)), dim3(TPB), 0, s, n, v, w, wl, wl2, x, epln, dlta, sc, tau, upd_w, upd_x); }
+void l_qlp_step(int64_t n, double *v, double *w, double *wl, double *wl2, double *x, double *xl2, const QlpStep &c,
+                hipStream_t s)
+{ hipLaunchKernelGGL(k_qlp_step, dim3(nblk(n)), dim3(TPB), 0, s, n, v, w, wl, wl2, x, xl2, c); }
 void l_copy(int64_t n, const double *a, double *b, hipStream_t s)
 { hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(TPB), 0, s, n, a, b); }
 void l_sub(int64_t n, const double *b, const double *ax, double *r, hipStream_t s)

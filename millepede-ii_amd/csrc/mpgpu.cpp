@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <numeric>
@@ -1311,6 +1312,214 @@ done:
     }
 }
 
+// Givens reflection of MINRES-QLP (symortho, src/minresqlpModule.f90:1337-1380)
+static void symortho(double a, double b, double &c, double &s, double &r)
+{
+    const double aa = std::fabs(a), ab = std::fabs(b);
+    if (ab <= DBL_MIN) { s = 0.0; r = aa; c = (a == 0.0) ? 1.0 : a / aa; }
+    else if (aa <= DBL_MIN) { c = 0.0; r = ab; s = b / ab; }
+    else if (ab > aa) { double t = a / b; s = (b / ab) / std::sqrt(1.0 + t * t); c = s * t; r = b / s; }
+    else { double t = b / a; c = (a / aa) / std::sqrt(1.0 + t * t); s = c * t; r = a / c; }
+}
+
+static double hyp(double a, double b, double c = 0.0) { return std::sqrt(a * a + b * b + c * c); }
+
+// MINRES-QLP, restating src/minresqlpModule.f90:411-1330 as mminrsqlp calls it
+// (src/pede.f90:8341-8437): rtol = mrestl, maxxnorm = n/sqrt(eps), trancond from mresmode
+// (1: QR only, 2: QLP only, else mrestranscond), Acondlim and shift absent, symmetry checks on.
+// Lanczos vectors live in HBM; the recurrences run on the host in FP64.
+void minresqlp_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats *st)
+{
+    const int64_t n = c->nall;
+    hipStream_t s = c->stream;
+    Precond P;
+    setup_precond(c, P);
+    const bool precon = P.kind != 0;
+    DBuf<double> r1, r2, v, w, wl, wl2, xl2, y;
+    r1.alloc(n); r2.alloc(n); v.alloc(n); w.alloc(n); wl.alloc(n); wl2.alloc(n); xl2.alloc(n); y.alloc(n);
+    const double eps = 2.220446049250313e-16, one = 1.0, zero = 0.0;
+    const double EPSINV = std::pow(10.0, std::floor(std::log10(one / eps)));
+    const double NORMMAX = std::pow(10.0, std::floor(std::log10(one / eps) / 2));
+    const double rtol = c->opt.mrestl;
+    const int itnlim = c->opt.itnlim;
+    double trancond = c->opt.mrmode == 1 ? one / eps : (c->opt.mrmode == 2 ? one : c->opt.mrtcnd);
+    trancond = std::min(trancond, NORMMAX);
+    const double maxxnorm = std::min((double)n / std::sqrt(eps), one / eps);
+    const double Acondlim = EPSINV, ieps = 0.1 / eps;
+    auto nrm = [&](const double *a) { return std::sqrt(dot_dev(c, n, a, a)); };
+
+    int istop = 0, itn = 0, QLPiter = 0;
+    bool lastiter = false;
+    double t1 = 0, t2 = 0, gminl = 0, gminl2 = 0, sc, t, z, epsa;
+    double xnorm = 0, xl2norm = 0, Axnorm = 0, Anorm = 0, Acond = 1, pnorm = 0, relresl = 0, relAresl = 0, Arnorm = 0;
+    HIPCHK(hipMemsetAsync(x_dev, 0, sizeof(double) * n, s));
+    xl2.zero(s);
+    l_copy(n, b_dev, y.p, s);
+    l_copy(n, b_dev, r1.p, s);
+    if (precon) apply_precond(c, P, b_dev, y.p);
+    double beta1 = dot_dev(c, n, b_dev, y.p);
+    if (beta1 < zero && nrm(y.p) > eps) istop = 11;
+    if (beta1 == zero) istop = 3;
+    beta1 = std::sqrt(beta1);
+    if (precon) {
+        apply_precond(c, P, y.p, r2.p);
+        sc = dot_dev(c, n, y.p, y.p);
+        t = dot_dev(c, n, r1.p, r2.p);
+        z = std::fabs(sc - t);
+        epsa = (std::fabs(sc) + eps) * std::pow(eps, 0.33333);
+        if (z > epsa) istop = 10;
+    }
+    avprod_dev(c, y.p, w.p);
+    avprod_dev(c, w.p, r2.p);
+    sc = dot_dev(c, n, w.p, w.p);
+    t = dot_dev(c, n, y.p, r2.p);
+    z = std::fabs(sc - t);
+    epsa = (std::fabs(sc) + eps) * std::pow(eps, 0.33333);
+    if (z > epsa) istop = 9;
+
+    double tau = 0, taul = 0, taul2 = 0, gmin = 0, beta = 0, betal = 0, betan = beta1, dbar = 0, dltan = 0, eta = 0,
+           etal = 0, etal2 = 0, vepln = 0, veplnl = 0, veplnl2 = 0, eplnn = 0, epln = 0, gama = 0, gamal = 0, gamal2 = 0,
+           gamal3 = 0, phi = beta1, cr1 = -1, sr1 = 0, cr2 = -1, sr2 = 0, cs = -1, sn = 0, ul4 = 0, ul3 = 0, ul2 = 0, ul = 0,
+           u = 0, gamal_QLP = 0, vepln_QLP = 0, gama_QLP = 0, ul_QLP = 0, u_QLP = 0;
+    double rnorm = betan, relres = rnorm / (Anorm * xnorm + beta1);
+    l_copy(n, b_dev, r2.p, s);
+    w.zero(s);
+    wl.zero(s);
+    while (istop <= 0) {
+        itn++;
+        betal = beta;
+        beta = betan;
+        l_scale_copy(n, one / beta, y.p, v.p, s);
+        avprod_dev(c, v.p, y.p);
+        if (itn >= 2) l_axpy_neg(n, beta / betal, r1.p, y.p, s);
+        double alfa = dot_dev(c, n, v.p, y.p);
+        l_axpy_neg(n, alfa / beta, r2.p, y.p, s);
+        std::swap(r1.p, r2.p);          // r1 = r2
+        l_copy(n, y.p, r2.p, s);        // r2 = y
+        if (!precon) betan = nrm(y.p);
+        else {
+            apply_precond(c, P, r2.p, y.p);
+            betan = dot_dev(c, n, r2.p, y.p);
+            if (betan > zero) betan = std::sqrt(betan);
+            else if (nrm(y.p) > eps) { istop = 11; break; }
+        }
+        pnorm = itn == 1 ? hyp(alfa, betan) : hyp(beta, alfa, betan);
+
+        dbar = dltan;
+        double dlta = cs * dbar + sn * alfa;
+        epln = eplnn;
+        double gbar = sn * dbar - cs * alfa;
+        eplnn = sn * betan;
+        dltan = -cs * betan;
+        const double dlta_QLP = dlta;
+        gamal3 = gamal2;
+        gamal2 = gamal;
+        gamal = gama;
+        symortho(gbar, betan, cs, sn, gama);
+        const double gama_tmp = gama;
+        taul2 = taul;
+        taul = tau;
+        tau = cs * phi;
+        phi = sn * phi;
+        Axnorm = std::sqrt(Axnorm * Axnorm + tau * tau);
+        if (itn > 2) {
+            veplnl2 = veplnl;
+            etal2 = etal;
+            etal = eta;
+            double dlta_tmp = sr2 * vepln - cr2 * dlta;
+            veplnl = cr2 * vepln + sr2 * dlta;
+            dlta = dlta_tmp;
+            eta = sr2 * gama;
+            gama = -cr2 * gama;
+        }
+        if (itn > 1) {
+            double gtmp;
+            symortho(gamal, dlta, cr1, sr1, gtmp);
+            gamal = gtmp;
+            vepln = sr1 * gama;
+            gama = -cr1 * gama;
+        }
+        ul4 = ul3;
+        ul3 = ul2;
+        if (itn > 2) ul2 = (taul2 - etal2 * ul4 - veplnl2 * ul3) / gamal2;
+        if (itn > 1) ul = (taul - etal * ul3 - veplnl * ul2) / gamal;
+        const double xnorm_tmp = hyp(xl2norm, ul2, ul);
+        if (std::fabs(gama) > eps) {
+            u = (tau - eta * ul2 - vepln * ul) / gama;
+            if (relAresl < relresl && hyp(xnorm_tmp, u) > maxxnorm) { u = zero; istop = 12; }
+        } else {
+            u = zero;
+            istop = 14;
+        }
+        xl2norm = hyp(xl2norm, ul2);
+        xnorm = hyp(xl2norm, ul, u);
+        if (Acond < trancond && istop == 0 && QLPiter == 0) {          // MINRES updates
+            const bool upd_x = xnorm < maxxnorm;
+            l_qlp_minres(n, v.p, w.p, wl.p, wl2.p, x_dev, epln, dlta_QLP, gama_tmp > eps ? one / gama_tmp : 0.0, tau,
+                         gama_tmp > eps, upd_x, s);
+            if (!upd_x) { istop = 12; lastiter = true; }
+        } else {                                                       // MINRES-QLP updates
+            QLPiter++;
+            QlpStep q{};
+            q.first = QLPiter == 1;
+            q.itn = itn;
+            q.gamal3 = gamal3; q.veplnl2 = veplnl2; q.etal = etal; q.gamal_qlp = gamal_QLP; q.vepln_qlp = vepln_QLP;
+            q.gama_qlp = gama_QLP; q.ul_qlp = ul_QLP; q.u_qlp = u_QLP; q.sr1 = sr1; q.cr1 = cr1; q.sr2 = sr2; q.cr2 = cr2;
+            q.ul2 = ul2; q.ul = ul; q.u = u;
+            l_qlp_step(n, v.p, w.p, wl.p, wl2.p, x_dev, xl2.p, q, s);
+        }
+        double gtmp = gamal;
+        symortho(gtmp, eplnn, cr2, sr2, gamal);
+        gamal_QLP = gtmp;
+        vepln_QLP = vepln;
+        gama_QLP = gama;
+        ul_QLP = ul;
+        u_QLP = u;
+        const double abs_gama = std::fabs(gama);
+        Anorm = std::max(std::max(Anorm, pnorm), std::max(gamal, abs_gama));
+        if (itn == 1) { gmin = gama; gminl = gmin; }
+        else { gminl2 = gminl; gminl = gmin; gmin = std::min(std::min(gminl2, gamal), abs_gama); }
+        const double Acondl = Acond;
+        Acond = Anorm / gmin;
+        const double rnorml = rnorm;
+        relresl = relres;
+        if (istop != 14) rnorm = phi;
+        relres = rnorm / (Anorm * xnorm + beta1);
+        const double rootl = hyp(gbar, dltan);
+        relAresl = rootl / Anorm;
+        const double epsx = Anorm * xnorm * eps;
+        if (istop == 0 || istop == 14) { t1 = one + relres; t2 = one + relAresl; }
+        if (t1 <= one) istop = 5;
+        else if (t2 <= one) istop = 7;
+        else if (relres <= rtol) istop = 4;
+        else if (relAresl <= rtol) istop = 6;
+        else if (epsx >= beta1) istop = 2;
+        else if (xnorm >= maxxnorm) istop = 12;
+        else if (Acond >= Acondlim || Acond >= ieps) istop = 13;
+        else if (itn >= itnlim) istop = 8;
+        else if (betan < eps) istop = 1;
+        if (istop != 0) {
+            if (istop == 6 || istop == 7 || istop == 12 || istop == 13) lastiter = true;
+            if (lastiter) { itn--; Acond = Acondl; rnorm = rnorml; relres = relresl; }
+            avprod_dev(c, x_dev, r1.p);
+            l_sub(n, b_dev, r1.p, r1.p, s);
+            avprod_dev(c, r1.p, wl2.p);
+            Arnorm = nrm(wl2.p);
+            break;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        st->istop = istop;
+        st->itn = itn;
+        st->anorm = Anorm;
+        st->acond = Acond;
+        st->rnorm = rnorm;
+        st->arnorm = Arnorm;
+        st->ynorm = xnorm;
+    }
+}
+
 double rel_residual(mpgpu_ctx *c, const double *b_dev, const double *x_dev)
 {
     const int64_t n = c->nall;
@@ -1375,8 +1584,11 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
         loc.rel_residual = rel_residual(c, b.p, xd.p);
         std::swap(keep.p, c->d_mat.p);
         c->have_inverse = true;
-    } else if (c->opt.metsol == MPGPU_MINRES || c->opt.metsol == MPGPU_MINRES_QLP) {
+    } else if (c->opt.metsol == MPGPU_MINRES) {
         minres_dev(c, b.p, xd.p, &loc);
+        loc.rel_residual = rel_residual(c, b.p, xd.p);
+    } else if (c->opt.metsol == MPGPU_MINRES_QLP) {
+        minresqlp_dev(c, b.p, xd.p, &loc);
         loc.rel_residual = rel_residual(c, b.p, xd.p);
     } else if (c->opt.metsol == MPGPU_DECOMPOSITION || c->opt.metsol == MPGPU_CHOLESKY_FP32_IR) {
         if (c->opt.matsto == MPGPU_SPARSE) throw MpError(36, "method decomposition needs full matrix storage");

@@ -62,4 +62,10 @@ struct LdsPlan {
     int32_t bytes;
 };
 
+// scalars of one MINRES-QLP vector step (k_qlp_step)
+struct QlpStep {
+    int first, itn;
+    double gamal3, veplnl2, etal, gamal_qlp, vepln_qlp, gama_qlp, ul_qlp, u_qlp, sr1, cr1, sr2, cr2, ul2, ul, u;
+};
+
 } // namespace mpgpu

@@ -89,6 +89,10 @@ void l_scale_copy(int64_t n, double sc, const double *a, double *o, hipStream_t 
 void l_axpy_neg(int64_t n, double c, const double *r, double *y, hipStream_t s);
 void l_minres_update(int64_t n, const double *v, double *w, double *w1, double *w2, double *x, double oldeps, double delta,
                      double denom, double phi, hipStream_t s);
+void l_qlp_minres(int64_t n, const double *v, double *w, double *wl, double *wl2, double *x, double epln, double dlta,
+                  double sc, double tau, int upd_w, int upd_x, hipStream_t s);
+void l_qlp_step(int64_t n, double *v, double *w, double *wl, double *wl2, double *x, double *xl2, const QlpStep &c,
+                hipStream_t s);
 void l_copy(int64_t n, const double *a, double *b, hipStream_t s);
 void l_sub(int64_t n, const double *b, const double *ax, double *r, hipStream_t s);
 void l_precons(int32_t n, int32_t p, const double *diag, double *cu, int *nrkd, const int64_t *cptr, const int32_t *cidx,

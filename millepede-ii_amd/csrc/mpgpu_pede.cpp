@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <functional>
+#include <map>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -28,7 +30,7 @@ struct Steering {
     std::vector<ParLine> params;
     std::vector<EqLine> cons, meas;
     mpgpu_options opt;
-    int mitera = 1, matrit = 0, lsearch = 2, ipcntr = 0, igcorr = 0;
+    int mitera = 1, matrit = 0, lsearch = 2, ipcntr = 0, igcorr = 0, isubit = 0, nofeas = 0;
     float dflim = 0.0f;
     bool method_given = false;
     std::string err;
@@ -181,6 +183,8 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
         if (key_match(k, "maxlocalcond")) { if (!nums.empty()) S.opt.cndlmx = (float)nums[0]; continue; }
         if (key_match(k, "printcounts")) { S.ipcntr = nums.empty() ? 1 : (int)std::llround(nums[0]); continue; }
         if (key_match(k, "globalcorr")) { S.igcorr = 1; continue; }
+        if (key_match(k, "subito")) { S.isubit = 1; continue; }
+        if (key_match(k, "nofeasibilitycheck")) { S.nofeas = 1; continue; }
         if (key_match(k, "withmultipliers")) { S.opt.withmultipliers = 1; continue; }
         if (key_match(k, "withelimination")) { S.opt.withmultipliers = 1; continue; } // same solution, see DESIGN
         if (key_match(k, "countrecords")) { S.opt.mcount = 1; continue; }
@@ -312,6 +316,144 @@ struct LineSearch {
         tot = tot + alpha;
         step = tot;
         return -1;
+    }
+};
+
+// Feasibility of the linear constraints (feasma/feasib, src/pede.f90:1377-1680): the
+// constraints are split into groups that share no parameter; per group the product matrix
+// A A^T (A = constraint factors on the variable parameters) is inverted once, and a correction
+// A^T (A A^T)^-1 r moves the parameters onto the constraint surface (two passes, the first
+// skipped when the rms discrepancy is below the cut).
+struct Feasibility {
+    struct Group { std::vector<int> cons; std::vector<double> inv; };
+    std::vector<Group> groups;
+    const std::vector<EqLine> *cons = nullptr;
+    std::vector<int32_t> tot_of_con_term;   // total index of each constraint term (-1: unknown label)
+    std::vector<int64_t> term_ptr;
+
+    // symmetric Gauss-Jordan with diagonal pivoting on a dense n x n matrix (the sqminv
+    // algorithm, src/mpnum.f90:97-213): rows without a pivot above 16 eps of their original
+    // diagonal are cleared, so dependent constraints get zero weight
+    static void invert(std::vector<double> &V, int n)
+    {
+        std::vector<double> diag(n);
+        std::vector<char> done(n, 0);
+        for (int i = 0; i < n; i++) diag[i] = std::fabs(V[(size_t)i * n + i]);
+        const double eps = 16.0 * 2.220446049250313e-16;
+        for (int it = 0; it < n; it++) {
+            int k = -1;
+            double vkk = 0.0;
+            for (int j = 0; j < n; j++)
+                if (!done[j] && std::fabs(V[(size_t)j * n + j]) > std::max(std::fabs(vkk), eps * diag[j])) {
+                    vkk = V[(size_t)j * n + j];
+                    k = j;
+                }
+            if (k < 0) break;
+            done[k] = 1;
+            double piv = 1.0 / V[(size_t)k * n + k];
+            for (int j = 0; j < n; j++)
+                for (int l = 0; l < n; l++)
+                    if (j != k && l != k) V[(size_t)j * n + l] -= V[(size_t)j * n + k] * piv * V[(size_t)k * n + l];
+            for (int j = 0; j < n; j++)
+                if (j != k) { V[(size_t)j * n + k] *= piv; V[(size_t)k * n + j] *= piv; }
+            V[(size_t)k * n + k] = -piv;
+        }
+        for (int j = 0; j < n; j++)
+            for (int l = 0; l < n; l++) V[(size_t)j * n + l] = (done[j] && done[l]) ? -V[(size_t)j * n + l] : 0.0;
+    }
+
+    void setup(const std::vector<EqLine> &C, const std::vector<int32_t> &labels, const std::vector<int32_t> &var)
+    {
+        cons = &C;
+        const int nc = (int)C.size();
+        term_ptr.assign(1, 0);
+        tot_of_con_term.clear();
+        for (auto &e : C) {
+            for (int32_t l : e.labels) {
+                auto it = std::lower_bound(labels.begin(), labels.end(), l);
+                tot_of_con_term.push_back(it != labels.end() && *it == l ? (int32_t)(it - labels.begin()) : -1);
+            }
+            term_ptr.push_back((int64_t)tot_of_con_term.size());
+        }
+        // connected components of constraints over shared variable parameters
+        std::vector<int> parent(nc);
+        for (int i = 0; i < nc; i++) parent[i] = i;
+        std::function<int(int)> root = [&](int i) { while (parent[i] != i) i = parent[i] = parent[parent[i]]; return i; };
+        std::map<int32_t, int> owner;
+        for (int i = 0; i < nc; i++)
+            for (int64_t q = term_ptr[i]; q < term_ptr[i + 1]; q++) {
+                int32_t t = tot_of_con_term[q];
+                if (t < 0 || var[t] <= 0) continue;
+                auto ins = owner.emplace(var[t], i);
+                if (!ins.second) parent[root(i)] = root(ins.first->second);
+            }
+        std::map<int, int> gid;
+        groups.clear();
+        for (int i = 0; i < nc; i++) {
+            auto ins = gid.emplace(root(i), (int)groups.size());
+            if (ins.second) groups.emplace_back();
+            groups[ins.first->second].cons.push_back(i);
+        }
+        for (auto &g : groups) {
+            int n = (int)g.cons.size();
+            std::map<int32_t, std::vector<std::pair<int, double>>> cols;   // var -> (row, factor)
+            for (int a = 0; a < n; a++) {
+                int i = g.cons[a];
+                for (int64_t q = term_ptr[i]; q < term_ptr[i + 1]; q++) {
+                    int32_t t = tot_of_con_term[q];
+                    if (t >= 0 && var[t] > 0) cols[var[t]].push_back({a, C[i].f[q - term_ptr[i]]});
+                }
+            }
+            g.inv.assign((size_t)n * n, 0.0);
+            for (auto &cv : cols) {
+                // merge repeated labels of one constraint first
+                std::map<int, double> f;
+                for (auto &rf : cv.second) f[rf.first] += rf.second;
+                for (auto &x : f)
+                    for (auto &y : f) g.inv[(size_t)x.first * n + y.first] += x.second * y.second;
+            }
+            invert(g.inv, n);
+        }
+    }
+
+    // returns 1 if a correction was applied; par is indexed by total index
+    int apply(double concut, std::vector<double> &par, const std::vector<int32_t> &var) const
+    {
+        const auto &C = *cons;
+        const int nc = (int)C.size();
+        if (nc == 0) return 0;
+        int iact = 0;
+        std::vector<double> r(nc);
+        for (int pass = 1; pass <= 2; pass++) {
+            double s1 = 0;
+            for (int i = 0; i < nc; i++) {
+                double rhs = C[i].rhs;
+                for (int64_t q = term_ptr[i]; q < term_ptr[i + 1]; q++) {
+                    int32_t t = tot_of_con_term[q];
+                    if (t >= 0) rhs -= C[i].f[q - term_ptr[i]] * par[t];
+                }
+                r[i] = rhs;
+                s1 += rhs * rhs;
+            }
+            s1 = std::sqrt(s1 / nc);
+            if (pass == 1 && s1 < concut) return 0;
+            iact = 1;
+            std::vector<double> lam(nc, 0.0);
+            for (auto &g : groups) {
+                int n = (int)g.cons.size();
+                for (int a = 0; a < n; a++) {
+                    double s = 0;
+                    for (int b2 = 0; b2 < n; b2++) s += g.inv[(size_t)a * n + b2] * r[g.cons[b2]];
+                    lam[g.cons[a]] = s;
+                }
+            }
+            for (int i = 0; i < nc; i++)
+                for (int64_t q = term_ptr[i]; q < term_ptr[i + 1]; q++) {
+                    int32_t t = tot_of_con_term[q];
+                    if (t >= 0 && var[t] > 0) par[t] += lam[i] * C[i].f[q - term_ptr[i]];
+                }
+        }
+        return iact;
     }
 };
 
@@ -460,11 +602,11 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     float wolfc2 = (metsol >= 4 && metsol <= 6) ? 0.1f : 0.5f;
     int minf = (metsol >= 4 && metsol <= 6) ? 3 : (metsol == 2 ? 2 : 1);
     const int matrit = std::min(S.matrit, S.mitera);
-    int nloopn = 0, iterat = 0, lastit = 0;
+    int nloopn = 0, iterat = -1, lastit = -1;
     float chicut = S.opt.chicut;
     int64_t nrej_last = 0;
     mpgpu_pass_stats ps_st{};
-    std::vector<double> b(nall), x(nall), xls(nvar), corr(nvar), gvec(nvar);
+    std::vector<double> b(nall), x(nall), xls(nvar), corr(nvar);
     std::vector<int64_t> cnt_var(nvar);
     double tp = 0, tsol = 0;
     auto pass = [&](int icalcm) -> int {
@@ -490,61 +632,111 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
         mpgpu_get_vector(c, b.data(), cnt_var.data());
         return 0;
     };
-    if ((rc = pass(1))) return bail(rc);
-    double fvalue = ps_st.chi2;
-    std::printf(" it fc   fcn_value dfcn_exp  rejects\n  0  1 %12.5E          %8lld\n", fvalue, (long long)nrej_last);
+    double fvalue = 0.0;
     LineSearch LS;
     std::vector<double> err_v, gcor_v;
     bool have_err = false;
-    for (iterat = 1;; iterat++) {
-        // solve with the matrix of the last matrix pass and the current gradient
-        for (int32_t v = 0; v < nvar; v++) xls[v] = par[var2tot[v]];
-        double t = now_ms();
-        mpgpu_solve_stats sst{};
-        if ((rc = mpgpu_solve(c, b.data(), x.data(), &sst))) return bail(rc);
-        tsol += now_ms() - t;
-        if (metsol == MPGPU_INVERSION) {
-            err_v.assign(nvar, 0.0);
-            gcor_v.assign(nvar, 0.0);
-            have_err = mpgpu_get_errors(c, err_v.data(), gcor_v.data()) == 0;
-        }
-        for (int32_t v = 0; v < nvar; v++) { corr[v] = x[v]; gvec[v] = b[v]; }
-        double db = 0, db1 = 0, db2 = 0;
-        for (int32_t v = 0; v < nvar; v++) { db += corr[v] * gvec[v]; db1 += corr[v] * corr[v]; db2 += gvec[v] * gvec[v]; }
-        float delfun = (float)db;
-        double dbsig = 16.0 * std::sqrt(std::max(db1, db2)) * 2.220446049250313e-16;
-        bool lsflag = (S.lsearch > 2 || (S.lsearch == 2 && chicut < 2.25f) ||
-                       (S.lsearch == 1 && chicut < 2.25f && (delfun <= S.dflim || iterat >= S.mitera)));
-        lsflag = lsflag && (db > dbsig);
-        if (lsflag) {
-            LS.define(wolfc2, 10.0f, minf, 10);
-            wolfc2 = (metsol >= 4 && metsol <= 6) ? 0.1f : 0.5f;
-            minf = (metsol >= 4 && metsol <= 6) ? 4 : 3;
-            double step = 0, flines = 0.5 * fvalue;
-            for (;;) {
-                int info = LS.line(nvar, xls.data(), flines, gvec.data(), corr.data(), step);
-                for (int32_t v = 0; v < nvar; v++) par[var2tot[v]] = xls[v];
-                if (info >= 0) break;
-                if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
-                if ((rc = pass(0))) return bail(rc);
-                fvalue = ps_st.chi2;
-                flines = 0.5 * fvalue;
-                for (int32_t v = 0; v < nvar; v++) gvec[v] = b[v];
-            }
-            if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
-        } else {
-            for (int32_t v = 0; v < nvar; v++) { xls[v] += corr[v]; par[var2tot[v]] = xls[v]; }
-            if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
-            if ((rc = pass(0))) return bail(rc);   // function value and gradient at the new point
-            fvalue = ps_st.chi2;
-        }
-        std::printf(" %2d %2d %12.5E %8.2E %8lld\n", iterat, nloopn, fvalue, (double)delfun, (long long)nrej_last);
-        if (delfun <= S.dflim || iterat >= S.mitera) break;
-        if (iterat < matrit) {                   // new matrix (matiter), else reuse it
-            if ((rc = pass(1))) return bail(rc);
-            fvalue = ps_st.chi2;
-        }
+    Feasibility FS;
+    FS.setup(S.cons, labels, var);
+    float concut = 1.0e-12f;                     // first check, then 1e-6 (src/pede.f90:8693-8694)
+    const float concu2 = 1.0e-6f;
+    if (S.nofeas == 0 && FS.apply(concut, par, var)) {
+        std::printf(" Checking feasibility of parameters: parameters are made feasible\n");
+        if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
     }
+    concut = concu2;
+    std::printf(" it fc   fcn_value dfcn_exp  rejects\n");
+
+    // The iteration state machine of xloopn (src/pede.f90:8755-9009).  icalcm: 1 = data pass
+    // with matrix, then solve; 0 = function pass only (line search); -1 = solve with the
+    // matrix of the last matrix pass and the gradient of the last pass.
+    int icalcm = 1, jcalcm = -1, litera = 0, iagain = 0, nloopsol = 0;
+    bool lsflag = false;
+    float delfun = 0.0f;
+    iterat = -1;
+    auto line = [&](int it) {
+        std::printf(" %2d %2d %12.5E %8.2E %8lld\n", it, nloopn, fvalue, (double)delfun, (long long)nrej_last);
+    };
+    for (;;) {
+        if (iterat >= 0 && jcalcm + 1 != 0) {
+            if (iterat == 0) iterat = 1;
+            else if (iterat != litera) { litera = iterat; line(iterat - 1); }
+        }
+        jcalcm = icalcm;
+        if (icalcm >= 0) {                        // block 1: data pass (b = gradient at par)
+            if ((rc = pass(icalcm))) return bail(rc);
+            fvalue = ps_st.chi2;
+        }
+        if (std::abs(icalcm) == 1) {              // block 2: new solution
+            for (int32_t v = 0; v < nvar; v++) xls[v] = par[var2tot[v]];
+            iterat++;
+            double t = now_ms();
+            mpgpu_solve_stats sst{};
+            if ((rc = mpgpu_solve(c, b.data(), x.data(), &sst))) return bail(rc);
+            tsol += now_ms() - t;
+            nloopsol = nloopn;
+            if (metsol == MPGPU_INVERSION) {
+                err_v.assign(nvar, 0.0);
+                gcor_v.assign(nvar, 0.0);
+                have_err = mpgpu_get_errors(c, err_v.data(), gcor_v.data()) == 0;
+            }
+            // make the step feasible: apply it, correct the parameters, take the difference
+            std::vector<double> trial = par;
+            for (int32_t v = 0; v < nvar; v++) trial[var2tot[v]] += x[v];
+            FS.apply(concut, trial, var);
+            concut = concu2;
+            for (int32_t v = 0; v < nvar; v++) corr[v] = trial[var2tot[v]] - par[var2tot[v]];
+            double db = 0, db1 = 0, db2 = 0;
+            for (int32_t v = 0; v < nvar; v++) { db += corr[v] * b[v]; db1 += corr[v] * corr[v]; db2 += b[v] * b[v]; }
+            delfun = (float)db;
+            double dbsig = 16.0 * std::sqrt(std::max(db1, db2)) * 2.220446049250313e-16;
+            lsflag = (S.lsearch > 2 || (S.lsearch == 2 && chicut < 2.25f) ||
+                      (S.lsearch == 1 && chicut < 2.25f && (delfun <= S.dflim || iterat >= S.mitera)));
+            lsflag = lsflag && (db > dbsig);
+            if (lsflag) {
+                LS.define(wolfc2, 10.0f, minf, 10);
+                wolfc2 = (metsol >= 4 && metsol <= 6) ? 0.1f : 0.5f;
+                minf = (metsol >= 4 && metsol <= 6) ? 4 : 3;
+            }
+            if (db <= -dbsig) {                   // function not decreasing
+                std::printf(" Function not decreasing: %g\n", db);
+                if (db > -1.0e-3 && ++iagain <= 1) { icalcm = 1; continue; }
+                if (db <= -1.0e-3) iagain = -1;
+                break;
+            }
+            iagain = 0;
+            icalcm = 0;
+        }
+        // block 3: line search step (or the full step)
+        int info = 10;
+        if (lsflag) {
+            double step = 0, flines = 0.5 * fvalue;
+            info = LS.line(nvar, xls.data(), flines, b.data(), corr.data(), step);
+        } else if (nloopn == nloopsol) {
+            for (int32_t v = 0; v < nvar; v++) xls[v] += corr[v];
+        }
+        int nan = 0;
+        for (int32_t v = 0; v < nvar; v++) {
+            if (!(xls[v] <= 0.0) && !(xls[v] > 0.0)) nan++;
+            par[var2tot[v]] = xls[v];
+        }
+        if (nan > 0) {
+            std::fprintf(stderr, "Result vector contains %d NaNs - stop\n", nan);
+            mpgpu_destroy(c);
+            return finish(25);
+        }
+        if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
+        if (S.isubit) { std::printf(" Subito!     Exit after first step.\n"); break; }
+        if (info == 0 && iagain <= 0) { icalcm = 1; continue; }
+        if (info < 0 || nloopn == nloopsol) continue;   // icalcm = 0: function pass next
+        // block 4: line search converged
+        int iact = FS.apply(concut, par, var);
+        if (iact && (rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
+        if (iact || chicut > 1.0f) { icalcm = iterat < matrit ? 1 : -1; continue; }
+        if (delfun <= S.dflim || iterat >= S.mitera) break;
+        icalcm = iterat < matrit ? 1 : -1;
+    }
+    line(iterat);
     loc.final_chi2 = ps_st.chi2;
     loc.final_ndf = ps_st.ndf;
     loc.iterations = iterat;

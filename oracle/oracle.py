@@ -77,6 +77,8 @@ class Oracle:
         L.orc_presols.argtypes = [C.c_int, C.c_int, C.c_int, D, D, I32, D, D, D]
         L.orc_minres.argtypes = [C.c_int, OP, OP, C.c_void_p, D, C.c_double, C.c_int, D, C.c_int, C.c_double,
                                  I32, I32, D]
+        L.orc_minresqlp.argtypes = [C.c_int, OP, OP, C.c_void_p, D, C.c_double, C.c_int, D, C.c_int, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, I32, I32, D]
         L.orc_packed_matvec.argtypes = [C.c_int, D, D, D]
         L.orc_loopn.argtypes = [C.POINTER(_RecStruct), C.POINTER(_PassOpts), D, D, D, I64, C.POINTER(_PassStats)]
         L.orc_loopn.restype = C.c_int
@@ -160,6 +162,31 @@ class Oracle:
         out = np.zeros(5)
         self.lib.orc_minres(n, fa, fm, None, _p(b, D), 0.0, 1 if precon else 0, _p(x, D), itnlim, rtol,
                             C.byref(istop), C.byref(itn), _p(out, D))
+        return x, istop.value, itn.value, out
+
+    def minresqlp(self, n, aprod, msolve, b, rtol, itnlim=2000, precon=True, maxxnorm=None, trancond=1.0e7,
+                  acondlim=0.0, shift=0.0):
+        """MINRES-QLP as mminrsqlp calls it; maxxnorm defaults to n/sqrt(eps) (src/pede.f90:8365)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(n)
+        if maxxnorm is None:
+            maxxnorm = n / np.sqrt(np.finfo(float).eps)
+
+        def _ap(ctx, nn, xp, yp):
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] = aprod(xa)
+
+        def _ms(ctx, nn, xp, yp):
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] = msolve(xa) if msolve else xa
+
+        fa, fm = OP(_ap), OP(_ms)
+        istop, itn = C.c_int32(), C.c_int32()
+        out = np.zeros(6)
+        self.lib.orc_minresqlp(n, fa, fm, None, _p(b, D), shift, 1 if precon else 0, _p(x, D), itnlim, rtol, maxxnorm,
+                               trancond, acondlim, C.byref(istop), C.byref(itn), _p(out, D))
         return x, istop.value, itn.value, out
 
     # ---- record loop ---------------------------------------------------------------------------
@@ -368,4 +395,38 @@ class Reference:
         f(self._i(n), fa, fm, _p(b, D), C.byref(C.c_double(0.0)), C.byref(C.c_int32(0)),
           C.byref(C.c_int32(1 if precon else 0)), _p(x, D), self._i(itnlim), self._i(0), C.byref(C.c_double(rtol)),
           C.byref(istop), C.byref(itn), *[C.byref(o) for o in out])
+        return x, istop.value, itn.value, np.array([o.value for o in out])
+
+    def minresqlp(self, n, aprod, msolve, b, rtol, itnlim=2000, precon=True, maxxnorm=None, trancond=1.0e7,
+                  acondlim=None, shift=None):
+        """The reference MINRESQLP (src/minresqlpModule.f90:411); absent optional arguments are
+        passed as null pointers, nout = 0 (a preconnected unit, so nothing is printed)."""
+        FOP = C.CFUNCTYPE(None, I32, D, D)
+
+        def _ap(np_, xp, yp):
+            nn = np_[0]
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] = aprod(xa)
+
+        def _ms(np_, xp, yp):
+            nn = np_[0]
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] = msolve(xa)
+
+        if maxxnorm is None:
+            maxxnorm = n / np.sqrt(np.finfo(float).eps)
+        fa = FOP(_ap)
+        fm = FOP(_ms) if precon else C.cast(None, FOP)
+        b = np.asfortranarray(b, dtype=np.float64)
+        x = np.zeros(n)
+        istop, itn = C.c_int32(), C.c_int32()
+        out = [C.c_double() for _ in range(5)]
+        opt = lambda v: None if v is None else C.byref(C.c_double(v))  # noqa: E731
+        f = self.lib._QMminresqlpmodulePminresqlp
+        f.restype = None
+        f(self._i(n), fa, _p(b, D), opt(shift), fm, None, self._i(0), self._i(itnlim), C.byref(C.c_double(rtol)),
+          C.byref(C.c_double(maxxnorm)), C.byref(C.c_double(trancond)), opt(acondlim), _p(x, D), C.byref(istop),
+          C.byref(itn), *[C.byref(o) for o in out])
         return x, istop.value, itn.value, np.array([o.value for o in out])

@@ -612,6 +612,289 @@ done:
 
 /* Dense packed symmetric product y = A x for full storage (avprd0 matsto=1 path,
  * src/pede.f90:5069-5232): every stored off-diagonal element serves both halves. */
+/* ---------------------------------------------------------------------------------------
+ * MINRES-QLP (src/minresqlpModule.f90:411-1330, Choi/Paige/Saunders), as pede calls it
+ * (mminrsqlp, src/pede.f90:8341-8437): no shift unless given, symmetry checks of A and M on,
+ * disable off.  maxxnorm, trancond and acondlim are clipped as in the reference
+ * (1/eps, 10^floor(log10(1/eps)/2), 10^floor(log10(1/eps))).  acondlim <= 0 means "absent".
+ * out6 = rnorm, Arnorm, xnorm, Anorm, Acond, QLPiter.
+ * --------------------------------------------------------------------------------------- */
+static double nrm2s(int n, const double *x)   /* scaled 2-norm, src/minresqlpBlasModule.f90:184-217 */
+{
+    if (n < 1) return 0.0;
+    if (n == 1) return fabs(x[0]);
+    double scale = 0.0, ssq = 1.0;
+    for (int i = 0; i < n; i++) {
+        if (x[i] != 0.0) {
+            double a = fabs(x[i]);
+            if (scale < a) { double r = scale / a; ssq = 1.0 + ssq * (r * r); scale = a; }
+            else { double r = a / scale; ssq = ssq + r * r; }
+        }
+    }
+    return scale * sqrt(ssq);
+}
+static double nrm2v(int n, double a, double b, double c)
+{
+    double v[3] = {a, b, c};
+    return nrm2s(n, v);
+}
+static void symortho(double a, double b, double *c, double *s, double *r)  /* src/minresqlpModule.f90:1337-1380 */
+{
+    double aa = fabs(a), ab = fabs(b), t;
+    if (ab <= DBL_MIN) { *s = 0.0; *r = aa; *c = (a == 0.0) ? 1.0 : a / aa; }
+    else if (aa <= DBL_MIN) { *c = 0.0; *r = ab; *s = b / ab; }
+    else if (ab > aa) { t = a / b; *s = (b / ab) / sqrt(1.0 + t * t); *c = *s * t; *r = b / *s; }
+    else { t = b / a; *c = (a / aa) / sqrt(1.0 + t * t); *s = *c * t; *r = a / *c; }
+}
+
+void orc_minresqlp(int n, orc_op aprod, orc_op msolve, void *ctx, const double *b, double shift, int precon,
+                   double *x, int itnlim, double rtol, double maxxnorm, double trancond, double acondlim,
+                   int *istop_out, int *itn_out, double *out6)
+{
+    const double eps = DBL_EPSILON, one = 1.0, zero = 0.0;
+    const double EPSINV = pow(10.0, floor(log10(one / eps)));
+    const double NORMMAX = pow(10.0, floor(log10(one / eps) / 2));
+    double maxxnorm_ = fmin(maxxnorm, one / eps);
+    double trancond_ = fmin(trancond, NORMMAX);
+    double Acondlim_ = acondlim > 0 ? fmin(acondlim, EPSINV) : EPSINV;
+    double *r1 = calloc(n, 8), *r2 = calloc(n, 8), *v = calloc(n, 8), *w = calloc(n, 8), *wl = calloc(n, 8),
+           *wl2 = calloc(n, 8), *xl2 = calloc(n, 8), *y = calloc(n, 8);
+    double t1 = 0, t2 = 0, gminl = 0, gminl2 = 0, s, t, z, epsa;
+    int lastiter = 0, istop = 0, itn = 0, QLPiter = 0;
+    double beta1 = nrm2s(n, b);
+    const double ieps = 0.1 / eps;
+    double xnorm = 0, xl2norm = 0, Axnorm = 0, Anorm = 0, Acond = 1, pnorm = 0, relresl = 0, relAresl = 0;
+    double Arnorm = 0, relAres = 0;
+    for (int i = 0; i < n; i++) { x[i] = 0.0; xl2[i] = 0.0; y[i] = b[i]; r1[i] = b[i]; }
+    if (precon) msolve(ctx, n, b, y);
+    beta1 = dotp(n, b, y);
+    if (beta1 < zero && nrm2s(n, y) > eps) istop = 11;
+    if (beta1 == zero) istop = 3;
+    beta1 = sqrt(beta1);
+    if (precon) {
+        msolve(ctx, n, y, r2);
+        s = dotp(n, y, y);
+        t = dotp(n, r1, r2);
+        z = fabs(s - t);
+        epsa = (fabs(s) + eps) * pow(eps, 0.33333);
+        if (z > epsa) istop = 10;
+    }
+    aprod(ctx, n, y, w);
+    aprod(ctx, n, w, r2);
+    s = dotp(n, w, w);
+    t = dotp(n, y, r2);
+    z = fabs(s - t);
+    epsa = (fabs(s) + eps) * pow(eps, 0.33333);
+    if (z > epsa) istop = 9;
+
+    double tau = 0, taul = 0, taul2 = 0, gmin = 0, beta = 0, betal = 0, betan = beta1, dbar = 0, dltan = 0, eta = 0, etal = 0,
+           etal2 = 0, vepln = 0, veplnl = 0, veplnl2 = 0, eplnn = 0, epln = 0, gama = 0, gamal = 0, gamal2 = 0, gamal3 = 0,
+           phi = beta1, cr1 = -1, sr1 = 0, cr2 = -1, sr2 = 0, cs = -1, sn = 0, ul4 = 0, ul3 = 0, ul2 = 0, ul = 0, u = 0;
+    double gamal_QLP = 0, vepln_QLP = 0, gama_QLP = 0, ul_QLP = 0, u_QLP = 0;
+    double alfa, dlta, dlta_QLP, dlta_tmp, gbar, gama_tmp, gamal_tmp, xnorm_tmp, xnorml, abs_gama, Anorml, Acondl, rnorml,
+        rootl, Arnorml = 0, epsx;
+    double rnorm = betan;
+    double relres = rnorm / (Anorm * xnorm + beta1);
+    for (int i = 0; i < n; i++) { r2[i] = b[i]; w[i] = 0.0; wl[i] = 0.0; }
+
+    while (istop <= 0) {
+        itn++;
+        betal = beta;
+        beta = betan;
+        s = one / beta;
+        for (int i = 0; i < n; i++) v[i] = s * y[i];
+        aprod(ctx, n, v, y);
+        if (shift != zero) for (int i = 0; i < n; i++) y[i] = y[i] - shift * v[i];
+        if (itn >= 2) { double c = -beta / betal; for (int i = 0; i < n; i++) y[i] = y[i] + c * r1[i]; }
+        alfa = dotp(n, v, y);
+        { double c = -alfa / beta; for (int i = 0; i < n; i++) y[i] = y[i] + c * r2[i]; }
+        memcpy(r1, r2, 8 * (size_t)n);
+        memcpy(r2, y, 8 * (size_t)n);
+        if (!precon) betan = nrm2s(n, y);
+        else {
+            msolve(ctx, n, r2, y);
+            betan = dotp(n, r2, y);
+            if (betan > zero) betan = sqrt(betan);
+            else if (nrm2s(n, y) > eps) { istop = 11; break; }
+        }
+        pnorm = itn == 1 ? nrm2v(2, alfa, betan, 0) : nrm2v(3, beta, alfa, betan);
+
+        dbar = dltan;
+        dlta = cs * dbar + sn * alfa;
+        epln = eplnn;
+        gbar = sn * dbar - cs * alfa;
+        eplnn = sn * betan;
+        dltan = -cs * betan;
+        dlta_QLP = dlta;
+
+        gamal3 = gamal2;
+        gamal2 = gamal;
+        gamal = gama;
+        symortho(gbar, betan, &cs, &sn, &gama);
+        gama_tmp = gama;
+        taul2 = taul;
+        taul = tau;
+        tau = cs * phi;
+        phi = sn * phi;
+        Axnorm = sqrt(Axnorm * Axnorm + tau * tau);
+
+        if (itn > 2) {
+            veplnl2 = veplnl;
+            etal2 = etal;
+            etal = eta;
+            dlta_tmp = sr2 * vepln - cr2 * dlta;
+            veplnl = cr2 * vepln + sr2 * dlta;
+            dlta = dlta_tmp;
+            eta = sr2 * gama;
+            gama = -cr2 * gama;
+        }
+        if (itn > 1) {
+            symortho(gamal, dlta, &cr1, &sr1, &gamal_tmp);
+            gamal = gamal_tmp;
+            vepln = sr1 * gama;
+            gama = -cr1 * gama;
+        }
+
+        xnorml = xnorm;
+        ul4 = ul3;
+        ul3 = ul2;
+        if (itn > 2) ul2 = (taul2 - etal2 * ul4 - veplnl2 * ul3) / gamal2;
+        if (itn > 1) ul = (taul - etal * ul3 - veplnl * ul2) / gamal;
+        xnorm_tmp = nrm2v(3, xl2norm, ul2, ul);
+        if (fabs(gama) > eps) {
+            u = (tau - eta * ul2 - vepln * ul) / gama;
+            int likeLS = relAresl < relresl;
+            if (likeLS && nrm2v(2, xnorm_tmp, u, 0) > maxxnorm_) { u = zero; istop = 12; }
+        } else {
+            u = zero;
+            istop = 14;
+        }
+        xl2norm = nrm2v(2, xl2norm, ul2, 0);
+        xnorm = nrm2v(3, xl2norm, ul, u);
+
+        if (Acond < trancond_ && istop == 0 && QLPiter == 0) {    /* MINRES updates */
+            memcpy(wl2, wl, 8 * (size_t)n);
+            memcpy(wl, w, 8 * (size_t)n);
+            if (gama_tmp > eps) {
+                s = one / gama_tmp;
+                for (int i = 0; i < n; i++) w[i] = (v[i] - epln * wl2[i] - dlta_QLP * wl[i]) * s;
+            }
+            if (xnorm < maxxnorm_) {
+                for (int i = 0; i < n; i++) x[i] = x[i] + tau * w[i];
+            } else {
+                istop = 12;
+                lastiter = 1;
+            }
+        } else {                                                     /* MINRES-QLP updates */
+            QLPiter++;
+            if (QLPiter == 1) {
+                for (int i = 0; i < n; i++) xl2[i] = 0.0;
+                if (itn > 1) {
+                    if (itn > 3) for (int i = 0; i < n; i++) wl2[i] = gamal3 * wl2[i] + veplnl2 * wl[i] + etal * w[i];
+                    if (itn > 2) for (int i = 0; i < n; i++) wl[i] = gamal_QLP * wl[i] + vepln_QLP * w[i];
+                    for (int i = 0; i < n; i++) w[i] = gama_QLP * w[i];
+                    for (int i = 0; i < n; i++) xl2[i] = x[i] - ul_QLP * wl[i] - u_QLP * w[i];
+                }
+            }
+            if (itn == 1) {
+                for (int i = 0; i < n; i++) { wl2[i] = wl[i]; wl[i] = sr1 * v[i]; w[i] = -cr1 * v[i]; }
+            } else if (itn == 2) {
+                for (int i = 0; i < n; i++) {
+                    double wo = w[i];
+                    wl2[i] = wl[i];
+                    wl[i] = cr1 * wo + sr1 * v[i];
+                    w[i] = sr1 * wo - cr1 * v[i];
+                }
+            } else {
+                for (int i = 0; i < n; i++) {
+                    double a2 = wl[i], a1 = w[i], vv;
+                    /* wl2 = wl; wl = w; w = sr2*wl2 - cr2*v; wl2 = cr2*wl2 + sr2*v;
+                       v = cr1*wl + sr1*w; w = sr1*wl - cr1*w; wl = v */
+                    double nw = sr2 * a2 - cr2 * v[i];
+                    double nwl2 = cr2 * a2 + sr2 * v[i];
+                    vv = cr1 * a1 + sr1 * nw;
+                    w[i] = sr1 * a1 - cr1 * nw;
+                    wl2[i] = nwl2;
+                    v[i] = vv;
+                    wl[i] = vv;
+                }
+            }
+            for (int i = 0; i < n; i++) {
+                xl2[i] = xl2[i] + ul2 * wl2[i];
+                x[i] = xl2[i] + ul * wl[i] + u * w[i];
+            }
+        }
+
+        gamal_tmp = gamal;
+        symortho(gamal_tmp, eplnn, &cr2, &sr2, &gamal);
+        gamal_QLP = gamal_tmp;
+        vepln_QLP = vepln;
+        gama_QLP = gama;
+        ul_QLP = ul;
+        u_QLP = u;
+
+        abs_gama = fabs(gama);
+        Anorml = Anorm;
+        Anorm = fmax(fmax(Anorm, pnorm), fmax(gamal, abs_gama));
+        if (itn == 1) {
+            gmin = gama;
+            gminl = gmin;
+        } else {
+            gminl2 = gminl;
+            gminl = gmin;
+            gmin = fmin(fmin(gminl2, gamal), abs_gama);
+        }
+        Acondl = Acond;
+        Acond = Anorm / gmin;
+        rnorml = rnorm;
+        relresl = relres;
+        if (istop != 14) rnorm = phi;
+        relres = rnorm / (Anorm * xnorm + beta1);
+        rootl = nrm2v(2, gbar, dltan, 0);
+        Arnorml = rnorml * rootl;
+        relAresl = rootl / Anorm;
+
+        epsx = Anorm * xnorm * eps;
+        if (istop == 0 || istop == 14) {
+            t1 = one + relres;
+            t2 = one + relAresl;
+        }
+        if (t1 <= one) istop = 5;
+        else if (t2 <= one) istop = 7;
+        else if (relres <= rtol) istop = 4;
+        else if (relAresl <= rtol) istop = 6;
+        else if (epsx >= beta1) istop = 2;
+        else if (xnorm >= maxxnorm_) istop = 12;
+        else if (Acond >= Acondlim_ || Acond >= ieps) istop = 13;
+        else if (itn >= itnlim) istop = 8;
+        else if (betan < eps) istop = 1;
+
+        if (istop != 0) {
+            if (istop == 6 || istop == 7 || istop == 12 || istop == 13) lastiter = 1;
+            if (lastiter) {
+                itn = itn - 1;
+                Acond = Acondl;
+                rnorm = rnorml;
+                relres = relresl;
+            }
+            aprod(ctx, n, x, r1);
+            for (int i = 0; i < n; i++) r1[i] = b[i] - r1[i] + shift * x[i];
+            aprod(ctx, n, r1, wl2);
+            for (int i = 0; i < n; i++) wl2[i] = wl2[i] - shift * r1[i];
+            Arnorm = nrm2s(n, wl2);
+            if (rnorm > zero && Anorm > zero) relAres = Arnorm / (Anorm * rnorm);
+            break;
+        }
+    }
+    (void)Arnorml; (void)relAres; (void)xnorml; (void)Anorml;
+    *istop_out = istop;
+    *itn_out = itn;
+    if (out6) {
+        out6[0] = rnorm; out6[1] = Arnorm; out6[2] = xnorm; out6[3] = Anorm; out6[4] = Acond; out6[5] = QLPiter;
+    }
+    free(r1); free(r2); free(v); free(w); free(wl); free(wl2); free(xl2); free(y);
+}
+
 void orc_packed_matvec(int n, const double *a, const double *x, double *y)
 {
     for (int i = 0; i < n; i++) y[i] = 0.0;

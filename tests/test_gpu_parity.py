@@ -161,6 +161,29 @@ def test_minres_sparse_constraints(orc, mbandw):
         assert np.max(np.abs(x2[:P.nvar] - xo[:P.nvar]) / sig) < 1e-4
 
 
+@pytest.mark.parametrize("mbandw,mrmode", [(-1, 0), (-1, 2), (6, 0), (6, 2)])
+def test_minresqlp_sparse_constraints(orc, mbandw, mrmode):
+    """MINRES-QLP (mresmode 0: QR steps, QLP once ill-conditioned; 2: QLP only) on the
+    group-block KKT matrix against the oracle's restatement (pinned bitwise to
+    src/minresqlpModule.f90) driven with the same matrix and preconditioner: same stop code,
+    iteration count within a few, solution within 1e-4 of the parameter errors."""
+    rp, ii, gg, cons, gam = _pixel_cons_problem()
+    P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES_QLP, mreqenf=5,
+              mcount=1, mbandw=mbandw, mrestl=1e-12, mrmode=mrmode)
+    P.loopn()
+    x, st = P.solve()
+    xr, A, b = _kkt_reference(P)
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-8
+    sig = np.sqrt(np.abs(np.diag(np.linalg.inv(A))[:P.nvar]))
+    assert np.max(np.abs(x[:P.nvar] - xr[:P.nvar]) / sig) < 1e-4
+    msolve = _oracle_band_precond(orc, A, P.nvar, P.ncon, mbandw) if mbandw > 0 else None
+    trancond = 1.0 if mrmode == 2 else 1.0e7
+    xo, so, io, _ = orc.minresqlp(P.nall, lambda v: A @ v, msolve, b, 1e-12, precon=mbandw > 0, trancond=trancond)
+    assert so == st["istop"], (so, st["istop"])
+    assert abs(io - st["itn"]) <= max(3, io // 20), (io, st["itn"])
+    assert np.max(np.abs(x[:P.nvar] - xo[:P.nvar]) / sig) < 1e-4
+
+
 def _oracle_band_precond(orc, A, n, m, mb):
     """pede's band preconditioner (vmprep india layout, constraint blocks of overlapping ranges,
     equdecs/equslvs) built by the oracle for the dense KKT matrix A."""

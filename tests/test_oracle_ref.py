@@ -199,6 +199,66 @@ def test_minres_kkt_indefinite(orc, ref):
     assert np.allclose(x1, np.linalg.solve(K, b), rtol=1e-7, atol=1e-9)
 
 
+def _qlp_case(kind, rng):
+    if kind == "spd":
+        n = 70
+        A = spd(rng, n, 1e5)
+    elif kind == "kkt":
+        n0, m = 40, 5
+        A0 = spd(rng, n0, 100.0)
+        Cm = rng.normal(size=(m, n0))
+        A = np.block([[A0, Cm.T], [Cm, np.zeros((m, m))]])
+        n = n0 + m
+    else:                                       # singular, consistent right-hand side
+        n = 50
+        X = rng.normal(size=(n, n - 3))
+        A = X @ X.T
+    b = A @ rng.normal(size=n) if kind == "singular" else rng.normal(size=n)
+    return n, A, b
+
+
+@pytest.mark.parametrize("kind", ["spd", "kkt", "singular"])
+@pytest.mark.parametrize("trancond", [1.0e7, 1.0, 1.0e2])
+@pytest.mark.parametrize("precon", [False, True])
+def test_minresqlp_matches_reference(orc, ref, kind, trancond, precon):
+    """MINRES-QLP (src/minresqlpModule.f90:411-1330) in the oracle against the reference's own
+    routine: trancond 1e7 is pede's default (QR, QLP once ill-conditioned), 1 forces QLP from
+    the first iteration (mresmode 2), 1e2 switches mid-run."""
+    rng = np.random.default_rng(11 + len(kind))
+    n, A, b = _qlp_case(kind, rng)
+    d = np.abs(np.diag(A)) + 1.0
+    ms = (lambda x: x / d) if precon else None
+    x1, s1, i1, o1 = orc.minresqlp(n, lambda x: A @ x, ms, b, 1e-10, precon=precon, trancond=trancond)
+    x2, s2, i2, o2 = ref.minresqlp(n, lambda x: A @ x, ms, b, 1e-10, precon=precon, trancond=trancond)
+    assert (s1, i1) == (s2, i2)
+    np.testing.assert_allclose(x1, x2, rtol=1e-13, atol=1e-13 * np.abs(x2).max())
+    np.testing.assert_allclose(o1[:5], o2, rtol=1e-12)
+    # stopping rule istop 4: ||r|| <= rtol (||A|| ||x|| + ||b||) for the recurrence residual
+    assert s1 == 4
+    scale = np.linalg.norm(A, 2) * np.linalg.norm(x1) + np.linalg.norm(b)
+    assert np.linalg.norm(A @ x1 - b) <= 1e-8 * scale
+    if kind != "singular":
+        assert np.allclose(x1, np.linalg.solve(A, b), rtol=1e-4, atol=1e-5 * np.abs(x1).max())
+    elif not precon:
+        # minimum-length solution of a consistent singular system: x in range(A)
+        xp = np.linalg.pinv(A) @ b
+        assert np.allclose(x1, xp, rtol=1e-5, atol=1e-6 * np.abs(xp).max())
+
+
+def test_minresqlp_early_exits(orc, ref):
+    """b = 0 (istop 3) and an unsymmetric operator (istop 9) stop before the first iteration."""
+    rng = np.random.default_rng(3)
+    A = spd(rng, 10)
+    x1, s1, i1, _ = orc.minresqlp(10, lambda x: A @ x, None, np.zeros(10), 1e-10, precon=False)
+    x2, s2, i2, _ = ref.minresqlp(10, lambda x: A @ x, None, np.zeros(10), 1e-10, precon=False)
+    assert (s1, i1) == (s2, i2) == (3, 0) and not x1.any() and not x2.any()
+    U = A + np.triu(rng.normal(size=(10, 10)), 1)
+    b = rng.normal(size=10)
+    x1, s1, i1, _ = orc.minresqlp(10, lambda x: U @ x, None, b, 1e-10, precon=False)
+    x2, s2, i2, _ = ref.minresqlp(10, lambda x: U @ x, None, b, 1e-10, precon=False)
+    assert (s1, i1) == (s2, i2) == (9, 0)
+
+
 def test_packed_matvec(orc):
     rng = np.random.default_rng(8)
     A = spd(rng, 25)
