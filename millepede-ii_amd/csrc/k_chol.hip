@@ -260,6 +260,43 @@ void chol_solve32(int n, const float *a, int lda, double *x, float *xf, hipStrea
     chol_solve_t<float, double>(n, a, lda, x, s);
 }
 
+// A += rho C^T C over the full column-major matrix (both triangles); one block per constraint
+__global__ void k_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
+                          double *a, int lda)
+{
+    const int j = blockIdx.x;
+    const int64_t q0 = cptr[j];
+    const int64_t len = cptr[j + 1] - q0;
+    for (int64_t t = threadIdx.x; t < len * len; t += blockDim.x) {
+        const int64_t p = t / len, q = t % len;
+        const int64_t row = cidx[q0 + p] - 1, col = cidx[q0 + q] - 1;
+        atomicAdd(&a[col * lda + row], rho * cval[q0 + p] * cval[q0 + q]);
+    }
+}
+// y += rho C^T (C x) for the parameter block; one block per constraint
+__global__ void k_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
+                            const double *x, double *y)
+{
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    double t = 0.0;
+    for (int64_t q = cptr[j] + threadIdx.x; q < cptr[j + 1]; q += blockDim.x) t += cval[q] * x[cidx[q] - 1];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double cx = red[0];
+    for (int64_t q = cptr[j] + threadIdx.x; q < cptr[j + 1]; q += blockDim.x) atomicAdd(&y[cidx[q] - 1], rho * cval[q] * cx);
+}
+void l_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, double *a,
+               int lda, hipStream_t s)
+{ if (ncon > 0) hipLaunchKernelGGL(k_add_cct, dim3(ncon), dim3(256), 0, s, ncon, cptr, cidx, cval, rho, a, lda); }
+void l_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
+                 const double *x, double *y, hipStream_t s)
+{ if (ncon > 0) hipLaunchKernelGGL(k_cct_apply, dim3(ncon), dim3(256), 0, s, ncon, cptr, cidx, cval, rho, x, y); }
+
 void l_packed_to_full(int n, const double *packed, double *full, int lda, hipStream_t s)
 {
     int64_t tot = (int64_t)n * (n + 1) / 2;

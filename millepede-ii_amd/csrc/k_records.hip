@@ -1,7 +1,8 @@
 // Record-loop kernel: one 64-lane wavefront per Mille record.
 //
 // Restates loopbf (src/pede.f90:3387-4442) for records whose local fit takes the full
-// (non-banded) path: residuals after the current global parameters, local normal equations,
+// (non-banded) path; records with bordered-band structure go to k_records_band.  Residuals
+// after the current global parameters, local normal equations,
 // symmetric Gauss-Jordan inversion with diagonal pivoting (sqminv, src/mpnum.f90:97-213),
 // chi^2 / down-weighting / reject logic, then the "Millepede" update of the global system:
 //   globalVector += dw * w * r * g                       (src/pede.f90:4184-4186)
@@ -103,11 +104,11 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     int32_t *seggr = segof + P.max_var;      // [max_var] group of each segment
     int32_t *pblk = seggr + P.max_var;       // [max_pair] block id per segment pair
     int32_t *nxt = pblk + P.max_pair;        // [nalc_max+1] pivot list
-    int32_t *ibh = nxt + P.nalc_max + 1;     // [2*nalc_max+2] band-width histogram
-    int32_t *sh = ibh + 2 * P.nalc_max + 2;  // scalars
+    int32_t *sh = nxt + P.nalc_max + 1;      // scalars
 
     for (int64_t ri = rec_begin + blockIdx.x; ri < rec_end; ri += gridDim.x) {
         const int64_t r = R.order ? R.order[ri] : ri;
+        if (R.band && R.band[r] != 0) continue;   // bordered-band local fit: k_records_band
         const int64_t e0 = R.rec_eq[r];
         const int neq = (int)(R.rec_eq[r + 1] - e0);
         const int64_t g0 = R.rec_gl[r];
@@ -167,54 +168,6 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             continue;
         }
         const int npk = (nalc * nalc + nalc) / 2;
-        // ---- bordered-band structure check (3815-3874), first iteration only ----
-        bool band = false;
-        if (nalc > 5 && o.lfitbb > 0) {
-            for (int i = lane; i < 2 * nalc + 2; i += WAVE) ibh[i] = 0;
-            __syncthreads();
-            for (int i = lane; i < neq; i += WAVE) {
-                int64_t sl = R.eq_sl[e0 + i];
-                int nl = R.eq_n[e0 + i] & 0xffff;
-                for (int j = 1; j <= nl; j++) {
-                    int ij = R.widx[sl + j];
-                    for (int k = 1; k <= j; k++) {
-                        int ik = R.widx[sl + k];
-                        int id = abs(ij - ik) + 1;
-                        atomicMax(&ibh[id], min(ij, ik));
-                        atomicMax(&ibh[nalc + id], min(nalc + 1 - ij, nalc + 1 - ik));
-                    }
-                }
-            }
-            __syncthreads();
-            if (lane == 0) {
-                int kx = -1;
-                long long icmn = (long long)nalc * nalc * nalc;
-                int kbdr = 0;
-                for (int k = nalc; k >= 2; k--) {
-                    kbdr = max(kbdr, ibh[k]);
-                    long long t = (k - 2) + kbdr + 1;
-                    long long icost = 6LL * (nalc - kbdr) * t * t + 2LL * kbdr * kbdr * kbdr;
-                    if (icost < icmn) { icmn = icost; kx = k; }
-                }
-                if (kx < 0) {
-                    kbdr = 0;
-                    for (int k = nalc; k >= 2; k--) {
-                        kbdr = max(kbdr, ibh[k + nalc]);
-                        long long t = (k - 2) + kbdr + 1;
-                        long long icost = 6LL * (nalc - kbdr) * t * t + 2LL * kbdr * kbdr * kbdr;
-                        if (icost < icmn) { icmn = icost; kx = k; }
-                    }
-                }
-                sh[0] = kx > 0 ? 1 : 0;
-            }
-            __syncthreads();
-            band = sh[0] != 0;
-        }
-        if (band) {
-            if (lane == 0) { out.status[r] = 8; out.chi2[r] = 0.0; out.ndf[r] = 0; }
-            __syncthreads();
-            continue;
-        }
         // ---- outlier-suppression iterations (3759-4094) ----
         const int nter = (o.nloopn != 1 && o.lhuber != 0) ? o.lhuber : 1;
         int status = 0, nrank = 0, ndf = 0;
@@ -561,7 +514,7 @@ int lds_bytes(LdsPlan &P)
 {
     int npk = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
     size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max + 2 * (size_t)P.max_var * P.nalc_max;
-    size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + P.max_pair + (P.nalc_max + 1) + (2 * P.nalc_max + 2) + 8;
+    size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + P.max_pair + (P.nalc_max + 1) + 8;
     size_t b = d * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;

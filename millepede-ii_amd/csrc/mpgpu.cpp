@@ -67,6 +67,14 @@ struct mpgpu_ctx {
     DBuf<int32_t> d_cidx;
     DBuf<double> d_cval;
     LdsPlan plan{};
+    // bordered-band local fits
+    DBuf<int32_t> d_band_info;
+    DBuf<int64_t> d_band_ids;
+    DBuf<double> d_band_scratch;
+    BandPlan bplan{};
+    int64_t nband = 0;
+    int band_grid = 0;
+    int band_lfitbb = -1;
     int64_t nrec = 0;
     int64_t max_pair = 0;
 
@@ -317,6 +325,8 @@ inline bool decode_record(const int32_t *I, const float *F, const double *D, int
         }
         while (last < end && idx(last + 1) != 0) last++;
         int64_t nl = sG - sL - 1, ng = last - sG;
+        for (int64_t k = sL + 1; k < sG; k++)
+            if (idx(k) < 1) return false;          // local parameter indices are 1-based
         if (nl > 0xffff || ng > 0xffff) return false;
         sl.push_back(w0 + sL - 1);
         nn.push_back((uint32_t)nl | ((uint32_t)ng << 16));
@@ -325,6 +335,42 @@ inline bool decode_record(const int32_t *I, const float *F, const double *D, int
 }
 
 } // namespace
+
+// Bordered-band structure of one record's local normal matrix: the band-width histogram and
+// cost model of loopbf (src/pede.f90:3815-3874).  Returns 0 for the full (sqminv) path, else
+// (mbnd+1) | mbdr << 10 | mside << 20.
+static int32_t band_info(const int32_t *inder, const int64_t *eq_sl, const uint32_t *eq_n, int64_t e0, int64_t e1,
+                         int nalc, std::vector<int> &ibh)
+{
+    ibh.assign(2 * (size_t)nalc + 2, 0);
+    for (int64_t e = e0; e < e1; e++) {
+        const int nl = eq_n[e] & 0xffff;
+        const int64_t sl = eq_sl[e];
+        for (int j = 1; j <= nl; j++) {
+            const int ij = inder[sl + j];
+            for (int k = 1; k <= j; k++) {
+                const int ik = inder[sl + k];
+                const int id = std::abs(ij - ik) + 1;
+                ibh[id] = std::max(ibh[id], std::min(ij, ik));
+                ibh[nalc + id] = std::max(ibh[nalc + id], std::min(nalc + 1 - ij, nalc + 1 - ik));
+            }
+        }
+    }
+    int kx = -1, kbdrx = 0, mside = 0;
+    int64_t icmn = (int64_t)nalc * nalc * nalc;
+    for (int side = 1; side <= 2 && kx < 0; side++) {
+        int kbdr = 0;
+        for (int k = nalc; k >= 2; k--) {
+            const int kbnd = k - 2;
+            kbdr = std::max(kbdr, ibh[side == 1 ? k : k + nalc]);
+            const int64_t t = kbnd + kbdr + 1;
+            const int64_t icost = 6 * (int64_t)(nalc - kbdr) * t * t + 2 * (int64_t)kbdr * kbdr * kbdr;
+            if (icost < icmn) { icmn = icost; kx = k; kbdrx = kbdr; mside = side; }
+        }
+    }
+    if (kx <= 0) return 0;
+    return (kx - 1) | (kbdrx << 10) | (mside << 20);
+}
 
 extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *nvar_out, int32_t *ncon_out)
 {
@@ -597,8 +643,11 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     // ---- matrix structure ----
     const bool sparse = c->opt.matsto == MPGPU_SPARSE;
     int32_t max_gl = 1, max_eq = 1, max_nalc = 1, max_var = 1, max_seg = 1;
+    int32_t bmax_gl = 1, bmax_eq = 1, bmax_nalc = 1, bmax_var = 1, bmax_seg = 1, bmax_mp1 = 1, bmax_bdr = 0, bmax_sup = 1;
     std::vector<int64_t> rec_pb(nrec + 1, 0);
     std::vector<int32_t> pair_blk;
+    std::vector<int32_t> binfo(nrec, 0);
+    const int lfitbb = c->opt.lfitbb;
     {
         // per-record maxima and (sparse) group pairs
         std::vector<int32_t> nseg_rec(nrec, 0);
@@ -606,17 +655,19 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         const int64_t npairs = ng * (ng + 1) / 2;
         std::vector<std::atomic<uint64_t>> pbits(sparse ? (npairs + 63) / 64 : 0);
         for (auto &b : pbits) b.store(0, std::memory_order_relaxed);
-#pragma omp parallel for schedule(dynamic, 256) reduction(max : max_gl, max_eq, max_nalc, max_var, max_seg)
+#pragma omp parallel for schedule(dynamic, 256) reduction(max : max_gl, max_eq, max_nalc, max_var, max_seg, bmax_gl, \
+    bmax_eq, bmax_nalc, bmax_var, bmax_seg, bmax_mp1, bmax_bdr, bmax_sup)
         for (int64_t r = 0; r < nrec; r++) {
+            thread_local std::vector<int> ibh;
             int32_t ngl = ngl_rec[r];
-            max_gl = std::max(max_gl, ngl);
-            max_eq = std::max(max_eq, (int32_t)(rec_eq[r + 1] - rec_eq[r]));
+            const int32_t neq_r = (int32_t)(rec_eq[r + 1] - rec_eq[r]);
             int nalc = 0;
+            int64_t sup = 0;
             for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
                 int nl = eq_n[e] & 0xffff;
                 for (int j = 1; j <= nl; j++) nalc = std::max(nalc, c->inder[eq_sl[e] + j]);
+                sup += (int64_t)nl * (eq_n[e] >> 16);
             }
-            max_nalc = std::max(max_nalc, nalc);
             int nv_r = 0, nseg = 0, lastg = -1;
             for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
                 int v = c->var_index[gl_tix[q]];
@@ -626,7 +677,25 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
                     if (g != lastg) { nseg++; lastg = g; }
                 }
             }
-            max_var = std::max(max_var, nv_r);
+            // bordered-band local fit (decided once, as pede does in the first iteration)
+            int32_t bi = 0;
+            if (nalc > 5 && lfitbb > 0) bi = band_info(c->inder.data(), eq_sl.data(), eq_n.data(), rec_eq[r], rec_eq[r + 1], nalc, ibh);
+            binfo[r] = bi;
+            if (bi) {
+                bmax_gl = std::max(bmax_gl, ngl);
+                bmax_eq = std::max(bmax_eq, neq_r);
+                bmax_nalc = std::max(bmax_nalc, nalc);
+                bmax_var = std::max(bmax_var, nv_r);
+                bmax_seg = std::max(bmax_seg, nseg);
+                bmax_mp1 = std::max(bmax_mp1, bi & 0x3ff);
+                bmax_bdr = std::max(bmax_bdr, (bi >> 10) & 0x3ff);
+                bmax_sup = std::max(bmax_sup, (int32_t)std::min<int64_t>(sup, INT32_MAX));
+            } else {
+                max_gl = std::max(max_gl, ngl);
+                max_eq = std::max(max_eq, neq_r);
+                max_nalc = std::max(max_nalc, nalc);
+                max_var = std::max(max_var, nv_r);
+            }
             max_seg = std::max(max_seg, nseg);
             nseg_rec[r] = nseg;
             if (sparse) {
@@ -723,6 +792,30 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     lds_bytes(c->plan);
     if (c->plan.bytes > 160 * 1024)
         throw MpError(30, "record too large for the record kernel (LDS " + std::to_string(c->plan.bytes) + " bytes)");
+    // bordered-band records: their own kernel, LDS plan and per-workgroup inverse scratch
+    std::vector<int64_t> band_ids;
+    c->nband = 0;
+    for (int64_t r = 0; r < nrec; r++) if (binfo[r]) c->nband++;
+    c->band_lfitbb = lfitbb;
+    if (c->nband > 0) {
+        if (bmax_nalc > 256) throw MpError(30, "bordered-band local fits support at most 256 local parameters");
+        if (bmax_bdr > 16) throw MpError(30, "bordered-band local fits support at most 16 border rows");
+        BandPlan &B = c->bplan;
+        B.max_gl = bmax_gl;
+        B.max_eq = bmax_eq;
+        B.nalc_max = bmax_nalc;
+        B.max_var = bmax_var;
+        B.max_pair = sparse ? (int32_t)((int64_t)bmax_seg * (bmax_seg + 1) / 2) : 1;
+        B.mp1_max = bmax_mp1;
+        B.nbdr_max = std::max(1, bmax_bdr);
+        B.max_sup = bmax_sup;
+        B.vinv_stride = (int64_t)bmax_nalc * (bmax_nalc + 1) / 2;
+        band_lds_bytes(B);
+        if (B.bytes > 160 * 1024)
+            throw MpError(30, "record too large for the bordered-band kernel (LDS " + std::to_string(B.bytes) + " bytes)");
+        const int per_cu = std::max(1, std::min(8, (160 * 1024) / B.bytes));
+        c->band_grid = (int)std::min<int64_t>(c->nband, 256 * per_cu);
+    }
 
     // ---- processing order: records sorted by the first parameter group they touch, so that the
     // records in flight at the same time update a compact set of matrix blocks (cache reuse of
@@ -742,6 +835,18 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     // ---- device upload ----
     hipStream_t s = c->stream;
     c->d_order.upload(order, s);
+    if (c->nband > 0) {
+        std::vector<int64_t> ids;
+        ids.reserve(c->nband);
+        for (int64_t q = 0; q < nrec; q++) if (binfo[order[q]]) ids.push_back(order[q]);
+        c->d_band_ids.upload(ids, s);
+        c->d_band_info.upload(binfo, s);
+        c->d_band_scratch.alloc((size_t)c->band_grid * c->bplan.vinv_stride);
+    } else {
+        c->d_band_ids.release();
+        c->d_band_info.release();
+        c->d_band_scratch.release();
+    }
     c->d_rec_eq.upload(rec_eq, s);
     c->d_rec_gl.upload(rec_gl, s);
     c->d_eq_sl.upload(eq_sl, s);
@@ -860,6 +965,7 @@ static DevRecords records_of(mpgpu_ctx *c)
     R.pair_blk = c->d_pair_blk.p;
     R.rec_w = nullptr;
     R.order = c->d_order.p;
+    R.band = c->nband > 0 ? c->d_band_info.p : nullptr;
     R.max_gl = c->plan.max_gl;
     R.max_eq = c->plan.max_eq;
     R.max_nalc = c->plan.nalc_max;
@@ -885,6 +991,8 @@ static void device_mupdat(mpgpu_ctx *c, const std::vector<int32_t> &ii, const st
 extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float chicut, mpgpu_pass_stats *st)
 {
     if (!c || !c->built) return fail(c, 30, "structure not built");
+    if (c->opt.lfitbb != c->band_lfitbb)
+        return fail(c, 30, "the local-fit band option changed after the structure was built");
     API_TRY
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -911,6 +1019,9 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     int grid = (int)std::min<int64_t>(c->nrec, 256 * 40);
     HIPCHK(hipEventRecord(e1, s));
     if (c->nrec > 0) HIPCHK(launch_records(R, M, o, out, c->plan, 0, c->nrec, std::max(grid, 1), s));
+    if (c->nband > 0)
+        HIPCHK(launch_records_band(R, M, o, out, c->bplan, c->d_band_ids.p, c->nband, c->d_band_info.p,
+                                   c->d_band_scratch.p, c->band_grid, s));
     HIPCHK(hipEventRecord(e2, s));
     // pass statistics
     c->d_ndfsum.zero(s);
@@ -924,9 +1035,7 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     c->d_hist.download(hist, 24, s);
     HIPCHK(hipStreamSynchronize(s));
     if (hist[9] > 0) throw MpError(30, "record exceeds record-kernel limits (" + std::to_string(hist[9]) + " records)");
-    if (hist[8] > 0)
-        throw MpError(30, "local fits with bordered-band structure are not provided by this kernel (" +
-                              std::to_string(hist[8]) + " records)");
+    if (hist[8] > 0) throw MpError(30, "records left without a local fit (" + std::to_string(hist[8]) + ")");
     // pre-sigma diagonal (icalcm=1), regularisation and external measurements (loopn 2711-2798)
     std::vector<int32_t> ii, jj;
     std::vector<double> vv;
@@ -1601,6 +1710,26 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
         info.alloc(4);
         // parameter block of the KKT matrix (leading nv x nv of the packed matrix)
         l_packed_to_full(nv, c->d_mat.p, A.p, (int)lda, s);
+        // With constraints the parameter block alone may be singular (modes fixed only by the
+        // constraints, as pede's decomposition removes them by elimination).  The block is
+        // augmented to A + rho C^T C and the right-hand side to b + rho C^T d: on the constraint
+        // surface C x = d this changes neither x nor the multipliers, and the augmented block is
+        // positive definite whenever the constrained problem is well posed.
+        double rho = 0.0;
+        if (m > 0) {
+            DBuf<double> dg;
+            dg.alloc(nv);
+            l_get_diag(model_of(c), nv, dg.p, s);
+            std::vector<double> hd(nv), cc(nv, 0.0);
+            dg.download(hd.data(), nv, s);
+            HIPCHK(hipStreamSynchronize(s));
+            for (int j = 0; j < m; j++)
+                for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) cc[c->cidx[q] - 1] += c->cval[q] * c->cval[q];
+            double amax = 0.0, cmax = 0.0;
+            for (int i = 0; i < nv; i++) { amax = std::max(amax, std::fabs(hd[i])); cmax = std::max(cmax, cc[i]); }
+            rho = cmax > 0.0 ? (amax > 0.0 ? amax : 1.0) / cmax : 0.0;
+            l_add_cct(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, A.p, (int)lda, s);
+        }
         const bool fp32 = c->opt.metsol == MPGPU_CHOLESKY_FP32_IR;
         DBuf<float> A32;
         DBuf<float> xf;
@@ -1623,8 +1752,9 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
                 l_copy(nv, r.p, d.p, s);
                 chol_solve32(nv, A32.p, (int)lda, d.p, xf.p, s);
                 l_axpy_neg(nv, -1.0, d.p, xacc.p, s); // xacc += d
-                // r = rhs - A_pp xacc, with the FP64 packed matrix restricted to the parameter block
+                // r = rhs - (A_pp + rho C^T C) xacc with the FP64 packed matrix (parameter block)
                 l_packed_symv(nv, c->d_mat.p, xacc.p, ax.p, s);
+                l_cct_apply(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, xacc.p, ax.p, s);
                 l_sub(nv, v, ax.p, r.p, s);
                 double rr = dot_dev(c, nv, r.p, r.p), bb = dot_dev(c, nv, v, v);
                 loc.itn = it + 1;
@@ -1651,7 +1781,10 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
             std::vector<double> bp(n);
             b.download(bp.data(), n, s);
             HIPCHK(hipStreamSynchronize(s));
-            zc.upload(bp.data(), nv, s);
+            std::vector<double> bt(bp.begin(), bp.begin() + nv);     // b + rho C^T d
+            for (int j = 0; j < m; j++)
+                for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) bt[c->cidx[q] - 1] += rho * c->cval[q] * bp[nv + j];
+            zc.upload(bt.data(), nv, s);
             solveA(zc.p);
             std::vector<double> y(nv);
             zc.download(y.data(), nv, s);

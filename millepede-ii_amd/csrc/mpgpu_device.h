@@ -24,6 +24,8 @@ struct DevRecords {
     const int32_t *pair_blk; // block id per (segment_i >= segment_j) pair of a record
     const float *rec_w;      // per-record (file) weight, nullptr = 1
     const int64_t *order;    // processing order of the records (nullptr = file order)
+    const int32_t *band;     // per record: 0 full local fit, else bordered-band structure
+                             // (mbnd+1) | mbdr << 10 | mside << 20 (record kernel skips these)
     int32_t max_gl, max_eq, max_nalc, max_var, max_seg;
 };
 
@@ -59,6 +61,13 @@ struct RecOut {
 // LDS plan of the record kernel (all counts in elements)
 struct LdsPlan {
     int32_t max_gl, max_eq, nalc_max, max_var, max_pair;
+    int32_t bytes;
+};
+
+// LDS plan of the bordered-band record kernel
+struct BandPlan {
+    int32_t max_gl, max_eq, nalc_max, max_var, max_pair, mp1_max, nbdr_max, max_sup;
+    int64_t vinv_stride;     // doubles of global scratch per workgroup (packed local inverse)
     int32_t bytes;
 };
 

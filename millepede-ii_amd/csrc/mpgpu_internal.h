@@ -70,6 +70,10 @@ struct EqBlock {
 hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out, const LdsPlan &P,
                           int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s);
 int lds_bytes(LdsPlan &P);
+hipError_t launch_records_band(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out,
+                               const BandPlan &P, const int64_t *ids, int64_t nb, const int32_t *binfo,
+                               double *scratch, int grid, hipStream_t s);
+int band_lds_bytes(BandPlan &P);
 void l_mupdat(const DevModel &M, int32_t n, const int32_t *i, const int32_t *j, const double *v, hipStream_t s);
 void l_get_diag(const DevModel &M, int32_t n, double *d, hipStream_t s);
 void l_get_band(const DevModel &M, int32_t n, int32_t mbandw, const int32_t *india, double *c, hipStream_t s);
@@ -118,5 +122,9 @@ void chol_solve(int n, const double *a, int lda, double *x, hipStream_t s);
 void chol_solve32(int n, const float *a, int lda, double *x, float *xf, hipStream_t s);
 void l_packed_to_full(int n, const double *packed, double *full, int lda, hipStream_t s);
 void l_full_to_f32(int64_t nn, const double *a, float *b, hipStream_t s);
+void l_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, double *a,
+               int lda, hipStream_t s);
+void l_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
+                 const double *x, double *y, hipStream_t s);
 
 } // namespace mpgpu

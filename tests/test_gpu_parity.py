@@ -88,6 +88,35 @@ def test_downweighting_pass(orc):
         _compare_pass(P, orc, rp, ii, gg, lab, var, nloopn=nloopn, lhuber=3, chicut=chicut, dwcut=0.2)
 
 
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_brokenlines_band_pass(orc, matsto):
+    """Records whose local fits take pede's bordered-band path (sqmibb, band 3 + border 4):
+    the GPU band kernel's matrix, vector, chi^2, ndf, reject counts and counters against the
+    oracle, which follows sqmibb step by step."""
+    rp, ii, gg = mpgpu.brokenlines_records(31, 30, 4, 3, 1500, 8, 30)
+    P = _pede(rp, ii, gg, matsto=matsto, mreqenf=1, mcount=1)
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=1, mcount=1)
+    s_gpu, mat, vec = _compare_pass(P, orc, rp, ii, gg, lab, var)
+    assert s_gpu["naccept"] > 1400
+
+
+def test_brokenlines_band_downweighting(orc):
+    """Band-path records with outliers: down-weighting iterations (lhuber 3), chi^2 cut and
+    parameters away from zero, against the oracle."""
+    rp, ii, gg = mpgpu.brokenlines_records(32, 24, 3, 3, 1200, 8, 24)
+    gg = gg.copy()
+    recs = LB.decode(rp, ii, gg)
+    for r in range(0, 1200, 13):
+        gg[recs[r][min(5, len(recs[r]) - 1)][0]] += 0.05
+    P = _pede(rp, ii, gg, matsto=mpgpu.SPARSE, mreqenf=1, mcount=1, lhuber=3, dwcut=0.2)
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=1, mcount=1)
+    par = np.zeros(P.ntot)
+    par[::5] = 2e-4
+    P.set_parameters(par)
+    for nloopn, chicut in ((1, 0.0), (2, 60.0), (3, 25.0)):
+        _compare_pass(P, orc, rp, ii, gg, lab, var, nloopn=nloopn, lhuber=3, chicut=chicut, dwcut=0.2)
+
+
 def test_inversion_matches_sqminl(orc):
     rp, ii, gg, truth = mpgpu.toy_records(42, 10000)
     pres = [-1.0] * len(TOY_FIXED)
@@ -149,16 +178,17 @@ def test_minres_sparse_constraints(orc, mbandw):
         assert abs(io - st["itn"]) <= max(3, io // 20), (io, st["itn"])
         assert np.max(np.abs(x[:P.nvar] - xo[:P.nvar]) / sig) < 1e-4
     if mbandw == 0:
-        # the same matrix through the oracle's MINRES restatement with the same stop rule
-        d = np.diag(A).copy()
-        xo, so, io, _ = orc.minres(P.nall, lambda v: A @ v, None, b, 1e-12, precon=False)
+        # Without preconditioner this KKT system (condition ~1e15) does not converge, and
+        # iterates of runs that differ only in rounding drift apart after some 50 iterations.
+        # The recurrences are compared instead on the first 30 iterations: GPU iterate against
+        # the oracle's MINRES restatement on the same matrix.
+        xo, so, io, _ = orc.minres(P.nall, lambda v: A @ v, None, b, 1e-12, itnlim=30, precon=False)
         P2 = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES,
-                   mreqenf=5, mcount=1, mbandw=-1, mrestl=1e-12)
+                   mreqenf=5, mcount=1, mbandw=-1, mrestl=1e-12, itnlim=30)
         P2.loopn()
         x2, st2 = P2.solve()
-        assert st2["istop"] == so
-        assert abs(st2["itn"] - io) <= max(3, io // 50)
-        assert np.max(np.abs(x2[:P.nvar] - xo[:P.nvar]) / sig) < 1e-4
+        assert (st2["istop"], st2["itn"]) == (so, io) == (5, 30)
+        assert np.max(np.abs(x2[:P.nvar] - xo[:P.nvar]) / sig) < 1e-6
 
 
 @pytest.mark.parametrize("mbandw,mrmode", [(-1, 0), (-1, 2), (6, 0), (6, 2)])
@@ -168,20 +198,23 @@ def test_minresqlp_sparse_constraints(orc, mbandw, mrmode):
     src/minresqlpModule.f90) driven with the same matrix and preconditioner: same stop code,
     iteration count within a few, solution within 1e-4 of the parameter errors."""
     rp, ii, gg, cons, gam = _pixel_cons_problem()
+    itnlim = 2000 if mbandw > 0 else 30      # unpreconditioned: first 30 iterations (see MINRES test)
     P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES_QLP, mreqenf=5,
-              mcount=1, mbandw=mbandw, mrestl=1e-12, mrmode=mrmode)
+              mcount=1, mbandw=mbandw, mrestl=1e-12, mrmode=mrmode, itnlim=itnlim)
     P.loopn()
     x, st = P.solve()
     xr, A, b = _kkt_reference(P)
-    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-8
     sig = np.sqrt(np.abs(np.diag(np.linalg.inv(A))[:P.nvar]))
-    assert np.max(np.abs(x[:P.nvar] - xr[:P.nvar]) / sig) < 1e-4
+    if mbandw > 0:
+        assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-8
+        assert np.max(np.abs(x[:P.nvar] - xr[:P.nvar]) / sig) < 1e-4
     msolve = _oracle_band_precond(orc, A, P.nvar, P.ncon, mbandw) if mbandw > 0 else None
     trancond = 1.0 if mrmode == 2 else 1.0e7
-    xo, so, io, _ = orc.minresqlp(P.nall, lambda v: A @ v, msolve, b, 1e-12, precon=mbandw > 0, trancond=trancond)
+    xo, so, io, _ = orc.minresqlp(P.nall, lambda v: A @ v, msolve, b, 1e-12, itnlim=itnlim, precon=mbandw > 0,
+                                  trancond=trancond)
     assert so == st["istop"], (so, st["istop"])
     assert abs(io - st["itn"]) <= max(3, io // 20), (io, st["itn"])
-    assert np.max(np.abs(x[:P.nvar] - xo[:P.nvar]) / sig) < 1e-4
+    assert np.max(np.abs(x[:P.nvar] - xo[:P.nvar]) / sig) < (1e-4 if mbandw > 0 else 1e-6)
 
 
 def _oracle_band_precond(orc, A, n, m, mb):
