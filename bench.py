@@ -118,7 +118,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=120)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy kernel three times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied 3 x 1 GiB")
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X, "
               "and the multi-GPU path (record slices + one all-reduce) is not built.", file=sys.stderr)

@@ -65,7 +65,7 @@ typedef struct mpgpu_options {
     double mrestl;           /* 'mrestol' (1e-6)                                      */
     double mrtcnd;           /* 'mrestranscond' (1e7)                                 */
     int32_t nregul;          /* regularisation flag                                   */
-    int32_t refine_steps;    /* FP32+IR: maximum FP64 refinement steps (5)            */
+    int32_t refine_steps;    /* FP32+IR: maximum FP64 CG refinement steps (20)        */
     int32_t reserved[6];
 } mpgpu_options;
 

@@ -96,13 +96,18 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Mfma;
+// Operands: lane l supplies A[row = l%16][k = l/16] and B[k = l/16][col = l%16].  Result
+// element e of lane l is C[row][col = l%16] with row = l/16 + 4e for the FP64 instruction and
+// row = 4*(l/16) + e for the FP32 one (measured on gfx950).
 template <> struct Mfma<double> {
     typedef d4 acc_t;
     static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lk, int e) { return lk + 4 * e; }
 };
 template <> struct Mfma<float> {
     typedef f4 acc_t;
     static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lk, int e) { return 4 * lk + e; }
 };
 
 template <typename T>
@@ -131,8 +136,6 @@ __global__ __launch_bounds__(256) void k_syrk_update(int n, T *a, int lda, int k
     typedef typename Mfma<T>::acc_t acc_t;
     acc_t acc[2][2];
     for (int p = 0; p < 2; p++) for (int q = 0; q < 2; q++) for (int e = 0; e < 4; e++) acc[p][q][e] = (T)0;
-    // MFMA 16x16x4 operand layout: lane l supplies A[row = l%16][k = l/16] and B[k = l/16][col = l%16];
-    // result element e of lane l is C[row = 4*(l/16)+e][col = l%16]
     const int lr = lane & 15, lk = lane >> 4;
     for (int kk = 0; kk < NB; kk += 4) {
         for (int p = 0; p < 2; p++) {
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(256) void k_syrk_update(int n, T *a, int lda, int k
     for (int p = 0; p < 2; p++)
         for (int q = 0; q < 2; q++)
             for (int e = 0; e < 4; e++) {
-                int i = r0 + wr + p * 16 + 4 * lk + e;
+                int i = r0 + wr + p * 16 + Mfma<T>::row(lk, e);
                 int j = c0 + wc + q * 16 + lr;
                 if (i < n && j < n && i >= j) {
                     int64_t o = (int64_t)j * lda + i;

@@ -440,6 +440,11 @@ __global__ void k_qlp_step(int64_t n, double *v, double *w, double *wl, double *
         v[i] = V; w[i] = W; wl[i] = WL; wl2[i] = WL2; x[i] = X; xl2[i] = XL2;
     }
 }
+// p = z + beta p
+__global__ void k_xpby(int64_t n, const double *z, double beta, double *p)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
 __global__ void k_copy(int64_t n, const double *a, double *b)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = a[i];
@@ -823,6 +828,8 @@ void l_qlp_minres(int64_t n, const double *v, double *w, double *wl, double *wl2
 void l_qlp_step(int64_t n, double *v, double *w, double *wl, double *wl2, double *x, double *xl2, const QlpStep &c,
                 hipStream_t s)
 { hipLaunchKernelGGL(k_qlp_step, dim3(nblk(n)), dim3(TPB), 0, s, n, v, w, wl, wl2, x, xl2, c); }
+void l_xpby(int64_t n, const double *z, double beta, double *p, hipStream_t s)
+{ hipLaunchKernelGGL(k_xpby, dim3(nblk(n)), dim3(TPB), 0, s, n, z, beta, p); }
 void l_copy(int64_t n, const double *a, double *b, hipStream_t s)
 { hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(TPB), 0, s, n, a, b); }
 void l_sub(int64_t n, const double *b, const double *ax, double *r, hipStream_t s)

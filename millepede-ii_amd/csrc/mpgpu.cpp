@@ -128,7 +128,7 @@ extern "C" void mpgpu_default_options(mpgpu_options *o)
     o->mrestl = 1.0e-6;
     o->mrtcnd = 1.0e7;
     o->nregul = 0;
-    o->refine_steps = 5;
+    o->refine_steps = 20;
 }
 
 extern "C" int mpgpu_create(const mpgpu_options *opt, int32_t device, mpgpu_ctx **out)
@@ -1740,25 +1740,41 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
         // solve with constraints via the Schur complement S = C A^-1 C^T (block elimination)
         auto solveA = [&](double *v) {
             if (!fp32) { chol_solve(nv, A.p, (int)lda, v, s); return; }
-            // FP32 factor + FP64 iterative refinement against the FP64 parameter block
-            DBuf<double> r, d, ax;
-            r.alloc(nv); d.alloc(nv); ax.alloc(nv);
-            l_copy(nv, v, r.p, s);                    // r = rhs
-            HIPCHK(hipMemsetAsync(d.p, 0, sizeof(double) * nv, s));
-            DBuf<double> xacc;
-            xacc.alloc(nv);
-            HIPCHK(hipMemsetAsync(xacc.p, 0, sizeof(double) * nv, s));
-            for (int it = 0; it <= c->opt.refine_steps; it++) {
-                l_copy(nv, r.p, d.p, s);
-                chol_solve32(nv, A32.p, (int)lda, d.p, xf.p, s);
-                l_axpy_neg(nv, -1.0, d.p, xacc.p, s); // xacc += d
-                // r = rhs - (A_pp + rho C^T C) xacc with the FP64 packed matrix (parameter block)
-                l_packed_symv(nv, c->d_mat.p, xacc.p, ax.p, s);
-                l_cct_apply(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, xacc.p, ax.p, s);
-                l_sub(nv, v, ax.p, r.p, s);
-                double rr = dot_dev(c, nv, r.p, r.p), bb = dot_dev(c, nv, v, v);
-                loc.itn = it + 1;
-                if (bb == 0.0 || std::sqrt(rr / bb) < 1e-15) break;
+            // FP32 factor as the preconditioner of FP64 conjugate gradients on the (augmented)
+            // parameter block: each step is one FP64 matrix-vector product and one FP32
+            // triangular solve pair; stops at ||r|| <= 1e-14 ||b|| or after refine_steps steps
+            DBuf<double> r, z, p, q, xacc;
+            r.alloc(nv); z.alloc(nv); p.alloc(nv); q.alloc(nv); xacc.alloc(nv);
+            auto apply = [&](const double *in, double *out) {
+                l_packed_symv(nv, c->d_mat.p, in, out, s);
+                l_cct_apply(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, in, out, s);
+            };
+            auto precond = [&](const double *in, double *out) {
+                l_copy(nv, in, out, s);
+                chol_solve32(nv, A32.p, (int)lda, out, xf.p, s);
+            };
+            xacc.zero(s);
+            l_copy(nv, v, r.p, s);
+            const double bb = dot_dev(c, nv, v, v);
+            if (bb == 0.0) { l_copy(nv, xacc.p, v, s); return; }
+            precond(r.p, z.p);
+            l_copy(nv, z.p, p.p, s);
+            double rz = dot_dev(c, nv, r.p, z.p);
+            const int maxit = std::max(1, c->opt.refine_steps);
+            for (int it = 1; it <= maxit; it++) {
+                apply(p.p, q.p);
+                const double pq = dot_dev(c, nv, p.p, q.p);
+                if (!(pq > 0.0)) break;
+                const double alpha = rz / pq;
+                l_axpy_neg(nv, -alpha, p.p, xacc.p, s);   // x += alpha p
+                l_axpy_neg(nv, alpha, q.p, r.p, s);       // r -= alpha q
+                loc.itn = std::max(loc.itn, it);
+                const double rr = dot_dev(c, nv, r.p, r.p);
+                if (std::sqrt(rr / bb) <= 1e-14) break;
+                precond(r.p, z.p);
+                const double rz2 = dot_dev(c, nv, r.p, z.p);
+                l_xpby(nv, z.p, rz2 / rz, p.p, s);         // p = z + beta p
+                rz = rz2;
             }
             l_copy(nv, xacc.p, v, s);
         };

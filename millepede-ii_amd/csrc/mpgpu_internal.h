@@ -98,6 +98,7 @@ void l_qlp_minres(int64_t n, const double *v, double *w, double *wl, double *wl2
 void l_qlp_step(int64_t n, double *v, double *w, double *wl, double *wl2, double *x, double *xl2, const QlpStep &c,
                 hipStream_t s);
 void l_copy(int64_t n, const double *a, double *b, hipStream_t s);
+void l_xpby(int64_t n, const double *z, double beta, double *p, hipStream_t s);
 void l_sub(int64_t n, const double *b, const double *ax, double *r, hipStream_t s);
 void l_precons(int32_t n, int32_t p, const double *diag, double *cu, int *nrkd, const int64_t *cptr, const int32_t *cidx,
                const double *cval, double *sm, double *scratch, hipStream_t s);
