@@ -39,6 +39,16 @@ MODULE mpgpu_mod
         INTEGER(c_int32_t) :: exit_code
     END TYPE mpgpu_run_stats
 
+    ! serving: statistics of one finished job (mpgpu_job_stats)
+    TYPE, BIND(C) :: mpgpu_job_stats
+        INTEGER(c_int32_t) :: status, nvar
+        REAL(c_double)     :: latency_ms, queue_ms, run_ms
+        REAL(c_double)     :: chi2
+        INTEGER(c_int64_t) :: ndf
+        INTEGER(c_int32_t) :: istop, itn
+        REAL(c_double)     :: rel_residual
+    END TYPE mpgpu_job_stats
+
     INTERFACE
         SUBROUTINE mpgpu_default_options(opt) BIND(C, name='mpgpu_default_options')
             IMPORT :: mpgpu_options
@@ -171,6 +181,47 @@ MODULE mpgpu_mod
             CHARACTER(KIND=c_char), DIMENSION(*), INTENT(IN) :: steer, outdir
             INTEGER(c_int32_t), VALUE :: device
             TYPE(mpgpu_run_stats), INTENT(OUT) :: st
+        END FUNCTION
+        ! serving: concurrent jobs on one GPU (no pede counterpart)
+        INTEGER(c_int) FUNCTION mpgpu_server_create(device, nworkers, budget_bytes, batch_max_params, srv) &
+                BIND(C, name='mpgpu_server_create')
+            IMPORT :: c_int, c_int32_t, c_int64_t, c_ptr
+            INTEGER(c_int32_t), VALUE :: device, nworkers
+            INTEGER(c_int64_t), VALUE :: budget_bytes
+            INTEGER(c_int32_t), VALUE :: batch_max_params
+            TYPE(c_ptr), INTENT(OUT) :: srv
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_server_submit(srv, opt, nrec, rec_ptr, inder, glder, npar, plab, pval, psig, &
+                ncon, cptr, clab, cfac, crhs, job_id) BIND(C, name='mpgpu_server_submit')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_int64_t, c_float, c_double, mpgpu_options
+            TYPE(c_ptr), VALUE :: srv
+            TYPE(mpgpu_options), INTENT(IN) :: opt
+            INTEGER(c_int64_t), VALUE :: nrec
+            INTEGER(c_int64_t), INTENT(IN) :: rec_ptr(*)
+            INTEGER(c_int32_t), INTENT(IN) :: inder(*)
+            REAL(c_float), INTENT(IN) :: glder(*)
+            INTEGER(c_int32_t), VALUE :: npar
+            INTEGER(c_int32_t), INTENT(IN) :: plab(*)
+            REAL(c_double), INTENT(IN) :: pval(*), psig(*)
+            INTEGER(c_int32_t), VALUE :: ncon
+            INTEGER(c_int64_t), INTENT(IN) :: cptr(*)
+            INTEGER(c_int32_t), INTENT(IN) :: clab(*)
+            REAL(c_double), INTENT(IN) :: cfac(*), crhs(*)
+            INTEGER(c_int64_t), INTENT(OUT) :: job_id
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_server_wait(srv, job_id, capacity, labels, corr, st) &
+                BIND(C, name='mpgpu_server_wait')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_int64_t, c_double, mpgpu_job_stats
+            TYPE(c_ptr), VALUE :: srv
+            INTEGER(c_int64_t), VALUE :: job_id
+            INTEGER(c_int32_t), VALUE :: capacity
+            INTEGER(c_int32_t), INTENT(OUT) :: labels(*)
+            REAL(c_double), INTENT(OUT) :: corr(*)
+            TYPE(mpgpu_job_stats), INTENT(OUT) :: st
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_server_destroy(srv) BIND(C, name='mpgpu_server_destroy')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: srv
         END FUNCTION
     END INTERFACE
 END MODULE mpgpu_mod

@@ -39,39 +39,20 @@ template <> __device__ __forceinline__ double wval<double>(const DevRecords &R, 
 __device__ __forceinline__ int pk(int i, int j) { return (i * i - i) / 2 + j - 1; }
 __device__ __forceinline__ int64_t pk64(int64_t i, int64_t j) { return (i * i - i) / 2 + j - 1; }
 
-// address of element (va >= vb) (1-based variable indices) in the block layout;
-// segbase: block id of the record-local segment pair, fa/fb first variable of the groups
-__device__ __forceinline__ int64_t block_addr(const DevModel &M, int32_t blk, int va, int vb, int ga, int gb)
-{
-    int fa = M.gfirst[ga], fb = M.gfirst[gb];
-    int nc = M.gfirst[gb + 1] - fb;
-    return M.boff[blk] + (int64_t)(va - fa) * nc + (vb - fb);
-}
-
-// binary search of column group gb in block row ga
-__device__ int32_t find_block(const DevModel &M, int ga, int gb)
-{
-    int64_t lo = M.brow[ga], hi = M.brow[ga + 1] - 1;
-    while (lo <= hi) {
-        int64_t mid = (lo + hi) >> 1;
-        int c = M.bcol[mid];
-        if (c == gb) return (int32_t)mid;
-        if (c < gb) lo = mid + 1; else hi = mid - 1;
-    }
-    return -1;
-}
-
+// element (va >= vb, 1-based variable indices) of the global matrix; pos_a/pos_b are the
+// positions of va/vb in the record's variable list.  Group-block storage: the address of
+// (va, vb) in the block of segment pair (sa, sb) is pbase[pair] + va * segnc[sb] + vb, with
+// pbase = boff[blk] - first(ga) * nc - first(gb) precomputed per record in LDS.
 __device__ __forceinline__ void mat_add(const DevModel &M, int va, int vb, double v, const int32_t *seg_of,
-                                        const int32_t *seg_grp, const int32_t *pblk, int pos_a, int pos_b)
+                                        const int32_t *segnc, const int64_t *pbase, int pos_a, int pos_b)
 {
-    // va >= vb; pos_a/pos_b are the positions of va/vb in the record's variable list
     if (!M.sparse) {
         atomicAdd(&M.mat[pk64(va, vb)], v);
     } else {
         int sa = seg_of[pos_a], sb = seg_of[pos_b];
-        int blk = pblk[(sa * sa + sa) / 2 + sb];
-        if (blk < 0) return;
-        atomicAdd(&M.mat[block_addr(M, blk, va, vb, seg_grp[sa], seg_grp[sb])], v);
+        int64_t base = pbase[(sa * sa + sa) / 2 + sb];
+        if (base == INT64_MIN) return;             // pair without a block (not stored)
+        atomicAdd(&M.mat[base + (int64_t)va * segnc[sb] + vb], v);
     }
 }
 
@@ -97,13 +78,14 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *co = cs + P.nalc_max;            // [nalc_max] original pivot column
     double *G = co + P.nalc_max;             // [max_var * nalc_max]
     double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max]
-    int32_t *gv = (int32_t *)(Gh + (size_t)P.max_var * P.nalc_max); // [max_gl] var index
+    int64_t *pbase = (int64_t *)(Gh + (size_t)P.max_var * P.nalc_max); // [max_pair] block base per segment pair
+    int32_t *gv = (int32_t *)(pbase + P.max_pair); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
     int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
     int32_t *segof = gvl + P.max_var;        // [max_var] segment of each record variable
     int32_t *seggr = segof + P.max_var;      // [max_var] group of each segment
-    int32_t *pblk = seggr + P.max_var;       // [max_pair] block id per segment pair
-    int32_t *nxt = pblk + P.max_pair;        // [nalc_max+1] pivot list
+    int32_t *segnc = seggr + P.max_var;      // [max_var] parameters in the group of each segment
+    int32_t *nxt = segnc + P.max_var;        // [nalc_max+1] pivot list
     int32_t *sh = nxt + P.nalc_max + 1;      // scalars
 
     for (int64_t ri = rec_begin + blockIdx.x; ri < rec_end; ri += gridDim.x) {
@@ -394,9 +376,23 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                     if (a < nalg) { segof[a] = s; if (st) seggr[s] = g; }
                     nseg += __popcll(m);
                 }
+                for (int q = lane; q < nseg; q += WAVE) {
+                    const int g = seggr[q];
+                    segnc[q] = M.gfirst[g + 1] - M.gfirst[g];
+                }
                 int npair = (nseg * nseg + nseg) / 2;
                 const int64_t p0 = R.rec_pb[r];
-                for (int t = lane; t < npair; t += WAVE) pblk[t] = t < P.max_pair ? R.pair_blk[p0 + t] : -1;
+                for (int t = lane; t < npair && t < P.max_pair; t += WAVE) {
+                    int sa = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+                    while ((sa * sa + sa) / 2 > t) sa--;
+                    while (((sa + 1) * (sa + 1) + (sa + 1)) / 2 <= t) sa++;
+                    const int sb = t - (sa * sa + sa) / 2;
+                    const int blk = R.pair_blk[p0 + t];
+                    const int ga = seggr[sa], gb = seggr[sb];
+                    const int fa = M.gfirst[ga], fb = M.gfirst[gb];
+                    const int nc = M.gfirst[gb + 1] - fb;
+                    pbase[t] = blk < 0 ? INT64_MIN : M.boff[blk] - (int64_t)fa * nc - fb;
+                }
             }
         }
         __syncthreads();
@@ -438,8 +434,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                             if (vk > 0) {
                                 double val = dw1 * wght * gj * wval<T>(R, sg + k);
                                 int ak = gpos[pk_];
-                                if (vj >= vk) mat_add(M, vj, vk, val, segof, seggr, pblk, aj, ak);
-                                else mat_add(M, vk, vj, val, segof, seggr, pblk, ak, aj);
+                                if (vj >= vk) mat_add(M, vj, vk, val, segof, segnc, pbase, aj, ak);
+                                else mat_add(M, vk, vj, val, segof, segnc, pbase, ak, aj);
                             }
                         }
                     }
@@ -477,15 +473,18 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             }
             __syncthreads();
             // Schur complement: element (a>=b) -= sum_k Gh[a,k] G[b,k]
+            // packed (a, b) enumeration: lane starts at element `lane`, then steps by 64
             const int ngg = (nalg * nalg + nalg) / 2;
+            int a = (int)((sqrt(8.0 * lane + 1.0) - 1.0) * 0.5);
+            while ((a * a + a) / 2 > lane) a--;
+            while (((a + 1) * (a + 1) + (a + 1)) / 2 <= lane) a++;
+            int b = lane - (a * a + a) / 2;
             for (int t = lane; t < ngg; t += WAVE) {
-                int a = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-                while ((a * a + a) / 2 > t) a--;
-                while (((a + 1) * (a + 1) + (a + 1)) / 2 <= t) a++;
-                int b = t - (a * a + a) / 2;
                 double c = 0.0;
                 for (int k = 0; k < nalc; k++) c = c + Gh[a * nalc + k] * G[b * nalc + k];
-                mat_add(M, gvl[a], gvl[b], -c, segof, seggr, pblk, a, b);
+                mat_add(M, gvl[a], gvl[b], -c, segof, segnc, pbase, a, b);
+                b += WAVE;
+                while (b > a) { b -= a + 1; a++; }
             }
         }
         if (lane == 0) { out.status[r] = 0; out.chi2[r] = summ * dw1; out.ndf[r] = ndf; }
@@ -514,8 +513,8 @@ int lds_bytes(LdsPlan &P)
 {
     int npk = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
     size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max + 2 * (size_t)P.max_var * P.nalc_max;
-    size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + P.max_pair + (P.nalc_max + 1) + 8;
-    size_t b = d * 8 + i * 4;
+    size_t i = 2 * (size_t)P.max_gl + 4 * (size_t)P.max_var + (P.nalc_max + 1) + 8;
+    size_t b = d * 8 + (size_t)P.max_pair * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;
 }

@@ -337,6 +337,95 @@ __global__ void k_block_symv(int32_t ngroup, const int64_t *brow, const int32_t 
     }
 }
 
+// Symmetric product with the group-block matrix in two passes without atomics.
+// Rows pass: one wavefront per block row ga owns y[rows of ga] = sum over the stored blocks
+// (ga, gb <= ga) of B x[gb], the diagonal block used with both of its triangles.
+// Columns pass: one wavefront per column group gb adds sum over the blocks (ga > gb) of
+// B^T x[ga] to y[rows of gb] (column lists ccol_ptr / ccol_blk built with the structure).
+// Lanes take whole blocks; per-lane partial sums of up to 8 rows are reduced across the wave.
+#define BSY_R 8
+__device__ __forceinline__ double wsum64(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_t *brow, const int32_t *bcol,
+                                                    const int64_t *boff, const int32_t *gfirst, const double *a,
+                                                    const double *x, double *y)
+{
+    const int ga = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (ga >= ngroup) return;
+    const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
+    const int64_t b0 = brow[ga], b1 = brow[ga + 1];
+    for (int r0 = 0; r0 < nr; r0 += BSY_R) {
+        double acc[BSY_R];
+#pragma unroll
+        for (int r = 0; r < BSY_R; r++) acc[r] = 0.0;
+        for (int64_t bk = b0 + lane; bk < b1; bk += 64) {
+            const int gb = bcol[bk];
+            const int fb = gfirst[gb] - 1, nc = gfirst[gb + 1] - 1 - fb;
+            const double *B = a + boff[bk];
+            if (gb != ga) {
+#pragma unroll
+                for (int r = 0; r < BSY_R; r++) {
+                    if (r0 + r >= nr) break;
+                    double s = 0.0;
+                    for (int c = 0; c < nc; c++) s += B[(int64_t)(r0 + r) * nc + c] * x[fb + c];
+                    acc[r] += s;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < BSY_R; r++) {
+                    const int R = r0 + r;
+                    if (R >= nr) break;
+                    double s = 0.0;
+                    for (int c = 0; c < nc; c++) s += (c <= R ? B[(int64_t)R * nc + c] : B[(int64_t)c * nc + R]) * x[fb + c];
+                    acc[r] += s;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < BSY_R; r++) {
+            double v = wsum64(acc[r]);
+            if (lane == 0 && r0 + r < nr) y[fa + r0 + r] = v;
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_bsymv_cols(int32_t ngroup, const int64_t *ccol_ptr, const int64_t *ccol_blk,
+                                                    const int32_t *brg, const int64_t *boff, const int32_t *gfirst,
+                                                    const double *a, const double *x, double *y)
+{
+    const int gb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (gb >= ngroup) return;
+    const int fb = gfirst[gb] - 1, nc = gfirst[gb + 1] - 1 - fb;
+    const int64_t q0 = ccol_ptr[gb], q1 = ccol_ptr[gb + 1];
+    if (q1 <= q0) return;
+    for (int c0 = 0; c0 < nc; c0 += BSY_R) {
+        double acc[BSY_R];
+#pragma unroll
+        for (int c = 0; c < BSY_R; c++) acc[c] = 0.0;
+        for (int64_t q = q0 + lane; q < q1; q += 64) {
+            const int64_t bk = ccol_blk[q];
+            const int ga = brg[bk];
+            const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
+            const double *B = a + boff[bk];
+            for (int r = 0; r < nr; r++) {
+                const double xr = x[fa + r];
+#pragma unroll
+                for (int c = 0; c < BSY_R; c++)
+                    if (c0 + c < nc) acc[c] += B[(int64_t)r * nc + c0 + c] * xr;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < BSY_R; c++) {
+            double v = wsum64(acc[c]);
+            if (lane == 0 && c0 + c < nc) y[fb + c0 + c] += v;
+        }
+    }
+}
+
 // constraint rows (Lagrange multipliers) in CSR over variable indices:
 // y[nvar+j] += sum c_ji x_i ; y_i += c_ji x[nvar+j]
 __global__ void k_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx,
@@ -582,37 +671,95 @@ __global__ void k_presols_b(int64_t n, const double *cu, double *x)
 // Variable-band Cholesky preconditioner (lltdec/lltfwd/lltbwd, src/mpnum.f90:2155-2336).
 // The recurrences are sequential in the row index; they run on one lane.
 // ---------------------------------------------------------------------------------------------
-__global__ void k_lltdec(int n, double *c, const int *india, int *nrkd, int iopt)
+// lltdec (src/mpnum.f90:2155-2222): variable-band Cholesky C = L L^T with the
+// inverse square roots of the pivots on the diagonal, restated operation by operation.  The
+// recurrence is sequential in the row index; the band values are staged through LDS in tiles
+// (coalesced loads and stores by the whole workgroup) and one thread runs the recurrence on
+// LDS, with the last rows of the previous tile kept in a small ring.
+#define LLT_TILE 6144     // band values per tile
+#define LLT_ROWS 2048     // rows per tile
+#define LLT_MW 64         // widest row supported by the tiled path
+__global__ __launch_bounds__(256) void k_lltdec(int n, double *c, const int *india, int *nrkd, int iopt)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double *C = c - 1;
-    const int *IND = india - 1;
+    __shared__ double buf[LLT_TILE];
+    __shared__ int ind[LLT_ROWS + 1];
+    __shared__ double ring[LLT_MW * LLT_MW];   // element (j, i) of row j < k0 at ring[(j % MW)*MW + j - i]
+    __shared__ int sh[4];
+    const int tid = threadIdx.x;
     const double eps = 16.0 * DBL_EPSILON;
     int nr = 0;
     double diag = 0.0;
-    if (C[IND[1]] > 0.0) C[IND[1]] = 1.0 / sqrt(C[IND[1]]);
-    else { C[IND[1]] = 0.0; nr = -1; }
-    for (int k = 2; k <= n; k++) {
-        int mk = k - IND[k] + IND[k - 1] + 1;
-        for (int j = mk; j <= k; j++) {
-            int mj = 1;
-            if (j > 1) mj = j - IND[j] + IND[j - 1] + 1;
-            int kj = IND[k] - k + j;
-            diag = C[IND[j]];
-            for (int i = max(mj, mk); i <= j - 1; i++) C[kj] = C[kj] - C[IND[k] - k + i] * C[IND[j] - j + i];
-            if (j != k) C[kj] = C[kj] * diag;
+    int k0 = 1;
+    while (k0 <= n) {
+        if (tid == 0) {
+            const int base = k0 > 1 ? india[k0 - 2] : 0;   // IND[k0-1] (0 for k0 = 1)
+            int k1 = k0;
+            while (k1 + 1 <= n && k1 + 1 - k0 + 1 <= LLT_ROWS && india[k1] - base <= LLT_TILE) k1++;
+            sh[0] = k1;
+            sh[1] = base;
         }
-        if (C[IND[k]] > eps * diag) C[IND[k]] = 1.0 / sqrt(C[IND[k]]);
-        else {
-            for (int j = mk; j <= k; j++) C[IND[k] - k + j] = 0.0;
-            if (iopt > 0 && diag > 0.0) C[IND[k]] = 1.0 / sqrt(diag);
-            else {
-                if (nr == 0) nr = -k;
-                else { if (nr < 0) nr = 1; nr++; }
+        __syncthreads();
+        const int k1 = sh[0], base = sh[1];
+        const int nel = india[k1 - 1] - base;
+        for (int t = tid; t < nel; t += 256) buf[t] = c[base + t];
+        for (int t = tid; t <= k1 - k0; t += 256) ind[t + 1] = india[k0 - 1 + t];
+        if (tid == 0) ind[0] = base;
+        __syncthreads();
+        if (tid == 0) {
+            // IND(k) for k in [k0-1, k1]: ind[k - k0 + 1]; C(p) for p > base: buf[p - base - 1]
+#define IND_(k) ind[(k) - k0 + 1]
+            auto C = [&](int j, int i) -> double & {
+                if (j >= k0) return buf[IND_(j) - j + i - base - 1];
+                return ring[(j % LLT_MW) * LLT_MW + (j - i)];
+            };
+            for (int k = k0; k <= k1; k++) {
+                if (k == 1) {
+                    double &d = C(1, 1);
+                    if (d > 0.0) d = 1.0 / sqrt(d);
+                    else { d = 0.0; nr = -1; }
+                    continue;
+                }
+                const int indk = IND_(k);
+                const int indk1 = (k - 1 >= k0) ? IND_(k - 1) : base;
+                const int mk = k - indk + indk1 + 1;
+                for (int j = mk; j <= k; j++) {
+                    int mj = 1;
+                    if (j > 1) {
+                        // width of row j (rows before the tile: from the global pointers)
+                        const int indj = j >= k0 ? IND_(j) : india[j - 1];
+                        const int indj1 = (j - 1 >= k0) ? IND_(j - 1) : (j - 1 >= 1 ? india[j - 2] : 0);
+                        mj = j - indj + indj1 + 1;
+                    }
+                    diag = C(j, j);
+                    double &ckj = C(k, j);
+                    for (int i = max(mj, mk); i <= j - 1; i++) ckj = ckj - C(k, i) * C(j, i);
+                    if (j != k) ckj = ckj * diag;
+                }
+                double &ckk = C(k, k);
+                if (ckk > eps * diag) ckk = 1.0 / sqrt(ckk);
+                else {
+                    for (int j = mk; j <= k; j++) C(k, j) = 0.0;
+                    if (iopt > 0 && diag > 0.0) C(k, k) = 1.0 / sqrt(diag);
+                    else {
+                        if (nr == 0) nr = -k;
+                        else { if (nr < 0) nr = 1; nr++; }
+                    }
+                }
             }
+            // keep the last rows of the tile for the next one
+            for (int k = max(k0, k1 - LLT_MW + 1); k <= k1; k++) {
+                const int indk1 = (k - 1 >= k0) ? IND_(k - 1) : base;
+                const int mk = k - IND_(k) + indk1 + 1;
+                for (int j = mk; j <= k; j++) ring[(k % LLT_MW) * LLT_MW + (k - j)] = C(k, j);
+            }
+#undef IND_
         }
+        __syncthreads();
+        for (int t = tid; t < nel; t += 256) c[base + t] = buf[t];
+        __syncthreads();
+        k0 = k1 + 1;
     }
-    *nrkd = nr;
+    if (tid == 0) *nrkd = nr;
 }
 
 __device__ void lltfwd_dev(int n, const double *c, const int *india, double *x)
@@ -802,14 +949,20 @@ void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream
     int64_t tiles = nt * (nt + 1) / 2;
     hipLaunchKernelGGL(k_packed_symv, dim3((unsigned)tiles), dim3(256), 0, s, n, a, x, y);
 }
-void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const int32_t *gfirst,
-                  const int32_t *brg, int64_t nblock, const double *a, const double *x, double *y, int64_t nall,
-                  hipStream_t s)
+__global__ void k_zero(int64_t n, double *y)
 {
-    hipMemsetAsync(y, 0, sizeof(double) * nall, s);
-    int64_t t = nblock * 8;
-    hipLaunchKernelGGL(k_block_symv, dim3((unsigned)((t + TPB - 1) / TPB)), dim3(TPB), 0, s, ngroup, brow, bcol, boff,
-                       gfirst, brg, nblock, a, x, y);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = 0.0;
+}
+void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const int32_t *gfirst,
+                  const int32_t *brg, const int64_t *ccol_ptr, const int64_t *ccol_blk, const double *a,
+                  const double *x, double *y, int32_t nvar, int64_t nall, hipStream_t s)
+{
+    // rows beyond the parameter groups (Lagrange multipliers) start at zero
+    if (nall > nvar) hipLaunchKernelGGL(k_zero, dim3(nblk(nall - nvar)), dim3(TPB), 0, s, nall - nvar, y + nvar);
+    hipLaunchKernelGGL(k_bsymv_rows, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcol, boff, gfirst,
+                       a, x, y);
+    hipLaunchKernelGGL(k_bsymv_cols, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, ccol_blk, brg,
+                       boff, gfirst, a, x, y);
 }
 void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
                  double *y, hipStream_t s)
@@ -851,7 +1004,7 @@ void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, cons
 }
 
 void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, hipStream_t s)
-{ hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); }
+{ hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt); }
 void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
               hipStream_t s)
 { if (ncon > 0) hipLaunchKernelGGL(k_band_k, dim3((ncon + 63) / 64), dim3(64), 0, s, ncon, n, c, india, cb, koff, kmat); }
@@ -943,47 +1096,56 @@ __global__ void k_spk_local(int n, int w, int nch, const double *c, const int *i
     else spk_bwd_chunk(k0, k1, w, n, c, india, x, nullptr);
 }
 
-// carry: boundary values of every chunk, sequential over chunks (w x w products)
-__global__ void k_spk_carry(int n, int w, int nch, const double *x, const double *S, double *t, int backward)
+// carry: boundary values of every chunk, sequential over chunks (w x w products).  The data
+// of a tile of chunks (their boundary rows of x and of the response matrix S) is staged in LDS
+// by the whole workgroup; one thread then runs the chunk-to-chunk recurrence from LDS.
+#define SPK_LDS_DOUBLES 6144
+__global__ __launch_bounds__(256) void k_spk_carry(int n, int w, int nch, const double *x, const double *S, double *t,
+                                                   int backward)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double cur[SPK_WMAX], nxt[SPK_WMAX];
-    if (!backward) {
-        // t[ch] = true last w values of chunk ch-1
-        for (int i = 0; i < w; i++) cur[i] = 0.0;
-        for (int ch = 0; ch < nch; ch++) {
-            int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
-            for (int i = 0; i < w; i++) t[(size_t)ch * w + i] = cur[i];
-            for (int i = 0; i < w; i++) {
-                int k = k1 - w + 1 + i;              // 1-based row
-                double v = 0.0;
-                if (k >= k0) {
-                    v = x[k - 1];
-                    if (ch > 0) for (int q = 0; q < w; q++) v += S[(size_t)(k - 1) * w + q] * cur[q];
-                } else {
-                    v = cur[i + (k1 - k0 + 1)]; // chunk shorter than w: carry through
-                }
-                nxt[i] = v;
-            }
-            for (int i = 0; i < w; i++) cur[i] = nxt[i];
+    extern __shared__ double dyn[];
+    const int tc = SPK_LDS_DOUBLES / (w + w * w);      // chunks per tile
+    double *sx = dyn;                                  // [tc * w]
+    double *sS = sx + tc * w;                          // [tc * w * w]
+    __shared__ double cur[SPK_WMAX];
+    const int tid = threadIdx.x;
+    if (tid < SPK_WMAX) cur[tid] = 0.0;
+    const int ntile = (nch + tc - 1) / tc;
+    for (int tt = 0; tt < ntile; tt++) {
+        // tiles in processing order (backward: from the last chunk)
+        const int tile = backward ? ntile - 1 - tt : tt;
+        const int c0 = tile * tc, c1 = min(nch, c0 + tc);
+        __syncthreads();
+        for (int e = tid; e < (c1 - c0) * w; e += 256) {
+            const int ch = c0 + e / w, i = e % w;
+            const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
+            const int k = backward ? k0 + i : k1 - w + 1 + i;
+            const bool in = backward ? (k <= k1) : (k >= k0);
+            sx[e] = in ? x[k - 1] : 0.0;
+            for (int q = 0; q < w; q++) sS[e * w + q] = in ? S[(size_t)(k - 1) * w + q] : 0.0;
         }
-    } else {
-        for (int i = 0; i < w; i++) cur[i] = 0.0;
-        for (int ch = nch - 1; ch >= 0; ch--) {
-            int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
-            for (int i = 0; i < w; i++) t[(size_t)ch * w + i] = cur[i];
-            for (int i = 0; i < w; i++) {
-                int k = k0 + i;
-                double v = 0.0;
-                if (k <= k1) {
-                    v = x[k - 1];
-                    if (k1 < n) for (int q = 0; q < w; q++) v += S[(size_t)(k - 1) * w + q] * cur[q];
-                } else {
-                    v = cur[i - (k1 - k0 + 1)];
+        __syncthreads();
+        if (tid == 0) {
+            double nxt[SPK_WMAX];
+            for (int cc = 0; cc < c1 - c0; cc++) {
+                const int ch = backward ? c1 - 1 - cc : c0 + cc;
+                const int e0 = (ch - c0) * w;
+                const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1), L = k1 - k0 + 1;
+                for (int i = 0; i < w; i++) t[(size_t)ch * w + i] = cur[i];
+                const bool coupled = backward ? (k1 < n) : (ch > 0);
+                for (int i = 0; i < w; i++) {
+                    const bool in = backward ? (i < L) : (w - 1 - i < L);
+                    double v;
+                    if (in) {
+                        v = sx[e0 + i];
+                        if (coupled) for (int q = 0; q < w; q++) v += sS[(e0 + i) * w + q] * cur[q];
+                    } else {
+                        v = backward ? cur[i - L] : cur[i + L];   // chunk shorter than w: carry through
+                    }
+                    nxt[i] = v;
                 }
-                nxt[i] = v;
+                for (int i = 0; i < w; i++) cur[i] = nxt[i];
             }
-            for (int i = 0; i < w; i++) cur[i] = nxt[i];
         }
     }
 }
@@ -1049,7 +1211,7 @@ void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, con
 {
     int nch = (n + SPK_CH - 1) / SPK_CH;
     hipLaunchKernelGGL(k_spk_local, dim3(g1(nch, 64)), dim3(64), 0, s, n, w, nch, c, india, x, 0);
-    hipLaunchKernelGGL(k_spk_carry, dim3(1), dim3(64), 0, s, n, w, nch, x, Sf, tbuf, 0);
+    hipLaunchKernelGGL(k_spk_carry, dim3(1), dim3(256), SPK_LDS_DOUBLES * sizeof(double), s, n, w, nch, x, Sf, tbuf, 0);
     hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sf, tbuf, x, 0);
     if (m > 0) {
         hipLaunchKernelGGL(k_eqs_cdot, dim3(m), dim3(TPB), 0, s, n, cb, koff, kmat, x, dtmp);
@@ -1057,7 +1219,7 @@ void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, con
         hipLaunchKernelGGL(k_eqs_upd, dim3(8, m), dim3(TPB), 0, s, n, cb, koff, kmat, x);
     }
     hipLaunchKernelGGL(k_spk_local, dim3(g1(nch, 64)), dim3(64), 0, s, n, w, nch, c, india, x, 1);
-    hipLaunchKernelGGL(k_spk_carry, dim3(1), dim3(64), 0, s, n, w, nch, x, Sb, tbuf, 1);
+    hipLaunchKernelGGL(k_spk_carry, dim3(1), dim3(256), SPK_LDS_DOUBLES * sizeof(double), s, n, w, nch, x, Sb, tbuf, 1);
     hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sb, tbuf, x, 1);
 }
 

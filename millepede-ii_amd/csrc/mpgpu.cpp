@@ -39,6 +39,7 @@ struct mpgpu_ctx {
     std::vector<int32_t> gfirst, group_of; // variable groups
     std::vector<int64_t> brow, boff;
     std::vector<int32_t> bcol, brg;
+    std::vector<int64_t> ccol_ptr, ccol_blk; // off-diagonal blocks by column group (symmetric product)
     int64_t nblock = 0, nstored = 0;
     // constraints (CSR over variable indices) and residual bookkeeping
     std::vector<int64_t> cptr;
@@ -55,7 +56,7 @@ struct mpgpu_ctx {
     DBuf<int32_t> d_widx, d_gl_tix, d_pair_blk;
     DBuf<double> d_par, d_mat, d_vec, d_pre_weight;
     DBuf<int32_t> d_var, d_group_of, d_gfirst, d_bcol, d_brg;
-    DBuf<int64_t> d_brow, d_boff;
+    DBuf<int64_t> d_brow, d_boff, d_ccol_ptr, d_ccol_blk;
     DBuf<unsigned long long> d_cnt;
     DBuf<double> d_chi2;
     DBuf<int32_t> d_ndf;
@@ -758,6 +759,16 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
                 off += nr * nc;
             }
             c->nstored = off;
+            // column lists of the off-diagonal blocks (ascending block row within a column)
+            c->ccol_ptr.assign(ng + 1, 0);
+            for (int64_t b = 0; b < c->nblock; b++) if (c->brg[b] != c->bcol[b]) c->ccol_ptr[c->bcol[b] + 1]++;
+            for (int64_t g = 0; g < ng; g++) c->ccol_ptr[g + 1] += c->ccol_ptr[g];
+            c->ccol_blk.assign(c->ccol_ptr[ng], 0);
+            {
+                std::vector<int64_t> fill(c->ccol_ptr.begin(), c->ccol_ptr.end() - 1);
+                for (int64_t b = 0; b < c->nblock; b++)
+                    if (c->brg[b] != c->bcol[b]) c->ccol_blk[fill[c->bcol[b]]++] = b;
+            }
             // per record: block id of each segment pair
             for (int64_t r = 0; r < nrec; r++) rec_pb[r + 1] = rec_pb[r] + (int64_t)nseg_rec[r] * (nseg_rec[r] + 1) / 2;
             pair_blk.assign(rec_pb[nrec], -1);
@@ -861,6 +872,8 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->d_bcol.upload(c->bcol, s);
         c->d_brg.upload(c->brg, s);
         c->d_boff.upload(c->boff, s);
+        c->d_ccol_ptr.upload(c->ccol_ptr, s);
+        c->d_ccol_blk.upload(c->ccol_blk, s);
     }
     c->d_group_of.upload(c->group_of, s);
     c->d_gfirst.upload(c->gfirst, s);
@@ -1192,8 +1205,8 @@ void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
 {
     hipStream_t s = c->stream;
     if (c->opt.matsto == MPGPU_SPARSE) {
-        l_block_symv(c->ngroup, c->d_brow.p, c->d_bcol.p, c->d_boff.p, c->d_gfirst.p, c->d_brg.p, c->nblock, c->d_mat.p, x, y,
-                     c->nall, s);
+        l_block_symv(c->ngroup, c->d_brow.p, c->d_bcol.p, c->d_boff.p, c->d_gfirst.p, c->d_brg.p, c->d_ccol_ptr.p,
+                     c->d_ccol_blk.p, c->d_mat.p, x, y, c->nvar, c->nall, s);
         l_cons_symv(c->ncon, c->nvar, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, x, y, s);
     } else {
         l_packed_symv(c->nall, c->d_mat.p, x, y, s);
@@ -1243,6 +1256,7 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
     // band preconditioner (vmprep 7690-7700, equdecs)
     P.kind = 2;
     const int mb = c->opt.mbandw;
+    if (mb > 64) throw MpError(30, "band preconditioner supports bandwidth <= 64");
     std::vector<int> india(P.n);
     for (int i = 1; i <= std::min(mb, P.n); i++) india[i - 1] = (i * i + i) / 2;
     for (int i = std::min(mb, P.n) + 1; i <= P.n; i++) india[i - 1] = india[i - 2] + mb;

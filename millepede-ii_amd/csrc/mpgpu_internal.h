@@ -85,8 +85,8 @@ void l_gj_step(int n, double *v, const double *diag, int *used, int *info, doubl
 void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, hipStream_t s);
 void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream_t s);
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const int32_t *gfirst,
-                  const int32_t *brg, int64_t nblock, const double *a, const double *x, double *y, int64_t nall,
-                  hipStream_t s);
+                  const int32_t *brg, const int64_t *ccol_ptr, const int64_t *ccol_blk, const double *a,
+                  const double *x, double *y, int32_t nvar, int64_t nall, hipStream_t s);
 void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
                  double *y, hipStream_t s);
 void l_scale_copy(int64_t n, double sc, const double *a, double *o, hipStream_t s);

@@ -48,3 +48,29 @@ def test_fortran_binding_compiles():
     with tempfile.TemporaryDirectory() as d:
         r = subprocess.run(["flang", "-c", src, "-J", d, "-o", os.path.join(d, "m.o")], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.skipif(shutil.which("flang") is None, reason="flang not available")
+def test_fortran_program_calls_library():
+    """A Fortran caller (as pede would be) links libmpgpu.so through the ISO_C_BINDING module and
+    reads pede's defaults back (no device call)."""
+    src = os.path.join(ROOT, "include", "mpgpu_mod.f90")
+    prog = """
+PROGRAM t
+    USE mpgpu_mod
+    IMPLICIT NONE
+    TYPE(mpgpu_options) :: o
+    CALL mpgpu_default_options(o)
+    PRINT '(3I8,F8.2)', o%itnlim, o%mreqenf, o%lfitbb, o%chhuge
+END PROGRAM t
+"""
+    libdir = os.path.dirname(mpgpu.LIB_PATH)
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "t.f90"), "w") as f:
+            f.write(prog)
+        r = subprocess.run(["flang", "-J", d, src, os.path.join(d, "t.f90"), "-L" + libdir, "-lmpgpu",
+                            "-Wl,-rpath," + libdir, "-o", os.path.join(d, "t")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        r = subprocess.run([os.path.join(d, "t")], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout.split() == ["2000", "25", "1", "50.00"]
