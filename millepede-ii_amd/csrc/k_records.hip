@@ -19,6 +19,7 @@
 namespace mpgpu {
 
 #define WAVE 64
+#define QE 4          // equations per record variable on the merged update path
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -78,14 +79,19 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *co = cs + P.nalc_max;            // [nalc_max] original pivot column
     double *G = co + P.nalc_max;             // [max_var * nalc_max]
     double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max]
-    int64_t *pbase = (int64_t *)(Gh + (size_t)P.max_var * P.nalc_max); // [max_pair] block base per segment pair
+    double *qg = Gh + (size_t)P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
+    int64_t *pbase = (int64_t *)(qg + (size_t)P.max_var * QE); // [max_pair] block base per segment pair
     int32_t *gv = (int32_t *)(pbase + P.max_pair); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
     int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
     int32_t *segof = gvl + P.max_var;        // [max_var] segment of each record variable
     int32_t *seggr = segof + P.max_var;      // [max_var] group of each segment
     int32_t *segnc = seggr + P.max_var;      // [max_var] parameters in the group of each segment
-    int32_t *nxt = segnc + P.max_var;        // [nalc_max+1] pivot list
+    int32_t *segst = segnc + P.max_var;      // [max_var+1] first record variable of each segment
+    int32_t *pcnt = segst + P.max_var + 1;   // [max_pair] elements of each segment pair
+    int32_t *qn = pcnt + P.max_pair;         // [max_var] equations of each record variable
+    int32_t *qe = qn + P.max_var;            // [max_var*QE] their indices (ascending)
+    int32_t *nxt = qe + P.max_var * QE;      // [nalc_max+1] pivot list
     int32_t *sh = nxt + P.nalc_max + 1;      // scalars
 
     for (int64_t ri = rec_begin + blockIdx.x; ri < rec_end; ri += gridDim.x) {
@@ -361,37 +367,51 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             continue;
         }
         // ---- fourth loop: vector, counters, g g^T, G (4162-4229) ----
-        if (o.icalcm == 1) {
+        // Every record variable a collects the equations it occurs in (up to QE, ascending), so
+        // that the vector, the counters and g g^T are formed per variable and merged with the
+        // Schur complement into one FP64 atomic per matrix element.  Records with a variable in
+        // more than QE equations, or twice in one equation, take the per-equation path instead.
+        if (o.icalcm == 1)
             for (int t = lane; t < nalg * nalc; t += WAVE) G[t] = 0.0;
-            if (M.sparse) {
-                // segments: runs of record variables in the same parameter group
-                int nseg = 0;
-                for (int base = 0; base < nalg; base += WAVE) {
-                    int a = base + lane;
-                    int g = a < nalg ? M.group_of[gvl[a]] : -1;
-                    int gprev = (a > 0 && a < nalg) ? M.group_of[gvl[a - 1]] : -2;
-                    bool st = (a < nalg) && (a == 0 || g != gprev);
-                    unsigned long long m = __ballot(st);
-                    int s = nseg + __popcll(m & ((2ull << lane) - 1ull)) - 1;
-                    if (a < nalg) { segof[a] = s; if (st) seggr[s] = g; }
-                    nseg += __popcll(m);
-                }
-                for (int q = lane; q < nseg; q += WAVE) {
-                    const int g = seggr[q];
-                    segnc[q] = M.gfirst[g + 1] - M.gfirst[g];
-                }
-                int npair = (nseg * nseg + nseg) / 2;
-                const int64_t p0 = R.rec_pb[r];
-                for (int t = lane; t < npair && t < P.max_pair; t += WAVE) {
+        for (int t = lane; t < nalg; t += WAVE) qn[t] = 0;
+        if (lane == 0) sh[5] = 0;
+        // segments: runs of record variables in the same parameter group (consecutive variables)
+        if (o.icalcm == 1) {
+            int nseg = 0;
+            for (int base = 0; base < nalg; base += WAVE) {
+                int a = base + lane;
+                int g = a < nalg ? M.group_of[gvl[a]] : -1;
+                int gprev = (a > 0 && a < nalg) ? M.group_of[gvl[a - 1]] : -2;
+                bool st = (a < nalg) && (a == 0 || g != gprev);
+                unsigned long long m = __ballot(st);
+                int s = nseg + __popcll(m & ((2ull << lane) - 1ull)) - 1;
+                if (a < nalg) { segof[a] = s; if (st) { seggr[s] = g; segst[s] = a; } }
+                nseg += __popcll(m);
+            }
+            if (lane == 0) { segst[nseg] = nalg; sh[6] = nseg; }
+            __syncthreads();
+            for (int q = lane; q < nseg; q += WAVE) {
+                const int g = seggr[q];
+                segnc[q] = M.gfirst[g + 1] - M.gfirst[g];
+            }
+            const int npair = (nseg * nseg + nseg) / 2;
+            if (npair > P.max_pair) { if (lane == 0) sh[5] = 2; }
+            else {
+                const int64_t p0 = M.sparse ? R.rec_pb[r] : 0;
+                for (int t = lane; t < npair; t += WAVE) {
                     int sa = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
                     while ((sa * sa + sa) / 2 > t) sa--;
                     while (((sa + 1) * (sa + 1) + (sa + 1)) / 2 <= t) sa++;
                     const int sb = t - (sa * sa + sa) / 2;
-                    const int blk = R.pair_blk[p0 + t];
-                    const int ga = seggr[sa], gb = seggr[sb];
-                    const int fa = M.gfirst[ga], fb = M.gfirst[gb];
-                    const int nc = M.gfirst[gb + 1] - fb;
-                    pbase[t] = blk < 0 ? INT64_MIN : M.boff[blk] - (int64_t)fa * nc - fb;
+                    const int na = segst[sa + 1] - segst[sa], nb = segst[sb + 1] - segst[sb];
+                    pcnt[t] = sa == sb ? (na * na + na) / 2 : na * nb;
+                    if (M.sparse) {
+                        const int blk = R.pair_blk[p0 + t];
+                        const int ga = seggr[sa], gb = seggr[sb];
+                        const int fa = M.gfirst[ga], fb = M.gfirst[gb];
+                        const int nc = M.gfirst[gb + 1] - fb;
+                        pbase[t] = blk < 0 ? INT64_MIN : M.boff[blk] - (int64_t)fa * nc - fb;
+                    }
                 }
             }
         }
@@ -421,12 +441,55 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             for (int j = 1; j <= ng; j++) {
                 double gj = wval<T>(R, sg + j);
                 int pj = R.widx[sg + j];
-                int vj = gv[pj];
-                if (gj == 0.0) continue;
-                if (vj > 0) {
+                if (gj == 0.0 || gv[pj] <= 0) continue;
+                int aj = gpos[pj];
+                int slot = atomicAdd(&qn[aj], 1);
+                if (slot < QE) { qe[aj * QE + slot] = i; qg[aj * QE + slot] = gj; }
+            }
+        }
+        summ = wave_sum(s_chi);
+        __syncthreads();
+        // order each variable's equations; duplicates (a label twice in one equation) or more
+        // than QE equations select the per-equation path
+        for (int a = lane; a < nalg; a += WAVE) {
+            const int n = qn[a];
+            if (n > QE) { sh[5] = 1; continue; }
+            for (int x = 1; x < n; x++)
+                for (int y = x; y > 0 && qe[a * QE + y - 1] >= qe[a * QE + y]; y--) {
+                    if (qe[a * QE + y - 1] == qe[a * QE + y]) sh[5] = 1;
+                    int te = qe[a * QE + y]; qe[a * QE + y] = qe[a * QE + y - 1]; qe[a * QE + y - 1] = te;
+                    double tg = qg[a * QE + y]; qg[a * QE + y] = qg[a * QE + y - 1]; qg[a * QE + y - 1] = tg;
+                }
+        }
+        __syncthreads();
+        const bool perq = sh[5] != 0;
+        if (!perq) {
+            // vector and counter: one update per record variable (equations in order)
+            for (int a = lane; a < nalg; a += WAVE) {
+                const int n = qn[a];
+                double v = 0.0;
+                for (int x = 0; x < n; x++) {
+                    const int i = qe[a * QE + x];
+                    v = v + dw1 * lw[i] * rm[i] * qg[a * QE + x];
+                }
+                atomicAdd(&M.vec[gvl[a] - 1], v);
+                atomicAdd(&M.cnt[gvl[a] - 1], (unsigned long long)n);
+            }
+        } else {
+            for (int i = lane; i < neq; i += WAVE) {
+                int64_t sl = R.eq_sl[e0 + i];
+                uint32_t en = R.eq_n[e0 + i];
+                int nl = en & 0xffff, ng = en >> 16;
+                int64_t sg = sl + nl + 1;
+                const double wght = lw[i], rmeas = rm[i];
+                for (int j = 1; j <= ng; j++) {
+                    double gj = wval<T>(R, sg + j);
+                    int pj = R.widx[sg + j];
+                    int vj = gv[pj];
+                    if (gj == 0.0 || vj <= 0) continue;
                     atomicAdd(&M.vec[vj - 1], dw1 * wght * rmeas * gj);
                     atomicAdd(&M.cnt[vj - 1], 1ull);
-                    if (o.icalcm == 1) {
+                    if (o.icalcm == 1 && sh[5] == 1) {
                         int aj = gpos[pj];
                         for (int k = 1; k <= j; k++) {
                             int pk_ = R.widx[sg + k];
@@ -442,8 +505,13 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 }
             }
         }
-        summ = wave_sum(s_chi);
-        __syncthreads();
+        if (o.icalcm == 1 && sh[5] == 2) {
+            // more segment pairs than the LDS plan holds: cannot happen for records compiled
+            // with this structure; report instead of writing out of bounds
+            if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            __syncthreads();
+            continue;
+        }
         if (o.icalcm == 1) {
             // G = sum over equations of sqrt(dw) w g l^T, equations in order (4206-4219)
             for (int i = 0; i < neq; i++) {
@@ -472,19 +540,44 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 }
             }
             __syncthreads();
-            // Schur complement: element (a>=b) -= sum_k Gh[a,k] G[b,k]
-            // packed (a, b) enumeration: lane starts at element `lane`, then steps by 64
-            const int ngg = (nalg * nalg + nalg) / 2;
-            int a = (int)((sqrt(8.0 * lane + 1.0) - 1.0) * 0.5);
-            while ((a * a + a) / 2 > lane) a--;
-            while (((a + 1) * (a + 1) + (a + 1)) / 2 <= lane) a++;
-            int b = lane - (a * a + a) / 2;
-            for (int t = lane; t < ngg; t += WAVE) {
+            // the record's update, block by block in storage order (consecutive lanes write
+            // consecutive matrix elements): element (a >= b) += sum_e dw w g_ea g_eb
+            // (common equations of a and b) - sum_k Gh[a,k] G[b,k]
+            const int nseg = sh[6];
+            const int npair = (nseg * nseg + nseg) / 2;
+            int p = 0, e = lane, sa = 0, sb = 0;
+            while (p < npair && e >= pcnt[p]) { e -= pcnt[p]; p++; if (++sb > sa) { sa++; sb = 0; } }
+            while (p < npair) {
+                const int a0 = segst[sa], b0 = segst[sb];
+                int ra, rb;
+                if (sa != sb) {
+                    const int nb = segst[sb + 1] - b0;
+                    ra = e / nb;
+                    rb = e - ra * nb;
+                } else {
+                    ra = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
+                    while ((ra * ra + ra) / 2 > e) ra--;
+                    while (((ra + 1) * (ra + 1) + (ra + 1)) / 2 <= e) ra++;
+                    rb = e - (ra * ra + ra) / 2;
+                }
+                const int a = a0 + ra, b = b0 + rb;
                 double c = 0.0;
-                for (int k = 0; k < nalc; k++) c = c + Gh[a * nalc + k] * G[b * nalc + k];
-                mat_add(M, gvl[a], gvl[b], -c, segof, segnc, pbase, a, b);
-                b += WAVE;
-                while (b > a) { b -= a + 1; a++; }
+                if (!perq) {
+                    const int na = qn[a], nbq = qn[b];
+                    for (int x = 0; x < na; x++) {
+                        const int i = qe[a * QE + x];
+                        for (int y = 0; y < nbq; y++)
+                            if (qe[b * QE + y] == i) c = c + dw1 * lw[i] * qg[a * QE + x] * qg[b * QE + y];
+                    }
+                }
+                double d = 0.0;
+                for (int k = 0; k < nalc; k++) d = d + Gh[a * nalc + k] * G[b * nalc + k];
+                c = c - d;
+                const int va = gvl[a], vb = gvl[b];
+                if (!M.sparse) atomicAdd(&M.mat[pk64(va, vb)], c);
+                else if (pbase[p] != INT64_MIN) atomicAdd(&M.mat[pbase[p] + (int64_t)va * segnc[sb] + vb], c);
+                e += WAVE;
+                while (p < npair && e >= pcnt[p]) { e -= pcnt[p]; p++; if (++sb > sa) { sa++; sb = 0; } }
             }
         }
         if (lane == 0) { out.status[r] = 0; out.chi2[r] = summ * dw1; out.ndf[r] = ndf; }
@@ -512,8 +605,9 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
 int lds_bytes(LdsPlan &P)
 {
     int npk = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
-    size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max + 2 * (size_t)P.max_var * P.nalc_max;
-    size_t i = 2 * (size_t)P.max_gl + 4 * (size_t)P.max_var + (P.nalc_max + 1) + 8;
+    size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max + 2 * (size_t)P.max_var * P.nalc_max +
+               (size_t)P.max_var * QE;
+    size_t i = 2 * (size_t)P.max_gl + 6 * (size_t)P.max_var + 1 + P.max_pair + (size_t)P.max_var * QE + (P.nalc_max + 1) + 8;
     size_t b = d * 8 + (size_t)P.max_pair * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;

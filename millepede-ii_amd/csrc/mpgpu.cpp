@@ -799,7 +799,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->plan.max_eq = max_eq;
     c->plan.nalc_max = std::max(1, std::min(max_nalc, 64));
     c->plan.max_var = max_var;
-    c->plan.max_pair = sparse ? (int32_t)c->max_pair : 1;
+    c->plan.max_pair = (int32_t)std::max<int64_t>(1, c->max_pair);
     lds_bytes(c->plan);
     if (c->plan.bytes > 160 * 1024)
         throw MpError(30, "record too large for the record kernel (LDS " + std::to_string(c->plan.bytes) + " bytes)");
