@@ -40,21 +40,14 @@ template <> __device__ __forceinline__ double wval<double>(const DevRecords &R, 
 __device__ __forceinline__ int pk(int i, int j) { return (i * i - i) / 2 + j - 1; }
 __device__ __forceinline__ int64_t pk64(int64_t i, int64_t j) { return (i * i - i) / 2 + j - 1; }
 
-// element (va >= vb, 1-based variable indices) of the global matrix; pos_a/pos_b are the
-// positions of va/vb in the record's variable list.  Group-block storage: the address of
-// (va, vb) in the block of segment pair (sa, sb) is pbase[pair] + va * segnc[sb] + vb, with
-// pbase = boff[blk] - first(ga) * nc - first(gb) precomputed per record in LDS.
-__device__ __forceinline__ void mat_add(const DevModel &M, int va, int vb, double v, const int32_t *seg_of,
-                                        const int32_t *segnc, const int64_t *pbase, int pos_a, int pos_b)
+// a rejected record contributes nothing: its slot of the update buffer is cleared
+__device__ __forceinline__ void zero_update(const DevRecords &R, const DevModel &M, const PassOpts &o, int64_t r, int lane)
 {
-    if (!M.sparse) {
-        atomicAdd(&M.mat[pk64(va, vb)], v);
-    } else {
-        int sa = seg_of[pos_a], sb = seg_of[pos_b];
-        int64_t base = pbase[(sa * sa + sa) / 2 + sb];
-        if (base == INT64_MIN) return;             // pair without a block (not stored)
-        atomicAdd(&M.mat[base + (int64_t)va * segnc[sb] + vb], v);
-    }
+    if (o.icalcm != 1 || !M.sparse || R.cbuf == nullptr) return;
+    const int64_t off = R.rec_cb[r];
+    if (off < 0) return;
+    const int n = R.rec_cn[r];
+    for (int t = lane; t < n; t += WAVE) R.cbuf[off + t] = 0.0;
 }
 
 template <typename T>
@@ -80,16 +73,11 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *G = co + P.nalc_max;             // [max_var * nalc_max]
     double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max]
     double *qg = Gh + (size_t)P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
-    int64_t *pbase = (int64_t *)(qg + (size_t)P.max_var * QE); // [max_pair] block base per segment pair
-    int32_t *gv = (int32_t *)(pbase + P.max_pair); // [max_gl] var index
+    int32_t *gv = (int32_t *)(qg + (size_t)P.max_var * QE); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
     int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
-    int32_t *segof = gvl + P.max_var;        // [max_var] segment of each record variable
-    int32_t *seggr = segof + P.max_var;      // [max_var] group of each segment
-    int32_t *segnc = seggr + P.max_var;      // [max_var] parameters in the group of each segment
-    int32_t *segst = segnc + P.max_var;      // [max_var+1] first record variable of each segment
-    int32_t *pcnt = segst + P.max_var + 1;   // [max_pair] elements of each segment pair
-    int32_t *qn = pcnt + P.max_pair;         // [max_var] equations of each record variable
+    int32_t *segst = gvl + P.max_var;        // [max_var+1] first record variable of each segment
+    int32_t *qn = segst + P.max_var + 1;     // [max_var] equations of each record variable
     int32_t *qe = qn + P.max_var;            // [max_var*QE] their indices (ascending)
     int32_t *nxt = qe + P.max_var * QE;      // [nalc_max+1] pivot list
     int32_t *sh = nxt + P.nalc_max + 1;      // scalars
@@ -105,6 +93,7 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         const double dw2 = sqrt(dw1);
         if (neq > P.max_eq || ngl > P.max_gl) {
             if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            zero_update(R, M, o, r, lane);
             continue;
         }
         // ---- distinct labels: parameter value, variable index, position among variables ----
@@ -127,6 +116,7 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         }
         if (nalg > P.max_var) {
             if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            zero_update(R, M, o, r, lane);
             continue;
         }
         __syncthreads();
@@ -147,11 +137,13 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         nalc = wave_max(nalc);
         if (nalc <= 0) {
             if (lane == 0) { out.status[r] = 7; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            zero_update(R, M, o, r, lane);
             __syncthreads();
             continue;
         }
         if (nalc > P.nalc_max) {
             if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            zero_update(R, M, o, r, lane);
             __syncthreads();
             continue;
         }
@@ -363,20 +355,26 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         }
         if (status != 0) {
             if (lane == 0) { out.status[r] = (int8_t)status; out.chi2[r] = chi_add * dw1; out.ndf[r] = ndf_add; }
+            zero_update(R, M, o, r, lane);
             __syncthreads();
             continue;
         }
         // ---- fourth loop: vector, counters, g g^T, G (4162-4229) ----
-        // Every record variable a collects the equations it occurs in (up to QE, ascending), so
-        // that the vector, the counters and g g^T are formed per variable and merged with the
-        // Schur complement into one FP64 atomic per matrix element.  Records with a variable in
-        // more than QE equations, or twice in one equation, take the per-equation path instead.
+        // Every record variable collects the equations it occurs in (up to QE, ascending): the
+        // vector and the counters get one update per variable.  Full storage: g g^T per
+        // equation and the Schur complement per element with FP64 atomics.  Group-block
+        // storage: the record's complete update (g g^T of the common equations minus the Schur
+        // complement) is written block by block into the update buffer, to be summed per matrix
+        // block by k_block_gather.  A variable in more than QE equations, or twice in one
+        // equation, makes the record take the per-equation path for the vector (and, with
+        // group-block storage, an equation scan for g g^T).
+        const bool twophase = M.sparse != 0;   // the host always provides the update buffer then
         if (o.icalcm == 1)
             for (int t = lane; t < nalg * nalc; t += WAVE) G[t] = 0.0;
         for (int t = lane; t < nalg; t += WAVE) qn[t] = 0;
         if (lane == 0) sh[5] = 0;
-        // segments: runs of record variables in the same parameter group (consecutive variables)
-        if (o.icalcm == 1) {
+        if (o.icalcm == 1 && twophase) {
+            // segments: runs of record variables in the same parameter group
             int nseg = 0;
             for (int base = 0; base < nalg; base += WAVE) {
                 int a = base + lane;
@@ -385,35 +383,10 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 bool st = (a < nalg) && (a == 0 || g != gprev);
                 unsigned long long m = __ballot(st);
                 int s = nseg + __popcll(m & ((2ull << lane) - 1ull)) - 1;
-                if (a < nalg) { segof[a] = s; if (st) { seggr[s] = g; segst[s] = a; } }
+                if (a < nalg && st) segst[s] = a;
                 nseg += __popcll(m);
             }
             if (lane == 0) { segst[nseg] = nalg; sh[6] = nseg; }
-            __syncthreads();
-            for (int q = lane; q < nseg; q += WAVE) {
-                const int g = seggr[q];
-                segnc[q] = M.gfirst[g + 1] - M.gfirst[g];
-            }
-            const int npair = (nseg * nseg + nseg) / 2;
-            if (npair > P.max_pair) { if (lane == 0) sh[5] = 2; }
-            else {
-                const int64_t p0 = M.sparse ? R.rec_pb[r] : 0;
-                for (int t = lane; t < npair; t += WAVE) {
-                    int sa = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-                    while ((sa * sa + sa) / 2 > t) sa--;
-                    while (((sa + 1) * (sa + 1) + (sa + 1)) / 2 <= t) sa++;
-                    const int sb = t - (sa * sa + sa) / 2;
-                    const int na = segst[sa + 1] - segst[sa], nb = segst[sb + 1] - segst[sb];
-                    pcnt[t] = sa == sb ? (na * na + na) / 2 : na * nb;
-                    if (M.sparse) {
-                        const int blk = R.pair_blk[p0 + t];
-                        const int ga = seggr[sa], gb = seggr[sb];
-                        const int fa = M.gfirst[ga], fb = M.gfirst[gb];
-                        const int nc = M.gfirst[gb + 1] - fb;
-                        pbase[t] = blk < 0 ? INT64_MIN : M.boff[blk] - (int64_t)fa * nc - fb;
-                    }
-                }
-            }
         }
         __syncthreads();
         double s_chi = 0.0;
@@ -441,16 +414,27 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             for (int j = 1; j <= ng; j++) {
                 double gj = wval<T>(R, sg + j);
                 int pj = R.widx[sg + j];
-                if (gj == 0.0 || gv[pj] <= 0) continue;
+                int vj = gv[pj];
+                if (gj == 0.0 || vj <= 0) continue;
                 int aj = gpos[pj];
                 int slot = atomicAdd(&qn[aj], 1);
                 if (slot < QE) { qe[aj * QE + slot] = i; qg[aj * QE + slot] = gj; }
+                if (o.icalcm == 1 && !M.sparse) {
+                    // full storage: g g^T of this equation
+                    for (int k = 1; k <= j; k++) {
+                        int pk_ = R.widx[sg + k];
+                        int vk = gv[pk_];
+                        if (vk > 0) {
+                            double val = dw1 * wght * gj * wval<T>(R, sg + k);
+                            if (vj >= vk) atomicAdd(&M.mat[pk64(vj, vk)], val);
+                            else atomicAdd(&M.mat[pk64(vk, vj)], val);
+                        }
+                    }
+                }
             }
         }
         summ = wave_sum(s_chi);
         __syncthreads();
-        // order each variable's equations; duplicates (a label twice in one equation) or more
-        // than QE equations select the per-equation path
         for (int a = lane; a < nalg; a += WAVE) {
             const int n = qn[a];
             if (n > QE) { sh[5] = 1; continue; }
@@ -464,7 +448,6 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         __syncthreads();
         const bool perq = sh[5] != 0;
         if (!perq) {
-            // vector and counter: one update per record variable (equations in order)
             for (int a = lane; a < nalg; a += WAVE) {
                 const int n = qn[a];
                 double v = 0.0;
@@ -481,36 +464,14 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 uint32_t en = R.eq_n[e0 + i];
                 int nl = en & 0xffff, ng = en >> 16;
                 int64_t sg = sl + nl + 1;
-                const double wght = lw[i], rmeas = rm[i];
                 for (int j = 1; j <= ng; j++) {
                     double gj = wval<T>(R, sg + j);
-                    int pj = R.widx[sg + j];
-                    int vj = gv[pj];
+                    int vj = gv[R.widx[sg + j]];
                     if (gj == 0.0 || vj <= 0) continue;
-                    atomicAdd(&M.vec[vj - 1], dw1 * wght * rmeas * gj);
+                    atomicAdd(&M.vec[vj - 1], dw1 * lw[i] * rm[i] * gj);
                     atomicAdd(&M.cnt[vj - 1], 1ull);
-                    if (o.icalcm == 1 && sh[5] == 1) {
-                        int aj = gpos[pj];
-                        for (int k = 1; k <= j; k++) {
-                            int pk_ = R.widx[sg + k];
-                            int vk = gv[pk_];
-                            if (vk > 0) {
-                                double val = dw1 * wght * gj * wval<T>(R, sg + k);
-                                int ak = gpos[pk_];
-                                if (vj >= vk) mat_add(M, vj, vk, val, segof, segnc, pbase, aj, ak);
-                                else mat_add(M, vk, vj, val, segof, segnc, pbase, ak, aj);
-                            }
-                        }
-                    }
                 }
             }
-        }
-        if (o.icalcm == 1 && sh[5] == 2) {
-            // more segment pairs than the LDS plan holds: cannot happen for records compiled
-            // with this structure; report instead of writing out of bounds
-            if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
-            __syncthreads();
-            continue;
         }
         if (o.icalcm == 1) {
             // G = sum over equations of sqrt(dw) w g l^T, equations in order (4206-4219)
@@ -540,48 +501,117 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 }
             }
             __syncthreads();
-            // the record's update, block by block in storage order (consecutive lanes write
-            // consecutive matrix elements): element (a >= b) += sum_e dw w g_ea g_eb
-            // (common equations of a and b) - sum_k Gh[a,k] G[b,k]
-            const int nseg = sh[6];
-            const int npair = (nseg * nseg + nseg) / 2;
-            int p = 0, e = lane, sa = 0, sb = 0;
-            while (p < npair && e >= pcnt[p]) { e -= pcnt[p]; p++; if (++sb > sa) { sa++; sb = 0; } }
-            while (p < npair) {
-                const int a0 = segst[sa], b0 = segst[sb];
-                int ra, rb;
-                if (sa != sb) {
-                    const int nb = segst[sb + 1] - b0;
-                    ra = e / nb;
-                    rb = e - ra * nb;
-                } else {
-                    ra = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
-                    while ((ra * ra + ra) / 2 > e) ra--;
-                    while (((ra + 1) * (ra + 1) + (ra + 1)) / 2 <= e) ra++;
-                    rb = e - (ra * ra + ra) / 2;
+            if (!twophase) {
+                // full storage: element (a>=b) -= sum_k Gh[a,k] G[b,k]; packed (a, b) enumeration,
+                // lane starts at element `lane`, then steps by 64
+                const int ngg = (nalg * nalg + nalg) / 2;
+                int a = (int)((sqrt(8.0 * lane + 1.0) - 1.0) * 0.5);
+                while ((a * a + a) / 2 > lane) a--;
+                while (((a + 1) * (a + 1) + (a + 1)) / 2 <= lane) a++;
+                int b = lane - (a * a + a) / 2;
+                for (int t = lane; t < ngg; t += WAVE) {
+                    double c = 0.0;
+                    for (int k = 0; k < nalc; k++) c = c + Gh[a * nalc + k] * G[b * nalc + k];
+                    atomicAdd(&M.mat[pk64(gvl[a], gvl[b])], -c);
+                    b += WAVE;
+                    while (b > a) { b -= a + 1; a++; }
                 }
-                const int a = a0 + ra, b = b0 + rb;
-                double c = 0.0;
-                if (!perq) {
-                    const int na = qn[a], nbq = qn[b];
-                    for (int x = 0; x < na; x++) {
-                        const int i = qe[a * QE + x];
-                        for (int y = 0; y < nbq; y++)
-                            if (qe[b * QE + y] == i) c = c + dw1 * lw[i] * qg[a * QE + x] * qg[b * QE + y];
+            } else {
+                // group-block storage: the record's update per segment pair (sa >= sb), r-major,
+                // lower triangle for sa == sb, contiguous in the update buffer
+                const int nseg = sh[6];
+                double *dst = R.cbuf + R.rec_cb[r];
+                int sa = 0, sb = 0, e = lane;
+                int64_t off = 0;                            // first value of the current pair
+                auto pcount = [&](int x, int y) {
+                    const int nx = segst[x + 1] - segst[x], ny = segst[y + 1] - segst[y];
+                    return x == y ? (nx * nx + nx) / 2 : nx * ny;
+                };
+                while (sa < nseg) {
+                    const int cntp = pcount(sa, sb);
+                    if (e >= cntp) {
+                        e -= cntp;
+                        off += cntp;
+                        if (++sb > sa) { sa++; sb = 0; }
+                        continue;
                     }
+                    const int a0 = segst[sa], b0 = segst[sb];
+                    int ra, rb;
+                    if (sa != sb) {
+                        const int nb = segst[sb + 1] - b0;
+                        ra = e / nb;
+                        rb = e - ra * nb;
+                    } else {
+                        ra = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
+                        while ((ra * ra + ra) / 2 > e) ra--;
+                        while (((ra + 1) * (ra + 1) + (ra + 1)) / 2 <= e) ra++;
+                        rb = e - (ra * ra + ra) / 2;
+                    }
+                    const int a = a0 + ra, b = b0 + rb;
+                    double c = 0.0;
+                    if (!perq) {
+                        const int na = qn[a], nbq = qn[b];
+                        for (int x = 0; x < na; x++) {
+                            const int i = qe[a * QE + x];
+                            for (int y = 0; y < nbq; y++)
+                                if (qe[b * QE + y] == i) c = c + dw1 * lw[i] * qg[a * QE + x] * qg[b * QE + y];
+                        }
+                    } else {
+                        // equation scan: g g^T of every equation holding both variables
+                        for (int i = 0; i < neq; i++) {
+                            const int64_t sl = R.eq_sl[e0 + i];
+                            const uint32_t en = R.eq_n[e0 + i];
+                            const int64_t sg = sl + (en & 0xffff) + 1;
+                            const int ng = en >> 16;
+                            for (int j = 1; j <= ng; j++) {
+                                if (gpos[R.widx[sg + j]] != a) continue;
+                                const double gj = wval<T>(R, sg + j);
+                                if (gj == 0.0) continue;
+                                for (int k = 1; k <= ng; k++) {
+                                    if (gpos[R.widx[sg + k]] != b) continue;
+                                    // pede adds each (j, k <= j) word pair once
+                                    if (a == b ? k > j : false) continue;
+                                    c = c + dw1 * lw[i] * gj * wval<T>(R, sg + k);
+                                }
+                            }
+                        }
+                    }
+                    double d = 0.0;
+                    for (int k = 0; k < nalc; k++) d = d + Gh[a * nalc + k] * G[b * nalc + k];
+                    dst[off + e] = c - d;
+                    e += WAVE;
                 }
-                double d = 0.0;
-                for (int k = 0; k < nalc; k++) d = d + Gh[a * nalc + k] * G[b * nalc + k];
-                c = c - d;
-                const int va = gvl[a], vb = gvl[b];
-                if (!M.sparse) atomicAdd(&M.mat[pk64(va, vb)], c);
-                else if (pbase[p] != INT64_MIN) atomicAdd(&M.mat[pbase[p] + (int64_t)va * segnc[sb] + vb], c);
-                e += WAVE;
-                while (p < npair && e >= pcnt[p]) { e -= pcnt[p]; p++; if (++sb > sa) { sa++; sb = 0; } }
             }
         }
         if (lane == 0) { out.status[r] = 0; out.chi2[r] = summ * dw1; out.ndf[r] = ndf; }
         __syncthreads();
+    }
+}
+
+// Block gather of the two-phase matrix build: every stored block adds the values that the
+// records of the batch wrote for it (in processing order).  One wavefront per block; lane e
+// handles element e of the r x c block (row-major), diagonal blocks read the lower triangle
+// the records wrote packed.
+__global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg,
+                                                      const int32_t *bcol, const int32_t *gfirst, const int64_t *ptr,
+                                                      const int64_t *src, const double *cbuf, double *mat)
+{
+    const int64_t k = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= nblock) return;
+    const int64_t i0 = ptr[k], i1 = ptr[k + 1];
+    if (i1 <= i0) return;
+    const int ga = brg[k], gb = bcol[k];
+    const int nr = gfirst[ga + 1] - gfirst[ga], nc = gfirst[gb + 1] - gfirst[gb];
+    const bool diag = ga == gb;
+    double *B = mat + boff[k];
+    for (int e = lane; e < nr * nc; e += 64) {
+        const int rr = e / nc, cc = e - rr * nc;
+        if (diag && cc > rr) continue;
+        const int se = diag ? (rr * rr + rr) / 2 + cc : e;
+        double acc = 0.0;
+        for (int64_t q = i0; q < i1; q++) acc += cbuf[src[q] + se];
+        B[e] += acc;
     }
 }
 
@@ -602,13 +632,23 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
     return hipGetLastError();
 }
 
+hipError_t launch_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg, const int32_t *bcol,
+                               const int32_t *gfirst, const int64_t *ptr, const int64_t *src, const double *cbuf,
+                               double *mat, hipStream_t s)
+{
+    if (nblock <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_block_gather, dim3((unsigned)((nblock + 3) / 4)), dim3(256), 0, s, nblock, boff, brg, bcol, gfirst,
+                       ptr, src, cbuf, mat);
+    return hipGetLastError();
+}
+
 int lds_bytes(LdsPlan &P)
 {
     int npk = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
     size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max + 2 * (size_t)P.max_var * P.nalc_max +
                (size_t)P.max_var * QE;
-    size_t i = 2 * (size_t)P.max_gl + 6 * (size_t)P.max_var + 1 + P.max_pair + (size_t)P.max_var * QE + (P.nalc_max + 1) + 8;
-    size_t b = d * 8 + (size_t)P.max_pair * 8 + i * 4;
+    size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * QE + (P.nalc_max + 1) + 8;
+    size_t b = d * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;
 }

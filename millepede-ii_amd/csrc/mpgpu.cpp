@@ -7,6 +7,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <numeric>
 #include <omp.h>
 
@@ -74,6 +75,14 @@ struct mpgpu_ctx {
     DBuf<double> d_band_scratch;
     BandPlan bplan{};
     int64_t nband = 0;
+    // two-phase matrix build (group-block storage): record updates -> buffer -> block gather
+    struct Batch { int64_t q0, q1, base, len, inc0; };
+    std::vector<Batch> batches;
+    DBuf<int64_t> d_rec_cb;      // per record: offset of its update in the batch buffer (-1: none)
+    DBuf<int32_t> d_rec_cn;      // per record: number of update values
+    DBuf<int64_t> d_inc_ptr;     // per batch: (nblock+1) pointers into d_inc_src
+    DBuf<int64_t> d_inc_src;     // buffer offsets of the records' values for each block
+    DBuf<double> d_cbuf;         // update buffer (largest batch)
     int band_grid = 0;
     int band_lfitbb = -1;
     int64_t nrec = 0;
@@ -843,9 +852,118 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
     }
+    // ---- two-phase build index (group-block storage) ----
+    // Each record writes its update of every segment pair (r-major block, lower triangle for the
+    // diagonal pairs) contiguously into a buffer; the block gather sums, for every stored block,
+    // the values of the records that touch it, in processing order.  Records are taken in
+    // batches whose updates fit the buffer budget.
+    std::vector<int64_t> rec_cb(nrec, -1);
+    std::vector<int32_t> rec_cn(nrec, 0);
+    std::vector<int64_t> inc_ptr_all, inc_src_all;
+    c->batches.clear();
+    if (sparse) {
+        std::vector<int64_t> cnt_r(nrec, 0);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t r = 0; r < nrec; r++) {
+            if (binfo[r]) continue;                 // bordered-band records update the matrix directly
+            std::vector<int32_t> sz;
+            int lastg = -1;
+            for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
+                int v = c->var_index[gl_tix[q]];
+                if (v <= 0) continue;
+                if (c->group_of[v] != lastg) { lastg = c->group_of[v]; sz.push_back(0); }
+                sz.back()++;
+            }
+            int64_t n = 0;
+            for (size_t a = 0; a < sz.size(); a++) {
+                n += (int64_t)sz[a] * (sz[a] + 1) / 2;
+                for (size_t b = 0; b < a; b++) n += (int64_t)sz[a] * sz[b];
+            }
+            cnt_r[r] = n;
+        }
+        const char *env = std::getenv("MPGPU_UPDATE_BUFFER_GB");
+        const double gb = env ? std::atof(env) : 48.0;
+        const int64_t budget = std::max<int64_t>(1 << 20, (int64_t)(gb * (1ull << 30) / 8.0));
+        int64_t q0 = 0, acc = 0;
+        for (int64_t q = 0; q < nrec; q++) {
+            const int64_t r = order[q];
+            if (acc + cnt_r[r] > budget && q > q0) {
+                c->batches.push_back({q0, q, 0, acc, 0});
+                q0 = q;
+                acc = 0;
+            }
+            rec_cb[r] = cnt_r[r] > 0 ? acc : -1;
+            rec_cn[r] = (int32_t)cnt_r[r];
+            acc += cnt_r[r];
+        }
+        if (nrec > q0 || c->batches.empty()) c->batches.push_back({q0, nrec, 0, acc, 0});
+        // incidences: per batch, per block, buffer offsets of the (record, pair) values
+        const int64_t nb = c->nblock, nbat = (int64_t)c->batches.size();
+        inc_ptr_all.assign((size_t)nbat * (nb + 1), 0);
+        int64_t inc_total = 0;
+        for (int64_t bt = 0; bt < nbat; bt++) {
+            auto &B = c->batches[bt];
+            int64_t *ptr = inc_ptr_all.data() + (size_t)bt * (nb + 1);
+            std::vector<std::atomic<int64_t>> cntb(nb);
+            for (auto &x : cntb) x.store(0, std::memory_order_relaxed);
+#pragma omp parallel for schedule(dynamic, 256)
+            for (int64_t q = B.q0; q < B.q1; q++) {
+                const int64_t r = order[q];
+                if (rec_cb[r] < 0) continue;
+                for (int64_t p = rec_pb[r]; p < rec_pb[r + 1]; p++) cntb[pair_blk[p]].fetch_add(1, std::memory_order_relaxed);
+            }
+            ptr[0] = 0;
+            for (int64_t k = 0; k < nb; k++) ptr[k + 1] = ptr[k] + cntb[k].load(std::memory_order_relaxed);
+            B.inc0 = inc_total;
+            inc_src_all.resize((size_t)(inc_total + ptr[nb]));
+            int64_t *src = inc_src_all.data() + inc_total;
+            for (int64_t k = 0; k < nb; k++) cntb[k].store(ptr[k], std::memory_order_relaxed);
+#pragma omp parallel for schedule(dynamic, 256)
+            for (int64_t q = B.q0; q < B.q1; q++) {
+                const int64_t r = order[q];
+                if (rec_cb[r] < 0) continue;
+                // segment sizes again, to place every pair's values
+                std::vector<int32_t> sz;
+                int lastg = -1;
+                for (int64_t g = rec_gl[r]; g < rec_gl[r + 1]; g++) {
+                    int v = c->var_index[gl_tix[g]];
+                    if (v <= 0) continue;
+                    if (c->group_of[v] != lastg) { lastg = c->group_of[v]; sz.push_back(0); }
+                    sz.back()++;
+                }
+                int64_t off = rec_cb[r];
+                int64_t p = rec_pb[r];
+                for (size_t a = 0; a < sz.size(); a++)
+                    for (size_t b = 0; b <= a; b++) {
+                        src[cntb[pair_blk[p]].fetch_add(1, std::memory_order_relaxed)] = off;
+                        off += a == b ? (int64_t)sz[a] * (sz[a] + 1) / 2 : (int64_t)sz[a] * sz[b];
+                        p++;
+                    }
+            }
+            // processing order within every block (deterministic summation order)
+#pragma omp parallel for schedule(dynamic, 4096)
+            for (int64_t k = 0; k < nb; k++) std::sort(src + ptr[k], src + ptr[k + 1]);
+            inc_total += ptr[nb];
+        }
+    }
     // ---- device upload ----
     hipStream_t s = c->stream;
     c->d_order.upload(order, s);
+    if (sparse) {
+        c->d_rec_cb.upload(rec_cb, s);
+        c->d_rec_cn.upload(rec_cn, s);
+        c->d_inc_ptr.upload(inc_ptr_all, s);
+        c->d_inc_src.upload(inc_src_all, s);
+        int64_t mx = 1;
+        for (auto &B : c->batches) mx = std::max(mx, B.len);
+        c->d_cbuf.alloc((size_t)mx);
+    } else {
+        c->d_rec_cb.release();
+        c->d_rec_cn.release();
+        c->d_inc_ptr.release();
+        c->d_inc_src.release();
+        c->d_cbuf.release();
+    }
     if (c->nband > 0) {
         std::vector<int64_t> ids;
         ids.reserve(c->nband);
@@ -979,6 +1097,10 @@ static DevRecords records_of(mpgpu_ctx *c)
     R.rec_w = nullptr;
     R.order = c->d_order.p;
     R.band = c->nband > 0 ? c->d_band_info.p : nullptr;
+    const bool two = c->opt.matsto == MPGPU_SPARSE;
+    R.rec_cb = two ? c->d_rec_cb.p : nullptr;
+    R.rec_cn = two ? c->d_rec_cn.p : nullptr;
+    R.cbuf = two ? c->d_cbuf.p : nullptr;
     R.max_gl = c->plan.max_gl;
     R.max_eq = c->plan.max_eq;
     R.max_nalc = c->plan.nalc_max;
@@ -1031,7 +1153,22 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     DevRecords R = records_of(c);
     int grid = (int)std::min<int64_t>(c->nrec, 256 * 40);
     HIPCHK(hipEventRecord(e1, s));
-    if (c->nrec > 0) HIPCHK(launch_records(R, M, o, out, c->plan, 0, c->nrec, std::max(grid, 1), s));
+    if (c->nrec > 0) {
+        if (icalcm == 1 && c->opt.matsto == MPGPU_SPARSE) {
+            // two-phase build: each batch of records writes its updates, then the blocks gather them
+            for (size_t bt = 0; bt < c->batches.size(); bt++) {
+                const auto &B = c->batches[bt];
+                if (B.q1 <= B.q0) continue;
+                const int g = (int)std::min<int64_t>(B.q1 - B.q0, 256 * 40);
+                HIPCHK(launch_records(R, M, o, out, c->plan, B.q0, B.q1, std::max(g, 1), s));
+                HIPCHK(launch_block_gather(c->nblock, c->d_boff.p, c->d_brg.p, c->d_bcol.p, c->d_gfirst.p,
+                                           c->d_inc_ptr.p + bt * (c->nblock + 1), c->d_inc_src.p + B.inc0, c->d_cbuf.p,
+                                           c->d_mat.p, s));
+            }
+        } else {
+            HIPCHK(launch_records(R, M, o, out, c->plan, 0, c->nrec, std::max(grid, 1), s));
+        }
+    }
     if (c->nband > 0)
         HIPCHK(launch_records_band(R, M, o, out, c->bplan, c->d_band_ids.p, c->nband, c->d_band_info.p,
                                    c->d_band_scratch.p, c->band_grid, s));

@@ -26,6 +26,9 @@ struct DevRecords {
     const int64_t *order;    // processing order of the records (nullptr = file order)
     const int32_t *band;     // per record: 0 full local fit, else bordered-band structure
                              // (mbnd+1) | mbdr << 10 | mside << 20 (record kernel skips these)
+    const int64_t *rec_cb;   // group-block storage: offset of the record's update in cbuf (-1: none)
+    const int32_t *rec_cn;   //   number of update values of the record
+    double *cbuf;            //   update buffer of the current batch (two-phase matrix build)
     int32_t max_gl, max_eq, max_nalc, max_var, max_seg;
 };
 
