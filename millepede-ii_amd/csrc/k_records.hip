@@ -40,6 +40,43 @@ template <> __device__ __forceinline__ double wval<double>(const DevRecords &R, 
 __device__ __forceinline__ int pk(int i, int j) { return (i * i - i) / 2 + j - 1; }
 __device__ __forceinline__ int64_t pk64(int64_t i, int64_t j) { return (i * i - i) / 2 + j - 1; }
 
+// g g^T part of element (a, b) of a record's update: dw * w_i * g_ia * g_ib summed over the
+// equations i holding both variables (from the per-variable equation lists, or, for records
+// that overflow them, by scanning the equations as pede's per-equation update does)
+template <typename T>
+__device__ __forceinline__ double dc1(int a, int b, bool perq, const int32_t *qn, const int32_t *qe, const double *qg,
+                                      const double *lw, double dw1, const DevRecords &R, int64_t e0, int neq,
+                                      const int32_t *gpos)
+{
+    double c = 0.0;
+    if (!perq) {
+        const int na = qn[a], nb = qn[b];
+        for (int x = 0; x < na; x++) {
+            const int i = qe[a * QE + x];
+            for (int y = 0; y < nb; y++)
+                if (qe[b * QE + y] == i) c = c + dw1 * lw[i] * qg[a * QE + x] * qg[b * QE + y];
+        }
+        return c;
+    }
+    for (int i = 0; i < neq; i++) {
+        const int64_t sl = R.eq_sl[e0 + i];
+        const uint32_t en = R.eq_n[e0 + i];
+        const int64_t sg = sl + (en & 0xffff) + 1;
+        const int ng = en >> 16;
+        for (int j = 1; j <= ng; j++) {
+            if (gpos[R.widx[sg + j]] != a) continue;
+            const double gj = wval<T>(R, sg + j);
+            if (gj == 0.0) continue;
+            for (int k = 1; k <= ng; k++) {
+                if (gpos[R.widx[sg + k]] != b) continue;
+                if (a == b && k > j) continue;           // each (j, k <= j) word pair once
+                c = c + dw1 * lw[i] * gj * wval<T>(R, sg + k);
+            }
+        }
+    }
+    return c;
+}
+
 // a rejected record contributes nothing: its slot of the update buffer is cleared
 __device__ __forceinline__ void zero_update(const DevRecords &R, const DevModel &M, const PassOpts &o, int64_t r, int lane)
 {
@@ -73,7 +110,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *G = co + P.nalc_max;             // [max_var * nalc_max]
     double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max]
     double *qg = Gh + (size_t)P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
-    int32_t *gv = (int32_t *)(qg + (size_t)P.max_var * QE); // [max_gl] var index
+    unsigned long long *segm = (unsigned long long *)(qg + (size_t)P.max_var * QE); // [max_var] equations per segment
+    int32_t *gv = (int32_t *)(segm + P.max_var); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
     int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
     int32_t *segst = gvl + P.max_var;        // [max_var+1] first record variable of each segment
@@ -518,68 +556,61 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 }
             } else {
                 // group-block storage: the record's update per segment pair (sa >= sb), r-major,
-                // lower triangle for sa == sb, contiguous in the update buffer
+                // lower triangle for sa == sb, contiguous in the update buffer.  Work unit = one
+                // row of one pair: the lane keeps Gh[a][:] (and a's equations) in registers and
+                // writes the row's values contiguously.
                 const int nseg = sh[6];
                 double *dst = R.cbuf + R.rec_cb[r];
-                int sa = 0, sb = 0, e = lane;
+                // equations touching each segment (bit i for equation i < 64; bit 63 also stands
+                // for all equations >= 63), so that pairs without a common equation skip g g^T
+                for (int q = lane; q < nseg; q += WAVE) {
+                    unsigned long long m = 0ull;
+                    for (int a = segst[q]; a < segst[q + 1]; a++) {
+                        if (perq) { m = ~0ull; break; }
+                        for (int x = 0; x < qn[a]; x++) m |= 1ull << min(qe[a * QE + x], 63);
+                    }
+                    segm[q] = m;
+                }
+                __syncthreads();
+                int sa = 0, sb = 0, u = lane;
                 int64_t off = 0;                            // first value of the current pair
-                auto pcount = [&](int x, int y) {
-                    const int nx = segst[x + 1] - segst[x], ny = segst[y + 1] - segst[y];
-                    return x == y ? (nx * nx + nx) / 2 : nx * ny;
-                };
                 while (sa < nseg) {
-                    const int cntp = pcount(sa, sb);
-                    if (e >= cntp) {
-                        e -= cntp;
-                        off += cntp;
+                    const int na = segst[sa + 1] - segst[sa];
+                    if (u >= na) {
+                        const int nb = segst[sb + 1] - segst[sb];
+                        u -= na;
+                        off += sa == sb ? (na * na + na) / 2 : na * nb;
                         if (++sb > sa) { sa++; sb = 0; }
                         continue;
                     }
-                    const int a0 = segst[sa], b0 = segst[sb];
-                    int ra, rb;
-                    if (sa != sb) {
-                        const int nb = segst[sb + 1] - b0;
-                        ra = e / nb;
-                        rb = e - ra * nb;
-                    } else {
-                        ra = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
-                        while ((ra * ra + ra) / 2 > e) ra--;
-                        while (((ra + 1) * (ra + 1) + (ra + 1)) / 2 <= e) ra++;
-                        rb = e - (ra * ra + ra) / 2;
-                    }
-                    const int a = a0 + ra, b = b0 + rb;
-                    double c = 0.0;
-                    if (!perq) {
-                        const int na = qn[a], nbq = qn[b];
-                        for (int x = 0; x < na; x++) {
-                            const int i = qe[a * QE + x];
-                            for (int y = 0; y < nbq; y++)
-                                if (qe[b * QE + y] == i) c = c + dw1 * lw[i] * qg[a * QE + x] * qg[b * QE + y];
+                    const int ra = u, a = segst[sa] + ra, b0 = segst[sb];
+                    const int nb = segst[sb + 1] - b0;
+                    const int ncol = sa == sb ? ra + 1 : nb;
+                    const int64_t roff = off + (sa == sb ? (ra * ra + ra) / 2 : ra * nb);
+                    const bool common = (segm[sa] & segm[sb]) != 0ull;
+                    if (nalc <= 8) {
+                        double gh[8];
+#pragma unroll
+                        for (int k = 0; k < 8; k++) gh[k] = k < nalc ? Gh[a * nalc + k] : 0.0;
+                        for (int cb = 0; cb < ncol; cb++) {
+                            const int bb = b0 + cb;
+                            double d = 0.0;
+#pragma unroll
+                            for (int k = 0; k < 8; k++)
+                                if (k < nalc) d = d + gh[k] * G[bb * nalc + k];
+                            double c = common ? dc1<T>(a, bb, perq, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
+                            dst[roff + cb] = c - d;
                         }
                     } else {
-                        // equation scan: g g^T of every equation holding both variables
-                        for (int i = 0; i < neq; i++) {
-                            const int64_t sl = R.eq_sl[e0 + i];
-                            const uint32_t en = R.eq_n[e0 + i];
-                            const int64_t sg = sl + (en & 0xffff) + 1;
-                            const int ng = en >> 16;
-                            for (int j = 1; j <= ng; j++) {
-                                if (gpos[R.widx[sg + j]] != a) continue;
-                                const double gj = wval<T>(R, sg + j);
-                                if (gj == 0.0) continue;
-                                for (int k = 1; k <= ng; k++) {
-                                    if (gpos[R.widx[sg + k]] != b) continue;
-                                    // pede adds each (j, k <= j) word pair once
-                                    if (a == b ? k > j : false) continue;
-                                    c = c + dw1 * lw[i] * gj * wval<T>(R, sg + k);
-                                }
-                            }
+                        for (int cb = 0; cb < ncol; cb++) {
+                            const int bb = b0 + cb;
+                            double d = 0.0;
+                            for (int k = 0; k < nalc; k++) d = d + Gh[a * nalc + k] * G[bb * nalc + k];
+                            double c = common ? dc1<T>(a, bb, perq, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
+                            dst[roff + cb] = c - d;
                         }
                     }
-                    double d = 0.0;
-                    for (int k = 0; k < nalc; k++) d = d + Gh[a * nalc + k] * G[b * nalc + k];
-                    dst[off + e] = c - d;
-                    e += WAVE;
+                    u += WAVE;
                 }
             }
         }
@@ -609,9 +640,17 @@ __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int6
         const int rr = e / nc, cc = e - rr * nc;
         if (diag && cc > rr) continue;
         const int se = diag ? (rr * rr + rr) / 2 + cc : e;
-        double acc = 0.0;
-        for (int64_t q = i0; q < i1; q++) acc += cbuf[src[q] + se];
-        B[e] += acc;
+        // four independent partial sums keep four loads in flight
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+        int64_t q = i0;
+        for (; q + 4 <= i1; q += 4) {
+            acc0 += cbuf[src[q] + se];
+            acc1 += cbuf[src[q + 1] + se];
+            acc2 += cbuf[src[q + 2] + se];
+            acc3 += cbuf[src[q + 3] + se];
+        }
+        for (; q < i1; q++) acc0 += cbuf[src[q] + se];
+        B[e] += (acc0 + acc1) + (acc2 + acc3);
     }
 }
 
@@ -648,7 +687,7 @@ int lds_bytes(LdsPlan &P)
     size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max + 2 * (size_t)P.max_var * P.nalc_max +
                (size_t)P.max_var * QE;
     size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * QE + (P.nalc_max + 1) + 8;
-    size_t b = d * 8 + i * 4;
+    size_t b = d * 8 + (size_t)P.max_var * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;
 }

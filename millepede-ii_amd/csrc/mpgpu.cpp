@@ -882,7 +882,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
             cnt_r[r] = n;
         }
         const char *env = std::getenv("MPGPU_UPDATE_BUFFER_GB");
-        const double gb = env ? std::atof(env) : 48.0;
+        const double gb = env ? std::atof(env) : 64.0;
         const int64_t budget = std::max<int64_t>(1 << 20, (int64_t)(gb * (1ull << 30) / 8.0));
         int64_t q0 = 0, acc = 0;
         for (int64_t q = 0; q < nrec; q++) {
