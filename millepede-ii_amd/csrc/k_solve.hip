@@ -342,11 +342,12 @@ __global__ void k_block_symv(int32_t ngroup, const int64_t *brow, const int32_t 
 // (ga, gb <= ga) of B x[gb], the diagonal block used with both of its triangles.
 // Columns pass: one wavefront per column group gb adds sum over the blocks (ga > gb) of
 // B^T x[ga] to y[rows of gb] (column lists ccol_ptr / ccol_blk built with the structure).
-// Lanes take whole blocks; per-lane partial sums of up to 8 rows are reduced across the wave.
-#define BSY_R 8
-__device__ __forceinline__ double wsum64(double v)
+// Lanes work in groups of 8 per block (lane = block slot * 8 + row or column), so that one
+// wave instruction reads eight blocks' rows at once; partial sums are reduced over the slots.
+__device__ __forceinline__ double sum_slots(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    // reduce over the 8 block slots (lanes with equal lane % 8)
+    for (int o = 32; o >= 8; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 __global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_t *brow, const int32_t *bcol,
@@ -354,42 +355,28 @@ __global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_
                                                     const double *x, double *y)
 {
     const int ga = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, slot = lane >> 3, rl = lane & 7;
     if (ga >= ngroup) return;
     const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
     const int64_t b0 = brow[ga], b1 = brow[ga + 1];
-    for (int r0 = 0; r0 < nr; r0 += BSY_R) {
-        double acc[BSY_R];
-#pragma unroll
-        for (int r = 0; r < BSY_R; r++) acc[r] = 0.0;
-        for (int64_t bk = b0 + lane; bk < b1; bk += 64) {
+    for (int r0 = 0; r0 < nr; r0 += 8) {
+        const int R = r0 + rl;
+        double acc = 0.0;
+        for (int64_t bk = b0 + slot; bk < b1; bk += 8) {
+            if (R >= nr) continue;
             const int gb = bcol[bk];
             const int fb = gfirst[gb] - 1, nc = gfirst[gb + 1] - 1 - fb;
             const double *B = a + boff[bk];
+            double sv = 0.0;
             if (gb != ga) {
-#pragma unroll
-                for (int r = 0; r < BSY_R; r++) {
-                    if (r0 + r >= nr) break;
-                    double s = 0.0;
-                    for (int c = 0; c < nc; c++) s += B[(int64_t)(r0 + r) * nc + c] * x[fb + c];
-                    acc[r] += s;
-                }
+                for (int c = 0; c < nc; c++) sv += B[(int64_t)R * nc + c] * x[fb + c];
             } else {
-#pragma unroll
-                for (int r = 0; r < BSY_R; r++) {
-                    const int R = r0 + r;
-                    if (R >= nr) break;
-                    double s = 0.0;
-                    for (int c = 0; c < nc; c++) s += (c <= R ? B[(int64_t)R * nc + c] : B[(int64_t)c * nc + R]) * x[fb + c];
-                    acc[r] += s;
-                }
+                for (int c = 0; c < nc; c++) sv += (c <= R ? B[(int64_t)R * nc + c] : B[(int64_t)c * nc + R]) * x[fb + c];
             }
+            acc += sv;
         }
-#pragma unroll
-        for (int r = 0; r < BSY_R; r++) {
-            double v = wsum64(acc[r]);
-            if (lane == 0 && r0 + r < nr) y[fa + r0 + r] = v;
-        }
+        acc = sum_slots(acc);
+        if (slot == 0 && R < nr) y[fa + R] = acc;
     }
 }
 __global__ __launch_bounds__(256) void k_bsymv_cols(int32_t ngroup, const int64_t *ccol_ptr, const int64_t *ccol_blk,
@@ -397,32 +384,26 @@ __global__ __launch_bounds__(256) void k_bsymv_cols(int32_t ngroup, const int64_
                                                     const double *a, const double *x, double *y)
 {
     const int gb = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, slot = lane >> 3, cl = lane & 7;
     if (gb >= ngroup) return;
     const int fb = gfirst[gb] - 1, nc = gfirst[gb + 1] - 1 - fb;
     const int64_t q0 = ccol_ptr[gb], q1 = ccol_ptr[gb + 1];
     if (q1 <= q0) return;
-    for (int c0 = 0; c0 < nc; c0 += BSY_R) {
-        double acc[BSY_R];
-#pragma unroll
-        for (int c = 0; c < BSY_R; c++) acc[c] = 0.0;
-        for (int64_t q = q0 + lane; q < q1; q += 64) {
+    for (int c0 = 0; c0 < nc; c0 += 8) {
+        const int Cc = c0 + cl;
+        double acc = 0.0;
+        for (int64_t q = q0 + slot; q < q1; q += 8) {
+            if (Cc >= nc) continue;
             const int64_t bk = ccol_blk[q];
             const int ga = brg[bk];
             const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
             const double *B = a + boff[bk];
-            for (int r = 0; r < nr; r++) {
-                const double xr = x[fa + r];
-#pragma unroll
-                for (int c = 0; c < BSY_R; c++)
-                    if (c0 + c < nc) acc[c] += B[(int64_t)r * nc + c0 + c] * xr;
-            }
+            double sv = 0.0;
+            for (int r = 0; r < nr; r++) sv += B[(int64_t)r * nc + Cc] * x[fa + r];
+            acc += sv;
         }
-#pragma unroll
-        for (int c = 0; c < BSY_R; c++) {
-            double v = wsum64(acc[c]);
-            if (lane == 0 && c0 + c < nc) y[fb + c0 + c] += v;
-        }
+        acc = sum_slots(acc);
+        if (slot == 0 && Cc < nc) y[fb + Cc] += acc;
     }
 }
 
@@ -731,9 +712,11 @@ __global__ __launch_bounds__(256) void k_lltdec(int n, double *c, const int *ind
                         mj = j - indj + indj1 + 1;
                     }
                     diag = C(j, j);
-                    double &ckj = C(k, j);
+                    // accumulate in a register (the loads of the loop are independent)
+                    double ckj = C(k, j);
                     for (int i = max(mj, mk); i <= j - 1; i++) ckj = ckj - C(k, i) * C(j, i);
                     if (j != k) ckj = ckj * diag;
+                    C(k, j) = ckj;
                 }
                 double &ckk = C(k, k);
                 if (ckk > eps * diag) ckk = 1.0 / sqrt(ckk);
@@ -1107,9 +1090,10 @@ __global__ __launch_bounds__(256) void k_spk_carry(int n, int w, int nch, const 
     const int tc = SPK_LDS_DOUBLES / (w + w * w);      // chunks per tile
     double *sx = dyn;                                  // [tc * w]
     double *sS = sx + tc * w;                          // [tc * w * w]
-    __shared__ double cur[SPK_WMAX];
+    double cur[SPK_WMAX];                              // carried values (thread 0)
+#pragma unroll
+    for (int i = 0; i < SPK_WMAX; i++) cur[i] = 0.0;
     const int tid = threadIdx.x;
-    if (tid < SPK_WMAX) cur[tid] = 0.0;
     const int ntile = (nch + tc - 1) / tc;
     for (int tt = 0; tt < ntile; tt++) {
         // tiles in processing order (backward: from the last chunk)
@@ -1131,20 +1115,47 @@ __global__ __launch_bounds__(256) void k_spk_carry(int n, int w, int nch, const 
                 const int ch = backward ? c1 - 1 - cc : c0 + cc;
                 const int e0 = (ch - c0) * w;
                 const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1), L = k1 - k0 + 1;
-                for (int i = 0; i < w; i++) t[(size_t)ch * w + i] = cur[i];
+#pragma unroll
+                for (int i = 0; i < SPK_WMAX; i++)
+                    if (i < w) t[(size_t)ch * w + i] = cur[i];
                 const bool coupled = backward ? (k1 < n) : (ch > 0);
-                for (int i = 0; i < w; i++) {
-                    const bool in = backward ? (i < L) : (w - 1 - i < L);
-                    double v;
-                    if (in) {
-                        v = sx[e0 + i];
-                        if (coupled) for (int q = 0; q < w; q++) v += sS[(e0 + i) * w + q] * cur[q];
-                    } else {
-                        v = backward ? cur[i - L] : cur[i + L];   // chunk shorter than w: carry through
+                if (L >= w) {
+                    // full chunk: every boundary value from its row of S (registers only)
+#pragma unroll
+                    for (int i = 0; i < SPK_WMAX; i++) {
+                        if (i >= w) break;
+                        double v = sx[e0 + i];
+                        if (coupled) {
+#pragma unroll
+                            for (int q = 0; q < SPK_WMAX; q++)
+                                if (q < w) v += sS[(e0 + i) * w + q] * cur[q];
+                        }
+                        nxt[i] = v;
                     }
-                    nxt[i] = v;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < SPK_WMAX; i++) {
+                        if (i >= w) break;
+                        const bool in = backward ? (i < L) : (w - 1 - i < L);
+                        double v = 0.0;
+                        if (in) {
+                            v = sx[e0 + i];
+                            if (coupled) {
+#pragma unroll
+                                for (int q = 0; q < SPK_WMAX; q++)
+                                    if (q < w) v += sS[(e0 + i) * w + q] * cur[q];
+                            }
+                        } else {
+                            // chunk shorter than w: carry through
+#pragma unroll
+                            for (int q = 0; q < SPK_WMAX; q++)
+                                if (q == (backward ? i - L : i + L)) v = cur[q];
+                        }
+                        nxt[i] = v;
+                    }
                 }
-                for (int i = 0; i < w; i++) cur[i] = nxt[i];
+#pragma unroll
+                for (int i = 0; i < SPK_WMAX; i++) cur[i] = i < w ? nxt[i] : 0.0;
             }
         }
     }
