@@ -19,7 +19,6 @@
 namespace mpgpu {
 
 #define WAVE 64
-#define QE 4          // equations per record variable on the merged update path
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -44,9 +43,9 @@ __device__ __forceinline__ int64_t pk64(int64_t i, int64_t j) { return (i * i - 
 // equations i holding both variables (from the per-variable equation lists, or, for records
 // that overflow them, by scanning the equations as pede's per-equation update does)
 template <typename T>
-__device__ __forceinline__ double dc1(int a, int b, bool perq, const int32_t *qn, const int32_t *qe, const double *qg,
-                                      const double *lw, double dw1, const DevRecords &R, int64_t e0, int neq,
-                                      const int32_t *gpos)
+__device__ __forceinline__ double dc1(int a, int b, bool perq, int QE, const int32_t *qn, const int32_t *qe,
+                                      const double *qg, const double *lw, double dw1, const DevRecords &R, int64_t e0,
+                                      int neq, const int32_t *gpos)
 {
     double c = 0.0;
     if (!perq) {
@@ -94,6 +93,7 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const double chuber = 1.345, cauchy = 2.3849;
+    const int QE = P.qe;                     // equations per record variable on the merged update path
 
     // ---- carve LDS ----
     double *gp = lds;                        // [max_gl] parameter value per distinct label
@@ -108,8 +108,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *cs = dg + P.nalc_max;            // [nalc_max] scaled pivot column
     double *co = cs + P.nalc_max;            // [nalc_max] original pivot column
     double *G = co + P.nalc_max;             // [max_var * nalc_max]
-    double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max]
-    double *qg = Gh + (size_t)P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
+    double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max] (full storage only)
+    double *qg = Gh + (size_t)P.gh * P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
     unsigned long long *segm = (unsigned long long *)(qg + (size_t)P.max_var * QE); // [max_var] equations per segment
     int32_t *gv = (int32_t *)(segm + P.max_var); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
@@ -530,12 +530,15 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 }
                 __syncthreads();
             }
-            // Gh = G Gamma^-1 (row-wise, l ascending as dbavat)
-            for (int a = lane; a < nalg; a += WAVE) {
-                for (int k = 1; k <= nalc; k++) {
-                    double c = 0.0;
-                    for (int l = 1; l <= nalc; l++) c = c + G[a * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
-                    Gh[a * nalc + k - 1] = c;
+            // Gh = G Gamma^-1 (row-wise, l ascending as dbavat); group-block storage forms the
+            // rows of Gh in registers where they are used
+            if (!twophase) {
+                for (int a = lane; a < nalg; a += WAVE) {
+                    for (int k = 1; k <= nalc; k++) {
+                        double c = 0.0;
+                        for (int l = 1; l <= nalc; l++) c = c + G[a * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
+                        Gh[a * nalc + k - 1] = c;
+                    }
                 }
             }
             __syncthreads();
@@ -591,22 +594,33 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                     if (nalc <= 8) {
                         double gh[8];
 #pragma unroll
-                        for (int k = 0; k < 8; k++) gh[k] = k < nalc ? Gh[a * nalc + k] : 0.0;
+                        for (int k = 0; k < 8; k++) {
+                            double c = 0.0;
+                            if (k < nalc)
+                                for (int l = 1; l <= nalc; l++)
+                                    c = c + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
+                            gh[k] = c;
+                        }
                         for (int cb = 0; cb < ncol; cb++) {
                             const int bb = b0 + cb;
                             double d = 0.0;
 #pragma unroll
                             for (int k = 0; k < 8; k++)
                                 if (k < nalc) d = d + gh[k] * G[bb * nalc + k];
-                            double c = common ? dc1<T>(a, bb, perq, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
+                            double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
                             dst[roff + cb] = c - d;
                         }
                     } else {
                         for (int cb = 0; cb < ncol; cb++) {
                             const int bb = b0 + cb;
                             double d = 0.0;
-                            for (int k = 0; k < nalc; k++) d = d + Gh[a * nalc + k] * G[bb * nalc + k];
-                            double c = common ? dc1<T>(a, bb, perq, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
+                            for (int k = 0; k < nalc; k++) {
+                                double gk = 0.0;
+                                for (int l = 1; l <= nalc; l++)
+                                    gk = gk + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
+                                d = d + gk * G[bb * nalc + k];
+                            }
+                            double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
                             dst[roff + cb] = c - d;
                         }
                     }
@@ -684,9 +698,9 @@ hipError_t launch_block_gather(int64_t nblock, const int64_t *boff, const int32_
 int lds_bytes(LdsPlan &P)
 {
     int npk = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
-    size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max + 2 * (size_t)P.max_var * P.nalc_max +
-               (size_t)P.max_var * QE;
-    size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * QE + (P.nalc_max + 1) + 8;
+    size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max +
+               (size_t)(1 + P.gh) * P.max_var * P.nalc_max + (size_t)P.max_var * P.qe;
+    size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * P.qe + (P.nalc_max + 1) + 8;
     size_t b = d * 8 + (size_t)P.max_var * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;

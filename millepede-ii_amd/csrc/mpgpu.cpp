@@ -654,6 +654,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     const bool sparse = c->opt.matsto == MPGPU_SPARSE;
     int32_t max_gl = 1, max_eq = 1, max_nalc = 1, max_var = 1, max_seg = 1;
     int32_t bmax_gl = 1, bmax_eq = 1, bmax_nalc = 1, bmax_var = 1, bmax_seg = 1, bmax_mp1 = 1, bmax_bdr = 0, bmax_sup = 1;
+    int32_t max_qe = 1;   // most equations any record variable occurs in (full local fits)
     std::vector<int64_t> rec_pb(nrec + 1, 0);
     std::vector<int32_t> pair_blk;
     std::vector<int32_t> binfo(nrec, 0);
@@ -666,7 +667,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         std::vector<std::atomic<uint64_t>> pbits(sparse ? (npairs + 63) / 64 : 0);
         for (auto &b : pbits) b.store(0, std::memory_order_relaxed);
 #pragma omp parallel for schedule(dynamic, 256) reduction(max : max_gl, max_eq, max_nalc, max_var, max_seg, bmax_gl, \
-    bmax_eq, bmax_nalc, bmax_var, bmax_seg, bmax_mp1, bmax_bdr, bmax_sup)
+    bmax_eq, bmax_nalc, bmax_var, bmax_seg, bmax_mp1, bmax_bdr, bmax_sup, max_qe)
         for (int64_t r = 0; r < nrec; r++) {
             thread_local std::vector<int> ibh;
             int32_t ngl = ngl_rec[r];
@@ -705,6 +706,19 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
                 max_eq = std::max(max_eq, neq_r);
                 max_nalc = std::max(max_nalc, nalc);
                 max_var = std::max(max_var, nv_r);
+                // equations per record variable (non-zero derivative), as the record kernel lists them
+                thread_local std::vector<int32_t> qcnt;
+                qcnt.assign(ngl, 0);
+                for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
+                    const int nl = eq_n[e] & 0xffff, ng = eq_n[e] >> 16;
+                    const int64_t sg = eq_sl[e] + nl + 1;
+                    for (int j = 1; j <= ng; j++) {
+                        const double gj = c->dbl ? c->vd[sg + j] : (double)c->vf[sg + j];
+                        const int pos = widx[sg + j];
+                        if (gj != 0.0 && c->var_index[gl_tix[rec_gl[r] + pos]] > 0) qcnt[pos]++;
+                    }
+                }
+                for (int32_t q : qcnt) max_qe = std::max(max_qe, q);
             }
             max_seg = std::max(max_seg, nseg);
             nseg_rec[r] = nseg;
@@ -809,6 +823,8 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->plan.nalc_max = std::max(1, std::min(max_nalc, 64));
     c->plan.max_var = max_var;
     c->plan.max_pair = (int32_t)std::max<int64_t>(1, c->max_pair);
+    c->plan.qe = std::min(8, std::max(1, max_qe));
+    c->plan.gh = sparse ? 0 : 1;
     lds_bytes(c->plan);
     if (c->plan.bytes > 160 * 1024)
         throw MpError(30, "record too large for the record kernel (LDS " + std::to_string(c->plan.bytes) + " bytes)");
