@@ -110,7 +110,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *G = co + P.nalc_max;             // [max_var * nalc_max]
     double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max] (full storage only)
     double *qg = Gh + (size_t)P.gh * P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
-    unsigned long long *segm = (unsigned long long *)(qg + (size_t)P.max_var * QE); // [max_var] equations per segment
+    double *stage = qg + (size_t)P.max_var * QE;   // [stage] values of one step (coalesced stores)
+    unsigned long long *segm = (unsigned long long *)(stage + P.stage); // [max_var] equations per segment
     int32_t *gv = (int32_t *)(segm + P.max_var); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
     int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
@@ -575,54 +576,78 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                     segm[q] = m;
                 }
                 __syncthreads();
+                // The 64 rows of one step are consecutive in the buffer: their values are staged in
+                // LDS and written as one contiguous range (coalesced stores).
                 int sa = 0, sb = 0, u = lane;
                 int64_t off = 0;                            // first value of the current pair
-                while (sa < nseg) {
-                    const int na = segst[sa + 1] - segst[sa];
-                    if (u >= na) {
+                for (;;) {
+                    while (sa < nseg) {
+                        const int na = segst[sa + 1] - segst[sa];
+                        if (u < na) break;
                         const int nb = segst[sb + 1] - segst[sb];
                         u -= na;
                         off += sa == sb ? (na * na + na) / 2 : na * nb;
                         if (++sb > sa) { sa++; sb = 0; }
-                        continue;
                     }
-                    const int ra = u, a = segst[sa] + ra, b0 = segst[sb];
-                    const int nb = segst[sb + 1] - b0;
-                    const int ncol = sa == sb ? ra + 1 : nb;
-                    const int64_t roff = off + (sa == sb ? (ra * ra + ra) / 2 : ra * nb);
-                    const bool common = (segm[sa] & segm[sb]) != 0ull;
-                    if (nalc <= 8) {
-                        double gh[8];
+                    const bool valid = sa < nseg;
+                    if (__ballot(valid) == 0ull) break;
+                    int ra = 0, a = 0, b0 = 0, nb = 0, ncol = 0;
+                    int64_t roff = 0;
+                    if (valid) {
+                        ra = u;
+                        a = segst[sa] + ra;
+                        b0 = segst[sb];
+                        nb = segst[sb + 1] - b0;
+                        ncol = sa == sb ? ra + 1 : nb;
+                        roff = off + (sa == sb ? (ra * ra + ra) / 2 : ra * nb);
+                    }
+                    // step range [start, end): lane 0 holds the first row (rows are consecutive)
+                    const int64_t start = __shfl(roff, 0, WAVE);
+                    int64_t endv = valid ? roff + ncol : 0;
+                    for (int o = 32; o > 0; o >>= 1) endv = max(endv, __shfl_xor(endv, o, WAVE));
+                    const int span = (int)(endv - start);
+                    const bool staged = span <= P.stage;
+                    double *out_row = staged ? stage + (roff - start) : dst + roff;
+                    if (valid) {
+                        const bool common = (segm[sa] & segm[sb]) != 0ull;
+                        if (nalc <= 8) {
+                            double gh[8];
 #pragma unroll
-                        for (int k = 0; k < 8; k++) {
-                            double c = 0.0;
-                            if (k < nalc)
-                                for (int l = 1; l <= nalc; l++)
-                                    c = c + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
-                            gh[k] = c;
-                        }
-                        for (int cb = 0; cb < ncol; cb++) {
-                            const int bb = b0 + cb;
-                            double d = 0.0;
-#pragma unroll
-                            for (int k = 0; k < 8; k++)
-                                if (k < nalc) d = d + gh[k] * G[bb * nalc + k];
-                            double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
-                            dst[roff + cb] = c - d;
-                        }
-                    } else {
-                        for (int cb = 0; cb < ncol; cb++) {
-                            const int bb = b0 + cb;
-                            double d = 0.0;
-                            for (int k = 0; k < nalc; k++) {
-                                double gk = 0.0;
-                                for (int l = 1; l <= nalc; l++)
-                                    gk = gk + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
-                                d = d + gk * G[bb * nalc + k];
+                            for (int k = 0; k < 8; k++) {
+                                double c = 0.0;
+                                if (k < nalc)
+                                    for (int l = 1; l <= nalc; l++)
+                                        c = c + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
+                                gh[k] = c;
                             }
-                            double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
-                            dst[roff + cb] = c - d;
+                            for (int cb = 0; cb < ncol; cb++) {
+                                const int bb = b0 + cb;
+                                double d = 0.0;
+#pragma unroll
+                                for (int k = 0; k < 8; k++)
+                                    if (k < nalc) d = d + gh[k] * G[bb * nalc + k];
+                                double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
+                                out_row[cb] = c - d;
+                            }
+                        } else {
+                            for (int cb = 0; cb < ncol; cb++) {
+                                const int bb = b0 + cb;
+                                double d = 0.0;
+                                for (int k = 0; k < nalc; k++) {
+                                    double gk = 0.0;
+                                    for (int l = 1; l <= nalc; l++)
+                                        gk = gk + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
+                                    d = d + gk * G[bb * nalc + k];
+                                }
+                                double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
+                                out_row[cb] = c - d;
+                            }
                         }
+                    }
+                    if (staged) {
+                        __syncthreads();
+                        for (int t = lane; t < span; t += WAVE) dst[start + t] = stage[t];
+                        __syncthreads();
                     }
                     u += WAVE;
                 }
@@ -701,7 +726,7 @@ int lds_bytes(LdsPlan &P)
     size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max +
                (size_t)(1 + P.gh) * P.max_var * P.nalc_max + (size_t)P.max_var * P.qe;
     size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * P.qe + (P.nalc_max + 1) + 8;
-    size_t b = d * 8 + (size_t)P.max_var * 8 + i * 4;
+    size_t b = d * 8 + (size_t)P.stage * 8 + (size_t)P.max_var * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;
 }

@@ -658,71 +658,61 @@ __global__ void k_presols_b(int64_t n, const double *cu, double *x)
 // (coalesced loads and stores by the whole workgroup) and one thread runs the recurrence on
 // LDS, with the last rows of the previous tile kept in a small ring.
 #define LLT_TILE 6144     // band values per tile
-#define LLT_ROWS 2048     // rows per tile
 #define LLT_MW 64         // widest row supported by the tiled path
-__global__ __launch_bounds__(256) void k_lltdec(int n, double *c, const int *india, int *nrkd, int iopt)
+__global__ __launch_bounds__(256) void k_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw)
 {
     __shared__ double buf[LLT_TILE];
-    __shared__ int ind[LLT_ROWS + 1];
-    __shared__ double ring[LLT_MW * LLT_MW];   // element (j, i) of row j < k0 at ring[(j % MW)*MW + j - i]
-    __shared__ int sh[4];
+    __shared__ int ind[LLT_TILE + 1];
+    __shared__ double ring[LLT_MW * LLT_MW];   // row j < k0 in slot j % MW: element (j, i) at [slot*MW + MW-1 - j + i]
     const int tid = threadIdx.x;
     const double eps = 16.0 * DBL_EPSILON;
+    const int rows = max(1, LLT_TILE / max(1, mbw));  // rows per tile (each row holds <= mbw values)
     int nr = 0;
     double diag = 0.0;
-    int k0 = 1;
-    while (k0 <= n) {
-        if (tid == 0) {
-            const int base = k0 > 1 ? india[k0 - 2] : 0;   // IND[k0-1] (0 for k0 = 1)
-            int k1 = k0;
-            while (k1 + 1 <= n && k1 + 1 - k0 + 1 <= LLT_ROWS && india[k1] - base <= LLT_TILE) k1++;
-            sh[0] = k1;
-            sh[1] = base;
-        }
-        __syncthreads();
-        const int k1 = sh[0], base = sh[1];
+    for (int k0 = 1; k0 <= n; k0 += rows) {
+        const int k1 = min(n, k0 + rows - 1);
+        const int base = k0 > 1 ? india[k0 - 2] : 0;   // IND(k0-1)
         const int nel = india[k1 - 1] - base;
         for (int t = tid; t < nel; t += 256) buf[t] = c[base + t];
         for (int t = tid; t <= k1 - k0; t += 256) ind[t + 1] = india[k0 - 1 + t];
         if (tid == 0) ind[0] = base;
         __syncthreads();
         if (tid == 0) {
-            // IND(k) for k in [k0-1, k1]: ind[k - k0 + 1]; C(p) for p > base: buf[p - base - 1]
-#define IND_(k) ind[(k) - k0 + 1]
-            auto C = [&](int j, int i) -> double & {
-                if (j >= k0) return buf[IND_(j) - j + i - base - 1];
-                return ring[(j % LLT_MW) * LLT_MW + (j - i)];
+            // row pointers: element (j, i) = P[i]
+            auto rowp = [&](int j) -> double * {
+                if (j >= k0) return &buf[ind[j - k0 + 1] - j - base - 1];
+                return &ring[(j % LLT_MW) * LLT_MW + LLT_MW - 1 - j];
+            };
+            auto indv = [&](int j) -> int {          // IND(j) for j >= 0
+                if (j >= k0 - 1) return ind[j - k0 + 1];
+                return j >= 1 ? india[j - 1] : 0;
             };
             for (int k = k0; k <= k1; k++) {
+                double *Pk = rowp(k);
                 if (k == 1) {
-                    double &d = C(1, 1);
-                    if (d > 0.0) d = 1.0 / sqrt(d);
-                    else { d = 0.0; nr = -1; }
+                    if (Pk[1] > 0.0) Pk[1] = 1.0 / sqrt(Pk[1]);
+                    else { Pk[1] = 0.0; nr = -1; }
                     continue;
                 }
-                const int indk = IND_(k);
-                const int indk1 = (k - 1 >= k0) ? IND_(k - 1) : base;
-                const int mk = k - indk + indk1 + 1;
+                const int mk = k - indv(k) + indv(k - 1) + 1;
+                int indj1 = indv(mk - 1);
                 for (int j = mk; j <= k; j++) {
-                    int mj = 1;
-                    if (j > 1) {
-                        // width of row j (rows before the tile: from the global pointers)
-                        const int indj = j >= k0 ? IND_(j) : india[j - 1];
-                        const int indj1 = (j - 1 >= k0) ? IND_(j - 1) : (j - 1 >= 1 ? india[j - 2] : 0);
-                        mj = j - indj + indj1 + 1;
-                    }
-                    diag = C(j, j);
-                    // accumulate in a register (the loads of the loop are independent)
-                    double ckj = C(k, j);
-                    for (int i = max(mj, mk); i <= j - 1; i++) ckj = ckj - C(k, i) * C(j, i);
+                    const int indj = indv(j);
+                    const int mj = j > 1 ? j - indj + indj1 + 1 : 1;
+                    indj1 = indj;
+                    const double *Pj = rowp(j);
+                    diag = Pj[j];
+                    // accumulate in a register; the loads of the loop are independent
+                    double ckj = Pk[j];
+#pragma unroll 4
+                    for (int i = max(mj, mk); i <= j - 1; i++) ckj = ckj - Pk[i] * Pj[i];
                     if (j != k) ckj = ckj * diag;
-                    C(k, j) = ckj;
+                    Pk[j] = ckj;
                 }
-                double &ckk = C(k, k);
-                if (ckk > eps * diag) ckk = 1.0 / sqrt(ckk);
+                if (Pk[k] > eps * diag) Pk[k] = 1.0 / sqrt(Pk[k]);
                 else {
-                    for (int j = mk; j <= k; j++) C(k, j) = 0.0;
-                    if (iopt > 0 && diag > 0.0) C(k, k) = 1.0 / sqrt(diag);
+                    for (int j = mk; j <= k; j++) Pk[j] = 0.0;
+                    if (iopt > 0 && diag > 0.0) Pk[k] = 1.0 / sqrt(diag);
                     else {
                         if (nr == 0) nr = -k;
                         else { if (nr < 0) nr = 1; nr++; }
@@ -731,16 +721,15 @@ __global__ __launch_bounds__(256) void k_lltdec(int n, double *c, const int *ind
             }
             // keep the last rows of the tile for the next one
             for (int k = max(k0, k1 - LLT_MW + 1); k <= k1; k++) {
-                const int indk1 = (k - 1 >= k0) ? IND_(k - 1) : base;
-                const int mk = k - IND_(k) + indk1 + 1;
-                for (int j = mk; j <= k; j++) ring[(k % LLT_MW) * LLT_MW + (k - j)] = C(k, j);
+                const int mk = k - indv(k) + indv(k - 1) + 1;
+                const double *Pk = rowp(k);
+                double *Rk = &ring[(k % LLT_MW) * LLT_MW + LLT_MW - 1 - k];
+                for (int j = mk; j <= k; j++) Rk[j] = Pk[j];
             }
-#undef IND_
         }
         __syncthreads();
         for (int t = tid; t < nel; t += 256) c[base + t] = buf[t];
         __syncthreads();
-        k0 = k1 + 1;
     }
     if (tid == 0) *nrkd = nr;
 }
@@ -986,8 +975,8 @@ void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, cons
     hipLaunchKernelGGL(k_presols_b, dim3(nblk(n)), dim3(TPB), 0, s, (int64_t)n, cu, x);
 }
 
-void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, hipStream_t s)
-{ hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt); }
+void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, hipStream_t s)
+{ hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt, mbw); }
 void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
               hipStream_t s)
 { if (ncon > 0) hipLaunchKernelGGL(k_band_k, dim3((ncon + 63) / 64), dim3(64), 0, s, ncon, n, c, india, cb, koff, kmat); }

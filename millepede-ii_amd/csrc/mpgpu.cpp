@@ -825,6 +825,12 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->plan.max_pair = (int32_t)std::max<int64_t>(1, c->max_pair);
     c->plan.qe = std::min(8, std::max(1, max_qe));
     c->plan.gh = sparse ? 0 : 1;
+    {
+        int32_t maxgrp = 1;
+        for (int32_t g = 0; g < c->ngroup; g++) maxgrp = std::max(maxgrp, c->gfirst[g + 1] - c->gfirst[g]);
+        // 64 rows of the widest group pair; larger steps are written row by row
+        c->plan.stage = sparse ? std::min(64 * maxgrp + 64, 4096) : 0;
+    }
     lds_bytes(c->plan);
     if (c->plan.bytes > 160 * 1024)
         throw MpError(30, "record too large for the record kernel (LDS " + std::to_string(c->plan.bytes) + " bytes)");
@@ -1417,7 +1423,7 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
     P.band.alloc(india[P.n - 1]);
     l_get_band(M, P.n, mb, P.india.p, P.band.p, s);
     P.nrkd.alloc(2);
-    l_lltdec(P.n, P.band.p, P.india.p, P.nrkd.p, c->opt.lprecm, s);
+    l_lltdec(P.n, P.band.p, P.india.p, P.nrkd.p, c->opt.lprecm, mb, s);
     if (P.m > 0) {
         // constraint blocks: merge overlapping variable ranges of constraints
         std::vector<int> first(P.m), last(P.m);

@@ -107,7 +107,7 @@ void l_precons(int32_t n, int32_t p, const double *diag, double *cu, int *nrkd, 
                const double *cval, double *sm, double *scratch, hipStream_t s);
 void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, const int32_t *cidx, const double *cval,
                const double *sm, const double *y, double *x, hipStream_t s);
-void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, hipStream_t s);
+void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, hipStream_t s);
 void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
               hipStream_t s);
 void l_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h, const int *hind, int *nrkd2,
