@@ -44,8 +44,8 @@ __device__ __forceinline__ int64_t pk64(int64_t i, int64_t j) { return (i * i - 
 // that overflow them, by scanning the equations as pede's per-equation update does)
 template <typename T>
 __device__ __forceinline__ double dc1(int a, int b, bool perq, int QE, const int32_t *qn, const int32_t *qe,
-                                      const double *qg, const double *lw, double dw1, const DevRecords &R, int64_t e0,
-                                      int neq, const int32_t *gpos)
+                                      const double *qg, const double *lw, double dw1, const int32_t *les,
+                                      const uint32_t *len, const int32_t *li, const T *lv, int neq, const int32_t *gpos)
 {
     double c = 0.0;
     if (!perq) {
@@ -58,18 +58,18 @@ __device__ __forceinline__ double dc1(int a, int b, bool perq, int QE, const int
         return c;
     }
     for (int i = 0; i < neq; i++) {
-        const int64_t sl = R.eq_sl[e0 + i];
-        const uint32_t en = R.eq_n[e0 + i];
-        const int64_t sg = sl + (en & 0xffff) + 1;
+        const int sl = les[i];
+        const uint32_t en = len[i];
+        const int sg = sl + (int)(en & 0xffff) + 1;
         const int ng = en >> 16;
         for (int j = 1; j <= ng; j++) {
-            if (gpos[R.widx[sg + j]] != a) continue;
-            const double gj = wval<T>(R, sg + j);
+            if (gpos[li[sg + j]] != a) continue;
+            const double gj = (double)lv[sg + j];
             if (gj == 0.0) continue;
             for (int k = 1; k <= ng; k++) {
-                if (gpos[R.widx[sg + k]] != b) continue;
+                if (gpos[li[sg + k]] != b) continue;
                 if (a == b && k > j) continue;           // each (j, k <= j) word pair once
-                c = c + dw1 * lw[i] * gj * wval<T>(R, sg + k);
+                c = c + dw1 * lw[i] * gj * (double)lv[sg + k];
             }
         }
     }
@@ -111,7 +111,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max] (full storage only)
     double *qg = Gh + (size_t)P.gh * P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
     double *stage = qg + (size_t)P.max_var * QE;   // [stage] values of one step (coalesced stores)
-    unsigned long long *segm = (unsigned long long *)(stage + P.stage); // [max_var] equations per segment
+    T *lv = (T *)(stage + P.stage);                // [max_w] the record's word values
+    unsigned long long *segm = (unsigned long long *)(stage + P.stage + ((size_t)P.max_w * sizeof(T) + 7) / 8); // [max_var]
     int32_t *gv = (int32_t *)(segm + P.max_var); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
     int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
@@ -119,7 +120,11 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     int32_t *qn = segst + P.max_var + 1;     // [max_var] equations of each record variable
     int32_t *qe = qn + P.max_var;            // [max_var*QE] their indices (ascending)
     int32_t *nxt = qe + P.max_var * QE;      // [nalc_max+1] pivot list
-    int32_t *sh = nxt + P.nalc_max + 1;      // scalars
+    int32_t *sh = nxt + P.nalc_max + 1;      // [8] scalars
+    int32_t *li = sh + 8;                    // [max_w] the record's word indices (local / global position)
+    int32_t *les = li + P.max_w;             // [max_eq] measurement word of each equation (record-local)
+    uint32_t *len = (uint32_t *)(les + P.max_eq); // [max_eq] local / global word counts
+    auto lwv = [&](int x) { return (double)lv[x]; };
 
     for (int64_t ri = rec_begin + blockIdx.x; ri < rec_end; ri += gridDim.x) {
         const int64_t r = R.order ? R.order[ri] : ri;
@@ -130,10 +135,21 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         const int ngl = (int)(R.rec_gl[r + 1] - g0);
         const double dw1 = R.rec_w ? (double)R.rec_w[r] : 1.0;
         const double dw2 = sqrt(dw1);
-        if (neq > P.max_eq || ngl > P.max_gl) {
+        const int64_t w0 = R.rec_word[r];
+        const int nw = (int)(R.rec_word[r + 1] - w0);
+        if (neq > P.max_eq || ngl > P.max_gl || nw > P.max_w) {
             if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
             zero_update(R, M, o, r, lane);
             continue;
+        }
+        // ---- the record's words and equation table into LDS (coalesced, once) ----
+        for (int t = lane; t < nw; t += WAVE) {
+            lv[t] = R.vf ? (T)R.vf[w0 + t] : (T)R.vd[w0 + t];
+            li[t] = R.widx[w0 + t];
+        }
+        for (int i = lane; i < neq; i += WAVE) {
+            les[i] = (int)(R.eq_sl[e0 + i] - w0);
+            len[i] = R.eq_n[e0 + i];
         }
         // ---- distinct labels: parameter value, variable index, position among variables ----
         int nalg = 0;
@@ -162,16 +178,16 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         // ---- first loop: subtract global contribution, count local parameters (3655-3693) ----
         int nalc = 0;
         for (int i = lane; i < neq; i += WAVE) {
-            int64_t sl = R.eq_sl[e0 + i];
-            uint32_t en = R.eq_n[e0 + i];
+            const int sl = les[i];
+            const uint32_t en = len[i];
             int nl = en & 0xffff, ng = en >> 16;
-            int64_t sg = sl + nl + 1;
-            double rmeas = wval<T>(R, sl);
-            for (int j = 1; j <= ng; j++) rmeas = rmeas - wval<T>(R, sg + j) * gp[R.widx[sg + j]];
+            const int sg = sl + nl + 1;
+            double rmeas = lwv(sl);
+            for (int j = 1; j <= ng; j++) rmeas = rmeas - lwv(sg + j) * gp[li[sg + j]];
             rm[i] = rmeas;
-            re[i] = wval<T>(R, sg);
+            re[i] = lwv(sg);
             lc[i] = 0.0;
-            for (int j = 1; j <= nl; j++) nalc = max(nalc, R.widx[sl + j]);
+            for (int j = 1; j <= nl; j++) nalc = max(nalc, li[sl + j]);
         }
         nalc = wave_max(nalc);
         if (nalc <= 0) {
@@ -223,14 +239,14 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 }
                 double acc = 0.0;
                 for (int i = 0; i < neq; i++) {
-                    int64_t sl = R.eq_sl[e0 + i];
-                    int nl = R.eq_n[e0 + i] & 0xffff;
+                    const int sl = les[i];
+                    int nl = len[i] & 0xffff;
                     double dj = 0.0, dk = 0.0;
                     bool hj = false, hk = false;
                     for (int j = 1; j <= nl; j++) {
-                        int ij = R.widx[sl + j];
-                        if (ij == J) { dj = wval<T>(R, sl + j); hj = true; }
-                        if (K > 0 && ij == K) { dk = wval<T>(R, sl + j); hk = true; }
+                        int ij = li[sl + j];
+                        if (ij == J) { dj = lwv(sl + j); hj = true; }
+                        if (K > 0 && ij == K) { dk = lwv(sl + j); hk = true; }
                     }
                     if (K == 0) {
                         if (hj) acc = acc + lw[i] * rm[i] * dj;
@@ -323,12 +339,12 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             // ---- third loop: residuals, chi^2 (3919-4035) ----
             double s_chi = 0.0, s_wt = 0.0;
             for (int i = lane; i < neq; i += WAVE) {
-                int64_t sl = R.eq_sl[e0 + i];
-                int nl = R.eq_n[e0 + i] & 0xffff;
+                const int sl = les[i];
+                int nl = len[i] & 0xffff;
                 double rerr = re[i];
                 double wght = 1.0 / (rerr * rerr);
                 double rmloc = 0.0;
-                for (int j = 1; j <= nl; j++) rmloc = rmloc + wval<T>(R, sl + j) * bl[R.widx[sl + j] - 1];
+                for (int j = 1; j <= nl; j++) rmloc = rmloc + lwv(sl + j) * bl[li[sl + j] - 1];
                 lc[i] = rmloc;
                 double rmeas = rm[i] - rmloc;
                 double dchi2 = wght * rmeas * rmeas;
@@ -431,10 +447,10 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         double s_chi = 0.0;
         const bool down = (o.nloopn != 1 && o.lhuber != 0);
         for (int i = lane; i < neq; i += WAVE) {
-            int64_t sl = R.eq_sl[e0 + i];
-            uint32_t en = R.eq_n[e0 + i];
+            const int sl = les[i];
+            const uint32_t en = len[i];
             int nl = en & 0xffff, ng = en >> 16;
-            int64_t sg = sl + nl + 1;
+            const int sg = sl + nl + 1;
             double rmeas = rm[i], rerr = re[i];
             double wght = 1.0 / (rerr * rerr);
             double dchi2 = wght * rmeas * rmeas;
@@ -451,8 +467,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             s_chi += dchi2;
             lw[i] = wght; // weight used by the global update
             for (int j = 1; j <= ng; j++) {
-                double gj = wval<T>(R, sg + j);
-                int pj = R.widx[sg + j];
+                double gj = lwv(sg + j);
+                int pj = li[sg + j];
                 int vj = gv[pj];
                 if (gj == 0.0 || vj <= 0) continue;
                 int aj = gpos[pj];
@@ -461,10 +477,10 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 if (o.icalcm == 1 && !M.sparse) {
                     // full storage: g g^T of this equation
                     for (int k = 1; k <= j; k++) {
-                        int pk_ = R.widx[sg + k];
+                        int pk_ = li[sg + k];
                         int vk = gv[pk_];
                         if (vk > 0) {
-                            double val = dw1 * wght * gj * wval<T>(R, sg + k);
+                            double val = dw1 * wght * gj * lwv(sg + k);
                             if (vj >= vk) atomicAdd(&M.mat[pk64(vj, vk)], val);
                             else atomicAdd(&M.mat[pk64(vk, vj)], val);
                         }
@@ -499,13 +515,13 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             }
         } else {
             for (int i = lane; i < neq; i += WAVE) {
-                int64_t sl = R.eq_sl[e0 + i];
-                uint32_t en = R.eq_n[e0 + i];
+                const int sl = les[i];
+                const uint32_t en = len[i];
                 int nl = en & 0xffff, ng = en >> 16;
-                int64_t sg = sl + nl + 1;
+                const int sg = sl + nl + 1;
                 for (int j = 1; j <= ng; j++) {
-                    double gj = wval<T>(R, sg + j);
-                    int vj = gv[R.widx[sg + j]];
+                    double gj = lwv(sg + j);
+                    int vj = gv[li[sg + j]];
                     if (gj == 0.0 || vj <= 0) continue;
                     atomicAdd(&M.vec[vj - 1], dw1 * lw[i] * rm[i] * gj);
                     atomicAdd(&M.cnt[vj - 1], 1ull);
@@ -515,18 +531,18 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         if (o.icalcm == 1) {
             // G = sum over equations of sqrt(dw) w g l^T, equations in order (4206-4219)
             for (int i = 0; i < neq; i++) {
-                int64_t sl = R.eq_sl[e0 + i];
-                uint32_t en = R.eq_n[e0 + i];
+                const int sl = les[i];
+                const uint32_t en = len[i];
                 int nl = en & 0xffff, ng = en >> 16;
-                int64_t sg = sl + nl + 1;
+                const int sg = sl + nl + 1;
                 double wght = lw[i];
                 for (int t = lane; t < ng * nl; t += WAVE) {
                     int j = t / nl + 1, k = t % nl + 1;
-                    int pj = R.widx[sg + j];
+                    int pj = li[sg + j];
                     int a = gpos[pj];
                     if (a >= 0) {
-                        int ik = R.widx[sl + k];
-                        G[a * nalc + ik - 1] = G[a * nalc + ik - 1] + dw2 * wght * wval<T>(R, sg + j) * wval<T>(R, sl + k);
+                        int ik = li[sl + k];
+                        G[a * nalc + ik - 1] = G[a * nalc + ik - 1] + dw2 * wght * lwv(sg + j) * lwv(sl + k);
                     }
                 }
                 __syncthreads();
@@ -626,7 +642,7 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
 #pragma unroll
                                 for (int k = 0; k < 8; k++)
                                     if (k < nalc) d = d + gh[k] * G[bb * nalc + k];
-                                double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
+                                double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos) : 0.0;
                                 out_row[cb] = c - d;
                             }
                         } else {
@@ -639,7 +655,7 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                                         gk = gk + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
                                     d = d + gk * G[bb * nalc + k];
                                 }
-                                double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, R, e0, neq, gpos) : 0.0;
+                                double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos) : 0.0;
                                 out_row[cb] = c - d;
                             }
                         }
@@ -726,7 +742,8 @@ int lds_bytes(LdsPlan &P)
     size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max +
                (size_t)(1 + P.gh) * P.max_var * P.nalc_max + (size_t)P.max_var * P.qe;
     size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * P.qe + (P.nalc_max + 1) + 8;
-    size_t b = d * 8 + (size_t)P.stage * 8 + (size_t)P.max_var * 8 + i * 4;
+    i += (size_t)P.max_w + 2 * (size_t)P.max_eq;
+    size_t b = d * 8 + (size_t)P.stage * 8 + (((size_t)P.max_w * 8 + 7) / 8) * 8 + (size_t)P.max_var * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;
 }

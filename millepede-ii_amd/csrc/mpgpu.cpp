@@ -50,7 +50,7 @@ struct mpgpu_ctx {
     std::vector<std::vector<std::pair<int32_t, double>>> con_tot; // (total index, factor) per constraint
 
     // ---- device ----
-    DBuf<int64_t> d_rec_eq, d_rec_gl, d_rec_pb, d_eq_sl, d_order;
+    DBuf<int64_t> d_rec_eq, d_rec_gl, d_rec_pb, d_eq_sl, d_order, d_rec_w;
     DBuf<uint32_t> d_eq_n;
     DBuf<float> d_vf;
     DBuf<double> d_vd;
@@ -824,6 +824,12 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->plan.max_var = max_var;
     c->plan.max_pair = (int32_t)std::max<int64_t>(1, c->max_pair);
     c->plan.qe = std::min(8, std::max(1, max_qe));
+    {
+        int64_t mw = 1;
+        for (int64_t r = 0; r < nrec; r++)
+            if (!binfo[r]) mw = std::max<int64_t>(mw, c->rec_ptr[r + 1] - c->rec_ptr[r]);
+        c->plan.max_w = (int32_t)mw;
+    }
     c->plan.gh = sparse ? 0 : 1;
     {
         int32_t maxgrp = 1;
@@ -999,6 +1005,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->d_band_scratch.release();
     }
     c->d_rec_eq.upload(rec_eq, s);
+    c->d_rec_w.upload(c->rec_ptr, s);
     c->d_rec_gl.upload(rec_gl, s);
     c->d_eq_sl.upload(eq_sl, s);
     c->d_eq_n.upload(eq_n, s);
@@ -1107,6 +1114,7 @@ static DevRecords records_of(mpgpu_ctx *c)
     DevRecords R{};
     R.nrec = c->nrec;
     R.rec_eq = c->d_rec_eq.p;
+    R.rec_word = c->d_rec_w.p;
     R.rec_gl = c->d_rec_gl.p;
     R.rec_pb = c->d_rec_pb.p;
     R.eq_sl = c->d_eq_sl.p;

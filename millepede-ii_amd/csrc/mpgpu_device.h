@@ -26,6 +26,7 @@ struct DevRecords {
     const int64_t *order;    // processing order of the records (nullptr = file order)
     const int32_t *band;     // per record: 0 full local fit, else bordered-band structure
                              // (mbnd+1) | mbdr << 10 | mside << 20 (record kernel skips these)
+    const int64_t *rec_word; // [nrec+1] word range of each record
     const int64_t *rec_cb;   // group-block storage: offset of the record's update in cbuf (-1: none)
     const int32_t *rec_cn;   //   number of update values of the record
     double *cbuf;            //   update buffer of the current batch (two-phase matrix build)
@@ -67,6 +68,7 @@ struct LdsPlan {
     int32_t qe;      // equations kept per record variable (1..8)
     int32_t gh;      // 1: G Gamma^-1 held in LDS (full storage), 0: formed in registers
     int32_t stage;   // doubles of the LDS staging area for the update writes (group-block storage)
+    int32_t max_w;   // words of the largest record (staged in LDS)
     int32_t bytes;
 };
 
