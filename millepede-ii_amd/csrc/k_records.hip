@@ -112,11 +112,13 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *qg = Gh + (size_t)P.gh * P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
     double *stage = qg + (size_t)P.max_var * QE;   // [stage] values of one step (coalesced stores)
     T *lv = (T *)(stage + P.stage);                // [max_w] the record's word values
-    unsigned long long *segm = (unsigned long long *)(stage + P.stage + ((size_t)P.max_w * sizeof(T) + 7) / 8); // [max_var]
+    double *ghs = stage + P.stage + ((size_t)P.max_w * sizeof(T) + 7) / 8; // [max_grp * nalc_max] rows of G Gamma^-1
+    unsigned long long *segm = (unsigned long long *)(ghs + (size_t)P.max_grp * P.nalc_max); // [max_var]
     int32_t *gv = (int32_t *)(segm + P.max_var); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
     int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
-    int32_t *segst = gvl + P.max_var;        // [max_var+1] first record variable of each segment
+    int32_t *segof = gvl + P.max_var;        // [max_var] segment of each record variable
+    int32_t *segst = segof + P.max_var;      // [max_var+1] first record variable of each segment
     int32_t *qn = segst + P.max_var + 1;     // [max_var] equations of each record variable
     int32_t *qe = qn + P.max_var;            // [max_var*QE] their indices (ascending)
     int32_t *nxt = qe + P.max_var * QE;      // [nalc_max+1] pivot list
@@ -575,14 +577,18 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                     while (b > a) { b -= a + 1; a++; }
                 }
             } else {
-                // group-block storage: the record's update per segment pair (sa >= sb), r-major,
-                // lower triangle for sa == sb, contiguous in the update buffer.  Work unit = one
-                // row of one pair: the lane keeps Gh[a][:] (and a's equations) in registers and
-                // writes the row's values contiguously.
+                // Group-block storage.  Layout of a record's update: for every segment sa (in
+                // order) the columns b = 0 .. end of sa (all record variables up to and including
+                // segment sa), each column holding the na values of the segment's rows
+                // (value (ra, b) at b * na + ra); the diagonal block is kept square, its upper
+                // part is zero.  The block of segment pair (sa, sb) is thus contiguous
+                // (columns of sb), which the block gather reads.  Lanes take columns, keep the
+                // segment's rows of G Gamma^-1 in registers, and stage 64 columns in LDS so that
+                // the buffer is written in contiguous runs.
                 const int nseg = sh[6];
                 double *dst = R.cbuf + R.rec_cb[r];
                 // equations touching each segment (bit i for equation i < 64; bit 63 also stands
-                // for all equations >= 63), so that pairs without a common equation skip g g^T
+                // for all equations >= 63): pairs without a common equation skip g g^T
                 for (int q = lane; q < nseg; q += WAVE) {
                     unsigned long long m = 0ull;
                     for (int a = segst[q]; a < segst[q + 1]; a++) {
@@ -591,81 +597,72 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                     }
                     segm[q] = m;
                 }
+                // segment of every record variable
+                for (int q = 0; q < nseg; q++)
+                    for (int a = segst[q] + lane; a < segst[q + 1]; a += WAVE) segof[a] = q;
                 __syncthreads();
-                // The 64 rows of one step are consecutive in the buffer: their values are staged in
-                // LDS and written as one contiguous range (coalesced stores).
-                int sa = 0, sb = 0, u = lane;
-                int64_t off = 0;                            // first value of the current pair
-                for (;;) {
-                    while (sa < nseg) {
-                        const int na = segst[sa + 1] - segst[sa];
-                        if (u < na) break;
-                        const int nb = segst[sb + 1] - segst[sb];
-                        u -= na;
-                        off += sa == sb ? (na * na + na) / 2 : na * nb;
-                        if (++sb > sa) { sa++; sb = 0; }
+                int64_t roff = 0;                              // start of segment sa's columns
+                for (int sa = 0; sa < nseg; sa++) {
+                    const int a0 = segst[sa], na = segst[sa + 1] - a0;
+                    const int ncolumns = a0 + na;
+                    // rows of G Gamma^-1 of the segment (l ascending, as dbavat)
+                    for (int t = lane; t < na * nalc; t += WAVE) {
+                        const int ra = t / nalc, k = t - ra * nalc + 1;
+                        double c = 0.0;
+                        for (int l = 1; l <= nalc; l++) c = c + G[(a0 + ra) * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
+                        ghs[t] = c;
                     }
-                    const bool valid = sa < nseg;
-                    if (__ballot(valid) == 0ull) break;
-                    int ra = 0, a = 0, b0 = 0, nb = 0, ncol = 0;
-                    int64_t roff = 0;
-                    if (valid) {
-                        ra = u;
-                        a = segst[sa] + ra;
-                        b0 = segst[sb];
-                        nb = segst[sb + 1] - b0;
-                        ncol = sa == sb ? ra + 1 : nb;
-                        roff = off + (sa == sb ? (ra * ra + ra) / 2 : ra * nb);
-                    }
-                    // step range [start, end): lane 0 holds the first row (rows are consecutive)
-                    const int64_t start = __shfl(roff, 0, WAVE);
-                    int64_t endv = valid ? roff + ncol : 0;
-                    for (int o = 32; o > 0; o >>= 1) endv = max(endv, __shfl_xor(endv, o, WAVE));
-                    const int span = (int)(endv - start);
-                    const bool staged = span <= P.stage;
-                    double *out_row = staged ? stage + (roff - start) : dst + roff;
-                    if (valid) {
-                        const bool common = (segm[sa] & segm[sb]) != 0ull;
-                        if (nalc <= 8) {
-                            double gh[8];
+                    __syncthreads();
+                    const bool fast = na <= 8 && nalc <= 8 && 64 * na <= P.stage;
+                    for (int B0 = 0; B0 < ncolumns; B0 += WAVE) {
+                        const int bcol = B0 + lane;
+                        const int nb_here = min(WAVE, ncolumns - B0);
+                        if (fast) {
+                            if (bcol < ncolumns) {
+                                double g[8];
 #pragma unroll
-                            for (int k = 0; k < 8; k++) {
-                                double c = 0.0;
-                                if (k < nalc)
-                                    for (int l = 1; l <= nalc; l++)
-                                        c = c + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
-                                gh[k] = c;
-                            }
-                            for (int cb = 0; cb < ncol; cb++) {
-                                const int bb = b0 + cb;
-                                double d = 0.0;
+                                for (int k = 0; k < 8; k++) g[k] = k < nalc ? G[bcol * nalc + k] : 0.0;
+                                const int sb = segof[bcol];
+                                const bool common = (segm[sa] & segm[sb]) != 0ull;
 #pragma unroll
-                                for (int k = 0; k < 8; k++)
-                                    if (k < nalc) d = d + gh[k] * G[bb * nalc + k];
-                                double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos) : 0.0;
-                                out_row[cb] = c - d;
-                            }
-                        } else {
-                            for (int cb = 0; cb < ncol; cb++) {
-                                const int bb = b0 + cb;
-                                double d = 0.0;
-                                for (int k = 0; k < nalc; k++) {
-                                    double gk = 0.0;
-                                    for (int l = 1; l <= nalc; l++)
-                                        gk = gk + G[a * nalc + l - 1] * cl[l >= k + 1 ? pk(l, k + 1) : pk(k + 1, l)];
-                                    d = d + gk * G[bb * nalc + k];
+                                for (int ra = 0; ra < 8; ra++) {
+                                    if (ra >= na) break;
+                                    const int aa = a0 + ra;
+                                    double v = 0.0;
+                                    if (bcol <= aa) {
+                                        double d = 0.0;
+#pragma unroll
+                                        for (int k = 0; k < 8; k++)
+                                            if (k < nalc) d = d + ghs[ra * nalc + k] * g[k];
+                                        const double c =
+                                            common ? dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos) : 0.0;
+                                        v = c - d;
+                                    }
+                                    stage[lane * na + ra] = v;
                                 }
-                                double c = common ? dc1<T>(a, bb, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos) : 0.0;
-                                out_row[cb] = c - d;
+                            }
+                            __syncthreads();
+                            for (int t = lane; t < nb_here * na; t += WAVE) dst[roff + (int64_t)B0 * na + t] = stage[t];
+                            __syncthreads();
+                        } else if (bcol < ncolumns) {
+                            const int sb = segof[bcol];
+                            const bool common = (segm[sa] & segm[sb]) != 0ull;
+                            for (int ra = 0; ra < na; ra++) {
+                                const int aa = a0 + ra;
+                                double v = 0.0;
+                                if (bcol <= aa) {
+                                    double d = 0.0;
+                                    for (int k = 0; k < nalc; k++) d = d + ghs[ra * nalc + k] * G[bcol * nalc + k];
+                                    const double c =
+                                        common ? dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos) : 0.0;
+                                    v = c - d;
+                                }
+                                dst[roff + (int64_t)bcol * na + ra] = v;
                             }
                         }
                     }
-                    if (staged) {
-                        __syncthreads();
-                        for (int t = lane; t < span; t += WAVE) dst[start + t] = stage[t];
-                        __syncthreads();
-                    }
-                    u += WAVE;
+                    roff += (int64_t)ncolumns * na;
+                    __syncthreads();
                 }
             }
         }
@@ -676,8 +673,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
 
 // Block gather of the two-phase matrix build: every stored block adds the values that the
 // records of the batch wrote for it (in processing order).  One wavefront per block; lane e
-// handles element e of the r x c block (row-major), diagonal blocks read the lower triangle
-// the records wrote packed.
+// handles element e of the r x c block (row-major in the matrix, column-major in the records'
+// updates); diagonal blocks use their lower triangle.
 __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg,
                                                       const int32_t *bcol, const int32_t *gfirst, const int64_t *ptr,
                                                       const int64_t *src, const double *cbuf, double *mat)
@@ -694,7 +691,7 @@ __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int6
     for (int e = lane; e < nr * nc; e += 64) {
         const int rr = e / nc, cc = e - rr * nc;
         if (diag && cc > rr) continue;
-        const int se = diag ? (rr * rr + rr) / 2 + cc : e;
+        const int se = cc * nr + rr;        // records store their blocks column by column
         // four independent partial sums keep four loads in flight
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
         int64_t q = i0;
@@ -742,8 +739,9 @@ int lds_bytes(LdsPlan &P)
     size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max +
                (size_t)(1 + P.gh) * P.max_var * P.nalc_max + (size_t)P.max_var * P.qe;
     size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * P.qe + (P.nalc_max + 1) + 8;
-    i += (size_t)P.max_w + 2 * (size_t)P.max_eq;
-    size_t b = d * 8 + (size_t)P.stage * 8 + (((size_t)P.max_w * 8 + 7) / 8) * 8 + (size_t)P.max_var * 8 + i * 4;
+    i += (size_t)P.max_w + 2 * (size_t)P.max_eq + P.max_var;
+    size_t b = d * 8 + (size_t)P.stage * 8 + (((size_t)P.max_w * 8 + 7) / 8) * 8 + (size_t)P.max_grp * P.nalc_max * 8 +
+               (size_t)P.max_var * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;
 }

@@ -835,7 +835,8 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         int32_t maxgrp = 1;
         for (int32_t g = 0; g < c->ngroup; g++) maxgrp = std::max(maxgrp, c->gfirst[g + 1] - c->gfirst[g]);
         // 64 rows of the widest group pair; larger steps are written row by row
-        c->plan.stage = sparse ? std::min(64 * maxgrp + 64, 4096) : 0;
+        c->plan.stage = sparse ? std::min(64 * maxgrp, 64 * 8) : 0;
+        c->plan.max_grp = maxgrp;
     }
     lds_bytes(c->plan);
     if (c->plan.bytes > 160 * 1024)
@@ -902,10 +903,11 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
                 if (c->group_of[v] != lastg) { lastg = c->group_of[v]; sz.push_back(0); }
                 sz.back()++;
             }
-            int64_t n = 0;
+            // segment sa stores the columns of all variables up to its end, na values each
+            int64_t n = 0, upto = 0;
             for (size_t a = 0; a < sz.size(); a++) {
-                n += (int64_t)sz[a] * (sz[a] + 1) / 2;
-                for (size_t b = 0; b < a; b++) n += (int64_t)sz[a] * sz[b];
+                upto += sz[a];
+                n += (int64_t)sz[a] * upto;
             }
             cnt_r[r] = n;
         }
@@ -959,14 +961,18 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
                     if (c->group_of[v] != lastg) { lastg = c->group_of[v]; sz.push_back(0); }
                     sz.back()++;
                 }
-                int64_t off = rec_cb[r];
+                int64_t region = rec_cb[r], upto = 0;
                 int64_t p = rec_pb[r];
-                for (size_t a = 0; a < sz.size(); a++)
+                for (size_t a = 0; a < sz.size(); a++) {
+                    int64_t col = 0;                       // first column of segment b
                     for (size_t b = 0; b <= a; b++) {
-                        src[cntb[pair_blk[p]].fetch_add(1, std::memory_order_relaxed)] = off;
-                        off += a == b ? (int64_t)sz[a] * (sz[a] + 1) / 2 : (int64_t)sz[a] * sz[b];
+                        src[cntb[pair_blk[p]].fetch_add(1, std::memory_order_relaxed)] = region + col * sz[a];
+                        col += sz[b];
                         p++;
                     }
+                    upto += sz[a];
+                    region += (int64_t)sz[a] * upto;
+                }
             }
             // processing order within every block (deterministic summation order)
 #pragma omp parallel for schedule(dynamic, 4096)

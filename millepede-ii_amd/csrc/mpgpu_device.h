@@ -69,6 +69,7 @@ struct LdsPlan {
     int32_t gh;      // 1: G Gamma^-1 held in LDS (full storage), 0: formed in registers
     int32_t stage;   // doubles of the LDS staging area for the update writes (group-block storage)
     int32_t max_w;   // words of the largest record (staged in LDS)
+    int32_t max_grp; // parameters of the largest group
     int32_t bytes;
 };
 
