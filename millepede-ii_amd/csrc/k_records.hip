@@ -15,6 +15,7 @@
 // with FP64 atomics, either into pede's packed full matrix or into the group-block layout.
 #include "mpgpu_device.h"
 #include <float.h>
+#include <algorithm>
 
 namespace mpgpu {
 
@@ -113,7 +114,9 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
     double *stage = qg + (size_t)P.max_var * QE;   // [stage] values of one step (coalesced stores)
     T *lv = (T *)(stage + P.stage);                // [max_w] the record's word values
     double *ghs = stage + P.stage + ((size_t)P.max_w * sizeof(T) + 7) / 8; // [max_grp * nalc_max] rows of G Gamma^-1
-    unsigned long long *segm = (unsigned long long *)(ghs + (size_t)P.max_grp * P.nalc_max); // [max_var]
+    const int d1n = min(P.max_grp, 16);
+    double *d1 = ghs + (size_t)P.max_grp * P.nalc_max;  // [min(max_grp,16)^2] g g^T of a diagonal block
+    unsigned long long *segm = (unsigned long long *)(d1 + (size_t)d1n * d1n); // [max_var]
     int32_t *gv = (int32_t *)(segm + P.max_var); // [max_gl] var index
     int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
     int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
@@ -612,6 +615,15 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                         for (int l = 1; l <= nalc; l++) c = c + G[(a0 + ra) * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
                         ghs[t] = c;
                     }
+                    // g g^T of the diagonal block, one element per lane
+                    const bool dcommon = segm[sa] != 0ull;
+                    const bool use_d1 = na <= 16;           // d1 holds blocks of up to 16 x 16
+                    for (int t = lane; use_d1 && t < na * na; t += WAVE) {
+                        const int ra = t / na, rb = t - ra * na;
+                        d1[t] = (dcommon && rb <= ra)
+                                    ? dc1<T>(a0 + ra, a0 + rb, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos)
+                                    : 0.0;
+                    }
                     __syncthreads();
                     const bool fast = na <= 8 && nalc <= 8 && 64 * na <= P.stage;
                     for (int B0 = 0; B0 < ncolumns; B0 += WAVE) {
@@ -623,7 +635,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
 #pragma unroll
                                 for (int k = 0; k < 8; k++) g[k] = k < nalc ? G[bcol * nalc + k] : 0.0;
                                 const int sb = segof[bcol];
-                                const bool common = (segm[sa] & segm[sb]) != 0ull;
+                                const bool diagblk = sb == sa;
+                                const bool common = !diagblk && (segm[sa] & segm[sb]) != 0ull;
 #pragma unroll
                                 for (int ra = 0; ra < 8; ra++) {
                                     if (ra >= na) break;
@@ -634,8 +647,10 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
 #pragma unroll
                                         for (int k = 0; k < 8; k++)
                                             if (k < nalc) d = d + ghs[ra * nalc + k] * g[k];
-                                        const double c =
-                                            common ? dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos) : 0.0;
+                                        double c = 0.0;
+                                        if (diagblk) c = d1[ra * na + bcol - a0];     // na <= 8 here
+                                        else if (common)
+                                            c = dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
                                         v = c - d;
                                     }
                                     stage[lane * na + ra] = v;
@@ -646,15 +661,18 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                             __syncthreads();
                         } else if (bcol < ncolumns) {
                             const int sb = segof[bcol];
-                            const bool common = (segm[sa] & segm[sb]) != 0ull;
+                            const bool diagblk = sb == sa;
+                            const bool common = !diagblk && (segm[sa] & segm[sb]) != 0ull;
                             for (int ra = 0; ra < na; ra++) {
                                 const int aa = a0 + ra;
                                 double v = 0.0;
                                 if (bcol <= aa) {
                                     double d = 0.0;
                                     for (int k = 0; k < nalc; k++) d = d + ghs[ra * nalc + k] * G[bcol * nalc + k];
-                                    const double c =
-                                        common ? dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos) : 0.0;
+                                    double c = 0.0;
+                                    if (diagblk && use_d1) c = d1[ra * na + bcol - a0];
+                                    else if (diagblk || common)
+                                        c = dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
                                     v = c - d;
                                 }
                                 dst[roff + (int64_t)bcol * na + ra] = v;
@@ -741,7 +759,7 @@ int lds_bytes(LdsPlan &P)
     size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * P.qe + (P.nalc_max + 1) + 8;
     i += (size_t)P.max_w + 2 * (size_t)P.max_eq + P.max_var;
     size_t b = d * 8 + (size_t)P.stage * 8 + (((size_t)P.max_w * 8 + 7) / 8) * 8 + (size_t)P.max_grp * P.nalc_max * 8 +
-               (size_t)P.max_var * 8 + i * 4;
+               (size_t)std::min(P.max_grp, 16) * std::min(P.max_grp, 16) * 8 + (size_t)P.max_var * 8 + i * 4;
     P.bytes = (int)b;
     return P.bytes;
 }
