@@ -1,4 +1,4 @@
-// Record-loop kernel: one 64-lane wavefront per Mille record.
+// Record-loop kernel: one workgroup of RT threads (RW wavefronts) per Mille record.
 //
 // Restates loopbf (src/pede.f90:3387-4442) for records whose local fit takes the full
 // (non-banded) path; records with bordered-band structure go to k_records_band.  Residuals
@@ -11,8 +11,12 @@
 // The per-element arithmetic of the local fit follows the reference expression by expression
 // (same operand order), so local-fit decisions (rank, chi^2 cuts) match pede; sums across
 // equations that pede forms sequentially are formed sequentially here as well, except the
-// chi^2 and weight sums, which are wavefront reductions.  Matrix contributions are scattered
-// with FP64 atomics, either into pede's packed full matrix or into the group-block layout.
+// chi^2 and weight sums, which are wavefront reductions.  The local fit (a few dozen elements)
+// runs in the first wavefront; loading, the equation lists, G and the update of the global
+// system use the whole workgroup, which also keeps several wavefronts per record in flight to
+// hide LDS and memory latency.  Matrix contributions go either into pede's packed full matrix
+// (FP64 atomics) or, for group-block storage, into a per-record update buffer that
+// k_block_gather sums per matrix block.
 #include "mpgpu_device.h"
 #include <float.h>
 #include <algorithm>
@@ -20,6 +24,17 @@
 namespace mpgpu {
 
 #define WAVE 64
+#define RT 256                  // threads per record workgroup
+#define RW (RT / WAVE)          // wavefronts per record workgroup
+// Lanes of one wavefront exchanging data through LDS: a wavefront executes its LDS operations
+// in order, so a wavefront-scope barrier (no wait for the wave's outstanding global stores,
+// unlike a workgroup barrier) orders them.
+#define WSYNC()                                                    \
+    do {                                                           \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");     \
+        __builtin_amdgcn_wave_barrier();                           \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");     \
+    } while (0)
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -77,58 +92,189 @@ __device__ __forceinline__ double dc1(int a, int b, bool perq, int QE, const int
     return c;
 }
 
+// Value of a lane-held double at a uniform lane
+__device__ __forceinline__ double rdlane(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// sqminv (src/mpnum.f90:97-213) for a packed symmetric n x n matrix held one element per lane
+// (lane t < npk = n(n+1)/2) with the right-hand side in lanes npk .. npk+n-1 (npk + n <= 64):
+// pivots in the reference's order (remaining candidates ascending, first largest diagonal
+// above eps times its original size), elimination with lane shuffles.  Writes the negated
+// result to cl[npk] and the solution to bl[n] (the form the LDS variant leaves) and returns
+// the rank.
+__device__ int gj_lanes(double acc, int npk, int n, int lane, double *cl, double *bl)
+{
+    const double eps = 16.0 * DBL_EPSILON;
+    int J = 1, K = 1;
+    if (lane < npk) {
+        J = (int)((sqrt(8.0 * lane + 1.0) - 1.0) * 0.5) + 1;
+        while ((J * J - J) / 2 > lane) J--;
+        while ((J * J + J) / 2 <= lane) J++;
+        K = lane - (J * J - J) / 2 + 1;
+    }
+    const int L1 = lane < n ? lane + 1 : 1;
+    double cv = lane < npk ? acc : 0.0;
+    const double cv0 = cv;
+    double bv = __shfl(acc, min(npk + lane, WAVE - 1), WAVE);
+    unsigned long long rem = n >= 64 ? ~0ull : (1ull << n) - 1ull;   // candidates not yet pivoted
+    int nrank = 0;
+    for (int step = 1; step <= n; step++) {
+        int k = 0;
+        double vkk = 0.0;
+        for (int j = 1; j <= n; j++) {
+            if (!((rem >> (j - 1)) & 1ull)) continue;
+            const int d = pk(j, j);
+            const double vjj = rdlane(cv, d);
+            const double m = fmax(fabs(vkk), eps * fabs(rdlane(cv0, d)));
+            if (fabs(vjj) > m) { vkk = vjj; k = j; }
+        }
+        if (k == 0) {
+            // no acceptable pivot left: the remaining rows and columns are set to zero
+            if (lane < npk && ((rem >> (J - 1)) & 1ull) && ((rem >> (K - 1)) & 1ull)) cv = 0.0;
+            if (lane < n && ((rem >> lane) & 1ull)) bv = 0.0;
+            break;
+        }
+        rem &= ~(1ull << (k - 1));
+        nrank++;
+        const double vinv = 1.0 / rdlane(cv, pk(k, k));
+        const double coJ = __shfl(cv, J >= k ? pk(J, k) : pk(k, J), WAVE);
+        const double coK = __shfl(cv, K >= k ? pk(K, k) : pk(k, K), WAVE);
+        const double coL = __shfl(cv, L1 >= k ? pk(L1, k) : pk(k, L1), WAVE);
+        const double bk = rdlane(bv, k - 1) * vinv;
+        double nv;
+        if (J == k && K == k) nv = -vinv;
+        else if (J == k) nv = vinv * coK;
+        else if (K == k) nv = vinv * coJ;
+        else nv = cv - (vinv * coK) * coJ;
+        if (lane < npk) cv = nv;
+        if (lane < n) bv = (L1 == k) ? bk : bv - bk * coL;
+    }
+    if (lane < npk) cl[lane] = -cv;
+    if (lane < n) bl[lane] = bv;
+    return nrank;
+}
+
 // a rejected record contributes nothing: its slot of the update buffer is cleared
-__device__ __forceinline__ void zero_update(const DevRecords &R, const DevModel &M, const PassOpts &o, int64_t r, int lane)
+__device__ __forceinline__ void zero_update(const DevRecords &R, const DevModel &M, const PassOpts &o, int64_t r, int tid)
 {
     if (o.icalcm != 1 || !M.sparse || R.cbuf == nullptr) return;
     const int64_t off = R.rec_cb[r];
     if (off < 0) return;
     const int n = R.rec_cn[r];
-    for (int t = lane; t < n; t += WAVE) R.cbuf[off + t] = 0.0;
+    for (int t = tid; t < n; t += RT) R.cbuf[off + t] = 0.0;
+}
+
+// LDS layout of the record kernel (byte offsets); the kernel carves its dynamic LDS with it and
+// the host sizes the allocation with it, so both always agree
+struct RecLds {
+    int gp, rm, re, lc, lw, cl, bl, dg, cs, co, G, Gh, qg, dsh, wav, segm, lmask, lv;
+    int gv, gg, gpos, gvl, ggl, segof, segst, segro, qn, qe, nxt, sh, wcnt, li, les, len;
+    int wav_stride; // doubles of each wavefront's update-stage area
+    int bytes;
+};
+__host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
+{
+    RecLds L;
+    int o = 0;
+    auto d = [&](int n) { int p = o; o += n * 8; return p; };
+    auto i4 = [&](int n) { int p = o; o += n * 4; return p; };
+    const int nv = P.max_var, ne = P.max_eq, ng = P.max_gl, nl = P.nalc_max, QE = P.qe;
+    const int d1n = P.max_grp < 16 ? P.max_grp : 16;
+    L.gp = d(ng);
+    L.rm = d(ne);
+    L.re = d(ne);
+    L.lc = d(ne);
+    L.lw = d(ne);
+    L.cl = d((nl * nl + nl) / 2);
+    L.bl = d(nl);
+    L.dg = d(nl);
+    L.cs = d(nl);
+    L.co = d(nl);
+    L.G = d(nv * nl);
+    L.Gh = d(P.gh * nv * nl);
+    L.qg = d(nv * QE);
+    L.dsh = d(8);
+    L.wav_stride = P.max_grp * P.nalc_max + d1n * d1n + P.stage;
+    L.wav = d(RW * L.wav_stride);
+    L.segm = d(nv);
+    L.lmask = d(ne);
+    L.lv = o;
+    o += ((P.max_w > 0 ? P.max_w : 1) * tsz + 7) / 8 * 8;
+    L.gv = i4(ng);
+    L.gg = i4(ng);
+    L.gpos = i4(ng);
+    L.gvl = i4(nv);
+    L.ggl = i4(nv);
+    L.segof = i4(nv);
+    L.segst = i4(nv + 1);
+    L.segro = i4(nv + 1);
+    L.qn = i4(nv);
+    L.qe = i4(nv * QE);
+    L.nxt = i4(nl + 1);
+    L.sh = i4(16);
+    L.wcnt = i4((ng + WAVE - 1) / WAVE);
+    L.li = i4(P.max_w);
+    L.les = i4(ne);
+    L.len = i4(ne);
+    L.bytes = o;
+    return L;
 }
 
 template <typename T>
-__global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, PassOpts o, RecOut out, LdsPlan P,
-                                                  int64_t rec_begin, int64_t rec_end)
+__global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_records(DevRecords R, DevModel M, PassOpts o, RecOut out, LdsPlan P,
+                                                int64_t rec_begin, int64_t rec_end)
 {
     extern __shared__ double lds[];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
     const double chuber = 1.345, cauchy = 2.3849;
     const int QE = P.qe;                     // equations per record variable on the merged update path
 
     // ---- carve LDS ----
-    double *gp = lds;                        // [max_gl] parameter value per distinct label
-    double *rm = gp + P.max_gl;              // [max_eq] corrected measurement / residual
-    double *re = rm + P.max_eq;              // [max_eq] sigma
-    double *lc = re + P.max_eq;              // [max_eq] local-fit prediction
-    double *lw = lc + P.max_eq;              // [max_eq] weight of the last local-fit pass
-    const int npk_max = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
-    double *cl = lw + P.max_eq;              // [npk_max] local matrix -> inverse
-    double *bl = cl + npk_max;               // [nalc_max] local vector -> solution
-    double *dg = bl + P.nalc_max;            // [nalc_max] |diag| for pivot test
-    double *cs = dg + P.nalc_max;            // [nalc_max] scaled pivot column
-    double *co = cs + P.nalc_max;            // [nalc_max] original pivot column
-    double *G = co + P.nalc_max;             // [max_var * nalc_max]
-    double *Gh = G + (size_t)P.max_var * P.nalc_max; // [max_var * nalc_max] (full storage only)
-    double *qg = Gh + (size_t)P.gh * P.max_var * P.nalc_max; // [max_var*QE] derivative in those equations
-    double *stage = qg + (size_t)P.max_var * QE;   // [stage] values of one step (coalesced stores)
-    T *lv = (T *)(stage + P.stage);                // [max_w] the record's word values
-    double *ghs = stage + P.stage + ((size_t)P.max_w * sizeof(T) + 7) / 8; // [max_grp * nalc_max] rows of G Gamma^-1
+    const RecLds L = rec_lds(P, (int)sizeof(T));
+    char *lb = (char *)lds;
+    double *gp = (double *)(lb + L.gp);      // [max_gl] parameter value per distinct label
+    double *rm = (double *)(lb + L.rm);      // [max_eq] corrected measurement / residual
+    double *re = (double *)(lb + L.re);      // [max_eq] sigma
+    double *lc = (double *)(lb + L.lc);      // [max_eq] local-fit prediction
+    double *lw = (double *)(lb + L.lw);      // [max_eq] weight of the last local-fit pass
+    double *cl = (double *)(lb + L.cl);      // [npk_max] local matrix -> inverse
+    double *bl = (double *)(lb + L.bl);      // [nalc_max] local vector -> solution
+    double *dg = (double *)(lb + L.dg);      // [nalc_max] |diag| for pivot test
+    double *cs = (double *)(lb + L.cs);      // [nalc_max] scaled pivot column
+    double *co = (double *)(lb + L.co);      // [nalc_max] original pivot column
+    double *G = (double *)(lb + L.G);        // [max_var * nalc_max]; during the local fit the
+    double *LD = G;                          //   dense local derivatives [neq][nalc] when they fit
+    double *Gh = (double *)(lb + L.Gh);      // [max_var * nalc_max] G Gamma^-1 (full storage only)
+    double *qg = (double *)(lb + L.qg);      // [max_var*QE] derivative of a variable in its equations
+    double *dsh = (double *)(lb + L.dsh);    // [8] double scalars
+    unsigned long long *segm = (unsigned long long *)(lb + L.segm);   // [max_var] equation mask of each segment
+    unsigned long long *lmask = (unsigned long long *)(lb + L.lmask); // [max_eq] local parameters of each equation
+    T *lv = (T *)(lb + L.lv);                // [max_w] the record's word values
+    int32_t *gv = (int32_t *)(lb + L.gv);    // [max_gl] var index
+    int32_t *gg = (int32_t *)(lb + L.gg);    // [max_gl] parameter group (variables)
+    int32_t *gpos = (int32_t *)(lb + L.gpos);   // [max_gl] position among the record's variables
+    int32_t *gvl = (int32_t *)(lb + L.gvl);     // [max_var] sorted variable list
+    int32_t *ggl = (int32_t *)(lb + L.ggl);     // [max_var] their parameter groups
+    int32_t *segof = (int32_t *)(lb + L.segof); // [max_var] segment of each record variable
+    int32_t *segst = (int32_t *)(lb + L.segst); // [max_var+1] first record variable of each segment
+    int32_t *segro = (int32_t *)(lb + L.segro); // [max_var+1] first update value of each segment
+    int32_t *qn = (int32_t *)(lb + L.qn);       // [max_var] equations of each record variable
+    int32_t *qe = (int32_t *)(lb + L.qe);       // [max_var*QE] their indices (ascending)
+    int32_t *nxt = (int32_t *)(lb + L.nxt);     // [nalc_max+1] pivot list
+    int32_t *sh = (int32_t *)(lb + L.sh);       // [16] int scalars
+    int32_t *wcnt = (int32_t *)(lb + L.wcnt);   // [max_gl/64] variables among each 64 labels
+    int32_t *li = (int32_t *)(lb + L.li);       // [max_w] the record's word indices (local / global position)
+    int32_t *les = (int32_t *)(lb + L.les);     // [max_eq] measurement word of each equation (record-local)
+    uint32_t *len = (uint32_t *)(lb + L.len);   // [max_eq] local / global word counts
+    // this wavefront's update-stage area: rows of G Gamma^-1, diagonal g g^T, staged values
     const int d1n = min(P.max_grp, 16);
-    double *d1 = ghs + (size_t)P.max_grp * P.nalc_max;  // [min(max_grp,16)^2] g g^T of a diagonal block
-    unsigned long long *segm = (unsigned long long *)(d1 + (size_t)d1n * d1n); // [max_var]
-    int32_t *gv = (int32_t *)(segm + P.max_var); // [max_gl] var index
-    int32_t *gpos = gv + P.max_gl;           // [max_gl] position among the record's variables
-    int32_t *gvl = gpos + P.max_gl;          // [max_var] sorted variable list
-    int32_t *segof = gvl + P.max_var;        // [max_var] segment of each record variable
-    int32_t *segst = segof + P.max_var;      // [max_var+1] first record variable of each segment
-    int32_t *qn = segst + P.max_var + 1;     // [max_var] equations of each record variable
-    int32_t *qe = qn + P.max_var;            // [max_var*QE] their indices (ascending)
-    int32_t *nxt = qe + P.max_var * QE;      // [nalc_max+1] pivot list
-    int32_t *sh = nxt + P.nalc_max + 1;      // [8] scalars
-    int32_t *li = sh + 8;                    // [max_w] the record's word indices (local / global position)
-    int32_t *les = li + P.max_w;             // [max_eq] measurement word of each equation (record-local)
-    uint32_t *len = (uint32_t *)(les + P.max_eq); // [max_eq] local / global word counts
+    double *ghs = (double *)(lb + L.wav) + (size_t)wv * L.wav_stride; // [max_grp * nalc_max]
+    double *d1 = ghs + (size_t)P.max_grp * P.nalc_max;          // [min(max_grp,16)^2]
+    double *stage = d1 + (size_t)d1n * d1n;                     // [P.stage]
     auto lwv = [&](int x) { return (double)lv[x]; };
 
     for (int64_t ri = rec_begin + blockIdx.x; ri < rec_end; ri += gridDim.x) {
@@ -143,303 +289,391 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         const int64_t w0 = R.rec_word[r];
         const int nw = (int)(R.rec_word[r + 1] - w0);
         if (neq > P.max_eq || ngl > P.max_gl || nw > P.max_w) {
-            if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
-            zero_update(R, M, o, r, lane);
+            if (tid == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            zero_update(R, M, o, r, tid);
             continue;
         }
         // ---- the record's words and equation table into LDS (coalesced, once) ----
-        for (int t = lane; t < nw; t += WAVE) {
+        for (int t = tid; t < nw; t += RT) {
             lv[t] = R.vf ? (T)R.vf[w0 + t] : (T)R.vd[w0 + t];
             li[t] = R.widx[w0 + t];
         }
-        for (int i = lane; i < neq; i += WAVE) {
+        for (int i = tid; i < neq; i += RT) {
             les[i] = (int)(R.eq_sl[e0 + i] - w0);
             len[i] = R.eq_n[e0 + i];
         }
-        // ---- distinct labels: parameter value, variable index, position among variables ----
-        int nalg = 0;
-        for (int base = 0; base < ngl; base += WAVE) {
-            int k = base + lane;
-            int v = 0;
-            if (k < ngl) {
-                int t = R.gl_tix[g0 + k];
-                gp[k] = M.par[t];
-                v = M.var[t];
-                gv[k] = v;
-            }
-            bool isv = (k < ngl) && v > 0;
-            unsigned long long m = __ballot(isv);
-            int pos = nalg + __popcll(m & ((1ull << lane) - 1ull));
-            if (k < ngl) gpos[k] = isv ? pos : -1;
-            if (isv && pos < P.max_var) gvl[pos] = v;
-            nalg += __popcll(m);
-        }
-        if (nalg > P.max_var) {
-            if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
-            zero_update(R, M, o, r, lane);
-            continue;
+        if (tid == 0) { sh[0] = 0; /* nalc */ sh[7] = 0; /* next update segment */ }
+        // ---- distinct labels: parameter value, variable index and group (loaded together with
+        // the words; positions among the variables follow after the barrier) ----
+        for (int k = tid; k < ngl; k += RT) {
+            const int t = R.gl_tix[g0 + k];
+            gp[k] = M.par[t];
+            const int v = M.var[t];
+            gv[k] = v;
+            gg[k] = v > 0 ? M.group_of[v] : -1;
+            const unsigned long long m = __ballot(v > 0);
+            if (lane == 0) wcnt[k >> 6] = __popcll(m);
         }
         __syncthreads();
-        // ---- first loop: subtract global contribution, count local parameters (3655-3693) ----
-        int nalc = 0;
-        for (int i = lane; i < neq; i += WAVE) {
-            const int sl = les[i];
-            const uint32_t en = len[i];
-            int nl = en & 0xffff, ng = en >> 16;
-            const int sg = sl + nl + 1;
-            double rmeas = lwv(sl);
-            for (int j = 1; j <= ng; j++) rmeas = rmeas - lwv(sg + j) * gp[li[sg + j]];
-            rm[i] = rmeas;
-            re[i] = lwv(sg);
-            lc[i] = 0.0;
-            for (int j = 1; j <= nl; j++) nalc = max(nalc, li[sl + j]);
-        }
-        nalc = wave_max(nalc);
-        if (nalc <= 0) {
-            if (lane == 0) { out.status[r] = 7; out.chi2[r] = 0.0; out.ndf[r] = 0; }
-            zero_update(R, M, o, r, lane);
+        int nalg = 0;
+        for (int c = 0; c < ((ngl + WAVE - 1) >> 6); c++) nalg += wcnt[c];
+        if (nalg > P.max_var) {
+            if (tid == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            zero_update(R, M, o, r, tid);
             __syncthreads();
             continue;
         }
-        if (nalc > P.nalc_max) {
-            if (lane == 0) { out.status[r] = 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
-            zero_update(R, M, o, r, lane);
+        for (int k = tid; k < ngl; k += RT) {
+            const int v = gv[k];
+            const unsigned long long m = __ballot(v > 0);
+            int pos = __popcll(m & ((1ull << lane) - 1ull));
+            for (int c = 0; c < (k >> 6); c++) pos += wcnt[c];
+            gpos[k] = v > 0 ? pos : -1;
+            if (v > 0) { gvl[pos] = v; ggl[pos] = gg[k]; }
+        }
+        // ---- first loop: subtract global contribution, count local parameters (3655-3693) ----
+        {
+            int nalc_t = 0;
+            for (int i = tid; i < neq; i += RT) {
+                const int sl = les[i];
+                const uint32_t en = len[i];
+                int nl = en & 0xffff, ng = en >> 16;
+                const int sg = sl + nl + 1;
+                double rmeas = lwv(sl);
+                for (int j = 1; j <= ng; j++) rmeas = rmeas - lwv(sg + j) * gp[li[sg + j]];
+                rm[i] = rmeas;
+                re[i] = lwv(sg);
+                lc[i] = 0.0;
+                for (int j = 1; j <= nl; j++) nalc_t = max(nalc_t, li[sl + j]);
+            }
+            nalc_t = wave_max(nalc_t);
+            if (lane == 0) atomicMax(&sh[0], nalc_t);
+        }
+        __syncthreads();
+        const int nalc = sh[0];
+        if (nalc <= 0 || nalc > P.nalc_max) {
+            if (tid == 0) { out.status[r] = nalc <= 0 ? 7 : 9; out.chi2[r] = 0.0; out.ndf[r] = 0; }
+            zero_update(R, M, o, r, tid);
             __syncthreads();
             continue;
         }
         const int npk = (nalc * nalc + nalc) / 2;
-        // ---- outlier-suppression iterations (3759-4094) ----
-        const int nter = (o.nloopn != 1 && o.lhuber != 0) ? o.lhuber : 1;
-        int status = 0, nrank = 0, ndf = 0;
-        double summ = 0.0;
-        float resing = 0.0f;
-        for (int iter = 1; iter <= nter; iter++) {
-            // weights
-            for (int i = lane; i < neq; i += WAVE) {
-                double rerr = re[i];
-                double wght = 1.0 / (rerr * rerr);
-                double resid = rm[i] - lc[i];
-                if (o.nloopn != 1 && iter != 1 && o.lhuber != 0) {
-                    if (iter <= 3) {
-                        if (fabs(resid) > chuber * rerr) wght = wght * chuber * rerr / fabs(resid);
-                    } else {
-                        double t = resid / rerr / cauchy;
-                        wght = wght / (1.0 + t * t);
-                    }
+        // dense local derivatives (row per equation) and the mask of the local parameters present
+        const bool ldense = (int64_t)neq * nalc <= (int64_t)P.max_var * P.nalc_max;
+        if (ldense)
+            for (int i = tid; i < neq; i += RT) {
+                const int sl = les[i];
+                const int nl = len[i] & 0xffff;
+                double *row = LD + (size_t)i * nalc;
+                for (int k = 0; k < nalc; k++) row[k] = 0.0;
+                unsigned long long msk = 0ull;
+                for (int j = 1; j <= nl; j++) {
+                    const int ij = li[sl + j];
+                    if (ij >= 1) { row[ij - 1] = lwv(sl + j); msk |= 1ull << (ij - 1); }
                 }
-                lw[i] = wght;
+                lmask[i] = msk;
             }
-            __syncthreads();
-            // local normal equations, element-parallel, equations in order (3781-3836)
-            for (int t = lane; t < npk + nalc; t += WAVE) {
-                int J, K;
-                if (t < npk) {
-                    J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
-                    while ((J * J - J) / 2 > t) J--;
-                    while ((J * J + J) / 2 <= t) J++;
-                    K = t - (J * J - J) / 2 + 1;
-                } else {
-                    J = t - npk + 1;
-                    K = 0;
+        __syncthreads();
+        // ---- local fit, chi^2 cuts and the final weights: wavefront 0 ----
+        if (wv == 0) {
+            // outlier-suppression iterations (3759-4094)
+            const int nter = (o.nloopn != 1 && o.lhuber != 0) ? o.lhuber : 1;
+            int status = 0, nrank = 0, ndf = 0;
+            double summ = 0.0;
+            float resing = 0.0f;
+            for (int iter = 1; iter <= nter; iter++) {
+                // weights
+                for (int i = lane; i < neq; i += WAVE) {
+                    double rerr = re[i];
+                    double wght = 1.0 / (rerr * rerr);
+                    double resid = rm[i] - lc[i];
+                    if (o.nloopn != 1 && iter != 1 && o.lhuber != 0) {
+                        if (iter <= 3) {
+                            if (fabs(resid) > chuber * rerr) wght = wght * chuber * rerr / fabs(resid);
+                        } else {
+                            double t = resid / rerr / cauchy;
+                            wght = wght / (1.0 + t * t);
+                        }
+                    }
+                    lw[i] = wght;
                 }
-                double acc = 0.0;
-                for (int i = 0; i < neq; i++) {
-                    const int sl = les[i];
-                    int nl = len[i] & 0xffff;
-                    double dj = 0.0, dk = 0.0;
-                    bool hj = false, hk = false;
-                    for (int j = 1; j <= nl; j++) {
-                        int ij = li[sl + j];
-                        if (ij == J) { dj = lwv(sl + j); hj = true; }
-                        if (K > 0 && ij == K) { dk = lwv(sl + j); hk = true; }
-                    }
-                    if (K == 0) {
-                        if (hj) acc = acc + lw[i] * rm[i] * dj;
-                    } else if (hj && hk) {
-                        acc = acc + lw[i] * dj * dk;
-                    }
-                }
-                if (K == 0) bl[J - 1] = acc; else cl[t] = acc;
-            }
-            __syncthreads();
-            // ---- sqminv (src/mpnum.f90:97-213), element-parallel Gauss-Jordan ----
-            const double eps = 16.0 * DBL_EPSILON;
-            for (int i = lane; i < nalc; i += WAVE) {
-                dg[i] = fabs(cl[pk(i + 1, i + 1)]);
-                nxt[i + 1] = i + 2;
-            }
-            __syncthreads();
-            if (lane == 0) { nxt[nalc] = -1; sh[1] = 1; /* next0 */ sh[2] = 0; /* nrank */ }
-            __syncthreads();
-            for (int step = 1; step <= nalc; step++) {
-                if (lane == 0) {
-                    int k = 0, l = 0, last = 0;
-                    double vkk = 0.0;
-                    int j = sh[1];
-                    while (j > 0) {
-                        double vjj = cl[pk(j, j)];
-                        double m = fmax(fabs(vkk), eps * dg[j - 1]);
-                        if (fabs(vjj) > m) { vkk = vjj; k = j; l = last; }
-                        last = j;
-                        j = nxt[last];
-                    }
-                    if (k != 0) {
-                        if (l == 0) sh[1] = nxt[k]; else nxt[l] = nxt[k];
-                        nxt[k] = 0;
-                        sh[2] = sh[2] + 1;
-                    }
-                    sh[3] = k;
-                }
-                __syncthreads();
-                const int k = sh[3];
-                if (k == 0) {
-                    for (int t = lane; t < npk; t += WAVE) {
-                        int J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
+                WSYNC();
+                // local normal equations, element-parallel, equations in order (3781-3836); small
+                // systems keep the elements in registers for the inversion
+                const bool regfit = npk + nalc <= WAVE;
+                double acc_r = 0.0;
+                for (int t = lane; t < npk + nalc; t += WAVE) {
+                    int J, K;
+                    if (t < npk) {
+                        J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
                         while ((J * J - J) / 2 > t) J--;
                         while ((J * J + J) / 2 <= t) J++;
-                        int K = t - (J * J - J) / 2 + 1;
-                        if (nxt[J] != 0 && nxt[K] != 0) cl[t] = 0.0;
+                        K = t - (J * J - J) / 2 + 1;
+                    } else {
+                        J = t - npk + 1;
+                        K = 0;
                     }
-                    for (int i = lane; i < nalc; i += WAVE) if (nxt[i + 1] != 0) bl[i] = 0.0;
-                    __syncthreads();
-                    break;
-                }
-                const int kk = pk(k, k);
-                const double vkk = 1.0 / cl[kk];
-                for (int j = lane + 1; j <= nalc; j += WAVE) {
-                    if (j != k) {
-                        double v = cl[j < k ? pk(k, j) : pk(j, k)];
-                        co[j - 1] = v;
-                        cs[j - 1] = vkk * v;
-                    }
-                }
-                __syncthreads();
-                const double bk = bl[k - 1] * vkk;
-                for (int t = lane; t < npk; t += WAVE) {
-                    int J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
-                    while ((J * J - J) / 2 > t) J--;
-                    while ((J * J + J) / 2 <= t) J++;
-                    int K = t - (J * J - J) / 2 + 1;
-                    if (J == k && K == k) cl[t] = -vkk;
-                    else if (J == k) cl[t] = cs[K - 1];
-                    else if (K == k) cl[t] = cs[J - 1];
-                    else cl[t] = cl[t] - cs[K - 1] * co[J - 1];
-                }
-                for (int j = lane + 1; j <= nalc; j += WAVE) {
-                    if (j == k) bl[j - 1] = bk;
-                    else bl[j - 1] = bl[j - 1] - bk * co[j - 1];
-                }
-                __syncthreads();
-            }
-            for (int t = lane; t < npk; t += WAVE) cl[t] = -cl[t];
-            nrank = sh[2];
-            __syncthreads();
-            // NaN check (3903-3906)
-            int nan = 0;
-            for (int i = lane; i < nalc; i += WAVE) {
-                double b = bl[i];
-                if (!(b <= 0.0) && !(b > 0.0)) nan++;
-            }
-            nan = wave_max(nan);
-            // ---- third loop: residuals, chi^2 (3919-4035) ----
-            double s_chi = 0.0, s_wt = 0.0;
-            for (int i = lane; i < neq; i += WAVE) {
-                const int sl = les[i];
-                int nl = len[i] & 0xffff;
-                double rerr = re[i];
-                double wght = 1.0 / (rerr * rerr);
-                double rmloc = 0.0;
-                for (int j = 1; j <= nl; j++) rmloc = rmloc + lwv(sl + j) * bl[li[sl + j] - 1];
-                lc[i] = rmloc;
-                double rmeas = rm[i] - rmloc;
-                double dchi2 = wght * rmeas * rmeas;
-                double resid = rmeas;
-                if (o.nloopn != 1 && iter != 1 && o.lhuber != 0) {
-                    if (iter <= 3) {
-                        if (fabs(resid) > chuber * rerr) {
-                            wght = wght * chuber * rerr / fabs(resid);
-                            dchi2 = 2.0 * chuber * (fabs(resid) / rerr - 0.5 * chuber);
+                    double acc = 0.0;
+                    if (ldense) {
+                        // equations without both parameters add an exact zero
+                        if (K == 0) {
+#pragma unroll 4
+                            for (int i = 0; i < neq; i++) {
+                                const double term = lw[i] * rm[i] * LD[i * nalc + J - 1];
+                                acc = acc + (((lmask[i] >> (J - 1)) & 1ull) ? term : 0.0);
+                            }
+                        } else {
+                            const unsigned long long need = (1ull << (J - 1)) | (1ull << (K - 1));
+#pragma unroll 4
+                            for (int i = 0; i < neq; i++) {
+                                const double term = lw[i] * LD[i * nalc + J - 1] * LD[i * nalc + K - 1];
+                                acc = acc + (((lmask[i] & need) == need) ? term : 0.0);
+                            }
                         }
                     } else {
-                        double t = resid / rerr / cauchy;
-                        wght = wght / (1.0 + t * t);
-                        dchi2 = log(1.0 + t * t) * cauchy * cauchy;
+                        for (int i = 0; i < neq; i++) {
+                            const int sl = les[i];
+                            int nl = len[i] & 0xffff;
+                            double dj = 0.0, dk = 0.0;
+                            bool hj = false, hk = false;
+                            for (int j = 1; j <= nl; j++) {
+                                int ij = li[sl + j];
+                                if (ij == J) { dj = lwv(sl + j); hj = true; }
+                                if (K > 0 && ij == K) { dk = lwv(sl + j); hk = true; }
+                            }
+                            if (K == 0) {
+                                if (hj) acc = acc + lw[i] * rm[i] * dj;
+                            } else if (hj && hk) {
+                                acc = acc + lw[i] * dj * dk;
+                            }
+                        }
+                    }
+                    if (regfit) acc_r = acc;
+                    else if (K == 0) bl[J - 1] = acc;
+                    else cl[t] = acc;
+                }
+                if (regfit) {
+                    nrank = gj_lanes(acc_r, npk, nalc, lane, cl, bl);
+                    WSYNC();
+                } else {
+                    WSYNC();
+                    // ---- sqminv (src/mpnum.f90:97-213), element-parallel Gauss-Jordan ----
+                    const double eps = 16.0 * DBL_EPSILON;
+                    for (int i = lane; i < nalc; i += WAVE) {
+                        dg[i] = fabs(cl[pk(i + 1, i + 1)]);
+                        nxt[i + 1] = i + 2;
+                    }
+                    WSYNC();
+                    if (lane == 0) { nxt[nalc] = -1; sh[1] = 1; /* next0 */ sh[2] = 0; /* nrank */ }
+                    WSYNC();
+                    for (int step = 1; step <= nalc; step++) {
+                        if (lane == 0) {
+                            int k = 0, l = 0, last = 0;
+                            double vkk = 0.0;
+                            int j = sh[1];
+                            while (j > 0) {
+                                double vjj = cl[pk(j, j)];
+                                double m = fmax(fabs(vkk), eps * dg[j - 1]);
+                                if (fabs(vjj) > m) { vkk = vjj; k = j; l = last; }
+                                last = j;
+                                j = nxt[last];
+                            }
+                            if (k != 0) {
+                                if (l == 0) sh[1] = nxt[k]; else nxt[l] = nxt[k];
+                                nxt[k] = 0;
+                                sh[2] = sh[2] + 1;
+                            }
+                            sh[3] = k;
+                        }
+                        WSYNC();
+                        const int k = sh[3];
+                        if (k == 0) {
+                            for (int t = lane; t < npk; t += WAVE) {
+                                int J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
+                                while ((J * J - J) / 2 > t) J--;
+                                while ((J * J + J) / 2 <= t) J++;
+                                int K = t - (J * J - J) / 2 + 1;
+                                if (nxt[J] != 0 && nxt[K] != 0) cl[t] = 0.0;
+                            }
+                            for (int i = lane; i < nalc; i += WAVE) if (nxt[i + 1] != 0) bl[i] = 0.0;
+                            WSYNC();
+                            break;
+                        }
+                        const int kk = pk(k, k);
+                        const double vkk = 1.0 / cl[kk];
+                        for (int j = lane + 1; j <= nalc; j += WAVE) {
+                            if (j != k) {
+                                double v = cl[j < k ? pk(k, j) : pk(j, k)];
+                                co[j - 1] = v;
+                                cs[j - 1] = vkk * v;
+                            }
+                        }
+                        WSYNC();
+                        const double bk = bl[k - 1] * vkk;
+                        for (int t = lane; t < npk; t += WAVE) {
+                            int J = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5) + 1;
+                            while ((J * J - J) / 2 > t) J--;
+                            while ((J * J + J) / 2 <= t) J++;
+                            int K = t - (J * J - J) / 2 + 1;
+                            if (J == k && K == k) cl[t] = -vkk;
+                            else if (J == k) cl[t] = cs[K - 1];
+                            else if (K == k) cl[t] = cs[J - 1];
+                            else cl[t] = cl[t] - cs[K - 1] * co[J - 1];
+                        }
+                        for (int j = lane + 1; j <= nalc; j += WAVE) {
+                            if (j == k) bl[j - 1] = bk;
+                            else bl[j - 1] = bl[j - 1] - bk * co[j - 1];
+                        }
+                        WSYNC();
+                    }
+                    for (int t = lane; t < npk; t += WAVE) cl[t] = -cl[t];
+                    nrank = sh[2];
+                    WSYNC();
+                }
+                // NaN check (3903-3906)
+                int nan = 0;
+                for (int i = lane; i < nalc; i += WAVE) {
+                    double b = bl[i];
+                    if (!(b <= 0.0) && !(b > 0.0)) nan++;
+                }
+                nan = wave_max(nan);
+                // ---- third loop: residuals, chi^2 (3919-4035) ----
+                double s_chi = 0.0, s_wt = 0.0;
+                for (int i = lane; i < neq; i += WAVE) {
+                    const int sl = les[i];
+                    int nl = len[i] & 0xffff;
+                    double rerr = re[i];
+                    double wght = 1.0 / (rerr * rerr);
+                    double rmloc = 0.0;
+                    for (int j = 1; j <= nl; j++) rmloc = rmloc + lwv(sl + j) * bl[li[sl + j] - 1];
+                    lc[i] = rmloc;
+                    double rmeas = rm[i] - rmloc;
+                    double dchi2 = wght * rmeas * rmeas;
+                    double resid = rmeas;
+                    if (o.nloopn != 1 && iter != 1 && o.lhuber != 0) {
+                        if (iter <= 3) {
+                            if (fabs(resid) > chuber * rerr) {
+                                wght = wght * chuber * rerr / fabs(resid);
+                                dchi2 = 2.0 * chuber * (fabs(resid) / rerr - 0.5 * chuber);
+                            }
+                        } else {
+                            double t = resid / rerr / cauchy;
+                            wght = wght / (1.0 + t * t);
+                            dchi2 = log(1.0 + t * t) * cauchy * cauchy;
+                        }
+                    }
+                    double down = 1.0 / sqrt(wght);
+                    s_wt += rerr / down;
+                    s_chi += dchi2;
+                    if (iter == nter) rm[i] = rmeas; // remaining residual
+                }
+                summ = wave_sum(s_chi);
+                double suwt = wave_sum(s_wt);
+                WSYNC();
+                ndf = neq - nrank;
+                resing = ((float)neq - (float)suwt) / (float)neq;
+                if (nan > 0) { status = 1; break; }
+                if (nrank != nalc) { status = 2; break; }
+                if (ndf <= 0) { status = 4; break; }
+            }
+            double chi_add = 0.0;
+            int ndf_add = 0;
+            if (status == 0) {
+                // chi^2 cuts with REAL(4) thresholds (4106-4131)
+                float chichi;
+                {
+                    // chindl(3, ndf) (src/mpnum.f90:2078-2115)
+                    const float tab3[30] = {9.0000f, 5.9146f, 4.7184f, 4.0628f, 3.6410f, 3.3436f, 3.1209f, 2.9468f,
+                                            2.8063f, 2.6902f, 2.5922f, 2.5082f, 2.4352f, 2.3711f, 2.3143f, 2.2635f,
+                                            2.2178f, 2.1764f, 2.1386f, 2.1040f, 2.0722f, 2.0428f, 2.0155f, 1.9901f,
+                                            1.9665f, 1.9443f, 1.9235f, 1.9040f, 1.8855f, 1.8681f};
+                    float c;
+                    if (ndf < 1) c = 0.0f;
+                    else if (ndf <= 30) c = tab3[ndf - 1];
+                    else {
+                        float t = 2.782170f + sqrtf((float)(ndf + ndf - 1));
+                        c = t * t / (float)(ndf + ndf);
+                    }
+                    chichi = c * (float)ndf;
+                }
+                if (o.chicut >= 0.0f) {
+                    if (summ > (double)(o.chhuge * chichi)) status = 5;
+                    else if (o.chicut > 0.0f) {
+                        float chlimt = o.chicut * chichi;
+                        if (summ > (double)chlimt) { status = 6; chi_add = (double)chlimt; ndf_add = ndf; }
                     }
                 }
-                double down = 1.0 / sqrt(wght);
-                s_wt += rerr / down;
-                s_chi += dchi2;
-                if (iter == nter) rm[i] = rmeas; // remaining residual
-            }
-            summ = wave_sum(s_chi);
-            double suwt = wave_sum(s_wt);
-            __syncthreads();
-            ndf = neq - nrank;
-            resing = ((float)neq - (float)suwt) / (float)neq;
-            if (nan > 0) { status = 1; break; }
-            if (nrank != nalc) { status = 2; break; }
-            if (ndf <= 0) { status = 4; break; }
-        }
-        double chi_add = 0.0;
-        int ndf_add = 0;
-        if (status == 0) {
-            // chi^2 cuts with REAL(4) thresholds (4106-4131)
-            float chichi;
-            {
-                // chindl(3, ndf) (src/mpnum.f90:2078-2115)
-                const float tab3[30] = {9.0000f, 5.9146f, 4.7184f, 4.0628f, 3.6410f, 3.3436f, 3.1209f, 2.9468f,
-                                        2.8063f, 2.6902f, 2.5922f, 2.5082f, 2.4352f, 2.3711f, 2.3143f, 2.2635f,
-                                        2.2178f, 2.1764f, 2.1386f, 2.1040f, 2.0722f, 2.0428f, 2.0155f, 1.9901f,
-                                        1.9665f, 1.9443f, 1.9235f, 1.9040f, 1.8855f, 1.8681f};
-                float c;
-                if (ndf < 1) c = 0.0f;
-                else if (ndf <= 30) c = tab3[ndf - 1];
-                else {
-                    float t = 2.782170f + sqrtf((float)(ndf + ndf - 1));
-                    c = t * t / (float)(ndf + ndf);
-                }
-                chichi = c * (float)ndf;
-            }
-            if (o.chicut >= 0.0f) {
-                if (summ > (double)(o.chhuge * chichi)) status = 5;
-                else if (o.chicut > 0.0f) {
-                    float chlimt = o.chicut * chichi;
-                    if (summ > (double)chlimt) { status = 6; chi_add = (double)chlimt; ndf_add = ndf; }
+                if (status == 0 && o.lhuber > 1 && o.dwcut != 0.0f && resing > o.dwcut) {
+                    status = 6;
+                    chi_add = summ;
+                    ndf_add = ndf;
                 }
             }
-            if (status == 0 && o.lhuber > 1 && o.dwcut != 0.0f && resing > o.dwcut) {
-                status = 6;
-                chi_add = summ;
-                ndf_add = ndf;
+            if (status == 0) {
+                // fourth loop, first part: chi^2 of the accepted fit and the weights of the global
+                // update (4162-4183)
+                double s_chi = 0.0;
+                const bool down = (o.nloopn != 1 && o.lhuber != 0);
+                for (int i = lane; i < neq; i += WAVE) {
+                    double rmeas = rm[i], rerr = re[i];
+                    double wght = 1.0 / (rerr * rerr);
+                    double dchi2 = wght * rmeas * rmeas;
+                    if (down) {
+                        double resid = fabs(rmeas);
+                        if (nter <= 3) {
+                            if (resid > chuber * rerr) dchi2 = 2.0 * chuber * (resid / rerr - 0.5 * chuber);
+                        } else {
+                            double t = resid / rerr / cauchy;
+                            dchi2 = log(1.0 + t * t) * cauchy * cauchy;
+                        }
+                        wght = lw[i];
+                    }
+                    s_chi += dchi2;
+                    lw[i] = wght; // weight used by the global update
+                }
+                summ = wave_sum(s_chi);
+            }
+            if (lane == 0) {
+                sh[8] = status;
+                sh[9] = ndf;
+                sh[10] = ndf_add;
+                dsh[0] = summ;
+                dsh[1] = chi_add;
             }
         }
-        if (status != 0) {
-            if (lane == 0) { out.status[r] = (int8_t)status; out.chi2[r] = chi_add * dw1; out.ndf[r] = ndf_add; }
-            zero_update(R, M, o, r, lane);
-            __syncthreads();
-            continue;
+        __syncthreads();
+        {
+            const int status = sh[8];
+            if (status != 0) {
+                if (tid == 0) { out.status[r] = (int8_t)status; out.chi2[r] = dsh[1] * dw1; out.ndf[r] = sh[10]; }
+                zero_update(R, M, o, r, tid);
+                __syncthreads();
+                continue;
+            }
         }
+        const int ndf = sh[9];
+        const double summ = dsh[0];
         // ---- fourth loop: vector, counters, g g^T, G (4162-4229) ----
         // Every record variable collects the equations it occurs in (up to QE, ascending): the
-        // vector and the counters get one update per variable.  Full storage: g g^T per
+        // vector, the counters and G get one update per variable.  Full storage: g g^T per
         // equation and the Schur complement per element with FP64 atomics.  Group-block
         // storage: the record's complete update (g g^T of the common equations minus the Schur
         // complement) is written block by block into the update buffer, to be summed per matrix
         // block by k_block_gather.  A variable in more than QE equations, or twice in one
-        // equation, makes the record take the per-equation path for the vector (and, with
+        // equation, makes the record take the per-equation path for the vector and G (and, with
         // group-block storage, an equation scan for g g^T).
         const bool twophase = M.sparse != 0;   // the host always provides the update buffer then
         if (o.icalcm == 1)
-            for (int t = lane; t < nalg * nalc; t += WAVE) G[t] = 0.0;
-        for (int t = lane; t < nalg; t += WAVE) qn[t] = 0;
-        if (lane == 0) sh[5] = 0;
-        if (o.icalcm == 1 && twophase) {
+            for (int t = tid; t < nalg * nalc; t += RT) G[t] = 0.0;
+        for (int t = tid; t < nalg; t += RT) qn[t] = 0;
+        if (tid == 0) sh[5] = 0;
+        if (o.icalcm == 1 && twophase && wv == 0) {
             // segments: runs of record variables in the same parameter group
             int nseg = 0;
             for (int base = 0; base < nalg; base += WAVE) {
                 int a = base + lane;
-                int g = a < nalg ? M.group_of[gvl[a]] : -1;
-                int gprev = (a > 0 && a < nalg) ? M.group_of[gvl[a - 1]] : -2;
+                int g = a < nalg ? ggl[a] : -1;
+                int gprev = (a > 0 && a < nalg) ? ggl[a - 1] : -2;
                 bool st = (a < nalg) && (a == 0 || g != gprev);
                 unsigned long long m = __ballot(st);
                 int s = nseg + __popcll(m & ((2ull << lane) - 1ull)) - 1;
@@ -449,28 +683,12 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             if (lane == 0) { segst[nseg] = nalg; sh[6] = nseg; }
         }
         __syncthreads();
-        double s_chi = 0.0;
-        const bool down = (o.nloopn != 1 && o.lhuber != 0);
-        for (int i = lane; i < neq; i += WAVE) {
+        for (int i = tid; i < neq; i += RT) {
             const int sl = les[i];
             const uint32_t en = len[i];
             int nl = en & 0xffff, ng = en >> 16;
             const int sg = sl + nl + 1;
-            double rmeas = rm[i], rerr = re[i];
-            double wght = 1.0 / (rerr * rerr);
-            double dchi2 = wght * rmeas * rmeas;
-            if (down) {
-                double resid = fabs(rmeas);
-                if (nter <= 3) {
-                    if (resid > chuber * rerr) dchi2 = 2.0 * chuber * (resid / rerr - 0.5 * chuber);
-                } else {
-                    double t = resid / rerr / cauchy;
-                    dchi2 = log(1.0 + t * t) * cauchy * cauchy;
-                }
-                wght = lw[i];
-            }
-            s_chi += dchi2;
-            lw[i] = wght; // weight used by the global update
+            const double wght = lw[i];
             for (int j = 1; j <= ng; j++) {
                 double gj = lwv(sg + j);
                 int pj = li[sg + j];
@@ -493,9 +711,8 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 }
             }
         }
-        summ = wave_sum(s_chi);
         __syncthreads();
-        for (int a = lane; a < nalg; a += WAVE) {
+        for (int a = tid; a < nalg; a += RT) {
             const int n = qn[a];
             if (n > QE) { sh[5] = 1; continue; }
             for (int x = 1; x < n; x++)
@@ -508,7 +725,7 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
         __syncthreads();
         const bool perq = sh[5] != 0;
         if (!perq) {
-            for (int a = lane; a < nalg; a += WAVE) {
+            for (int a = tid; a < nalg; a += RT) {
                 const int n = qn[a];
                 double v = 0.0;
                 for (int x = 0; x < n; x++) {
@@ -519,7 +736,7 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 atomicAdd(&M.cnt[gvl[a] - 1], (unsigned long long)n);
             }
         } else {
-            for (int i = lane; i < neq; i += WAVE) {
+            for (int i = tid; i < neq; i += RT) {
                 const int sl = les[i];
                 const uint32_t en = len[i];
                 int nl = en & 0xffff, ng = en >> 16;
@@ -534,49 +751,66 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
             }
         }
         if (o.icalcm == 1) {
-            // G = sum over equations of sqrt(dw) w g l^T, equations in order (4206-4219)
-            for (int i = 0; i < neq; i++) {
-                const int sl = les[i];
-                const uint32_t en = len[i];
-                int nl = en & 0xffff, ng = en >> 16;
-                const int sg = sl + nl + 1;
-                double wght = lw[i];
-                for (int t = lane; t < ng * nl; t += WAVE) {
-                    int j = t / nl + 1, k = t % nl + 1;
-                    int pj = li[sg + j];
-                    int a = gpos[pj];
-                    if (a >= 0) {
-                        int ik = li[sl + k];
-                        G[a * nalc + ik - 1] = G[a * nalc + ik - 1] + dw2 * wght * lwv(sg + j) * lwv(sl + k);
+            // G = sum over equations of sqrt(dw) w g l^T, equations in order (4206-4219): each
+            // variable's row sums over its (ascending) equations; records off the list path add
+            // equation by equation
+            if (!perq) {
+                for (int a = tid; a < nalg; a += RT) {
+                    double *Ga = G + (size_t)a * nalc;
+                    for (int x = 0; x < qn[a]; x++) {
+                        const int i = qe[a * QE + x];
+                        const double gj = qg[a * QE + x];
+                        const double wght = lw[i];
+                        const int sl = les[i];
+                        const int nl = len[i] & 0xffff;
+                        for (int k = 1; k <= nl; k++) {
+                            const int ik = li[sl + k];
+                            Ga[ik - 1] = Ga[ik - 1] + dw2 * wght * gj * lwv(sl + k);
+                        }
                     }
                 }
-                __syncthreads();
+            } else {
+                for (int i = 0; i < neq; i++) {
+                    const int sl = les[i];
+                    const uint32_t en = len[i];
+                    int nl = en & 0xffff, ng = en >> 16;
+                    const int sg = sl + nl + 1;
+                    double wght = lw[i];
+                    for (int t = tid; t < ng * nl; t += RT) {
+                        int j = t / nl + 1, k = t % nl + 1;
+                        int pj = li[sg + j];
+                        int a = gpos[pj];
+                        if (a >= 0) {
+                            int ik = li[sl + k];
+                            G[a * nalc + ik - 1] = G[a * nalc + ik - 1] + dw2 * wght * lwv(sg + j) * lwv(sl + k);
+                        }
+                    }
+                    __syncthreads();
+                }
             }
-            // Gh = G Gamma^-1 (row-wise, l ascending as dbavat); group-block storage forms the
-            // rows of Gh in registers where they are used
+            __syncthreads();
             if (!twophase) {
-                for (int a = lane; a < nalg; a += WAVE) {
+                // Gh = G Gamma^-1 (row-wise, l ascending as dbavat)
+                for (int a = tid; a < nalg; a += RT) {
                     for (int k = 1; k <= nalc; k++) {
                         double c = 0.0;
                         for (int l = 1; l <= nalc; l++) c = c + G[a * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
                         Gh[a * nalc + k - 1] = c;
                     }
                 }
-            }
-            __syncthreads();
-            if (!twophase) {
+                __syncthreads();
                 // full storage: element (a>=b) -= sum_k Gh[a,k] G[b,k]; packed (a, b) enumeration,
-                // lane starts at element `lane`, then steps by 64
+                // thread starts at element `tid`, then steps by RT
                 const int ngg = (nalg * nalg + nalg) / 2;
-                int a = (int)((sqrt(8.0 * lane + 1.0) - 1.0) * 0.5);
-                while ((a * a + a) / 2 > lane) a--;
-                while (((a + 1) * (a + 1) + (a + 1)) / 2 <= lane) a++;
-                int b = lane - (a * a + a) / 2;
-                for (int t = lane; t < ngg; t += WAVE) {
+                int a = (int)((sqrt(8.0 * tid + 1.0) - 1.0) * 0.5);
+                while ((a * a + a) / 2 > tid) a--;
+                while (((a + 1) * (a + 1) + (a + 1)) / 2 <= tid) a++;
+                int b = tid - (a * a + a) / 2;
+                for (int t = tid; t < ngg; t += RT) {
                     double c = 0.0;
                     for (int k = 0; k < nalc; k++) c = c + Gh[a * nalc + k] * G[b * nalc + k];
                     atomicAdd(&M.mat[pk64(gvl[a], gvl[b])], -c);
-                    b += WAVE;
+                    b += RT;
                     while (b > a) { b -= a + 1; a++; }
                 }
             } else {
@@ -585,29 +819,49 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                 // segment sa), each column holding the na values of the segment's rows
                 // (value (ra, b) at b * na + ra); the diagonal block is kept square, its upper
                 // part is zero.  The block of segment pair (sa, sb) is thus contiguous
-                // (columns of sb), which the block gather reads.  Lanes take columns, keep the
-                // segment's rows of G Gamma^-1 in registers, and stage 64 columns in LDS so that
-                // the buffer is written in contiguous runs.
+                // (columns of sb), which the block gather reads.  Each wavefront takes whole
+                // segments (largest first, from a shared counter); its lanes take columns, keep
+                // the segment's rows of G Gamma^-1 in registers, and stage 64 columns in LDS so
+                // that the buffer is written in contiguous runs.
                 const int nseg = sh[6];
                 double *dst = R.cbuf + R.rec_cb[r];
                 // equations touching each segment (bit i for equation i < 64; bit 63 also stands
-                // for all equations >= 63): pairs without a common equation skip g g^T
-                for (int q = lane; q < nseg; q += WAVE) {
+                // for all equations >= 63): pairs without a common equation skip g g^T; segment
+                // of every record variable
+                for (int q = tid; q < nseg; q += RT) {
                     unsigned long long m = 0ull;
                     for (int a = segst[q]; a < segst[q + 1]; a++) {
-                        if (perq) { m = ~0ull; break; }
+                        segof[a] = q;
+                        if (perq) { m = ~0ull; continue; }
                         for (int x = 0; x < qn[a]; x++) m |= 1ull << min(qe[a * QE + x], 63);
                     }
                     segm[q] = m;
                 }
-                // segment of every record variable
-                for (int q = 0; q < nseg; q++)
-                    for (int a = segst[q] + lane; a < segst[q + 1]; a += WAVE) segof[a] = q;
+                // first update value of each segment (exclusive prefix of ncolumns * na)
+                if (wv == 0) {
+                    int run = 0;
+                    for (int base = 0; base < nseg; base += WAVE) {
+                        const int q = base + lane;
+                        int v = q < nseg ? segst[q + 1] * (segst[q + 1] - segst[q]) : 0;
+                        int x = v;
+                        for (int d = 1; d < WAVE; d <<= 1) {
+                            const int y = __shfl_up(x, d, WAVE);
+                            if (lane >= d) x += y;
+                        }
+                        if (q < nseg) segro[q] = run + x - v;
+                        run += __shfl(x, WAVE - 1, WAVE);
+                    }
+                }
                 __syncthreads();
-                int64_t roff = 0;                              // start of segment sa's columns
-                for (int sa = 0; sa < nseg; sa++) {
+                for (;;) {
+                    int qq = 0;
+                    if (lane == 0) qq = atomicAdd(&sh[7], 1);
+                    qq = __shfl(qq, 0, WAVE);
+                    if (qq >= nseg) break;
+                    const int sa = nseg - 1 - qq;
                     const int a0 = segst[sa], na = segst[sa + 1] - a0;
                     const int ncolumns = a0 + na;
+                    const int64_t roff = segro[sa];
                     // rows of G Gamma^-1 of the segment (l ascending, as dbavat)
                     for (int t = lane; t < na * nalc; t += WAVE) {
                         const int ra = t / nalc, k = t - ra * nalc + 1;
@@ -624,7 +878,7 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                                     ? dc1<T>(a0 + ra, a0 + rb, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos)
                                     : 0.0;
                     }
-                    __syncthreads();
+                    WSYNC();
                     const bool fast = na <= 8 && nalc <= 8 && 64 * na <= P.stage;
                     for (int B0 = 0; B0 < ncolumns; B0 += WAVE) {
                         const int bcol = B0 + lane;
@@ -656,9 +910,9 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                                     stage[lane * na + ra] = v;
                                 }
                             }
-                            __syncthreads();
+                            WSYNC();
                             for (int t = lane; t < nb_here * na; t += WAVE) dst[roff + (int64_t)B0 * na + t] = stage[t];
-                            __syncthreads();
+                            WSYNC();
                         } else if (bcol < ncolumns) {
                             const int sb = segof[bcol];
                             const bool diagblk = sb == sa;
@@ -679,12 +933,11 @@ __global__ __launch_bounds__(WAVE) void k_records(DevRecords R, DevModel M, Pass
                             }
                         }
                     }
-                    roff += (int64_t)ncolumns * na;
-                    __syncthreads();
+                    WSYNC();
                 }
             }
         }
-        if (lane == 0) { out.status[r] = 0; out.chi2[r] = summ * dw1; out.ndf[r] = ndf; }
+        if (tid == 0) { out.status[r] = 0; out.chi2[r] = summ * dw1; out.ndf[r] = ndf; }
         __syncthreads();
     }
 }
@@ -734,10 +987,11 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
                             : hipFuncSetAttribute((const void *)k_records<double>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes);
         if (e != hipSuccess) return e;
     }
+    grid = std::max(1, std::min(grid, 256 * 8));   // workgroups loop over the records
     if (R.vf)
-        hipLaunchKernelGGL(k_records<float>, dim3(grid), dim3(WAVE), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
+        hipLaunchKernelGGL(k_records<float>, dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
     else
-        hipLaunchKernelGGL(k_records<double>, dim3(grid), dim3(WAVE), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
+        hipLaunchKernelGGL(k_records<double>, dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
     return hipGetLastError();
 }
 
@@ -753,14 +1007,8 @@ hipError_t launch_block_gather(int64_t nblock, const int64_t *boff, const int32_
 
 int lds_bytes(LdsPlan &P)
 {
-    int npk = (P.nalc_max * P.nalc_max + P.nalc_max) / 2;
-    size_t d = (size_t)P.max_gl + 4 * (size_t)P.max_eq + npk + 4 * (size_t)P.nalc_max +
-               (size_t)(1 + P.gh) * P.max_var * P.nalc_max + (size_t)P.max_var * P.qe;
-    size_t i = 2 * (size_t)P.max_gl + 3 * (size_t)P.max_var + 1 + (size_t)P.max_var * P.qe + (P.nalc_max + 1) + 8;
-    i += (size_t)P.max_w + 2 * (size_t)P.max_eq + P.max_var;
-    size_t b = d * 8 + (size_t)P.stage * 8 + (((size_t)P.max_w * 8 + 7) / 8) * 8 + (size_t)P.max_grp * P.nalc_max * 8 +
-               (size_t)std::min(P.max_grp, 16) * std::min(P.max_grp, 16) * 8 + (size_t)P.max_var * 8 + i * 4;
-    P.bytes = (int)b;
+    // the larger payload type sizes the word stage
+    P.bytes = rec_lds(P, (int)sizeof(double)).bytes;
     return P.bytes;
 }
 

@@ -89,6 +89,34 @@ def test_downweighting_pass(orc):
 
 
 @pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_rank_deficient_local_fits(orc, matsto):
+    """Local systems without full rank: in every 5th record local parameter 3 never occurs (an
+    empty column), in every 7th its derivative is ten times that of parameter 1 in every
+    equation (proportional columns, the pivot search stops at rank 3).  pede rejects both
+    (nrank != nalc); rejects, chi^2, ndf and the update must agree with the oracle."""
+    rp, ii, gg, _ = mpgpu.pixel_records(13, 4, 6, 3, 3000, 3, 6)
+    ii = ii.copy()
+    gg = gg.copy()
+    recs = LB.decode(rp, ii, gg)
+    nmod = 0
+    for r, eqs in enumerate(recs):
+        if r % 5 and r % 7:
+            continue
+        nmod += 1
+        for (sL, sG, lG) in eqs:
+            for w in range(sL + 1, sG):
+                if ii[w] == 3:
+                    if r % 5 == 0:
+                        ii[w] = 4 if all(ii[x] != 4 for x in range(sL + 1, sG)) else ii[w]
+                    else:
+                        gg[w] = 10.0 * gg[sL + 1]
+    P = _pede(rp, ii, gg, matsto=matsto, mreqenf=5, mcount=1, metsol=1 if matsto == 1 else 4)
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=5, mcount=1)
+    s_gpu, _, _ = _compare_pass(P, orc, rp, ii, gg, lab, var)
+    assert s_gpu["nrej"][1] >= nmod * 9 // 10
+
+
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
 def test_brokenlines_band_pass(orc, matsto):
     """Records whose local fits take pede's bordered-band path (sqmibb, band 3 + border 4):
     the GPU band kernel's matrix, vector, chi^2, ndf, reject counts and counters against the
