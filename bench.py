@@ -40,6 +40,19 @@ def workload_bytes(rp):
     return float(np.sum(4 + 8 * nw + 16 * (nalg * (nalg + 1) // 2 + nalg))), float(np.mean(nalg))
 
 
+def committed_traffic():
+    """HBM bytes per k_records launch from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by the same counter's reading for a copy of known size);
+    None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "k_records_pixel2M_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return {"bytes_per_launch": float(line.split(":", 1)[1]), "source": "profiles/k_records_pixel2M_traffic.txt"}
+    return None
+
+
 def cpu_baseline(lab_var, rp, ii, gg, target_s=12.0):
     """The oracle (single-threaded C restatement of the record loop) on a bounded sample of
     the same records, accumulating into a hash table (pede's sparse layout is not rebuilt)."""
@@ -95,16 +108,20 @@ def serving_run(njobs, rate=200.0, seed=7):
         if delay > 0:
             time.sleep(delay)
         ids.append(S.submit(rec, par, cons, **opt))
-    lat, bad = [], 0
-    for i in ids:
+    lat, bad, codes = [], 0, {}
+    for k, i in enumerate(ids):
         rc, st, _, _ = S.wait(i)
         lat.append(st["latency_ms"])
         bad += rc != 0
+        if rc:
+            key = f"rc{rc}/{'cholesky' if jobs[k][4] <= 5000 else 'minres'}"
+            codes[key] = codes.get(key, 0) + 1
     t_end = time.perf_counter()
     S.close()
     lat = np.array(lat)
     out = dict(jobs=njobs, arrival_rate=rate, jobs_per_s=njobs / (t_end - t0), p50_ms=float(np.percentile(lat, 50)),
-               p99_ms=float(np.percentile(lat, 99)), failed=int(bad), workers=8, budget_gb=200)
+               p99_ms=float(np.percentile(lat, 99)), failed=int(bad), failed_codes=codes, workers=8,
+               budget_gb=200)
     log(f"serving: {out}")
     return out
 
@@ -157,13 +174,17 @@ def main():
     for _ in range(args.warmup):
         st = P.loopn()
         log(f"warmup pass: kernel {st['ms_kernel']:.2f} ms, total {st['ms_total']:.2f} ms, rejects {st['nrej']}")
-    times, kms = [], []
+    times, recs_ms, recs_launches, gath_ms, gath_launches = [], [], [], [], []
     for k in range(args.steps):
         t = time.perf_counter()
         st = P.loopn()           # synchronises the device before returning
         times.append(time.perf_counter() - t)
-        kms.append(st["ms_kernel"])
-        log(f"step {k}: {times[-1] * 1e3:.2f} ms wall, kernel {st['ms_kernel']:.2f} ms")
+        recs_ms.append(st["ms_records"])
+        recs_launches.append(st["n_records"])
+        gath_ms.append(st["ms_gather"])
+        gath_launches.append(st["n_gather"])
+        log(f"step {k}: {times[-1] * 1e3:.2f} ms wall, record loop {st['ms_kernel']:.2f} ms "
+            f"(k_records {st['ms_records']:.2f} ms in {st['n_records']}, gather {st['ms_gather']:.2f} ms)")
     tot = sum(times)
     nrec = len(rp) - 1
     value = nrec * args.steps / tot
@@ -177,13 +198,22 @@ def main():
                       "records": nrec, "ntot": ntot, "nvar": nvar, "ncon": ncon,
                       "stored_matrix_words": info["stored"], "structure_s": t_struct}}
 
+    # roofline of the dominant kernel (k_records): algorithmic bytes per launch over its average
+    # launch duration, both from the timed steps (HIP events around every launch, on the
+    # library's stream)
     abytes, mean_nalg = workload_bytes(rp)
-    kavg = float(np.mean(kms)) / 1e3
-    achieved = abytes / kavg / 1e9
+    nl = sum(recs_launches)
+    kavg = sum(recs_ms) / max(nl, 1) / 1e3
+    per_launch = abytes * args.steps / max(nl, 1)
+    achieved = per_launch / kavg / 1e9
+    traffic = committed_traffic()
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                       "kernel": "k_records<float>", "kernel_ms": kavg * 1e3,
-                       "algorithmic_bytes_per_launch": abytes, "mean_nalg": mean_nalg}
+                       "frac": achieved / HBM_PEAK_GBS,
+                       "traffic": traffic["bytes_per_launch"] if traffic else None,
+                       "kernel": "k_records<float>", "kernel_ms": kavg * 1e3, "launches_per_step": nl / args.steps,
+                       "algorithmic_bytes_per_launch": per_launch, "mean_nalg": mean_nalg,
+                       "gather_ms_per_launch": sum(gath_ms) / max(sum(gath_launches), 1),
+                       "traffic_source": traffic["source"] if traffic else None}
     lab, var, cnt, gsl = P.labels()
     lab_var = (lab, var, P.parameters(), P.nvar)
     out["pass_chi2"] = st["chi2"]

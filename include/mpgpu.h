@@ -77,8 +77,12 @@ typedef struct mpgpu_pass_stats {
     int64_t nrej[6];     /* rejects: NaN, rank, cond(band), ndf<=0, huge chi2, chi2 cut   */
     int64_t naccept;     /* accepted local fits                                           */
     int64_t nrec;        /* records processed                                             */
-    double ms_kernel;    /* device time of the record kernel (HIP events)                 */
+    double ms_kernel;    /* device time of the record loop (all record and gather kernels) */
     double ms_total;     /* device time of the whole pass                                 */
+    double ms_records;   /* device time of the record-kernel launches (HIP events each)  */
+    int64_t n_records;   /* number of record-kernel launches                              */
+    double ms_gather;    /* device time of the block-gather launches (group-block storage) */
+    int64_t n_gather;    /* number of block-gather launches                               */
 } mpgpu_pass_stats;
 
 /* Statistics of one solve */

@@ -22,7 +22,10 @@ MODULE mpgpu_mod
         INTEGER(c_int64_t) :: ndf
         REAL(c_double)     :: wndf
         INTEGER(c_int64_t) :: nrej(6), naccept, nrec
-        REAL(c_double)     :: ms_kernel, ms_total
+        REAL(c_double)     :: ms_kernel, ms_total, ms_records
+        INTEGER(c_int64_t) :: n_records
+        REAL(c_double)     :: ms_gather
+        INTEGER(c_int64_t) :: n_gather
     END TYPE mpgpu_pass_stats
 
     TYPE, BIND(C) :: mpgpu_solve_stats

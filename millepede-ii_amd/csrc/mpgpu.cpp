@@ -1167,10 +1167,13 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     API_TRY
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    hipEvent_t e0, e1, e2;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventCreate(&e2));
+    EventPool ev;    // destroys its events on every exit
+    hipEvent_t e0 = ev.get(), e1 = ev.get(), e2 = ev.get();
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> t_rec, t_gat;   // per launch
+    auto timed = [&](std::vector<std::pair<hipEvent_t, hipEvent_t>> &v) {
+        v.push_back({ev.get(), ev.get()});
+        return v.back();
+    };
     HIPCHK(hipEventRecord(e0, s));
     c->d_vec.zero(s);
     c->d_cnt.zero(s);
@@ -1196,13 +1199,22 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
                 const auto &B = c->batches[bt];
                 if (B.q1 <= B.q0) continue;
                 const int g = (int)std::min<int64_t>(B.q1 - B.q0, 256 * 40);
+                auto tr = timed(t_rec);
+                HIPCHK(hipEventRecord(tr.first, s));
                 HIPCHK(launch_records(R, M, o, out, c->plan, B.q0, B.q1, std::max(g, 1), s));
+                HIPCHK(hipEventRecord(tr.second, s));
+                auto tg = timed(t_gat);
+                HIPCHK(hipEventRecord(tg.first, s));
                 HIPCHK(launch_block_gather(c->nblock, c->d_boff.p, c->d_brg.p, c->d_bcol.p, c->d_gfirst.p,
                                            c->d_inc_ptr.p + bt * (c->nblock + 1), c->d_inc_src.p + B.inc0, c->d_cbuf.p,
                                            c->d_mat.p, s));
+                HIPCHK(hipEventRecord(tg.second, s));
             }
         } else {
+            auto tr = timed(t_rec);
+            HIPCHK(hipEventRecord(tr.first, s));
             HIPCHK(launch_records(R, M, o, out, c->plan, 0, c->nrec, std::max(grid, 1), s));
+            HIPCHK(hipEventRecord(tr.second, s));
         }
     }
     if (c->nband > 0)
@@ -1290,9 +1302,16 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     float k_ms = 0.f, t_ms = 0.f;
     HIPCHK(hipEventElapsedTime(&k_ms, e1, e2));
     HIPCHK(hipEventElapsedTime(&t_ms, e0, e2));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
+    auto sum_ms = [](const std::vector<std::pair<hipEvent_t, hipEvent_t>> &v) {
+        double t = 0.0;
+        for (const auto &p : v) {
+            float m = 0.f;
+            HIPCHK(hipEventElapsedTime(&m, p.first, p.second));
+            t += m;
+        }
+        return t;
+    };
+    const double r_ms = sum_ms(t_rec), g_ms = sum_ms(t_gat);
     if (st) {
         std::memset(st, 0, sizeof(*st));
         st->chi2 = chi2 + chi_extra;
@@ -1303,6 +1322,10 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
         st->nrec = c->nrec;
         st->ms_kernel = k_ms;
         st->ms_total = t_ms;
+        st->ms_records = r_ms;
+        st->n_records = (int64_t)t_rec.size();
+        st->ms_gather = g_ms;
+        st->n_gather = (int64_t)t_gat.size();
     }
     return 0;
     API_CATCH(c)

@@ -59,6 +59,25 @@ template <typename T> struct DBuf {
     }
 };
 
+// HIP events that live until the end of the enclosing scope
+struct EventPool {
+    std::vector<hipEvent_t> ev;
+    EventPool() = default;
+    EventPool(const EventPool &) = delete;
+    EventPool &operator=(const EventPool &) = delete;
+    ~EventPool()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    hipEvent_t get()
+    {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ev.push_back(e);
+        return e;
+    }
+};
+
 struct EqBlock {
     int kind; // 0 constraint, 1 measurement
     double rhs, sigma;

@@ -46,7 +46,9 @@ class Options(C.Structure):
 
 class PassStats(C.Structure):
     _fields_ = [("chi2", C.c_double), ("ndf", C.c_int64), ("wndf", C.c_double), ("nrej", C.c_int64 * 6),
-                ("naccept", C.c_int64), ("nrec", C.c_int64), ("ms_kernel", C.c_double), ("ms_total", C.c_double)]
+                ("naccept", C.c_int64), ("nrec", C.c_int64), ("ms_kernel", C.c_double), ("ms_total", C.c_double),
+                ("ms_records", C.c_double), ("n_records", C.c_int64), ("ms_gather", C.c_double),
+                ("n_gather", C.c_int64)]
 
 
 class SolveStats(C.Structure):
@@ -237,7 +239,8 @@ class Pede:
         st = PassStats()
         self._chk(self.L.mpgpu_loopn(self.h, icalcm, nloopn, chicut, C.byref(st)))
         return dict(chi2=st.chi2, ndf=st.ndf, nrej=list(st.nrej), naccept=st.naccept, nrec=st.nrec,
-                    ms_kernel=st.ms_kernel, ms_total=st.ms_total)
+                    ms_kernel=st.ms_kernel, ms_total=st.ms_total,
+                    ms_records=st.ms_records, n_records=st.n_records, ms_gather=st.ms_gather, n_gather=st.n_gather)
 
     def vector(self):
         v = np.zeros(self.nall)
