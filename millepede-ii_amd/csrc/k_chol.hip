@@ -161,8 +161,9 @@ __global__ __launch_bounds__(256) void k_syrk_update(int n, T *a, int lda, int k
 // forward / backward substitution with the factor (column-major lower L)
 // forward: L y = b ; tile-by-tile: solve diagonal tile (one block), then update the rest
 template <typename T, typename V>
-__global__ __launch_bounds__(256) void k_trsv_tile(int n, const T *a, int lda, int k0, V *x, int trans)
+__global__ __launch_bounds__(256) void k_trsv_tile(int n, const T *a, int lda, int k0, V *x, int64_t ldx, int trans)
 {
+    x += blockIdx.y * ldx;   // right-hand side of this block
     __shared__ T l[NB][NB + 1];
     __shared__ V xs[NB];
     const int nb = min(NB, n - k0);
@@ -192,8 +193,9 @@ __global__ __launch_bounds__(256) void k_trsv_tile(int n, const T *a, int lda, i
 }
 // forward update: x[i] -= sum_{j in tile} L[i][j] x[j] for i >= k0+nb
 template <typename T, typename V>
-__global__ void k_gemv_fwd(int n, const T *a, int lda, int k0, V *x)
+__global__ void k_gemv_fwd(int n, const T *a, int lda, int k0, V *x, int64_t ldx)
 {
+    x += blockIdx.y * ldx;
     const int nb = min(NB, n - k0);
     int i = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -203,8 +205,9 @@ __global__ void k_gemv_fwd(int n, const T *a, int lda, int k0, V *x)
 }
 // backward update: x[j] -= sum_{i in tile} L[i][j] x[i] for j < k0
 template <typename T, typename V>
-__global__ void k_gemv_bwd(int n, const T *a, int lda, int k0, V *x)
+__global__ void k_gemv_bwd(int n, const T *a, int lda, int k0, V *x, int64_t ldx)
 {
+    x += blockIdx.y * ldx;
     const int nb = min(NB, n - k0);
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= k0) return;
@@ -240,27 +243,46 @@ int chol_factor(int n, double *a, int lda, double *work, int *info_dev, hipStrea
     return chol_factor_t<float>(n, a32, lda, info_dev, s);
 }
 
-template <typename T, typename V> static void chol_solve_t(int n, const T *a, int lda, V *x, hipStream_t s)
+// nrhs right-hand sides, column q at x + q * ldx, solved side by side (grid dimension y)
+template <typename T, typename V>
+static void chol_solve_t(int n, const T *a, int lda, V *x, int nrhs, int64_t ldx, hipStream_t s)
 {
     for (int k0 = 0; k0 < n; k0 += NB) {
-        hipLaunchKernelGGL((k_trsv_tile<T, V>), dim3(1), dim3(256), 0, s, n, a, lda, k0, x, 0);
+        hipLaunchKernelGGL((k_trsv_tile<T, V>), dim3(1, nrhs), dim3(256), 0, s, n, a, lda, k0, x, ldx, 0);
         int rest = n - k0 - min(NB, n - k0);
-        if (rest > 0) hipLaunchKernelGGL((k_gemv_fwd<T, V>), dim3((rest + 255) / 256), dim3(256), 0, s, n, a, lda, k0, x);
+        if (rest > 0)
+            hipLaunchKernelGGL((k_gemv_fwd<T, V>), dim3((rest + 255) / 256, nrhs), dim3(256), 0, s, n, a, lda, k0, x, ldx);
     }
     int last = ((n - 1) / NB) * NB;
     for (int k0 = last; k0 >= 0; k0 -= NB) {
-        hipLaunchKernelGGL((k_trsv_tile<T, V>), dim3(1), dim3(256), 0, s, n, a, lda, k0, x, 1);
-        if (k0 > 0) hipLaunchKernelGGL((k_gemv_bwd<T, V>), dim3((k0 + 255) / 256), dim3(256), 0, s, n, a, lda, k0, x);
+        hipLaunchKernelGGL((k_trsv_tile<T, V>), dim3(1, nrhs), dim3(256), 0, s, n, a, lda, k0, x, ldx, 1);
+        if (k0 > 0) hipLaunchKernelGGL((k_gemv_bwd<T, V>), dim3((k0 + 255) / 256, nrhs), dim3(256), 0, s, n, a, lda, k0, x, ldx);
     }
 }
 
-void chol_solve(int n, const double *a, int lda, double *x, hipStream_t s) { chol_solve_t<double, double>(n, a, lda, x, s); }
+void chol_solve(int n, const double *a, int lda, double *x, hipStream_t s) { chol_solve_t<double, double>(n, a, lda, x, 1, 0, s); }
+
+void chol_solve_multi(int n, const double *a, int lda, double *x, int nrhs, int64_t ldx, hipStream_t s)
+{
+    if (nrhs > 0) chol_solve_t<double, double>(n, a, lda, x, nrhs, ldx, s);
+}
+
+// diagonal of a column-major FP32 factor
+__global__ void k_diag_f32(int n, const float *a, int lda, float *d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = a[(int64_t)i * lda + i];
+}
+void chol_diag32(int n, const float *a, int lda, float *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_diag_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, a, lda, d);
+}
 
 // FP32 factor, FP64 right-hand side accumulated in FP64 (the factor entries are widened)
 void chol_solve32(int n, const float *a, int lda, double *x, float *xf, hipStream_t s)
 {
     (void)xf;
-    chol_solve_t<float, double>(n, a, lda, x, s);
+    chol_solve_t<float, double>(n, a, lda, x, 1, 0, s);
 }
 
 // A += rho C^T C over the full column-major matrix (both triangles); one block per constraint

@@ -1940,11 +1940,31 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
             rho = cmax > 0.0 ? (amax > 0.0 ? amax : 1.0) / cmax : 0.0;
             l_add_cct(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, A.p, (int)lda, s);
         }
-        const bool fp32 = c->opt.metsol == MPGPU_CHOLESKY_FP32_IR;
+        bool fp32 = c->opt.metsol == MPGPU_CHOLESKY_FP32_IR;
         DBuf<float> A32;
         DBuf<float> xf;
         if (fp32) { A32.alloc((size_t)lda * nv); xf.alloc(nv); }
         int rc = chol_factor(nv, A.p, (int)lda, work.p, info.p, s, fp32 ? 1 : 0, fp32 ? A32.p : nullptr);
+        if (rc == 0 && fp32) {
+            // (max/min diagonal of the factor)^2 bounds the condition number from below; beyond
+            // 1e4 the FP32 factor preconditions FP64 conjugate gradients poorly (the error
+            // shrinks by ~eps32*cond per step at best)
+            chol_diag32(nv, A32.p, (int)lda, xf.p, s);
+            std::vector<float> dg(nv);
+            HIPCHK(hipMemcpyAsync(dg.data(), xf.p, sizeof(float) * nv, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            double dmax = 0.0, dmin = DBL_MAX;
+            for (float d : dg) { dmax = std::max(dmax, (double)std::fabs(d)); dmin = std::min(dmin, (double)std::fabs(d)); }
+            if (!(dmin > 0.0) || (dmax / dmin) * (dmax / dmin) > 1e4) rc = -1;
+        }
+        if (rc != 0 && fp32) {
+            // FP32 lacks the range for this system (a pivot vanished, or the factor would need
+            // many refinement steps): the FP64 block, still intact, is factorised in FP64
+            // instead and the solve is a direct one
+            fp32 = false;
+            loc.method = MPGPU_DECOMPOSITION;
+            rc = chol_factor(nv, A.p, (int)lda, work.p, info.p, s, 0, nullptr);
+        }
         if (rc > 0) throw MpError(29, "Aborted, matrix factorisation failed (not positive definite at " + std::to_string(rc) + ")");
         loc.nrank = nv;
         // solve with constraints via the Schur complement S = C A^-1 C^T (block elimination)
@@ -1996,25 +2016,32 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
             std::vector<double> Ct((size_t)m * nv, 0.0);
             for (int j = 0; j < m; j++)
                 for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) Ct[(size_t)j * nv + c->cidx[q] - 1] += c->cval[q];
-            std::vector<double> Z(Ct);
-            DBuf<double> zc;
-            zc.alloc(nv);
-            for (int j = 0; j < m; j++) {
-                zc.upload(Z.data() + (size_t)j * nv, nv, s);
-                solveA(zc.p);
-                zc.download(Z.data() + (size_t)j * nv, nv, s);
-            }
             std::vector<double> bp(n);
             b.download(bp.data(), n, s);
             HIPCHK(hipStreamSynchronize(s));
-            std::vector<double> bt(bp.begin(), bp.begin() + nv);     // b + rho C^T d
+            // right-hand sides: the m columns of C^T and b + rho C^T d, solved together by the
+            // FP64 factor (the FP32 path refines each one)
+            std::vector<double> Z(Ct);
+            Z.resize((size_t)(m + 1) * nv);
+            double *bt = Z.data() + (size_t)m * nv;
+            for (int i = 0; i < nv; i++) bt[i] = bp[i];
             for (int j = 0; j < m; j++)
                 for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) bt[c->cidx[q] - 1] += rho * c->cval[q] * bp[nv + j];
-            zc.upload(bt.data(), nv, s);
-            solveA(zc.p);
-            std::vector<double> y(nv);
-            zc.download(y.data(), nv, s);
+            DBuf<double> zc;
+            if (!fp32) {
+                zc.upload(Z, s);
+                chol_solve_multi(nv, A.p, (int)lda, zc.p, m + 1, nv, s);
+                zc.download(Z.data(), Z.size(), s);
+            } else {
+                zc.alloc(nv);
+                for (int j = 0; j <= m; j++) {
+                    zc.upload(Z.data() + (size_t)j * nv, nv, s);
+                    solveA(zc.p);
+                    zc.download(Z.data() + (size_t)j * nv, nv, s);
+                }
+            }
             HIPCHK(hipStreamSynchronize(s));
+            const double *y = Z.data() + (size_t)m * nv;
             // S (m x m) and rhs
             std::vector<double> S((size_t)m * m), g(m);
             for (int i = 0; i < m; i++) {

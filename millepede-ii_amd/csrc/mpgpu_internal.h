@@ -143,6 +143,8 @@ void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, con
 int chol_factor(int n, double *a, int lda, double *work, int *info_dev, hipStream_t s, int fp32, float *a32);
 void chol_solve(int n, const double *a, int lda, double *x, hipStream_t s);
 void chol_solve32(int n, const float *a, int lda, double *x, float *xf, hipStream_t s);
+void chol_solve_multi(int n, const double *a, int lda, double *x, int nrhs, int64_t ldx, hipStream_t s);
+void chol_diag32(int n, const float *a, int lda, float *d, hipStream_t s);
 void l_packed_to_full(int n, const double *packed, double *full, int lda, hipStream_t s);
 void l_full_to_f32(int64_t nn, const double *a, float *b, hipStream_t s);
 void l_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, double *a,
