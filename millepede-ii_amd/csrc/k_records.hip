@@ -158,6 +158,80 @@ __device__ int gj_lanes(double acc, int npk, int n, int lane, double *cl, double
     return nrank;
 }
 
+// One segment's update for records with nalc = NALC local parameters and a segment of na <= 8
+// rows: lanes take columns (the column's row of G in registers), pairs of rows of G Gamma^-1 are
+// read together, values are staged per 64 columns in LDS and written out in contiguous runs.
+template <int NALC, typename T>
+__device__ __forceinline__ void stage_segment(int sa, int a0, int na, int ncolumns, double *dst, const double *G,
+                                              const double *ghs, const double *d1, double *stage, const int32_t *segof,
+                                              const unsigned long long *segm, int lane, bool perq, int QE,
+                                              const int32_t *qn, const int32_t *qe, const double *qg, const double *lw,
+                                              double dw1, const int32_t *les, const uint32_t *len, const int32_t *li,
+                                              const T *lv, int neq, const int32_t *gpos)
+{
+    const unsigned long long msa = segm[sa];
+    for (int B0 = 0; B0 < ncolumns; B0 += WAVE) {
+        const int bcol = B0 + lane;
+        const int nb_here = min(WAVE, ncolumns - B0);
+        if (bcol < ncolumns) {
+            double g[NALC];
+#pragma unroll
+            for (int k = 0; k < NALC; k++) g[k] = G[bcol * NALC + k];
+            const int sb = segof[bcol];
+            const bool diagblk = sb == sa;
+            const bool common = !diagblk && (msa & segm[sb]) != 0ull;
+            auto emit = [&](int ra, double d) {
+                const int aa = a0 + ra;
+                double v = 0.0;
+                if (bcol <= aa) {
+                    double c = 0.0;
+                    if (diagblk) c = d1[ra * na + bcol - a0];     // na <= 8 here
+                    else if (common)
+                        c = dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
+                    v = c - d;
+                }
+                stage[lane * na + ra] = v;
+            };
+#pragma unroll
+            for (int r2 = 0; r2 < 8; r2 += 2) {
+                if (r2 >= na) break;
+                // rows r2 and r2+1 (the second one is not used when na is odd; its read stays
+                // inside this wavefront's stage area)
+                double h0[NALC], h1[NALC];
+#pragma unroll
+                for (int k = 0; k < NALC; k++) {
+                    h0[k] = ghs[r2 * NALC + k];
+                    h1[k] = ghs[(r2 + 1) * NALC + k];
+                }
+                double e0 = 0.0, e1 = 0.0;
+#pragma unroll
+                for (int k = 0; k < NALC; k++) {
+                    e0 = e0 + h0[k] * g[k];
+                    e1 = e1 + h1[k] * g[k];
+                }
+                emit(r2, e0);
+                if (r2 + 1 < na) emit(r2 + 1, e1);
+            }
+        }
+        WSYNC();
+        // the staged values of this step are contiguous in the update buffer
+        const int ntot = nb_here * na;   // <= 8 * 64
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = lane + q * WAVE;
+            v[q] = t < ntot ? stage[t] : 0.0;
+        }
+        double *o = dst + (int64_t)B0 * na;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = lane + q * WAVE;
+            if (t < ntot) o[t] = v[q];
+        }
+        WSYNC();
+    }
+}
+
 // a rejected record contributes nothing: its slot of the update buffer is cleared
 __device__ __forceinline__ void zero_update(const DevRecords &R, const DevModel &M, const PassOpts &o, int64_t r, int tid)
 {
@@ -880,40 +954,26 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     }
                     WSYNC();
                     const bool fast = na <= 8 && nalc <= 8 && 64 * na <= P.stage;
-                    for (int B0 = 0; B0 < ncolumns; B0 += WAVE) {
-                        const int bcol = B0 + lane;
-                        const int nb_here = min(WAVE, ncolumns - B0);
-                        if (fast) {
-                            if (bcol < ncolumns) {
-                                double g[8];
-#pragma unroll
-                                for (int k = 0; k < 8; k++) g[k] = k < nalc ? G[bcol * nalc + k] : 0.0;
-                                const int sb = segof[bcol];
-                                const bool diagblk = sb == sa;
-                                const bool common = !diagblk && (segm[sa] & segm[sb]) != 0ull;
-#pragma unroll
-                                for (int ra = 0; ra < 8; ra++) {
-                                    if (ra >= na) break;
-                                    const int aa = a0 + ra;
-                                    double v = 0.0;
-                                    if (bcol <= aa) {
-                                        double d = 0.0;
-#pragma unroll
-                                        for (int k = 0; k < 8; k++)
-                                            if (k < nalc) d = d + ghs[ra * nalc + k] * g[k];
-                                        double c = 0.0;
-                                        if (diagblk) c = d1[ra * na + bcol - a0];     // na <= 8 here
-                                        else if (common)
-                                            c = dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
-                                        v = c - d;
-                                    }
-                                    stage[lane * na + ra] = v;
-                                }
-                            }
-                            WSYNC();
-                            for (int t = lane; t < nb_here * na; t += WAVE) dst[roff + (int64_t)B0 * na + t] = stage[t];
-                            WSYNC();
-                        } else if (bcol < ncolumns) {
+                    if (fast) {
+                        double *dsa = dst + roff;
+#define STAGE_SEGMENT(NL)                                                                                      \
+    stage_segment<NL, T>(sa, a0, na, ncolumns, dsa, G, ghs, d1, stage, segof, segm, lane, perq, QE, qn, qe, qg, lw, \
+                         dw1, les, len, li, lv, neq, gpos)
+                        switch (nalc) {
+                        case 1: STAGE_SEGMENT(1); break;
+                        case 2: STAGE_SEGMENT(2); break;
+                        case 3: STAGE_SEGMENT(3); break;
+                        case 4: STAGE_SEGMENT(4); break;
+                        case 5: STAGE_SEGMENT(5); break;
+                        case 6: STAGE_SEGMENT(6); break;
+                        case 7: STAGE_SEGMENT(7); break;
+                        default: STAGE_SEGMENT(8); break;
+                        }
+#undef STAGE_SEGMENT
+                    } else {
+                        for (int B0 = 0; B0 < ncolumns; B0 += WAVE) {
+                            const int bcol = B0 + lane;
+                            if (bcol >= ncolumns) continue;
                             const int sb = segof[bcol];
                             const bool diagblk = sb == sa;
                             const bool common = !diagblk && (segm[sa] & segm[sb]) != 0ull;
