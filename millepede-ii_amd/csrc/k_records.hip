@@ -92,6 +92,84 @@ __device__ __forceinline__ double dc1(int a, int b, bool perq, int QE, const int
     return c;
 }
 
+// sqminv (src/mpnum.f90:97-213) for n = N <= 5 local parameters, every lane holding the whole
+// packed matrix and right-hand side in registers (no cross-lane traffic): pivots in the
+// reference's order (remaining candidates ascending, first largest diagonal above eps times its
+// original size), the same elimination arithmetic as gj_lanes.  Reads cl/bl, writes the
+// negated inverse and the solution back (lane 0) and returns the rank.
+template <int N>
+__device__ __forceinline__ int gj_regs(double *cl, double *bl, int lane)
+{
+    constexpr int NP = N * (N + 1) / 2;
+    const double eps = 16.0 * DBL_EPSILON;
+    double c[NP], b[N], dg[N];
+#pragma unroll
+    for (int t = 0; t < NP; t++) c[t] = cl[t];
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        b[j] = bl[j];
+        dg[j] = fabs(c[j * (j + 1) / 2 + j]);
+    }
+    unsigned rem = (1u << N) - 1u;   // candidates not yet pivoted
+    int nrank = 0;
+#pragma unroll
+    for (int step = 0; step < N; step++) {
+        int k = -1;
+        double vkk = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double vjj = c[j * (j + 1) / 2 + j];
+            if (((rem >> j) & 1u) && fabs(vjj) > fmax(fabs(vkk), eps * dg[j])) { vkk = vjj; k = j; }
+        }
+        if (k < 0) {
+            // no acceptable pivot left: the remaining rows and columns are set to zero
+#pragma unroll
+            for (int J = 0; J < N; J++)
+#pragma unroll
+                for (int K = 0; K <= J; K++)
+                    if (((rem >> J) & 1u) && ((rem >> K) & 1u)) c[J * (J + 1) / 2 + K] = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; j++)
+                if ((rem >> j) & 1u) b[j] = 0.0;
+            break;
+        }
+        rem &= ~(1u << k);
+        nrank++;
+        double co[N];   // column k of the current matrix
+        double ckk = 0.0, bkk = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < N; q++)
+                if (q == k) v = j >= q ? c[j * (j + 1) / 2 + q] : c[q * (q + 1) / 2 + j];
+            co[j] = v;
+            if (j == k) { ckk = v; bkk = b[j]; }
+        }
+        const double vinv = 1.0 / ckk;
+        const double bk = bkk * vinv;
+#pragma unroll
+        for (int J = 0; J < N; J++)
+#pragma unroll
+            for (int K = 0; K <= J; K++) {
+                double &e = c[J * (J + 1) / 2 + K];
+                if (J == k && K == k) e = -vinv;
+                else if (J == k) e = vinv * co[K];
+                else if (K == k) e = vinv * co[J];
+                else e = e - (vinv * co[K]) * co[J];
+            }
+#pragma unroll
+        for (int j = 0; j < N; j++) b[j] = (j == k) ? bk : b[j] - bk * co[j];
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int t = 0; t < NP; t++) cl[t] = -c[t];
+#pragma unroll
+        for (int j = 0; j < N; j++) bl[j] = b[j];
+    }
+    return nrank;
+}
+
 // Value of a lane-held double at a uniform lane
 __device__ __forceinline__ double rdlane(double v, int l)
 {
@@ -472,9 +550,10 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     lw[i] = wght;
                 }
                 WSYNC();
-                // local normal equations, element-parallel, equations in order (3781-3836); small
-                // systems keep the elements in registers for the inversion
-                const bool regfit = npk + nalc <= WAVE;
+                // local normal equations, element-parallel, equations in order (3781-3836); systems
+                // of up to 9 parameters keep the elements in registers for the inversion
+                const bool regs5 = nalc <= 5;
+                const bool regfit = !regs5 && npk + nalc <= WAVE;
                 double acc_r = 0.0;
                 for (int t = lane; t < npk + nalc; t += WAVE) {
                     int J, K;
@@ -489,20 +568,27 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     }
                     double acc = 0.0;
                     if (ldense) {
-                        // equations without both parameters add an exact zero
-                        if (K == 0) {
-#pragma unroll 4
-                            for (int i = 0; i < neq; i++) {
-                                const double term = lw[i] * rm[i] * LD[i * nalc + J - 1];
-                                acc = acc + (((lmask[i] >> (J - 1)) & 1ull) ? term : 0.0);
+                        // equations without both parameters add an exact zero; the loads of eight
+                        // equations are issued together
+                        const unsigned long long need = (1ull << (J - 1)) | (K > 0 ? 1ull << (K - 1) : 0ull);
+                        const int jo = J - 1, ko = (K > 0 ? K : J) - 1;
+                        for (int i0 = 0; i0 < neq; i0 += 8) {
+                            double tv[8];
+                            bool tm[8];
+#pragma unroll
+                            for (int u = 0; u < 8; u++) {
+                                const int i = i0 + u;
+                                tm[u] = false;
+                                tv[u] = 0.0;
+                                if (i < neq) {
+                                    const double dj = LD[i * nalc + jo], dk = LD[i * nalc + ko];
+                                    const double x = K == 0 ? rm[i] : dj, y = K == 0 ? dj : dk;
+                                    tm[u] = (lmask[i] & need) == need;
+                                    tv[u] = lw[i] * x * y;
+                                }
                             }
-                        } else {
-                            const unsigned long long need = (1ull << (J - 1)) | (1ull << (K - 1));
-#pragma unroll 4
-                            for (int i = 0; i < neq; i++) {
-                                const double term = lw[i] * LD[i * nalc + J - 1] * LD[i * nalc + K - 1];
-                                acc = acc + (((lmask[i] & need) == need) ? term : 0.0);
-                            }
+#pragma unroll
+                            for (int u = 0; u < 8; u++) acc = acc + (tm[u] ? tv[u] : 0.0);
                         }
                     } else {
                         for (int i = 0; i < neq; i++) {
@@ -526,7 +612,17 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     else if (K == 0) bl[J - 1] = acc;
                     else cl[t] = acc;
                 }
-                if (regfit) {
+                if (regs5) {
+                    WSYNC();
+                    switch (nalc) {
+                    case 1: nrank = gj_regs<1>(cl, bl, lane); break;
+                    case 2: nrank = gj_regs<2>(cl, bl, lane); break;
+                    case 3: nrank = gj_regs<3>(cl, bl, lane); break;
+                    case 4: nrank = gj_regs<4>(cl, bl, lane); break;
+                    default: nrank = gj_regs<5>(cl, bl, lane); break;
+                    }
+                    WSYNC();
+                } else if (regfit) {
                     nrank = gj_lanes(acc_r, npk, nalc, lane, cl, bl);
                     WSYNC();
                 } else {
@@ -1047,7 +1143,14 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
                             : hipFuncSetAttribute((const void *)k_records<double>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes);
         if (e != hipSuccess) return e;
     }
-    grid = std::max(1, std::min(grid, 256 * 8));   // workgroups loop over the records
+    // as many workgroups as fit on the device at once; each loops over the records
+    int per_cu = 0, dev = 0, ncu = 0;
+    hipError_t e = R.vf ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_records<float>, RT, P.bytes)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_records<double>, RT, P.bytes);
+    if (e != hipSuccess) return e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    grid = std::max(1, std::min<int>(grid, std::max(1, per_cu) * std::max(1, ncu)));
     if (R.vf)
         hipLaunchKernelGGL(k_records<float>, dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
     else
