@@ -20,6 +20,8 @@
 #include "mpgpu_device.h"
 #include <float.h>
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 namespace mpgpu {
 
@@ -1151,6 +1153,9 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
     grid = std::max(1, std::min<int>(grid, std::max(1, per_cu) * std::max(1, ncu)));
+    if (std::getenv("MPGPU_VERBOSE"))
+        std::fprintf(stderr, "k_records: %d threads x %d workgroups, LDS %d bytes, %d workgroups per CU\n", RT, grid,
+                     P.bytes, per_cu);
     if (R.vf)
         hipLaunchKernelGGL(k_records<float>, dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
     else
