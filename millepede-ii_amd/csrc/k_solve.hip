@@ -734,6 +734,102 @@ __global__ __launch_bounds__(256) void k_lltdec(int n, double *c, const int *ind
     if (tid == 0) *nrkd = nr;
 }
 
+// lltdec for rows of at most MW band values: the same recurrence and operation order, with the
+// current row in registers (statically indexed, right-aligned: r[o] = element (k, k-MW+1+o)),
+// the previous MW-1 finished rows in a right-aligned LDS ring, and every load of a row issued
+// before its arithmetic.
+template <int MW>
+__global__ __launch_bounds__(256) void k_lltdec_w(int n, double *c, const int *india, int *nrkd, int iopt)
+{
+    __shared__ double buf[LLT_TILE];
+    __shared__ int ind[LLT_TILE + 1];
+    __shared__ double ring[MW * MW];     // row j at slot j % MW, element (j, j-MW+1+o) at [slot*MW + o]
+    const int tid = threadIdx.x;
+    const double eps = 16.0 * DBL_EPSILON;
+    const int rows = LLT_TILE / MW;      // rows per tile (each row holds <= MW values)
+    int nr = 0;
+    for (int k0 = 1; k0 <= n; k0 += rows) {
+        const int k1 = min(n, k0 + rows - 1);
+        const int base = k0 > 1 ? india[k0 - 2] : 0;   // IND(k0-1)
+        const int nel = india[k1 - 1] - base;
+        for (int t = tid; t < nel; t += 256) buf[t] = c[base + t];
+        for (int t = tid; t <= k1 - k0; t += 256) ind[t + 1] = india[k0 - 1 + t];
+        if (tid == 0) ind[0] = base;
+        __syncthreads();
+        if (tid == 0) {
+            for (int k = k0; k <= k1; k++) {
+                const int indk = ind[k - k0 + 1], indk1 = ind[k - k0];
+                const int mk = k - indk + indk1 + 1;          // first column of row k
+                double *Pk = &buf[indk - k - base - 1];       // element (k, j) at Pk[j]
+                double r[MW], pr[MW][MW], dj[MW];
+                // every load is unconditional (in-bounds addresses) and selected afterwards, so that
+                // no branch separates the loads of a row
+#pragma unroll
+                for (int o = 0; o < MW; o++) {
+                    const int col = k - MW + 1 + o;
+                    const double v = Pk[max(col, mk)];
+                    r[o] = col >= mk ? v : 0.0;
+                }
+#pragma unroll
+                for (int jo = 0; jo < MW - 1; jo++) {
+                    const int j = k - MW + 1 + jo;
+                    const int slot = (j + MW) % MW;
+                    const bool valid = j >= mk;
+                    const double d = ring[slot * MW + MW - 1];
+                    dj[jo] = valid ? d : 0.0;
+#pragma unroll
+                    for (int io = 0; io < jo; io++) {
+                        const double v = ring[slot * MW + MW - 1 - (jo - io)];
+                        pr[jo][io] = valid ? v : 0.0;
+                    }
+                }
+                if (k == 1) {
+                    double v = r[MW - 1];
+                    if (v > 0.0) v = 1.0 / sqrt(v);
+                    else { v = 0.0; nr = -1; }
+                    r[MW - 1] = v;
+                } else {
+                    // terms outside a row's band are exact zeros (r is zero below column mk, the
+                    // ring rows below their own first column, rows before mk are read as zero),
+                    // so the products run over the full window without selects in the chain
+#pragma unroll
+                    for (int jo = 0; jo < MW - 1; jo++) {
+                        double ckj = r[jo];
+#pragma unroll
+                        for (int io = 0; io < jo; io++) ckj = ckj - r[io] * pr[jo][io];
+                        r[jo] = ckj * dj[jo];
+                    }
+                    const double diag = r[MW - 1];
+                    double ckk = r[MW - 1];
+#pragma unroll
+                    for (int io = 0; io < MW - 1; io++) ckk = ckk - r[io] * r[io];
+                    if (ckk > eps * diag) r[MW - 1] = 1.0 / sqrt(ckk);
+                    else {
+#pragma unroll
+                        for (int o = 0; o < MW; o++) r[o] = 0.0;
+                        if (iopt > 0 && diag > 0.0) r[MW - 1] = 1.0 / sqrt(diag);
+                        else {
+                            if (nr == 0) nr = -k;
+                            else { if (nr < 0) nr = 1; nr++; }
+                        }
+                    }
+                }
+                const int slot = k % MW;
+#pragma unroll
+                for (int o = 0; o < MW; o++) {
+                    const int col = k - MW + 1 + o;
+                    if (col >= mk) Pk[col] = r[o];   // the row's own values only
+                    ring[slot * MW + o] = r[o];
+                }
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < nel; t += 256) c[base + t] = buf[t];
+        __syncthreads();
+    }
+    if (tid == 0) *nrkd = nr;
+}
+
 __device__ void lltfwd_dev(int n, const double *c, const int *india, double *x)
 {
     const double *C = c - 1;
@@ -976,7 +1072,11 @@ void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, cons
 }
 
 void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, hipStream_t s)
-{ hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt, mbw); }
+{
+    if (mbw <= 4) hipLaunchKernelGGL(k_lltdec_w<4>, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt);
+    else if (mbw <= 8) hipLaunchKernelGGL(k_lltdec_w<8>, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt);
+    else hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt, mbw);
+}
 void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
               hipStream_t s)
 { if (ncon > 0) hipLaunchKernelGGL(k_band_k, dim3((ncon + 63) / 64), dim3(64), 0, s, ncon, n, c, india, cb, koff, kmat); }
@@ -1070,19 +1170,26 @@ __global__ void k_spk_local(int n, int w, int nch, const double *c, const int *i
 
 // carry: boundary values of every chunk, sequential over chunks (w x w products).  The data
 // of a tile of chunks (their boundary rows of x and of the response matrix S) is staged in LDS
-// by the whole workgroup; one thread then runs the chunk-to-chunk recurrence from LDS.
+// by the whole workgroup; the first wavefront then runs the chunk-to-chunk recurrence with one
+// boundary row per lane (lane i < w: its row of S and value of x in registers, the next chunk's
+// row read ahead) and the carried values broadcast with readlane.
 #define SPK_LDS_DOUBLES 6144
-__global__ __launch_bounds__(256) void k_spk_carry(int n, int w, int nch, const double *x, const double *S, double *t,
-                                                   int backward)
+__device__ __forceinline__ double spk_rdlane(double v, int l)
 {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+template <int W>
+__global__ __launch_bounds__(256) void k_spk_carry(int n, int nch, const double *x, const double *S, double *t, int backward)
+{
+    constexpr int w = W;
     extern __shared__ double dyn[];
     const int tc = SPK_LDS_DOUBLES / (w + w * w);      // chunks per tile
     double *sx = dyn;                                  // [tc * w]
     double *sS = sx + tc * w;                          // [tc * w * w]
-    double cur[SPK_WMAX];                              // carried values (thread 0)
-#pragma unroll
-    for (int i = 0; i < SPK_WMAX; i++) cur[i] = 0.0;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
+    double cur = 0.0;                                  // carried value of boundary row `lane` (wave 0)
     const int ntile = (nch + tc - 1) / tc;
     for (int tt = 0; tt < ntile; tt++) {
         // tiles in processing order (backward: from the last chunk)
@@ -1098,53 +1205,40 @@ __global__ __launch_bounds__(256) void k_spk_carry(int n, int w, int nch, const 
             for (int q = 0; q < w; q++) sS[e * w + q] = in ? S[(size_t)(k - 1) * w + q] : 0.0;
         }
         __syncthreads();
-        if (tid == 0) {
-            double nxt[SPK_WMAX];
-            for (int cc = 0; cc < c1 - c0; cc++) {
+        if (tid < 64) {
+            const int i = lane < w ? lane : 0;
+            double xa = 0.0, sa[W];
+            auto load = [&](int cc, double &xv, double *sv) {
                 const int ch = backward ? c1 - 1 - cc : c0 + cc;
                 const int e0 = (ch - c0) * w;
+                xv = sx[e0 + i];
+#pragma unroll
+                for (int q = 0; q < W; q++) sv[q] = sS[(e0 + i) * w + q];
+            };
+            load(0, xa, sa);
+            for (int cc = 0; cc < c1 - c0; cc++) {
+                const int ch = backward ? c1 - 1 - cc : c0 + cc;
                 const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1), L = k1 - k0 + 1;
-#pragma unroll
-                for (int i = 0; i < SPK_WMAX; i++)
-                    if (i < w) t[(size_t)ch * w + i] = cur[i];
+                double xb = 0.0, sb[W];
+                if (cc + 1 < c1 - c0) load(cc + 1, xb, sb);   // read ahead
+                if (lane < w) t[(size_t)ch * w + lane] = cur;
                 const bool coupled = backward ? (k1 < n) : (ch > 0);
-                if (L >= w) {
-                    // full chunk: every boundary value from its row of S (registers only)
+                double v = xa;
+                if (coupled) {
 #pragma unroll
-                    for (int i = 0; i < SPK_WMAX; i++) {
-                        if (i >= w) break;
-                        double v = sx[e0 + i];
-                        if (coupled) {
-#pragma unroll
-                            for (int q = 0; q < SPK_WMAX; q++)
-                                if (q < w) v += sS[(e0 + i) * w + q] * cur[q];
-                        }
-                        nxt[i] = v;
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < SPK_WMAX; i++) {
-                        if (i >= w) break;
-                        const bool in = backward ? (i < L) : (w - 1 - i < L);
-                        double v = 0.0;
-                        if (in) {
-                            v = sx[e0 + i];
-                            if (coupled) {
-#pragma unroll
-                                for (int q = 0; q < SPK_WMAX; q++)
-                                    if (q < w) v += sS[(e0 + i) * w + q] * cur[q];
-                            }
-                        } else {
-                            // chunk shorter than w: carry through
-#pragma unroll
-                            for (int q = 0; q < SPK_WMAX; q++)
-                                if (q == (backward ? i - L : i + L)) v = cur[q];
-                        }
-                        nxt[i] = v;
-                    }
+                    for (int q = 0; q < W; q++) v += sa[q] * spk_rdlane(cur, q);
                 }
+                if (L < w) {
+                    // chunk shorter than w (the last one): rows outside it carry through
+                    const bool in = backward ? (lane < L) : (w - 1 - lane < L);
+                    const int src = backward ? lane - L : lane + L;
+                    const double through = __shfl(cur, src >= 0 && src < 64 ? src : 0, 64);
+                    v = in ? v : (lane < w ? through : 0.0);
+                }
+                cur = lane < w ? v : 0.0;
+                xa = xb;
 #pragma unroll
-                for (int i = 0; i < SPK_WMAX; i++) cur[i] = i < w ? nxt[i] : 0.0;
+                for (int q = 0; q < W; q++) sa[q] = sb[q];
             }
         }
     }
@@ -1178,9 +1272,32 @@ __global__ void k_eqs_cdot(int n, const int *cb, const int64_t *koff, const doub
     }
     if (threadIdx.x == 0) d[j] = x[n + j] - red[0];
 }
-__global__ void k_eqs_small(int n, int m, const double *h, const int *hind, const double *d, double *x)
+// the constraint block's factor and right-hand side are staged in LDS when they fit (the
+// substitutions then run on LDS instead of global memory)
+#define EQS_LDS 4096
+__global__ __launch_bounds__(64) void k_eqs_small(int n, int m, const double *h, const int *hind, const double *d, double *x)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double sh_h[EQS_LDS];
+    __shared__ int sh_i[EQS_LDS / 2];
+    const int nh = m > 0 ? hind[m - 1] : 0;
+    const bool fit = nh + m <= EQS_LDS && m <= EQS_LDS / 2;
+    if (fit) {
+        for (int t = threadIdx.x; t < nh; t += 64) sh_h[t] = h[t];
+        for (int t = threadIdx.x; t < m; t += 64) {
+            sh_h[nh + t] = d[t];
+            sh_i[t] = hind[t];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double *xl = sh_h + nh;
+            lltfwd_dev(m, sh_h, sh_i, xl);
+            lltbwd_dev(m, sh_h, sh_i, xl);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < m; t += 64) x[n + t] = -sh_h[nh + t];
+        return;
+    }
+    if (threadIdx.x != 0) return;
     for (int j = 0; j < m; j++) x[n + j] = d[j];
     lltfwd_dev(m, h, hind, x + n);
     lltbwd_dev(m, h, hind, x + n);
@@ -1199,6 +1316,31 @@ namespace {
 inline int g1(int64_t n, int t = TPB) { return (int)((n + t - 1) / t); }
 }
 
+static void spk_carry(int n, int w, int nch, const double *x, const double *S, double *t, int backward, hipStream_t s)
+{
+    const size_t lds = SPK_LDS_DOUBLES * sizeof(double);
+#define SPK_CARRY(W) hipLaunchKernelGGL(k_spk_carry<W>, dim3(1), dim3(256), lds, s, n, nch, x, S, t, backward)
+    switch (w) {
+    case 1: SPK_CARRY(1); break;
+    case 2: SPK_CARRY(2); break;
+    case 3: SPK_CARRY(3); break;
+    case 4: SPK_CARRY(4); break;
+    case 5: SPK_CARRY(5); break;
+    case 6: SPK_CARRY(6); break;
+    case 7: SPK_CARRY(7); break;
+    case 8: SPK_CARRY(8); break;
+    case 9: SPK_CARRY(9); break;
+    case 10: SPK_CARRY(10); break;
+    case 11: SPK_CARRY(11); break;
+    case 12: SPK_CARRY(12); break;
+    case 13: SPK_CARRY(13); break;
+    case 14: SPK_CARRY(14); break;
+    case 15: SPK_CARRY(15); break;
+    default: SPK_CARRY(16); break;
+    }
+#undef SPK_CARRY
+}
+
 void l_spk_build(int n, int w, const double *c, const int *india, double *Sf, double *Sb, double *scr, hipStream_t s)
 {
     int nch = (n + SPK_CH - 1) / SPK_CH;
@@ -1211,7 +1353,7 @@ void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, con
 {
     int nch = (n + SPK_CH - 1) / SPK_CH;
     hipLaunchKernelGGL(k_spk_local, dim3(g1(nch, 64)), dim3(64), 0, s, n, w, nch, c, india, x, 0);
-    hipLaunchKernelGGL(k_spk_carry, dim3(1), dim3(256), SPK_LDS_DOUBLES * sizeof(double), s, n, w, nch, x, Sf, tbuf, 0);
+    spk_carry(n, w, nch, x, Sf, tbuf, 0, s);
     hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sf, tbuf, x, 0);
     if (m > 0) {
         hipLaunchKernelGGL(k_eqs_cdot, dim3(m), dim3(TPB), 0, s, n, cb, koff, kmat, x, dtmp);
@@ -1219,7 +1361,7 @@ void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, con
         hipLaunchKernelGGL(k_eqs_upd, dim3(8, m), dim3(TPB), 0, s, n, cb, koff, kmat, x);
     }
     hipLaunchKernelGGL(k_spk_local, dim3(g1(nch, 64)), dim3(64), 0, s, n, w, nch, c, india, x, 1);
-    hipLaunchKernelGGL(k_spk_carry, dim3(1), dim3(256), SPK_LDS_DOUBLES * sizeof(double), s, n, w, nch, x, Sb, tbuf, 1);
+    spk_carry(n, w, nch, x, Sb, tbuf, 1, s);
     hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sb, tbuf, x, 1);
 }
 
