@@ -350,7 +350,28 @@ __device__ __forceinline__ double sum_slots(double v)
     for (int o = 32; o >= 8; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-__global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_t *brow, const int32_t *bcol,
+// row R of block B (nr x nc, row-major; the diagonal block holds its lower triangle and is used
+// with both) times x[fb .. fb+nc): loads of up to 8 columns issued together
+__device__ __forceinline__ double bsymv_row(const double *B, int nc, int R, bool diag, const double *xb)
+{
+    double sv = 0.0;
+    if (nc <= 8) {
+        double v[8], xv[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int cc = c < nc ? c : nc - 1;
+            v[c] = (diag && cc > R) ? B[(int64_t)cc * nc + R] : B[(int64_t)R * nc + cc];
+            xv[c] = xb[cc];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            if (c < nc) sv += v[c] * xv[c];
+        return sv;
+    }
+    for (int c = 0; c < nc; c++) sv += ((diag && c > R) ? B[(int64_t)c * nc + R] : B[(int64_t)R * nc + c]) * xb[c];
+    return sv;
+}
+__global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_t *brow, const int32_t *bcf,
                                                     const int64_t *boff, const int32_t *gfirst, const double *a,
                                                     const double *x, double *y)
 {
@@ -360,28 +381,48 @@ __global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_
     const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
     const int64_t b0 = brow[ga], b1 = brow[ga + 1];
     for (int r0 = 0; r0 < nr; r0 += 8) {
-        const int R = r0 + rl;
-        double acc = 0.0;
-        for (int64_t bk = b0 + slot; bk < b1; bk += 8) {
-            if (R >= nr) continue;
-            const int gb = bcol[bk];
-            const int fb = gfirst[gb] - 1, nc = gfirst[gb + 1] - 1 - fb;
-            const double *B = a + boff[bk];
-            double sv = 0.0;
-            if (gb != ga) {
-                for (int c = 0; c < nc; c++) sv += B[(int64_t)R * nc + c] * x[fb + c];
-            } else {
-                for (int c = 0; c < nc; c++) sv += (c <= R ? B[(int64_t)R * nc + c] : B[(int64_t)c * nc + R]) * x[fb + c];
-            }
-            acc += sv;
+        const int R = min(r0 + rl, nr - 1);   // lanes past the last row repeat it (result unused)
+        double acc0 = 0.0, acc1 = 0.0;
+        int64_t bk = b0 + slot;
+        // two blocks per slot and step: their index and value loads are issued together
+        for (; bk + 8 < b1; bk += 16) {
+            const int fb0 = bcf[2 * bk], nc0 = bcf[2 * bk + 1];
+            const int fb1 = bcf[2 * bk + 16], nc1 = bcf[2 * bk + 17];
+            const double *B0 = a + boff[bk], *B1 = a + boff[bk + 8];
+            acc0 += bsymv_row(B0, nc0, R, fb0 == fa, x + fb0);
+            acc1 += bsymv_row(B1, nc1, R, fb1 == fa, x + fb1);
         }
-        acc = sum_slots(acc);
-        if (slot == 0 && R < nr) y[fa + R] = acc;
+        if (bk < b1) {
+            const int fb0 = bcf[2 * bk], nc0 = bcf[2 * bk + 1];
+            acc0 += bsymv_row(a + boff[bk], nc0, R, fb0 == fa, x + fb0);
+        }
+        const double acc = sum_slots(acc0 + acc1);
+        if (slot == 0 && r0 + rl < nr) y[fa + r0 + rl] = acc;
     }
 }
-__global__ __launch_bounds__(256) void k_bsymv_cols(int32_t ngroup, const int64_t *ccol_ptr, const int64_t *ccol_blk,
-                                                    const int32_t *brg, const int64_t *boff, const int32_t *gfirst,
-                                                    const double *a, const double *x, double *y)
+// column Cc of block B (nr x nc, row-major) times x[fa .. fa+nr)
+__device__ __forceinline__ double bsymv_col(const double *B, int nr, int nc, int Cc, const double *xa)
+{
+    double sv = 0.0;
+    if (nr <= 8) {
+        double v[8], xv[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int rr = r < nr ? r : nr - 1;
+            v[r] = B[(int64_t)rr * nc + Cc];
+            xv[r] = xa[rr];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+            if (r < nr) sv += v[r] * xv[r];
+        return sv;
+    }
+    for (int r = 0; r < nr; r++) sv += B[(int64_t)r * nc + Cc] * xa[r];
+    return sv;
+}
+__global__ __launch_bounds__(256) void k_bsymv_cols(int32_t ngroup, const int64_t *ccol_ptr, const int64_t *cc_off,
+                                                    const int32_t *cc_fn, const int32_t *gfirst, const double *a,
+                                                    const double *x, double *y)
 {
     const int gb = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63, slot = lane >> 3, cl = lane & 7;
@@ -390,20 +431,19 @@ __global__ __launch_bounds__(256) void k_bsymv_cols(int32_t ngroup, const int64_
     const int64_t q0 = ccol_ptr[gb], q1 = ccol_ptr[gb + 1];
     if (q1 <= q0) return;
     for (int c0 = 0; c0 < nc; c0 += 8) {
-        const int Cc = c0 + cl;
-        double acc = 0.0;
-        for (int64_t q = q0 + slot; q < q1; q += 8) {
-            if (Cc >= nc) continue;
-            const int64_t bk = ccol_blk[q];
-            const int ga = brg[bk];
-            const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
-            const double *B = a + boff[bk];
-            double sv = 0.0;
-            for (int r = 0; r < nr; r++) sv += B[(int64_t)r * nc + Cc] * x[fa + r];
-            acc += sv;
+        const int Cc = min(c0 + cl, nc - 1);   // lanes past the last column repeat it (result unused)
+        double acc0 = 0.0, acc1 = 0.0;
+        int64_t q = q0 + slot;
+        for (; q + 8 < q1; q += 16) {
+            const int fa0 = cc_fn[2 * q], nr0 = cc_fn[2 * q + 1];
+            const int fa1 = cc_fn[2 * q + 16], nr1 = cc_fn[2 * q + 17];
+            const double *B0 = a + cc_off[q], *B1 = a + cc_off[q + 8];
+            acc0 += bsymv_col(B0, nr0, nc, Cc, x + fa0);
+            acc1 += bsymv_col(B1, nr1, nc, Cc, x + fa1);
         }
-        acc = sum_slots(acc);
-        if (slot == 0 && Cc < nc) y[fb + Cc] += acc;
+        if (q < q1) acc0 += bsymv_col(a + cc_off[q], cc_fn[2 * q + 1], nc, Cc, x + cc_fn[2 * q]);
+        const double acc = sum_slots(acc0 + acc1);
+        if (slot == 0 && c0 + cl < nc) y[fb + c0 + cl] += acc;
     }
 }
 
@@ -1021,16 +1061,16 @@ __global__ void k_zero(int64_t n, double *y)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = 0.0;
 }
-void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const int32_t *gfirst,
-                  const int32_t *brg, const int64_t *ccol_ptr, const int64_t *ccol_blk, const double *a,
+void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
+                  const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
                   const double *x, double *y, int32_t nvar, int64_t nall, hipStream_t s)
 {
     // rows beyond the parameter groups (Lagrange multipliers) start at zero
     if (nall > nvar) hipLaunchKernelGGL(k_zero, dim3(nblk(nall - nvar)), dim3(TPB), 0, s, nall - nvar, y + nvar);
-    hipLaunchKernelGGL(k_bsymv_rows, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcol, boff, gfirst,
+    hipLaunchKernelGGL(k_bsymv_rows, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff, gfirst,
                        a, x, y);
-    hipLaunchKernelGGL(k_bsymv_cols, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, ccol_blk, brg,
-                       boff, gfirst, a, x, y);
+    hipLaunchKernelGGL(k_bsymv_cols, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_off, cc_fn,
+                       gfirst, a, x, y);
 }
 void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
                  double *y, hipStream_t s)

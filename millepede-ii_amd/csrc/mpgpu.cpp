@@ -57,7 +57,8 @@ struct mpgpu_ctx {
     DBuf<int32_t> d_widx, d_gl_tix, d_pair_blk;
     DBuf<double> d_par, d_mat, d_vec, d_pre_weight;
     DBuf<int32_t> d_var, d_group_of, d_gfirst, d_bcol, d_brg;
-    DBuf<int64_t> d_brow, d_boff, d_ccol_ptr, d_ccol_blk;
+    DBuf<int64_t> d_brow, d_boff, d_ccol_ptr, d_cc_off;
+    DBuf<int32_t> d_bcf, d_cc_fn;   // per block / per column-list entry: first index and size of the other group
     DBuf<unsigned long long> d_cnt;
     DBuf<double> d_chi2;
     DBuf<int32_t> d_ndf;
@@ -1026,7 +1027,26 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->d_brg.upload(c->brg, s);
         c->d_boff.upload(c->boff, s);
         c->d_ccol_ptr.upload(c->ccol_ptr, s);
-        c->d_ccol_blk.upload(c->ccol_blk, s);
+        {
+            // the product kernels read a block's column group (rows pass) or row group (columns
+            // pass) as (first index, size) next to its offset
+            std::vector<int32_t> bcf(2 * (size_t)c->nblock), ccfn(2 * c->ccol_blk.size());
+            std::vector<int64_t> ccoff(c->ccol_blk.size());
+            for (int64_t b = 0; b < c->nblock; b++) {
+                bcf[2 * b] = c->gfirst[c->bcol[b]] - 1;
+                bcf[2 * b + 1] = c->gfirst[c->bcol[b] + 1] - c->gfirst[c->bcol[b]];
+            }
+            for (size_t q = 0; q < c->ccol_blk.size(); q++) {
+                const int64_t b = c->ccol_blk[q];
+                ccoff[q] = c->boff[b];
+                ccfn[2 * q] = c->gfirst[c->brg[b]] - 1;
+                ccfn[2 * q + 1] = c->gfirst[c->brg[b] + 1] - c->gfirst[c->brg[b]];
+            }
+            c->d_bcf.upload(bcf, s);
+            c->d_cc_off.upload(ccoff, s);
+            c->d_cc_fn.upload(ccfn, s);
+            HIPCHK(hipStreamSynchronize(s));
+        }
     }
     c->d_group_of.upload(c->group_of, s);
     c->d_gfirst.upload(c->gfirst, s);
@@ -1401,8 +1421,8 @@ void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
 {
     hipStream_t s = c->stream;
     if (c->opt.matsto == MPGPU_SPARSE) {
-        l_block_symv(c->ngroup, c->d_brow.p, c->d_bcol.p, c->d_boff.p, c->d_gfirst.p, c->d_brg.p, c->d_ccol_ptr.p,
-                     c->d_ccol_blk.p, c->d_mat.p, x, y, c->nvar, c->nall, s);
+        l_block_symv(c->ngroup, c->d_brow.p, c->d_bcf.p, c->d_boff.p, c->d_gfirst.p, c->d_ccol_ptr.p, c->d_cc_off.p,
+                     c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, s);
         l_cons_symv(c->ncon, c->nvar, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, x, y, s);
     } else {
         l_packed_symv(c->nall, c->d_mat.p, x, y, s);

@@ -106,8 +106,8 @@ void l_gj_init(int n, const double *v, double *diag, int *used, int *info, hipSt
 void l_gj_step(int n, double *v, const double *diag, int *used, int *info, double *vk, double *b, hipStream_t s);
 void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, hipStream_t s);
 void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream_t s);
-void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const int32_t *gfirst,
-                  const int32_t *brg, const int64_t *ccol_ptr, const int64_t *ccol_blk, const double *a,
+void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
+                  const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
                   const double *x, double *y, int32_t nvar, int64_t nall, hipStream_t s);
 void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
                  double *y, hipStream_t s);
