@@ -479,6 +479,118 @@ __global__ void k_scale_copy(int64_t n, double s, const double *a, double *out)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = s * a[i];
 }
+// ---- MINRES with the scalar recurrence on the device (see MrState) ----
+__global__ void k_mr_scale(int64_t n, const MrState *st, const double *y, double *v)
+{
+    if (st->istop != 0) return;
+    const double sc = 1.0 / st->beta;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) v[i] = sc * y[i];
+}
+// y -= (beta/oldb) r (which = 0) or y -= (alfa/beta) r (which = 1)
+__global__ void k_mr_axpy(int64_t n, const MrState *st, int which, const double *r, double *y)
+{
+    if (st->istop != 0) return;
+    const double c = which == 0 ? st->beta / st->oldb : st->alfa / st->beta;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = y[i] - c * r[i];
+}
+// one step of the scalar recurrence (src/minresModule.f90:468-600), the host restatement's
+// operations in the same order, without contraction into fused multiply-adds
+__global__ void k_mr_step(MrState *st)
+{
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (st->istop != 0) { st->upd = 0; return; }
+    const double eps = 2.220446049250313e-16;
+    const int itn = ++st->itn;
+    const double alfa = st->alfa;
+    const double oldb = st->beta;
+    double beta = st->bnew;
+    st->oldb = oldb;
+    if (beta < 0.0) { st->istop = 6; st->upd = 0; return; }
+    beta = sqrt(beta);
+    st->beta = beta;
+    double tnorm2 = st->tnorm2 + alfa * alfa + oldb * oldb + beta * beta;
+    int istop = st->istop;
+    double gmax = st->gmax, gmin = st->gmin;
+    if (itn == 1) {
+        if (beta / st->beta1 <= 10.0 * eps) istop = -1;
+        gmax = fabs(alfa);
+        gmin = gmax;
+    }
+    const double cs0 = st->cs, sn0 = st->sn, dbar0 = st->dbar;
+    const double oldeps = st->epsln;
+    const double delta = cs0 * dbar0 + sn0 * alfa;
+    const double gbar = sn0 * dbar0 - cs0 * alfa;
+    const double epsln = sn0 * beta;
+    const double dbar = -cs0 * beta;
+    const double gamma = sqrt(gbar * gbar + beta * beta);
+    const double cs = gbar / gamma;
+    const double sn = beta / gamma;
+    const double phi = cs * st->phibar;
+    const double phibar = sn * st->phibar;
+    const double denom = 1.0 / gamma;
+    st->oldeps = oldeps;
+    st->delta = delta;
+    st->denom = denom;
+    st->phi = phi;
+    st->upd = 1;
+    gmax = fmax(gmax, gamma);
+    gmin = fmin(gmin, gamma);
+    const double z = st->rhs1 / gamma;
+    const double ynorm2 = z * z + st->ynorm2;
+    st->rhs1 = st->rhs2 - delta * z;
+    st->rhs2 = -epsln * z;
+    const double Anorm = sqrt(tnorm2);
+    const double ynorm = sqrt(ynorm2);
+    const double epsx = Anorm * ynorm * eps;
+    const double epsr = Anorm * ynorm * st->rtol;
+    const double qrnorm = phibar;
+    const double rnorml = st->rnorm;
+    const double rnorm = qrnorm;
+    const double rootl = sqrt(gbar * gbar + dbar * dbar);
+    const double Arnorml = rnorml * rootl;
+    const double relArnorml = rootl / Anorm;
+    const double Acond = gmax / gmin;
+    if (istop == 0) {
+        if (itn >= st->itnlim) istop = 5;
+        if (Acond >= 0.1 / eps) istop = 4;
+        if (epsx >= st->beta1) istop = 3;
+        if (qrnorm <= epsx || relArnorml <= eps) istop = 2;
+        if (qrnorm <= epsr || relArnorml <= st->rtol) istop = 1;
+    }
+    st->tnorm2 = tnorm2;
+    st->gmax = gmax;
+    st->gmin = gmin;
+    st->epsln = epsln;
+    st->dbar = dbar;
+    st->cs = cs;
+    st->sn = sn;
+    st->phibar = phibar;
+    st->ynorm2 = ynorm2;
+    st->Anorm = Anorm;
+    st->ynorm = ynorm;
+    st->qrnorm = qrnorm;
+    st->rnorm = rnorm;
+    st->Arnorml = Arnorml;
+    st->Acond = Acond;
+    st->istop = istop;
+}
+// w1 = w2; w2 = w; w = (v - oldeps*w1 - delta*w2)*denom; x = x + phi*w  (iterations that ran)
+__global__ void k_mr_update(int64_t n, const MrState *st, const double *v, double *w, double *w1, double *w2, double *x)
+{
+    if (st->upd == 0) return;
+    const double oldeps = st->oldeps, delta = st->delta, denom = st->denom, phi = st->phi;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double a1 = w2[i];
+        double a2 = w[i];
+        w1[i] = a1;
+        w2[i] = a2;
+        double wn = (v[i] - oldeps * a1 - delta * a2) * denom;
+        w[i] = wn;
+        x[i] = x[i] + phi * wn;
+    }
+}
+
 // y = y - c*r
 __global__ void k_axpy_neg(int64_t n, double c, const double *r, double *y)
 {
@@ -1078,6 +1190,13 @@ void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t 
 
 void l_scale_copy(int64_t n, double sc, const double *a, double *o, hipStream_t s)
 { hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n)), dim3(TPB), 0, s, n, sc, a, o); }
+void l_mr_scale(int64_t n, const MrState *st, const double *y, double *v, hipStream_t s)
+{ hipLaunchKernelGGL(k_mr_scale, dim3(nblk(n)), dim3(TPB), 0, s, n, st, y, v); }
+void l_mr_axpy(int64_t n, const MrState *st, int which, const double *r, double *y, hipStream_t s)
+{ hipLaunchKernelGGL(k_mr_axpy, dim3(nblk(n)), dim3(TPB), 0, s, n, st, which, r, y); }
+void l_mr_step(MrState *st, hipStream_t s) { hipLaunchKernelGGL(k_mr_step, dim3(1), dim3(64), 0, s, st); }
+void l_mr_update(int64_t n, const MrState *st, const double *v, double *w, double *w1, double *w2, double *x, hipStream_t s)
+{ hipLaunchKernelGGL(k_mr_update, dim3(nblk(n)), dim3(TPB), 0, s, n, st, v, w, w1, w2, x); }
 void l_axpy_neg(int64_t n, double c, const double *r, double *y, hipStream_t s)
 { hipLaunchKernelGGL(k_axpy_neg, dim3(nblk(n)), dim3(TPB), 0, s, n, c, r, y); }
 void l_minres_update(int64_t n, const double *v, double *w, double *w1, double *w2, double *x, double oldeps, double delta,

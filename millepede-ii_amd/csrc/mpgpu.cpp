@@ -1546,7 +1546,7 @@ void apply_precond(mpgpu_ctx *c, Precond &P, const double *x, double *y)
     }
 }
 
-// MINRES, restating src/minresModule.f90:294-660 with device vectors and host scalars
+// MINRES, restating src/minresModule.f90:294-660 with device vectors and the scalar recurrence on the device
 void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats *st)
 {
     const int64_t n = c->nall;
@@ -1556,87 +1556,72 @@ void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
     const bool precon = P.kind != 0;
     DBuf<double> r1, r2, v, w, w1, w2, y;
     r1.alloc(n); r2.alloc(n); v.alloc(n); w.alloc(n); w1.alloc(n); w2.alloc(n); y.alloc(n);
-    const double eps = 2.220446049250313e-16;
     const double rtol = c->opt.mrestl;
     const int itnlim = c->opt.itnlim;
     int istop = 0, itn = 0;
-    double Anorm = 0, Acond = 0, rnorm = 0, ynorm = 0, Arnorml = 0, gmin = 0, gmax = 0;
+    double Anorm = 0, Acond = 0, rnorm = 0, ynorm = 0, Arnorml = 0;
     HIPCHK(hipMemsetAsync(x_dev, 0, sizeof(double) * n, s));
     l_copy(n, b_dev, y.p, s);
     l_copy(n, b_dev, r1.p, s);
     if (precon) apply_precond(c, P, b_dev, y.p);
     double beta1 = dot_dev(c, n, b_dev, y.p);
-    double alfa, beta, cs, dbar, delta, denom, epsa, epsln, epsr, epsx, gamma, gbar, oldb, oldeps, qrnorm, phi, phibar, rhs1,
-        rhs2, rnorml, rootl, relArnorml, z, tnorm2, ynorm2, sn;
     if (beta1 < 0.0) { istop = 8; goto done; }
     if (beta1 == 0.0) { istop = 0; goto done; }
     beta1 = std::sqrt(beta1);
     avprod_dev(c, y.p, w.p);
     Arnorml = std::sqrt(dot_dev(c, n, w.p, w.p));
-    oldb = 0; beta = beta1; dbar = 0; epsln = 0; qrnorm = beta1; phibar = beta1; rhs1 = beta1; rhs2 = 0;
-    tnorm2 = 0; ynorm2 = 0; cs = -1; sn = 0;
-    w.zero(s);
-    w2.zero(s);
-    l_copy(n, r1.p, r2.p, s);
-    for (;;) {
-        itn++;
-        l_scale_copy(n, 1.0 / beta, y.p, v.p, s);
-        avprod_dev(c, v.p, y.p);
-        if (itn >= 2) l_axpy_neg(n, beta / oldb, r1.p, y.p, s);
-        alfa = dot_dev(c, n, v.p, y.p);
-        l_axpy_neg(n, alfa / beta, r2.p, y.p, s);
-        std::swap(r1.p, r2.p);          // r1 = r2 (old r2 buffer becomes r1)
-        l_copy(n, y.p, r2.p, s);        // r2 = y
-        if (precon) apply_precond(c, P, r2.p, y.p);
-        oldb = beta;
-        beta = dot_dev(c, n, r2.p, y.p);
-        if (beta < 0.0) { istop = 6; break; }
-        beta = std::sqrt(beta);
-        tnorm2 = tnorm2 + alfa * alfa + oldb * oldb + beta * beta;
-        if (itn == 1) {
-            if (beta / beta1 <= 10.0 * eps) istop = -1;
-            gmax = std::fabs(alfa);
-            gmin = gmax;
+    {
+        // the iterations run with the scalar recurrence on the device (k_mr_step): no host round
+        // trip per iteration; the host reads the stop code every few iterations
+        MrState h{};
+        h.beta1 = beta1;
+        h.beta = beta1;
+        h.oldb = 0.0;
+        h.dbar = 0.0;
+        h.epsln = 0.0;
+        h.qrnorm = beta1;
+        h.phibar = beta1;
+        h.rhs1 = beta1;
+        h.rhs2 = 0.0;
+        h.tnorm2 = 0.0;
+        h.ynorm2 = 0.0;
+        h.cs = -1.0;
+        h.sn = 0.0;
+        h.Arnorml = Arnorml;
+        h.rtol = rtol;
+        h.itnlim = itnlim;
+        DBuf<MrState> dst;
+        dst.upload(&h, 1, s);
+        MrState *d = dst.p;
+        w.zero(s);
+        w2.zero(s);
+        l_copy(n, r1.p, r2.p, s);
+        const int check = 8;
+        for (int k = 1;; k++) {
+            l_mr_scale(n, d, y.p, v.p, s);
+            avprod_dev(c, v.p, y.p);
+            if (k >= 2) l_mr_axpy(n, d, 0, r1.p, y.p, s);
+            l_dot(n, v.p, y.p, c->d_part.p, &d->alfa, s);
+            l_mr_axpy(n, d, 1, r2.p, y.p, s);
+            std::swap(r1.p, r2.p);          // r1 = r2 (old r2 buffer becomes r1)
+            l_copy(n, y.p, r2.p, s);        // r2 = y
+            if (precon) apply_precond(c, P, r2.p, y.p);
+            l_dot(n, r2.p, y.p, c->d_part.p, &d->bnew, s);
+            l_mr_step(d, s);
+            l_mr_update(n, d, v.p, w.p, w1.p, w2.p, x_dev, s);
+            if (k % check == 0 || k >= itnlim) {
+                dst.download(&h, 1, s);
+                HIPCHK(hipStreamSynchronize(s));
+                if (h.istop != 0) break;
+            }
         }
-        oldeps = epsln;
-        delta = cs * dbar + sn * alfa;
-        gbar = sn * dbar - cs * alfa;
-        epsln = sn * beta;
-        dbar = -cs * beta;
-        gamma = std::sqrt(gbar * gbar + beta * beta);
-        cs = gbar / gamma;
-        sn = beta / gamma;
-        phi = cs * phibar;
-        phibar = sn * phibar;
-        denom = 1.0 / gamma;
-        l_minres_update(n, v.p, w.p, w1.p, w2.p, x_dev, oldeps, delta, denom, phi, s);
-        gmax = std::max(gmax, gamma);
-        gmin = std::min(gmin, gamma);
-        z = rhs1 / gamma;
-        ynorm2 = z * z + ynorm2;
-        rhs1 = rhs2 - delta * z;
-        rhs2 = -epsln * z;
-        Anorm = std::sqrt(tnorm2);
-        ynorm = std::sqrt(ynorm2);
-        epsa = Anorm * eps;
-        epsx = Anorm * ynorm * eps;
-        epsr = Anorm * ynorm * rtol;
-        (void)epsa;
-        qrnorm = phibar;
-        rnorml = rnorm;
-        rnorm = qrnorm;
-        rootl = std::sqrt(gbar * gbar + dbar * dbar);
-        Arnorml = rnorml * rootl;
-        relArnorml = rootl / Anorm;
-        Acond = gmax / gmin;
-        if (istop == 0) {
-            if (itn >= itnlim) istop = 5;
-            if (Acond >= 0.1 / eps) istop = 4;
-            if (epsx >= beta1) istop = 3;
-            if (qrnorm <= epsx || relArnorml <= eps) istop = 2;
-            if (qrnorm <= epsr || relArnorml <= rtol) istop = 1;
-        }
-        if (istop != 0) break;
+        istop = h.istop;
+        itn = h.itn;
+        Anorm = h.Anorm;
+        Acond = h.Acond;
+        rnorm = h.rnorm;
+        ynorm = h.ynorm;
+        Arnorml = h.Arnorml;
     }
 done:
     HIPCHK(hipStreamSynchronize(s));

@@ -80,6 +80,15 @@ struct BandPlan {
     int32_t bytes;
 };
 
+// MINRES recurrence state in device memory (src/minresModule.f90:294-660): the dot products of an
+// iteration land here, one thread advances the scalar recurrence and the vector kernels read
+// their coefficients from it; once istop is set the remaining launched iterations do nothing
+struct MrState {
+    double beta1, beta, oldb, alfa, bnew, dbar, epsln, oldeps, qrnorm, phibar, rhs1, rhs2, tnorm2, ynorm2, cs, sn;
+    double gmin, gmax, Anorm, Acond, rnorm, ynorm, Arnorml, delta, denom, phi, rtol;
+    int itn, itnlim, istop, upd;
+};
+
 // scalars of one MINRES-QLP vector step (k_qlp_step)
 struct QlpStep {
     int first, itn;

@@ -106,6 +106,10 @@ void l_gj_init(int n, const double *v, double *diag, int *used, int *info, hipSt
 void l_gj_step(int n, double *v, const double *diag, int *used, int *info, double *vk, double *b, hipStream_t s);
 void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, hipStream_t s);
 void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream_t s);
+void l_mr_scale(int64_t n, const MrState *st, const double *y, double *v, hipStream_t s);
+void l_mr_axpy(int64_t n, const MrState *st, int which, const double *r, double *y, hipStream_t s);
+void l_mr_step(MrState *st, hipStream_t s);
+void l_mr_update(int64_t n, const MrState *st, const double *v, double *w, double *w1, double *w2, double *x, hipStream_t s);
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
                   const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
                   const double *x, double *y, int32_t nvar, int64_t nall, hipStream_t s);
