@@ -1162,9 +1162,11 @@ void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, 
     hipLaunchKernelGGL(k_gj_finish, dim3(g), dim3(TPB), 0, s, n, v, b, used, info);
 }
 
+__global__ void k_zero(int64_t n, double *y);
 void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream_t s)
 {
-    hipMemsetAsync(y, 0, sizeof(double) * n, s);
+    // cleared by a kernel (not a memset node), so that the product can be captured in a graph
+    hipLaunchKernelGGL(k_zero, dim3(nblk(n)), dim3(TPB), 0, s, (int64_t)n, y);
     int64_t nt = (n + ST - 1) / ST;
     int64_t tiles = nt * (nt + 1) / 2;
     hipLaunchKernelGGL(k_packed_symv, dim3((unsigned)tiles), dim3(256), 0, s, n, a, x, y);
@@ -1318,13 +1320,60 @@ __global__ void k_spk_build(int n, int w, int nch, const double *c, const int *i
     for (int i = 0; i < L; i++) Sb[((size_t)(k0 - 1 + i)) * w + q] = k1 < n ? x[k0 - 1 + i] : 0.0;
 }
 
-__global__ void k_spk_local(int n, int w, int nch, const double *c, const int *india, double *x, int backward)
+// chunk-local solves with a zero boundary, one thread per chunk.  The last W values of x stay in
+// registers (a sliding window); only the band values and the right-hand side are read per row.
+// Same operations in the same order as spk_fwd_chunk / spk_bwd_chunk.
+template <int W>
+__global__ void k_spk_local(int n, int nch, const double *c, const int *india, double *x, int backward)
 {
-    int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
     if (ch >= nch) return;
-    int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
-    if (!backward) spk_fwd_chunk(k0, k1, w, c, india, x, nullptr);
-    else spk_bwd_chunk(k0, k1, w, n, c, india, x, nullptr);
+    const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
+    double xw[W + 1];
+#pragma unroll
+    for (int q = 0; q <= W; q++) xw[q] = 0.0;
+    if (!backward) {
+        // xw[q] = x_{k-W-1+q} for q < W (zero before the chunk); x_k = (b_k - sum_j C_kj x_j) * C_kk
+        for (int k = k0; k <= k1; k++) {
+            const int ik = india[k - 1], len = ik - (k > 1 ? india[k - 2] : 0);
+            const int first = max(k - len + 1, k0);
+            double xk = x[k - 1];
+#pragma unroll
+            for (int q = 0; q < W; q++) {
+                const int j = k - W + q;
+                const double cj = c[ik - k + max(j, first) - 1];
+                if (j >= first) xk = xk - cj * xw[q + 1];
+            }
+            xk = xk * c[ik - 1];
+            x[k - 1] = xk;
+#pragma unroll
+            for (int q = 0; q < W; q++) xw[q] = xw[q + 1];
+            xw[W] = xk;
+        }
+    } else {
+        // xw[q] holds the pending x_{k-W+q} (q < W) and xw[W] the pending x_k
+        for (int q = 0; q <= W; q++) {
+            const int j = k1 - W + q;
+            xw[q] = j >= k0 ? x[j - 1] : 0.0;
+        }
+        for (int k = k1; k >= k0; k--) {
+            const int ik = india[k - 1], len = ik - (k > 1 ? india[k - 2] : 0);
+            const double xk = xw[W] * c[ik - 1];
+            x[k - 1] = xk;
+            if (k == 1) break;
+            const int first = max(k - len + 1, k0);
+#pragma unroll
+            for (int q = 0; q < W; q++) {
+                const int j = k - W + q;
+                const double cj = c[ik - k + max(j, first) - 1];
+                if (j >= first) xw[q] = xw[q] - cj * xk;
+            }
+#pragma unroll
+            for (int q = W; q > 0; q--) xw[q] = xw[q - 1];
+            const int jn = k - 1 - W;
+            xw[0] = jn >= k0 ? x[jn - 1] : 0.0;
+        }
+    }
 }
 
 // carry: boundary values of every chunk, sequential over chunks (w x w products).  The data
@@ -1447,6 +1496,30 @@ __global__ __launch_bounds__(64) void k_eqs_small(int n, int m, const double *h,
             sh_i[t] = hind[t];
         }
         __syncthreads();
+        if (m <= 64) {
+            // one row per lane, both substitutions column by column: step j finalises x_j and
+            // subtracts its column from the rows below (forward), or x_k's column from the rows
+            // above (backward); every x_i receives its terms in the order of lltfwd / lltbwd
+            const int lane = threadIdx.x, k = lane + 1;
+            double xv = lane < m ? sh_h[nh + lane] : 0.0;
+            const int indk = lane < m ? sh_i[lane] : 0, indk1 = (lane >= 1 && lane < m) ? sh_i[lane - 1] : 0;
+            const int fk = k - indk + indk1 + 1;       // first column of row k
+            for (int j = 1; j <= m; j++) {
+                if (lane == j - 1) xv = xv * sh_h[indk - 1];
+                const double xj = spk_rdlane(xv, j - 1);
+                if (lane < m && k > j && j >= fk) xv = xv - sh_h[indk - k + j - 1] * xj;
+            }
+            for (int kk = m; kk >= 2; kk--) {
+                const int ik = sh_i[kk - 1], ik1 = sh_i[kk - 2];
+                if (lane == kk - 1) xv = xv * sh_h[ik - 1];
+                const double xk = spk_rdlane(xv, kk - 1);
+                const int fkk = kk - ik + ik1 + 1;
+                if (k < kk && k >= fkk) xv = xv - sh_h[ik - kk + k - 1] * xk;
+            }
+            if (lane == 0) xv = xv * sh_h[indk - 1];
+            if (lane < m) x[n + lane] = -xv;
+            return;
+        }
         if (threadIdx.x == 0) {
             double *xl = sh_h + nh;
             lltfwd_dev(m, sh_h, sh_i, xl);
@@ -1462,17 +1535,46 @@ __global__ __launch_bounds__(64) void k_eqs_small(int n, int m, const double *h,
     lltbwd_dev(m, h, hind, x + n);
     for (int i = 0; i < m; i++) x[n + i] = -x[n + i];
 }
-__global__ void k_eqs_upd(int n, const int *cb, const int64_t *koff, const double *kmat, double *x)
+// x_i -= K_ji lambda_j over the constraints covering i in ascending j (the order of equslvs'
+// sequential loop), one thread per parameter: deterministic, no atomics
+__global__ void k_eqs_upd(int n, int m, const int *cb, const int64_t *koff, const double *kmat, double *x)
 {
-    int j = blockIdx.y;
-    int i0 = cb[2 * j], in = cb[2 * j + 1];
-    double lam = x[n + j];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < in; i += gridDim.x * blockDim.x)
-        if (i0 + i < n) atomicAdd(&x[i0 + i], -kmat[koff[j] + i] * lam);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double xi = x[i];
+        for (int j = 0; j < m; j++) {
+            const int i0 = cb[2 * j], in = cb[2 * j + 1];
+            if (i >= i0 && i < i0 + in) xi = xi - kmat[koff[j] + (i - i0)] * x[n + j];
+        }
+        x[i] = xi;
+    }
 }
 
 namespace {
 inline int g1(int64_t n, int t = TPB) { return (int)((n + t - 1) / t); }
+}
+
+static void spk_local(int n, int w, int nch, const double *c, const int *india, double *x, int backward, hipStream_t s)
+{
+#define SPK_LOCAL(W) hipLaunchKernelGGL(k_spk_local<W>, dim3(g1(nch, 64)), dim3(64), 0, s, n, nch, c, india, x, backward)
+    switch (w) {
+    case 1: SPK_LOCAL(1); break;
+    case 2: SPK_LOCAL(2); break;
+    case 3: SPK_LOCAL(3); break;
+    case 4: SPK_LOCAL(4); break;
+    case 5: SPK_LOCAL(5); break;
+    case 6: SPK_LOCAL(6); break;
+    case 7: SPK_LOCAL(7); break;
+    case 8: SPK_LOCAL(8); break;
+    case 9: SPK_LOCAL(9); break;
+    case 10: SPK_LOCAL(10); break;
+    case 11: SPK_LOCAL(11); break;
+    case 12: SPK_LOCAL(12); break;
+    case 13: SPK_LOCAL(13); break;
+    case 14: SPK_LOCAL(14); break;
+    case 15: SPK_LOCAL(15); break;
+    default: SPK_LOCAL(16); break;
+    }
+#undef SPK_LOCAL
 }
 
 static void spk_carry(int n, int w, int nch, const double *x, const double *S, double *t, int backward, hipStream_t s)
@@ -1511,15 +1613,15 @@ void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, con
                      double *dtmp, double *x, hipStream_t s)
 {
     int nch = (n + SPK_CH - 1) / SPK_CH;
-    hipLaunchKernelGGL(k_spk_local, dim3(g1(nch, 64)), dim3(64), 0, s, n, w, nch, c, india, x, 0);
+    spk_local(n, w, nch, c, india, x, 0, s);
     spk_carry(n, w, nch, x, Sf, tbuf, 0, s);
     hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sf, tbuf, x, 0);
     if (m > 0) {
         hipLaunchKernelGGL(k_eqs_cdot, dim3(m), dim3(TPB), 0, s, n, cb, koff, kmat, x, dtmp);
         hipLaunchKernelGGL(k_eqs_small, dim3(1), dim3(64), 0, s, n, m, h, hind, dtmp, x);
-        hipLaunchKernelGGL(k_eqs_upd, dim3(8, m), dim3(TPB), 0, s, n, cb, koff, kmat, x);
+        hipLaunchKernelGGL(k_eqs_upd, dim3(g1(n)), dim3(TPB), 0, s, n, m, cb, koff, kmat, x);
     }
-    hipLaunchKernelGGL(k_spk_local, dim3(g1(nch, 64)), dim3(64), 0, s, n, w, nch, c, india, x, 1);
+    spk_local(n, w, nch, c, india, x, 1, s);
     spk_carry(n, w, nch, x, Sb, tbuf, 1, s);
     hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sb, tbuf, x, 1);
 }

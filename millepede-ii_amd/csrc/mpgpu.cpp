@@ -1596,8 +1596,7 @@ void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
         w.zero(s);
         w2.zero(s);
         l_copy(n, r1.p, r2.p, s);
-        const int check = 8;
-        for (int k = 1;; k++) {
+        auto iteration = [&](int k) {
             l_mr_scale(n, d, y.p, v.p, s);
             avprod_dev(c, v.p, y.p);
             if (k >= 2) l_mr_axpy(n, d, 0, r1.p, y.p, s);
@@ -1609,12 +1608,38 @@ void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
             l_dot(n, r2.p, y.p, c->d_part.p, &d->bnew, s);
             l_mr_step(d, s);
             l_mr_update(n, d, v.p, w.p, w1.p, w2.p, x_dev, s);
-            if (k % check == 0 || k >= itnlim) {
-                dst.download(&h, 1, s);
-                HIPCHK(hipStreamSynchronize(s));
-                if (h.istop != 0) break;
+        };
+        auto stopped = [&]() {
+            dst.download(&h, 1, s);
+            HIPCHK(hipStreamSynchronize(s));
+            return h.istop != 0;
+        };
+        // the first iteration eagerly; then blocks of `check` iterations (an even number, so the
+        // r1/r2 roles return to the same buffers) captured once into a HIP graph and replayed:
+        // one launch per block instead of some twenty kernel launches per iteration
+        const int check = 8;
+        iteration(1);
+        int k = 1;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (!stopped() && k < itnlim) {
+            bool graphed = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (graphed) {
+                for (int q = 1; q <= check; q++) iteration(k + q);
+                graphed = hipStreamEndCapture(s, &graph) == hipSuccess && graph != nullptr &&
+                          hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+                if (!graphed) (void)hipGetLastError();
+            }
+            for (;;) {
+                if (graphed) HIPCHK(hipGraphLaunch(exec, s));
+                else
+                    for (int q = 1; q <= check; q++) iteration(k + q);
+                k += check;
+                if (stopped() || k >= itnlim + check) break;
             }
         }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         istop = h.istop;
         itn = h.itn;
         Anorm = h.Anorm;

@@ -219,6 +219,27 @@ def test_minres_sparse_constraints(orc, mbandw):
         assert np.max(np.abs(x2[:P.nvar] - xo[:P.nvar]) / sig) < 1e-6
 
 
+@pytest.mark.parametrize("mbandw", [0, 6])
+def test_minres_repeatable(mbandw):
+    """MINRES runs its iterations from a captured graph of eight iterations (replayed until the
+    device-side stop code is set): solving the same system three times, with eager work and
+    synchronisation in between, gives bitwise the same solution, iteration count and stop code."""
+    rp, ii, gg, cons, gam = _pixel_cons_problem(seed=23)
+    P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=5,
+              mcount=1, mbandw=mbandw, mrestl=1e-12, itnlim=203)
+    P.loopn()
+    runs = []
+    for _ in range(3):
+        x, st = P.solve()
+        P.avprod(x)                     # eager work between the solves (the matrix is not rebuilt:
+        P.vector()                      # the vector's FP64 atomics would change its last bits)
+        runs.append((x.copy(), st["itn"], st["istop"]))
+    for x, itn, istop in runs[1:]:
+        assert np.array_equal(x, runs[0][0])
+        assert (itn, istop) == runs[0][1:]
+    assert runs[0][1] <= 203
+
+
 @pytest.mark.parametrize("mbandw,mrmode", [(-1, 0), (-1, 2), (6, 0), (6, 2)])
 def test_minresqlp_sparse_constraints(orc, mbandw, mrmode):
     """MINRES-QLP (mresmode 0: QR steps, QLP once ill-conditioned; 2: QLP only) on the
