@@ -1320,60 +1320,68 @@ __global__ void k_spk_build(int n, int w, int nch, const double *c, const int *i
     for (int i = 0; i < L; i++) Sb[((size_t)(k0 - 1 + i)) * w + q] = k1 < n ? x[k0 - 1 + i] : 0.0;
 }
 
-// chunk-local solves with a zero boundary, one thread per chunk.  The last W values of x stay in
-// registers (a sliding window); only the band values and the right-hand side are read per row.
-// Same operations in the same order as spk_fwd_chunk / spk_bwd_chunk.
+// chunk-local solves with a zero boundary, one wavefront per chunk: the lanes stage the chunk's
+// band rows (W+1 values, zero outside the band) and right-hand side in LDS, lane 0 runs the
+// recurrence from LDS with the last W values of x in registers, the lanes write the result.
+// Terms outside the band or before the chunk are exact zeros, so the recurrence needs no tests
+// (the operations of spk_fwd_chunk / spk_bwd_chunk in the same order, up to the sign of zero).
 template <int W>
-__global__ void k_spk_local(int n, int nch, const double *c, const int *india, double *x, int backward)
+__global__ __launch_bounds__(64) void k_spk_local(int n, int nch, const double *c, const int *india, double *x, int backward)
 {
-    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sc[SPK_CH * (W + 1)];   // row r = k - k0: element (k, k-W+o) at [r*(W+1) + o]
+    __shared__ double sx[SPK_CH];
+    const int ch = blockIdx.x, lane = threadIdx.x;
     if (ch >= nch) return;
-    const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
-    double xw[W + 1];
+    const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1), L = k1 - k0 + 1;
+    for (int t = lane; t < L * (W + 1); t += 64) {
+        const int r = t / (W + 1), o = t - r * (W + 1);
+        const int k = k0 + r, j = k - W + o;
+        const int ik = india[k - 1], len = ik - (k > 1 ? india[k - 2] : 0);
+        sc[t] = (j >= k - len + 1 && j >= k0) ? c[ik - k + j - 1] : 0.0;
+    }
+    for (int r = lane; r < L; r += 64) sx[r] = x[k0 + r - 1];
+    __syncthreads();
+    if (lane == 0) {
+        double xw[W + 1];
 #pragma unroll
-    for (int q = 0; q <= W; q++) xw[q] = 0.0;
-    if (!backward) {
-        // xw[q] = x_{k-W-1+q} for q < W (zero before the chunk); x_k = (b_k - sum_j C_kj x_j) * C_kk
-        for (int k = k0; k <= k1; k++) {
-            const int ik = india[k - 1], len = ik - (k > 1 ? india[k - 2] : 0);
-            const int first = max(k - len + 1, k0);
-            double xk = x[k - 1];
+        for (int q = 0; q <= W; q++) xw[q] = 0.0;
+        if (!backward) {
+            // at row k: xw[q+1] = x_{k-W+q}
+#pragma unroll 2
+            for (int r = 0; r < L; r++) {
+                const double *row = sc + r * (W + 1);
+                double xk = sx[r];
 #pragma unroll
-            for (int q = 0; q < W; q++) {
-                const int j = k - W + q;
-                const double cj = c[ik - k + max(j, first) - 1];
-                if (j >= first) xk = xk - cj * xw[q + 1];
+                for (int q = 0; q < W; q++) xk = xk - row[q] * xw[q + 1];
+                xk = xk * row[W];
+                sx[r] = xk;
+#pragma unroll
+                for (int q = 0; q < W; q++) xw[q] = xw[q + 1];
+                xw[W] = xk;
             }
-            xk = xk * c[ik - 1];
-            x[k - 1] = xk;
+        } else {
+            // at row k: xw[q] = pending x_{k-W+q} (q < W), xw[W] = pending x_k
 #pragma unroll
-            for (int q = 0; q < W; q++) xw[q] = xw[q + 1];
-            xw[W] = xk;
-        }
-    } else {
-        // xw[q] holds the pending x_{k-W+q} (q < W) and xw[W] the pending x_k
-        for (int q = 0; q <= W; q++) {
-            const int j = k1 - W + q;
-            xw[q] = j >= k0 ? x[j - 1] : 0.0;
-        }
-        for (int k = k1; k >= k0; k--) {
-            const int ik = india[k - 1], len = ik - (k > 1 ? india[k - 2] : 0);
-            const double xk = xw[W] * c[ik - 1];
-            x[k - 1] = xk;
-            if (k == 1) break;
-            const int first = max(k - len + 1, k0);
-#pragma unroll
-            for (int q = 0; q < W; q++) {
-                const int j = k - W + q;
-                const double cj = c[ik - k + max(j, first) - 1];
-                if (j >= first) xw[q] = xw[q] - cj * xk;
+            for (int q = 0; q <= W; q++) {
+                const int r = L - 1 - W + q;
+                xw[q] = r >= 0 ? sx[r] : 0.0;
             }
+#pragma unroll 2
+            for (int r = L - 1; r >= 0; r--) {
+                const double *row = sc + r * (W + 1);
+                const double xk = xw[W] * row[W];
+                sx[r] = xk;
 #pragma unroll
-            for (int q = W; q > 0; q--) xw[q] = xw[q - 1];
-            const int jn = k - 1 - W;
-            xw[0] = jn >= k0 ? x[jn - 1] : 0.0;
+                for (int q = 0; q < W; q++) xw[q] = xw[q] - row[q] * xk;
+#pragma unroll
+                for (int q = W; q > 0; q--) xw[q] = xw[q - 1];
+                const int rn = r - 1 - W;
+                xw[0] = rn >= 0 ? sx[rn] : 0.0;
+            }
         }
     }
+    __syncthreads();
+    for (int r = lane; r < L; r += 64) x[k0 + r - 1] = sx[r];
 }
 
 // carry: boundary values of every chunk, sequential over chunks (w x w products).  The data
@@ -1555,7 +1563,7 @@ inline int g1(int64_t n, int t = TPB) { return (int)((n + t - 1) / t); }
 
 static void spk_local(int n, int w, int nch, const double *c, const int *india, double *x, int backward, hipStream_t s)
 {
-#define SPK_LOCAL(W) hipLaunchKernelGGL(k_spk_local<W>, dim3(g1(nch, 64)), dim3(64), 0, s, n, nch, c, india, x, backward)
+#define SPK_LOCAL(W) hipLaunchKernelGGL(k_spk_local<W>, dim3(nch), dim3(64), 0, s, n, nch, c, india, x, backward)
     switch (w) {
     case 1: SPK_LOCAL(1); break;
     case 2: SPK_LOCAL(2); break;
