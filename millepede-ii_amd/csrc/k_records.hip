@@ -26,6 +26,7 @@
 namespace mpgpu {
 
 #define WAVE 64
+typedef double d4 __attribute__((ext_vector_type(4)));
 #define RT 256                  // threads per record workgroup
 #define RW (RT / WAVE)          // wavefronts per record workgroup
 // Lanes of one wavefront exchanging data through LDS: a wavefront executes its LDS operations
@@ -238,80 +239,6 @@ __device__ int gj_lanes(double acc, int npk, int n, int lane, double *cl, double
     return nrank;
 }
 
-// One segment's update for records with nalc = NALC local parameters and a segment of na <= 8
-// rows: lanes take columns (the column's row of G in registers), pairs of rows of G Gamma^-1 are
-// read together, values are staged per 64 columns in LDS and written out in contiguous runs.
-template <int NALC, typename T>
-__device__ __forceinline__ void stage_segment(int sa, int a0, int na, int ncolumns, double *dst, const double *G,
-                                              const double *ghs, const double *d1, double *stage, const int32_t *segof,
-                                              const unsigned long long *segm, int lane, bool perq, int QE,
-                                              const int32_t *qn, const int32_t *qe, const double *qg, const double *lw,
-                                              double dw1, const int32_t *les, const uint32_t *len, const int32_t *li,
-                                              const T *lv, int neq, const int32_t *gpos)
-{
-    const unsigned long long msa = segm[sa];
-    for (int B0 = 0; B0 < ncolumns; B0 += WAVE) {
-        const int bcol = B0 + lane;
-        const int nb_here = min(WAVE, ncolumns - B0);
-        if (bcol < ncolumns) {
-            double g[NALC];
-#pragma unroll
-            for (int k = 0; k < NALC; k++) g[k] = G[bcol * NALC + k];
-            const int sb = segof[bcol];
-            const bool diagblk = sb == sa;
-            const bool common = !diagblk && (msa & segm[sb]) != 0ull;
-            auto emit = [&](int ra, double d) {
-                const int aa = a0 + ra;
-                double v = 0.0;
-                if (bcol <= aa) {
-                    double c = 0.0;
-                    if (diagblk) c = d1[ra * na + bcol - a0];     // na <= 8 here
-                    else if (common)
-                        c = dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
-                    v = c - d;
-                }
-                stage[lane * na + ra] = v;
-            };
-#pragma unroll
-            for (int r2 = 0; r2 < 8; r2 += 2) {
-                if (r2 >= na) break;
-                // rows r2 and r2+1 (the second one is not used when na is odd; its read stays
-                // inside this wavefront's stage area)
-                double h0[NALC], h1[NALC];
-#pragma unroll
-                for (int k = 0; k < NALC; k++) {
-                    h0[k] = ghs[r2 * NALC + k];
-                    h1[k] = ghs[(r2 + 1) * NALC + k];
-                }
-                double e0 = 0.0, e1 = 0.0;
-#pragma unroll
-                for (int k = 0; k < NALC; k++) {
-                    e0 = e0 + h0[k] * g[k];
-                    e1 = e1 + h1[k] * g[k];
-                }
-                emit(r2, e0);
-                if (r2 + 1 < na) emit(r2 + 1, e1);
-            }
-        }
-        WSYNC();
-        // the staged values of this step are contiguous in the update buffer
-        const int ntot = nb_here * na;   // <= 8 * 64
-        double v[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = lane + q * WAVE;
-            v[q] = t < ntot ? stage[t] : 0.0;
-        }
-        double *o = dst + (int64_t)B0 * na;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = lane + q * WAVE;
-            if (t < ntot) o[t] = v[q];
-        }
-        WSYNC();
-    }
-}
-
 // a rejected record contributes nothing: its slot of the update buffer is cleared
 __device__ __forceinline__ void zero_update(const DevRecords &R, const DevModel &M, const PassOpts &o, int64_t r, int tid)
 {
@@ -325,9 +252,8 @@ __device__ __forceinline__ void zero_update(const DevRecords &R, const DevModel 
 // LDS layout of the record kernel (byte offsets); the kernel carves its dynamic LDS with it and
 // the host sizes the allocation with it, so both always agree
 struct RecLds {
-    int gp, rm, re, lc, lw, cl, bl, dg, cs, co, G, Gh, qg, dsh, wav, segm, lmask, lv;
-    int gv, gg, gpos, gvl, ggl, segof, segst, segro, qn, qe, nxt, sh, wcnt, li, les, len;
-    int wav_stride; // doubles of each wavefront's update-stage area
+    int gp, rm, re, lc, lw, cl, bl, dg, cs, co, G, Gh, qg, dsh, segm, lmask, lv;
+    int gv, gg, gpos, gvl, ggl, segof, segst, segro, rowb, rown, qn, qe, nxt, sh, wcnt, li, les, len;
     int bytes;
 };
 __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
@@ -337,7 +263,6 @@ __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
     auto d = [&](int n) { int p = o; o += n * 8; return p; };
     auto i4 = [&](int n) { int p = o; o += n * 4; return p; };
     const int nv = P.max_var, ne = P.max_eq, ng = P.max_gl, nl = P.nalc_max, QE = P.qe;
-    const int d1n = P.max_grp < 16 ? P.max_grp : 16;
     L.gp = d(ng);
     L.rm = d(ne);
     L.re = d(ne);
@@ -349,11 +274,9 @@ __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
     L.cs = d(nl);
     L.co = d(nl);
     L.G = d(nv * nl);
-    L.Gh = d(P.gh * nv * nl);
+    L.Gh = d(nv * nl);
     L.qg = d(nv * QE);
     L.dsh = d(8);
-    L.wav_stride = P.max_grp * P.nalc_max + d1n * d1n + P.stage;
-    L.wav = d(RW * L.wav_stride);
     L.segm = d(nv);
     L.lmask = d(ne);
     L.lv = o;
@@ -366,6 +289,8 @@ __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
     L.segof = i4(nv);
     L.segst = i4(nv + 1);
     L.segro = i4(nv + 1);
+    L.rowb = i4(P.stage > 0 ? nv : 0);
+    L.rown = i4(P.stage > 0 ? nv : 0);
     L.qn = i4(nv);
     L.qe = i4(nv * QE);
     L.nxt = i4(nl + 1);
@@ -379,7 +304,7 @@ __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
 }
 
 template <typename T>
-__global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_records(DevRecords R, DevModel M, PassOpts o, RecOut out, LdsPlan P,
+__global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_records(DevRecords R, DevModel M, PassOpts o, RecOut out, LdsPlan P,
                                                 int64_t rec_begin, int64_t rec_end)
 {
     extern __shared__ double lds[];
@@ -402,7 +327,7 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     double *co = (double *)(lb + L.co);      // [nalc_max] original pivot column
     double *G = (double *)(lb + L.G);        // [max_var * nalc_max]; during the local fit the
     double *LD = G;                          //   dense local derivatives [neq][nalc] when they fit
-    double *Gh = (double *)(lb + L.Gh);      // [max_var * nalc_max] G Gamma^-1 (full storage only)
+    double *Gh = (double *)(lb + L.Gh);      // [max_var * nalc_max] G Gamma^-1
     double *qg = (double *)(lb + L.qg);      // [max_var*QE] derivative of a variable in its equations
     double *dsh = (double *)(lb + L.dsh);    // [8] double scalars
     unsigned long long *segm = (unsigned long long *)(lb + L.segm);   // [max_var] equation mask of each segment
@@ -416,6 +341,8 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     int32_t *segof = (int32_t *)(lb + L.segof); // [max_var] segment of each record variable
     int32_t *segst = (int32_t *)(lb + L.segst); // [max_var+1] first record variable of each segment
     int32_t *segro = (int32_t *)(lb + L.segro); // [max_var+1] first update value of each segment
+    int32_t *rowb = (int32_t *)(lb + L.rowb);   // [max_var] update-buffer offset of variable a's row (column 0)
+    int32_t *rown = (int32_t *)(lb + L.rown);   // [max_var] column stride of that row (its segment's size)
     int32_t *qn = (int32_t *)(lb + L.qn);       // [max_var] equations of each record variable
     int32_t *qe = (int32_t *)(lb + L.qe);       // [max_var*QE] their indices (ascending)
     int32_t *nxt = (int32_t *)(lb + L.nxt);     // [nalc_max+1] pivot list
@@ -424,11 +351,6 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     int32_t *li = (int32_t *)(lb + L.li);       // [max_w] the record's word indices (local / global position)
     int32_t *les = (int32_t *)(lb + L.les);     // [max_eq] measurement word of each equation (record-local)
     uint32_t *len = (uint32_t *)(lb + L.len);   // [max_eq] local / global word counts
-    // this wavefront's update-stage area: rows of G Gamma^-1, diagonal g g^T, staged values
-    const int d1n = min(P.max_grp, 16);
-    double *ghs = (double *)(lb + L.wav) + (size_t)wv * L.wav_stride; // [max_grp * nalc_max]
-    double *d1 = ghs + (size_t)P.max_grp * P.nalc_max;          // [min(max_grp,16)^2]
-    double *stage = d1 + (size_t)d1n * d1n;                     // [P.stage]
     auto lwv = [&](int x) { return (double)lv[x]; };
 
     for (int64_t ri = rec_begin + blockIdx.x; ri < rec_end; ri += gridDim.x) {
@@ -850,6 +772,7 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 unsigned long long m = __ballot(st);
                 int s = nseg + __popcll(m & ((2ull << lane) - 1ull)) - 1;
                 if (a < nalg && st) segst[s] = a;
+                if (a < nalg) segof[a] = s;
                 nseg += __popcll(m);
             }
             if (lane == 0) { segst[nseg] = nalg; sh[6] = nseg; }
@@ -961,16 +884,15 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 }
             }
             __syncthreads();
+            // Gh = G Gamma^-1 (row-wise, l ascending as dbavat), one element per thread
+            for (int t = tid; t < nalg * nalc; t += RT) {
+                const int a = t / nalc, k = t - a * nalc + 1;
+                double c = 0.0;
+                for (int l = 1; l <= nalc; l++) c = c + G[a * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
+                Gh[t] = c;
+            }
+            __syncthreads();
             if (!twophase) {
-                // Gh = G Gamma^-1 (row-wise, l ascending as dbavat)
-                for (int a = tid; a < nalg; a += RT) {
-                    for (int k = 1; k <= nalc; k++) {
-                        double c = 0.0;
-                        for (int l = 1; l <= nalc; l++) c = c + G[a * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
-                        Gh[a * nalc + k - 1] = c;
-                    }
-                }
-                __syncthreads();
                 // full storage: element (a>=b) -= sum_k Gh[a,k] G[b,k]; packed (a, b) enumeration,
                 // thread starts at element `tid`, then steps by RT
                 const int ngg = (nalg * nalg + nalg) / 2;
@@ -991,20 +913,20 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 // segment sa), each column holding the na values of the segment's rows
                 // (value (ra, b) at b * na + ra); the diagonal block is kept square, its upper
                 // part is zero.  The block of segment pair (sa, sb) is thus contiguous
-                // (columns of sb), which the block gather reads.  Each wavefront takes whole
-                // segments (largest first, from a shared counter); its lanes take columns, keep
-                // the segment's rows of G Gamma^-1 in registers, and stage 64 columns in LDS so
-                // that the buffer is written in contiguous runs.
+                // (columns of sb), which the block gather reads.  The values
+                // C1(a,b) - sum_k Gh[a,k] G[b,k] (b <= a) of different segments come from 16 x 16
+                // tiles of the record's lower triangle, one tile per wavefront step, the products
+                // on the matrix cores (v_mfma_f64_16x16x4f64, nalc in steps of four); the
+                // diagonal blocks are formed element by element; every value goes straight to its
+                // place in the update buffer.
                 const int nseg = sh[6];
                 double *dst = R.cbuf + R.rec_cb[r];
                 // equations touching each segment (bit i for equation i < 64; bit 63 also stands
-                // for all equations >= 63): pairs without a common equation skip g g^T; segment
-                // of every record variable
+                // for all equations >= 63): pairs without a common equation have no g g^T part
                 for (int q = tid; q < nseg; q += RT) {
                     unsigned long long m = 0ull;
                     for (int a = segst[q]; a < segst[q + 1]; a++) {
-                        segof[a] = q;
-                        if (perq) { m = ~0ull; continue; }
+                        if (perq) { m = ~0ull; break; }
                         for (int x = 0; x < qn[a]; x++) m |= 1ull << min(qe[a * QE + x], 63);
                     }
                     segm[q] = m;
@@ -1025,73 +947,54 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     }
                 }
                 __syncthreads();
-                for (;;) {
-                    int qq = 0;
-                    if (lane == 0) qq = atomicAdd(&sh[7], 1);
-                    qq = __shfl(qq, 0, WAVE);
-                    if (qq >= nseg) break;
-                    const int sa = nseg - 1 - qq;
-                    const int a0 = segst[sa], na = segst[sa + 1] - a0;
-                    const int ncolumns = a0 + na;
-                    const int64_t roff = segro[sa];
-                    // rows of G Gamma^-1 of the segment (l ascending, as dbavat)
-                    for (int t = lane; t < na * nalc; t += WAVE) {
-                        const int ra = t / nalc, k = t - ra * nalc + 1;
-                        double c = 0.0;
-                        for (int l = 1; l <= nalc; l++) c = c + G[(a0 + ra) * nalc + l - 1] * cl[l >= k ? pk(l, k) : pk(k, l)];
-                        ghs[t] = c;
+                // row descriptors; the upper part of the diagonal blocks is zero
+                for (int a = tid; a < nalg; a += RT) {
+                    const int q = segof[a], a0 = segst[q], na = segst[q + 1] - a0;
+                    const int base = segro[q] + (a - a0);
+                    rowb[a] = base;
+                    rown[a] = na;
+                    for (int b = a + 1; b < a0 + na; b++) dst[base + (int64_t)b * na] = 0.0;
+                }
+                __syncthreads();
+                // diagonal blocks (a, b <= a in one segment), one element per thread: g g^T of the
+                // common equations minus sum_k Gh[a,k] G[b,k] (k ascending)
+                const int gw = P.max_grp;
+                for (int t = tid; t < nalg * gw; t += RT) {
+                    const int a = t / gw, j = t - a * gw;
+                    const int b = segst[segof[a]] + j;
+                    if (b > a) continue;
+                    double d = 0.0;
+                    for (int k = 0; k < nalc; k++) d = d + Gh[a * nalc + k] * G[b * nalc + k];
+                    const double c = dc1<T>(a, b, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
+                    dst[rowb[a] + (int64_t)b * rown[a]] = c - d;
+                }
+                const int nt = (nalg + 15) >> 4;
+                const int lr = lane & 15, lk = lane >> 4;
+                for (int tt = wv; tt < nt * (nt + 1) / 2; tt += RW) {
+                    int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
+                    while (ti * (ti + 1) / 2 > tt) ti--;
+                    while ((ti + 1) * (ti + 2) / 2 <= tt) ti++;
+                    const int tj = tt - ti * (ti + 1) / 2;
+                    const int ra = 16 * ti + lr, rb = 16 * tj + lr;
+                    d4 acc = {0.0, 0.0, 0.0, 0.0};
+                    for (int kk = 0; kk < nalc; kk += 4) {
+                        const int k = kk + lk;
+                        const double av = (ra < nalg && k < nalc) ? Gh[ra * nalc + k] : 0.0;
+                        const double bv = (rb < nalg && k < nalc) ? G[rb * nalc + k] : 0.0;
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
                     }
-                    // g g^T of the diagonal block, one element per lane
-                    const bool dcommon = segm[sa] != 0ull;
-                    const bool use_d1 = na <= 16;           // d1 holds blocks of up to 16 x 16
-                    for (int t = lane; use_d1 && t < na * na; t += WAVE) {
-                        const int ra = t / na, rb = t - ra * na;
-                        d1[t] = (dcommon && rb <= ra)
-                                    ? dc1<T>(a0 + ra, a0 + rb, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos)
-                                    : 0.0;
-                    }
-                    WSYNC();
-                    const bool fast = na <= 8 && nalc <= 8 && 64 * na <= P.stage;
-                    if (fast) {
-                        double *dsa = dst + roff;
-#define STAGE_SEGMENT(NL)                                                                                      \
-    stage_segment<NL, T>(sa, a0, na, ncolumns, dsa, G, ghs, d1, stage, segof, segm, lane, perq, QE, qn, qe, qg, lw, \
-                         dw1, les, len, li, lv, neq, gpos)
-                        switch (nalc) {
-                        case 1: STAGE_SEGMENT(1); break;
-                        case 2: STAGE_SEGMENT(2); break;
-                        case 3: STAGE_SEGMENT(3); break;
-                        case 4: STAGE_SEGMENT(4); break;
-                        case 5: STAGE_SEGMENT(5); break;
-                        case 6: STAGE_SEGMENT(6); break;
-                        case 7: STAGE_SEGMENT(7); break;
-                        default: STAGE_SEGMENT(8); break;
-                        }
-#undef STAGE_SEGMENT
-                    } else {
-                        for (int B0 = 0; B0 < ncolumns; B0 += WAVE) {
-                            const int bcol = B0 + lane;
-                            if (bcol >= ncolumns) continue;
-                            const int sb = segof[bcol];
-                            const bool diagblk = sb == sa;
-                            const bool common = !diagblk && (segm[sa] & segm[sb]) != 0ull;
-                            for (int ra = 0; ra < na; ra++) {
-                                const int aa = a0 + ra;
-                                double v = 0.0;
-                                if (bcol <= aa) {
-                                    double d = 0.0;
-                                    for (int k = 0; k < nalc; k++) d = d + ghs[ra * nalc + k] * G[bcol * nalc + k];
-                                    double c = 0.0;
-                                    if (diagblk && use_d1) c = d1[ra * na + bcol - a0];
-                                    else if (diagblk || common)
-                                        c = dc1<T>(aa, bcol, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
-                                    v = c - d;
-                                }
-                                dst[roff + (int64_t)bcol * na + ra] = v;
-                            }
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int a = 16 * ti + lk + 4 * e, b = 16 * tj + lr;   // result element e of this lane
+                        if (a < nalg && b <= a) {
+                            const int sa = segof[a], sb = segof[b];
+                            if (sa == sb) continue;                 // diagonal blocks: above
+                            double c = 0.0;
+                            if ((segm[sa] & segm[sb]) != 0ull)
+                                c = dc1<T>(a, b, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
+                            dst[rowb[a] + (int64_t)b * rown[a]] = c - acc[e];
                         }
                     }
-                    WSYNC();
                 }
             }
         }

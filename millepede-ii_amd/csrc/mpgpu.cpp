@@ -835,8 +835,8 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     {
         int32_t maxgrp = 1;
         for (int32_t g = 0; g < c->ngroup; g++) maxgrp = std::max(maxgrp, c->gfirst[g + 1] - c->gfirst[g]);
-        // 64 rows of the widest group pair; larger steps are written row by row
-        c->plan.stage = sparse ? std::min(64 * maxgrp, 64 * 8) : 0;
+        // group-block storage: row descriptors of the update buffer
+        c->plan.stage = sparse ? 1 : 0;
         c->plan.max_grp = maxgrp;
     }
     lds_bytes(c->plan);

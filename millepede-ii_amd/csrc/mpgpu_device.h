@@ -66,8 +66,8 @@ struct RecOut {
 struct LdsPlan {
     int32_t max_gl, max_eq, nalc_max, max_var, max_pair;
     int32_t qe;      // equations kept per record variable (1..8)
-    int32_t gh;      // 1: G Gamma^-1 held in LDS (full storage), 0: formed in registers
-    int32_t stage;   // doubles of the LDS staging area for the update writes (group-block storage)
+    int32_t gh;      // (unused)
+    int32_t stage;   // 1: group-block storage (update-buffer row descriptors in LDS)
     int32_t max_w;   // words of the largest record (staged in LDS)
     int32_t max_grp; // parameters of the largest group
     int32_t bytes;
