@@ -252,8 +252,8 @@ __device__ __forceinline__ void zero_update(const DevRecords &R, const DevModel 
 // LDS layout of the record kernel (byte offsets); the kernel carves its dynamic LDS with it and
 // the host sizes the allocation with it, so both always agree
 struct RecLds {
-    int gp, rm, re, lc, lw, cl, bl, dg, cs, co, G, Gh, qg, dsh, segm, lmask, lv;
-    int gv, gg, gpos, gvl, ggl, segof, segst, segro, rowb, rown, qn, qe, nxt, sh, wcnt, li, les, len;
+    int gp, rm, re, lc, lw, cl, bl, dg, cs, co, G, Gh, qg, dsh, segm, lmask, rdesc, vsegm, lv;
+    int gv, gg, gpos, gvl, ggl, segof, segst, segro, qn, qe, nxt, sh, wcnt, li, les, len;
     int bytes;
 };
 __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
@@ -279,6 +279,9 @@ __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
     L.dsh = d(8);
     L.segm = d(nv);
     L.lmask = d(ne);
+    o = (o + 15) / 16 * 16;
+    L.rdesc = d(P.stage > 0 ? 2 * nv : 0);
+    L.vsegm = d(P.stage > 0 ? nv : 0);
     L.lv = o;
     o += ((P.max_w > 0 ? P.max_w : 1) * tsz + 7) / 8 * 8;
     L.gv = i4(ng);
@@ -289,8 +292,6 @@ __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
     L.segof = i4(nv);
     L.segst = i4(nv + 1);
     L.segro = i4(nv + 1);
-    L.rowb = i4(P.stage > 0 ? nv : 0);
-    L.rown = i4(P.stage > 0 ? nv : 0);
     L.qn = i4(nv);
     L.qe = i4(nv * QE);
     L.nxt = i4(nl + 1);
@@ -341,8 +342,10 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     int32_t *segof = (int32_t *)(lb + L.segof); // [max_var] segment of each record variable
     int32_t *segst = (int32_t *)(lb + L.segst); // [max_var+1] first record variable of each segment
     int32_t *segro = (int32_t *)(lb + L.segro); // [max_var+1] first update value of each segment
-    int32_t *rowb = (int32_t *)(lb + L.rowb);   // [max_var] update-buffer offset of variable a's row (column 0)
-    int32_t *rown = (int32_t *)(lb + L.rown);   // [max_var] column stride of that row (its segment's size)
+    // [max_var] per record variable a: update-buffer offset of its row (column 0), the row's
+    // column stride (its segment's size), its segment; and the segment's equation mask
+    int4 *rdesc = (int4 *)(lb + L.rdesc);
+    unsigned long long *vsegm = (unsigned long long *)(lb + L.vsegm);
     int32_t *qn = (int32_t *)(lb + L.qn);       // [max_var] equations of each record variable
     int32_t *qe = (int32_t *)(lb + L.qe);       // [max_var*QE] their indices (ascending)
     int32_t *nxt = (int32_t *)(lb + L.nxt);     // [nalc_max+1] pivot list
@@ -923,16 +926,26 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 double *dst = R.cbuf + R.rec_cb[r];
                 // equations touching each segment (bit i for equation i < 64; bit 63 also stands
                 // for all equations >= 63): pairs without a common equation have no g g^T part
-                for (int q = tid; q < nseg; q += RT) {
-                    unsigned long long m = 0ull;
-                    for (int a = segst[q]; a < segst[q + 1]; a++) {
-                        if (perq) { m = ~0ull; break; }
-                        for (int x = 0; x < qn[a]; x++) m |= 1ull << min(qe[a * QE + x], 63);
-                    }
-                    segm[q] = m;
-                }
-                // first update value of each segment (exclusive prefix of ncolumns * na)
+                // first update value of each segment (exclusive prefix of ncolumns * na); whether
+                // two segments share an equation at all (else no cross-segment g g^T)
                 if (wv == 0) {
+                    unsigned long long mor = 0ull;
+                    int bits = 0;
+                    for (int q = lane; q < nseg; q += WAVE) {
+                        unsigned long long m = 0ull;
+                        for (int a = segst[q]; a < segst[q + 1]; a++) {
+                            if (perq) { m = ~0ull; break; }
+                            for (int x = 0; x < qn[a]; x++) m |= 1ull << min(qe[a * QE + x], 63);
+                        }
+                        segm[q] = m;
+                        mor |= m;
+                        bits += __popcll(m);
+                    }
+                    for (int o = 32; o > 0; o >>= 1) {
+                        mor |= __shfl_xor(mor, o, WAVE);
+                        bits += __shfl_xor(bits, o, WAVE);
+                    }
+                    if (lane == 0) sh[11] = (perq || bits > __popcll(mor)) ? 1 : 0;
                     int run = 0;
                     for (int base = 0; base < nseg; base += WAVE) {
                         const int q = base + lane;
@@ -948,11 +961,12 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
                 __syncthreads();
                 // row descriptors; the upper part of the diagonal blocks is zero
+                const bool anycommon = sh[11] != 0;
                 for (int a = tid; a < nalg; a += RT) {
                     const int q = segof[a], a0 = segst[q], na = segst[q + 1] - a0;
                     const int base = segro[q] + (a - a0);
-                    rowb[a] = base;
-                    rown[a] = na;
+                    rdesc[a] = make_int4(base, na, q, 0);
+                    vsegm[a] = segm[q];
                     for (int b = a + 1; b < a0 + na; b++) dst[base + (int64_t)b * na] = 0.0;
                 }
                 __syncthreads();
@@ -966,33 +980,39 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     double d = 0.0;
                     for (int k = 0; k < nalc; k++) d = d + Gh[a * nalc + k] * G[b * nalc + k];
                     const double c = dc1<T>(a, b, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
-                    dst[rowb[a] + (int64_t)b * rown[a]] = c - d;
+                    const int4 da = rdesc[a];
+                    dst[da.x + (int64_t)b * da.y] = c - d;
                 }
                 const int nt = (nalg + 15) >> 4;
                 const int lr = lane & 15, lk = lane >> 4;
+                // the product is formed transposed (rows of G as the first operand): lane l holds
+                // row a = 16 ti + l%16 and four columns b, so the sixteen lanes of one store write
+                // a column run of consecutive rows, contiguous within a segment
                 for (int tt = wv; tt < nt * (nt + 1) / 2; tt += RW) {
                     int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
                     while (ti * (ti + 1) / 2 > tt) ti--;
                     while ((ti + 1) * (ti + 2) / 2 <= tt) ti++;
                     const int tj = tt - ti * (ti + 1) / 2;
-                    const int ra = 16 * ti + lr, rb = 16 * tj + lr;
+                    const int a = 16 * ti + lr, rb = 16 * tj + lr;
+                    const bool arow = a < nalg;
+                    const int4 da = rdesc[arow ? a : nalg - 1];
                     d4 acc = {0.0, 0.0, 0.0, 0.0};
                     for (int kk = 0; kk < nalc; kk += 4) {
                         const int k = kk + lk;
-                        const double av = (ra < nalg && k < nalc) ? Gh[ra * nalc + k] : 0.0;
-                        const double bv = (rb < nalg && k < nalc) ? G[rb * nalc + k] : 0.0;
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                        const double gb = (rb < nalg && k < nalc) ? G[rb * nalc + k] : 0.0;
+                        const double ha = (arow && k < nalc) ? Gh[a * nalc + k] : 0.0;
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(gb, ha, acc, 0, 0, 0);
                     }
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
-                        const int a = 16 * ti + lk + 4 * e, b = 16 * tj + lr;   // result element e of this lane
-                        if (a < nalg && b <= a) {
-                            const int sa = segof[a], sb = segof[b];
-                            if (sa == sb) continue;                 // diagonal blocks: above
+                        const int b = 16 * tj + lk + 4 * e;   // result element e: (row a, column b)
+                        const int bc = b < nalg ? b : nalg - 1;
+                        const int sb = segof[bc];
+                        if (arow && b <= a && sb != da.z) {   // diagonal blocks: above
                             double c = 0.0;
-                            if ((segm[sa] & segm[sb]) != 0ull)
+                            if (anycommon && (vsegm[arow ? a : nalg - 1] & vsegm[bc]) != 0ull)
                                 c = dc1<T>(a, b, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
-                            dst[rowb[a] + (int64_t)b * rown[a]] = c - acc[e];
+                            dst[da.x + (int64_t)b * da.y] = c - acc[e];
                         }
                     }
                 }
