@@ -384,13 +384,26 @@ __global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_
         const int R = min(r0 + rl, nr - 1);   // lanes past the last row repeat it (result unused)
         double acc0 = 0.0, acc1 = 0.0;
         int64_t bk = b0 + slot;
-        // two blocks per slot and step: their index and value loads are issued together
+        // two blocks per slot and step; the next step's block descriptors are read one step ahead
+        int2 d0 = make_int2(0, 1), d1 = make_int2(0, 1);
+        int64_t o0 = 0, o1 = 0;
+        if (bk + 8 < b1) {
+            d0 = *(const int2 *)(bcf + 2 * bk);
+            d1 = *(const int2 *)(bcf + 2 * bk + 16);
+            o0 = boff[bk];
+            o1 = boff[bk + 8];
+        }
         for (; bk + 8 < b1; bk += 16) {
-            const int fb0 = bcf[2 * bk], nc0 = bcf[2 * bk + 1];
-            const int fb1 = bcf[2 * bk + 16], nc1 = bcf[2 * bk + 17];
-            const double *B0 = a + boff[bk], *B1 = a + boff[bk + 8];
-            acc0 += bsymv_row(B0, nc0, R, fb0 == fa, x + fb0);
-            acc1 += bsymv_row(B1, nc1, R, fb1 == fa, x + fb1);
+            const int2 c0 = d0, c1 = d1;
+            const int64_t p0 = o0, p1 = o1;
+            if (bk + 24 < b1) {
+                d0 = *(const int2 *)(bcf + 2 * bk + 32);
+                d1 = *(const int2 *)(bcf + 2 * bk + 48);
+                o0 = boff[bk + 16];
+                o1 = boff[bk + 24];
+            }
+            acc0 += bsymv_row(a + p0, c0.y, R, c0.x == fa, x + c0.x);
+            acc1 += bsymv_row(a + p1, c1.y, R, c1.x == fa, x + c1.x);
         }
         if (bk < b1) {
             const int fb0 = bcf[2 * bk], nc0 = bcf[2 * bk + 1];
@@ -434,12 +447,26 @@ __global__ __launch_bounds__(256) void k_bsymv_cols(int32_t ngroup, const int64_
         const int Cc = min(c0 + cl, nc - 1);   // lanes past the last column repeat it (result unused)
         double acc0 = 0.0, acc1 = 0.0;
         int64_t q = q0 + slot;
+        // the next step's entries are read one step ahead
+        int2 d0 = make_int2(0, 1), d1 = make_int2(0, 1);
+        int64_t o0 = 0, o1 = 0;
+        if (q + 8 < q1) {
+            d0 = *(const int2 *)(cc_fn + 2 * q);
+            d1 = *(const int2 *)(cc_fn + 2 * q + 16);
+            o0 = cc_off[q];
+            o1 = cc_off[q + 8];
+        }
         for (; q + 8 < q1; q += 16) {
-            const int fa0 = cc_fn[2 * q], nr0 = cc_fn[2 * q + 1];
-            const int fa1 = cc_fn[2 * q + 16], nr1 = cc_fn[2 * q + 17];
-            const double *B0 = a + cc_off[q], *B1 = a + cc_off[q + 8];
-            acc0 += bsymv_col(B0, nr0, nc, Cc, x + fa0);
-            acc1 += bsymv_col(B1, nr1, nc, Cc, x + fa1);
+            const int2 c0 = d0, c1 = d1;
+            const int64_t p0 = o0, p1 = o1;
+            if (q + 24 < q1) {
+                d0 = *(const int2 *)(cc_fn + 2 * q + 32);
+                d1 = *(const int2 *)(cc_fn + 2 * q + 48);
+                o0 = cc_off[q + 16];
+                o1 = cc_off[q + 24];
+            }
+            acc0 += bsymv_col(a + p0, c0.y, nc, Cc, x + c0.x);
+            acc1 += bsymv_col(a + p1, c1.y, nc, Cc, x + c1.x);
         }
         if (q < q1) acc0 += bsymv_col(a + cc_off[q], cc_fn[2 * q + 1], nc, Cc, x + cc_fn[2 * q]);
         const double acc = sum_slots(acc0 + acc1);
