@@ -1040,21 +1040,37 @@ __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int6
     const int nr = gfirst[ga + 1] - gfirst[ga], nc = gfirst[gb + 1] - gfirst[gb];
     const bool diag = ga == gb;
     double *B = mat + boff[k];
-    for (int e = lane; e < nr * nc; e += 64) {
-        const int rr = e / nc, cc = e - rr * nc;
-        if (diag && cc > rr) continue;
+    for (int e0 = 0; e0 < nr * nc; e0 += 64) {
+        const int e = e0 + lane;
+        const bool act = e < nr * nc;
+        const int rr = act ? e / nc : 0, cc = act ? e - rr * nc : 0;
         const int se = cc * nr + rr;        // records store their blocks column by column
-        // four independent partial sums keep four loads in flight
-        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
-        int64_t q = i0;
-        for (; q + 4 <= i1; q += 4) {
-            acc0 += cbuf[src[q] + se];
-            acc1 += cbuf[src[q + 1] + se];
-            acc2 += cbuf[src[q + 2] + se];
-            acc3 += cbuf[src[q + 3] + se];
+        // the contributions' offsets, 64 at a time, one per lane (a coalesced read), then eight
+        // independent partial sums over them, each offset broadcast from its lane
+        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int64_t c0 = i0; c0 < i1; c0 += 64) {
+            const int n = (int)min((int64_t)64, i1 - c0);
+            const int64_t mine = src[c0 + (lane < n ? lane : n - 1)];
+            const int lo = (int)mine, hi = (int)(mine >> 32);
+            int q = 0;
+            for (; q + 8 <= n; q += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int64_t o = ((int64_t)__builtin_amdgcn_readlane(hi, q + u) << 32) |
+                                      (uint32_t)__builtin_amdgcn_readlane(lo, q + u);
+                    v[u] = cbuf[o + se];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) acc[u] += v[u];
+            }
+            for (; q < n; q++) {
+                const int64_t o = ((int64_t)__builtin_amdgcn_readlane(hi, q) << 32) | (uint32_t)__builtin_amdgcn_readlane(lo, q);
+                acc[0] += cbuf[o + se];
+            }
         }
-        for (; q < i1; q++) acc0 += cbuf[src[q] + se];
-        B[e] += (acc0 + acc1) + (acc2 + acc3);
+        if (act && !(diag && cc > rr))
+            B[e] += ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     }
 }
 
