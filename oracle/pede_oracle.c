@@ -10,9 +10,11 @@
  *
  * Pinning (how each function is checked, see tests/test_oracle_*.py):
  *   orc_sqminv, orc_sqminl, orc_chdec2/chslv2, orc_dbavat, orc_chindl, orc_lltdec/fwd/bwd,
- *   orc_equdecs/equslvs, orc_precons/presols, orc_minres
+ *   orc_equdecs/equslvs, orc_precons/presols, orc_sqmibb (with the dbc* band routines),
+ *   orc_minres, orc_minresqlp
  *       -> bit-for-bit (or 1e-13) against the reference Fortran routines compiled from
- *          src/mpnum.f90 / src/minresModule.f90 into oracle/_ref, plus numpy closed forms.
+ *          src/mpnum.f90, src/Dbandmatrix.f90, src/minresModule.f90 and src/minresqlpModule.f90
+ *          into oracle/_ref, plus numpy closed forms.
  *   orc_loopn (record loop)
  *       -> brute-force joint least-squares solve (globals + every record's locals) on tiny
  *          inputs (Schur identity, doc/draftManual.md:291-330), per-record explicit inverse.
