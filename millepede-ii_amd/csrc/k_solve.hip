@@ -1009,6 +1009,96 @@ __global__ __launch_bounds__(256) void k_lltdec_w(int n, double *c, const int *i
     if (tid == 0) *nrkd = nr;
 }
 
+// lltdec for a band of exactly MW values per row (rows 1..MW-1 shorter): the previous MW-1
+// rows live in registers.  Rows are taken in groups of MW aligned to row 1, so that row k sits in
+// ring slot (k-1) % MW, a compile-time index in the unrolled group; the band values are staged
+// per tile in LDS as in k_lltdec.  Same operations in the same order as k_lltdec_w.
+template <int MW>
+__global__ __launch_bounds__(64) void k_lltdec_r(int n, double *c, const int *india, int *nrkd, int iopt)
+{
+    constexpr int ROWS = (LLT_TILE / MW / MW) * MW;   // rows per tile, a multiple of MW
+    __shared__ double buf[LLT_TILE];
+    __shared__ int ind[LLT_TILE + 1];
+    const int tid = threadIdx.x;
+    const double eps = 16.0 * DBL_EPSILON;
+    double ring[MW][MW];                  // slot s: row j with (j-1) % MW == s, element (j, j-MW+1+o) at [s][o]
+#pragma unroll
+    for (int q = 0; q < MW; q++)
+#pragma unroll
+        for (int o = 0; o < MW; o++) ring[q][o] = 0.0;
+    int nr = 0;
+    for (int k0 = 1; k0 <= n; k0 += ROWS) {
+        const int k1 = min(n, k0 + ROWS - 1);
+        const int base = k0 > 1 ? india[k0 - 2] : 0;   // IND(k0-1)
+        const int nel = india[k1 - 1] - base;
+        for (int t = tid; t < nel; t += 64) buf[t] = c[base + t];
+        for (int t = tid; t <= k1 - k0; t += 64) ind[t + 1] = india[k0 - 1 + t];
+        if (tid == 0) ind[0] = base;
+        __syncthreads();
+        if (tid == 0) {
+            for (int kg = k0; kg <= k1; kg += MW) {
+#pragma unroll
+                for (int s = 0; s < MW; s++) {
+                    const int k = kg + s;                   // (k-1) % MW == s
+                    if (k > k1) break;
+                    const int indk = ind[k - k0 + 1], indk1 = ind[k - k0];
+                    const int mk = k - indk + indk1 + 1;
+                    double *Pk = &buf[indk - k - base - 1];
+                    double r[MW];
+#pragma unroll
+                    for (int o = 0; o < MW; o++) {
+                        const int col = k - MW + 1 + o;
+                        const double v = Pk[max(col, mk)];
+                        r[o] = col >= mk ? v : 0.0;
+                    }
+                    if (k == 1) {
+                        double v = r[MW - 1];
+                        if (v > 0.0) v = 1.0 / sqrt(v);
+                        else { v = 0.0; nr = -1; }
+                        r[MW - 1] = v;
+                    } else {
+                        // rows j = k-MW+1+jo (jo < MW-1) are in slot (s + 1 + jo) % MW; terms
+                        // outside a row's band are exact zeros (see k_lltdec_w)
+#pragma unroll
+                        for (int jo = 0; jo < MW - 1; jo++) {
+                            const int sj = (s + 1 + jo) % MW;
+                            const bool valid = k - MW + 1 + jo >= mk;
+                            double ckj = r[jo];
+#pragma unroll
+                            for (int io = 0; io < jo; io++) ckj = ckj - r[io] * (valid ? ring[sj][MW - 1 - (jo - io)] : 0.0);
+                            r[jo] = ckj * (valid ? ring[sj][MW - 1] : 0.0);
+                        }
+                        const double diag = r[MW - 1];
+                        double ckk = r[MW - 1];
+#pragma unroll
+                        for (int io = 0; io < MW - 1; io++) ckk = ckk - r[io] * r[io];
+                        if (ckk > eps * diag) r[MW - 1] = 1.0 / sqrt(ckk);
+                        else {
+#pragma unroll
+                            for (int o = 0; o < MW; o++) r[o] = 0.0;
+                            if (iopt > 0 && diag > 0.0) r[MW - 1] = 1.0 / sqrt(diag);
+                            else {
+                                if (nr == 0) nr = -k;
+                                else { if (nr < 0) nr = 1; nr++; }
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int o = 0; o < MW; o++) {
+                        const int col = k - MW + 1 + o;
+                        if (col >= mk) Pk[col] = r[o];
+                        ring[s][o] = r[o];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < nel; t += 64) c[base + t] = buf[t];
+        __syncthreads();
+    }
+    if (tid == 0) *nrkd = nr;
+}
+
 __device__ void lltfwd_dev(int n, const double *c, const int *india, double *x)
 {
     const double *C = c - 1;
@@ -1261,9 +1351,17 @@ void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, cons
 
 void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, hipStream_t s)
 {
-    if (mbw <= 4) hipLaunchKernelGGL(k_lltdec_w<4>, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt);
-    else if (mbw <= 8) hipLaunchKernelGGL(k_lltdec_w<8>, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt);
-    else hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt, mbw);
+    switch (mbw) {
+    case 1: hipLaunchKernelGGL(k_lltdec_r<1>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 2: hipLaunchKernelGGL(k_lltdec_r<2>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 3: hipLaunchKernelGGL(k_lltdec_r<3>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 4: hipLaunchKernelGGL(k_lltdec_r<4>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 5: hipLaunchKernelGGL(k_lltdec_r<5>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 6: hipLaunchKernelGGL(k_lltdec_r<6>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 7: hipLaunchKernelGGL(k_lltdec_r<7>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 8: hipLaunchKernelGGL(k_lltdec_r<8>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    default: hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt, mbw); break;
+    }
 }
 void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
               hipStream_t s)
