@@ -413,6 +413,73 @@ __global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_
         if (slot == 0 && r0 + rl < nr) y[fa + r0 + rl] = acc;
     }
 }
+// Rows pass for groups of at most 8 parameters with contiguous reads: lane t of a slot takes
+// column t of the slot's block and reads its rows in turn, so each load of a slot covers one
+// block row (nc consecutive doubles).  ys[r] collects row r of the lane's columns; a diagonal
+// block (lower triangle stored) also adds B[r][t] x[fa+r] (r > t) to row t in cs.  The rows are
+// summed over the eight lanes of a slot by a halving butterfly (lane t ends with row t) and then
+// over the slots.
+__global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64_t *brow, const int32_t *bcf,
+                                                     const int64_t *boff, const int32_t *gfirst, const double *a,
+                                                     const double *x, double *y)
+{
+    const int ga = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, slot = lane >> 3, t = lane & 7;
+    if (ga >= ngroup) return;
+    const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
+    const int64_t b0 = brow[ga], b1 = brow[ga + 1];
+    double xa[8], ys[8], cs = 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const double v = x[fa + min(r, nr - 1)];   // clamped index: every load in bounds
+        xa[r] = r < nr ? v : 0.0;
+        ys[r] = 0.0;
+    }
+    for (int64_t bk = b0 + slot; bk < b1; bk += 8) {
+        const int2 d = *(const int2 *)(bcf + 2 * bk);
+        const int fb = d.x, nc = d.y, tc = min(t, nc - 1);
+        const double *B = a + boff[bk] + tc;
+        const double xv = x[fb + tc];
+        const double xb = t < nc ? xv : 0.0;
+        double bv[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) bv[r] = B[min(r, nr - 1) * nc];
+        if (fb != fa) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) ys[r] += bv[r] * xb;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                if (t <= r) ys[r] += bv[r] * xb;
+                if (t < r) cs += bv[r] * xa[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+        if (r == t) ys[r] += cs;
+    const bool h2 = t & 4, h1 = t & 2, h0 = t & 1;
+    double z[4], w[2];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double snd = h2 ? ys[k] : ys[k + 4], kp = h2 ? ys[k + 4] : ys[k];
+        z[k] = kp + __shfl_xor(snd, 4, 64);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const double snd = h1 ? z[k] : z[k + 2], kp = h1 ? z[k + 2] : z[k];
+        w[k] = kp + __shfl_xor(snd, 2, 64);
+    }
+    double v;
+    {
+        const double snd = h0 ? w[0] : w[1], kp = h0 ? w[1] : w[0];
+        v = kp + __shfl_xor(snd, 1, 64);
+    }
+    v += __shfl_xor(v, 8, 64);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (slot == 0 && t < nr) y[fa + t] = v;
+}
 // column Cc of block B (nr x nc, row-major) times x[fa .. fa+nr)
 __device__ __forceinline__ double bsymv_col(const double *B, int nr, int nc, int Cc, const double *xa)
 {
@@ -1294,12 +1361,16 @@ __global__ void k_zero(int64_t n, double *y)
 }
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
                   const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
-                  const double *x, double *y, int32_t nvar, int64_t nall, hipStream_t s)
+                  const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, hipStream_t s)
 {
     // rows beyond the parameter groups (Lagrange multipliers) start at zero
     if (nall > nvar) hipLaunchKernelGGL(k_zero, dim3(nblk(nall - nvar)), dim3(TPB), 0, s, nall - nvar, y + nvar);
-    hipLaunchKernelGGL(k_bsymv_rows, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff, gfirst,
-                       a, x, y);
+    if (maxgrp <= 8)
+        hipLaunchKernelGGL(k_bsymv_rows8, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff,
+                           gfirst, a, x, y);
+    else
+        hipLaunchKernelGGL(k_bsymv_rows, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff,
+                           gfirst, a, x, y);
     hipLaunchKernelGGL(k_bsymv_cols, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_off, cc_fn,
                        gfirst, a, x, y);
 }

@@ -1422,7 +1422,7 @@ void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
     hipStream_t s = c->stream;
     if (c->opt.matsto == MPGPU_SPARSE) {
         l_block_symv(c->ngroup, c->d_brow.p, c->d_bcf.p, c->d_boff.p, c->d_gfirst.p, c->d_ccol_ptr.p, c->d_cc_off.p,
-                     c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, s);
+                     c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, c->plan.max_grp, s);
         l_cons_symv(c->ncon, c->nvar, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, x, y, s);
     } else {
         l_packed_symv(c->nall, c->d_mat.p, x, y, s);
