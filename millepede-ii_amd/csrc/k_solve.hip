@@ -413,12 +413,21 @@ __global__ __launch_bounds__(256) void k_bsymv_rows(int32_t ngroup, const int64_
         if (slot == 0 && r0 + rl < nr) y[fa + r0 + rl] = acc;
     }
 }
-// Rows pass for groups of at most 8 parameters with contiguous reads: lane t of a slot takes
-// column t of the slot's block and reads its rows in turn, so each load of a slot covers one
-// block row (nc consecutive doubles).  ys[r] collects row r of the lane's columns; a diagonal
-// block (lower triangle stored) also adds B[r][t] x[fa+r] (r > t) to row t in cs.  The rows are
-// summed over the eight lanes of a slot by a halving butterfly (lane t ends with row t) and then
-// over the slots.
+// Blocks of groups of at most 8 parameters: lane t of a slot takes column t of the slot's block
+// (clamped to the last column) and loads its 8 rows (clamped to the last row), so one load of a
+// slot reads one block row, nc consecutive doubles.
+__device__ __forceinline__ void bsymv_ld8(double (&bv)[8], const double *B, int nr, int nc, int t)
+{
+    const double *p = B + min(t, nc - 1);
+#pragma unroll
+    for (int r = 0; r < 8; r++) bv[r] = p[min(r, nr - 1) * nc];
+}
+// Rows pass for groups of at most 8 parameters.  ys[r] collects row r of the lane's columns over
+// the off-diagonal blocks of the block row; the values of a slot's next block are loaded while the
+// current one is multiplied, and its descriptors one step earlier still.  The diagonal block (the
+// last block of the row, lower triangle stored) is done by the whole wave: lane (slot s, t) takes
+// the symmetric element (t, s) times x[fa+s].  Rows are summed over the eight lanes of a slot by a
+// halving butterfly (lane t ends with row t) and then over the slots.
 __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64_t *brow, const int32_t *bcf,
                                                      const int64_t *boff, const int32_t *gfirst, const double *a,
                                                      const double *x, double *y)
@@ -428,36 +437,50 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
     if (ga >= ngroup) return;
     const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
     const int64_t b0 = brow[ga], b1 = brow[ga + 1];
-    double xa[8], ys[8], cs = 0.0;
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-        const double v = x[fa + min(r, nr - 1)];   // clamped index: every load in bounds
-        xa[r] = r < nr ? v : 0.0;
-        ys[r] = 0.0;
+    int64_t be = b1;
+    double dv = 0.0, dx = 0.0;
+    if (b1 > b0 && bcf[2 * (b1 - 1)] == fa) {
+        be = b1 - 1;
+        const int tr = min(t, nr - 1), sr = min(slot, nr - 1);
+        dv = a[boff[b1 - 1] + (tr >= sr ? tr * nr + sr : sr * nr + tr)];
+        dx = (t < nr && slot < nr) ? x[fa + sr] : 0.0;
     }
-    for (int64_t bk = b0 + slot; bk < b1; bk += 8) {
+    double ys[8], bv[8], xb = 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) ys[r] = bv[r] = 0.0;
+    int64_t bk = b0 + slot;
+    int2 dn = make_int2(0, 1);
+    int64_t on = 0;
+    if (bk < be) {
         const int2 d = *(const int2 *)(bcf + 2 * bk);
-        const int fb = d.x, nc = d.y, tc = min(t, nc - 1);
-        const double *B = a + boff[bk] + tc;
-        const double xv = x[fb + tc];
-        const double xb = t < nc ? xv : 0.0;
-        double bv[8];
-#pragma unroll
-        for (int r = 0; r < 8; r++) bv[r] = B[min(r, nr - 1) * nc];
-        if (fb != fa) {
-#pragma unroll
-            for (int r = 0; r < 8; r++) ys[r] += bv[r] * xb;
-        } else {
-#pragma unroll
-            for (int r = 0; r < 8; r++) {
-                if (t <= r) ys[r] += bv[r] * xb;
-                if (t < r) cs += bv[r] * xa[r];
-            }
-        }
+        bsymv_ld8(bv, a + boff[bk], nr, d.y, t);
+        const double xv = x[d.x + min(t, d.y - 1)];
+        xb = t < d.y ? xv : 0.0;
     }
+    if (bk + 8 < be) {
+        dn = *(const int2 *)(bcf + 2 * (bk + 8));
+        on = boff[bk + 8];
+    }
+    for (; bk < be; bk += 8) {
+        double bn[8], xn = 0.0;
 #pragma unroll
-    for (int r = 0; r < 8; r++)
-        if (r == t) ys[r] += cs;
+        for (int r = 0; r < 8; r++) bn[r] = 0.0;
+        if (bk + 8 < be) {
+            bsymv_ld8(bn, a + on, nr, dn.y, t);
+            const double xv = x[dn.x + min(t, dn.y - 1)];
+            xn = t < dn.y ? xv : 0.0;
+        }
+        if (bk + 16 < be) {
+            dn = *(const int2 *)(bcf + 2 * (bk + 16));
+            on = boff[bk + 16];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            ys[r] += bv[r] * xb;
+            bv[r] = bn[r];
+        }
+        xb = xn;
+    }
     const bool h2 = t & 4, h1 = t & 2, h0 = t & 1;
     double z[4], w[2];
 #pragma unroll
@@ -475,10 +498,64 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
         const double snd = h0 ? w[0] : w[1], kp = h0 ? w[1] : w[0];
         v = kp + __shfl_xor(snd, 1, 64);
     }
+    v += dv * dx;
     v += __shfl_xor(v, 8, 64);
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
     if (slot == 0 && t < nr) y[fa + t] = v;
+}
+// Columns pass for groups of at most 8 parameters: the off-diagonal blocks (row group a, column
+// group gb) of a column group, lane t of a slot taking column t of the slot's block and loading its
+// rows as in the rows pass; lane t also loads x[fa+t] and the slot's lanes pass these to each other.
+// Values of the next block are loaded while the current one is multiplied.
+__global__ __launch_bounds__(256) void k_bsymv_cols8(int32_t ngroup, const int64_t *ccol_ptr, const int64_t *cc_off,
+                                                     const int32_t *cc_fn, const int32_t *gfirst, const double *a,
+                                                     const double *x, double *y)
+{
+    const int gb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, slot = lane >> 3, t = lane & 7;
+    if (gb >= ngroup) return;
+    const int fb = gfirst[gb] - 1, nc = gfirst[gb + 1] - 1 - fb;
+    const int64_t q0 = ccol_ptr[gb], q1 = ccol_ptr[gb + 1];
+    if (q1 <= q0) return;
+    double bv[8], xr = 0.0, acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) bv[r] = 0.0;
+    int64_t q = q0 + slot;
+    int2 dn = make_int2(0, 1);
+    int64_t on = 0;
+    if (q < q1) {
+        const int2 d = *(const int2 *)(cc_fn + 2 * q);
+        bsymv_ld8(bv, a + cc_off[q], d.y, nc, t);
+        const double xv = x[d.x + min(t, d.y - 1)];
+        xr = t < d.y ? xv : 0.0;
+    }
+    if (q + 8 < q1) {
+        dn = *(const int2 *)(cc_fn + 2 * (q + 8));
+        on = cc_off[q + 8];
+    }
+    for (; q < q1; q += 8) {
+        double bn[8], xn = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; r++) bn[r] = 0.0;
+        if (q + 8 < q1) {
+            bsymv_ld8(bn, a + on, dn.y, nc, t);
+            const double xv = x[dn.x + min(t, dn.y - 1)];
+            xn = t < dn.y ? xv : 0.0;
+        }
+        if (q + 16 < q1) {
+            dn = *(const int2 *)(cc_fn + 2 * (q + 16));
+            on = cc_off[q + 16];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            acc += bv[r] * __shfl(xr, r, 8);
+            bv[r] = bn[r];
+        }
+        xr = xn;
+    }
+    acc = sum_slots(acc);
+    if (slot == 0 && t < nc) y[fb + t] += acc;
 }
 // column Cc of block B (nr x nc, row-major) times x[fa .. fa+nr)
 __device__ __forceinline__ double bsymv_col(const double *B, int nr, int nc, int Cc, const double *xa)
@@ -1371,8 +1448,12 @@ void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const
     else
         hipLaunchKernelGGL(k_bsymv_rows, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff,
                            gfirst, a, x, y);
-    hipLaunchKernelGGL(k_bsymv_cols, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_off, cc_fn,
-                       gfirst, a, x, y);
+    if (maxgrp <= 8)
+        hipLaunchKernelGGL(k_bsymv_cols8, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_off,
+                           cc_fn, gfirst, a, x, y);
+    else
+        hipLaunchKernelGGL(k_bsymv_cols, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_off,
+                           cc_fn, gfirst, a, x, y);
 }
 void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
                  double *y, hipStream_t s)
