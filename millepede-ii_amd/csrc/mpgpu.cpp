@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <numeric>
 #include <omp.h>
 
@@ -48,6 +49,11 @@ struct mpgpu_ctx {
     std::vector<double> cval;
     std::vector<double> crhs;
     std::vector<std::vector<std::pair<int32_t, double>>> con_tot; // (total index, factor) per constraint
+    // constraint elimination (the MINRES methods without 'withmultipliers'): QL decomposition of
+    // the constraint matrix; the solve works on the first nfit components of Q^T x
+    bool elim = false;
+    int32_t nfit = 0;
+    std::unique_ptr<QlElim> ql;
 
     // ---- device ----
     DBuf<int64_t> d_rec_eq, d_rec_gl, d_rec_pb, d_eq_sl, d_order, d_rec_w;
@@ -646,10 +652,20 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         }
         c->cptr.push_back((int64_t)c->cidx.size());
     }
-    c->ncon = c->opt.withmultipliers ? (int32_t)c->con_tot.size() : 0;
-    if (!c->opt.withmultipliers && !c->con_tot.empty())
-        throw MpError(36, "constraints need 'withmultipliers' (elimination is not provided)");
+    // Without 'withmultipliers' the MINRES methods eliminate the constraints (pede's default,
+    // src/pede.f90:6492-6498); inversion and decomposition keep Lagrange multipliers (the same
+    // constrained solution; decomposition augments the parameter block, see mpgpu_solve)
+    const bool minres_method = c->opt.metsol == MPGPU_MINRES || c->opt.metsol == MPGPU_MINRES_QLP;
+    c->elim = !c->opt.withmultipliers && !c->con_tot.empty() && minres_method;
+    c->ncon = c->elim ? 0 : (int32_t)c->con_tot.size();
     c->nall = c->nvar + c->ncon;
+    c->ql.reset();
+    c->nfit = c->nvar;
+    if (c->elim) {
+        c->ql = std::make_unique<QlElim>();
+        c->ql->build(c->nvar, c->con_tot, c->var_index);
+        c->nfit = c->ql->nfit;
+    }
 
     // ---- matrix structure ----
     const bool sparse = c->opt.matsto == MPGPU_SPARSE;
@@ -1068,6 +1084,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->d_cidx.upload(c->cidx, s);
         c->d_cval.upload(c->cval, s);
     }
+    if (c->elim) c->ql->upload(s);
     HIPCHK(hipStreamSynchronize(s));
     c->built = true;
     if (ntot_out) *ntot_out = c->ntot;
@@ -1429,6 +1446,23 @@ void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
     }
 }
 
+// the operator MINRES sees: A itself, or with eliminated constraints the leading nfit x nfit
+// block of Q^T A Q (pede's avprod, src/pede.f90:5340-5358): x padded with zeros, Q x, A, Q^T
+struct ElimWork {
+    DBuf<double> e, f;
+};
+void op_dev(mpgpu_ctx *c, ElimWork &E, const double *x, double *y)
+{
+    if (!c->elim) { avprod_dev(c, x, y); return; }
+    hipStream_t s = c->stream;
+    const int32_t nv = c->nvar, nf = c->nfit;
+    l_ql_pad(nf, nv, x, E.e.p, s);
+    l_ql_apply(c->ql->dev, 0, E.e.p, s);
+    avprod_dev(c, E.e.p, E.f.p);
+    l_ql_apply(c->ql->dev, 1, E.f.p, s);
+    l_copy(nf, E.f.p, y, s);
+}
+
 double dot_dev(mpgpu_ctx *c, int64_t n, const double *a, const double *b)
 {
     l_dot(n, a, b, c->d_part.p, c->d_scal.p, c->stream);
@@ -1448,10 +1482,39 @@ struct Precond {
     bool spike = false;
 };
 
+// With eliminated constraints the preconditioner is built from pede's band of Q^T A Q (qlpssq,
+// src/mpqldec.f90:711-945, called in mminrs, src/pede.f90:8294,8306): P = A V (one product per
+// reflector), X = V^T P, the correction coefficients from X on the host (an m^3 recurrence),
+// Y = 2 (P + V C^T), and the band (or diagonal) of A updated entry by entry.
+void elim_band(mpgpu_ctx *c, int32_t mb, const int32_t *india, double *band)
+{
+    hipStream_t s = c->stream;
+    const QlDev &q = c->ql->dev;
+    const int32_t nv = c->nvar, m = q.m;
+    DBuf<double> Vd, P, X, Cf, Sd;
+    Vd.alloc((size_t)nv * m);
+    P.alloc((size_t)nv * m);
+    X.alloc((size_t)m * m);
+    l_ql_vdense(q, Vd.p, s);
+    for (int k = 0; k < m; k++) avprod_dev(c, Vd.p + (size_t)k * nv, P.p + (size_t)k * nv);
+    l_ql_vtx_cols(q, P.p, nv, m, X.p, m, s);
+    std::vector<double> hX((size_t)m * m), hC, hD;
+    X.download(hX.data(), hX.size(), s);
+    HIPCHK(hipStreamSynchronize(s));
+    c->ql->band_coef(hX, hC, hD);
+    std::vector<double> hS((size_t)m * m, 0.0);
+    for (int k = 0; k < m; k++) hS[(size_t)k * m + k] = hD[k];
+    Cf.upload(hC, s);
+    Sd.upload(hS, s);
+    l_ql_y(q, Cf.p, P.p, s);
+    l_ql_band(q, c->nfit, mb, india, P.p, Sd.p, band, s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 void setup_precond(mpgpu_ctx *c, Precond &P)
 {
     hipStream_t s = c->stream;
-    P.n = c->nvar;
+    P.n = c->elim ? c->nfit : c->nvar;
     P.m = c->ncon;
     DevModel M = model_of(c);
     if (c->opt.mbandw < 0) { P.kind = 0; return; }
@@ -1461,6 +1524,13 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
         DBuf<double> diag;
         diag.alloc(P.n);
         l_get_diag(M, P.n, diag.p, s);
+        if (c->elim) {
+            std::vector<int32_t> one(P.n);
+            for (int i = 0; i < P.n; i++) one[i] = i + 1;
+            DBuf<int32_t> di;
+            di.upload(one, s);
+            elim_band(c, 1, di.p, diag.p);
+        }
         P.cu.alloc(P.n);
         P.nrkd.alloc(1);
         P.sm.alloc(std::max(1, (P.m * P.m + P.m) / 2));
@@ -1479,6 +1549,7 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
     P.india.upload(india, s);
     P.band.alloc(india[P.n - 1]);
     l_get_band(M, P.n, mb, P.india.p, P.band.p, s);
+    if (c->elim) elim_band(c, mb, P.india.p, P.band.p);
     P.nrkd.alloc(2);
     l_lltdec(P.n, P.band.p, P.india.p, P.nrkd.p, c->opt.lprecm, mb, s);
     if (P.m > 0) {
@@ -1549,8 +1620,10 @@ void apply_precond(mpgpu_ctx *c, Precond &P, const double *x, double *y)
 // MINRES, restating src/minresModule.f90:294-660 with device vectors and the scalar recurrence on the device
 void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats *st)
 {
-    const int64_t n = c->nall;
+    const int64_t n = c->elim ? c->nfit : c->nall;
     hipStream_t s = c->stream;
+    ElimWork E;
+    if (c->elim) { E.e.alloc(c->nvar); E.f.alloc(c->nvar); }
     Precond P;
     setup_precond(c, P);
     const bool precon = P.kind != 0;
@@ -1568,7 +1641,7 @@ void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
     if (beta1 < 0.0) { istop = 8; goto done; }
     if (beta1 == 0.0) { istop = 0; goto done; }
     beta1 = std::sqrt(beta1);
-    avprod_dev(c, y.p, w.p);
+    op_dev(c, E, y.p, w.p);
     Arnorml = std::sqrt(dot_dev(c, n, w.p, w.p));
     {
         // the iterations run with the scalar recurrence on the device (k_mr_step): no host round
@@ -1598,7 +1671,7 @@ void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
         l_copy(n, r1.p, r2.p, s);
         auto iteration = [&](int k) {
             l_mr_scale(n, d, y.p, v.p, s);
-            avprod_dev(c, v.p, y.p);
+            op_dev(c, E, v.p, y.p);
             if (k >= 2) l_mr_axpy(n, d, 0, r1.p, y.p, s);
             l_dot(n, v.p, y.p, c->d_part.p, &d->alfa, s);
             l_mr_axpy(n, d, 1, r2.p, y.p, s);
@@ -1679,8 +1752,10 @@ static double hyp(double a, double b, double c = 0.0) { return std::sqrt(a * a +
 // Lanczos vectors live in HBM; the recurrences run on the host in FP64.
 void minresqlp_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats *st)
 {
-    const int64_t n = c->nall;
+    const int64_t n = c->elim ? c->nfit : c->nall;
     hipStream_t s = c->stream;
+    ElimWork E;
+    if (c->elim) { E.e.alloc(c->nvar); E.f.alloc(c->nvar); }
     Precond P;
     setup_precond(c, P);
     const bool precon = P.kind != 0;
@@ -1718,8 +1793,8 @@ void minresqlp_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve
         epsa = (std::fabs(sc) + eps) * std::pow(eps, 0.33333);
         if (z > epsa) istop = 10;
     }
-    avprod_dev(c, y.p, w.p);
-    avprod_dev(c, w.p, r2.p);
+    op_dev(c, E, y.p, w.p);
+    op_dev(c, E, w.p, r2.p);
     sc = dot_dev(c, n, w.p, w.p);
     t = dot_dev(c, n, y.p, r2.p);
     z = std::fabs(sc - t);
@@ -1739,7 +1814,7 @@ void minresqlp_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve
         betal = beta;
         beta = betan;
         l_scale_copy(n, one / beta, y.p, v.p, s);
-        avprod_dev(c, v.p, y.p);
+        op_dev(c, E, v.p, y.p);
         if (itn >= 2) l_axpy_neg(n, beta / betal, r1.p, y.p, s);
         double alfa = dot_dev(c, n, v.p, y.p);
         l_axpy_neg(n, alfa / beta, r2.p, y.p, s);
@@ -1850,9 +1925,9 @@ void minresqlp_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve
         if (istop != 0) {
             if (istop == 6 || istop == 7 || istop == 12 || istop == 13) lastiter = true;
             if (lastiter) { itn--; Acond = Acondl; rnorm = rnorml; relres = relresl; }
-            avprod_dev(c, x_dev, r1.p);
+            op_dev(c, E, x_dev, r1.p);
             l_sub(n, b_dev, r1.p, r1.p, s);
-            avprod_dev(c, r1.p, wl2.p);
+            op_dev(c, E, r1.p, wl2.p);
             Arnorm = nrm(wl2.p);
             break;
         }
@@ -1881,6 +1956,56 @@ double rel_residual(mpgpu_ctx *c, const double *b_dev, const double *x_dev)
     return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 
+// MINRES / MINRES-QLP with eliminated constraints (mminrs, src/pede.f90:8270-8290 and
+// 8318-8326; mminrsqlp 8370-8430): y = L^-T d from the constraint residuals d = rhs - C p (addcst,
+// src/pede.f90:730-761, in the QL order), the reduced right-hand side is the leading part of
+// Q^T (b - A Q [0; y]), and the solution is Q [x_fit; y].  rel_residual is that of the reduced
+// system.
+void elim_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats *st, bool qlp)
+{
+    hipStream_t s = c->stream;
+    const QlElim &q = *c->ql;
+    const int32_t nv = c->nvar, nf = c->nfit, m = q.ncon;
+    std::vector<double> d(m);
+    for (int k = 0; k < m; k++) {
+        const int j = q.order[k];
+        double r = c->crhs[j];
+        for (auto &p : c->con_tot[j]) r = r - p.second * c->par[p.first];
+        d[k] = r;
+    }
+    const std::vector<double> yc = q.bsub(d);
+    std::vector<double> he(nv, 0.0);
+    for (int k = 0; k < m; k++) he[nf + k] = yc[k];
+    DBuf<double> xe, bav, bred, xf;
+    xe.upload(he, s);
+    bav.alloc(nv);
+    bred.alloc(nv);
+    xf.alloc(nf);
+    l_ql_apply(q.dev, 0, xe.p, s);
+    avprod_dev(c, xe.p, bav.p);
+    l_sub(nv, b_dev, bav.p, bred.p, s);
+    l_ql_apply(q.dev, 1, bred.p, s);
+    if (qlp) minresqlp_dev(c, bred.p, xf.p, st);
+    else minres_dev(c, bred.p, xf.p, st);
+    {
+        ElimWork E;
+        E.e.alloc(nv);
+        E.f.alloc(nv);
+        DBuf<double> ax, r;
+        ax.alloc(nf);
+        r.alloc(nf);
+        op_dev(c, E, xf.p, ax.p);
+        l_sub(nf, bred.p, ax.p, r.p, s);
+        const double rr = dot_dev(c, nf, r.p, r.p), bb = dot_dev(c, nf, bred.p, bred.p);
+        st->rel_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    }
+    xe.upload(he, s);
+    l_copy(nf, xf.p, xe.p, s);
+    l_ql_apply(q.dev, 0, xe.p, s);
+    l_copy(nv, xe.p, x_dev, s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 } // namespace
 
 extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_solve_stats *st)
@@ -1897,6 +2022,8 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
     b.alloc(n);
     xd.alloc(n);
     if (rhs) b.upload(rhs, n, s); else b.upload(c->last_vec.data(), n, s);
+    if (c->elim && c->opt.metsol != MPGPU_MINRES && c->opt.metsol != MPGPU_MINRES_QLP)
+        throw MpError(36, "structure was built for constraint elimination (MINRES methods); rebuild it for this method");
     if (c->opt.metsol == MPGPU_INVERSION && c->have_inverse) {
         // matrix already inverted by an earlier solve: x = A^-1 b (dbsvxl, src/mpnum.f90:1308)
         l_packed_symv((int)n, c->d_mat.p, b.p, xd.p, s);
@@ -1934,11 +2061,17 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
         std::swap(keep.p, c->d_mat.p);
         c->have_inverse = true;
     } else if (c->opt.metsol == MPGPU_MINRES) {
-        minres_dev(c, b.p, xd.p, &loc);
-        loc.rel_residual = rel_residual(c, b.p, xd.p);
+        if (c->elim) elim_solve(c, b.p, xd.p, &loc, false);
+        else {
+            minres_dev(c, b.p, xd.p, &loc);
+            loc.rel_residual = rel_residual(c, b.p, xd.p);
+        }
     } else if (c->opt.metsol == MPGPU_MINRES_QLP) {
-        minresqlp_dev(c, b.p, xd.p, &loc);
-        loc.rel_residual = rel_residual(c, b.p, xd.p);
+        if (c->elim) elim_solve(c, b.p, xd.p, &loc, true);
+        else {
+            minresqlp_dev(c, b.p, xd.p, &loc);
+            loc.rel_residual = rel_residual(c, b.p, xd.p);
+        }
     } else if (c->opt.metsol == MPGPU_DECOMPOSITION || c->opt.metsol == MPGPU_CHOLESKY_FP32_IR) {
         if (c->opt.matsto == MPGPU_SPARSE) throw MpError(36, "method decomposition needs full matrix storage");
         const int nv = c->nvar, m = c->ncon;

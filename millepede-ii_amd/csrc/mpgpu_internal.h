@@ -156,4 +156,52 @@ void l_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const dou
 void l_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
                  const double *x, double *y, hipStream_t s);
 
+
+// ---- constraint elimination (mpgpu_qldec.cpp, k_qlelim.hip) ----
+// device view of the reflectors: rows over parameter ranges, the transpose per parameter, the WY
+// factor T and two m-vectors of scratch
+struct QlDev {
+    int32_t n = 0, m = 0;
+    const int64_t *rf_ptr = nullptr, *tr_ptr = nullptr;
+    const int32_t *rf_lo = nullptr, *rf_kn = nullptr, *tr_k = nullptr;
+    const double *rf_val = nullptr, *rf_vk = nullptr, *tr_v = nullptr, *T = nullptr;
+    double *w = nullptr, *u = nullptr;
+};
+struct QlElim {
+    int32_t nvar = 0, ncon = 0, nfit = 0, nblk = 0;
+    double evmin = 0.0, evmax = 0.0;
+    std::vector<int> order;            // input constraint of each QL column (pede's sorted order)
+    std::vector<int32_t> rcon, bcon;   // matConsRanges (4 per constraint), matConsBlocks (3 per block + end)
+    std::vector<int64_t> ioffRow;
+    std::vector<int32_t> ioffPar, irng;
+    std::vector<double> matV, vecVk, matL;
+    std::vector<int64_t> rf_ptr, tr_ptr;
+    std::vector<int32_t> rf_lo, rf_kn, tr_k;
+    std::vector<double> rf_val, rf_vk, tr_v, T;
+    std::vector<double> vtvp;          // V^T V with entries of magnitude <= 16 eps cleared, unit diagonal
+    DBuf<int64_t> d_rf_ptr, d_tr_ptr;
+    DBuf<int32_t> d_rf_lo, d_rf_kn, d_tr_k;
+    DBuf<double> d_rf_val, d_rf_vk, d_tr_v, d_T, d_w, d_u;
+    QlDev dev;
+
+    void build(int32_t nvar, const std::vector<std::vector<std::pair<int32_t, double>>> &con_tot,
+               const std::vector<int32_t> &var_index);
+    void upload(hipStream_t s);
+    std::vector<double> bsub(const std::vector<double> &d) const;
+    // coefficients of pede's band transform from X = V^T A V (column-major): Cf (row-major m x m)
+    // and the diagonal weights D
+    void band_coef(const std::vector<double> &X, std::vector<double> &Cf, std::vector<double> &D) const;
+
+  private:
+    void decompose();
+    void wy();
+};
+void l_ql_apply(const QlDev &q, int trans, double *x, hipStream_t s);   // x <- Q x (0) or Q^T x (1)
+void l_ql_pad(int32_t nfit, int32_t n, const double *x, double *e, hipStream_t s);
+void l_ql_vtx_cols(const QlDev &q, const double *x, int64_t ldx, int32_t ncols, double *w, int64_t ldw, hipStream_t s);
+void l_ql_vdense(const QlDev &q, double *Vd, hipStream_t s);
+void l_ql_y(const QlDev &q, const double *Cf, double *P, hipStream_t s);
+void l_ql_band(const QlDev &q, int32_t nfit, int32_t mb, const int32_t *india, const double *P, const double *S,
+               double *band, hipStream_t s);
+
 } // namespace mpgpu

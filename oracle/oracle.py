@@ -35,6 +35,85 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
+def ql_layout(nvar, cons):
+    """pede's constraint layout for the QL decomposition (src/pede.f90:840-1003 grpcon,
+    1077-1371 prpcon, 1423-1460 feasma) for constraints over variable parameters.
+
+    cons: list of (indices (1-based variable index), factors).  Constraints are sorted by their
+    first and last parameter (ties in input order), split into groups sharing no parameter
+    (numbered by first appearance), groups merged into blocks of overlapping ranges.  Returns
+    the block-compressed matrix a, bcon (3 x nblk+1), rcon (4 x ncon), bpar (2 x 2) as flat
+    column-major int32 arrays, and the input constraint of each QL column."""
+    m = len(cons)
+    first = [min(ix) for ix, _ in cons]
+    last = [max(ix) for ix, _ in cons]
+    srt = sorted(range(m), key=lambda j: (first[j], last[j], j))
+    parent = list(range(m))
+
+    def root(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+
+    owner = {}
+    for j in range(m):
+        for v in cons[j][0]:
+            if v in owner:
+                a, b = root(j), root(owner[v])
+                if a != b:
+                    parent[max(a, b)] = min(a, b)
+            else:
+                owner[v] = j
+    gid, grp = {}, [0] * m
+    for j in srt:
+        r = root(j)
+        if r not in gid:
+            gid[r] = len(gid)
+        grp[j] = gid[r]
+    pos = {j: q for q, j in enumerate(srt)}
+    order = sorted(range(m), key=lambda j: (grp[j], pos[j]))
+    vf = [first[j] for j in order]
+    vl = [last[j] for j in order]
+    gstart, gf, gl = [], [], []
+    for q in range(m):
+        if q == 0 or grp[order[q]] != grp[order[q - 1]]:
+            gstart.append(q)
+            gf.append(vf[q])
+            gl.append(vl[q])
+        else:
+            gf[-1] = min(gf[-1], vf[q])
+            gl[-1] = max(gl[-1], vl[q])
+    ng = len(gstart)
+    gstart.append(m)
+    rcon = np.zeros((m, 4), np.int32)
+    bcon = []
+    ilast, isblck = 0, 0
+    for g in range(ng):
+        ilast = max(ilast, gl[g])
+        nxt = gf[g + 1] if g + 1 < ng else 1 << 30
+        if nxt > ilast:
+            bcon.append((gstart[isblck] + 1, gf[isblck], ilast))
+            jfrst = gf[g]
+            for i in range(g, isblck - 1, -1):
+                jfrst = min(jfrst, gf[i])
+                for q in range(gstart[i], gstart[i + 1]):
+                    rcon[q] = (gf[i], gl[i], jfrst, ilast)
+            isblck = g + 1
+    nblk = len(bcon)
+    bcon.append((m + 1, 0, 0))
+    ioff = [0]
+    for q in range(m):
+        ioff.append(ioff[-1] + rcon[q, 3] - (rcon[q, 2] - 1))
+    a = np.zeros(max(1, ioff[-1]))
+    for q in range(m):
+        for v, f in zip(*cons[order[q]]):
+            a[ioff[q] + v - rcon[q, 2]] += f
+    bpar = np.array([0, 0, nvar, nblk], np.int32)
+    return {"ncon": m, "nblk": nblk, "a": a, "bcon": np.array(bcon, np.int32).ravel(), "rcon": rcon.ravel(),
+            "bpar": bpar, "order": order, "matsize": ioff[-1]}
+
+
 class _RecStruct(C.Structure):
     _fields_ = [("nrec", C.c_int64), ("rec_ptr", I64), ("inder", I32), ("glder", D),
                 ("ntot", C.c_int32), ("labels", I32), ("var_index", I32), ("global_par", D),
@@ -53,6 +132,7 @@ class _PassStats(C.Structure):
 
 
 OP = C.CFUNCTYPE(None, C.c_void_p, C.c_int, D, D)
+APS = C.CFUNCTYPE(None, C.c_void_p, C.c_int, D, C.c_int, C.c_int, D)
 
 
 class Oracle:
@@ -89,6 +169,53 @@ class Oracle:
         L.orc_read_mille.argtypes = [C.c_char_p, C.c_int, C.POINTER(I64), C.POINTER(I32), C.POINTER(D)]
         L.orc_read_mille.restype = C.c_int64
         L.orc_free.argtypes = [C.c_void_p]
+        L.orc_qldecb.argtypes = [C.c_int, C.c_int, C.c_int, D, I32, I32, D, I64, I32, I32, D, D, C.c_int]
+        L.orc_qlmlq.argtypes = [C.c_int, C.c_int, D, I64, I32, D, D, C.c_int, C.c_int]
+        L.orc_qlbsub.argtypes = [C.c_int, D, D, D]
+        L.orc_qlpssq.argtypes = [C.c_int, C.c_int, D, I64, I32, D, APS, C.c_void_p, D, C.c_int, C.c_int]
+
+    # ---- constraint elimination (QL) ------------------------------------------------------------
+    def qldecb(self, npar, lay, aligned=False):
+        """QL decomposition of a constraint layout from ql_layout (one parameter block); aligned
+        selects the corrected start of L (see orc_qldecb)."""
+        ncon = lay["ncon"]
+        a = np.ascontiguousarray(lay["a"], dtype=np.float64)
+        q = {"npar": npar, "ncon": ncon, "matV": np.zeros(max(1, len(a))), "ioffRow": np.zeros(ncon + 1, np.int64),
+             "ioffPar": np.zeros(ncon, np.int32), "irng": np.zeros(2 * ncon, np.int32), "vecVk": np.zeros(ncon),
+             "matL": np.zeros(ncon * ncon)}
+        self.lib.orc_qldecb(npar, ncon, lay["nblk"], _p(a, D), _p(lay["bcon"], I32), _p(lay["rcon"], I32),
+                            _p(q["matV"], D), _p(q["ioffRow"], I64), _p(q["ioffPar"], I32), _p(q["irng"], I32),
+                            _p(q["vecVk"], D), _p(q["matL"], D), 1 if aligned else 0)
+        return q
+
+    def qlmlq(self, q, x, t):
+        """Q x (t False) or Q^T x (t True); x is npar or npar x m (columns)."""
+        x = np.array(x, dtype=np.float64, order="F", copy=True)
+        m = 1 if x.ndim == 1 else x.shape[1]
+        self.lib.orc_qlmlq(q["npar"], q["ncon"], _p(q["matV"], D), _p(q["ioffRow"], I64), _p(q["irng"], I32),
+                           _p(q["vecVk"], D), _p(x, D), m, 1 if t else 0)
+        return x
+
+    def qlbsub(self, q, d):
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        y = np.zeros(q["ncon"])
+        self.lib.orc_qlbsub(q["ncon"], _p(q["matL"], D), _p(d, D), _p(y, D))
+        return y
+
+    def qlpssq(self, q, A, band, mb, t=True):
+        """pede's band of Q^T A Q (see orc_qlpssq) for a dense symmetric A."""
+        A = np.asarray(A, dtype=np.float64)
+
+        def _ap(ctx, nn, xp, i0, i1, yp):
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            ya[:] += A[:, i0 - 1:i1] @ xa[i0 - 1:i1]
+
+        fa = APS(_ap)
+        b = np.array(band, dtype=np.float64, copy=True)
+        self.lib.orc_qlpssq(q["npar"], q["ncon"], _p(q["matV"], D), _p(q["ioffRow"], I64), _p(q["irng"], I32),
+                            _p(q["vecVk"], D), fa, None, _p(b, D), mb, 1 if t else 0)
+        return b
 
     # ---- dense numerics ------------------------------------------------------------------------
     def chindl(self, n, nd):
@@ -259,10 +386,21 @@ class Reference:
     """The reference's Fortran numeric routines (src/mpnum.f90, src/Dbandmatrix.f90,
     src/minresModule.f90) built with flang; all arguments by reference."""
 
-    def __init__(self):
+    def __init__(self, fresh=False):
+        """fresh: load a private copy of the library, so that module state (the QL decomposition
+        of src/mpqldec.f90 allocates its arrays once per process) starts unallocated."""
         if not os.path.exists(REF):
             raise FileNotFoundError(REF)
-        self.lib = C.CDLL(REF)
+        path = REF
+        if fresh:
+            import shutil
+            import tempfile
+            fd, path = tempfile.mkstemp(suffix=".so", prefix="pedenum_")
+            os.close(fd)
+            shutil.copyfile(REF, path)
+        self.lib = C.CDLL(path)
+        if fresh:
+            os.unlink(path)
 
     @staticmethod
     def _i(v):
@@ -370,6 +508,51 @@ class Reference:
         self.lib.presols_(self._i(p), self._i(n), self._i(b), C.byref(C.c_int64(nm)), _p(cu, D), _p(a, D),
                           _p(np.asfortranarray(l, dtype=np.int32), I32), _p(s, D), _p(x, D), _p(y, D))
         return x
+
+    # QL decomposition of constraints (src/mpqldec.f90); module state lives in the library
+    def ql_decompose(self, npar, lay):
+        """qlini + qldecb + qlgete for one parameter block; returns (emin, emax)."""
+        self.lib.qlini_(self._i(npar), self._i(lay["ncon"]), self._i(1), C.byref(C.c_int64(lay["matsize"])),
+                        self._i(0))
+        self._ql_keep = [np.asfortranarray(lay["a"], dtype=np.float64), np.asfortranarray(lay["bpar"]),
+                         np.asfortranarray(lay["bcon"]), np.asfortranarray(lay["rcon"])]
+        a, bpar, bcon, rcon = self._ql_keep
+        self.lib.qldecb_(_p(a, D), _p(bpar, I32), _p(bcon, I32), _p(rcon, I32))
+        e1, e2 = C.c_double(), C.c_double()
+        self.lib.qlgete_(C.byref(e1), C.byref(e2))
+        self._ql_npar = npar
+        return e1.value, e2.value
+
+    def qlmlq(self, x, t):
+        x = np.array(x, dtype=np.float64, order="F", copy=True)
+        m = 1 if x.ndim == 1 else x.shape[1]
+        self.lib.qlmlq_(_p(x, D), self._i(m), self._i(1 if t else 0))
+        return x
+
+    def qlbsub(self, d, ncon):
+        d = np.asfortranarray(d, dtype=np.float64)
+        y = np.zeros(ncon)
+        self.lib.qlbsub_(_p(d, D), _p(y, D))
+        return y
+
+    def qlpssq(self, A, band, mb, t=True):
+        """band part (pede's variable-band layout, band width mb incl. the diagonal) of
+        Q^T A Q (t True) for a dense symmetric A, through pede's qlpssq with A*v products."""
+        A = np.asarray(A, dtype=np.float64)
+        n = A.shape[0]
+        FAP = C.CFUNCTYPE(None, I32, I64, D, I32, I32, D)
+
+        def _ap(np_, lp, xp, isp, iep, yp):
+            nn = np_[0]
+            xa = np.ctypeslib.as_array(xp, shape=(nn,))
+            ya = np.ctypeslib.as_array(yp, shape=(nn,))
+            i0, i1 = isp[0] - 1, iep[0]
+            ya[:] += A[:, i0:i1] @ xa[i0:i1]
+
+        fa = FAP(_ap)
+        b = np.array(band, dtype=np.float64, order="F", copy=True)
+        self.lib.qlpssq_(fa, _p(b, D), self._i(mb), self._i(1 if t else 0))
+        return b
 
     def minres(self, n, aprod, msolve, b, rtol, itnlim=2000, precon=True):
         FOP = C.CFUNCTYPE(None, I32, D, D)

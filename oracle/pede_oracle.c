@@ -1660,3 +1660,275 @@ int64_t orc_read_mille(const char *path, int fortran, int64_t **rec_ptr_out, int
 }
 
 void orc_free(void *p) { free(p); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* QL decomposition of the constraint matrix (elimination of constraints), one parameter block */
+/* covering all npar parameters (as pede uses it for MINRES, src/pede.f90:7147-7151).          */
+/* Inputs in pede's layout (1-based values, Fortran column-major shapes):                       */
+/*   a     block-compressed constraint columns (src/pede.f90:1423-1460, feasma)                 */
+/*   bcon  3 x (nblk+1): first constraint, first and last parameter of each constraint block    */
+/*   rcon  4 x ncon: group range (first, last), storage range (first, last) of each constraint  */
+/*         (src/pede.f90:1077-1371, prpcon)                                                     */
+/* Outputs: matV (reflectors, same size as a), ioffRow (ncon+1), ioffPar, irng (2 x ncon,       */
+/* non-zero range of each reflector), vecVk (element at kn), matL (ncon x ncon, column-major).  */
+/* aligned = 0 restates the reference's start of L: the stored part of column k at parameters   */
+/* kn..iplast goes to rows iclast-ncol..iclast (src/mpqldec.f90:300-305), which is the right     */
+/* row only when the constraint block's storage ends at the last parameter (iplast = npar);     */
+/* aligned = 1 puts parameter p at row p - (npar - ncon), the rows the Householder updates of L */
+/* use (line 366), so that C^T = Q [0; L] holds for every layout.                                */
+/* ------------------------------------------------------------------------------------------ */
+void orc_qldecb(int npar, int ncon, int nblk, const double *a, const int *bcon, const int *rcon, double *matV,
+                int64_t *ioffRow, int *ioffPar, int *irng, double *vecVk, double *matL, int aligned)
+{
+#define QB(i, j) bcon[3 * ((j) - 1) + (i) - 1]
+#define QR(i, j) rcon[4 * ((j) - 1) + (i) - 1]
+#define QV(i) matV[(i) - 1]
+#define QL(i) matL[(i) - 1]
+#define QN(i) vecN[(i) - 1]
+#define QIR(a_, k) irng[2 * ((k) - 1) + (a_) - 1]
+    /* src/mpqldec.f90:272-309 (one parameter block: iboff 0, iblast npar, icoff 0, iclast ncon) */
+    const int iblast = npar, icoff = 0, iclast = ncon, iboff = 0;
+    double *vecN = (double *)calloc((size_t)npar + 1, sizeof(double));
+    memset(vecVk, 0, sizeof(double) * ncon);
+    memset(matL, 0, sizeof(double) * (size_t)ncon * ncon);
+    ioffRow[0] = 0;
+    for (int ibcon = 1; ibcon <= nblk; ibcon++)
+        for (int i = QB(1, ibcon); i <= QB(1, ibcon + 1) - 1; i++) {
+            QIR(1, i) = QR(1, i);
+            QIR(2, i) = QR(2, i);
+            ioffPar[i - 1] = QR(3, i) - 1;
+            ioffRow[i] = ioffRow[i - 1] + QR(4, i) - ioffPar[i - 1];
+        }
+    memcpy(matV, a, sizeof(double) * (size_t)ioffRow[ncon]);
+    for (int k = icoff + 1; k <= iclast; k++) {
+        int kn = iblast + k - iclast;
+        int64_t ioff1 = ioffRow[k - 1];
+        int npb = (int)(ioffRow[k] - ioff1);
+        int ncol = ioffPar[k - 1] + npb - kn;
+        if (ncol >= 0) {
+            int64_t ioff3 = (int64_t)(k - 1) * ncon;
+            int row0 = aligned ? k - icoff : iclast - ncol - icoff;
+            for (int t = 0; t <= ncol; t++) QL(ioff3 + row0 + t) = QV(ioff1 + npb - ncol + t);
+        }
+    }
+    /* Householder procedure, src/mpqldec.f90:311-385 */
+    int ibcon = nblk, k1 = QB(1, ibcon);
+    for (int k = iclast; k >= icoff + 1; k--) {
+        int kn = iblast + k - iclast;
+        if (k < k1) { ibcon--; k1 = QB(1, ibcon); }
+        int ipoff = ioffPar[k - 1];
+        int ifirst = ipoff + 1;
+        if (ifirst > kn) continue;
+        int64_t ioff1 = ioffRow[k - 1];
+        int npb = (int)(ioffRow[k] - ioff1);
+        int iplast = ipoff + npb;
+        int ilast = iplast < kn ? iplast : kn;
+        int ncol = ilast - ipoff;
+        int in0 = iblast + k1 - iclast;
+        for (int t = in0; t <= kn; t++) QN(t) = 0.0;
+        for (int t = 0; t < ncol; t++) QN(ifirst + t) = QV(ioff1 + 1 + t);
+        double nrm = 0.0;
+        for (int t = ifirst; t <= ilast; t++) nrm += QN(t) * QN(t);
+        nrm = sqrt(nrm);
+        if (nrm == 0.0) continue;
+        if (QN(kn) >= 0.0) QN(kn) += nrm; else QN(kn) -= nrm;
+        double s2 = 0.0;
+        for (int t = ifirst; t <= ilast; t++) s2 += QN(t) * QN(t);
+        if (ilast < kn) {
+            nrm = sqrt(s2 + QN(kn) * QN(kn));
+            for (int t = ifirst; t <= ilast; t++) QN(t) /= nrm;
+            QN(kn) /= nrm;
+        } else {
+            nrm = sqrt(s2);
+            for (int t = ifirst; t <= ilast; t++) QN(t) /= nrm;
+        }
+        int64_t ioff3 = (int64_t)(k1 - 2) * ncon;
+        for (int i = k1; i <= k; i++) {
+            ioff3 += ncon;
+            if (QIR(1, k) > QIR(2, i)) continue;
+            int64_t ioff2 = ioffRow[i - 1] + ioffPar[k - 1] - ioffPar[i - 1];
+            double sp = 0.0;
+            for (int t = 0; t < ncol; t++) sp += QN(ifirst + t) * QV(ioff2 + 1 + t);
+            if (sp != 0.0) {
+                for (int t = 0; t < ncol; t++) QV(ioff2 + 1 + t) -= 2.0 * QN(ifirst + t) * sp;
+                int in1 = iblast + i - iclast;
+                for (int t = 0; t <= k - i; t++) QL(ioff3 + i - icoff + t) -= 2.0 * QN(in1 + t) * sp;
+                if (QIR(1, k) < QIR(1, i)) QIR(1, i) = QIR(1, k);
+                if (QIR(2, k) > QIR(2, i)) QIR(2, i) = QIR(2, k);
+            }
+        }
+        vecVk[icoff + k - 1] = QN(kn);
+        ifirst = QIR(1, k);
+        ilast = QIR(2, k) < kn - 1 ? QIR(2, k) : kn - 1;
+        ncol = ilast - ifirst + 1;
+        for (int t = 0; t < ncol; t++) QV(ioff1 + 1 + t) = QN(ifirst + t);
+        for (int t = ncol > 0 ? ncol : 0; t < npb; t++) QV(ioff1 + 1 + t) = 0.0;
+        QIR(1, k) = ifirst - iboff;
+        QIR(2, k) = ilast - iboff;
+    }
+    free(vecN);
+#undef QB
+#undef QR
+#undef QV
+#undef QL
+#undef QN
+#undef QIR
+}
+
+/* Q x (t = 0) or Q^T x (t = 1) for m columns of x (npar each), Q = H_ncon ... H_1
+   (src/mpqldec.f90:404-454, qlmlq, single block) */
+void orc_qlmlq(int npar, int ncon, const double *matV, const int64_t *ioffRow, const int *irng, const double *vecVk,
+               double *x, int m, int t)
+{
+    for (int j = 1; j <= ncon; j++) {
+        int k = t ? ncon + 1 - j : j;
+        int kn = npar + k - ncon;
+        int ifirst = irng[2 * (k - 1)], ilast = irng[2 * (k - 1) + 1];
+        const double *v = matV + ioffRow[k - 1];   /* v[p - ifirst] for p in [ifirst, ilast] */
+        double vk = vecVk[k - 1];
+        for (int c = 0; c < m; c++) {
+            double *xc = x + (int64_t)c * npar;
+            double sp = 0.0;
+            for (int p = ifirst; p <= ilast; p++) sp += v[p - ifirst] * xc[p - 1];
+            sp += vk * xc[kn - 1];
+            for (int p = ifirst; p <= ilast; p++) xc[p - 1] -= 2.0 * v[p - ifirst] * sp;
+            xc[kn - 1] -= 2.0 * vk * sp;
+        }
+    }
+}
+
+/* y from L^T y = d by backward substitution (src/mpqldec.f90:991-1014, qlbsub, single block) */
+void orc_qlbsub(int ncon, const double *matL, const double *d, double *y)
+{
+    for (int k = ncon; k >= 1; k--) {
+        double s = 0.0;
+        for (int r = k + 1; r <= ncon; r++) s += matL[(int64_t)(k - 1) * ncon + r - 1] * y[r - 1];
+        y[k - 1] = (d[k - 1] - s) / matL[(int64_t)(k - 1) * ncon + k - 1];
+    }
+}
+
+/* Band part of the similarity transform Q^T A Q for the preconditioner (src/mpqldec.f90:711-945,
+   qlpssq, single block), restated with its own arithmetic: the A*v products of the original
+   matrix (aprod adds A[:, is..ie] x[is..ie], 1-based, to y), products v^T v'' above 16 eps,
+   and the corrections to A*v carried as linear combinations of the reflectors.  B holds the
+   band (width m including the diagonal) in pede's variable-band layout and is updated in place. */
+typedef void (*orc_aprods)(void *ctx, int n, const double *x, int is, int ie, double *y);
+
+void orc_qlpssq(int npar, int ncon, const double *matV, const int64_t *ioffRow, const int *irng, const double *vecVk,
+                orc_aprods aprod, void *ctx, double *B, int m, int t)
+{
+    double *vecN = (double *)calloc((size_t)npar + 1, sizeof(double));
+    double *vecAv = (double *)calloc((size_t)npar + 1, sizeof(double));
+    double *vtvp = (double *)calloc((size_t)ncon * ncon + 1, sizeof(double));
+    double *vtAvp = (double *)calloc((size_t)ncon * ncon + 1, sizeof(double));
+    double *coef = (double *)calloc((size_t)ncon * ncon + 1, sizeof(double));
+    int *irc = (int *)calloc(2 * (size_t)ncon + 2, sizeof(int));
+    const int mbnd = m - 1 > 0 ? m - 1 : 0;
+    const double eps = DBL_EPSILON;
+#define PN(i) vecN[(i) - 1]
+#define PA(i) vecAv[(i) - 1]
+#define PV(i) matV[(i) - 1]
+#define PIR(a_, k) irng[2 * ((k) - 1) + (a_) - 1]
+#define PIC(a_, k) irc[2 * ((k) - 1) + (a_) - 1]
+#define KOF(j) (t ? ncon + 1 - (j) : (j))
+    /* initial A*v of every reflector, products, update with the initial A*v (lines 792-876) */
+    for (int j = 1; j <= ncon; j++) {
+        int k = KOF(j), kn = npar + k - ncon;
+        int64_t ioff1 = (int64_t)(k - 1) * ncon;
+        PIC(1, k) = ncon;
+        PIC(2, k) = 1;
+        int ifirst = PIR(1, k), ilast = PIR(2, k);
+        memset(vecN, 0, sizeof(double) * npar);
+        for (int p = ifirst; p <= ilast; p++) PN(p) = PV(ioffRow[k - 1] + 1 + p - ifirst);
+        PN(kn) = vecVk[k - 1];
+        memset(vecAv, 0, sizeof(double) * npar);
+        aprod(ctx, npar, vecN, ifirst, ilast, vecAv);
+        aprod(ctx, npar, vecN, kn, kn, vecAv);
+        for (int j2 = j + 1; j2 <= ncon; j2++) {
+            int k2 = KOF(j2), kn2 = npar + k2 - ncon;
+            int64_t ioff2 = (int64_t)(k2 - 1) * ncon;
+            int ifirst2 = PIR(1, k2), ilast2 = PIR(2, k2);
+            double v2kn = 0.0;
+            if (kn >= ifirst2 && kn <= ilast2) v2kn = PV(ioffRow[k2 - 1] + 1 + kn - ifirst2);
+            int l1 = ifirst > ifirst2 ? ifirst : ifirst2, l2 = ilast < ilast2 ? ilast : ilast2;
+            double s = PN(kn2) * vecVk[k2 - 1] + PN(kn) * v2kn;
+            if (l1 <= l2) {
+                double dd = 0.0;
+                for (int p = l1; p <= l2; p++) dd += PN(p) * PV(ioffRow[k2 - 1] + 1 + p - ifirst2);
+                s = s + dd;
+            }
+            if (fabs(s) > 16.0 * eps) { vtvp[ioff1 + k2 - 1] = s; vtvp[ioff2 + k - 1] = s; }
+        }
+        vtvp[ioff1 + k - 1] = 1.0;
+        for (int j2 = 1; j2 <= j; j2++) {
+            int k2 = KOF(j2), kn2 = npar + k2 - ncon;
+            int ifirst2 = PIR(1, k2), ilast2 = PIR(2, k2);
+            double dd = 0.0;
+            for (int p = ifirst2; p <= ilast2; p++) dd += PA(p) * PV(ioffRow[k2 - 1] + 1 + p - ifirst2);
+            vtAvp[ioff1 + k2 - 1] = vecVk[k2 - 1] * PA(kn2) + dd;
+        }
+        int64_t ib = 0;
+        double vtAv = vtAvp[ioff1 + k - 1];
+        for (int i = 1; i <= kn; i++)
+            for (int l = (i - mbnd > 1 ? i - mbnd : 1); l <= i; l++, ib++)
+                B[ib] = B[ib] + 2.0 * ((2.0 * PN(i) * vtAv - PA(i)) * PN(l) - PA(l) * PN(i));
+        for (int i = kn + 1; i <= npar; i++)
+            for (int l = (i - mbnd > 1 ? i - mbnd : 1); l <= i; l++, ib++) B[ib] = B[ib] - 2.0 * PA(i) * PN(l);
+    }
+    /* corrections of A*v as linear combinations of the reflectors (lines 878-935) */
+    for (int j = 1; j <= ncon; j++) {
+        int k = KOF(j), kn = npar + k - ncon;
+        int64_t ioff1 = (int64_t)(k - 1) * ncon;
+        int ifirst = PIR(1, k), ilast = PIR(2, k);
+        memset(vecN, 0, sizeof(double) * npar);
+        for (int p = ifirst; p <= ilast; p++) PN(p) = PV(ioffRow[k - 1] + 1 + p - ifirst);
+        PN(kn) = vecVk[k - 1];
+        int l1 = PIC(1, k), l2 = PIC(2, k);
+        double dd = 0.0;
+        for (int q = l1; q <= l2; q++) dd += coef[ioff1 + q - 1] * vtvp[ioff1 + q - 1];
+        double vtAv = vtAvp[ioff1 + k - 1] + dd;
+        memset(vecAv, 0, sizeof(double) * npar);
+        for (int k2 = l1; k2 <= l2; k2++) {
+            double cf = coef[ioff1 + k2 - 1];
+            if (cf == 0.0) continue;
+            int kn2 = npar + k2 - ncon, ifirst2 = PIR(1, k2), ilast2 = PIR(2, k2);
+            for (int p = ifirst2; p <= ilast2; p++) PA(p) = PA(p) + cf * PV(ioffRow[k2 - 1] + 1 + p - ifirst2);
+            PA(kn2) = PA(kn2) + cf * vecVk[k2 - 1];
+        }
+        int64_t ib = 0;
+        for (int i = 1; i <= kn; i++)
+            for (int l = (i - mbnd > 1 ? i - mbnd : 1); l <= i; l++, ib++)
+                B[ib] = B[ib] + 2.0 * ((2.0 * PN(i) * vtAv - PA(i)) * PN(l) - PA(l) * PN(i));
+        for (int i = kn + 1; i <= npar; i++)
+            for (int l = (i - mbnd > 1 ? i - mbnd : 1); l <= i; l++, ib++) B[ib] = B[ib] - 2.0 * PA(i) * PN(l);
+        for (int j2 = j + 1; j2 <= ncon; j2++) {
+            int k2 = KOF(j2);
+            int64_t ioff2 = (int64_t)(k2 - 1) * ncon;
+            double s = vtvp[ioff1 + k2 - 1];
+            int m1 = PIC(1, k2), m2 = PIC(2, k2);
+            double vta = vtAvp[ioff2 + k - 1];
+            if (m1 <= m2) {
+                double d2 = 0.0;
+                for (int q = m1; q <= m2; q++) d2 += coef[ioff2 + q - 1] * vtvp[ioff1 + q - 1];
+                vta = vta + d2;
+            }
+            if (k < m1) m1 = k;
+            if (k > m2) m2 = k;
+            coef[ioff2 + k - 1] = coef[ioff2 + k - 1] + 2.0 * (2.0 * vtAv * s - vta);
+            if (s != 0.0) {
+                if (PIC(1, k) < m1) m1 = PIC(1, k);
+                if (PIC(2, k) > m2) m2 = PIC(2, k);
+                for (int q = m1; q <= m2; q++) coef[ioff2 + q - 1] = coef[ioff2 + q - 1] - 2.0 * coef[ioff1 + q - 1] * s;
+            }
+            PIC(1, k2) = m1;
+            PIC(2, k2) = m2;
+        }
+    }
+#undef PN
+#undef PA
+#undef PV
+#undef PIR
+#undef PIC
+#undef KOF
+    free(vecN); free(vecAv); free(vtvp); free(vtAvp); free(coef); free(irc);
+}

@@ -309,6 +309,103 @@ def _oracle_band_precond(orc, A, n, m, mb):
     return msolve
 
 
+def _dense_constraints(P, cons):
+    """constraint matrix over the variable parameters and residuals d = rhs - C p (addcst,
+    src/pede.f90:730-761) from the constraint blocks and the current parameters"""
+    lab, var, _, _ = P.labels()
+    pos = {int(l): i for i, l in enumerate(lab)}
+    par = P.parameters()
+    C, d = np.zeros((len(cons), P.nvar)), np.zeros(len(cons))
+    for j, (r, _, labs, facs) in enumerate(cons):
+        d[j] = r
+        for l, f in zip(labs, facs):
+            t = pos[int(l)]
+            d[j] -= f * par[t]
+            if var[t] > 0:
+                C[j, var[t] - 1] += f
+    return C, d
+
+
+def _oracle_elimination(orc, A, b, C, d, mb, rtol, qlp=False, trancond=1.0e7):
+    """pede's MINRES with eliminated constraints (mminrs / mminrsqlp, src/pede.f90:8270-8330,
+    8370-8430) from the oracle's QL routines (pinned bitwise to src/mpqldec.f90): QL
+    decomposition in pede's constraint layout, reduced system (Q^T A Q)_ff, right-hand side
+    Q^T (b - A Q [0; y]) with L^T y = d, pede's band (or diagonal) preconditioner of Q^T A Q
+    (qlpssq) decomposed by equdecs (or precons), the oracle's MINRES / MINRES-QLP."""
+    import ctypes as CT
+    import oracle
+    n, m = A.shape[0], C.shape[0]
+    nf = n - m
+    cons = [(list(np.nonzero(C[j])[0] + 1), list(C[j][np.nonzero(C[j])[0]])) for j in range(m)]
+    lay = oracle.ql_layout(n, cons)
+    q = orc.qldecb(n, lay, aligned=True)
+    Q = orc.qlmlq(q, np.eye(n), False)
+    y = orc.qlbsub(q, d[lay["order"]])
+    e = np.zeros(n)
+    e[nf:] = y
+    bred = (Q.T @ (b - A @ (Q @ e)))[:nf]
+    M = (Q.T @ A @ Q)[:nf, :nf]
+    P = lambda a, t=CT.c_double: a.ctypes.data_as(CT.POINTER(t))
+    w = max(mb, 1)
+    band = np.concatenate([A[i, max(0, i - w + 1):i + 1] for i in range(n)])
+    bq = orc.qlpssq(q, A, band, w, True)[:sum(min(i + 1, w) for i in range(nf))].copy()
+    if mb > 0:
+        india = np.cumsum([min(i + 1, mb) for i in range(nf)]).astype(np.int32)
+        dummy = np.zeros(4, np.int32)
+        n1, n2 = CT.c_int32(), CT.c_int32()
+        orc.lib.orc_equdecs(nf, 0, 0, 0, P(bq), P(india, CT.c_int32), P(dummy, CT.c_int32), CT.byref(n1), CT.byref(n2))
+
+        def msolve(v):
+            yv = np.ascontiguousarray(v, dtype=np.float64).copy()
+            orc.lib.orc_equslvs(nf, 0, 0, P(bq), P(india, CT.c_int32), P(dummy, CT.c_int32), P(yv))
+            return yv
+    else:
+        cu = np.where(bq > 0, 1.0 / np.sqrt(np.where(bq > 0, bq, 1.0)), 0.0)
+
+        def msolve(v):
+            return cu * cu * v
+    if qlp:
+        xo, so, io, _ = orc.minresqlp(nf, lambda v: M @ v, msolve, bred, rtol, precon=True, trancond=trancond)
+    else:
+        xo, so, io, _ = orc.minres(nf, lambda v: M @ v, msolve, bred, rtol, precon=True)
+    return Q @ np.concatenate([xo, y]), so, io
+
+
+@pytest.mark.parametrize("matsto,metsol,mbandw,rhs", [
+    (mpgpu.SPARSE, mpgpu.MINRES, 6, 0.0), (mpgpu.SPARSE, mpgpu.MINRES, 0, 0.0), (mpgpu.FULL, mpgpu.MINRES, 6, 0.0),
+    (mpgpu.SPARSE, mpgpu.MINRES_QLP, 6, 0.0), (mpgpu.SPARSE, mpgpu.MINRES, 6, 2e-3)])
+def test_minres_elimination(orc, matsto, metsol, mbandw, rhs):
+    """Constraints eliminated by QL decomposition (pede's default constraint mode): the solve
+    works on nvar - ncon parameters.  Against the dense KKT solve of the same matrix (the
+    constrained least-squares solution), the constraints C x = d met, and against the oracle's
+    elimination MINRES (see _oracle_elimination): same stop code, iteration count within a few,
+    solution within 1e-4 of the parameter errors.  rhs != 0 gives non-zero constraint residuals,
+    so that the solve for the eliminated components (L^T y = d) matters."""
+    rp, ii, gg, cons, gam = _pixel_cons_problem()
+    cons = [(rhs * (j + 1), s, l, f) for j, (_, s, l, f) in enumerate(cons)]
+    P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=matsto, metsol=metsol, mreqenf=5, mcount=1,
+              mbandw=mbandw, mrestl=1e-12, withmultipliers=0)
+    assert P.ncon == 0 and P.nall == P.nvar
+    P.loopn()
+    x, st = P.solve()
+    A = packed_to_dense(P.matrix_packed(), P.nvar)
+    b, _ = P.vector()
+    C, d = _dense_constraints(P, cons)
+    m = C.shape[0]
+    K = np.block([[A, C.T], [C, np.zeros((m, m))]])
+    xr = np.linalg.solve(K, np.concatenate([b, d]))[:P.nvar]
+    sig = np.sqrt(np.abs(np.diag(np.linalg.inv(K))[:P.nvar]))
+    if metsol == mpgpu.MINRES:
+        assert st["istop"] in (1, 2)
+    assert st["rel_residual"] < 1e-9
+    assert np.abs(C @ x - d).max() < 1e-10 * max(1.0, np.abs(C).max() * np.abs(x).max())
+    assert np.max(np.abs(x - xr) / sig) < 1e-4
+    xo, so, io = _oracle_elimination(orc, A, b, C, d, mbandw, 1e-12, qlp=metsol == mpgpu.MINRES_QLP)
+    assert so == st["istop"], (so, st["istop"])
+    assert abs(io - st["itn"]) <= max(3, io // 20), (io, st["itn"])
+    assert np.max(np.abs(x - xo) / sig) < 1e-4
+
+
 def test_inversion_with_multipliers():
     rp, ii, gg, cons, gam = _pixel_cons_problem(seed=22, ntr=3000)
     P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=5, mcount=1)
