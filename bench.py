@@ -96,7 +96,8 @@ def serving_run(njobs, rate=200.0, seed=7):
         if 6 * nl * nx * ny <= 5000:
             opt = dict(matsto=mpgpu.FULL, metsol=mpgpu.CHOLESKY_FP32_IR, mreqenf=5, mcount=1)
         else:
-            opt = dict(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mbandw=6, mrestl=1e-10, mreqenf=5, mcount=1)
+            opt = dict(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mbandw=6, mrestl=1e-10, mreqenf=5, mcount=1,
+                       withmultipliers=1)
         jobs.append((rec, par, cons, opt, 6 * nl * nx * ny))
     log(f"serving: generated {njobs} jobs in {time.perf_counter() - t0:.1f}s")
     S = mpgpu.Server(nworkers=8, budget_bytes=200 << 30)
@@ -135,6 +136,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=120)
+    ap.add_argument("--elimination", action="store_true",
+                    help="eliminate the constraints by QL decomposition (pede's default) instead of the "
+                         "Lagrange multipliers of the benchmark configuration")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
@@ -159,7 +163,8 @@ def main():
     rp, ii, gg, truth = mpgpu.pixel_records(2026, nl, nx, ny, args.records, 8, 30, wx=2.0, wy=5.0, z0=10.0, dz=5.0,
                                             slope_sigma=0.1, sigma=0.002)
     log(f"generated {args.records} records ({rp[-1]} words) in {time.perf_counter() - t0:.1f}s")
-    P = mpgpu.Pede(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mbandw=6, mrestl=1e-10, mreqenf=10, mcount=1)
+    P = mpgpu.Pede(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mbandw=6, mrestl=1e-10, mreqenf=10, mcount=1,
+                   withmultipliers=0 if args.elimination else 1)
     P.add_records(rp, ii, gg)
     weak = mpgpu.pixel_weak_labels(nl, nx, ny)
     P.set_steering_parameters(weak, np.zeros(len(weak)), mpgpu.pixel_presigma(weak))
@@ -194,7 +199,8 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "large sparse detector: 20 layers x 1000 modules x 6 globals (120000 params), "
                                   f"{nrec} straight tracks, 8-30 hits, 2D FP32 measurements sigma 20um, "
-                                  "60 Lagrange constraints, sparse MINRES band 6 to 1e-10, FP64",
+                                  + ("60 constraints eliminated (QL)" if args.elimination else "60 Lagrange constraints")
+                                  + ", sparse MINRES band 6 to 1e-10, FP64",
                       "records": nrec, "ntot": ntot, "nvar": nvar, "ncon": ncon,
                       "stored_matrix_words": info["stored"], "structure_s": t_struct}}
 

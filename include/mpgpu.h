@@ -53,7 +53,9 @@ typedef struct mpgpu_options {
     int32_t mreqena;         /* 'entries f a': min entries from accepted fits (10)    */
     int32_t mcount;          /* 'countrecords': count per record (1) / equation (0)   */
     int32_t itnlim;          /* MINRES iteration limit (2000)                         */
-    int32_t withmultipliers; /* constraints by Lagrange multipliers (1) or not (0)     */
+    int32_t withmultipliers; /* constraints: Lagrange multipliers (1) or elimination by */
+                             /* QL decomposition (0, pede's default; MINRES methods --  */
+                             /* inversion and decomposition keep multipliers)           */
     int32_t mrmode;          /* MINRES-QLP mode                                       */
     float chicut;            /* 'chisqcut' first factor (0 = none)                    */
     float chirem;            /* 'chisqcut' second factor                              */

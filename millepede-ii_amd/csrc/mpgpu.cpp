@@ -133,7 +133,7 @@ extern "C" void mpgpu_default_options(mpgpu_options *o)
     o->mreqena = 10;
     o->mcount = 0;
     o->itnlim = 2000;
-    o->withmultipliers = 1;
+    o->withmultipliers = 0;   // elimination (icelim = 1, src/mpmod.f90:104)
     o->mrmode = 0;
     o->chicut = 0.0f;
     o->chirem = 0.0f;

@@ -186,7 +186,7 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
         if (key_match(k, "subito")) { S.isubit = 1; continue; }
         if (key_match(k, "nofeasibilitycheck")) { S.nofeas = 1; continue; }
         if (key_match(k, "withmultipliers")) { S.opt.withmultipliers = 1; continue; }
-        if (key_match(k, "withelimination")) { S.opt.withmultipliers = 1; continue; } // same solution, see DESIGN
+        if (key_match(k, "withelimination")) { S.opt.withmultipliers = 0; continue; }
         if (key_match(k, "countrecords")) { S.opt.mcount = 1; continue; }
         if (key_match(k, "mrestol")) {
             if (!nums.empty() && nums[0] >= 1e-10 && nums[0] <= 1e-4) S.opt.mrestl = nums[0];
@@ -537,7 +537,6 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     }
     Steering S;
     mpgpu_default_options(&S.opt);
-    S.opt.withmultipliers = 1;
     std::string sf = steering_file;
     std::string dir = sf.find('/') == std::string::npos ? "." : sf.substr(0, sf.rfind('/'));
     int ftype = 1;

@@ -187,7 +187,7 @@ def test_minres_sparse_constraints(orc, mbandw):
     matrix (without preconditioner, so that the run is fully specified)."""
     rp, ii, gg, cons, gam = _pixel_cons_problem()
     P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=5,
-              mcount=1, mbandw=mbandw, mrestl=1e-12)
+              mcount=1, mbandw=mbandw, mrestl=1e-12, withmultipliers=1)
     assert P.ncon == 12
     P.loopn()
     x, st = P.solve()
@@ -212,7 +212,7 @@ def test_minres_sparse_constraints(orc, mbandw):
         # the oracle's MINRES restatement on the same matrix.
         xo, so, io, _ = orc.minres(P.nall, lambda v: A @ v, None, b, 1e-12, itnlim=30, precon=False)
         P2 = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES,
-                   mreqenf=5, mcount=1, mbandw=-1, mrestl=1e-12, itnlim=30)
+                   mreqenf=5, mcount=1, mbandw=-1, mrestl=1e-12, itnlim=30, withmultipliers=1)
         P2.loopn()
         x2, st2 = P2.solve()
         assert (st2["istop"], st2["itn"]) == (so, io) == (5, 30)
@@ -226,7 +226,7 @@ def test_minres_repeatable(mbandw):
     synchronisation in between, gives bitwise the same solution, iteration count and stop code."""
     rp, ii, gg, cons, gam = _pixel_cons_problem(seed=23)
     P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=5,
-              mcount=1, mbandw=mbandw, mrestl=1e-12, itnlim=203)
+              mcount=1, mbandw=mbandw, mrestl=1e-12, itnlim=203, withmultipliers=1)
     P.loopn()
     runs = []
     for _ in range(3):
@@ -249,7 +249,7 @@ def test_minresqlp_sparse_constraints(orc, mbandw, mrmode):
     rp, ii, gg, cons, gam = _pixel_cons_problem()
     itnlim = 2000 if mbandw > 0 else 30      # unpreconditioned: first 30 iterations (see MINRES test)
     P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES_QLP, mreqenf=5,
-              mcount=1, mbandw=mbandw, mrestl=1e-12, mrmode=mrmode, itnlim=itnlim)
+              mcount=1, mbandw=mbandw, mrestl=1e-12, mrmode=mrmode, itnlim=itnlim, withmultipliers=1)
     P.loopn()
     x, st = P.solve()
     xr, A, b = _kkt_reference(P)
@@ -434,7 +434,8 @@ def test_cholesky(metsol):
 def test_avprod_matches_dense():
     rp, ii, gg, cons, gam = _pixel_cons_problem(seed=24, ntr=2000)
     for matsto in (mpgpu.FULL, mpgpu.SPARSE):
-        P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=matsto, metsol=4, mreqenf=5, mcount=1)
+        P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=matsto, metsol=4, mreqenf=5, mcount=1,
+                  withmultipliers=1)
         P.loopn()
         A = packed_to_dense(P.matrix_packed(), P.nall)
         x = np.random.default_rng(1).normal(size=P.nall)
