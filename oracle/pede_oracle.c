@@ -15,6 +15,11 @@
  *       -> bit-for-bit (or 1e-13) against the reference Fortran routines compiled from
  *          src/mpnum.f90, src/Dbandmatrix.f90, src/minresModule.f90 and src/minresqlpModule.f90
  *          into oracle/_ref, plus numpy closed forms.
+ *   orc_qldecb, orc_qlmlq, orc_qlbsub, orc_qlpssq (constraint elimination)
+ *       -> bit-for-bit against the reference's src/mpqldec.f90 in oracle/_ref on layouts built
+ *          by pede's grouping rules, plus closed forms (Q orthogonal, C^T = Q [0; L],
+ *          elimination = KKT solution); orc_qldecb's aligned start of L is pinned by the
+ *          closed form only (the reference's start differs, see orc_qldecb).
  *   orc_loopn (record loop)
  *       -> brute-force joint least-squares solve (globals + every record's locals) on tiny
  *          inputs (Schur identity, doc/draftManual.md:291-330), per-record explicit inverse.
