@@ -45,7 +45,7 @@ def build(jobs=8):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, HIP_SOURCES))
     lib = os.path.join(HERE, "libmpgpu.so")
-    run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-fopenmp", "-o", lib + ".tmp"] + objs)
+    run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-fopenmp", "-o", lib + ".tmp"] + objs + ["-lz"])
     os.replace(lib + ".tmp", lib)
     gen = os.path.join(HERE, "libmpgen.so")
     run(["g++", "-O3", "-fPIC", "-shared", "-std=c++17", "-fopenmp", "-o", gen + ".tmp",

@@ -11,6 +11,7 @@
 #include <memory>
 #include <numeric>
 #include <omp.h>
+#include <zlib.h>
 
 using namespace mpgpu;
 
@@ -222,12 +223,28 @@ extern "C" int mpgpu_add_records(mpgpu_ctx *c, int64_t nrec, const int64_t *rec_
     API_CATCH(c)
 }
 
+// Mille binary files (src/mille.f90:208-219; C files as written by the Mille library, which
+// pede reads plain or gzip-compressed, src/pede.f90:1689): zlib reads both, so one reader covers
+// plain and .gz files of either flavour.
 extern "C" int mpgpu_read_mille_file(mpgpu_ctx *c, const char *path, int32_t fortran, int64_t *nrec_out)
 {
     if (!c) return 30;
     API_TRY
-    FILE *f = std::fopen(path, "rb");
+    gzFile f = gzopen(path, "rb");
     if (!f) throw MpError(18, std::string("cannot open binary file ") + path);
+    struct Closer { gzFile f; ~Closer() { gzclose(f); } } closer{f};
+    gzbuffer(f, 1 << 20);
+    auto get = [&](void *dst, size_t size, size_t count) -> size_t {
+        const size_t want = size * count;
+        size_t got = 0;
+        while (got < want) {
+            const unsigned chunk = (unsigned)std::min<size_t>(want - got, 1u << 30);
+            const int r = gzread(f, (char *)dst + got, chunk);
+            if (r <= 0) break;
+            got += (size_t)r;
+        }
+        return got / size;
+    };
     std::vector<int64_t> rp{0};
     std::vector<int32_t> in;
     std::vector<float> ff;
@@ -238,10 +255,10 @@ extern "C" int mpgpu_read_mille_file(mpgpu_ctx *c, const char *path, int32_t for
     for (;;) {
         int32_t marker = 0, nr = 0;
         if (fortran) {
-            if (std::fread(&marker, 4, 1, f) != 1) break;
-            if (std::fread(&nr, 4, 1, f) != 1) { std::fclose(f); throw MpError(18, "truncated Fortran record"); }
+            if (get(&marker, 4, 1) != 1) break;
+            if (get(&nr, 4, 1) != 1) throw MpError(18, "truncated Fortran record");
         } else {
-            if (std::fread(&nr, 4, 1, f) != 1) break;
+            if (get(&nr, 4, 1) != 1) break;
         }
         bool dbl = nr < 0;
         int32_t nw = (dbl ? -nr : nr) / 2;
@@ -255,22 +272,24 @@ extern "C" int mpgpu_read_mille_file(mpgpu_ctx *c, const char *path, int32_t for
         in.resize(off + nw);
         if (dbl) {
             fd.resize(off + nw);
-            if (std::fread(fd.data() + off, 8, nw, f) != (size_t)nw) { std::fclose(f); throw MpError(18, "truncated record"); }
+            if (get(fd.data() + off, 8, nw) != (size_t)nw) throw MpError(18, "truncated record");
         } else {
             tmpf.resize(nw);
-            if (std::fread(tmpf.data(), 4, nw, f) != (size_t)nw) { std::fclose(f); throw MpError(18, "truncated record"); }
+            if (get(tmpf.data(), 4, nw) != (size_t)nw) throw MpError(18, "truncated record");
             if (any_dbl) { fd.resize(off + nw); for (int i = 0; i < nw; i++) fd[off + i] = tmpf[i]; }
             else ff.insert(ff.end(), tmpf.begin(), tmpf.end());
         }
-        if (std::fread(in.data() + off, 4, nw, f) != (size_t)nw) { std::fclose(f); throw MpError(18, "truncated record"); }
+        if (get(in.data() + off, 4, nw) != (size_t)nw) throw MpError(18, "truncated record");
         if (fortran) {
             int32_t m2 = 0;
-            if (std::fread(&m2, 4, 1, f) != 1 || m2 != marker) { std::fclose(f); throw MpError(18, "bad Fortran record marker"); }
+            if (get(&m2, 4, 1) != 1 || m2 != marker) throw MpError(18, "bad Fortran record marker");
         }
         rp.push_back((int64_t)in.size());
         nrec++;
     }
-    std::fclose(f);
+    int err = 0;
+    gzerror(f, &err);
+    if (err != Z_OK && err != Z_STREAM_END) throw MpError(18, std::string("corrupt binary file ") + path);
     int rc = mpgpu_add_records(c, nrec, rp.data(), in.data(), any_dbl ? nullptr : ff.data(), any_dbl ? fd.data() : nullptr);
     if (rc) return rc;
     if (nrec_out) *nrec_out = nrec;

@@ -31,6 +31,7 @@ struct Steering {
     std::vector<EqLine> cons, meas;
     mpgpu_options opt;
     int mitera = 1, matrit = 0, lsearch = 2, ipcntr = 0, igcorr = 0, isubit = 0, nofeas = 0;
+    std::vector<int32_t> errlabels;   // 'errlabels': MINRES error estimates (at most 100 labels)
     float dflim = 0.0f;
     bool method_given = false;
     std::string err;
@@ -183,6 +184,11 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
         if (key_match(k, "maxlocalcond")) { if (!nums.empty()) S.opt.cndlmx = (float)nums[0]; continue; }
         if (key_match(k, "printcounts")) { S.ipcntr = nums.empty() ? 1 : (int)std::llround(nums[0]); continue; }
         if (key_match(k, "globalcorr")) { S.igcorr = 1; continue; }
+        if (key_match(k, "errlabels")) {
+            for (double v : nums)
+                if (S.errlabels.size() < 100) S.errlabels.push_back((int32_t)std::llround(v));
+            continue;
+        }
         if (key_match(k, "subito")) { S.isubit = 1; continue; }
         if (key_match(k, "nofeasibilitycheck")) { S.nofeas = 1; continue; }
         if (key_match(k, "withmultipliers")) { S.opt.withmultipliers = 1; continue; }
@@ -743,6 +749,42 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     loc.ms_solves = tsol;
     std::printf(" Sum(Chi^2)/Sum(Ndf) = %.17g / %lld = %.17g\n", ps_st.chi2, (long long)(ps_st.ndf - nvar),
                 ps_st.ndf > nvar ? ps_st.chi2 / (double)(ps_st.ndf - nvar) : 0.0);
+    if ((metsol == MPGPU_MINRES || metsol == MPGPU_MINRES_QLP) && !S.errlabels.empty()) {
+        // errors and global correlations of selected parameters from MINRES (solglo/solgloqlp,
+        // src/pede.f90:537-701, called at 9288-9306): the row of the inverse from A x = e_i
+        // (at most 200 iterations), error sqrt|x_i| (negative for x_i < 0), gcor^2 = 1 - 1/(x_i A_ii);
+        // with eliminated constraints the row is that of the constrained system
+        mpgpu_options o2 = S.opt;
+        o2.itnlim = 200;
+        if ((rc = mpgpu_set_options(c, &o2))) return bail(rc);
+        std::vector<double> e(nall, 0.0), xe(nall), ye(nall);
+        bool head = false;
+        for (int32_t lab : S.errlabels) {
+            auto it = std::lower_bound(labels.begin(), labels.end(), lab);
+            if (lab == 0 || it == labels.end() || *it != lab) continue;
+            const int64_t t = it - labels.begin();
+            const int32_t v = var[t];
+            if (v <= 0) continue;
+            e[v - 1] = 1.0;
+            mpgpu_solve_stats s2{};
+            rc = mpgpu_solve(c, e.data(), xe.data(), &s2);
+            if (!rc) rc = mpgpu_avprod(c, e.data(), ye.data());
+            e[v - 1] = 0.0;
+            if (rc) return bail(rc);
+            const double gmati = xe[v - 1], diag = ye[v - 1];
+            float err = std::sqrt(std::fabs((float)gmati));
+            if (gmati < 0.0) err = -err;
+            const float gcor2 = (float)(1.0 - 1.0 / (gmati * diag));
+            if (!head) {
+                std::printf(" %s\n %s\n", "    label     parameter    presigma      differ       Error gcor^2   iit",
+                            "---------  -------------------------------------------------------  ----");
+                head = true;
+            }
+            std::printf("%10d  %s%s%s%s%7.4f%6d\n", lab, fmt_g((float)par[t], 12, 4).c_str(),
+                        fmt_g((float)presig[t], 12, 4).c_str(), fmt_g((float)(par[t] - start[t]), 12, 4).c_str(),
+                        fmt_g(err, 12, 4).c_str(), gcor2, s2.itn);
+        }
+    }
     rc = write_res(od + "/millepede.res", labels, var, par, start, presig, cnt_var, cnt_tot, have_err ? &err_v : nullptr,
                    (have_err && S.igcorr) ? &gcor_v : nullptr, S.ipcntr, S.opt.mreqena);
     mpgpu_destroy(c);

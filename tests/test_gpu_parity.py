@@ -406,6 +406,30 @@ def test_minres_elimination(orc, matsto, metsol, mbandw, rhs):
     assert np.max(np.abs(x - xo) / sig) < 1e-4
 
 
+@pytest.mark.parametrize("fortran", [False, True])
+def test_mille_file_plain_and_gzip(tmp_path, fortran):
+    """Mille binary files are read plain or gzip-compressed (pede reads C files through the
+    Mille library, which accepts both, src/pede.f90:1689): the same records written plain, as a
+    .gz file and read back through mpgpu_read_mille_file give the pass of the in-memory records."""
+    import gzip
+    rp, ii, gg, _ = mpgpu.pixel_records(31, 3, 4, 3, 1500, 3, 6)
+    plain = str(tmp_path / "r.bin")
+    mpgpu.write_mille(plain, rp, ii, gg, fortran=fortran)
+    with open(plain, "rb") as fi, gzip.open(plain + ".gz", "wb") as fo:
+        fo.write(fi.read())
+    res = []
+    for src in (None, plain, plain + ".gz"):
+        P = mpgpu.Pede(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=5, mcount=1)
+        if src is None:
+            P.add_records(rp, ii, gg)
+        else:
+            assert P.read_mille(src, fortran=fortran) == len(rp) - 1
+        P.build_structure()
+        st = P.loopn()
+        res.append((st["chi2"], st["ndf"], st["nrej"], st["naccept"], P.labels()[0].tolist()))
+    assert res[0] == res[1] == res[2]
+
+
 def test_inversion_with_multipliers():
     rp, ii, gg, cons, gam = _pixel_cons_problem(seed=22, ntr=3000)
     P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=5, mcount=1)

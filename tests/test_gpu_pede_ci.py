@@ -71,3 +71,41 @@ def test_ci_wirechamber_res():
     assert not bad, bad[:10]
     # final chi^2 of the CI log (CI/refFiles/millepede.log:153-155)
     assert abs(st["final_chi2"] - 8130954.4875403922) < 1e-6 * 8130954.4875403922
+
+
+@pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
+def test_errlabels_match_inversion(capfd):
+    """'errlabels' (solglo, src/pede.f90:537-612): for selected labels the MINRES run prints the
+    error sqrt|(A^-1)_ii| from one solve of A x = e_i.  On the CI wire chamber (constraints with
+    multipliers, as the inversion uses) the printed errors (4 significant digits) agree with the
+    error column that 'method inversion' writes to millepede.res."""
+    labs = [12, 14, 100, 210, 501, 550]
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([GEN, "20000"], cwd=d, check=True, capture_output=True)
+        # without down-weighting the matrix does not depend on the parameters, which the two
+        # methods reach by different routes
+        base = STEER.replace("outlierdownweighting 3\ndwfractioncut 0.1\n", "")
+        with open(os.path.join(d, "inv.txt"), "w") as f:
+            f.write(base)
+        st = mpgpu.run_steering(os.path.join(d, "inv.txt"), d)
+        assert st["exit_code"] == 0
+        inv = parse_res(os.path.join(d, "millepede.res"))
+        capfd.readouterr()
+        steer = base.replace("method inversion 3 0.8", "method sparseMINRES 3 0.8\nmrestol 1.0D-10\nwithmultipliers\n"
+                             "bandwidth 6\nerrlabels " + " ".join(map(str, labs)))
+        with open(os.path.join(d, "mr.txt"), "w") as f:
+            f.write(steer)
+        st = mpgpu.run_steering(os.path.join(d, "mr.txt"), d)
+        assert st["exit_code"] == 0
+        out = capfd.readouterr().out.splitlines()
+    i = next(k for k, l in enumerate(out) if "label     parameter    presigma      differ       Error gcor^2   iit" in l)
+    rows = {int(l.split()[0]): l for l in out[i + 2:i + 2 + len(labs)]}
+    seen = 0
+    for lab in labs:
+        if lab not in inv or len(inv[lab][0]) < 4:
+            continue                 # fixed or unknown label: no line
+        line = rows[lab]
+        err = float(line[12 + 36:12 + 48])
+        assert abs(err - inv[lab][0][3]) <= 2e-3 * abs(inv[lab][0][3]), (lab, err, inv[lab][0][3])
+        seen += 1
+    assert seen >= 3
