@@ -419,6 +419,15 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     const float *F = c->dbl ? nullptr : c->vf.data();
     const double *D = c->dbl ? c->vd.data() : nullptr;
     auto val = [&](int64_t w) { return D ? D[w] : (double)F[w]; };
+    // MPGPU_VERBOSE: host time of each phase on stderr
+    const bool verbose = std::getenv("MPGPU_VERBOSE") != nullptr;
+    double t_phase = now_ms();
+    auto phase = [&](const char *name) {
+        if (!verbose) return;
+        const double t = now_ms();
+        std::fprintf(stderr, "build_structure: %-28s %9.1f ms\n", name, t - t_phase);
+        t_phase = t;
+    };
 
     // ---- decode all records (equation tables) ----
     std::vector<int64_t> rec_eq(nrec + 1, 0);
@@ -459,6 +468,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         }
     }
 
+    phase("decode");
     // ---- label table: all labels of global words, steering parameters and equations ----
     int32_t lmin = INT32_MAX, lmax = 0;
 #pragma omp parallel for reduction(min : lmin) reduction(max : lmax) schedule(static)
@@ -515,6 +525,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         return rank_base[o >> 6] + __builtin_popcountll(b & ((1ull << (o & 63)) - 1ull));
     };
 
+    phase("label table");
     // ---- per record: sorted distinct labels (all global words) -> gl lists, widx ----
     std::vector<int64_t> rec_gl(nrec + 1, 0);
     std::vector<int32_t> ngl_rec(nrec, 0);
@@ -588,6 +599,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->counters.assign(ntot, 0);
     for (int32_t t = 0; t < ntot; t++) c->counters[t] = cnt[t].load();
 
+    phase("record label lists");
     // ---- parameters: start values, pre-sigmas, variable / fixed (loop1, src/pede.f90:5920-6062) ----
     c->par.assign(ntot, 0.0);
     c->presig_tot.assign(ntot, 0.0);
@@ -642,6 +654,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->pre_weight[v - 1] = (double)c->opt.regula * w;
     }
 
+    phase("parameters and groups");
     // ---- constraints: Lagrange rows over variable indices (addcst, src/pede.f90:704-771) ----
     c->cptr.assign(1, 0);
     c->cidx.clear();
@@ -686,6 +699,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->nfit = c->ql->nfit;
     }
 
+    phase("constraints");
     // ---- matrix structure ----
     const bool sparse = c->opt.matsto == MPGPU_SPARSE;
     int32_t max_gl = 1, max_eq = 1, max_nalc = 1, max_var = 1, max_seg = 1;
@@ -902,6 +916,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->band_grid = (int)std::min<int64_t>(c->nband, 256 * per_cu);
     }
 
+    phase("matrix structure");
     // ---- processing order: records sorted by the first parameter group they touch, so that the
     // records in flight at the same time update a compact set of matrix blocks (cache reuse of
     // the FP64 atomics); results do not depend on the order beyond FP64 summation order
@@ -917,6 +932,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
     }
+    phase("processing order");
     // ---- two-phase build index (group-block storage) ----
     // Each record writes its update of every segment pair (r-major block, lower triangle for the
     // diagonal pairs) contiguously into a buffer; the block gather sums, for every stored block,
@@ -947,8 +963,17 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
             }
             cnt_r[r] = n;
         }
+        // update-buffer cap: MPGPU_UPDATE_BUFFER_GB, else 45 % of the free device memory up to
+        // 128 GB (a whole pass of the large workloads in one batch: the matrix is read and
+        // written once per pass, one record-kernel launch)
         const char *env = std::getenv("MPGPU_UPDATE_BUFFER_GB");
-        const double gb = env ? std::atof(env) : 64.0;
+        double gb = 64.0;
+        if (env) gb = std::atof(env);
+        else {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > 0) gb = std::min(128.0, 0.45 * (double)fr / (1ull << 30));
+            else (void)hipGetLastError();
+        }
         const int64_t budget = std::max<int64_t>(1 << 20, (int64_t)(gb * (1ull << 30) / 8.0));
         int64_t q0 = 0, acc = 0;
         for (int64_t q = 0; q < nrec; q++) {
@@ -1016,6 +1041,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
             inc_total += ptr[nb];
         }
     }
+    phase("two-phase index");
     // ---- device upload ----
     hipStream_t s = c->stream;
     c->d_order.upload(order, s);
@@ -1105,6 +1131,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     }
     if (c->elim) c->ql->upload(s);
     HIPCHK(hipStreamSynchronize(s));
+    phase("device upload");
     c->built = true;
     if (ntot_out) *ntot_out = c->ntot;
     if (nvar_out) *nvar_out = c->nvar;
