@@ -68,7 +68,10 @@ typedef struct mpgpu_options {
     double mrtcnd;           /* 'mrestranscond' (1e7)                                 */
     int32_t nregul;          /* regularisation flag                                   */
     int32_t refine_steps;    /* FP32+IR: maximum FP64 CG refinement steps (20)        */
-    int32_t reserved[6];
+    int32_t monitor;         /* residual monitoring of the passes: 0 off, 1 normalized */
+                             /* residuals ('monitorresiduals'), 2 pulls ('monitorpulls') */
+    int32_t monitor_bins;    /* bins per measurement group (100)                       */
+    int32_t reserved[4];
 } mpgpu_options;
 
 /* Statistics of one data pass (loopn) */
@@ -162,6 +165,14 @@ int mpgpu_avprod(mpgpu_ctx *ctx, const double *x, double *y);
 /* parameter errors sqrt|A^-1_ii| and global correlations (prtglo, src/pede.f90:4538-4546);
  * only after an inversion solve.  err, gcor: nvar (gcor may be NULL) */
 int mpgpu_get_errors(const mpgpu_ctx *ctx, double *err, double *gcor);
+/* residual monitoring (src/pede.f90:3955-3980 in loopbf, 6700-6710 and 7075-7085 in loop2):
+ * histograms of the last data pass, one per measurement group -- the measurements with local
+ * and global derivatives, grouped by the label of their first global derivative, in label order
+ * -- of the normalized residuals (or pulls) of the first local-fit iteration, monitor_bins bins
+ * of width 0.1 centred on zero; mean_err is the group's mean measurement error (summed in
+ * record order, rounded to REAL(4) as pede prints it).  With labels == NULL only *ngroups is set.
+ * Requires options.monitor. */
+int mpgpu_get_monitor(const mpgpu_ctx *ctx, int32_t *ngroups, int32_t *labels, float *mean_err, int64_t *hist);
 
 /* ---- complete pede-like run: steering file -> millepede.res (src/pede.f90:33-528) -------- */
 typedef struct mpgpu_run_stats {

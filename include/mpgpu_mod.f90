@@ -14,7 +14,8 @@ MODULE mpgpu_mod
         REAL(c_float)      :: chicut, chirem, chhuge, dwcut, cndlmx, regula
         REAL(c_double)     :: default_presigma, mrestl, mrtcnd
         INTEGER(c_int32_t) :: nregul, refine_steps
-        INTEGER(c_int32_t) :: reserved(6)
+        INTEGER(c_int32_t) :: monitor, monitor_bins
+        INTEGER(c_int32_t) :: reserved(4)
     END TYPE mpgpu_options
 
     TYPE, BIND(C) :: mpgpu_pass_stats
@@ -178,6 +179,14 @@ MODULE mpgpu_mod
             IMPORT :: c_int, c_ptr, c_double
             TYPE(c_ptr), VALUE :: ctx
             REAL(c_double), INTENT(OUT) :: err(*), gcor(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_get_monitor(ctx, ngroups, labels, mean_err, hist) BIND(C, name='mpgpu_get_monitor')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_int64_t, c_float
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int32_t), INTENT(OUT) :: ngroups
+            INTEGER(c_int32_t), INTENT(OUT) :: labels(*)
+            REAL(c_float), INTENT(OUT) :: mean_err(*)
+            INTEGER(c_int64_t), INTENT(OUT) :: hist(*)
         END FUNCTION
         INTEGER(c_int) FUNCTION mpgpu_run_steering(steer, outdir, device, st) BIND(C, name='mpgpu_run_steering')
             IMPORT :: c_int, c_char, c_int32_t, mpgpu_run_stats

@@ -304,9 +304,9 @@ __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
     return L;
 }
 
-template <typename T>
+template <typename T, bool MON>
 __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_records(DevRecords R, DevModel M, PassOpts o, RecOut out, LdsPlan P,
-                                                int64_t rec_begin, int64_t rec_end)
+                                                int64_t rec_begin, int64_t rec_end, MonCtx mon)
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
@@ -645,6 +645,38 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     for (int j = 1; j <= nl; j++) rmloc = rmloc + lwv(sl + j) * bl[li[sl + j] - 1];
                     lc[i] = rmloc;
                     double rmeas = rm[i] - rmloc;
+                    if constexpr (MON) {
+                        // residual monitoring, first local-fit iteration (3938-3980): variance of
+                        // the fitted value g^T C g from the local inverse (element (j, k) addressed
+                        // as pede does, by the order of the local words), pull, and the bin of the
+                        // normalized residual pull * err / <err> (or of the pull) in the group of the
+                        // measurement's first global parameter; bin width 0.1 as REAL(4) widened
+                        if (iter == 1 && (len[i] >> 16) > 0) {
+                            double dvar = 0.0;
+                            for (int j = 1; j <= nl; j++) {
+                                const int ij = li[sl + j];
+                                const int jk = (ij * ij - ij) / 2;
+                                const double dj = lwv(sl + j);
+                                for (int k = 1; k < j; k++)
+                                    dvar = dvar + cl[jk + li[sl + k] - 1] * dj * lwv(sl + k) * 2.0;
+                                dvar = dvar + cl[jk + ij - 1] * dj * dj;
+                            }
+                            if (0.999999 / wght > dvar) {
+                                const double pull = rmeas / sqrt(1.0 / wght - dvar);
+                                const int t = R.gl_tix[g0 + li[sl + nl + 2]];
+                                const int grp = mon.grp[t];
+                                if (grp >= 0) {
+                                    const double bsz = (double)0.1f;
+                                    const double xv = (mon.mode == 1 ? pull * rerr / mon.res[grp] / bsz : pull / bsz) +
+                                                      0.5 * (double)mon.bins;
+                                    if (xv == xv) {
+                                        const int bin = xv >= (double)mon.bins ? mon.bins : (xv < 1.0 ? 1 : (int)xv);
+                                        atomicAdd(&mon.hist[(int64_t)grp * mon.bins + bin - 1], 1u);
+                                    }
+                                }
+                            }
+                        }
+                    }
                     double dchi2 = wght * rmeas * rmeas;
                     double resid = rmeas;
                     if (o.nloopn != 1 && iter != 1 && o.lhuber != 0) {
@@ -1075,19 +1107,18 @@ __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int6
 }
 
 // host launcher
-hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out,
-                          const LdsPlan &P, int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s)
+template <typename T, bool MON>
+static hipError_t launch_records_t(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out,
+                                   const LdsPlan &P, int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s,
+                                   const MonCtx &mon)
 {
-    if (rec_end <= rec_begin) return hipSuccess;
     if (P.bytes > 65536) {
-        hipError_t e = R.vf ? hipFuncSetAttribute((const void *)k_records<float>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes)
-                            : hipFuncSetAttribute((const void *)k_records<double>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes);
+        hipError_t e = hipFuncSetAttribute((const void *)k_records<T, MON>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes);
         if (e != hipSuccess) return e;
     }
     // as many workgroups as fit on the device at once; each loops over the records
     int per_cu = 0, dev = 0, ncu = 0;
-    hipError_t e = R.vf ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_records<float>, RT, P.bytes)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_records<double>, RT, P.bytes);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_records<T, MON>, RT, P.bytes);
     if (e != hipSuccess) return e;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
@@ -1095,11 +1126,20 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
     if (std::getenv("MPGPU_VERBOSE"))
         std::fprintf(stderr, "k_records: %d threads x %d workgroups, LDS %d bytes, %d workgroups per CU\n", RT, grid,
                      P.bytes, per_cu);
-    if (R.vf)
-        hipLaunchKernelGGL(k_records<float>, dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
-    else
-        hipLaunchKernelGGL(k_records<double>, dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end);
+    hipLaunchKernelGGL((k_records<T, MON>), dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end, mon);
     return hipGetLastError();
+}
+
+hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out,
+                          const LdsPlan &P, int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s,
+                          const MonCtx &mon)
+{
+    if (rec_end <= rec_begin) return hipSuccess;
+    const bool m = mon.mode != 0 && mon.hist != nullptr;
+    if (R.vf) return m ? launch_records_t<float, true>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon)
+                       : launch_records_t<float, false>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon);
+    return m ? launch_records_t<double, true>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon)
+             : launch_records_t<double, false>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon);
 }
 
 hipError_t launch_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg, const int32_t *bcol,

@@ -55,6 +55,14 @@ struct mpgpu_ctx {
     bool elim = false;
     int32_t nfit = 0;
     std::unique_ptr<QlElim> ql;
+    // residual monitoring: measurement groups (total index of the first global parameter), their
+    // mean errors and the histograms of the last pass
+    std::vector<int32_t> mon_tot;
+    std::vector<double> mon_res;
+    int32_t mon_bins = 0;
+    DBuf<int32_t> d_mon_grp;
+    DBuf<double> d_mon_res;
+    DBuf<unsigned int> d_mon_hist;
 
     // ---- device ----
     DBuf<int64_t> d_rec_eq, d_rec_gl, d_rec_pb, d_eq_sl, d_order, d_rec_w;
@@ -135,6 +143,8 @@ extern "C" void mpgpu_default_options(mpgpu_options *o)
     o->mcount = 0;
     o->itnlim = 2000;
     o->withmultipliers = 0;   // elimination (icelim = 1, src/mpmod.f90:104)
+    o->monitor = 0;
+    o->monitor_bins = 100;
     o->mrmode = 0;
     o->chicut = 0.0f;
     o->chirem = 0.0f;
@@ -599,6 +609,33 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->counters.assign(ntot, 0);
     for (int32_t t = 0; t < ntot; t++) c->counters[t] = cnt[t].load();
 
+    // residual monitoring: every measurement with local and global derivatives belongs to the
+    // group of its first global parameter; the group's mean error is summed in record order
+    // (loop2, src/pede.f90:6700-6710 and 7075-7082)
+    c->mon_tot.clear();
+    c->mon_res.clear();
+    std::vector<int32_t> mon_grp;
+    if (c->opt.monitor != 0) {
+        std::vector<int64_t> mcnt(ntot, 0);
+        std::vector<double> msum(ntot, 0.0);
+        for (int64_t r = 0; r < nrec; r++)
+            for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
+                const int nl = eq_n[e] & 0xffff, ng = eq_n[e] >> 16;
+                if (ng <= 0) continue;
+                const int64_t sg = eq_sl[e] + nl + 1;
+                const int32_t t = tix_of(I[sg + 1]);
+                mcnt[t]++;
+                msum[t] = msum[t] + val(sg);
+            }
+        mon_grp.assign(ntot, -1);
+        for (int32_t t = 0; t < ntot; t++)
+            if (mcnt[t] > 0) {
+                mon_grp[t] = (int32_t)c->mon_tot.size();
+                c->mon_tot.push_back(t);
+                c->mon_res.push_back(msum[t] / (double)mcnt[t]);
+            }
+        c->mon_bins = std::max(1, c->opt.monitor_bins);
+    }
     phase("record label lists");
     // ---- parameters: start values, pre-sigmas, variable / fixed (loop1, src/pede.f90:5920-6062) ----
     c->par.assign(ntot, 0.0);
@@ -1130,6 +1167,16 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->d_cval.upload(c->cval, s);
     }
     if (c->elim) c->ql->upload(s);
+    if (c->opt.monitor != 0 && !c->mon_tot.empty()) {
+        c->d_mon_grp.upload(mon_grp, s);
+        c->d_mon_res.upload(c->mon_res, s);
+        c->d_mon_hist.alloc((size_t)c->mon_tot.size() * c->mon_bins);
+        c->d_mon_hist.zero(s);
+    } else {
+        c->d_mon_grp.release();
+        c->d_mon_res.release();
+        c->d_mon_hist.release();
+    }
     HIPCHK(hipStreamSynchronize(s));
     phase("device upload");
     c->built = true;
@@ -1273,6 +1320,11 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     RecOut out{c->d_chi2.p, c->d_ndf.p, c->d_status.p};
     DevModel M = model_of(c);
     DevRecords R = records_of(c);
+    MonCtx mon{};
+    if (c->opt.monitor != 0 && c->d_mon_hist.p) {
+        mon = MonCtx{c->d_mon_grp.p, c->d_mon_res.p, c->d_mon_hist.p, c->mon_bins, c->opt.monitor};
+        c->d_mon_hist.zero(s);   // histograms of this pass only (src/pede.f90:2607)
+    }
     int grid = (int)std::min<int64_t>(c->nrec, 256 * 40);
     HIPCHK(hipEventRecord(e1, s));
     if (c->nrec > 0) {
@@ -1284,7 +1336,7 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
                 const int g = (int)std::min<int64_t>(B.q1 - B.q0, 256 * 40);
                 auto tr = timed(t_rec);
                 HIPCHK(hipEventRecord(tr.first, s));
-                HIPCHK(launch_records(R, M, o, out, c->plan, B.q0, B.q1, std::max(g, 1), s));
+                HIPCHK(launch_records(R, M, o, out, c->plan, B.q0, B.q1, std::max(g, 1), s, mon));
                 HIPCHK(hipEventRecord(tr.second, s));
                 auto tg = timed(t_gat);
                 HIPCHK(hipEventRecord(tg.first, s));
@@ -1296,7 +1348,7 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
         } else {
             auto tr = timed(t_rec);
             HIPCHK(hipEventRecord(tr.first, s));
-            HIPCHK(launch_records(R, M, o, out, c->plan, 0, c->nrec, std::max(grid, 1), s));
+            HIPCHK(launch_records(R, M, o, out, c->plan, 0, c->nrec, std::max(grid, 1), s, mon));
             HIPCHK(hipEventRecord(tr.second, s));
         }
     }
@@ -2350,4 +2402,26 @@ extern "C" int mpgpu_get_errors(const mpgpu_ctx *c0, double *err, double *gcor)
     }
     return 0;
     API_CATCH(c)
+}
+
+extern "C" int mpgpu_get_monitor(const mpgpu_ctx *c, int32_t *ngroups, int32_t *labels, float *mean_err, int64_t *hist)
+{
+    if (!c || !c->built) return 30;
+    if (c->opt.monitor == 0 || !c->d_mon_hist.p) return fail(const_cast<mpgpu_ctx *>(c), 24, "residual monitoring is off");
+    const int32_t ng = (int32_t)c->mon_tot.size();
+    if (ngroups) *ngroups = ng;
+    if (!labels) return 0;
+    API_TRY
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<unsigned int> h((size_t)ng * c->mon_bins);
+    c->d_mon_hist.download(h.data(), h.size(), c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int32_t g = 0; g < ng; g++) {
+        labels[g] = c->labels[c->mon_tot[g]];
+        if (mean_err) mean_err[g] = (float)c->mon_res[g];
+    }
+    if (hist)
+        for (size_t k = 0; k < h.size(); k++) hist[k] = (int64_t)h[k];
+    return 0;
+    API_CATCH(const_cast<mpgpu_ctx *>(c))
 }

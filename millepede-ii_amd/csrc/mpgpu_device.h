@@ -54,6 +54,17 @@ struct PassOpts {
     float chicut, chhuge, dwcut, cndlmx;
 };
 
+// residual monitoring (pede's measHists): per total index the measurement group of a
+// measurement whose first global derivative it is (-1: none), the group's mean error, and
+// bins counts per group
+struct MonCtx {
+    const int32_t *grp;
+    const double *res;
+    unsigned int *hist;
+    int32_t bins;
+    int32_t mode;   // 0 off, 1 normalized residuals, 2 pulls
+};
+
 // per-record results of one pass
 struct RecOut {
     double *chi2;    // contribution to Sum(chi^2)

@@ -87,7 +87,7 @@ struct EqBlock {
 
 // kernels (k_records.hip, k_solve.hip, k_chol.hip)
 hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out, const LdsPlan &P,
-                          int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s);
+                          int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s, const MonCtx &mon);
 int lds_bytes(LdsPlan &P);
 hipError_t launch_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg, const int32_t *bcol,
                                const int32_t *gfirst, const int64_t *ptr, const int64_t *src, const double *cbuf,

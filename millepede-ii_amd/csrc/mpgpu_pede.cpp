@@ -15,6 +15,7 @@
 #include <fstream>
 #include <functional>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -32,6 +33,7 @@ struct Steering {
     mpgpu_options opt;
     int mitera = 1, matrit = 0, lsearch = 2, ipcntr = 0, igcorr = 0, isubit = 0, nofeas = 0;
     std::vector<int32_t> errlabels;   // 'errlabels': MINRES error estimates (at most 100 labels)
+    int imonit = 0;                    // 'monitorresiduals'/'monitorpulls': bit 0 first, bit 1 last pass, < 0 all
     float dflim = 0.0f;
     bool method_given = false;
     std::string err;
@@ -200,6 +202,13 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
         }
         if (key_match(k, "mrestranscond")) { if (!nums.empty()) S.opt.mrtcnd = nums[0]; continue; }
         if (key_match(k, "mresmode")) { if (!nums.empty()) S.opt.mrmode = (int)std::llround(nums[0]); continue; }
+        if (key_match(k, "monitorresiduals") || key_match(k, "monitorpulls")) {
+            // src/pede.f90:10668-10684
+            S.imonit = nums.empty() ? 3 : (int)std::llround(nums[0]);
+            if (nums.size() > 1) S.opt.monitor_bins = std::max(S.opt.monitor_bins, (int)std::llround(nums[1]));
+            S.opt.monitor = S.imonit == 0 ? 0 : (key_match(k, "monitorpulls") ? 2 : 1);
+            continue;
+        }
         if (key_match(k, "fortranfiles")) { cur_ftype = 3; continue; }
         if (key_match(k, "cfiles")) { cur_ftype = 1; continue; }
         // file names (steering file only)
@@ -463,6 +472,42 @@ struct Feasibility {
     }
 };
 
+// millepede.mon rows of one pass (monres, src/pede.f90:7533-7633): per measurement group the
+// entries, the median from the cumulative histogram (interpolated in the bin), the median of the
+// absolute deviations from it times 1.4826, and the mean error; REAL(4) arithmetic as pede
+void write_monres(FILE *f, int nloopn, const std::vector<int32_t> &lab, const std::vector<float> &res,
+                  const std::vector<int64_t> &hist, int bins)
+{
+    const float bsz = 0.1f;
+    for (size_t g = 0; g < lab.size(); g++) {
+        const int64_t *h = hist.data() + g * bins;
+        std::vector<int64_t> isuml(bins + 1, 0), isums(bins + 1, 0);   // 1-based
+        isuml[1] = h[0];
+        for (int j = 2; j <= bins; j++) isuml[j] = isuml[j - 1] + h[j - 1];
+        const int64_t nent = isuml[bins];
+        float amed = 0.0f, amad = 0.0f;
+        if (nent > 0) {
+            int j = 2;
+            for (; j <= bins; j++) if (2 * isuml[j] > nent) break;
+            const int imed = j;
+            amed = (float)j;
+            if (isuml[j] > isuml[j - 1])
+                amed = amed + (float)(nent - 2 * isuml[j - 1]) / (float)(2 * isuml[j] - 2 * isuml[j - 1]);
+            amed = bsz * (amed - (float)(bins / 2));
+            for (int q = imed; q <= bins; q++) isums[q - imed + 1] += h[q - 1];
+            for (int q = imed - 1; q >= 1; q--) isums[imed - q] += h[q - 1];
+            for (int q = 2; q <= bins; q++) isums[q] += isums[q - 1];
+            for (j = 2; j <= bins; j++) if (2 * isums[j] > nent) break;
+            amad = (float)(j - 1);
+            if (isums[j] > isums[j - 1])
+                amad = amad + (float)(nent - 2 * isums[j - 1]) / (float)(2 * isums[j] - 2 * isums[j - 1]);
+            amad = bsz * amad;
+        }
+        std::fprintf(f, "%5d%10d%10lld%s%s%s\n", nloopn, lab[g], (long long)nent, fmt_g(amed).c_str(),
+                     fmt_g(amad * 1.4826f).c_str(), fmt_g(res[g]).c_str());
+    }
+}
+
 int write_res(const std::string &path, const std::vector<int32_t> &labels, const std::vector<int32_t> &var,
               const std::vector<double> &par, const std::vector<double> &start, const std::vector<double> &presig,
               const std::vector<int64_t> &cnt_var, const std::vector<int64_t> &cnt_tot, const std::vector<double> *err,
@@ -614,6 +659,26 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     std::vector<double> b(nall), x(nall), xls(nvar), corr(nvar);
     std::vector<int64_t> cnt_var(nvar);
     double tp = 0, tsol = 0;
+    std::unique_ptr<FILE, int (*)(FILE *)> fmon_keep(nullptr, std::fclose);   // closed on every exit
+    FILE *fmon = nullptr;
+    if (S.imonit != 0) {
+        fmon = std::fopen((od + "/millepede.mon").c_str(), "w");
+        fmon_keep.reset(fmon);
+        if (fmon)
+            std::fprintf(fmon, "%s\n! LFC     Label   Entries        Median  RMS(MAD)          <error>\n",
+                         S.opt.monitor == 2 ? "*** Pulls grouped by first global label (per local fit cycle) ***"
+                                            : "*** Normalized residuals grouped by first global label (per local fit cycle) ***");
+    }
+    auto monres = [&]() {
+        if (!fmon) return;
+        int32_t ng = 0;
+        if (mpgpu_get_monitor(c, &ng, nullptr, nullptr, nullptr) || ng <= 0) return;
+        std::vector<int32_t> ml(ng);
+        std::vector<float> mr(ng);
+        std::vector<int64_t> mh((size_t)ng * S.opt.monitor_bins);
+        if (mpgpu_get_monitor(c, &ng, ml.data(), mr.data(), mh.data())) return;
+        write_monres(fmon, nloopn, ml, mr, mh, S.opt.monitor_bins);
+    };
     auto pass = [&](int icalcm) -> int {
         nloopn++;
         // cut factor of a new iteration (loopn, src/pede.f90:2580-2595)
@@ -635,6 +700,7 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
         nrej_last = nrej;
         if (3 * nrej > ps_st.nrec) { std::fprintf(stderr, "Too many rejects (>33.3%%) - stop\n"); return 26; }
         mpgpu_get_vector(c, b.data(), cnt_var.data());
+        if (S.imonit < 0 || (nloopn == 1 && (S.imonit & 1))) monres();   // src/pede.f90:2879
         return 0;
     };
     double fvalue = 0.0;
@@ -785,6 +851,9 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
                         fmt_g(err, 12, 4).c_str(), gcor2, s2.itn);
         }
     }
+    if (S.imonit > 0 && (S.imonit & 2)) monres();                      // src/pede.f90:9011
+    fmon_keep.reset();
+    fmon = nullptr;
     rc = write_res(od + "/millepede.res", labels, var, par, start, presig, cnt_var, cnt_tot, have_err ? &err_v : nullptr,
                    (have_err && S.igcorr) ? &gcor_v : nullptr, S.ipcntr, S.opt.mreqena);
     mpgpu_destroy(c);

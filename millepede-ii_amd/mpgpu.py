@@ -29,7 +29,7 @@ EXPORTS = [
     "mpgpu_add_records", "mpgpu_read_mille_file", "mpgpu_set_steering_parameters", "mpgpu_add_equations",
     "mpgpu_build_structure", "mpgpu_get_labels", "mpgpu_matrix_info", "mpgpu_set_parameters",
     "mpgpu_get_parameters", "mpgpu_loopn", "mpgpu_get_vector", "mpgpu_get_matrix_packed", "mpgpu_mupdat",
-    "mpgpu_solve", "mpgpu_avprod", "mpgpu_get_errors", "mpgpu_run_steering",
+    "mpgpu_solve", "mpgpu_avprod", "mpgpu_get_errors", "mpgpu_get_monitor", "mpgpu_run_steering",
     "mpgpu_server_create", "mpgpu_server_submit", "mpgpu_server_wait", "mpgpu_server_destroy",
 ]
 
@@ -41,7 +41,8 @@ class Options(C.Structure):
                 ("mrmode", C.c_int32), ("chicut", C.c_float), ("chirem", C.c_float), ("chhuge", C.c_float),
                 ("dwcut", C.c_float), ("cndlmx", C.c_float), ("regula", C.c_float),
                 ("default_presigma", C.c_double), ("mrestl", C.c_double), ("mrtcnd", C.c_double),
-                ("nregul", C.c_int32), ("refine_steps", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("nregul", C.c_int32), ("refine_steps", C.c_int32), ("monitor", C.c_int32), ("monitor_bins", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class PassStats(C.Structure):
@@ -108,6 +109,7 @@ def lib():
         L.mpgpu_solve.argtypes = [C.c_void_p, D, D, C.POINTER(SolveStats)]
         L.mpgpu_avprod.argtypes = [C.c_void_p, D, D]
         L.mpgpu_get_errors.argtypes = [C.c_void_p, D, D]
+        L.mpgpu_get_monitor.argtypes = [C.c_void_p, I32, I32, C.POINTER(C.c_float), I64]
         L.mpgpu_run_steering.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(RunStats)]
         L.mpgpu_server_create.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]
         L.mpgpu_server_submit.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int64, I64, I32, F, C.c_int32, I32, D, D,
@@ -280,6 +282,18 @@ class Pede:
         g = np.zeros(self.nvar)
         self._chk(self.L.mpgpu_get_errors(self.h, _p(e, D), _p(g, D)))
         return e, g
+
+    def monitor(self):
+        """residual-monitoring histograms of the last pass: (labels, mean errors, counts[ngroups, bins])"""
+        ng = C.c_int32()
+        self._chk(self.L.mpgpu_get_monitor(self.h, C.byref(ng), None, None, None))
+        lab = np.zeros(ng.value, np.int32)
+        err = np.zeros(ng.value, np.float32)
+        bins = self.opt.monitor_bins if self.opt.monitor_bins > 0 else 100
+        h = np.zeros(ng.value * bins, np.int64)
+        self._chk(self.L.mpgpu_get_monitor(self.h, C.byref(ng), _p(lab, I32), err.ctypes.data_as(C.POINTER(C.c_float)),
+                                           _p(h, I64)))
+        return lab, err, h.reshape(ng.value, bins)
 
 
 class Server:

@@ -7,7 +7,10 @@ CI/refFiles/millepede.res (kept here as tests/golden/ci_wirechamber_millepede.re
 steering ends with 'method fullLAPACK' (needs LAPACK); this run uses 'method inversion' with
 the same iteration count and convergence limit, and Lagrange multipliers instead of
 elimination for the two constraints: the constrained least-squares solution is the same.
-Parameters are compared at the 5 significant digits the file holds; entry counts exactly.
+Parameters are compared at the 5 significant digits the file holds; entry counts exactly.  The
+residual monitoring of the CI steering ('monitorresiduals': normalized residuals of the first
+local-fit iteration per measurement group, after the first and the last pass) must reproduce
+CI/refFiles/millepede.mon (kept as tests/golden/ci_wirechamber_millepede.mon) line for line.
 """
 import os
 import subprocess
@@ -23,6 +26,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GEN = os.path.join(ROOT, "oracle", "_ref", "mptest1_gen")
 GOLD = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.res")
+GOLD_MON = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.mon")
 
 STEER = """fortranfiles
 mp2con.txt
@@ -35,6 +39,7 @@ dwfractioncut 0.1
 matiter 1
 printcounts
 countrecords
+monitorresiduals
 end
 """
 
@@ -59,6 +64,11 @@ def test_ci_wirechamber_res():
         assert st["exit_code"] == 0
         assert st["nrec"] == 100000 and st["ntot"] == 200 and st["ncon"] == 2
         got = parse_res(os.path.join(d, "millepede.res"))
+        mon = open(os.path.join(d, "millepede.mon")).read().splitlines()
+    gold_mon = open(GOLD_MON).read().splitlines()
+    assert len(mon) == len(gold_mon) == 202
+    diff = [(a, b) for a, b in zip(mon, gold_mon) if a != b]
+    assert not diff, diff[:5]
     gold = parse_res(GOLD)
     assert sorted(got) == sorted(gold)
     bad = []
