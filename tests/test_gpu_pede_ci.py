@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GEN = os.path.join(ROOT, "oracle", "_ref", "mptest1_gen")
 GOLD = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.res")
 GOLD_MON = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.mon")
+GOLD_END = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.end")
 
 STEER = """fortranfiles
 mp2con.txt
@@ -65,6 +66,7 @@ def test_ci_wirechamber_res():
         assert st["nrec"] == 100000 and st["ntot"] == 200 and st["ncon"] == 2
         got = parse_res(os.path.join(d, "millepede.res"))
         mon = open(os.path.join(d, "millepede.mon")).read().splitlines()
+        assert open(os.path.join(d, "millepede.end")).read() == open(GOLD_END).read()
     gold_mon = open(GOLD_MON).read().splitlines()
     assert len(mon) == len(gold_mon) == 202
     diff = [(a, b) for a, b in zip(mon, gold_mon) if a != b]
@@ -119,3 +121,26 @@ def test_errlabels_match_inversion(capfd):
         assert abs(err - inv[lab][0][3]) <= 2e-3 * abs(inv[lab][0][3]), (lab, err, inv[lab][0][3])
         seen += 1
     assert seen >= 3
+
+
+@pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
+@pytest.mark.parametrize("method", ["sparseMINRES", "fullMINRES", "sparseMINRES-QLP"])
+def test_ci_wirechamber_minres_elimination(method):
+    """The CI wire chamber solved by MINRES with the two constraints eliminated (pede's default
+    constraint mode, QL decomposition): the parameters of CI/refFiles/millepede.res at its 5
+    significant digits and the entry counts, as for the inversion."""
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([GEN, "100000"], cwd=d, check=True, capture_output=True)
+        steer = STEER.replace("method inversion 3 0.8", f"method {method} 3 0.8\nmrestol 1.0D-10")
+        with open(os.path.join(d, "steer.txt"), "w") as f:
+            f.write(steer)
+        st = mpgpu.run_steering(os.path.join(d, "steer.txt"), d)
+        assert st["exit_code"] == 0
+        assert st["ncon"] == 0              # no multiplier rows: the constraints are eliminated
+        got = parse_res(os.path.join(d, "millepede.res"))
+    gold = parse_res(GOLD)
+    assert sorted(got) == sorted(gold)
+    bad = [(lab, vals[0], got[lab][0][0]) for lab, (vals, cnt) in gold.items()
+           if abs(got[lab][0][0] - vals[0]) > 1.2e-4 * max(abs(vals[0]), 1e-6)]
+    assert not bad, bad[:10]
+    assert all(got[lab][1] == cnt for lab, (vals, cnt) in gold.items())
