@@ -427,10 +427,12 @@ __device__ __forceinline__ void bsymv_ld8(double (&bv)[8], const double *B, int 
 // current one is multiplied, and its descriptors one step earlier still.  The diagonal block (the
 // last block of the row, lower triangle stored) is done by the whole wave: lane (slot s, t) takes
 // the symmetric element (t, s) times x[fa+s].  Rows are summed over the eight lanes of a slot by a
-// halving butterfly (lane t ends with row t) and then over the slots.
+// halving butterfly (lane t ends with row t) and then over the slots.  With tb, the transposed
+// product of every off-diagonal block (column t: sum_r B[r][t] x[fa+r]) goes to tb[8 * block + t]
+// from the same loads, for the columns pass to sum without reading the matrix again.
 __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64_t *brow, const int32_t *bcf,
                                                      const int64_t *boff, const int32_t *gfirst, const double *a,
-                                                     const double *x, double *y)
+                                                     const double *x, double *y, double *tb)
 {
     const int ga = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63, slot = lane >> 3, t = lane & 7;
@@ -445,9 +447,14 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
         dv = a[boff[b1 - 1] + (tr >= sr ? tr * nr + sr : sr * nr + tr)];
         dx = (t < nr && slot < nr) ? x[fa + sr] : 0.0;
     }
-    double ys[8], bv[8], xb = 0.0;
+    double ys[8], bv[8], xa[8], xb = 0.0;
+    int ncb = 0;
 #pragma unroll
-    for (int r = 0; r < 8; r++) ys[r] = bv[r] = 0.0;
+    for (int r = 0; r < 8; r++) {
+        ys[r] = bv[r] = 0.0;
+        const double v = x[fa + min(r, nr - 1)];
+        xa[r] = r < nr ? v : 0.0;
+    }
     int64_t bk = b0 + slot;
     int2 dn = make_int2(0, 1);
     int64_t on = 0;
@@ -456,6 +463,7 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
         bsymv_ld8(bv, a + boff[bk], nr, d.y, t);
         const double xv = x[d.x + min(t, d.y - 1)];
         xb = t < d.y ? xv : 0.0;
+        ncb = d.y;
     }
     if (bk + 8 < be) {
         dn = *(const int2 *)(bcf + 2 * (bk + 8));
@@ -463,23 +471,29 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
     }
     for (; bk < be; bk += 8) {
         double bn[8], xn = 0.0;
+        int ncn = 0;
 #pragma unroll
         for (int r = 0; r < 8; r++) bn[r] = 0.0;
         if (bk + 8 < be) {
             bsymv_ld8(bn, a + on, nr, dn.y, t);
             const double xv = x[dn.x + min(t, dn.y - 1)];
             xn = t < dn.y ? xv : 0.0;
+            ncn = dn.y;
         }
         if (bk + 16 < be) {
             dn = *(const int2 *)(bcf + 2 * (bk + 16));
             on = boff[bk + 16];
         }
+        double tc = 0.0;
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             ys[r] += bv[r] * xb;
+            tc += bv[r] * xa[r];
             bv[r] = bn[r];
         }
+        if (tb && t < ncb) tb[8 * bk + t] = tc;
         xb = xn;
+        ncb = ncn;
     }
     const bool h2 = t & 4, h1 = t & 2, h0 = t & 1;
     double z[4], w[2];
@@ -503,6 +517,29 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
     if (slot == 0 && t < nr) y[fa + t] = v;
+}
+// Columns pass from the transposed block products of the rows pass: column group gb sums
+// tb[8 * block + t] over its off-diagonal blocks in column-list order (slot s takes entries
+// s, s+8, ...; one 64-byte read per block), then over the slots.
+__global__ __launch_bounds__(256) void k_bsymv_colst(int32_t ngroup, const int64_t *ccol_ptr, const int32_t *cc_blk,
+                                                     const int32_t *gfirst, const double *tb, double *y)
+{
+    const int gb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, slot = lane >> 3, t = lane & 7;
+    if (gb >= ngroup) return;
+    const int fb = gfirst[gb] - 1, nc = gfirst[gb + 1] - 1 - fb;
+    const int64_t q0 = ccol_ptr[gb], q1 = ccol_ptr[gb + 1];
+    if (q1 <= q0) return;
+    double a0 = 0.0, a1 = 0.0;
+    int64_t q = q0 + slot;
+    for (; q + 8 < q1; q += 16) {
+        const int64_t k0 = cc_blk[q], k1 = cc_blk[q + 8];
+        a0 += tb[8 * k0 + t];
+        a1 += tb[8 * k1 + t];
+    }
+    if (q < q1) a0 += tb[8 * (int64_t)cc_blk[q] + t];
+    const double acc = sum_slots(a0 + a1);
+    if (slot == 0 && t < nc) y[fb + t] += acc;
 }
 // Columns pass for groups of at most 8 parameters: the off-diagonal blocks (row group a, column
 // group gb) of a column group, lane t of a slot taking column t of the slot's block and loading its
@@ -1438,13 +1475,22 @@ __global__ void k_zero(int64_t n, double *y)
 }
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
                   const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
-                  const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, hipStream_t s)
+                  const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, const int32_t *cc_blk,
+                  double *tb, hipStream_t s)
 {
     // rows beyond the parameter groups (Lagrange multipliers) start at zero
     if (nall > nvar) hipLaunchKernelGGL(k_zero, dim3(nblk(nall - nvar)), dim3(TPB), 0, s, nall - nvar, y + nvar);
+    if (maxgrp <= 8 && tb && cc_blk) {
+        // the matrix is read once: rows pass with the transposed block products, then their sums
+        hipLaunchKernelGGL(k_bsymv_rows8, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff,
+                           gfirst, a, x, y, tb);
+        hipLaunchKernelGGL(k_bsymv_colst, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_blk,
+                           gfirst, tb, y);
+        return;
+    }
     if (maxgrp <= 8)
         hipLaunchKernelGGL(k_bsymv_rows8, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff,
-                           gfirst, a, x, y);
+                           gfirst, a, x, y, (double *)nullptr);
     else
         hipLaunchKernelGGL(k_bsymv_rows, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff,
                            gfirst, a, x, y);

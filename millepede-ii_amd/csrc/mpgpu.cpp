@@ -74,6 +74,8 @@ struct mpgpu_ctx {
     DBuf<int32_t> d_var, d_group_of, d_gfirst, d_bcol, d_brg;
     DBuf<int64_t> d_brow, d_boff, d_ccol_ptr, d_cc_off;
     DBuf<int32_t> d_bcf, d_cc_fn;   // per block / per column-list entry: first index and size of the other group
+    DBuf<int32_t> d_cc_blk;         // per column-list entry: the block
+    DBuf<double> d_tb;              // per off-diagonal block: its transposed product (8 values, groups <= 8)
     DBuf<unsigned long long> d_cnt;
     DBuf<double> d_chi2;
     DBuf<int32_t> d_ndf;
@@ -1143,6 +1145,14 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
             c->d_bcf.upload(bcf, s);
             c->d_cc_off.upload(ccoff, s);
             c->d_cc_fn.upload(ccfn, s);
+            if (c->plan.max_grp <= 8 && c->nblock < ((int64_t)1 << 31)) {
+                std::vector<int32_t> ccb(c->ccol_blk.begin(), c->ccol_blk.end());
+                c->d_cc_blk.upload(ccb.empty() ? std::vector<int32_t>(1, 0) : ccb, s);
+                c->d_tb.alloc((size_t)8 * std::max<int64_t>(1, c->nblock));
+            } else {
+                c->d_cc_blk.release();
+                c->d_tb.release();
+            }
             HIPCHK(hipStreamSynchronize(s));
         }
     }
@@ -1537,7 +1547,7 @@ void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
     hipStream_t s = c->stream;
     if (c->opt.matsto == MPGPU_SPARSE) {
         l_block_symv(c->ngroup, c->d_brow.p, c->d_bcf.p, c->d_boff.p, c->d_gfirst.p, c->d_ccol_ptr.p, c->d_cc_off.p,
-                     c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, c->plan.max_grp, s);
+                     c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, c->plan.max_grp, c->d_cc_blk.p, c->d_tb.p, s);
         l_cons_symv(c->ncon, c->nvar, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, x, y, s);
     } else {
         l_packed_symv(c->nall, c->d_mat.p, x, y, s);

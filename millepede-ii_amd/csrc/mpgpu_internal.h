@@ -112,7 +112,8 @@ void l_mr_step(MrState *st, hipStream_t s);
 void l_mr_update(int64_t n, const MrState *st, const double *v, double *w, double *w1, double *w2, double *x, hipStream_t s);
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
                   const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
-                  const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, hipStream_t s);
+                  const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, const int32_t *cc_blk,
+                  double *tb, hipStream_t s);
 void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
                  double *y, hipStream_t s);
 void l_scale_copy(int64_t n, double sc, const double *a, double *o, hipStream_t s);
