@@ -1783,6 +1783,79 @@ __global__ __launch_bounds__(256) void k_spk_carry(int n, int nch, const double 
     }
 }
 
+// Carry as a parallel scan.  The boundary values entering chunk k (in processing order) follow
+// s_{k+1} = A_k s_k + b_k, with A_k the w x w response of chunk k's boundary rows to the
+// incoming boundary (zero for the first chunk processed, which has none) and b_k its local
+// solution there.  Composing the maps by doubling (Kogge-Stone): at level l every chunk k >= 2^l
+// takes b_k += A_k^(l) b_{k-2^l}, A_k^(l) being the product of the maps (k-2^l, k].  The A^(l)
+// depend only on the band factor and are formed once (k_spk_scan_a0, k_spk_scan_up); an
+// application does the ~log2(nch) levels of w x w products on the vectors, in LDS.
+__device__ __forceinline__ int spk_brow(int n, int w, int nch, int k, int i, int backward, bool &in)
+{
+    const int ch = backward ? nch - 1 - k : k;
+    const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1);
+    const int row = backward ? k0 + i : k1 - w + 1 + i;
+    in = backward ? (row <= k1) : (row >= k0);
+    return row;
+}
+__global__ void k_spk_scan_a0(int n, int w, int nch, const double *S, double *A, int backward)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nch * w * w) return;
+    const int k = e / (w * w), i = (e / w) % w, q = e % w;
+    bool in;
+    const int row = spk_brow(n, w, nch, k, i, backward, in);
+    A[e] = (k > 0 && in) ? S[(size_t)(row - 1) * w + q] : 0.0;
+}
+__global__ void k_spk_scan_up(int w, int nch, const double *A, double *An, int step)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nch * w * w) return;
+    const int k = e / (w * w), i = (e / w) % w, q = e % w;
+    if (k < step) { An[e] = A[e]; return; }
+    double v = 0.0;
+    for (int r = 0; r < w; r++) v += A[((size_t)k * w + i) * w + r] * A[((size_t)(k - step) * w + r) * w + q];
+    An[e] = v;
+}
+template <int W>
+__global__ __launch_bounds__(1024) void k_spk_scan(int n, int nch, const double *x, const double *A, int nlev, double *t,
+                                                   int backward)
+{
+    extern __shared__ double dyn[];
+    double *b0 = dyn, *b1 = dyn + (size_t)nch * W;
+    const int ne = nch * W;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        bool in;
+        const int row = spk_brow(n, W, nch, e / W, e % W, backward, in);
+        b0[e] = in ? x[row - 1] : 0.0;
+    }
+    for (int l = 0; l < nlev; l++) {
+        __syncthreads();
+        const int step = 1 << l;
+        const double *Al = A + (size_t)l * ne * W;
+        for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+            const int k = e / W, i = e - k * W;
+            double v = b0[e];
+            if (k >= step) {
+                const double *a = Al + ((size_t)k * W + i) * W;
+                const double *bp = b0 + (k - step) * W;
+#pragma unroll
+                for (int q = 0; q < W; q++) v += a[q] * bp[q];
+            }
+            b1[e] = v;
+        }
+        double *tmp = b0;
+        b0 = b1;
+        b1 = tmp;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int k = e / W, i = e - k * W;
+        const int ch = backward ? nch - 1 - k : k;
+        t[(size_t)ch * W + i] = k > 0 ? b0[(k - 1) * W + i] : 0.0;
+    }
+}
+
 __global__ void k_spk_fix(int n, int w, const double *S, const double *t, double *x, int backward)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -1938,14 +2011,57 @@ void l_spk_build(int n, int w, const double *c, const int *india, double *Sf, do
     int nch = (n + SPK_CH - 1) / SPK_CH;
     hipLaunchKernelGGL(k_spk_build, dim3(g1((int64_t)nch * w, 64)), dim3(64), 0, s, n, w, nch, c, india, Sf, Sb, scr);
 }
+// scan levels of the carry (nlev = ceil(log2 nch)); A holds nlev * nch * w * w values per direction
+int spk_scan_levels(int n, int w)
+{
+    const int nch = (n + SPK_CH - 1) / SPK_CH;
+    if (nch < 2 || w > 8 || (size_t)2 * nch * w * sizeof(double) > 144 * 1024) return 0;
+    int l = 0;
+    while ((1 << l) < nch) l++;
+    return l;
+}
+void l_spk_scan_build(int n, int w, const double *Sf, const double *Sb, double *Af, double *Ab, int nlev, hipStream_t s)
+{
+    const int nch = (n + SPK_CH - 1) / SPK_CH;
+    const size_t lv = (size_t)nch * w * w;
+    const unsigned g = g1((int64_t)lv);
+    hipLaunchKernelGGL(k_spk_scan_a0, dim3(g), dim3(TPB), 0, s, n, w, nch, Sf, Af, 0);
+    hipLaunchKernelGGL(k_spk_scan_a0, dim3(g), dim3(TPB), 0, s, n, w, nch, Sb, Ab, 1);
+    for (int l = 0; l + 1 < nlev; l++) {
+        hipLaunchKernelGGL(k_spk_scan_up, dim3(g), dim3(TPB), 0, s, w, nch, Af + l * lv, Af + (l + 1) * lv, 1 << l);
+        hipLaunchKernelGGL(k_spk_scan_up, dim3(g), dim3(TPB), 0, s, w, nch, Ab + l * lv, Ab + (l + 1) * lv, 1 << l);
+    }
+}
+static void spk_scan(int n, int w, int nch, const double *x, const double *A, int nlev, double *t, int backward,
+                     hipStream_t s)
+{
+    const size_t lds = (size_t)2 * nch * w * sizeof(double);
+#define SPK_SCAN(W)                                                                                                   \
+    do {                                                                                                              \
+        if (lds > 65536) (void)hipFuncSetAttribute((const void *)k_spk_scan<W>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        hipLaunchKernelGGL(k_spk_scan<W>, dim3(1), dim3(1024), lds, s, n, nch, x, A, nlev, t, backward);                \
+    } while (0)
+    switch (w) {
+    case 1: SPK_SCAN(1); break;
+    case 2: SPK_SCAN(2); break;
+    case 3: SPK_SCAN(3); break;
+    case 4: SPK_SCAN(4); break;
+    case 5: SPK_SCAN(5); break;
+    case 6: SPK_SCAN(6); break;
+    case 7: SPK_SCAN(7); break;
+    default: SPK_SCAN(8); break;
+    }
+#undef SPK_SCAN
+}
 
 void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, const double *Sf, const double *Sb,
                      double *tbuf, const int *cb, const int64_t *koff, const double *kmat, const double *h, const int *hind,
-                     double *dtmp, double *x, hipStream_t s)
+                     double *dtmp, double *x, const double *Af, const double *Ab, int nlev, hipStream_t s)
 {
     int nch = (n + SPK_CH - 1) / SPK_CH;
     spk_local(n, w, nch, c, india, x, 0, s);
-    spk_carry(n, w, nch, x, Sf, tbuf, 0, s);
+    if (Af && nlev > 0) spk_scan(n, w, nch, x, Af, nlev, tbuf, 0, s);
+    else spk_carry(n, w, nch, x, Sf, tbuf, 0, s);
     hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sf, tbuf, x, 0);
     if (m > 0) {
         hipLaunchKernelGGL(k_eqs_cdot, dim3(m), dim3(TPB), 0, s, n, cb, koff, kmat, x, dtmp);
@@ -1953,7 +2069,8 @@ void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, con
         hipLaunchKernelGGL(k_eqs_upd, dim3(g1(n)), dim3(TPB), 0, s, n, m, cb, koff, kmat, x);
     }
     spk_local(n, w, nch, c, india, x, 1, s);
-    spk_carry(n, w, nch, x, Sb, tbuf, 1, s);
+    if (Ab && nlev > 0) spk_scan(n, w, nch, x, Ab, nlev, tbuf, 1, s);
+    else spk_carry(n, w, nch, x, Sb, tbuf, 1, s);
     hipLaunchKernelGGL(k_spk_fix, dim3(g1(n)), dim3(TPB), 0, s, n, w, Sb, tbuf, x, 1);
 }
 

@@ -1583,10 +1583,10 @@ double dot_dev(mpgpu_ctx *c, int64_t n, const double *a, const double *b)
 // preconditioner state for MINRES
 struct Precond {
     int kind = 0; // 0 none, 1 diagonal (+constraints), 2 band (+constraints)
-    DBuf<double> cu, sm, scratch, band, kmat, h, Sf, Sb, tbuf, dtmp;
+    DBuf<double> cu, sm, scratch, band, kmat, h, Sf, Sb, tbuf, dtmp, Af, Ab;   // Af/Ab: scan levels of the carry
     DBuf<int> nrkd, india, hind, cb;
     DBuf<int64_t> koff;
-    int n = 0, m = 0, w = 0;
+    int n = 0, m = 0, w = 0, nlev = 0;
     bool spike = false;
 };
 
@@ -1703,6 +1703,13 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
         DBuf<double> scr;
         scr.alloc((size_t)nch * P.w * 128);
         l_spk_build(P.n, P.w, P.band.p, P.india.p, P.Sf.p, P.Sb.p, scr.p, s);
+        P.nlev = spk_scan_levels(P.n, P.w);
+        if (P.nlev > 0) {
+            const size_t lv = (size_t)nch * P.w * P.w * P.nlev;
+            P.Af.alloc(lv);
+            P.Ab.alloc(lv);
+            l_spk_scan_build(P.n, P.w, P.Sf.p, P.Sb.p, P.Af.p, P.Ab.p, P.nlev, s);
+        }
         HIPCHK(hipStreamSynchronize(s));
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -1717,7 +1724,7 @@ void apply_precond(mpgpu_ctx *c, Precond &P, const double *x, double *y)
         l_copy(P.n + P.m, x, y, s);
         if (P.spike)
             l_equslvs_spike(P.n, P.m, P.w, P.band.p, P.india.p, P.Sf.p, P.Sb.p, P.tbuf.p, P.cb.p, P.koff.p, P.kmat.p,
-                            P.h.p, P.hind.p, P.dtmp.p, y, s);
+                            P.h.p, P.hind.p, P.dtmp.p, y, P.Af.p, P.Ab.p, P.nlev, s);
         else
             l_equslvs(P.n, P.m, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, P.h.p, P.hind.p, y, s);
     } else {

@@ -142,7 +142,9 @@ void l_equslvs(int n, int m, const double *c, const int *india, const int *cb, c
 void l_spk_build(int n, int w, const double *c, const int *india, double *Sf, double *Sb, double *scr, hipStream_t s);
 void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, const double *Sf, const double *Sb,
                      double *tbuf, const int *cb, const int64_t *koff, const double *kmat, const double *h, const int *hind,
-                     double *dtmp, double *x, hipStream_t s);
+                     double *dtmp, double *x, const double *Af, const double *Ab, int nlev, hipStream_t s);
+int spk_scan_levels(int n, int w);
+void l_spk_scan_build(int n, int w, const double *Sf, const double *Sb, double *Af, double *Ab, int nlev, hipStream_t s);
 
 // dense Cholesky (k_chol.hip): full column-major n x n matrix (lda = n)
 int chol_factor(int n, double *a, int lda, double *work, int *info_dev, hipStream_t s, int fp32, float *a32);
