@@ -44,7 +44,9 @@ struct Job {
 };
 
 // device bytes a job needs: records (8 B/word + compiled tables), the matrix (full packed or
-// an upper estimate of the group-block storage), vectors and solver work space
+// an upper estimate of the group-block storage), the update buffer of the two-phase build
+// (group-block storage: at most the lower triangle of each record's global words, 8 B each),
+// vectors and solver work space
 int64_t estimate_footprint(const Job &j)
 {
     int64_t nw = j.rec_ptr.empty() ? 0 : j.rec_ptr.back();
@@ -53,20 +55,25 @@ int64_t estimate_footprint(const Job &j)
     // distinct labels of the global words: after the second zero-index word of a measurement
     // (measurement, locals, sigma, globals; src/mille.f90:27-59)
     std::vector<int32_t> lab;
+    int64_t upd = 0;
     for (int64_t r = 0; r < nrec; r++) {
         int zeros = 0;
+        int64_t g = 0;
         for (int64_t w = j.rec_ptr[r] + 1; w < j.rec_ptr[r + 1]; w++) {
             if (j.inder[w] == 0) { zeros++; continue; }
-            if (zeros > 0 && zeros % 2 == 0) lab.push_back(j.inder[w]);
+            if (zeros > 0 && zeros % 2 == 0) { lab.push_back(j.inder[w]); g++; }
         }
+        upd += 8 * g * (g + 1) / 2;
     }
+    if (j.opt.matsto == MPGPU_FULL) upd = 0;
+    upd = std::min<int64_t>(upd, (int64_t)128 << 30);
     std::sort(lab.begin(), lab.end());
     int64_t nlab = std::unique(lab.begin(), lab.end()) - lab.begin();
     int64_t nall = nlab + (int64_t)j.crhs.size();
     int64_t mat;
     if (j.opt.matsto == MPGPU_FULL) mat = 8 * nall * (nall + 1) / 2 * (j.opt.metsol == MPGPU_DECOMPOSITION || j.opt.metsol == MPGPU_CHOLESKY_FP32_IR ? 3 : 2);
     else mat = std::min<int64_t>(8 * nall * (nall + 1) / 2, 8 * nw * 24);
-    return rec + mat + 200 * nall + (64 << 20);
+    return rec + mat + upd + 200 * nall + (64 << 20);
 }
 
 } // namespace
