@@ -1190,16 +1190,18 @@ __global__ __launch_bounds__(256) void k_lltdec_w(int n, double *c, const int *i
     if (tid == 0) *nrkd = nr;
 }
 
-// lltdec for a band of exactly MW values per row (rows 1..MW-1 shorter): the previous MW-1
-// rows live in registers.  Rows are taken in groups of MW aligned to row 1, so that row k sits in
-// ring slot (k-1) % MW, a compile-time index in the unrolled group; the band values are staged
-// per tile in LDS as in k_lltdec.  Same operations in the same order as k_lltdec_w.
+
+// The same recurrence with the tile's rows staged densely: the lanes copy row k's band (MW values,
+// element (k, k-MW+1+o) at d[(k-k0)*MW + o], zero before the row's first column) and its first
+// column into LDS, lane 0 reads and writes rows at fixed offsets, and the lanes scatter the
+// in-band values back.  Terms outside a row's band are exact zeros, so the operations and their
+// order are those of k_lltdec_w.
 template <int MW>
-__global__ __launch_bounds__(64) void k_lltdec_r(int n, double *c, const int *india, int *nrkd, int iopt)
+__global__ __launch_bounds__(64) void k_lltdec_d(int n, double *c, const int *india, int *nrkd, int iopt)
 {
     constexpr int ROWS = (LLT_TILE / MW / MW) * MW;   // rows per tile, a multiple of MW
-    __shared__ double buf[LLT_TILE];
-    __shared__ int ind[LLT_TILE + 1];
+    __shared__ double d[ROWS * MW];
+    __shared__ int mkr[ROWS];
     const int tid = threadIdx.x;
     const double eps = 16.0 * DBL_EPSILON;
     double ring[MW][MW];                  // slot s: row j with (j-1) % MW == s, element (j, j-MW+1+o) at [s][o]
@@ -1210,11 +1212,14 @@ __global__ __launch_bounds__(64) void k_lltdec_r(int n, double *c, const int *in
     int nr = 0;
     for (int k0 = 1; k0 <= n; k0 += ROWS) {
         const int k1 = min(n, k0 + ROWS - 1);
-        const int base = k0 > 1 ? india[k0 - 2] : 0;   // IND(k0-1)
-        const int nel = india[k1 - 1] - base;
-        for (int t = tid; t < nel; t += 64) buf[t] = c[base + t];
-        for (int t = tid; t <= k1 - k0; t += 64) ind[t + 1] = india[k0 - 1 + t];
-        if (tid == 0) ind[0] = base;
+        const int ne = (k1 - k0 + 1) * MW;
+        for (int t = tid; t < ne; t += 64) {
+            const int r = t / MW, o = t - r * MW, k = k0 + r;
+            const int indk = india[k - 1], indk1 = k > 1 ? india[k - 2] : 0;
+            const int mk = k - indk + indk1 + 1, col = k - MW + 1 + o;
+            d[t] = col >= mk ? c[indk - k + col - 1] : 0.0;
+            if (o == 0) mkr[r] = mk;
+        }
         __syncthreads();
         if (tid == 0) {
             for (int kg = k0; kg <= k1; kg += MW) {
@@ -1222,24 +1227,17 @@ __global__ __launch_bounds__(64) void k_lltdec_r(int n, double *c, const int *in
                 for (int s = 0; s < MW; s++) {
                     const int k = kg + s;                   // (k-1) % MW == s
                     if (k > k1) break;
-                    const int indk = ind[k - k0 + 1], indk1 = ind[k - k0];
-                    const int mk = k - indk + indk1 + 1;
-                    double *Pk = &buf[indk - k - base - 1];
+                    const int mk = mkr[k - k0];
+                    double *Pk = &d[(k - k0) * MW];
                     double r[MW];
 #pragma unroll
-                    for (int o = 0; o < MW; o++) {
-                        const int col = k - MW + 1 + o;
-                        const double v = Pk[max(col, mk)];
-                        r[o] = col >= mk ? v : 0.0;
-                    }
+                    for (int o = 0; o < MW; o++) r[o] = Pk[o];
                     if (k == 1) {
                         double v = r[MW - 1];
                         if (v > 0.0) v = 1.0 / sqrt(v);
                         else { v = 0.0; nr = -1; }
                         r[MW - 1] = v;
                     } else {
-                        // rows j = k-MW+1+jo (jo < MW-1) are in slot (s + 1 + jo) % MW; terms
-                        // outside a row's band are exact zeros (see k_lltdec_w)
 #pragma unroll
                         for (int jo = 0; jo < MW - 1; jo++) {
                             const int sj = (s + 1 + jo) % MW;
@@ -1266,15 +1264,18 @@ __global__ __launch_bounds__(64) void k_lltdec_r(int n, double *c, const int *in
                     }
 #pragma unroll
                     for (int o = 0; o < MW; o++) {
-                        const int col = k - MW + 1 + o;
-                        if (col >= mk) Pk[col] = r[o];
+                        Pk[o] = r[o];
                         ring[s][o] = r[o];
                     }
                 }
             }
         }
         __syncthreads();
-        for (int t = tid; t < nel; t += 64) c[base + t] = buf[t];
+        for (int t = tid; t < ne; t += 64) {
+            const int r = t / MW, o = t - r * MW, k = k0 + r;
+            const int col = k - MW + 1 + o;
+            if (col >= mkr[r]) c[india[k - 1] - k + col - 1] = d[t];
+        }
         __syncthreads();
     }
     if (tid == 0) *nrkd = nr;
@@ -1550,14 +1551,14 @@ void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, cons
 void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, hipStream_t s)
 {
     switch (mbw) {
-    case 1: hipLaunchKernelGGL(k_lltdec_r<1>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
-    case 2: hipLaunchKernelGGL(k_lltdec_r<2>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
-    case 3: hipLaunchKernelGGL(k_lltdec_r<3>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
-    case 4: hipLaunchKernelGGL(k_lltdec_r<4>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
-    case 5: hipLaunchKernelGGL(k_lltdec_r<5>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
-    case 6: hipLaunchKernelGGL(k_lltdec_r<6>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
-    case 7: hipLaunchKernelGGL(k_lltdec_r<7>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
-    case 8: hipLaunchKernelGGL(k_lltdec_r<8>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 1: hipLaunchKernelGGL(k_lltdec_d<1>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 2: hipLaunchKernelGGL(k_lltdec_d<2>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 3: hipLaunchKernelGGL(k_lltdec_d<3>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 4: hipLaunchKernelGGL(k_lltdec_d<4>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 5: hipLaunchKernelGGL(k_lltdec_d<5>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 6: hipLaunchKernelGGL(k_lltdec_d<6>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 7: hipLaunchKernelGGL(k_lltdec_d<7>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
+    case 8: hipLaunchKernelGGL(k_lltdec_d<8>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt); break;
     default: hipLaunchKernelGGL(k_lltdec, dim3(1), dim3(256), 0, s, n, c, india, nrkd, iopt, mbw); break;
     }
 }
