@@ -34,6 +34,10 @@ enum {
     MPGPU_DECOMPOSITION = 3,    /* method decomposition  -> mchdec  (src/pede.f90:7951)   */
     MPGPU_MINRES = 4,           /* method (full|sparse)MINRES -> mminrs (src/pede.f90:8243) */
     MPGPU_MINRES_QLP = 5,       /* method (full|sparse)MINRES-QLP (src/pede.f90:8341)       */
+    MPGPU_LAPACK_PACKED = 7,    /* method fullLAPACK     -> mdptrf (src/mplapack.f90:41): LL^T */
+                                /* of the parameter block on the device, constraints by the */
+                                /* Schur complement (same constrained solution)             */
+    MPGPU_LAPACK_UNPACKED = 8,  /* method unpackedLAPACK -> mdutrf (src/mplapack.f90:187)    */
     MPGPU_CHOLESKY_FP32_IR = 11 /* no pede counterpart: FP32-MFMA Cholesky + FP64 refinement */
 };
 
@@ -71,7 +75,9 @@ typedef struct mpgpu_options {
     int32_t monitor;         /* residual monitoring of the passes: 0 off, 1 normalized */
                              /* residuals ('monitorresiduals'), 2 pulls ('monitorpulls') */
     int32_t monitor_bins;    /* bins per measurement group (100)                       */
-    int32_t reserved[4];
+    int32_t lapack_errors;   /* 'LAPACKwitherrors': parameter errors from the factor of */
+                             /* the LAPACK methods (src/pede.f90:9017-9120, ilperr)     */
+    int32_t reserved[3];
 } mpgpu_options;
 
 /* Statistics of one data pass (loopn) */
@@ -163,7 +169,9 @@ int mpgpu_solve(mpgpu_ctx *ctx, const double *rhs, double *x, mpgpu_solve_stats 
 /* y = A x (avprod, src/pede.f90:5331-5358), x,y host arrays of nall */
 int mpgpu_avprod(mpgpu_ctx *ctx, const double *x, double *y);
 /* parameter errors sqrt|A^-1_ii| and global correlations (prtglo, src/pede.f90:4538-4546);
- * only after an inversion solve.  err, gcor: nvar (gcor may be NULL) */
+ * after an inversion solve, or a LAPACK-method solve with lapack_errors set (then from the
+ * covariance of the constrained parameters, src/pede.f90:9017-9120).  err, gcor: nvar
+ * (gcor may be NULL) */
 int mpgpu_get_errors(const mpgpu_ctx *ctx, double *err, double *gcor);
 /* residual monitoring (src/pede.f90:3955-3980 in loopbf, 6700-6710 and 7075-7085 in loop2):
  * histograms of the last data pass, one per measurement group -- the measurements with local

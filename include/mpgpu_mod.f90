@@ -15,7 +15,8 @@ MODULE mpgpu_mod
         REAL(c_double)     :: default_presigma, mrestl, mrtcnd
         INTEGER(c_int32_t) :: nregul, refine_steps
         INTEGER(c_int32_t) :: monitor, monitor_bins
-        INTEGER(c_int32_t) :: reserved(4)
+        INTEGER(c_int32_t) :: lapack_errors
+        INTEGER(c_int32_t) :: reserved(3)
     END TYPE mpgpu_options
 
     TYPE, BIND(C) :: mpgpu_pass_stats

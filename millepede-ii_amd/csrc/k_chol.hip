@@ -267,6 +267,32 @@ void chol_solve_multi(int n, const double *a, int lda, double *x, int nrhs, int6
     if (nrhs > 0) chol_solve_t<double, double>(n, a, lda, x, nrhs, ldx, s);
 }
 
+// columns j0 .. j0+nc of the identity (column-major n x nc)
+__global__ void k_unit_cols(int n, int j0, int nc, double *e)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * nc) return;
+    const int c = (int)(t / n), i = (int)(t - (int64_t)c * n);
+    e[t] = i == j0 + c ? 1.0 : 0.0;
+}
+__global__ void k_pick_diag(int n, int j0, int nc, const double *x, double *d)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) d[c] = x[(int64_t)c * n + j0 + c];
+}
+
+// diagonal of A^-1 from the factor: columns of A^-1 in blocks of nbk, solved together
+void chol_inv_diag(int n, const double *a, int lda, double *e, int nbk, double *d, hipStream_t s)
+{
+    for (int j0 = 0; j0 < n; j0 += nbk) {
+        const int nc = min(nbk, n - j0);
+        const int64_t tot = (int64_t)n * nc;
+        hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, j0, nc, e);
+        chol_solve_t<double, double>(n, a, lda, e, nc, n, s);
+        hipLaunchKernelGGL(k_pick_diag, dim3((nc + 255) / 256), dim3(256), 0, s, n, j0, nc, e, d + j0);
+    }
+}
+
 // diagonal of a column-major FP32 factor
 __global__ void k_diag_f32(int n, const float *a, int lda, float *d)
 {

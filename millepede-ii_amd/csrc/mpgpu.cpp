@@ -109,6 +109,8 @@ struct mpgpu_ctx {
     // ---- solve state ----
     bool have_inverse = false;
     std::vector<double> diag_before; // workspaceDiag (matrix diagonal before inversion)
+    bool have_cov = false;           // cov_diag holds the last LAPACK-method solve's covariance diagonal
+    std::vector<double> cov_diag;
     std::vector<double> last_vec;
     std::vector<unsigned long long> last_cnt;
 };
@@ -2187,9 +2189,18 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
             minresqlp_dev(c, b.p, xd.p, &loc);
             loc.rel_residual = rel_residual(c, b.p, xd.p);
         }
-    } else if (c->opt.metsol == MPGPU_DECOMPOSITION || c->opt.metsol == MPGPU_CHOLESKY_FP32_IR) {
-        if (c->opt.matsto == MPGPU_SPARSE) throw MpError(36, "method decomposition needs full matrix storage");
+    } else if (c->opt.metsol == MPGPU_DECOMPOSITION || c->opt.metsol == MPGPU_CHOLESKY_FP32_IR ||
+               c->opt.metsol == MPGPU_LAPACK_PACKED || c->opt.metsol == MPGPU_LAPACK_UNPACKED) {
+        if (c->opt.matsto == MPGPU_SPARSE) throw MpError(36, "this method needs full matrix storage");
         const int nv = c->nvar, m = c->ncon;
+        // 'LAPACKwitherrors': the parameter covariance on the constraint surface, as pede's
+        // inverse from the factor expanded by Q (src/pede.f90:9017-9120); with multipliers the
+        // same matrix is diag(A^-1) - diag(W S^-1 W^T), W = A^-1 C^T, S = C W, and A may carry
+        // the rho C^T C term below (it cancels on the constraint surface)
+        const bool want_cov = c->opt.lapack_errors &&
+                              (c->opt.metsol == MPGPU_LAPACK_PACKED || c->opt.metsol == MPGPU_LAPACK_UNPACKED);
+        c->have_cov = false;
+        std::vector<double> covcorr(want_cov ? nv : 0, 0.0);
         const int64_t lda = nv;
         DBuf<double> A, work;
         DBuf<int> info;
@@ -2332,6 +2343,31 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
                 for (int64_t q = c->cptr[i]; q < c->cptr[i + 1]; q++) acc += c->cval[q] * y[c->cidx[q] - 1];
                 g[i] = acc - bp[nv + i];
             }
+            if (want_cov) {
+                // S = R R^T (host Cholesky, m small); diag(W S^-1 W^T)_i = |R^-1 w_i|^2
+                std::vector<double> R((size_t)m * m, 0.0), y(m);
+                for (int j = 0; j < m; j++) {
+                    double d = S[(size_t)j * m + j];
+                    for (int k = 0; k < j; k++) d -= R[(size_t)j * m + k] * R[(size_t)j * m + k];
+                    if (!(d > 0.0)) throw MpError(29, "constraint matrix C A^-1 C^T not positive definite");
+                    R[(size_t)j * m + j] = std::sqrt(d);
+                    for (int i = j + 1; i < m; i++) {
+                        double e = S[(size_t)i * m + j];
+                        for (int k = 0; k < j; k++) e -= R[(size_t)i * m + k] * R[(size_t)j * m + k];
+                        R[(size_t)i * m + j] = e / R[(size_t)j * m + j];
+                    }
+                }
+                for (int i = 0; i < nv; i++) {
+                    double acc2 = 0.0;
+                    for (int j = 0; j < m; j++) {
+                        double e = Z[(size_t)j * nv + i];
+                        for (int k = 0; k < j; k++) e -= R[(size_t)j * m + k] * y[k];
+                        y[j] = e / R[(size_t)j * m + j];
+                        acc2 += y[j] * y[j];
+                    }
+                    covcorr[i] = acc2;
+                }
+            }
             // dense symmetric solve of S lambda = g (Gaussian elimination with partial pivoting, m small)
             std::vector<double> Aug(S);
             std::vector<int> piv(m);
@@ -2363,6 +2399,22 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
             xd.upload(xs.data(), n, s);
         }
         loc.rel_residual = rel_residual(c, b.p, xd.p);
+        if (want_cov) {
+            const int nbk = std::min(nv, 512);
+            DBuf<double> E, dd, dg;
+            E.alloc((size_t)nv * nbk);
+            dd.alloc(nv);
+            dg.alloc(nv);
+            chol_inv_diag(nv, A.p, (int)lda, E.p, nbk, dd.p, s);
+            l_get_diag(model_of(c), nv, dg.p, s);
+            c->cov_diag.assign(nv, 0.0);
+            c->diag_before.assign(n, 0.0);
+            dd.download(c->cov_diag.data(), nv, s);
+            dg.download(c->diag_before.data(), nv, s);
+            HIPCHK(hipStreamSynchronize(s));
+            for (int i = 0; i < nv; i++) c->cov_diag[i] -= covcorr[i];
+            c->have_cov = true;
+        }
     } else {
         throw MpError(36, "Unsupported solution method");
     }
@@ -2394,13 +2446,18 @@ extern "C" int mpgpu_get_errors(const mpgpu_ctx *c0, double *err, double *gcor)
     mpgpu_ctx *c = const_cast<mpgpu_ctx *>(c0);
     if (!c || !c->built) return 30;
     API_TRY
-    if (!c->have_inverse) throw MpError(30, "errors need an inversion solve");
-    DBuf<double> dg;
-    dg.alloc(c->nvar);
-    l_get_diag(model_of(c), c->nvar, dg.p, c->stream);
+    if (!c->have_inverse && !c->have_cov)
+        throw MpError(30, "errors need an inversion solve or a LAPACK-method solve with lapack_errors");
     std::vector<double> d(c->nvar);
-    dg.download(d.data(), c->nvar, c->stream);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->have_inverse) {
+        DBuf<double> dg;
+        dg.alloc(c->nvar);
+        l_get_diag(model_of(c), c->nvar, dg.p, c->stream);
+        dg.download(d.data(), c->nvar, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        d = c->cov_diag;
+    }
     // prtglo (src/pede.f90:4536-4546): err = sign * sqrt(|A^-1_ii|) in REAL(4)
     for (int32_t v = 0; v < c->nvar; v++) {
         double gmati = d[v];

@@ -151,6 +151,8 @@ int chol_factor(int n, double *a, int lda, double *work, int *info_dev, hipStrea
 void chol_solve(int n, const double *a, int lda, double *x, hipStream_t s);
 void chol_solve32(int n, const float *a, int lda, double *x, float *xf, hipStream_t s);
 void chol_solve_multi(int n, const double *a, int lda, double *x, int nrhs, int64_t ldx, hipStream_t s);
+// diagonal of A^-1 from the FP64 factor; e: scratch of n * nbk doubles, d: n
+void chol_inv_diag(int n, const double *a, int lda, double *e, int nbk, double *d, hipStream_t s);
 void chol_diag32(int n, const float *a, int lda, float *d, hipStream_t s);
 void l_packed_to_full(int n, const double *packed, double *full, int lda, hipStream_t s);
 void l_full_to_f32(int64_t nn, const double *a, float *b, hipStream_t s);

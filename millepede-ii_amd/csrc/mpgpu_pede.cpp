@@ -133,6 +133,8 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
             else if (m == "fullminres-qlp") { S.opt.metsol = MPGPU_MINRES_QLP; S.opt.matsto = MPGPU_FULL; }
             else if (m == "sparseminres-qlp") { S.opt.metsol = MPGPU_MINRES_QLP; S.opt.matsto = MPGPU_SPARSE; }
             else if (m == "cholesky-fp32") { S.opt.metsol = MPGPU_CHOLESKY_FP32_IR; S.opt.matsto = MPGPU_FULL; }
+            else if (m == "fulllapack") { S.opt.metsol = MPGPU_LAPACK_PACKED; S.opt.matsto = MPGPU_FULL; }
+            else if (m == "unpackedlapack") { S.opt.metsol = MPGPU_LAPACK_UNPACKED; S.opt.matsto = MPGPU_FULL; }
             else { S.opt.metsol = -1; S.err = "Unsupported solution method " + m; }
             std::vector<double> mn;
             for (size_t q = 2; q < t.size(); q++) { double v; if (is_number(t[q], v)) mn.push_back(v); }
@@ -184,6 +186,7 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
             continue;
         }
         if (key_match(k, "maxlocalcond")) { if (!nums.empty()) S.opt.cndlmx = (float)nums[0]; continue; }
+        if (key_match(k, "lapackwitherrors")) { S.opt.lapack_errors = 1; continue; }
         if (key_match(k, "printcounts")) { S.ipcntr = nums.empty() ? 1 : (int)std::llround(nums[0]); continue; }
         if (key_match(k, "globalcorr")) { S.igcorr = 1; continue; }
         if (key_match(k, "errlabels")) {
@@ -746,7 +749,8 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
             if ((rc = mpgpu_solve(c, b.data(), x.data(), &sst))) return bail(rc);
             tsol += now_ms() - t;
             nloopsol = nloopn;
-            if (metsol == MPGPU_INVERSION) {
+            if (metsol == MPGPU_INVERSION ||
+                ((metsol == MPGPU_LAPACK_PACKED || metsol == MPGPU_LAPACK_UNPACKED) && S.opt.lapack_errors)) {
                 err_v.assign(nvar, 0.0);
                 gcor_v.assign(nvar, 0.0);
                 have_err = mpgpu_get_errors(c, err_v.data(), gcor_v.data()) == 0;

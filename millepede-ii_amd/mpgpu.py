@@ -20,7 +20,7 @@ F = C.POINTER(C.c_float)
 I32 = C.POINTER(C.c_int32)
 I64 = C.POINTER(C.c_int64)
 
-INVERSION, DECOMPOSITION, MINRES, MINRES_QLP, CHOLESKY_FP32_IR = 1, 3, 4, 5, 11
+INVERSION, DECOMPOSITION, MINRES, MINRES_QLP, LAPACK_PACKED, LAPACK_UNPACKED, CHOLESKY_FP32_IR = 1, 3, 4, 5, 7, 8, 11
 FULL, SPARSE = 1, 2
 
 # every entry point declared in include/mpgpu.h
@@ -42,7 +42,7 @@ class Options(C.Structure):
                 ("dwcut", C.c_float), ("cndlmx", C.c_float), ("regula", C.c_float),
                 ("default_presigma", C.c_double), ("mrestl", C.c_double), ("mrtcnd", C.c_double),
                 ("nregul", C.c_int32), ("refine_steps", C.c_int32), ("monitor", C.c_int32), ("monitor_bins", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("lapack_errors", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class PassStats(C.Structure):

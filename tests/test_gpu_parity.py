@@ -455,6 +455,45 @@ def test_cholesky(metsol):
     assert np.max(np.abs(x[:P.nvar] - xr[:P.nvar]) / sig) < 1e-6
 
 
+@pytest.mark.parametrize("metsol", [mpgpu.LAPACK_PACKED, mpgpu.LAPACK_UNPACKED])
+@pytest.mark.parametrize("constrained", [True, False])
+def test_lapack_with_errors(metsol, constrained):
+    """'method fullLAPACK' / 'unpackedLAPACK' with 'LAPACKwitherrors' (mdptrf / mdutrf,
+    src/mplapack.f90:41-355; inverse from the factor after the iterations,
+    src/pede.f90:9017-9120).  The solution against numpy's solve of the same matrix, and the
+    errors sign * sqrt|Cov_ii| and global correlations (prtglo, src/pede.f90:4536-4546) with Cov
+    the parameter block of the inverse KKT matrix: with eliminated constraints pede forms
+    Q [Abar^-1 0; 0 0] Q^T, the same matrix (the first nvar - ncon columns of Q span the null
+    space of C).  Constrained: the pixel problem with its 18 constraints; unconstrained: a
+    pre-sigma on every parameter.  The errors also equal those of 'method inversion'."""
+    rp, ii, gg, cons, gam = _pixel_cons_problem(seed=25, nl=6, nx=8, ny=4, ntr=20000)
+    if constrained:
+        steer, presig = gam, mpgpu.pixel_presigma(gam)
+    else:
+        steer = [(l + 1) * 100000 + (m + 1) * 10 + k for l in range(6) for m in range(32) for k in range(1, 7)]
+        presig, cons = np.full(len(steer), 0.05), None
+    P = _pede(rp, ii, gg, steer, presig, cons, matsto=mpgpu.FULL, metsol=metsol, mreqenf=5, mcount=1,
+              lapack_errors=1)
+    P.loopn()
+    xr, A, b = _kkt_reference(P)
+    x, st = P.solve()
+    nv = P.nvar
+    assert st["method"] == metsol and P.ncon == (len(cons) if constrained else 0)
+    assert rel_err(x[:nv], xr[:nv]) < 1e-9
+    cov = np.diag(np.linalg.inv(A))[:nv]
+    e, g = P.errors()
+    e_ref = (np.sign(cov) * np.sqrt(np.abs(cov).astype(np.float32))).astype(np.float64)
+    assert np.max(np.abs(e - e_ref) / np.abs(e_ref)) < 1e-6
+    g2 = 1.0 - 1.0 / (cov * np.diag(A)[:nv])
+    g_ref = np.where((g2 >= 0) & (g2 <= 1), np.sqrt(np.clip(g2, 0, 1)).astype(np.float32), -1.0)
+    assert np.max(np.abs(g - g_ref)) < 1e-5
+    Q = _pede(rp, ii, gg, steer, presig, cons, matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=5, mcount=1)
+    Q.loopn()
+    Q.solve()
+    e2, g2 = Q.errors()
+    assert np.max(np.abs(e - e2) / np.abs(e2)) < 1e-6 and np.max(np.abs(g - g2)) < 1e-5
+
+
 def test_avprod_matches_dense():
     rp, ii, gg, cons, gam = _pixel_cons_problem(seed=24, ntr=2000)
     for matsto in (mpgpu.FULL, mpgpu.SPARSE):

@@ -4,9 +4,11 @@ The CI job of the reference (.gitlab-ci.yml:86-100) generates 100,000 wire-chamb
 with pede's own generator (src/mptest1.f90, built from the reference sources into
 oracle/_ref/mptest1_gen), solves with CI/mp2str.txt and compares millepede.res with
 CI/refFiles/millepede.res (kept here as tests/golden/ci_wirechamber_millepede.res).  The CI
-steering ends with 'method fullLAPACK' (needs LAPACK); this run uses 'method inversion' with
-the same iteration count and convergence limit, and Lagrange multipliers instead of
-elimination for the two constraints: the constrained least-squares solution is the same.
+steering ends with 'method fullLAPACK 3 0.8' (the Cholesky factorization of mplapack.f90);
+test_ci_wirechamber_res uses 'method inversion' with the same iteration count and convergence
+limit and Lagrange multipliers instead of elimination for the two constraints (the
+constrained least-squares solution is the same), test_ci_wirechamber_fulllapack the CI's own
+method.
 Parameters are compared at the 5 significant digits the file holds; entry counts exactly.  The
 residual monitoring of the CI steering ('monitorresiduals': normalized residuals of the first
 local-fit iteration per measurement group, after the first and the last pass) must reproduce
@@ -144,3 +146,41 @@ def test_ci_wirechamber_minres_elimination(method):
            if abs(got[lab][0][0] - vals[0]) > 1.2e-4 * max(abs(vals[0]), 1e-6)]
     assert not bad, bad[:10]
     assert all(got[lab][1] == cnt for lab, (vals, cnt) in gold.items())
+
+
+@pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
+def test_ci_wirechamber_fulllapack():
+    """The CI's own method, 'method fullLAPACK 3 0.8' (mdptrf, src/mplapack.f90:41-179), here
+    with 'LAPACKwitherrors': parameters and counts of CI/refFiles/millepede.res, the final chi^2
+    of the CI log, and the error and global-correlation columns equal to those 'method
+    inversion' writes for the same data (both print 5 significant digits)."""
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([GEN, "100000"], cwd=d, check=True, capture_output=True)
+        steer = STEER.replace("method inversion 3 0.8", "method fullLAPACK 3 0.8\nLAPACKwitherrors")
+        with open(os.path.join(d, "lp.txt"), "w") as f:
+            f.write(steer)
+        st = mpgpu.run_steering(os.path.join(d, "lp.txt"), d)
+        assert st["exit_code"] == 0
+        got = parse_res(os.path.join(d, "millepede.res"))
+        with open(os.path.join(d, "inv.txt"), "w") as f:
+            f.write(STEER)
+        st2 = mpgpu.run_steering(os.path.join(d, "inv.txt"), d)
+        assert st2["exit_code"] == 0
+        inv = parse_res(os.path.join(d, "millepede.res"))
+    gold = parse_res(GOLD)
+    assert sorted(got) == sorted(gold)
+    bad = [(lab, vals[0], got[lab][0][0]) for lab, (vals, cnt) in gold.items()
+           if abs(got[lab][0][0] - vals[0]) > 1.2e-4 * max(abs(vals[0]), 1e-6)]
+    assert not bad, bad[:10]
+    assert all(got[lab][1] == cnt for lab, (vals, cnt) in gold.items())
+    assert abs(st["final_chi2"] - 8130954.4875403922) < 1e-6 * 8130954.4875403922
+    nerr = 0
+    for lab, (vals, _) in inv.items():
+        if len(vals) < 4:
+            continue
+        g = got[lab][0]
+        assert len(g) == len(vals)
+        for a, b in zip(g[3:], vals[3:]):
+            assert abs(a - b) <= 2.5e-4 * max(abs(b), 1e-6), (lab, g, vals)
+        nerr += 1
+    assert nerr >= 150
