@@ -234,10 +234,9 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
     return true;
 }
 
-// Fortran Gw.d edit descriptor for a REAL(4) value (gfortran rendering)
-std::string fmt_g(float xf, int w = 14, int d = 5)
+// Fortran Gw.d edit descriptor (gfortran rendering)
+std::string fmt_gd(double x, int w, int d)
 {
-    double x = (double)xf;
     char buf[64];
     if (x == 0.0) {
         std::snprintf(buf, sizeof buf, "%*.*f%*s", w - 4, d - 1, 0.0, 4, "");
@@ -259,6 +258,71 @@ std::string fmt_g(float xf, int w = 14, int d = 5)
     std::snprintf(buf, sizeof buf, "%*s", w, s2);
     return buf;
 }
+std::string fmt_g(float xf, int w = 14, int d = 5) { return fmt_gd((double)xf, w, d); }
+
+static std::string rjust(const std::string &s, int w)
+{
+    if ((int)s.size() > w) return std::string(w, '*');
+    return std::string(w - s.size(), ' ') + s;
+}
+// Ew.d: [-]0.ddddE+xx, the leading zero dropped when the width is short
+std::string fmt_e(double x, int w, int d)
+{
+    char e[64];
+    std::snprintf(e, sizeof e, "%.*e", d - 1, std::fabs(x));
+    std::string digits;
+    for (const char *q = e; *q && *q != 'e'; q++) if (*q >= '0' && *q <= '9') digits += *q;
+    const int eo = x == 0.0 ? 0 : std::atoi(std::strchr(e, 'e') + 1) + 1;
+    char ex[16];
+    if (std::abs(eo) <= 99) std::snprintf(ex, sizeof ex, "E%c%02d", eo < 0 ? '-' : '+', std::abs(eo));
+    else std::snprintf(ex, sizeof ex, "%c%03d", eo < 0 ? '-' : '+', std::abs(eo));
+    std::string body = std::string(x < 0 ? "-" : "") + "0." + digits + ex;
+    if ((int)body.size() > w) body = std::string(x < 0 ? "-" : "") + "." + digits + ex;
+    return rjust(body, w);
+}
+// Fw.d
+std::string fmt_f(double x, int w, int d)
+{
+    char b[64];
+    std::snprintf(b, sizeof b, "%.*f", d, x);
+    std::string s = b;
+    if ((int)s.size() > w && s.rfind("0.", 0) == 0) s = s.substr(1);
+    if ((int)s.size() > w && s.rfind("-0.", 0) == 0) s = "-" + s.substr(2);
+    return rjust(s, w);
+}
+// list-directed output of REAL(8), REAL(4) and integers after a character item (gfortran)
+std::string ld_r8(double x)
+{
+    char e[64], b[64];
+    std::snprintf(e, sizeof e, "%.16e", std::fabs(x));
+    const int k = x == 0.0 ? 1 : std::atoi(std::strchr(e, 'e') + 1) + 1;
+    if (k >= 0 && k <= 17) {
+        std::snprintf(b, sizeof b, "%21.*f     ", 17 - k, x);
+        return b;
+    }
+    std::snprintf(e, sizeof e, "%.16E", x);
+    const char *q = std::strchr(e, 'E');
+    const int eo = std::atoi(q + 1);
+    std::snprintf(b, sizeof b, "%.*sE%c%03d", (int)(q - e), e, eo < 0 ? '-' : '+', std::abs(eo));
+    return rjust(b, 26);
+}
+std::string ld_r4(float xf)
+{
+    const double x = (double)xf;
+    char e[64], b[64];
+    std::snprintf(e, sizeof e, "%.8e", std::fabs(x));
+    const int k = x == 0.0 ? 1 : std::atoi(std::strchr(e, 'e') + 1) + 1;
+    if (k >= 0 && k <= 9) {
+        std::snprintf(b, sizeof b, "%13.*f    ", 9 - k, x);
+        return b;
+    }
+    std::snprintf(e, sizeof e, "%.8E", x);
+    const char *q = std::strchr(e, 'E');
+    const int eo = std::atoi(q + 1);
+    std::snprintf(b, sizeof b, "%.*sE%c%02d", (int)(q - e), e, eo < 0 ? '-' : '+', std::abs(eo));
+    return rjust(b, 17);
+}
+std::string ld_i(long long v, int w = 12) { return rjust(std::to_string(v), w); }
 
 // ptline (src/linesrch.f90:89-231) with ptldef (232-257)
 struct LineSearch {
@@ -334,6 +398,43 @@ struct LineSearch {
         tot = tot + alpha;
         step = tot;
         return -1;
+    }
+    // ptlopt (src/linesrch.f90:258-288): slopes and steps of the first and the last call
+    void opt(float slopes[3], float steps[3]) const
+    {
+        if (nsfd == 0) { for (int i = 0; i < 3; i++) slopes[i] = steps[i] = 0.0f; return; }
+        int m = 1;
+        for (int i = 1; i <= nsfd; i++) if (std::fabs(sfd[2][i]) < std::fabs(sfd[2][m])) m = i;
+        slopes[0] = (float)sfd[2][1]; slopes[1] = (float)sfd[2][nsfd]; slopes[2] = (float)sfd[2][m];
+        steps[0] = (float)sfd[0][1]; steps[1] = (float)sfd[0][nsfd]; steps[2] = (float)sfd[0][m];
+    }
+    // ptlprt (src/linesrch.f90:294-343): the table of this line search
+    void print(FILE *f, int lsinfo) const
+    {
+        if (!f || nsfd <= 0) return;
+        std::fprintf(f, "  \n PTLINE: line-search method based on slopes with sufficient slope-decrease\n");
+        std::fprintf(f, " PTLDEF: slope ratio limit=%s\n", ld_r8(gtol).c_str());
+        std::fprintf(f, " PTLDEF: maximum step =%s\n", ld_r8(stmx).c_str());
+        std::fprintf(f, " PTLDEF:%s  <= nr of calls <=%s\n", ld_i(minf).c_str(), ld_i(maxf).c_str());
+        std::fprintf(f, "  i      x              F(x)         F'(X)          minimum     F'(X)\n");
+        int m = 1;
+        for (int i = 1; i <= nsfd; i++) if (std::fabs(sfd[2][i]) < std::fabs(sfd[2][m])) m = i;
+        for (int i = 1; i <= nsfd; i++) {
+            char tlr[3] = "  ";
+            if (i == m) tlr[0] = tlr[1] = '*';
+            if (i == idgl) tlr[0] = 'L';
+            if (i == idgr) tlr[1] = 'R';
+            std::string l = rjust(std::to_string(i - 1), 3) + fmt_f(sfd[0][i], 12, 6) + " " + tlr +
+                            fmt_gd(sfd[1][i], 15, 6) + fmt_gd(sfd[2][i], 14, 6) + fmt_f(sfd[3][i], 12, 6);
+            if (i == 1) l += "     ratio";
+            else l += fmt_f((float)std::fabs(sfd[2][i] / sfd[2][1]), 10, 3);
+            std::fprintf(f, "%s\n", l.c_str());
+        }
+        static const char *msg[5] = {"PTLINE: INFO=0  input error (e.g. gradient not negative)",
+                                     "PTLINE: INFO=1  convergence reached", "PTLINE: INFO=2  too many function calls",
+                                     "PTLINE: INFO=3  maximum step reached", "PTLINE: INFO=4  step too small (< 0.001)"};
+        if (lsinfo >= 0 && lsinfo <= 4) std::fprintf(f, " %s\n", msg[lsinfo]);
+        std::fprintf(f, "  \n");
     }
 };
 
@@ -719,7 +820,17 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
         if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
     }
     concut = concu2;
-    std::printf(" it fc   fcn_value dfcn_exp  rejects\n");
+    // millepede.log: the iteration table (ploopa-ploopd, src/pede.f90:2891-3040), the line
+    // search tables (ptlprt, src/linesrch.f90:294-343) and the final chi^2 summary
+    std::unique_ptr<FILE, int (*)(FILE *)> flog_keep(std::fopen((od + "/millepede.log").c_str(), "w"), std::fclose);
+    FILE *flog = flog_keep.get();
+    auto both = [&](const std::string &l) {
+        std::printf("%s\n", l.c_str());
+        if (flog) std::fprintf(flog, "%s\n", l.c_str());
+    };
+    int lsinfo = 0, iitera = 0, istopa = 0;
+    float angras = 0.0f;
+    double rstart = now_ms();
 
     // The iteration state machine of xloopn (src/pede.f90:8755-9009).  icalcm: 1 = data pass
     // with matrix, then solve; 0 = function pass only (line search); -1 = solve with the
@@ -728,14 +839,55 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     bool lsflag = false;
     float delfun = 0.0f;
     iterat = -1;
-    auto line = [&](int it) {
-        std::printf(" %2d %2d %12.5E %8.2E %8lld\n", it, nloopn, fvalue, (double)delfun, (long long)nrej_last);
+    // the line of the loop just ended, at the top of the next (src/pede.f90:8764-8794)
+    auto loop_line = [&]() {
+        if (iterat < 0) return;
+        static const char *ccalcm[4] = {" end", "   S", " F  ", " FMS"};
+        const float dt = (float)((now_ms() - rstart) / 1000.0);        // sechms, src/pede.f90:5750-5765
+        const int nsecd = (int)std::lround(dt);
+        const int nhour = nsecd / 3600, minut = nsecd / 60 - 60 * nhour;
+        const float secnd = dt - 60.0f * (float)(minut + 60 * nhour);
+        char tm[32];
+        std::snprintf(tm, sizeof tm, "%3d%02d%02d%s", nhour, minut, (int)std::lround(secnd), ccalcm[jcalcm + 2]);
+        const std::string nrej = ld_i(std::min<int64_t>(nrej_last, 9999999), 7);
+        auto i3 = [](int v) { return ld_i(v, 3); };
+        float slopes[3], steps[3];
+        std::string l;
+        if (jcalcm + 1 != 0) {
+            if (iterat == 0) {
+                both("  ");
+                both(" it fc   fcn_value dfcn_exp  slpr costh  iit st ls  step cutf rejects hhmmss FMS");
+                both(" -- -- ----------- --------  ---- -----  --- -- -- ----- ---- ------- ------ ---");
+                l = i3(iterat) + i3(nloopn) + fmt_e(fvalue, 12, 5) + std::string(38, ' ') + fmt_f(chicut, 5, 1) + " " +
+                    nrej + tm;
+                iterat = 1;
+            } else if (iterat != litera) {
+                if (lsinfo == 10) {
+                    l = i3(iterat) + i3(nloopn) + fmt_e(fvalue, 12, 5) + " " + fmt_e(delfun, 8, 2) + std::string(12, ' ') +
+                        ld_i(iitera, 5) + i3(istopa) + std::string(9, ' ') + fmt_f(chicut, 5, 1) + " " + nrej + tm;
+                } else {
+                    LS.opt(slopes, steps);
+                    const float ratae = std::max(-99.9f, std::min(99.9f, slopes[1] / slopes[0]));
+                    l = i3(iterat) + i3(nloopn) + fmt_e(fvalue, 12, 5) + " " + fmt_e(delfun, 8, 2) + fmt_f(ratae, 6, 3) +
+                        fmt_f(angras, 6, 3) + ld_i(iitera, 5) + i3(istopa) + i3(lsinfo) + fmt_f(steps[1], 6, 3) +
+                        fmt_f(chicut, 5, 1) + " " + nrej + tm;
+                }
+                litera = iterat;
+            } else if (lsinfo == 10) {
+                l = "   " + i3(nloopn) + fmt_e(fvalue, 12, 5) + std::string(44, ' ') + nrej + tm;
+            } else {
+                LS.opt(slopes, steps);
+                l = "   " + i3(nloopn) + fmt_e(fvalue, 12, 5) + std::string(9, ' ') + fmt_f(std::fabs(slopes[1] / slopes[0]), 6, 3) +
+                    std::string(14, ' ') + i3(lsinfo) + fmt_f(steps[1], 6, 3) + std::string(6, ' ') + nrej + tm;
+            }
+        } else {
+            l = std::string(69, ' ') + tm;
+        }
+        both(l);
+        rstart = now_ms();
     };
     for (;;) {
-        if (iterat >= 0 && jcalcm + 1 != 0) {
-            if (iterat == 0) iterat = 1;
-            else if (iterat != litera) { litera = iterat; line(iterat - 1); }
-        }
+        loop_line();
         jcalcm = icalcm;
         if (icalcm >= 0) {                        // block 1: data pass (b = gradient at par)
             if ((rc = pass(icalcm))) return bail(rc);
@@ -749,6 +901,7 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
             if ((rc = mpgpu_solve(c, b.data(), x.data(), &sst))) return bail(rc);
             tsol += now_ms() - t;
             nloopsol = nloopn;
+            if (metsol == MPGPU_MINRES || metsol == MPGPU_MINRES_QLP) { iitera = sst.itn; istopa = sst.istop; }
             if (metsol == MPGPU_INVERSION ||
                 ((metsol == MPGPU_LAPACK_PACKED || metsol == MPGPU_LAPACK_UNPACKED) && S.opt.lapack_errors)) {
                 err_v.assign(nvar, 0.0);
@@ -764,6 +917,7 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
             double db = 0, db1 = 0, db2 = 0;
             for (int32_t v = 0; v < nvar; v++) { db += corr[v] * b[v]; db1 += corr[v] * corr[v]; db2 += b[v] * b[v]; }
             delfun = (float)db;
+            angras = (float)(db / std::sqrt(db1 * db2));
             double dbsig = 16.0 * std::sqrt(std::max(db1, db2)) * 2.220446049250313e-16;
             lsflag = (S.lsearch > 2 || (S.lsearch == 2 && chicut < 2.25f) ||
                       (S.lsearch == 1 && chicut < 2.25f && (delfun <= S.dflim || iterat >= S.mitera)));
@@ -790,6 +944,7 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
         } else if (nloopn == nloopsol) {
             for (int32_t v = 0; v < nvar; v++) xls[v] += corr[v];
         }
+        lsinfo = info;
         int nan = 0;
         for (int32_t v = 0; v < nvar; v++) {
             if (!(xls[v] <= 0.0) && !(xls[v] > 0.0)) nan++;
@@ -805,20 +960,35 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
         if (info == 0 && iagain <= 0) { icalcm = 1; continue; }
         if (info < 0 || nloopn == nloopsol) continue;   // icalcm = 0: function pass next
         // block 4: line search converged
+        LS.print(flog, lsinfo);
         int iact = FS.apply(concut, par, var);
         if (iact && (rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
         if (iact || chicut > 1.0f) { icalcm = iterat < matrit ? 1 : -1; continue; }
         if (delfun <= S.dflim || iterat >= S.mitera) break;
         icalcm = iterat < matrit ? 1 : -1;
     }
-    line(iterat);
+    loop_line();
     loc.final_chi2 = ps_st.chi2;
     loc.final_ndf = ps_st.ndf;
     loc.iterations = iterat;
     loc.ms_passes = tp;
     loc.ms_solves = tsol;
-    std::printf(" Sum(Chi^2)/Sum(Ndf) = %.17g / %lld = %.17g\n", ps_st.chi2, (long long)(ps_st.ndf - nvar),
-                ps_st.ndf > nvar ? ps_st.chi2 / (double)(ps_st.ndf - nvar) : 0.0);
+    {
+        // final chi^2 per degree of freedom (src/pede.f90:9124-9147); pede counts the fit
+        // parameters as nvar - ncon with eliminated constraints, nvar + ncon with multipliers
+        const int ncon_steer = ncon > 0 ? ncon : (int)S.cons.size();
+        const int64_t nfitpar = S.opt.withmultipliers ? (int64_t)nvar + ncon : (int64_t)nvar - ncon_steer;
+        const double dwmean = ps_st.ndf > 0 ? ps_st.wndf / (double)ps_st.ndf : 1.0;
+        const double dratio = fvalue / dwmean / (double)(ps_st.ndf - nfitpar);
+        float catio = (float)dratio;
+        if (nloopn != 1 && S.opt.lhuber != 0) catio = catio / 0.9326f;
+        both("  ");
+        both(" Sum(Chi^2)/Sum(Ndf) =" + ld_r8(fvalue));
+        both("                     / (" + ld_i(ps_st.ndf, 21) + " -" + ld_i(nfitpar) + " )");
+        both("                     =" + ld_r8(dratio));
+        both("  ");
+        if (nloopn != 1 && S.opt.lhuber != 0) both("             with correction for down-weighting   " + ld_r4(catio));
+    }
     if ((metsol == MPGPU_MINRES || metsol == MPGPU_MINRES_QLP) && !S.errlabels.empty()) {
         // errors and global correlations of selected parameters from MINRES (solglo/solgloqlp,
         // src/pede.f90:537-701, called at 9288-9306): the row of the inverse from A x = e_i

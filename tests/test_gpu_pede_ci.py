@@ -15,6 +15,7 @@ local-fit iteration per measurement group, after the first and the last pass) mu
 CI/refFiles/millepede.mon (kept as tests/golden/ci_wirechamber_millepede.mon) line for line.
 """
 import os
+import re
 import subprocess
 import tempfile
 
@@ -30,6 +31,7 @@ GEN = os.path.join(ROOT, "oracle", "_ref", "mptest1_gen")
 GOLD = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.res")
 GOLD_MON = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.mon")
 GOLD_END = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.end")
+GOLD_LOG = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede_iterations.txt")
 
 STEER = """fortranfiles
 mp2con.txt
@@ -55,6 +57,62 @@ def parse_res(path):
             continue
         out[int(t[0])] = ([float(x) for x in t[1:-1]], int(t[-1]))
     return out
+
+
+def _ulp(tok):
+    """one unit in the last printed digit of a Fortran E or F field"""
+    t = tok.lstrip("-")
+    if "E" in t:
+        mant, ex = t.split("E")
+        return 10.0 ** (int(ex) - len(mant.split(".")[1]))
+    return 10.0 ** -len(t.split(".")[1]) if "." in t else 0.0
+
+
+def compare_log(log, gold):
+    """The iteration table, line-search tables and chi^2 summary of millepede.log against the
+    CI golden (from the table header to the down-weighting line): the same lines and text;
+    integers equal; printed reals within 1.5 units of their last digit, plus 1e-9 relative for
+    the 17-digit chi^2 sums (summed in another order).  In a line-search table (ptlprt) F(x),
+    a difference of two chi^2 sums of 8e6, also within 1e-8; the slopes F'(X) within 1e-6
+    relative, and within 1e-6 of the first slope where they are that small (the zero crossing,
+    whose digits are rounding noise).  Columns 70-76 (hh mm ss) are skipped."""
+    def section(lines):
+        i = next(k for k, l in enumerate(lines) if l.startswith(" it fc   fcn_value"))
+        j = next(k for k, l in enumerate(lines) if "with correction for down-weighting" in l)
+        return lines[i:j + 1]
+    a, b = section(log), section(gold)
+    assert len(a) == len(b), (len(a), len(b))
+    scale = 0.0
+    for la, lb in zip(a, b):
+        if lb.rstrip().endswith(("FMS", " F", "   S", " end")) and len(lb) >= 76:
+            la, lb = la[:69] + la[76:], lb[:69] + lb[76:]
+        ta, tb = la.split(), lb.split()
+        assert len(ta) == len(tb), (la, lb)
+        table = len(tb) >= 6 and tb[0].isdigit() and re.fullmatch(r"-?\d+\.\d{6}", tb[1]) is not None
+        col = {}
+        if table:
+            first = tb[-1] == "ratio"
+            f_at, s_at = (2, 3) if first else (3, 4)
+            if first:
+                scale = abs(float(tb[s_at]))
+            col = {f_at: "F", s_at: "slope"}
+        for k, (x, y) in enumerate(zip(ta, tb)):
+            try:
+                fy = float(y)
+            except ValueError:
+                assert x == y, (la, lb)
+                continue
+            if "." not in y and "E" not in y:
+                assert x == y, (la, lb)
+                continue
+            tol = max(1.5 * _ulp(y), 1e-9 * abs(fy))
+            if col.get(k) == "F":
+                tol += 1e-8
+            elif col.get(k) == "slope":
+                tol += 1e-6 * abs(fy) + (1e-6 * scale if abs(fy) < 1e-5 * scale else 0.0)
+            elif table:
+                tol += 1e-6
+            assert abs(float(x) - fy) <= tol, (x, y, la, lb)
 
 
 @pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
@@ -151,9 +209,10 @@ def test_ci_wirechamber_minres_elimination(method):
 @pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
 def test_ci_wirechamber_fulllapack():
     """The CI's own method, 'method fullLAPACK 3 0.8' (mdptrf, src/mplapack.f90:41-179), here
-    with 'LAPACKwitherrors': parameters and counts of CI/refFiles/millepede.res, the final chi^2
-    of the CI log, and the error and global-correlation columns equal to those 'method
-    inversion' writes for the same data (both print 5 significant digits)."""
+    with 'LAPACKwitherrors': parameters and counts of CI/refFiles/millepede.res, the iteration
+    table, line-search tables and chi^2 summary of CI/refFiles/millepede.log, and the error and
+    global-correlation columns equal to those 'method inversion' writes for the same data (both
+    print 5 significant digits)."""
     with tempfile.TemporaryDirectory() as d:
         subprocess.run([GEN, "100000"], cwd=d, check=True, capture_output=True)
         steer = STEER.replace("method inversion 3 0.8", "method fullLAPACK 3 0.8\nLAPACKwitherrors")
@@ -162,6 +221,7 @@ def test_ci_wirechamber_fulllapack():
         st = mpgpu.run_steering(os.path.join(d, "lp.txt"), d)
         assert st["exit_code"] == 0
         got = parse_res(os.path.join(d, "millepede.res"))
+        log = open(os.path.join(d, "millepede.log")).read().splitlines()
         with open(os.path.join(d, "inv.txt"), "w") as f:
             f.write(STEER)
         st2 = mpgpu.run_steering(os.path.join(d, "inv.txt"), d)
@@ -174,6 +234,7 @@ def test_ci_wirechamber_fulllapack():
     assert not bad, bad[:10]
     assert all(got[lab][1] == cnt for lab, (vals, cnt) in gold.items())
     assert abs(st["final_chi2"] - 8130954.4875403922) < 1e-6 * 8130954.4875403922
+    compare_log(log, open(GOLD_LOG).read().splitlines())
     nerr = 0
     for lab, (vals, _) in inv.items():
         if len(vals) < 4:
