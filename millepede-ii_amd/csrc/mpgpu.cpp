@@ -1319,7 +1319,7 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     HIPCHK(hipEventRecord(e0, s));
     c->d_vec.zero(s);
     c->d_cnt.zero(s);
-    if (icalcm == 1) { c->d_mat.zero(s); c->have_inverse = false; }
+    if (icalcm == 1) { c->d_mat.zero(s); c->have_inverse = false; c->have_cov = false; }
     PassOpts o{};
     o.icalcm = icalcm;
     o.nloopn = nloopn;
