@@ -455,9 +455,15 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
         const double v = x[fa + min(r, nr - 1)];
         xa[r] = r < nr ? v : 0.0;
     }
+    // two blocks per slot in flight: bv (block bk) is used while bm (bk + 8) and bn (bk + 16)
+    // are loaded; the descriptors run one block further ahead
     int64_t bk = b0 + slot;
     int2 dn = make_int2(0, 1);
     int64_t on = 0;
+    double bm[8], x1 = 0.0;
+    int nc1 = 0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) bm[r] = 0.0;
     if (bk < be) {
         const int2 d = *(const int2 *)(bcf + 2 * bk);
         bsymv_ld8(bv, a + boff[bk], nr, d.y, t);
@@ -466,34 +472,44 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
         ncb = d.y;
     }
     if (bk + 8 < be) {
-        dn = *(const int2 *)(bcf + 2 * (bk + 8));
-        on = boff[bk + 8];
+        const int2 d = *(const int2 *)(bcf + 2 * (bk + 8));
+        bsymv_ld8(bm, a + boff[bk + 8], nr, d.y, t);
+        const double xv = x[d.x + min(t, d.y - 1)];
+        x1 = t < d.y ? xv : 0.0;
+        nc1 = d.y;
+    }
+    if (bk + 16 < be) {
+        dn = *(const int2 *)(bcf + 2 * (bk + 16));
+        on = boff[bk + 16];
     }
     for (; bk < be; bk += 8) {
         double bn[8], xn = 0.0;
         int ncn = 0;
 #pragma unroll
         for (int r = 0; r < 8; r++) bn[r] = 0.0;
-        if (bk + 8 < be) {
+        if (bk + 16 < be) {
             bsymv_ld8(bn, a + on, nr, dn.y, t);
             const double xv = x[dn.x + min(t, dn.y - 1)];
             xn = t < dn.y ? xv : 0.0;
             ncn = dn.y;
         }
-        if (bk + 16 < be) {
-            dn = *(const int2 *)(bcf + 2 * (bk + 16));
-            on = boff[bk + 16];
+        if (bk + 24 < be) {
+            dn = *(const int2 *)(bcf + 2 * (bk + 24));
+            on = boff[bk + 24];
         }
         double tc = 0.0;
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             ys[r] += bv[r] * xb;
             tc += bv[r] * xa[r];
-            bv[r] = bn[r];
+            bv[r] = bm[r];
+            bm[r] = bn[r];
         }
         if (tb && t < ncb) tb[8 * bk + t] = tc;
-        xb = xn;
-        ncb = ncn;
+        xb = x1;
+        ncb = nc1;
+        x1 = xn;
+        nc1 = ncn;
     }
     const bool h2 = t & 4, h1 = t & 2, h0 = t & 1;
     double z[4], w[2];
