@@ -15,6 +15,10 @@
 
 using namespace mpgpu;
 
+namespace {
+struct Precond;
+}
+
 struct mpgpu_ctx {
     mpgpu_options opt{};
     int device = 0;
@@ -110,6 +114,10 @@ struct mpgpu_ctx {
     bool have_inverse = false;
     std::vector<double> diag_before; // workspaceDiag (matrix diagonal before inversion)
     bool have_cov = false;           // cov_diag holds the last LAPACK-method solve's covariance diagonal
+    // MINRES preconditioner of the current matrix (pede builds it only after a matrix pass,
+    // src/pede.f90:8292-8306): kept until the matrix changes; pc_bw/pc_lp: its mbandw/lprecm
+    std::shared_ptr<Precond> pc;
+    int pc_bw = 0, pc_lp = 0;
     std::vector<double> cov_diag;
     std::vector<double> last_vec;
     std::vector<unsigned long long> last_cnt;
@@ -1192,6 +1200,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     HIPCHK(hipStreamSynchronize(s));
     phase("device upload");
     c->built = true;
+    c->pc.reset();
     if (ntot_out) *ntot_out = c->ntot;
     if (nvar_out) *nvar_out = c->nvar;
     if (ncon_out) *ncon_out = c->ncon;
@@ -1299,6 +1308,7 @@ static void device_mupdat(mpgpu_ctx *c, const std::vector<int32_t> &ii, const st
     dv.upload(vv, c->stream);
     l_mupdat(model_of(c), (int32_t)ii.size(), di.p, dj.p, dv.p, c->stream);
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->pc.reset();
 }
 
 extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float chicut, mpgpu_pass_stats *st)
@@ -1319,7 +1329,7 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     HIPCHK(hipEventRecord(e0, s));
     c->d_vec.zero(s);
     c->d_cnt.zero(s);
-    if (icalcm == 1) { c->d_mat.zero(s); c->have_inverse = false; c->have_cov = false; }
+    if (icalcm == 1) { c->d_mat.zero(s); c->have_inverse = false; c->have_cov = false; c->pc.reset(); }
     PassOpts o{};
     o.icalcm = icalcm;
     o.nloopn = nloopn;
@@ -1734,6 +1744,20 @@ void apply_precond(mpgpu_ctx *c, Precond &P, const double *x, double *y)
     }
 }
 
+// the preconditioner of the current matrix, built on first use after the matrix changed
+Precond &cached_precond(mpgpu_ctx *c)
+{
+    if (!c->pc || c->pc_bw != c->opt.mbandw || c->pc_lp != c->opt.lprecm) {
+        c->pc.reset();
+        auto P = std::make_shared<Precond>();
+        setup_precond(c, *P);
+        c->pc = P;
+        c->pc_bw = c->opt.mbandw;
+        c->pc_lp = c->opt.lprecm;
+    }
+    return *c->pc;
+}
+
 // MINRES, restating src/minresModule.f90:294-660 with device vectors and the scalar recurrence on the device
 void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats *st)
 {
@@ -1741,8 +1765,7 @@ void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
     hipStream_t s = c->stream;
     ElimWork E;
     if (c->elim) { E.e.alloc(c->nvar); E.f.alloc(c->nvar); }
-    Precond P;
-    setup_precond(c, P);
+    Precond &P = cached_precond(c);
     const bool precon = P.kind != 0;
     DBuf<double> r1, r2, v, w, w1, w2, y;
     r1.alloc(n); r2.alloc(n); v.alloc(n); w.alloc(n); w1.alloc(n); w2.alloc(n); y.alloc(n);
@@ -1873,8 +1896,7 @@ void minresqlp_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve
     hipStream_t s = c->stream;
     ElimWork E;
     if (c->elim) { E.e.alloc(c->nvar); E.f.alloc(c->nvar); }
-    Precond P;
-    setup_precond(c, P);
+    Precond &P = cached_precond(c);
     const bool precon = P.kind != 0;
     DBuf<double> r1, r2, v, w, wl, wl2, xl2, y;
     r1.alloc(n); r2.alloc(n); v.alloc(n); w.alloc(n); wl.alloc(n); wl2.alloc(n); xl2.alloc(n); y.alloc(n);
@@ -2177,6 +2199,7 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
         loc.rel_residual = rel_residual(c, b.p, xd.p);
         std::swap(keep.p, c->d_mat.p);
         c->have_inverse = true;
+        c->pc.reset();
     } else if (c->opt.metsol == MPGPU_MINRES) {
         if (c->elim) elim_solve(c, b.p, xd.p, &loc, false);
         else {

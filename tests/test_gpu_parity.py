@@ -494,6 +494,34 @@ def test_lapack_with_errors(metsol, constrained):
     assert np.max(np.abs(e - e2) / np.abs(e2)) < 1e-6 and np.max(np.abs(g - g2)) < 1e-5
 
 
+def test_preconditioner_kept_until_matrix_changes():
+    """pede builds the MINRES preconditioner only after a matrix pass (src/pede.f90:8292-8306);
+    the library keeps it until the matrix changes.  With one right-hand side for every solve
+    (the vector's FP64 atomics differ in the last bits between passes): two solves on one
+    matrix agree bit for bit, and after mupdat the solve equals, bit for bit, that of a fresh
+    context given the same update (a stale band factor would take another iteration path)."""
+    rp, ii, gg, cons, gam = _pixel_cons_problem(seed=26)
+
+    def make():
+        P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=5,
+                  mcount=1, mbandw=6, mrestl=1e-12, withmultipliers=1)
+        P.loopn()
+        return P
+    U = ([1, 2], [1, 2], [5.0e3, 5.0e3])
+    P = make()
+    b = P.vector()[0].copy()
+    x1, s1 = P.solve(b)
+    x2, s2 = P.solve(b)
+    assert np.array_equal(x1, x2) and s1["itn"] == s2["itn"]
+    P.mupdat(*U)
+    x3, _ = P.solve(b)
+    Q = make()
+    Q.mupdat(*U)
+    assert np.array_equal(Q.matrix_packed(), P.matrix_packed())
+    x4, _ = Q.solve(b)
+    assert np.array_equal(x3, x4) and not np.array_equal(x3, x1)
+
+
 def test_avprod_matches_dense():
     rp, ii, gg, cons, gam = _pixel_cons_problem(seed=24, ntr=2000)
     for matsto in (mpgpu.FULL, mpgpu.SPARSE):
