@@ -234,9 +234,19 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
     return true;
 }
 
+// NaN and infinities as gfortran writes them (right-justified; "Inf" where "Infinity" does not fit)
+static std::string nonfinite(double x, int w)
+{
+    std::string t = x != x ? "NaN" : (w >= 9 || (x > 0 && w >= 8) ? "Infinity" : "Inf");
+    if (x == x && x < 0) t = "-" + t;
+    if ((int)t.size() > w) return std::string(w, '*');
+    return std::string(w - t.size(), ' ') + t;
+}
+
 // Fortran Gw.d edit descriptor (gfortran rendering)
 std::string fmt_gd(double x, int w, int d)
 {
+    if (!std::isfinite(x)) return nonfinite(x, w);
     char buf[64];
     if (x == 0.0) {
         std::snprintf(buf, sizeof buf, "%*.*f%*s", w - 4, d - 1, 0.0, 4, "");
@@ -268,6 +278,7 @@ static std::string rjust(const std::string &s, int w)
 // Ew.d: [-]0.ddddE+xx, the leading zero dropped when the width is short
 std::string fmt_e(double x, int w, int d)
 {
+    if (!std::isfinite(x)) return nonfinite(x, w);
     char e[64];
     std::snprintf(e, sizeof e, "%.*e", d - 1, std::fabs(x));
     std::string digits;
@@ -283,6 +294,7 @@ std::string fmt_e(double x, int w, int d)
 // Fw.d
 std::string fmt_f(double x, int w, int d)
 {
+    if (!std::isfinite(x)) return nonfinite(x, w);
     char b[64];
     std::snprintf(b, sizeof b, "%.*f", d, x);
     std::string s = b;
@@ -293,6 +305,7 @@ std::string fmt_f(double x, int w, int d)
 // list-directed output of REAL(8), REAL(4) and integers after a character item (gfortran)
 std::string ld_r8(double x)
 {
+    if (!std::isfinite(x)) return nonfinite(x, 21) + "     ";
     char e[64], b[64];
     std::snprintf(e, sizeof e, "%.16e", std::fabs(x));
     const int k = x == 0.0 ? 1 : std::atoi(std::strchr(e, 'e') + 1) + 1;
@@ -309,6 +322,7 @@ std::string ld_r8(double x)
 std::string ld_r4(float xf)
 {
     const double x = (double)xf;
+    if (!std::isfinite(x)) return nonfinite(x, 13) + "    ";
     char e[64], b[64];
     std::snprintf(e, sizeof e, "%.8e", std::fabs(x));
     const int k = x == 0.0 ? 1 : std::atoi(std::strchr(e, 'e') + 1) + 1;
