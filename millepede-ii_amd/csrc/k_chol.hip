@@ -245,9 +245,10 @@ int chol_factor(int n, double *a, int lda, double *work, int *info_dev, hipStrea
 
 // nrhs right-hand sides, column q at x + q * ldx, solved side by side (grid dimension y)
 template <typename T, typename V>
-static void chol_solve_t(int n, const T *a, int lda, V *x, int nrhs, int64_t ldx, hipStream_t s)
+static void chol_solve_t(int n, const T *a, int lda, V *x, int nrhs, int64_t ldx, hipStream_t s, int first = 0)
 {
-    for (int k0 = 0; k0 < n; k0 += NB) {
+    // right-hand sides that are zero above row 'first': the forward sweep starts at its tile
+    for (int k0 = (first / NB) * NB; k0 < n; k0 += NB) {
         hipLaunchKernelGGL((k_trsv_tile<T, V>), dim3(1, nrhs), dim3(256), 0, s, n, a, lda, k0, x, ldx, 0);
         int rest = n - k0 - min(NB, n - k0);
         if (rest > 0)
@@ -288,7 +289,7 @@ void chol_inv_diag(int n, const double *a, int lda, double *e, int nbk, double *
         const int nc = min(nbk, n - j0);
         const int64_t tot = (int64_t)n * nc;
         hipLaunchKernelGGL(k_unit_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, j0, nc, e);
-        chol_solve_t<double, double>(n, a, lda, e, nc, n, s);
+        chol_solve_t<double, double>(n, a, lda, e, nc, n, s, j0);
         hipLaunchKernelGGL(k_pick_diag, dim3((nc + 255) / 256), dim3(256), 0, s, n, j0, nc, e, d + j0);
     }
 }
