@@ -77,7 +77,22 @@ typedef struct mpgpu_options {
     int32_t monitor_bins;    /* bins per measurement group (100)                       */
     int32_t lapack_errors;   /* 'LAPACKwitherrors': parameter errors from the factor of */
                              /* the LAPACK methods (src/pede.f90:9017-9120, ilperr)     */
-    int32_t reserved[3];
+    int32_t pair_entries;    /* 'pairentries n': mreqpe, minimum number of entries of a */
+                             /* pair of parameter groups for its matrix block to be      */
+                             /* stored; updates of rarer pairs are dropped (1 = all;     */
+                             /* src/pede.f90:10762-10772, 6564, 7036; src/mpbits.f90)    */
+    int32_t skip_empty_cons; /* 'skipemptycons' (iskpec): drop constraints without any  */
+                             /* variable parameter; without it such a constraint stops  */
+                             /* the run (src/pede.f90:1164-1190)                         */
+    int32_t resolve_redundancy_cons; /* 'resolveredundancycons' (irslvrc): a constraint */
+                             /* block whose product matrix is rank deficient is reduced */
+                             /* to its independent constraints (src/pede.f90:1028-1050) */
+    float scale_errors[2];   /* 'scaleerrors f1 f2': errorScaleFactor, measurement      */
+                             /* errors with (1) / without (2) global derivatives are     */
+                             /* multiplied by it (src/mppar.f90:312-316); default 1, 1   */
+    int32_t reserved[2];
+    double refine_tol;       /* FP32+IR: stop when ||r|| <= refine_tol ||x|| ||A||_F     */
+                             /* (0: eps sqrt(n), the test of LAPACK's DSPOSV)            */
 } mpgpu_options;
 
 /* Statistics of one data pass (loopn) */
@@ -99,12 +114,16 @@ typedef struct mpgpu_pass_stats {
 /* Statistics of one solve */
 typedef struct mpgpu_solve_stats {
     int32_t method;      /* method actually used                                          */
-    int32_t nrank;       /* rank found (inversion / decomposition)                        */
-    int32_t istop;       /* MINRES stop code (src/minresModule.f90:591-600)               */
+    int32_t nrank;       /* rank found (inversion / decomposition: positive pivots)       */
+    int32_t istop;       /* MINRES stop code (src/minresModule.f90:591-600); FP32+IR: 1   */
+                         /* refinement converged, 3 not within refine_steps (the FP64   */
+                         /* factor then solved the system: method = 3)                  */
     int32_t itn;         /* MINRES iterations / refinement steps                          */
     double anorm, acond, rnorm, arnorm, ynorm;
     double rel_residual; /* ||A x - b|| / ||b||, evaluated in FP64 on the device           */
     double ms;           /* wall time of the solve (device-synchronised)                  */
+    double evmax, evmin; /* decomposition: largest / smallest positive pivot of LDL^T       */
+                         /* (chdec2, src/mpnum.f90:917-925)                                */
 } mpgpu_solve_stats;
 
 /* ---- context ---------------------------------------------------------------------------- */
@@ -206,6 +225,9 @@ typedef struct mpgpu_job_stats {
     int64_t ndf;
     int32_t istop, itn;
     double rel_residual;
+    double t_arrival_s, t_done_s;        /* the steady (CLOCK_MONOTONIC) clock in seconds at   */
+                                         /* arrival (entry of submit) and at the result       */
+    int32_t method, reserved;            /* solution method that ran (mpgpu_solve_stats)      */
 } mpgpu_job_stats;
 int mpgpu_server_create(int32_t device, int32_t nworkers, int64_t budget_bytes, int32_t batch_max_params,
                         mpgpu_server **out);

@@ -16,7 +16,10 @@ MODULE mpgpu_mod
         INTEGER(c_int32_t) :: nregul, refine_steps
         INTEGER(c_int32_t) :: monitor, monitor_bins
         INTEGER(c_int32_t) :: lapack_errors
-        INTEGER(c_int32_t) :: reserved(3)
+        INTEGER(c_int32_t) :: pair_entries, skip_empty_cons, resolve_redundancy_cons
+        REAL(c_float)      :: scale_errors(2)
+        INTEGER(c_int32_t) :: reserved(2)
+        REAL(c_double)     :: refine_tol
     END TYPE mpgpu_options
 
     TYPE, BIND(C) :: mpgpu_pass_stats
@@ -33,6 +36,7 @@ MODULE mpgpu_mod
     TYPE, BIND(C) :: mpgpu_solve_stats
         INTEGER(c_int32_t) :: method, nrank, istop, itn
         REAL(c_double)     :: anorm, acond, rnorm, arnorm, ynorm, rel_residual, ms
+        REAL(c_double)     :: evmax, evmin
     END TYPE mpgpu_solve_stats
 
     TYPE, BIND(C) :: mpgpu_run_stats
@@ -52,6 +56,8 @@ MODULE mpgpu_mod
         INTEGER(c_int64_t) :: ndf
         INTEGER(c_int32_t) :: istop, itn
         REAL(c_double)     :: rel_residual
+        REAL(c_double)     :: t_arrival_s, t_done_s
+        INTEGER(c_int32_t) :: method, reserved
     END TYPE mpgpu_job_stats
 
     INTERFACE

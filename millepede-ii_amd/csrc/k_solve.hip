@@ -259,51 +259,79 @@ __global__ void k_gj_finish(int n, double *v, double *b, const int *used, const 
 // y = A x for the packed full matrix (avprd0 full storage, src/pede.f90:5069-5232):
 // 64x64 tiles of the lower triangle; each stored element serves both halves.
 // ---------------------------------------------------------------------------------------------
+// Rows are taken in blocks of 64 (block b: rows 64b .. 64b+63, packed rows contiguous); a
+// workgroup (b, s) stages the 64 x 64 tiles of columns 64 * (16 s .. 16 s + 15) of its row block
+// through LDS (read once, coalesced along the packed rows) and forms from each tile its row
+// products (summed over the workgroup's tiles into yr[s][row]) and its transposed products
+// (z[b][col], the part of column col from the rows of block b).  A second kernel adds, in a
+// fixed order, each row's partial sums over s and each column's over b: no atomics, the same
+// result on every run.
 #define ST 64
-__global__ __launch_bounds__(256) void k_packed_symv(int n, const double *a, const double *x, double *y)
+#define SSEG 16
+__global__ __launch_bounds__(256) void k_psymv_tiles(int n, const double *a, const double *x, double *yr, double *z)
 {
-    const int nt = (n + ST - 1) / ST;
-    int64_t tile = blockIdx.x;
-    // tile -> (ti >= tj)
-    int ti = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
-    while ((int64_t)ti * (ti + 1) / 2 > tile) ti--;
-    while ((int64_t)(ti + 1) * (ti + 2) / 2 <= tile) ti++;
-    int tj = (int)(tile - (int64_t)ti * (ti + 1) / 2);
-    if (ti >= nt) return;
-    __shared__ double xr[ST], xc[ST], yr[ST], yc[ST];
-    int r0 = ti * ST, c0 = tj * ST;
-    if (threadIdx.x < ST) {
-        int i = r0 + threadIdx.x, j = c0 + threadIdx.x;
-        xr[threadIdx.x] = i < n ? x[i] : 0.0;
-        xc[threadIdx.x] = j < n ? x[j] : 0.0;
-        yr[threadIdx.x] = 0.0;
-        yc[threadIdx.x] = 0.0;
-    }
-    __syncthreads();
-    // each thread: rows r = threadIdx.x/4 (+64/4 stride)... simple scheme: thread handles one
-    // row segment of 16 columns
-    const int lr = threadIdx.x >> 2, q = threadIdx.x & 3;
-    for (int rr = lr; rr < ST; rr += 64) {
-        int i = r0 + rr;
-        if (i >= n) continue;
-        double sr = 0.0;
-        const int64_t rowoff = (int64_t)i * (i + 1) / 2; // 0-based row i start
-        for (int cc = q * 16; cc < q * 16 + 16; cc++) {
-            int j = c0 + cc;
-            if (j > i || j >= n) continue;
-            double aij = a[rowoff + j];
-            sr += aij * xc[cc];
-            if (j != i) atomicAdd(&yc[cc], aij * xr[rr]);
+    const int nb = (n + ST - 1) / ST;
+    const int b = nb - 1 - (int)blockIdx.x;         // the longest row blocks first
+    const int sg = blockIdx.y;
+    const int t0 = sg * SSEG, t1 = min(t0 + SSEG, b + 1);
+    if (t0 > b) return;
+    __shared__ double tl[ST][ST + 1];
+    __shared__ double xr[ST], xc[ST];
+    __shared__ double rp[4][ST];
+    const int r0 = b * ST;
+    const int tid = threadIdx.x;
+    if (tid < ST) xr[tid] = r0 + tid < n ? x[r0 + tid] : 0.0;
+    double rsum = 0.0;                  // thread (row = tid & 63, quarter = tid >> 6)
+    const int row = tid & (ST - 1), qt = tid >> 6;
+    double *zb = z + 32 * (int64_t)b * (b + 1);   // z block b: columns 0 .. 64 b + 63
+    for (int tc = t0; tc < t1; tc++) {
+        const int c0 = tc * ST;
+        __syncthreads();
+        // stage the tile: element (i, j) of rows r0.., columns c0.. (j <= i), zero elsewhere
+        for (int e = tid; e < ST * ST; e += 256) {
+            const int i = e >> 6, j = e & (ST - 1);
+            const int gi = r0 + i, gj = c0 + j;
+            tl[i][j] = (gi < n && gj <= gi) ? a[(int64_t)gi * (gi + 1) / 2 + gj] : 0.0;
         }
-        atomicAdd(&yr[rr], sr);
+        if (tid < ST) xc[tid] = c0 + tid < n ? x[c0 + tid] : 0.0;
+        __syncthreads();
+        // row products: row `row`, columns 16 qt .. 16 qt + 15
+        double sr = 0.0;
+#pragma unroll
+        for (int j = 16 * qt; j < 16 * qt + 16; j++) sr += tl[row][j] * xc[j];
+        rsum += sr;
+        // transposed products: column `row` of the tile, rows 16 qt .. 16 qt + 15, strictly below
+        // the diagonal of the matrix
+        double sc = 0.0;
+#pragma unroll
+        for (int i = 16 * qt; i < 16 * qt + 16; i++) sc += (r0 + i > c0 + row ? tl[i][row] : 0.0) * xr[i];
+        rp[qt][row] = sc;
+        __syncthreads();
+        if (tid < ST) zb[c0 + tid] = rp[0][tid] + rp[1][tid] + rp[2][tid] + rp[3][tid];
     }
     __syncthreads();
-    if (threadIdx.x < ST) {
-        int i = r0 + threadIdx.x, j = c0 + threadIdx.x;
-        if (i < n) atomicAdd(&y[i], yr[threadIdx.x]);
-        if (ti != tj && j < n) atomicAdd(&y[j], yc[threadIdx.x]);
-        else if (ti == tj && j < n) atomicAdd(&y[j], yc[threadIdx.x]);
-    }
+    rp[qt][row] = rsum;
+    __syncthreads();
+    if (tid < ST && r0 + tid < n) yr[(int64_t)sg * n + r0 + tid] = rp[0][tid] + rp[1][tid] + rp[2][tid] + rp[3][tid];
+}
+// y_i = sum_s yr[s][i] (segments s covering row block i/64) + sum_{b >= i/64} z[b][i]
+__global__ void k_psymv_sum(int n, const double *yr, const double *z, double *y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int nb = (n + ST - 1) / ST;
+    const int bi = i / ST;
+    double s = 0.0;
+    for (int sg = 0; sg * SSEG <= bi; sg++) s += yr[(int64_t)sg * n + i];
+    for (int b = bi; b < nb; b++) s += z[32 * (int64_t)b * (b + 1) + i];
+    y[i] = s;
+}
+
+int64_t packed_symv_scratch(int n)
+{
+    const int64_t nb = (n + ST - 1) / ST;
+    const int64_t nseg = (nb + SSEG - 1) / SSEG;
+    return nseg * n + 32 * nb * (nb + 1);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1478,13 +1506,14 @@ void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, 
 }
 
 __global__ void k_zero(int64_t n, double *y);
-void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream_t s)
+void l_packed_symv(int n, const double *a, const double *x, double *y, double *scratch, hipStream_t s)
 {
-    // cleared by a kernel (not a memset node), so that the product can be captured in a graph
-    hipLaunchKernelGGL(k_zero, dim3(nblk(n)), dim3(TPB), 0, s, (int64_t)n, y);
-    int64_t nt = (n + ST - 1) / ST;
-    int64_t tiles = nt * (nt + 1) / 2;
-    hipLaunchKernelGGL(k_packed_symv, dim3((unsigned)tiles), dim3(256), 0, s, n, a, x, y);
+    const int nb = (n + ST - 1) / ST;
+    const int nseg = (nb + SSEG - 1) / SSEG;
+    double *yr = scratch, *z = scratch + (int64_t)nseg * n;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_psymv_tiles, dim3(nb, nseg), dim3(256), 0, s, n, a, x, yr, z);
+    hipLaunchKernelGGL(k_psymv_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, yr, z, y);
 }
 __global__ void k_zero(int64_t n, double *y)
 {

@@ -293,6 +293,57 @@ int mpgen_brokenlines(uint64_t seed, int nlayers, int nx, int ny, double wx, dou
     });
 }
 
+// ---- dense detector: every track crosses every layer ----------------------------------------
+// nlayers layers at z = 0 .. nlayers-1, each a grid of nx * ny unit modules that is periodic in
+// both directions (a barrel unrolled in phi and, for the synthetic case, also closed along the
+// beam), 6 rigid-body globals per module (labels as mpgen_pixel).  A track is a straight line
+// through (x0, y0) at the middle layer with slopes uniform in +-slope_frac * nx / (nlayers - 1)
+// (and the same for y), so that across the whole stack it sweeps slope_frac of the ring: each
+// track gives nlayers hits (2 measurements each, 4 local parameters) and the module pairs it
+// couples fill a large part of the normal matrix.  truth: 6*nlayers*nx*ny values (may be NULL).
+int mpgen_dense(uint64_t seed, int nlayers, int nx, int ny, int64_t ntracks, double slope_frac, double sigma,
+                mpgen_records *out, double *truth)
+{
+    const int nmod = nx * ny;
+    std::vector<double> tr((size_t)nlayers * nmod * 6);
+    {
+        Rng rt(seed, 0xfffffffffull);
+        for (size_t k = 0; k < tr.size(); k++) tr[k] = ((k % 6) < 3 ? 0.01 : 1e-3) * rt.gauss();
+    }
+    if (truth) std::memcpy(truth, tr.data(), sizeof(double) * tr.size());
+    const double zmid = 0.5 * (nlayers - 1);
+    const double sx = nlayers > 1 ? slope_frac * nx / (nlayers - 1) : 0.0;
+    const double sy = nlayers > 1 ? slope_frac * ny / (nlayers - 1) : 0.0;
+    return make_records(ntracks, out, [&](int64_t t, Rec &rec) {
+        Rng r(seed, (uint64_t)t + 1);
+        const double x0 = nx * r.uni(), y0 = ny * r.uni();
+        const double tx = sx * (2.0 * r.uni() - 1.0), ty = sy * (2.0 * r.uni() - 1.0);
+        for (int l = 0; l < nlayers; l++) {
+            const double dz = l - zmid;
+            const double x = x0 + tx * dz, y = y0 + ty * dz;   // unwrapped track position
+            const double xw = x - nx * std::floor(x / nx), yw = y - ny * std::floor(y / ny);
+            int ix = std::min((int)xw, nx - 1), iy = std::min((int)yw, ny - 1);
+            const int m = iy * nx + ix;
+            const double u = xw - (ix + 0.5), v = yw - (iy + 0.5);
+            const double *p = &tr[((size_t)l * nmod + m) * 6];
+            double gu[6] = {1.0, 0.0, -tx, -tx * v, tx * u, -v};
+            double gv[6] = {0.0, 1.0, -ty, -ty * v, ty * u, u};
+            double mu = x, mv = y;
+            for (int k = 0; k < 6; k++) { mu += gu[k] * p[k]; mv += gv[k] * p[k]; }
+            mu += sigma * r.gauss();
+            mv += sigma * r.gauss();
+            int lab[6];
+            for (int k = 0; k < 6; k++) lab[k] = (l + 1) * 100000 + (m + 1) * 10 + k + 1;
+            float dgu[6], dgv[6];
+            for (int k = 0; k < 6; k++) { dgu[k] = (float)gu[k]; dgv[k] = (float)gv[k]; }
+            float dlu[2] = {1.0f, (float)dz}, dlv[2] = {1.0f, (float)dz};
+            int ilu[2] = {1, 3}, ilv[2] = {2, 4};
+            rec.meas((float)mu, dlu, ilu, 2, (float)sigma, dgu, lab, 6);
+            rec.meas((float)mv, dlv, ilv, 2, (float)sigma, dgv, lab, 6);
+        }
+    });
+}
+
 // write records as a Mille C-binary (fortran = 0) or Fortran sequential (fortran != 0) file
 int mpgen_write(const char *path, const mpgen_records *r, int fortran)
 {

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <limits>
 #include <numeric>
 #include <omp.h>
 #include <zlib.h>
@@ -80,6 +81,7 @@ struct mpgpu_ctx {
     DBuf<int32_t> d_bcf, d_cc_fn;   // per block / per column-list entry: first index and size of the other group
     DBuf<int32_t> d_cc_blk;         // per column-list entry: the block
     DBuf<double> d_tb;              // per off-diagonal block: its transposed product (8 values, groups <= 8)
+    DBuf<double> d_psymv;           // full storage: partial sums of the packed product (deterministic)
     DBuf<unsigned long long> d_cnt;
     DBuf<double> d_chi2;
     DBuf<int32_t> d_ndf;
@@ -169,6 +171,12 @@ extern "C" void mpgpu_default_options(mpgpu_options *o)
     o->mrtcnd = 1.0e7;
     o->nregul = 0;
     o->refine_steps = 20;
+    o->pair_entries = 1;
+    o->skip_empty_cons = 0;
+    o->resolve_redundancy_cons = 0;
+    o->scale_errors[0] = 1.0f;
+    o->scale_errors[1] = 1.0f;
+    o->refine_tol = 0.0;
 }
 
 extern "C" int mpgpu_create(const mpgpu_options *opt, int32_t device, mpgpu_ctx **out)
@@ -207,6 +215,24 @@ extern "C" const char *mpgpu_last_error(const mpgpu_ctx *c) { return c ? c->err.
 extern "C" int mpgpu_set_options(mpgpu_ctx *c, const mpgpu_options *o)
 {
     if (!c || !o) return 30;
+    if (c->built) {
+        // options the structure build consumed: the device matrix, the update buffers, the
+        // pre-weights and the monitoring histograms are sized and filled for them, so a change
+        // would make later passes index memory laid out for another storage
+        const mpgpu_options &b = c->opt;
+        const char *which = nullptr;
+        if (o->matsto != b.matsto) which = "matsto";
+        else if (o->withmultipliers != b.withmultipliers) which = "withmultipliers";
+        else if (o->mcount != b.mcount) which = "mcount";
+        else if (o->mreqenf != b.mreqenf) which = "mreqenf";
+        else if (o->monitor != b.monitor || o->monitor_bins != b.monitor_bins) which = "monitor";
+        else if (o->lfitbb != b.lfitbb) which = "lfitbb";
+        else if (o->default_presigma != b.default_presigma || o->regula != b.regula) which = "presigma/regula";
+        else if (o->scale_errors[0] != b.scale_errors[0] || o->scale_errors[1] != b.scale_errors[1]) which = "scaleerrors";
+        else if (o->pair_entries != b.pair_entries) which = "pairentries";
+        if (which)
+            return fail(c, 30, std::string("option ") + which + " is fixed once the structure is built (rebuild the context to change it)");
+    }
     c->opt = *o;
     return 0;
 }
@@ -1179,6 +1205,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->d_status.alloc(std::max<int64_t>(1, nrec));
     c->d_part.alloc(2048);
     c->d_scal.alloc(16);
+    if (!sparse) c->d_psymv.alloc(packed_symv_scratch(c->nall));
     c->d_ndfsum.alloc(1);
     c->d_hist.alloc(24);
     if (c->ncon > 0) {
@@ -1562,7 +1589,7 @@ void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
                      c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, c->plan.max_grp, c->d_cc_blk.p, c->d_tb.p, s);
         l_cons_symv(c->ncon, c->nvar, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, x, y, s);
     } else {
-        l_packed_symv(c->nall, c->d_mat.p, x, y, s);
+        l_packed_symv(c->nall, c->d_mat.p, x, y, c->d_psymv.p, s);
     }
 }
 
@@ -2147,6 +2174,343 @@ void elim_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
 
 } // namespace
 
+// ---------------------------------------------------------------------------------------------
+// dense direct solves: 'decomposition' (chdec2/chslv2 semantics), the LAPACK methods and the
+// FP32 factor with FP64 refinement
+// ---------------------------------------------------------------------------------------------
+namespace {
+struct DenseFactor {
+    int n = 0;
+    bool rev = false;    // factor of the index-reversed matrix (chdec2's pivot order)
+    bool fp32 = false;
+    DBuf<double> a, w, work;
+    DBuf<float> a32, w32, work32;
+    std::vector<double> piv;
+    // v <- A^-1 v for nrhs columns of length n (leading dimension n) with the FP64 factor
+    void solve64(double *v, int nrhs, hipStream_t s)
+    {
+        if (rev) l_reverse(n, v, nrhs, n, s);
+        ldl_solve(n, a.p, n, w.p, work.p, v, nrhs, n, s);
+        if (rev) l_reverse(n, v, nrhs, n, s);
+    }
+    void solve32(double *v, int nrhs, hipStream_t s)
+    {
+        if (rev) l_reverse(n, v, nrhs, n, s);
+        ldl_solve32(n, a32.p, n, w32.p, work32.p, v, nrhs, n, s);
+        if (rev) l_reverse(n, v, nrhs, n, s);
+    }
+};
+} // namespace
+
+static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats &loc)
+{
+    hipStream_t s = c->stream;
+    const int64_t n = c->nall;
+    const int nv = c->nvar, m = c->ncon;
+    const bool lapack = c->opt.metsol == MPGPU_LAPACK_PACKED || c->opt.metsol == MPGPU_LAPACK_UNPACKED;
+    // 'LAPACKwitherrors': the parameter covariance on the constraint surface, as pede's
+    // inverse from the factor expanded by Q (src/pede.f90:9017-9120); with multipliers the
+    // same matrix is diag(A^-1) - diag(W S^-1 W^T), W = A^-1 C^T, S = C W, and A may carry
+    // the rho C^T C term below (it cancels on the constraint surface)
+    const bool want_cov = c->opt.lapack_errors && lapack;
+    c->have_cov = false;
+    std::vector<double> covcorr(want_cov ? nv : 0, 0.0);
+    // With constraints the parameter block alone may be singular (modes fixed only by the
+    // constraints, as pede's decomposition removes them by elimination).  The block is
+    // augmented to A + rho C^T C and the right-hand side to b + rho C^T d: on the constraint
+    // surface C x = d this changes neither x nor the multipliers, and the augmented block is
+    // positive definite whenever the constrained problem is well posed.
+    double rho = 0.0, cc_norm = 0.0;
+    if (m > 0) {
+        DBuf<double> dg;
+        dg.alloc(nv);
+        l_get_diag(model_of(c), nv, dg.p, s);
+        std::vector<double> hd(nv), cc(nv, 0.0);
+        dg.download(hd.data(), nv, s);
+        HIPCHK(hipStreamSynchronize(s));
+        for (int j = 0; j < m; j++) {
+            double rn = 0.0;
+            for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) {
+                cc[c->cidx[q] - 1] += c->cval[q] * c->cval[q];
+                rn += c->cval[q] * c->cval[q];
+            }
+            cc_norm += rn;   // ||c_j c_j^T||_F = ||c_j||^2
+        }
+        double amax = 0.0, cmax = 0.0;
+        for (int i = 0; i < nv; i++) { amax = std::max(amax, std::fabs(hd[i])); cmax = std::max(cmax, cc[i]); }
+        rho = cmax > 0.0 ? (amax > 0.0 ? amax : 1.0) / cmax : 0.0;
+    }
+    DenseFactor F;
+    F.n = nv;
+    // pede's decomposition (chdec2) eliminates from the last parameter; LAPACK's dpptrf from the
+    // first, and it stops at the first non-positive leading minor
+    F.rev = !lapack;
+    const int64_t nn = (int64_t)nv * nv;
+    auto pivots_to_host = [&](bool f32) {
+        DBuf<double> d;
+        d.alloc(nv);
+        if (f32) ldl_pivots32(nv, F.a32.p, nv, d.p, s); else ldl_pivots(nv, F.a.p, nv, d.p, s);
+        F.piv.assign(nv, 0.0);
+        d.download(F.piv.data(), nv, s);
+        HIPCHK(hipStreamSynchronize(s));
+    };
+    auto factor64 = [&]() {
+        F.fp32 = false;
+        F.a32.release(); F.w32.release(); F.work32.release();
+        F.a.alloc(nn);
+        F.w.alloc(nv);
+        F.work.alloc(ldl_work_elems(nv));
+        l_packed_to_full(nv, c->d_mat.p, F.a.p, nv, F.rev, s);
+        l_add_cct(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, F.a.p, nv, nv, F.rev, s);
+        HIPCHK(ldl_factor(nv, F.a.p, nv, F.w.p, F.work.p, s));
+        pivots_to_host(false);
+        int nrank = 0;
+        double evmax = 0.0, evmin = 0.0;
+        for (int k = 0; k < nv; k++) {
+            const double d = F.piv[k];
+            if (d > 0.0) {
+                if (nrank == 0) evmax = evmin = d;
+                evmax = std::max(evmax, d);
+                evmin = std::min(evmin, d);
+                nrank++;
+            } else if (lapack) {
+                // dpptrf: 'factorization ... failed at index', exit 29 (src/mplapack.f90:144-156)
+                throw MpError(29, "Aborted, factorization of global matrix failed at index " + std::to_string(k + 1));
+            }
+        }
+        loc.nrank = nrank;
+        loc.evmax = evmax;
+        loc.evmin = evmin;
+    };
+    bool fp32 = c->opt.metsol == MPGPU_CHOLESKY_FP32_IR;
+    if (fp32) {
+        F.fp32 = true;
+        F.a32.alloc(nn);
+        F.w32.alloc(nv);
+        F.work32.alloc(ldl_work_elems(nv));
+        l_packed_to_full32(nv, c->d_mat.p, F.a32.p, nv, F.rev, s);
+        l_add_cct32(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, F.a32.p, nv, nv, F.rev, s);
+        HIPCHK(ldl_factor32(nv, F.a32.p, nv, F.w32.p, F.work32.p, s));
+        pivots_to_host(true);
+        bool ok = true;
+        double evmax = 0.0, evmin = DBL_MAX;
+        for (int k = 0; k < nv; k++) {
+            const double d = F.piv[k];
+            if (!(d > 0.0) || !std::isfinite(d)) { ok = false; break; }
+            evmax = std::max(evmax, d);
+            evmin = std::min(evmin, d);
+        }
+        if (ok) {
+            loc.nrank = nv;
+            loc.evmax = evmax;
+            loc.evmin = evmin;
+        } else {
+            // the FP32 factor is not positive definite: it cannot precondition CG
+            fp32 = false;
+            loc.method = MPGPU_DECOMPOSITION;
+            loc.istop = 3;
+            factor64();
+        }
+    } else {
+        factor64();
+    }
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double tol = c->opt.refine_tol > 0.0 ? c->opt.refine_tol : eps * std::sqrt((double)nv);
+    double anorm = 0.0;
+    if (fp32) anorm = std::sqrt(l_packed_fro2(nv, c->d_mat.p, c->d_part.p, c->d_scal.p, s)) + rho * cc_norm;
+
+    // FP32-factor-preconditioned conjugate gradients on the (augmented) parameter block, nrhs
+    // columns side by side with their own recurrences.  A column has converged when
+    // ||r|| <= tol ||x|| ||A||_F (the backward-error test of LAPACK's mixed-precision Cholesky
+    // solver DSPOSV, in 2-norms).  Returns false when a column did not converge within
+    // refine_steps steps.
+    auto refine = [&](double *v, int nrhs) -> bool {
+        const int64_t len = (int64_t)nv * nrhs;
+        DBuf<double> X, R, Z, P, Q, sc;
+        X.alloc(len); R.alloc(len); Z.alloc(len); P.alloc(len); Q.alloc(len);
+        sc.alloc(4 * (size_t)nrhs);
+        std::vector<double> bb(nrhs), rz(nrhs), pq(nrhs), rr(nrhs), xx(nrhs), coef(nrhs);
+        std::vector<char> active(nrhs, 1), conv(nrhs, 0);
+        auto dots = [&](const double *A, const double *B, std::vector<double> &out) {
+            l_col_dots(nv, nrhs, A, B, sc.p, s);
+            sc.download(out.data(), nrhs, s);
+            HIPCHK(hipStreamSynchronize(s));
+        };
+        l_zero(len, X.p, s);
+        l_copy(len, v, R.p, s);
+        dots(R.p, R.p, bb);
+        for (int q = 0; q < nrhs; q++) if (!(bb[q] > 0.0)) { active[q] = 0; conv[q] = 1; }
+        l_copy(len, R.p, Z.p, s);
+        F.solve32(Z.p, nrhs, s);
+        l_copy(len, Z.p, P.p, s);
+        dots(R.p, Z.p, rz);
+        const int maxit = std::max(1, c->opt.refine_steps);
+        for (int it = 1; it <= maxit; it++) {
+            int nact = 0;
+            for (int q = 0; q < nrhs; q++) {
+                if (!active[q]) continue;
+                nact++;
+                l_packed_symv(nv, c->d_mat.p, P.p + (int64_t)q * nv, Q.p + (int64_t)q * nv, c->d_psymv.p, s);
+                l_cct_apply(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, P.p + (int64_t)q * nv, Q.p + (int64_t)q * nv, s);
+            }
+            if (nact == 0) break;
+            dots(P.p, Q.p, pq);
+            for (int q = 0; q < nrhs; q++) {
+                coef[q] = 0.0;
+                if (!active[q]) continue;
+                if (!(pq[q] > 0.0)) { active[q] = 0; continue; }   // breakdown: not converged
+                coef[q] = rz[q] / pq[q];
+            }
+            sc.upload(coef.data(), nrhs, s);
+            l_col_axpy(nv, nrhs, sc.p, 1.0, P.p, X.p, s);     // x += alpha p
+            l_col_axpy(nv, nrhs, sc.p, -1.0, Q.p, R.p, s);    // r -= alpha q
+            loc.itn = std::max(loc.itn, it);
+            dots(R.p, R.p, rr);
+            dots(X.p, X.p, xx);
+            int left = 0;
+            for (int q = 0; q < nrhs; q++) {
+                if (!active[q]) continue;
+                if (std::sqrt(rr[q]) <= tol * anorm * std::sqrt(xx[q])) { active[q] = 0; conv[q] = 1; }
+                else left++;
+            }
+            if (left == 0 || it == maxit) break;
+            l_copy(len, R.p, Z.p, s);
+            F.solve32(Z.p, nrhs, s);
+            std::vector<double> rz2(nrhs);
+            dots(R.p, Z.p, rz2);
+            for (int q = 0; q < nrhs; q++) coef[q] = active[q] && rz[q] != 0.0 ? rz2[q] / rz[q] : 0.0;
+            sc.upload(coef.data(), nrhs, s);
+            l_col_xpby(nv, nrhs, Z.p, sc.p, P.p, s);           // p = z + beta p
+            rz = rz2;
+        }
+        l_copy(len, X.p, v, s);
+        for (int q = 0; q < nrhs; q++) if (!conv[q]) return false;
+        return true;
+    };
+    // A v = rhs for nrhs columns: FP64 factor, or FP32-preconditioned CG with the FP64 factor
+    // as the fallback when it does not converge
+    auto solveA = [&](double *v, int nrhs) {
+        if (!fp32) { F.solve64(v, nrhs, s); return; }
+        DBuf<double> keep;
+        keep.alloc((size_t)nv * nrhs);
+        l_copy((int64_t)nv * nrhs, v, keep.p, s);
+        if (refine(v, nrhs)) { loc.istop = 1; return; }
+        fp32 = false;
+        loc.method = MPGPU_DECOMPOSITION;
+        loc.istop = 3;
+        factor64();
+        l_copy((int64_t)nv * nrhs, keep.p, v, s);
+        F.solve64(v, nrhs, s);
+    };
+
+    HIPCHK(hipMemcpyAsync(x_dev, b_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (m == 0) {
+        solveA(x_dev, 1);
+    } else {
+        // Z = A^-1 C^T (m columns), S = C Z, lambda = S^-1 (C A^-1 b_p - b_c), x = A^-1 (b_p - C^T lambda)
+        std::vector<double> bp(n);
+        HIPCHK(hipMemcpyAsync(bp.data(), b_dev, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        // right-hand sides: the m columns of C^T and b + rho C^T d, solved together
+        std::vector<double> Z((size_t)(m + 1) * nv, 0.0);
+        for (int j = 0; j < m; j++)
+            for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) Z[(size_t)j * nv + c->cidx[q] - 1] += c->cval[q];
+        double *bt = Z.data() + (size_t)m * nv;
+        for (int i = 0; i < nv; i++) bt[i] = bp[i];
+        for (int j = 0; j < m; j++)
+            for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) bt[c->cidx[q] - 1] += rho * c->cval[q] * bp[nv + j];
+        DBuf<double> zc;
+        zc.upload(Z, s);
+        solveA(zc.p, m + 1);
+        zc.download(Z.data(), Z.size(), s);
+        HIPCHK(hipStreamSynchronize(s));
+        const double *y = Z.data() + (size_t)m * nv;
+        // S (m x m) and rhs
+        std::vector<double> S((size_t)m * m), g(m);
+        for (int i = 0; i < m; i++) {
+            for (int j = 0; j < m; j++) {
+                double acc = 0.0;
+                for (int64_t q = c->cptr[i]; q < c->cptr[i + 1]; q++) acc += c->cval[q] * Z[(size_t)j * nv + c->cidx[q] - 1];
+                S[(size_t)i * m + j] = acc;
+            }
+            double acc = 0.0;
+            for (int64_t q = c->cptr[i]; q < c->cptr[i + 1]; q++) acc += c->cval[q] * y[c->cidx[q] - 1];
+            g[i] = acc - bp[nv + i];
+        }
+        if (want_cov) {
+            // S = R R^T (host Cholesky, m small); diag(W S^-1 W^T)_i = |R^-1 w_i|^2
+            std::vector<double> R((size_t)m * m, 0.0), yv(m);
+            for (int j = 0; j < m; j++) {
+                double d = S[(size_t)j * m + j];
+                for (int k = 0; k < j; k++) d -= R[(size_t)j * m + k] * R[(size_t)j * m + k];
+                if (!(d > 0.0)) throw MpError(29, "constraint matrix C A^-1 C^T not positive definite");
+                R[(size_t)j * m + j] = std::sqrt(d);
+                for (int i = j + 1; i < m; i++) {
+                    double e = S[(size_t)i * m + j];
+                    for (int k = 0; k < j; k++) e -= R[(size_t)i * m + k] * R[(size_t)j * m + k];
+                    R[(size_t)i * m + j] = e / R[(size_t)j * m + j];
+                }
+            }
+            for (int i = 0; i < nv; i++) {
+                double acc2 = 0.0;
+                for (int j = 0; j < m; j++) {
+                    double e = Z[(size_t)j * nv + i];
+                    for (int k = 0; k < j; k++) e -= R[(size_t)j * m + k] * yv[k];
+                    yv[j] = e / R[(size_t)j * m + j];
+                    acc2 += yv[j] * yv[j];
+                }
+                covcorr[i] = acc2;
+            }
+        }
+        // dense symmetric solve of S lambda = g (Gaussian elimination with partial pivoting, m small)
+        std::vector<double> Aug(S);
+        for (int k = 0; k < m; k++) {
+            int p = k;
+            for (int i = k + 1; i < m; i++) if (std::fabs(Aug[(size_t)i * m + k]) > std::fabs(Aug[(size_t)p * m + k])) p = i;
+            if (Aug[(size_t)p * m + k] == 0.0) throw MpError(29, "singular constraint system");
+            if (p != k) { for (int j = 0; j < m; j++) std::swap(Aug[(size_t)k * m + j], Aug[(size_t)p * m + j]); std::swap(g[k], g[p]); }
+            for (int i = k + 1; i < m; i++) {
+                double f = Aug[(size_t)i * m + k] / Aug[(size_t)k * m + k];
+                for (int j = k; j < m; j++) Aug[(size_t)i * m + j] -= f * Aug[(size_t)k * m + j];
+                g[i] -= f * g[k];
+            }
+        }
+        std::vector<double> lam(m);
+        for (int i = m - 1; i >= 0; i--) {
+            double acc = g[i];
+            for (int j = i + 1; j < m; j++) acc -= Aug[(size_t)i * m + j] * lam[j];
+            lam[i] = acc / Aug[(size_t)i * m + i];
+        }
+        std::vector<double> xs(n);
+        for (int i = 0; i < nv; i++) {
+            double acc = y[i];
+            for (int j = 0; j < m; j++) acc -= Z[(size_t)j * nv + i] * lam[j];
+            xs[i] = acc;
+        }
+        for (int j = 0; j < m; j++) xs[nv + j] = lam[j];
+        HIPCHK(hipMemcpyAsync(x_dev, xs.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    loc.rel_residual = rel_residual(c, b_dev, x_dev);
+    if (want_cov) {
+        const int nbk = std::min(nv, 512);
+        DBuf<double> E, dd, dg;
+        E.alloc((size_t)nv * nbk);
+        dd.alloc(nv);
+        dg.alloc(nv);
+        ldl_inv_diag(nv, F.a.p, nv, F.w.p, F.work.p, E.p, nbk, dd.p, s);
+        l_get_diag(model_of(c), nv, dg.p, s);
+        c->cov_diag.assign(nv, 0.0);
+        c->diag_before.assign(n, 0.0);
+        dd.download(c->cov_diag.data(), nv, s);
+        dg.download(c->diag_before.data(), nv, s);
+        HIPCHK(hipStreamSynchronize(s));
+        if (F.rev) std::reverse(c->cov_diag.begin(), c->cov_diag.end());
+        for (int i = 0; i < nv; i++) c->cov_diag[i] -= covcorr[i];
+        c->have_cov = true;
+    }
+}
+
 extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_solve_stats *st)
 {
     if (!c || !c->built) return fail(c, 30, "structure not built");
@@ -2165,7 +2529,7 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
         throw MpError(36, "structure was built for constraint elimination (MINRES methods); rebuild it for this method");
     if (c->opt.metsol == MPGPU_INVERSION && c->have_inverse) {
         // matrix already inverted by an earlier solve: x = A^-1 b (dbsvxl, src/mpnum.f90:1308)
-        l_packed_symv((int)n, c->d_mat.p, b.p, xd.p, s);
+        l_packed_symv((int)n, c->d_mat.p, b.p, xd.p, c->d_psymv.p, s);
         loc.nrank = -1;
         loc.rel_residual = -1.0;
     } else if (c->opt.metsol == MPGPU_INVERSION) {
@@ -2215,229 +2579,7 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
     } else if (c->opt.metsol == MPGPU_DECOMPOSITION || c->opt.metsol == MPGPU_CHOLESKY_FP32_IR ||
                c->opt.metsol == MPGPU_LAPACK_PACKED || c->opt.metsol == MPGPU_LAPACK_UNPACKED) {
         if (c->opt.matsto == MPGPU_SPARSE) throw MpError(36, "this method needs full matrix storage");
-        const int nv = c->nvar, m = c->ncon;
-        // 'LAPACKwitherrors': the parameter covariance on the constraint surface, as pede's
-        // inverse from the factor expanded by Q (src/pede.f90:9017-9120); with multipliers the
-        // same matrix is diag(A^-1) - diag(W S^-1 W^T), W = A^-1 C^T, S = C W, and A may carry
-        // the rho C^T C term below (it cancels on the constraint surface)
-        const bool want_cov = c->opt.lapack_errors &&
-                              (c->opt.metsol == MPGPU_LAPACK_PACKED || c->opt.metsol == MPGPU_LAPACK_UNPACKED);
-        c->have_cov = false;
-        std::vector<double> covcorr(want_cov ? nv : 0, 0.0);
-        const int64_t lda = nv;
-        DBuf<double> A, work;
-        DBuf<int> info;
-        A.alloc((size_t)lda * nv);
-        work.alloc(1 << 20);
-        info.alloc(4);
-        // parameter block of the KKT matrix (leading nv x nv of the packed matrix)
-        l_packed_to_full(nv, c->d_mat.p, A.p, (int)lda, s);
-        // With constraints the parameter block alone may be singular (modes fixed only by the
-        // constraints, as pede's decomposition removes them by elimination).  The block is
-        // augmented to A + rho C^T C and the right-hand side to b + rho C^T d: on the constraint
-        // surface C x = d this changes neither x nor the multipliers, and the augmented block is
-        // positive definite whenever the constrained problem is well posed.
-        double rho = 0.0;
-        if (m > 0) {
-            DBuf<double> dg;
-            dg.alloc(nv);
-            l_get_diag(model_of(c), nv, dg.p, s);
-            std::vector<double> hd(nv), cc(nv, 0.0);
-            dg.download(hd.data(), nv, s);
-            HIPCHK(hipStreamSynchronize(s));
-            for (int j = 0; j < m; j++)
-                for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) cc[c->cidx[q] - 1] += c->cval[q] * c->cval[q];
-            double amax = 0.0, cmax = 0.0;
-            for (int i = 0; i < nv; i++) { amax = std::max(amax, std::fabs(hd[i])); cmax = std::max(cmax, cc[i]); }
-            rho = cmax > 0.0 ? (amax > 0.0 ? amax : 1.0) / cmax : 0.0;
-            l_add_cct(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, A.p, (int)lda, s);
-        }
-        bool fp32 = c->opt.metsol == MPGPU_CHOLESKY_FP32_IR;
-        DBuf<float> A32;
-        DBuf<float> xf;
-        if (fp32) { A32.alloc((size_t)lda * nv); xf.alloc(nv); }
-        int rc = chol_factor(nv, A.p, (int)lda, work.p, info.p, s, fp32 ? 1 : 0, fp32 ? A32.p : nullptr);
-        if (rc == 0 && fp32) {
-            // (max/min diagonal of the factor)^2 bounds the condition number from below; beyond
-            // 1e4 the FP32 factor preconditions FP64 conjugate gradients poorly (the error
-            // shrinks by ~eps32*cond per step at best)
-            chol_diag32(nv, A32.p, (int)lda, xf.p, s);
-            std::vector<float> dg(nv);
-            HIPCHK(hipMemcpyAsync(dg.data(), xf.p, sizeof(float) * nv, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            double dmax = 0.0, dmin = DBL_MAX;
-            for (float d : dg) { dmax = std::max(dmax, (double)std::fabs(d)); dmin = std::min(dmin, (double)std::fabs(d)); }
-            if (!(dmin > 0.0) || (dmax / dmin) * (dmax / dmin) > 1e4) rc = -1;
-        }
-        if (rc != 0 && fp32) {
-            // FP32 lacks the range for this system (a pivot vanished, or the factor would need
-            // many refinement steps): the FP64 block, still intact, is factorised in FP64
-            // instead and the solve is a direct one
-            fp32 = false;
-            loc.method = MPGPU_DECOMPOSITION;
-            rc = chol_factor(nv, A.p, (int)lda, work.p, info.p, s, 0, nullptr);
-        }
-        if (rc > 0) throw MpError(29, "Aborted, matrix factorisation failed (not positive definite at " + std::to_string(rc) + ")");
-        loc.nrank = nv;
-        // solve with constraints via the Schur complement S = C A^-1 C^T (block elimination)
-        auto solveA = [&](double *v) {
-            if (!fp32) { chol_solve(nv, A.p, (int)lda, v, s); return; }
-            // FP32 factor as the preconditioner of FP64 conjugate gradients on the (augmented)
-            // parameter block: each step is one FP64 matrix-vector product and one FP32
-            // triangular solve pair; stops at ||r|| <= 1e-14 ||b|| or after refine_steps steps
-            DBuf<double> r, z, p, q, xacc;
-            r.alloc(nv); z.alloc(nv); p.alloc(nv); q.alloc(nv); xacc.alloc(nv);
-            auto apply = [&](const double *in, double *out) {
-                l_packed_symv(nv, c->d_mat.p, in, out, s);
-                l_cct_apply(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, in, out, s);
-            };
-            auto precond = [&](const double *in, double *out) {
-                l_copy(nv, in, out, s);
-                chol_solve32(nv, A32.p, (int)lda, out, xf.p, s);
-            };
-            xacc.zero(s);
-            l_copy(nv, v, r.p, s);
-            const double bb = dot_dev(c, nv, v, v);
-            if (bb == 0.0) { l_copy(nv, xacc.p, v, s); return; }
-            precond(r.p, z.p);
-            l_copy(nv, z.p, p.p, s);
-            double rz = dot_dev(c, nv, r.p, z.p);
-            const int maxit = std::max(1, c->opt.refine_steps);
-            for (int it = 1; it <= maxit; it++) {
-                apply(p.p, q.p);
-                const double pq = dot_dev(c, nv, p.p, q.p);
-                if (!(pq > 0.0)) break;
-                const double alpha = rz / pq;
-                l_axpy_neg(nv, -alpha, p.p, xacc.p, s);   // x += alpha p
-                l_axpy_neg(nv, alpha, q.p, r.p, s);       // r -= alpha q
-                loc.itn = std::max(loc.itn, it);
-                const double rr = dot_dev(c, nv, r.p, r.p);
-                if (std::sqrt(rr / bb) <= 1e-14) break;
-                precond(r.p, z.p);
-                const double rz2 = dot_dev(c, nv, r.p, z.p);
-                l_xpby(nv, z.p, rz2 / rz, p.p, s);         // p = z + beta p
-                rz = rz2;
-            }
-            l_copy(nv, xacc.p, v, s);
-        };
-        HIPCHK(hipMemcpyAsync(xd.p, b.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        if (m == 0) {
-            solveA(xd.p);
-        } else {
-            // Z = A^-1 C^T (m columns), S = C Z, lambda = S^-1 (C A^-1 b_p - b_c), x = A^-1 (b_p - C^T lambda)
-            std::vector<double> Ct((size_t)m * nv, 0.0);
-            for (int j = 0; j < m; j++)
-                for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) Ct[(size_t)j * nv + c->cidx[q] - 1] += c->cval[q];
-            std::vector<double> bp(n);
-            b.download(bp.data(), n, s);
-            HIPCHK(hipStreamSynchronize(s));
-            // right-hand sides: the m columns of C^T and b + rho C^T d, solved together by the
-            // FP64 factor (the FP32 path refines each one)
-            std::vector<double> Z(Ct);
-            Z.resize((size_t)(m + 1) * nv);
-            double *bt = Z.data() + (size_t)m * nv;
-            for (int i = 0; i < nv; i++) bt[i] = bp[i];
-            for (int j = 0; j < m; j++)
-                for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) bt[c->cidx[q] - 1] += rho * c->cval[q] * bp[nv + j];
-            DBuf<double> zc;
-            if (!fp32) {
-                zc.upload(Z, s);
-                chol_solve_multi(nv, A.p, (int)lda, zc.p, m + 1, nv, s);
-                zc.download(Z.data(), Z.size(), s);
-            } else {
-                zc.alloc(nv);
-                for (int j = 0; j <= m; j++) {
-                    zc.upload(Z.data() + (size_t)j * nv, nv, s);
-                    solveA(zc.p);
-                    zc.download(Z.data() + (size_t)j * nv, nv, s);
-                }
-            }
-            HIPCHK(hipStreamSynchronize(s));
-            const double *y = Z.data() + (size_t)m * nv;
-            // S (m x m) and rhs
-            std::vector<double> S((size_t)m * m), g(m);
-            for (int i = 0; i < m; i++) {
-                for (int j = 0; j < m; j++) {
-                    double acc = 0.0;
-                    for (int64_t q = c->cptr[i]; q < c->cptr[i + 1]; q++) acc += c->cval[q] * Z[(size_t)j * nv + c->cidx[q] - 1];
-                    S[(size_t)i * m + j] = acc;
-                }
-                double acc = 0.0;
-                for (int64_t q = c->cptr[i]; q < c->cptr[i + 1]; q++) acc += c->cval[q] * y[c->cidx[q] - 1];
-                g[i] = acc - bp[nv + i];
-            }
-            if (want_cov) {
-                // S = R R^T (host Cholesky, m small); diag(W S^-1 W^T)_i = |R^-1 w_i|^2
-                std::vector<double> R((size_t)m * m, 0.0), y(m);
-                for (int j = 0; j < m; j++) {
-                    double d = S[(size_t)j * m + j];
-                    for (int k = 0; k < j; k++) d -= R[(size_t)j * m + k] * R[(size_t)j * m + k];
-                    if (!(d > 0.0)) throw MpError(29, "constraint matrix C A^-1 C^T not positive definite");
-                    R[(size_t)j * m + j] = std::sqrt(d);
-                    for (int i = j + 1; i < m; i++) {
-                        double e = S[(size_t)i * m + j];
-                        for (int k = 0; k < j; k++) e -= R[(size_t)i * m + k] * R[(size_t)j * m + k];
-                        R[(size_t)i * m + j] = e / R[(size_t)j * m + j];
-                    }
-                }
-                for (int i = 0; i < nv; i++) {
-                    double acc2 = 0.0;
-                    for (int j = 0; j < m; j++) {
-                        double e = Z[(size_t)j * nv + i];
-                        for (int k = 0; k < j; k++) e -= R[(size_t)j * m + k] * y[k];
-                        y[j] = e / R[(size_t)j * m + j];
-                        acc2 += y[j] * y[j];
-                    }
-                    covcorr[i] = acc2;
-                }
-            }
-            // dense symmetric solve of S lambda = g (Gaussian elimination with partial pivoting, m small)
-            std::vector<double> Aug(S);
-            std::vector<int> piv(m);
-            for (int i = 0; i < m; i++) piv[i] = i;
-            for (int k = 0; k < m; k++) {
-                int p = k;
-                for (int i = k + 1; i < m; i++) if (std::fabs(Aug[(size_t)i * m + k]) > std::fabs(Aug[(size_t)p * m + k])) p = i;
-                if (Aug[(size_t)p * m + k] == 0.0) throw MpError(29, "singular constraint system");
-                if (p != k) { for (int j = 0; j < m; j++) std::swap(Aug[(size_t)k * m + j], Aug[(size_t)p * m + j]); std::swap(g[k], g[p]); }
-                for (int i = k + 1; i < m; i++) {
-                    double f = Aug[(size_t)i * m + k] / Aug[(size_t)k * m + k];
-                    for (int j = k; j < m; j++) Aug[(size_t)i * m + j] -= f * Aug[(size_t)k * m + j];
-                    g[i] -= f * g[k];
-                }
-            }
-            std::vector<double> lam(m);
-            for (int i = m - 1; i >= 0; i--) {
-                double acc = g[i];
-                for (int j = i + 1; j < m; j++) acc -= Aug[(size_t)i * m + j] * lam[j];
-                lam[i] = acc / Aug[(size_t)i * m + i];
-            }
-            std::vector<double> xs(n);
-            for (int i = 0; i < nv; i++) {
-                double acc = y[i];
-                for (int j = 0; j < m; j++) acc -= Z[(size_t)j * nv + i] * lam[j];
-                xs[i] = acc;
-            }
-            for (int j = 0; j < m; j++) xs[nv + j] = lam[j];
-            xd.upload(xs.data(), n, s);
-        }
-        loc.rel_residual = rel_residual(c, b.p, xd.p);
-        if (want_cov) {
-            const int nbk = std::min(nv, 512);
-            DBuf<double> E, dd, dg;
-            E.alloc((size_t)nv * nbk);
-            dd.alloc(nv);
-            dg.alloc(nv);
-            chol_inv_diag(nv, A.p, (int)lda, E.p, nbk, dd.p, s);
-            l_get_diag(model_of(c), nv, dg.p, s);
-            c->cov_diag.assign(nv, 0.0);
-            c->diag_before.assign(n, 0.0);
-            dd.download(c->cov_diag.data(), nv, s);
-            dg.download(c->diag_before.data(), nv, s);
-            HIPCHK(hipStreamSynchronize(s));
-            for (int i = 0; i < nv; i++) c->cov_diag[i] -= covcorr[i];
-            c->have_cov = true;
-        }
+        dense_solve(c, b.p, xd.p, loc);
     } else {
         throw MpError(36, "Unsupported solution method");
     }

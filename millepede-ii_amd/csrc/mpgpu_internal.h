@@ -105,7 +105,9 @@ void l_pass_stats(int64_t n, const double *chi2, const int32_t *ndf, const int8_
 void l_gj_init(int n, const double *v, double *diag, int *used, int *info, hipStream_t s);
 void l_gj_step(int n, double *v, const double *diag, int *used, int *info, double *vk, double *b, hipStream_t s);
 void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, hipStream_t s);
-void l_packed_symv(int n, const double *a, const double *x, double *y, hipStream_t s);
+// y = A x for the packed symmetric matrix; scratch: packed_symv_scratch(n) doubles
+void l_packed_symv(int n, const double *a, const double *x, double *y, double *scratch, hipStream_t s);
+int64_t packed_symv_scratch(int n);
 void l_mr_scale(int64_t n, const MrState *st, const double *y, double *v, hipStream_t s);
 void l_mr_axpy(int64_t n, const MrState *st, int which, const double *r, double *y, hipStream_t s);
 void l_mr_step(MrState *st, hipStream_t s);
@@ -146,21 +148,36 @@ void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, con
 int spk_scan_levels(int n, int w);
 void l_spk_scan_build(int n, int w, const double *Sf, const double *Sb, double *Af, double *Ab, int nlev, hipStream_t s);
 
-// dense Cholesky (k_chol.hip): full column-major n x n matrix (lda = n)
-int chol_factor(int n, double *a, int lda, double *work, int *info_dev, hipStream_t s, int fp32, float *a32);
-void chol_solve(int n, const double *a, int lda, double *x, hipStream_t s);
-void chol_solve32(int n, const float *a, int lda, double *x, float *xf, hipStream_t s);
-void chol_solve_multi(int n, const double *a, int lda, double *x, int nrhs, int64_t ldx, hipStream_t s);
-// diagonal of A^-1 from the FP64 factor; e: scratch of n * nbk doubles, d: n
-void chol_inv_diag(int n, const double *a, int lda, double *e, int nbk, double *d, hipStream_t s);
-void chol_diag32(int n, const float *a, int lda, float *d, hipStream_t s);
-void l_packed_to_full(int n, const double *packed, double *full, int lda, hipStream_t s);
-void l_full_to_f32(int64_t nn, const double *a, float *b, hipStream_t s);
+// dense LDL^T (k_chol.hip): full column-major n x n matrix (lda = n), lower triangle.  The
+// factor keeps the raw pivots on the diagonal, w_i = 1/d_i (d_i > 0) or d_i in w, and needs
+// ldl_work_elems(n) elements of work (panel buffer and the inverse diagonal blocks, used again
+// by the solves).
+int64_t ldl_work_elems(int n);
+hipError_t ldl_factor(int n, double *a, int lda, double *w, double *work, hipStream_t s);
+hipError_t ldl_factor32(int n, float *a, int lda, float *w, float *work, hipStream_t s);
+void ldl_solve(int n, const double *a, int lda, const double *w, const double *work, double *x, int nrhs, int64_t ldx,
+               hipStream_t s);
+void ldl_solve32(int n, const float *a, int lda, const float *w, const float *work, double *x, int nrhs, int64_t ldx,
+                 hipStream_t s);
+void ldl_pivots(int n, const double *a, int lda, double *d, hipStream_t s);
+void ldl_pivots32(int n, const float *a, int lda, double *d, hipStream_t s);
+// diagonal of A^-1 from the FP64 factor (in the factor's index order); e: scratch of n * nbk
+void ldl_inv_diag(int n, const double *a, int lda, const double *w, const double *work, double *e, int nbk, double *d,
+                  hipStream_t s);
+void l_reverse(int n, double *x, int nrhs, int64_t ldx, hipStream_t s);
+void l_packed_to_full(int n, const double *packed, double *full, int lda, int rev, hipStream_t s);
+void l_packed_to_full32(int n, const double *packed, float *full, int lda, int rev, hipStream_t s);
 void l_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, double *a,
-               int lda, hipStream_t s);
+               int lda, int n, int rev, hipStream_t s);
+void l_add_cct32(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, float *a,
+                 int lda, int n, int rev, hipStream_t s);
 void l_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
                  const double *x, double *y, hipStream_t s);
-
+void l_col_dots(int n, int nrhs, const double *a, const double *b, double *out, hipStream_t s);
+void l_col_axpy(int n, int nrhs, const double *alpha, double sg, const double *x, double *y, hipStream_t s);
+void l_col_xpby(int n, int nrhs, const double *z, const double *beta, double *p, hipStream_t s);
+double l_packed_fro2(int n, const double *a, double *part, double *scal, hipStream_t s);
+void l_zero(int64_t n, double *y, hipStream_t s);
 
 // ---- constraint elimination (mpgpu_qldec.cpp, k_qlelim.hip) ----
 // device view of the reflectors: rows over parameter ranges, the transpose per parameter, the WY

@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <omp.h>
 #include <thread>
 
 namespace {
@@ -71,8 +72,24 @@ int64_t estimate_footprint(const Job &j)
     int64_t nlab = std::unique(lab.begin(), lab.end()) - lab.begin();
     int64_t nall = nlab + (int64_t)j.crhs.size();
     int64_t mat;
-    if (j.opt.matsto == MPGPU_FULL) mat = 8 * nall * (nall + 1) / 2 * (j.opt.metsol == MPGPU_DECOMPOSITION || j.opt.metsol == MPGPU_CHOLESKY_FP32_IR ? 3 : 2);
-    else mat = std::min<int64_t>(8 * nall * (nall + 1) / 2, 8 * nw * 24);
+    if (j.opt.matsto == MPGPU_FULL) {
+        // packed matrix, plus what the solve allocates beside it: the inversion keeps a copy
+        // (1x packed); the FP64 factor is a full n x n FP64 matrix (2x packed); the FP32 factor
+        // a full FP32 one (1x) and, when the refinement does not converge, the FP64 factor
+        // as well (2x); the error diagonal of the LAPACK methods 512 unit columns
+        const int64_t packed = 8 * nall * (nall + 1) / 2;
+        int64_t mult = 2;
+        switch (j.opt.metsol) {
+        case MPGPU_DECOMPOSITION: case MPGPU_LAPACK_PACKED: case MPGPU_LAPACK_UNPACKED: mult = 3; break;
+        case MPGPU_CHOLESKY_FP32_IR: mult = 4; break;
+        default: mult = 2;
+        }
+        mat = packed * mult;
+        if (j.opt.lapack_errors && (j.opt.metsol == MPGPU_LAPACK_PACKED || j.opt.metsol == MPGPU_LAPACK_UNPACKED))
+            mat += 8 * nall * 512;
+    } else {
+        mat = std::min<int64_t>(8 * nall * (nall + 1) / 2, 8 * nw * 24);
+    }
     return rec + mat + upd + 200 * nall + (64 << 20);
 }
 
@@ -128,10 +145,14 @@ struct mpgpu_server {
         j.t_end = clk::now();
     }
 
+    int omp_threads = 1;
+
     void worker()
     {
         mpgpu_ctx *ctx = nullptr;
         (void)hipSetDevice(device);
+        // the host parts of the workers' jobs share the host's cores instead of each taking all
+        omp_set_num_threads(omp_threads);
         for (;;) {
             std::vector<std::shared_ptr<Job>> batch;
             {
@@ -176,6 +197,7 @@ int mpgpu_server_create(int32_t device, int32_t nworkers, int64_t budget_bytes, 
     s->device = device;
     s->budget = budget_bytes;
     s->batch_max_params = batch_max_params;
+    s->omp_threads = std::max(1, omp_get_num_procs() / nworkers);
     for (int i = 0; i < nworkers; i++) s->workers.emplace_back([s] { s->worker(); });
     *out = s;
     return 0;
@@ -188,6 +210,8 @@ int mpgpu_server_submit(mpgpu_server *s, const mpgpu_options *opt, int64_t nrec,
 {
     if (!s || !opt || nrec < 0) return 30;
     auto j = std::make_shared<Job>();
+    // the job has arrived when the caller hands it over, before it is copied and sized
+    j->t_arrival = clk::now();
     j->opt = *opt;
     int64_t w0 = rec_ptr[0], w1 = rec_ptr[nrec];
     j->rec_ptr.resize(nrec + 1);
@@ -207,7 +231,6 @@ int mpgpu_server_submit(mpgpu_server *s, const mpgpu_options *opt, int64_t nrec,
         j->crhs.assign(crhs, crhs + ncon);
     }
     j->footprint = estimate_footprint(*j);
-    j->t_arrival = clk::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (j->footprint > s->budget) return 30;
@@ -231,7 +254,9 @@ int mpgpu_server_wait(mpgpu_server *s, int64_t job_id, int32_t capacity, int32_t
         if (it == s->jobs.end()) return 30;
         j = it->second;
         s->cv_done.wait(lk, [&] { return j->done; });
-        s->jobs.erase(it);
+        // another waiter on the same job may have removed it meanwhile
+        auto again = s->jobs.find(job_id);
+        if (again != s->jobs.end()) s->jobs.erase(again);
     }
     if (st) {
         std::memset(st, 0, sizeof(*st));
@@ -245,6 +270,9 @@ int mpgpu_server_wait(mpgpu_server *s, int64_t job_id, int32_t capacity, int32_t
         st->istop = j->sst.istop;
         st->itn = j->sst.itn;
         st->rel_residual = j->sst.rel_residual;
+        st->t_arrival_s = std::chrono::duration<double>(j->t_arrival.time_since_epoch()).count();
+        st->t_done_s = std::chrono::duration<double>(j->t_end.time_since_epoch()).count();
+        st->method = j->sst.method;
     }
     if (labels && corr) {
         int32_t n = std::min<int32_t>(capacity, (int32_t)j->labels.size());

@@ -42,7 +42,9 @@ class Options(C.Structure):
                 ("dwcut", C.c_float), ("cndlmx", C.c_float), ("regula", C.c_float),
                 ("default_presigma", C.c_double), ("mrestl", C.c_double), ("mrtcnd", C.c_double),
                 ("nregul", C.c_int32), ("refine_steps", C.c_int32), ("monitor", C.c_int32), ("monitor_bins", C.c_int32),
-                ("lapack_errors", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("lapack_errors", C.c_int32), ("pair_entries", C.c_int32), ("skip_empty_cons", C.c_int32),
+                ("resolve_redundancy_cons", C.c_int32), ("scale_errors", C.c_float * 2), ("reserved", C.c_int32 * 2),
+                ("refine_tol", C.c_double)]
 
 
 class PassStats(C.Structure):
@@ -55,7 +57,8 @@ class PassStats(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("method", C.c_int32), ("nrank", C.c_int32), ("istop", C.c_int32), ("itn", C.c_int32),
                 ("anorm", C.c_double), ("acond", C.c_double), ("rnorm", C.c_double), ("arnorm", C.c_double),
-                ("ynorm", C.c_double), ("rel_residual", C.c_double), ("ms", C.c_double)]
+                ("ynorm", C.c_double), ("rel_residual", C.c_double), ("ms", C.c_double), ("evmax", C.c_double),
+                ("evmin", C.c_double)]
 
 
 class RunStats(C.Structure):
@@ -68,7 +71,8 @@ class RunStats(C.Structure):
 class JobStats(C.Structure):
     _fields_ = [("status", C.c_int32), ("nvar", C.c_int32), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
                 ("run_ms", C.c_double), ("chi2", C.c_double), ("ndf", C.c_int64), ("istop", C.c_int32),
-                ("itn", C.c_int32), ("rel_residual", C.c_double)]
+                ("itn", C.c_int32), ("rel_residual", C.c_double), ("t_arrival_s", C.c_double), ("t_done_s", C.c_double),
+                ("method", C.c_int32), ("reserved", C.c_int32)]
 
 
 class MpgpuError(RuntimeError):
@@ -269,7 +273,8 @@ class Pede:
         r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64)
         self._chk(self.L.mpgpu_solve(self.h, _p(r, D), _p(x, D), C.byref(st)))
         return x, dict(method=st.method, nrank=st.nrank, istop=st.istop, itn=st.itn, anorm=st.anorm,
-                       acond=st.acond, rnorm=st.rnorm, rel_residual=st.rel_residual, ms=st.ms)
+                       acond=st.acond, rnorm=st.rnorm, rel_residual=st.rel_residual, ms=st.ms, evmax=st.evmax,
+                       evmin=st.evmin)
 
     def avprod(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -397,6 +402,8 @@ def gen():
         G.mpgen_brokenlines.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                         C.POINTER(_GenRecords)]
+        G.mpgen_dense.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double,
+                                  C.POINTER(_GenRecords), D]
         G.mpgen_write.argtypes = [C.c_char_p, C.POINTER(_GenRecords), C.c_int]
         G.mpgen_free.argtypes = [C.POINTER(_GenRecords)]
         _gen = G
@@ -448,6 +455,18 @@ def brokenlines_records(seed, nlayers, nx, ny, ntracks, pmin, pmax, wx=2.0, wy=5
     if rc:
         raise MpgpuError(rc, "generator failed")
     return _take(r)
+
+
+def dense_records(seed, nlayers, nx, ny, ntracks, slope_frac=0.75, sigma=0.002, truth=True):
+    """Dense detector: every track crosses all nlayers layers of a periodic nx x ny module grid
+    (6 rigid-body globals per module, 4 local parameters, 2 measurements per hit); the slope
+    spread sweeps slope_frac of the grid across the stack, so the normal matrix is densely filled."""
+    r = _GenRecords()
+    tr = np.zeros(nlayers * nx * ny * 6) if truth else None
+    rc = gen().mpgen_dense(seed, nlayers, nx, ny, ntracks, slope_frac, sigma, C.byref(r), _p(tr, D))
+    if rc:
+        raise MpgpuError(rc, "generator failed")
+    return _take(r) + (tr,)
 
 
 def pixel_labels(nlayers, nx, ny):

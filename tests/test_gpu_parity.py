@@ -447,12 +447,18 @@ def test_cholesky(metsol):
     P.loopn()
     xr, A, b = _kkt_reference(P)
     x, st = P.solve()
+    # the method that actually ran: the FP32 factor with converged FP64 refinement
+    assert st["method"] == metsol, st
+    if metsol == mpgpu.CHOLESKY_FP32_IR:
+        assert st["istop"] == 1 and 1 <= st["itn"] <= 20, st
     tol = 1e-9 if metsol == mpgpu.DECOMPOSITION else 1e-8
     assert rel_err(x[:P.nvar], xr[:P.nvar]) < tol
     # max |dp| / sigma_p for the mixed-precision factorisation
     Ainv = np.linalg.inv(A)
     sig = np.sqrt(np.abs(np.diag(Ainv)[:P.nvar]))
-    assert np.max(np.abs(x[:P.nvar] - xr[:P.nvar]) / sig) < 1e-6
+    dps = np.max(np.abs(x[:P.nvar] - xr[:P.nvar]) / sig)
+    print(f"method {st['method']}: {st['itn']} refinement steps, max |dp|/sigma_p = {dps:.2e}")
+    assert dps < 1e-6
 
 
 @pytest.mark.parametrize("metsol", [mpgpu.LAPACK_PACKED, mpgpu.LAPACK_UNPACKED])
