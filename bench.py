@@ -149,6 +149,70 @@ def serving_run(njobs, rate=200.0, seed=7):
     return out
 
 
+def dense_run(ntracks, seed=2027):
+    """Dense mixed precision: 40 layers x 125 modules x 6 globals (30,000 parameters), every
+    track crosses all 40 layers (40 modules, 240 globals, 80 measurements); the slope spread
+    fills about half of the normal matrix.  Full storage, FP64 accumulation.  The same matrix
+    is solved by the FP64 LDL^T ('decomposition') and by the FP32 factor with FP64 refinement;
+    the deviation is stated as max |dp| / sigma_p over 64 parameters whose sigma_p comes from
+    FP64 solves A x = e_i."""
+    import mpgpu
+    nl, nx, ny = 40, 25, 5
+    t0 = time.perf_counter()
+    rp, ii, gg, _ = mpgpu.dense_records(seed, nl, nx, ny, ntracks)
+    log(f"dense: generated {ntracks} records ({rp[-1]} words) in {time.perf_counter() - t0:.1f}s")
+    labels = mpgpu.pixel_labels(nl, nx, ny)
+    presig = np.array([0.1 if l % 10 <= 3 else 0.01 for l in labels])
+    out = {"workload": f"dense detector: {nl} layers x {nx * ny} modules x 6 globals ({6 * nl * nx * ny} params), "
+                       f"{ntracks} tracks crossing all {nl} layers, 2D FP32 measurements sigma 20um, full storage, "
+                       "FP64 accumulation", "records": ntracks}
+    res = {}
+    for metsol in (mpgpu.DECOMPOSITION, mpgpu.CHOLESKY_FP32_IR):
+        P = mpgpu.Pede(matsto=mpgpu.FULL, metsol=metsol, mreqenf=5, mcount=1)
+        P.add_records(rp, ii, gg)
+        P.set_steering_parameters(labels, np.zeros(len(labels)), presig)
+        t = time.perf_counter()
+        P.build_structure()
+        t_struct = time.perf_counter() - t
+        P.loopn()                                   # warm-up pass
+        t = time.perf_counter()
+        st = P.loopn()
+        t_pass = time.perf_counter() - t
+        P.solve()                                   # warm-up solve (allocations, code loading)
+        P.mupdat([1], [1], [0.0])                   # same matrix, solve state reset
+        t = time.perf_counter()
+        x, sst = P.solve()
+        t_solve = time.perf_counter() - t
+        key = "fp64" if metsol == mpgpu.DECOMPOSITION else "fp32_ir"
+        res[key] = dict(x=x, P=P, sst=sst)
+        out[key] = {"pass_s": t_pass, "solve_s": t_solve, "end_to_end_s": t_pass + t_solve, "method": sst["method"],
+                    "refine_steps": sst["itn"] if metsol != mpgpu.DECOMPOSITION else 0, "istop": sst["istop"],
+                    "rel_residual": sst["rel_residual"], "nrank": sst["nrank"]}
+        out["structure_s"] = t_struct
+        out["nvar"] = P.nvar
+        out["records_per_s"] = ntracks / t_pass
+        log(f"dense {key}: pass {t_pass * 1e3:.1f} ms, solve {t_solve * 1e3:.1f} ms, {sst}")
+    x64, x32 = res["fp64"]["x"], res["fp32_ir"]["x"]
+    P = res["fp64"]["P"]
+    idx = np.random.default_rng(5).choice(P.nvar, 64, replace=False)
+    dps = 0.0
+    for i in idx:
+        e = np.zeros(P.nall)
+        e[i] = 1.0
+        xi, _ = P.solve(e)
+        dps = max(dps, abs(x32[i] - x64[i]) / np.sqrt(abs(xi[i])))
+    out["max_dp_over_sigma_64_params"] = dps
+    out["max_rel_dp"] = float(np.abs(x32 - x64).max() / np.abs(x64).max())
+    out["solve_speedup_fp32_ir"] = out["fp64"]["solve_s"] / out["fp32_ir"]["solve_s"]
+    out["end_to_end_speedup_fp32_ir"] = out["fp64"]["end_to_end_s"] / out["fp32_ir"]["end_to_end_s"]
+    n = P.nvar
+    out["factor_flops"] = n ** 3 / 3.0
+    for r in res.values():
+        r["P"].close()
+    log(f"dense: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +222,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=120)
+    ap.add_argument("--dense-records", type=int, default=2_000_000,
+                    help="tracks of the dense mixed-precision workload (0: skip it)")
     ap.add_argument("--elimination", action="store_true",
                     help="eliminate the constraints by QL decomposition (pede's default) instead of the "
                          "Lagrange multipliers of the benchmark configuration")
@@ -264,6 +330,11 @@ def main():
             out["p99_job_latency_ms"] = out["serving"]["p99_ms"]
         except Exception as e:
             out["serving"] = {"error": str(e)}
+    if args.dense_records > 0:
+        try:
+            out["dense"] = dense_run(args.dense_records)
+        except Exception as e:
+            out["dense"] = {"error": str(e)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(lab_var, rp, ii, gg)

@@ -30,7 +30,7 @@ namespace mpgpu {
 // packed lower-row (pede) -> full column-major lower part, optionally index-reversed
 // ---------------------------------------------------------------------------------------------
 template <typename T>
-__global__ void k_packed_to_full(int n, const double *p, T *a, int lda, int rev)
+__global__ void k_packed_to_full(int n, const double *p, T *a, int lda, int rev, const double *sc)
 {
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)n * (n + 1) / 2;
@@ -39,7 +39,7 @@ __global__ void k_packed_to_full(int n, const double *p, T *a, int lda, int rev)
     while (i * (i + 1) / 2 > t) i--;
     while ((i + 1) * (i + 2) / 2 <= t) i++;
     int64_t j = t - i * (i + 1) / 2; // 0-based, j <= i
-    const T v = (T)p[t];
+    const T v = sc ? (T)(p[t] * sc[i] * sc[j]) : (T)p[t];
     if (rev) {
         // (i, j) -> (n-1-i, n-1-j): row n-1-j >= column n-1-i
         const int64_t r = n - 1 - j, c = n - 1 - i;
@@ -146,66 +146,85 @@ template <> struct Mfma<float> {
 };
 
 // One workgroup forms a 128 x 128 block C (+)= X Y^T, X rows = C rows, Y rows = C columns,
-// both column-major (row index contiguous) with K columns.  K is consumed in chunks of KC
-// columns staged through LDS, double-buffered: the global loads of chunk c+1 are issued before
-// the matrix-core products of chunk c and written to the other buffer after them, so one
-// barrier per chunk separates the phases.  Four waves each own a 64 x 64 quarter (4 x 4
-// MFMA 16 x 16 accumulators).  The products are formed as Y X^T on the matrix cores, so a
-// lane's results lie in one column of C and 16 lanes store 16 consecutive rows.
+// both column-major (row index contiguous) with K columns (xrows, yrows >= 1 valid rows; K a
+// multiple of KC).  K is consumed in chunks of KC columns staged in LDS, double-buffered.  The
+// chunks travel from global memory straight into LDS (global_load_lds_dword: a wave copies one
+// contiguous 256-byte run of a column per instruction, no registers involved), issued for chunk
+// c+1 before the matrix-core products of chunk c.  Rows past the valid ones are read as copies
+// of the last valid row: they only reach rows / columns of C that are not stored.  Four waves
+// each own a 64 x 64 quarter (4 x 4 MFMA 16 x 16 accumulators); the products are formed as
+// Y X^T on the matrix cores, so a lane's results lie in one column of C and 16 lanes store 16
+// consecutive rows.
 template <typename T> struct Chunk { static constexpr int KC = sizeof(T) == 8 ? 32 : 64; };
 
+typedef __attribute__((address_space(3))) void *lds_ptr_t;
+
 template <typename T>
-__device__ __forceinline__ void block_nt(const T *X, int64_t ldx, int xrows, const T *Y, int64_t ldy, int yrows, int K,
-                                         T *lds, typename Mfma<T>::acc_t (&acc)[4][4])
+__device__ __forceinline__ void dma_chunk(const T *X, int64_t ldx, int xrows, const T *Y, int64_t ldy, int yrows, int kb,
+                                          T *b)
 {
     constexpr int KC = Chunk<T>::KC;
-    constexpr int PER = (2 * 128 * KC) / 256;   // staged values per thread and chunk
-    constexpr int SPAN = 256 / 128;             // k columns covered by one pass of 256 threads
-    T *buf[2] = {lds, lds + 2 * 128 * KC};
-    const int t = threadIdx.x, row = t & 127, k0 = t >> 7;
+    constexpr int RUN = 256 / (int)sizeof(T);   // elements of one 256-byte run
+    constexpr int NRUN = 128 / RUN;             // runs per column of 128 rows
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int eo = (lane * 4) / (int)sizeof(T), bo = (lane * 4) % (int)sizeof(T);
+#pragma unroll
+    for (int j = 0; j < KC / 4; j++) {
+        const int kl = wave + 4 * j;
+        const int64_t k = kb + kl;
+#pragma unroll
+        for (int q = 0; q < NRUN; q++) {
+            const int e = q * RUN + eo;
+            const char *gx = (const char *)(X + k * ldx + min(e, xrows - 1)) + bo;
+            const char *gy = (const char *)(Y + k * ldy + min(e, yrows - 1)) + bo;
+            __builtin_amdgcn_global_load_lds((const void *)gx, (lds_ptr_t)(b + kl * 128 + q * RUN), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)gy, (lds_ptr_t)(b + 128 * KC + kl * 128 + q * RUN), 4, 0, 0);
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void chunk_mfma(const T *bx, typename Mfma<T>::acc_t (&acc)[4][4])
+{
+    constexpr int KC = Chunk<T>::KC;
+    const T *by = bx + 128 * KC;
+    const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63, lr = lane & 15, lk = lane >> 4;
     const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-    T st[PER];
-    // chunk c into registers: X part then Y part, row = t & 127, k = k0 + SPAN * m
-    auto fetch = [&](int c) {
-        const int kb = c * KC;
 #pragma unroll
-        for (int m = 0; m < PER / 2; m++) {
-            const int k = kb + k0 + SPAN * m;
-            st[m] = (k < K && row < xrows) ? X[(int64_t)k * ldx + row] : (T)0;
-            st[PER / 2 + m] = (k < K && row < yrows) ? Y[(int64_t)k * ldy + row] : (T)0;
-        }
-    };
-    // layout in LDS: [k][row] for X, then [k][row] for Y
-    auto stash = [&](T *b) {
+    for (int kk = 0; kk < KC; kk += 4) {
+        T xa[4], yb[4];
 #pragma unroll
-        for (int m = 0; m < PER / 2; m++) {
-            const int k = k0 + SPAN * m;
-            b[k * 128 + row] = st[m];
-            b[128 * KC + k * 128 + row] = st[PER / 2 + m];
+        for (int p = 0; p < 4; p++) {
+            xa[p] = bx[(kk + lk) * 128 + wr + 16 * p + lr];
+            yb[p] = by[(kk + lk) * 128 + wc + 16 * p + lr];
         }
-    };
-    const int nchunk = (K + KC - 1) / KC;
-    fetch(0);
-    stash(buf[0]);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int p = 0; p < 4; p++) acc[q][p] = Mfma<T>::op(yb[q], xa[p], acc[q][p]);
+    }
+}
+
+// the two LDS stages are distinct objects, so that the compiler can tell the products' LDS
+// reads of one stage from the transfers into the other and keeps the transfers in flight
+template <typename T>
+__device__ __forceinline__ void block_nt(const T *X, int64_t ldx, int xrows, const T *Y, int64_t ldy, int yrows, int K,
+                                         T *sa, T *sb, typename Mfma<T>::acc_t (&acc)[4][4])
+{
+    constexpr int KC = Chunk<T>::KC;
+    const int nchunk = K / KC;     // even: K = 128
+    dma_chunk<T>(X, ldx, xrows, Y, ldy, yrows, 0, sa);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    for (int c = 0; c < nchunk; c++) {
-        if (c + 1 < nchunk) fetch(c + 1);
-        const T *bx = buf[c & 1], *by = buf[c & 1] + 128 * KC;
-#pragma unroll 2
-        for (int kk = 0; kk < KC; kk += 4) {
-            T xa[4], yb[4];
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                xa[p] = bx[(kk + lk) * 128 + wr + 16 * p + lr];
-                yb[p] = by[(kk + lk) * 128 + wc + 16 * p + lr];
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-#pragma unroll
-                for (int p = 0; p < 4; p++) acc[q][p] = Mfma<T>::op(yb[q], xa[p], acc[q][p]);
-        }
-        if (c + 1 < nchunk) stash(buf[(c + 1) & 1]);
+    for (int c = 0; c < nchunk; c += 2) {
+        dma_chunk<T>(X, ldx, xrows, Y, ldy, yrows, (c + 1) * KC, sb);
+        chunk_mfma<T>(sa, acc);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (c + 2 < nchunk) dma_chunk<T>(X, ldx, xrows, Y, ldy, yrows, (c + 2) * KC, sa);
+        chunk_mfma<T>(sb, acc);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 }
@@ -221,7 +240,8 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_ldl_gemm(int n, T *a, int lda, int k0, const T *linv, const T *w, T *wb,
                                                   int64_t ntiles, int base, int col_only)
 {
-    extern __shared__ double gemm_lds[];
+    __shared__ T stage_a[2 * 128 * Chunk<T>::KC];
+    __shared__ T stage_b[2 * 128 * Chunk<T>::KC];
     const int nb = min(LB, n - k0);
     const int t0 = k0 + nb + LB * base;
     int64_t tile = blockIdx.x;
@@ -254,30 +274,47 @@ __global__ __launch_bounds__(256) void k_ldl_gemm(int n, T *a, int lda, int k0, 
 #pragma unroll
             for (int e = 0; e < 4; e++) acc[q][p][e] = (T)0;
     if (MODE == MODE_PANEL)
-        block_nt<T>(a + (int64_t)k0 * lda + r0, lda, xrows, linv, LB, yrows, nb, (T *)gemm_lds, acc);
+        block_nt<T>(a + (int64_t)k0 * lda + r0, lda, xrows, linv, LB, yrows, LB, stage_a, stage_b, acc);
     else
-        block_nt<T>(a + (int64_t)k0 * lda + r0, lda, xrows, wb + c0, n, yrows, nb, (T *)gemm_lds, acc);
+        block_nt<T>(a + (int64_t)k0 * lda + r0, lda, xrows, wb + c0, n, yrows, LB, stage_a, stage_b, acc);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
     const int rbase = r0 + (wave >> 1) * 64, cbase = c0 + (wave & 1) * 64;
+    // The stores go out in four groups of 16 values per lane.  A group's loads (matrix elements,
+    // pivots) come from clamped valid addresses and are issued together, one group ahead of the
+    // group being stored, so the memory round trips overlap.
+    T cur[2][4][4];
+    auto fetch = [&](int q, T (&c)[4][4]) {
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = min(rbase + 16 * p + lr, n - 1);
+                const int cc = cbase + 16 * q + Mfma<T>::row(lk, e);
+                if (MODE == MODE_PANEL) c[p][e] = w[k0 + min(cc, nb - 1)];
+                else c[p][e] = a[(int64_t)min(cc, n - 1) * lda + r];
+            }
+    };
+    fetch(0, cur[0]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q + 1 < 4) fetch(q + 1, cur[(q + 1) & 1]);
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 const int r = rbase + 16 * p + lr;
                 const int cc = cbase + 16 * q + Mfma<T>::row(lk, e);
-                if (r >= n) continue;
+                const T cv = cur[q & 1][p][e];
                 if (MODE == MODE_PANEL) {
-                    if (cc >= nb) continue;
-                    wb[(int64_t)cc * n + r] = acc[q][p][e];
-                    a[(int64_t)(k0 + cc) * lda + r] = acc[q][p][e] * w[k0 + cc];
+                    if (r < n && cc < nb) {
+                        wb[(int64_t)cc * n + r] = acc[q][p][e];
+                        a[(int64_t)(k0 + cc) * lda + r] = acc[q][p][e] * cv;
+                    }
                 } else {
-                    if (cc >= n || cc > r) continue;
-                    const int64_t o = (int64_t)cc * lda + r;
-                    a[o] = a[o] - acc[q][p][e];
+                    if (r < n && cc < n && cc <= r) a[(int64_t)cc * lda + r] = cv - acc[q][p][e];
                 }
             }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -452,7 +489,6 @@ int64_t ldl_work_elems(int n) { return 2 * (int64_t)n * LB + (int64_t)nblocks_of
 
 template <typename T> static int diag_lds() { return (int)sizeof(T) * (3 * LB + LB * (LB + 1)); }
 
-template <typename T> static int gemm_lds_bytes() { return (int)sizeof(T) * 4 * 128 * Chunk<T>::KC; }
 
 // Right-looking factorization with one block of look-ahead: at step t the trailing update first
 // brings block column t+1 up to date (on the caller's stream), then a second stream factors
@@ -463,12 +499,8 @@ template <typename T> static hipError_t ldl_factor_t(int n, T *a, int lda, T *w,
 {
     T *wbuf[2] = {work, work + (int64_t)n * LB};
     T *linv_all = work + 2 * (int64_t)n * LB;
-    const int ldsd = diag_lds<T>(), ldsg = gemm_lds_bytes<T>();
+    const int ldsd = diag_lds<T>(), ldsg = 0;   // the products' LDS stages are static
     hipError_t e = hipFuncSetAttribute((const void *)k_ldl_diag<T>, hipFuncAttributeMaxDynamicSharedMemorySize, ldsd);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_ldl_gemm<T, MODE_PANEL>, hipFuncAttributeMaxDynamicSharedMemorySize, ldsg);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_ldl_gemm<T, MODE_SYRK>, hipFuncAttributeMaxDynamicSharedMemorySize, ldsg);
     if (e != hipSuccess) return e;
     const int nblk = nblocks_of(n);
     auto diag_panel = [&](int t, hipStream_t st) {
@@ -602,7 +634,7 @@ void ldl_inv_diag(int n, const double *a, int lda, const double *w, const double
 // one block per constraint
 template <typename T>
 __global__ void k_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
-                          T *a, int lda, int n, int rev)
+                          T *a, int lda, int n, int rev, const double *sc)
 {
     const int j = blockIdx.x;
     const int64_t q0 = cptr[j];
@@ -610,9 +642,10 @@ __global__ void k_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx
     for (int64_t t = threadIdx.x; t < len * len; t += blockDim.x) {
         const int64_t p = t / len, q = t % len;
         int64_t row = cidx[q0 + p] - 1, col = cidx[q0 + q] - 1;
+        const double f = sc ? sc[row] * sc[col] : 1.0;
         if (rev) { row = n - 1 - row; col = n - 1 - col; }
         if (row < col) continue;
-        atomicAdd(&a[col * lda + row], (T)(rho * cval[q0 + p] * cval[q0 + q]));
+        atomicAdd(&a[col * lda + row], (T)(rho * cval[q0 + p] * cval[q0 + q] * f));
     }
 }
 // y += rho C^T (C x) for the parameter block; one block per constraint
@@ -634,10 +667,10 @@ __global__ void k_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *ci
 }
 void l_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, double *a,
                int lda, int n, int rev, hipStream_t s)
-{ if (ncon > 0) hipLaunchKernelGGL(k_add_cct<double>, dim3(ncon), dim3(256), 0, s, ncon, cptr, cidx, cval, rho, a, lda, n, rev); }
+{ if (ncon > 0) hipLaunchKernelGGL(k_add_cct<double>, dim3(ncon), dim3(256), 0, s, ncon, cptr, cidx, cval, rho, a, lda, n, rev, (const double *)nullptr); }
 void l_add_cct32(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, float *a,
-                 int lda, int n, int rev, hipStream_t s)
-{ if (ncon > 0) hipLaunchKernelGGL(k_add_cct<float>, dim3(ncon), dim3(256), 0, s, ncon, cptr, cidx, cval, rho, a, lda, n, rev); }
+                 int lda, int n, int rev, const double *sc, hipStream_t s)
+{ if (ncon > 0) hipLaunchKernelGGL(k_add_cct<float>, dim3(ncon), dim3(256), 0, s, ncon, cptr, cidx, cval, rho, a, lda, n, rev, sc); }
 void l_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
                  const double *x, double *y, hipStream_t s)
 { if (ncon > 0) hipLaunchKernelGGL(k_cct_apply, dim3(ncon), dim3(256), 0, s, ncon, cptr, cidx, cval, rho, x, y); }
@@ -645,12 +678,24 @@ void l_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const d
 void l_packed_to_full(int n, const double *packed, double *full, int lda, int rev, hipStream_t s)
 {
     int64_t tot = (int64_t)n * (n + 1) / 2;
-    hipLaunchKernelGGL(k_packed_to_full<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, packed, full, lda, rev);
+    hipLaunchKernelGGL(k_packed_to_full<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, packed, full, lda, rev,
+                       (const double *)nullptr);
 }
-void l_packed_to_full32(int n, const double *packed, float *full, int lda, int rev, hipStream_t s)
+void l_packed_to_full32(int n, const double *packed, float *full, int lda, int rev, const double *sc, hipStream_t s)
 {
     int64_t tot = (int64_t)n * (n + 1) / 2;
-    hipLaunchKernelGGL(k_packed_to_full<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, packed, full, lda, rev);
+    hipLaunchKernelGGL(k_packed_to_full<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, packed, full, lda, rev, sc);
+}
+// x[:, q] <- d o x[:, q]
+__global__ void k_scale_cols(int n, const double *d, double *x, int64_t ldx)
+{
+    x += blockIdx.y * ldx;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= d[i];
+}
+void l_scale_cols(int n, const double *d, double *x, int nrhs, int64_t ldx, hipStream_t s)
+{
+    if (n > 0 && nrhs > 0) hipLaunchKernelGGL(k_scale_cols, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, d, x, ldx);
 }
 
 // ---------------------------------------------------------------------------------------------

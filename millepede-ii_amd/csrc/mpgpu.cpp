@@ -120,6 +120,8 @@ struct mpgpu_ctx {
     // src/pede.f90:8292-8306): kept until the matrix changes; pc_bw/pc_lp: its mbandw/lprecm
     std::shared_ptr<Precond> pc;
     int pc_bw = 0, pc_lp = 0;
+    // factor of the dense methods for the current matrix (built on first use after it changed)
+    std::unique_ptr<DenseFactor> dfac;
     std::vector<double> cov_diag;
     std::vector<double> last_vec;
     std::vector<unsigned long long> last_cnt;
@@ -1228,6 +1230,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     phase("device upload");
     c->built = true;
     c->pc.reset();
+    c->dfac.reset();
     if (ntot_out) *ntot_out = c->ntot;
     if (nvar_out) *nvar_out = c->nvar;
     if (ncon_out) *ncon_out = c->ncon;
@@ -1336,6 +1339,7 @@ static void device_mupdat(mpgpu_ctx *c, const std::vector<int32_t> &ii, const st
     l_mupdat(model_of(c), (int32_t)ii.size(), di.p, dj.p, dv.p, c->stream);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->pc.reset();
+    c->dfac.reset();
 }
 
 extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float chicut, mpgpu_pass_stats *st)
@@ -1356,7 +1360,7 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     HIPCHK(hipEventRecord(e0, s));
     c->d_vec.zero(s);
     c->d_cnt.zero(s);
-    if (icalcm == 1) { c->d_mat.zero(s); c->have_inverse = false; c->have_cov = false; c->pc.reset(); }
+    if (icalcm == 1) { c->d_mat.zero(s); c->have_inverse = false; c->have_cov = false; c->pc.reset(); c->dfac.reset(); }
     PassOpts o{};
     o.icalcm = icalcm;
     o.nloopn = nloopn;
@@ -2178,56 +2182,36 @@ void elim_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
 // dense direct solves: 'decomposition' (chdec2/chslv2 semantics), the LAPACK methods and the
 // FP32 factor with FP64 refinement
 // ---------------------------------------------------------------------------------------------
-namespace {
-struct DenseFactor {
-    int n = 0;
-    bool rev = false;    // factor of the index-reversed matrix (chdec2's pivot order)
-    bool fp32 = false;
-    DBuf<double> a, w, work;
-    DBuf<float> a32, w32, work32;
-    std::vector<double> piv;
-    // v <- A^-1 v for nrhs columns of length n (leading dimension n) with the FP64 factor
-    void solve64(double *v, int nrhs, hipStream_t s)
-    {
-        if (rev) l_reverse(n, v, nrhs, n, s);
-        ldl_solve(n, a.p, n, w.p, work.p, v, nrhs, n, s);
-        if (rev) l_reverse(n, v, nrhs, n, s);
-    }
-    void solve32(double *v, int nrhs, hipStream_t s)
-    {
-        if (rev) l_reverse(n, v, nrhs, n, s);
-        ldl_solve32(n, a32.p, n, w32.p, work32.p, v, nrhs, n, s);
-        if (rev) l_reverse(n, v, nrhs, n, s);
-    }
-};
-} // namespace
-
-static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats &loc)
+// Factor of the (augmented) parameter block for the dense methods: FP64 LDL^T with chdec2's
+// semantics ('decomposition'), LAPACK's stop on the first non-positive pivot (fullLAPACK,
+// unpackedLAPACK), or the FP32 LDL^T for the FP32 + FP64-refinement method (falls back to FP64
+// when an FP32 pivot is not positive).
+static void dense_factor(mpgpu_ctx *c, DenseFactor &F, bool want_fp32)
 {
     hipStream_t s = c->stream;
-    const int64_t n = c->nall;
     const int nv = c->nvar, m = c->ncon;
     const bool lapack = c->opt.metsol == MPGPU_LAPACK_PACKED || c->opt.metsol == MPGPU_LAPACK_UNPACKED;
-    // 'LAPACKwitherrors': the parameter covariance on the constraint surface, as pede's
-    // inverse from the factor expanded by Q (src/pede.f90:9017-9120); with multipliers the
-    // same matrix is diag(A^-1) - diag(W S^-1 W^T), W = A^-1 C^T, S = C W, and A may carry
-    // the rho C^T C term below (it cancels on the constraint surface)
-    const bool want_cov = c->opt.lapack_errors && lapack;
-    c->have_cov = false;
-    std::vector<double> covcorr(want_cov ? nv : 0, 0.0);
+    F.n = nv;
+    F.metsol = c->opt.metsol;
+    // pede's decomposition (chdec2) eliminates from the last parameter; LAPACK's dpptrf from the
+    // first, and it stops at the first non-positive leading minor
+    F.rev = !lapack;
     // With constraints the parameter block alone may be singular (modes fixed only by the
     // constraints, as pede's decomposition removes them by elimination).  The block is
     // augmented to A + rho C^T C and the right-hand side to b + rho C^T d: on the constraint
     // surface C x = d this changes neither x nor the multipliers, and the augmented block is
     // positive definite whenever the constrained problem is well posed.
-    double rho = 0.0, cc_norm = 0.0;
-    if (m > 0) {
+    double cc_norm = 0.0;
+    F.rho = 0.0;
+    std::vector<double> hd(nv), cc(nv, 0.0);
+    {
         DBuf<double> dg;
         dg.alloc(nv);
         l_get_diag(model_of(c), nv, dg.p, s);
-        std::vector<double> hd(nv), cc(nv, 0.0);
         dg.download(hd.data(), nv, s);
         HIPCHK(hipStreamSynchronize(s));
+    }
+    if (m > 0) {
         for (int j = 0; j < m; j++) {
             double rn = 0.0;
             for (int64_t q = c->cptr[j]; q < c->cptr[j + 1]; q++) {
@@ -2238,13 +2222,8 @@ static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_
         }
         double amax = 0.0, cmax = 0.0;
         for (int i = 0; i < nv; i++) { amax = std::max(amax, std::fabs(hd[i])); cmax = std::max(cmax, cc[i]); }
-        rho = cmax > 0.0 ? (amax > 0.0 ? amax : 1.0) / cmax : 0.0;
+        F.rho = cmax > 0.0 ? (amax > 0.0 ? amax : 1.0) / cmax : 0.0;
     }
-    DenseFactor F;
-    F.n = nv;
-    // pede's decomposition (chdec2) eliminates from the last parameter; LAPACK's dpptrf from the
-    // first, and it stops at the first non-positive leading minor
-    F.rev = !lapack;
     const int64_t nn = (int64_t)nv * nv;
     auto pivots_to_host = [&](bool f32) {
         DBuf<double> d;
@@ -2254,42 +2233,22 @@ static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_
         d.download(F.piv.data(), nv, s);
         HIPCHK(hipStreamSynchronize(s));
     };
-    auto factor64 = [&]() {
-        F.fp32 = false;
-        F.a32.release(); F.w32.release(); F.work32.release();
-        F.a.alloc(nn);
-        F.w.alloc(nv);
-        F.work.alloc(ldl_work_elems(nv));
-        l_packed_to_full(nv, c->d_mat.p, F.a.p, nv, F.rev, s);
-        l_add_cct(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, F.a.p, nv, nv, F.rev, s);
-        HIPCHK(ldl_factor(nv, F.a.p, nv, F.w.p, F.work.p, s));
-        pivots_to_host(false);
-        int nrank = 0;
-        double evmax = 0.0, evmin = 0.0;
-        for (int k = 0; k < nv; k++) {
-            const double d = F.piv[k];
-            if (d > 0.0) {
-                if (nrank == 0) evmax = evmin = d;
-                evmax = std::max(evmax, d);
-                evmin = std::min(evmin, d);
-                nrank++;
-            } else if (lapack) {
-                // dpptrf: 'factorization ... failed at index', exit 29 (src/mplapack.f90:144-156)
-                throw MpError(29, "Aborted, factorization of global matrix failed at index " + std::to_string(k + 1));
-            }
+    if (want_fp32) {
+        // FP32 keeps 24 bits relative to each element: the matrix is scaled to unit diagonal
+        // first, so that parameters of very different scale (shifts, rotations) lose the same
+        // relative precision in the rounding
+        std::vector<double> d(nv);
+        for (int i = 0; i < nv; i++) {
+            const double a = hd[i] + F.rho * cc[i];
+            d[i] = a > 0.0 ? 1.0 / std::sqrt(a) : 1.0;
         }
-        loc.nrank = nrank;
-        loc.evmax = evmax;
-        loc.evmin = evmin;
-    };
-    bool fp32 = c->opt.metsol == MPGPU_CHOLESKY_FP32_IR;
-    if (fp32) {
+        F.sc.upload(d, s);
         F.fp32 = true;
         F.a32.alloc(nn);
         F.w32.alloc(nv);
         F.work32.alloc(ldl_work_elems(nv));
-        l_packed_to_full32(nv, c->d_mat.p, F.a32.p, nv, F.rev, s);
-        l_add_cct32(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, rho, F.a32.p, nv, nv, F.rev, s);
+        l_packed_to_full32(nv, c->d_mat.p, F.a32.p, nv, F.rev, F.sc.p, s);
+        l_add_cct32(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, F.rho, F.a32.p, nv, nv, F.rev, F.sc.p, s);
         HIPCHK(ldl_factor32(nv, F.a32.p, nv, F.w32.p, F.work32.p, s));
         pivots_to_host(true);
         bool ok = true;
@@ -2301,23 +2260,68 @@ static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_
             evmin = std::min(evmin, d);
         }
         if (ok) {
-            loc.nrank = nv;
-            loc.evmax = evmax;
-            loc.evmin = evmin;
-        } else {
-            // the FP32 factor is not positive definite: it cannot precondition CG
-            fp32 = false;
-            loc.method = MPGPU_DECOMPOSITION;
-            loc.istop = 3;
-            factor64();
+            F.nrank = nv;
+            F.evmax = evmax;
+            F.evmin = evmin;
+            F.anorm = std::sqrt(l_packed_fro2(nv, c->d_mat.p, c->d_part.p, c->d_scal.p, s)) + F.rho * cc_norm;
+            return;
         }
-    } else {
-        factor64();
+        // the FP32 factor is not positive definite: it cannot precondition CG
     }
+    F.fp32 = false;
+    F.a32.release(); F.w32.release(); F.work32.release();
+    F.a.alloc(nn);
+    F.w.alloc(nv);
+    F.work.alloc(ldl_work_elems(nv));
+    l_packed_to_full(nv, c->d_mat.p, F.a.p, nv, F.rev, s);
+    l_add_cct(m, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, F.rho, F.a.p, nv, nv, F.rev, s);
+    HIPCHK(ldl_factor(nv, F.a.p, nv, F.w.p, F.work.p, s));
+    pivots_to_host(false);
+    int nrank = 0;
+    double evmax = 0.0, evmin = 0.0;
+    for (int k = 0; k < nv; k++) {
+        const double d = F.piv[k];
+        if (d > 0.0) {
+            if (nrank == 0) evmax = evmin = d;
+            evmax = std::max(evmax, d);
+            evmin = std::min(evmin, d);
+            nrank++;
+        } else if (lapack) {
+            // dpptrf: 'factorization ... failed at index', exit 29 (src/mplapack.f90:144-156)
+            throw MpError(29, "Aborted, factorization of global matrix failed at index " + std::to_string(k + 1));
+        }
+    }
+    F.nrank = nrank;
+    F.evmax = evmax;
+    F.evmin = evmin;
+}
+
+static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_stats &loc)
+{
+    hipStream_t s = c->stream;
+    const int64_t n = c->nall;
+    const int nv = c->nvar, m = c->ncon;
+    const bool lapack = c->opt.metsol == MPGPU_LAPACK_PACKED || c->opt.metsol == MPGPU_LAPACK_UNPACKED;
+    // 'LAPACKwitherrors': the parameter covariance on the constraint surface, as pede's
+    // inverse from the factor expanded by Q (src/pede.f90:9017-9120); with multipliers the
+    // same matrix is diag(A^-1) - diag(W S^-1 W^T), W = A^-1 C^T, S = C W, and A may carry
+    // the rho C^T C term (it cancels on the constraint surface)
+    const bool want_cov = c->opt.lapack_errors && lapack;
+    c->have_cov = false;
+    std::vector<double> covcorr(want_cov ? nv : 0, 0.0);
+    if (!c->dfac || c->dfac->metsol != c->opt.metsol) {
+        c->dfac.reset(new DenseFactor());
+        dense_factor(c, *c->dfac, c->opt.metsol == MPGPU_CHOLESKY_FP32_IR);
+    }
+    DenseFactor &F = *c->dfac;
+    if (!F.fp32 && c->opt.metsol == MPGPU_CHOLESKY_FP32_IR) { loc.method = MPGPU_DECOMPOSITION; loc.istop = 3; }
+    loc.nrank = F.nrank;
+    loc.evmax = F.evmax;
+    loc.evmin = F.evmin;
+    const double rho = F.rho;
     const double eps = std::numeric_limits<double>::epsilon();
     const double tol = c->opt.refine_tol > 0.0 ? c->opt.refine_tol : eps * std::sqrt((double)nv);
-    double anorm = 0.0;
-    if (fp32) anorm = std::sqrt(l_packed_fro2(nv, c->d_mat.p, c->d_part.p, c->d_scal.p, s)) + rho * cc_norm;
+    const bool verbose = std::getenv("MPGPU_VERBOSE") != nullptr;
 
     // FP32-factor-preconditioned conjugate gradients on the (augmented) parameter block, nrhs
     // columns side by side with their own recurrences.  A column has converged when
@@ -2370,7 +2374,10 @@ static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_
             int left = 0;
             for (int q = 0; q < nrhs; q++) {
                 if (!active[q]) continue;
-                if (std::sqrt(rr[q]) <= tol * anorm * std::sqrt(xx[q])) { active[q] = 0; conv[q] = 1; }
+                if (verbose && q == 0)
+                    std::fprintf(stderr, "mpgpu: refinement step %d: ||r|| / (||A||_F ||x||) = %.3e (tol %.3e), ||r||/||b|| = %.3e\n",
+                                 it, std::sqrt(rr[q]) / (F.anorm * std::sqrt(xx[q])), tol, std::sqrt(rr[q] / bb[q]));
+                if (std::sqrt(rr[q]) <= tol * F.anorm * std::sqrt(xx[q])) { active[q] = 0; conv[q] = 1; }
                 else left++;
             }
             if (left == 0 || it == maxit) break;
@@ -2388,17 +2395,19 @@ static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_
         return true;
     };
     // A v = rhs for nrhs columns: FP64 factor, or FP32-preconditioned CG with the FP64 factor
-    // as the fallback when it does not converge
+    // as the fallback when it does not converge (the FP64 factor then replaces the kept one)
     auto solveA = [&](double *v, int nrhs) {
-        if (!fp32) { F.solve64(v, nrhs, s); return; }
+        if (!F.fp32) { F.solve64(v, nrhs, s); return; }
         DBuf<double> keep;
         keep.alloc((size_t)nv * nrhs);
         l_copy((int64_t)nv * nrhs, v, keep.p, s);
         if (refine(v, nrhs)) { loc.istop = 1; return; }
-        fp32 = false;
         loc.method = MPGPU_DECOMPOSITION;
         loc.istop = 3;
-        factor64();
+        dense_factor(c, F, false);
+        loc.nrank = F.nrank;
+        loc.evmax = F.evmax;
+        loc.evmin = F.evmin;
         l_copy((int64_t)nv * nrhs, keep.p, v, s);
         F.solve64(v, nrhs, s);
     };
@@ -2564,6 +2573,7 @@ extern "C" int mpgpu_solve(mpgpu_ctx *c, const double *rhs, double *x, mpgpu_sol
         std::swap(keep.p, c->d_mat.p);
         c->have_inverse = true;
         c->pc.reset();
+        c->dfac.reset();
     } else if (c->opt.metsol == MPGPU_MINRES) {
         if (c->elim) elim_solve(c, b.p, xd.p, &loc, false);
         else {

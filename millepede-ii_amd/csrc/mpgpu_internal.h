@@ -166,11 +166,13 @@ void ldl_inv_diag(int n, const double *a, int lda, const double *w, const double
                   hipStream_t s);
 void l_reverse(int n, double *x, int nrhs, int64_t ldx, hipStream_t s);
 void l_packed_to_full(int n, const double *packed, double *full, int lda, int rev, hipStream_t s);
-void l_packed_to_full32(int n, const double *packed, float *full, int lda, int rev, hipStream_t s);
+// sc (may be NULL): symmetric scaling, element (i, j) multiplied by sc[i] sc[j]
+void l_packed_to_full32(int n, const double *packed, float *full, int lda, int rev, const double *sc, hipStream_t s);
+void l_scale_cols(int n, const double *d, double *x, int nrhs, int64_t ldx, hipStream_t s);
 void l_add_cct(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, double *a,
                int lda, int n, int rev, hipStream_t s);
 void l_add_cct32(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho, float *a,
-                 int lda, int n, int rev, hipStream_t s);
+                 int lda, int n, int rev, const double *sc, hipStream_t s);
 void l_cct_apply(int32_t ncon, const int64_t *cptr, const int32_t *cidx, const double *cval, double rho,
                  const double *x, double *y, hipStream_t s);
 void l_col_dots(int n, int nrhs, const double *a, const double *b, double *out, hipStream_t s);
@@ -178,6 +180,38 @@ void l_col_axpy(int n, int nrhs, const double *alpha, double sg, const double *x
 void l_col_xpby(int n, int nrhs, const double *z, const double *beta, double *p, hipStream_t s);
 double l_packed_fro2(int n, const double *a, double *part, double *scal, hipStream_t s);
 void l_zero(int64_t n, double *y, hipStream_t s);
+
+// factor of the dense solve methods, kept until the matrix changes (as pede's mchdec decomposes
+// only after a matrix pass and otherwise reuses the decomposition, src/pede.f90:8024-8047)
+struct DenseFactor {
+    int n = 0;
+    int metsol = 0;      // method the factor was made for
+    bool rev = false;    // factor of the index-reversed matrix (chdec2's pivot order)
+    bool fp32 = false;   // FP32 factor (refinement in FP64); false: FP64 factor
+    double rho = 0.0, anorm = 0.0;   // weight of the augmenting C^T C term, ||A + rho C^T C||_F bound
+    int nrank = 0;
+    double evmax = 0.0, evmin = 0.0;
+    DBuf<double> a, w, work;
+    DBuf<float> a32, w32, work32;
+    DBuf<double> sc;     // FP32 factor: it factors D A D with D = diag(A)^-1/2 (unit diagonal)
+    std::vector<double> piv;
+    // v <- A^-1 v for nrhs columns of length n (leading dimension n) with the FP64 factor
+    void solve64(double *v, int nrhs, hipStream_t s)
+    {
+        if (rev) l_reverse(n, v, nrhs, n, s);
+        ldl_solve(n, a.p, n, w.p, work.p, v, nrhs, n, s);
+        if (rev) l_reverse(n, v, nrhs, n, s);
+    }
+    // A^-1 = D (D A D)^-1 D
+    void solve32(double *v, int nrhs, hipStream_t s)
+    {
+        l_scale_cols(n, sc.p, v, nrhs, n, s);
+        if (rev) l_reverse(n, v, nrhs, n, s);
+        ldl_solve32(n, a32.p, n, w32.p, work32.p, v, nrhs, n, s);
+        if (rev) l_reverse(n, v, nrhs, n, s);
+        l_scale_cols(n, sc.p, v, nrhs, n, s);
+    }
+};
 
 // ---- constraint elimination (mpgpu_qldec.cpp, k_qlelim.hip) ----
 // device view of the reflectors: rows over parameter ranges, the transpose per parameter, the WY
