@@ -187,6 +187,13 @@ int mpgpu_mupdat(mpgpu_ctx *ctx, int32_t nel, const int32_t *i, const int32_t *j
 int mpgpu_solve(mpgpu_ctx *ctx, const double *rhs, double *x, mpgpu_solve_stats *st);
 /* y = A x (avprod, src/pede.f90:5331-5358), x,y host arrays of nall */
 int mpgpu_avprod(mpgpu_ctx *ctx, const double *x, double *y);
+/* y = A x like mpgpu_avprod, with x and y (nall each) in device memory, allocating no device
+ * memory: no counterpart in the reference; for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  Full storage only, nall <=
+ * 16384 (else 30).  The product's partial sums use a fixed ~19 MB device array per GPU, held
+ * until the process ends and shared by every call on that GPU: calls that overlap on one GPU
+ * race.  Returns after the product is complete. */
+int mpgpu_avprod_noalloc(mpgpu_ctx *ctx, const double *x_dev, double *y_dev);
 /* parameter errors sqrt|A^-1_ii| and global correlations (prtglo, src/pede.f90:4538-4546);
  * after an inversion solve, or a LAPACK-method solve with lapack_errors set (then from the
  * covariance of the constrained parameters, src/pede.f90:9017-9120).  err, gcor: nvar

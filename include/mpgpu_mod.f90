@@ -182,6 +182,11 @@ MODULE mpgpu_mod
             REAL(c_double), INTENT(IN) :: x(*)
             REAL(c_double), INTENT(OUT) :: y(*)
         END FUNCTION
+        ! y = A x with x, y device pointers, allocating nothing (full storage, nall <= 16384)
+        INTEGER(c_int) FUNCTION mpgpu_avprod_noalloc(ctx, x_dev, y_dev) BIND(C, name='mpgpu_avprod_noalloc')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx, x_dev, y_dev
+        END FUNCTION
         INTEGER(c_int) FUNCTION mpgpu_get_errors(ctx, err, gcor) BIND(C, name='mpgpu_get_errors')
             IMPORT :: c_int, c_ptr, c_double
             TYPE(c_ptr), VALUE :: ctx

@@ -2,6 +2,7 @@
 // matrix-vector products for packed and group-block storage, MINRES vector algebra and the
 // MINRES preconditioners.
 #include "mpgpu_device.h"
+#include <mutex>
 #include <float.h>
 
 namespace mpgpu {
@@ -332,6 +333,47 @@ int64_t packed_symv_scratch(int n)
     const int64_t nb = (n + ST - 1) / ST;
     const int64_t nseg = (nb + SSEG - 1) / SSEG;
     return nseg * n + 32 * nb * (nb + 1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free product for callers that cannot allocate device memory: the partial sums of
+// the packed product live in a file-scope device array of fixed size (enough for matrices of
+// NOALLOC_MAX_N rows), one copy per GPU, shared by every call on that GPU.  Its device address
+// is looked up once per device, with that device current; kernels get that address, never the
+// array's host-side name.  Each call fully overwrites the part of the array it reads, so no
+// clearing is needed; calls that overlap on one GPU race for the array.
+// ---------------------------------------------------------------------------------------------
+#define NOALLOC_MAX_N 16384
+#define NOALLOC_WORDS 2367488   // packed_symv_scratch(16384): 16 * 16384 + 32 * 256 * 257
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+
+int noalloc_max_n() { return NOALLOC_MAX_N; }
+
+static std::mutex noalloc_mu;
+static double *noalloc_ptr[256];
+
+// device address of this GPU's array (device must be current); nullptr on failure
+static double *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= 256) return nullptr;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        noalloc_ptr[device] = (double *)p;
+    }
+    return noalloc_ptr[device];
+}
+
+void l_packed_symv(int n, const double *a, const double *x, double *y, double *scratch, hipStream_t s);
+// y = A x for the packed matrix of n <= NOALLOC_MAX_N rows, x and y device arrays
+hipError_t noalloc_packed_symv(int device, int n, const double *a, const double *x, double *y, hipStream_t s)
+{
+    if (n > NOALLOC_MAX_N || packed_symv_scratch(n) > NOALLOC_WORDS) return hipErrorInvalidValue;
+    double *sc = noalloc_scratch(device);
+    if (!sc) return hipErrorInvalidSymbol;
+    l_packed_symv(n, a, x, y, sc, s);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -2616,6 +2616,23 @@ extern "C" int mpgpu_avprod(mpgpu_ctx *c, const double *x, double *y)
     API_CATCH(c)
 }
 
+// y = A x with x, y in device memory, allocating nothing (full storage, nall <= 16384)
+extern "C" int mpgpu_avprod_noalloc(mpgpu_ctx *c, const double *x_dev, double *y_dev)
+{
+    if (!c || !c->built) return 30;
+    if (!x_dev || !y_dev) return fail(c, 30, "null device pointer");
+    if (c->opt.matsto != MPGPU_FULL) return fail(c, 30, "the allocation-free product needs full matrix storage");
+    if (c->have_inverse) return fail(c, 30, "matrix holds its inverse after an inversion solve");
+    if (c->nall > noalloc_max_n())
+        return fail(c, 30, "the allocation-free product is limited to " + std::to_string(noalloc_max_n()) +
+                               " rows (matrix has " + std::to_string(c->nall) + ")");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, 30, "hipSetDevice failed");
+    hipError_t e = noalloc_packed_symv(c->device, (int)c->nall, c->d_mat.p, x_dev, y_dev, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, 30, std::string("HIP error: ") + hipGetErrorString(e));
+    return 0;
+}
+
 extern "C" int mpgpu_get_errors(const mpgpu_ctx *c0, double *err, double *gcor)
 {
     mpgpu_ctx *c = const_cast<mpgpu_ctx *>(c0);

@@ -108,6 +108,10 @@ void l_gj_finish(int n, double *v, double *b, const int *used, const int *info, 
 // y = A x for the packed symmetric matrix; scratch: packed_symv_scratch(n) doubles
 void l_packed_symv(int n, const double *a, const double *x, double *y, double *scratch, hipStream_t s);
 int64_t packed_symv_scratch(int n);
+// the same product with the partial sums in a fixed file-scope device array (no allocation);
+// n <= noalloc_max_n()
+hipError_t noalloc_packed_symv(int device, int n, const double *a, const double *x, double *y, hipStream_t s);
+int noalloc_max_n();
 void l_mr_scale(int64_t n, const MrState *st, const double *y, double *v, hipStream_t s);
 void l_mr_axpy(int64_t n, const MrState *st, int which, const double *r, double *y, hipStream_t s);
 void l_mr_step(MrState *st, hipStream_t s);

@@ -29,7 +29,7 @@ EXPORTS = [
     "mpgpu_add_records", "mpgpu_read_mille_file", "mpgpu_set_steering_parameters", "mpgpu_add_equations",
     "mpgpu_build_structure", "mpgpu_get_labels", "mpgpu_matrix_info", "mpgpu_set_parameters",
     "mpgpu_get_parameters", "mpgpu_loopn", "mpgpu_get_vector", "mpgpu_get_matrix_packed", "mpgpu_mupdat",
-    "mpgpu_solve", "mpgpu_avprod", "mpgpu_get_errors", "mpgpu_get_monitor", "mpgpu_run_steering",
+    "mpgpu_solve", "mpgpu_avprod", "mpgpu_avprod_noalloc", "mpgpu_get_errors", "mpgpu_get_monitor", "mpgpu_run_steering",
     "mpgpu_server_create", "mpgpu_server_submit", "mpgpu_server_wait", "mpgpu_server_destroy",
 ]
 
@@ -112,6 +112,7 @@ def lib():
         L.mpgpu_mupdat.argtypes = [C.c_void_p, C.c_int32, I32, I32, D]
         L.mpgpu_solve.argtypes = [C.c_void_p, D, D, C.POINTER(SolveStats)]
         L.mpgpu_avprod.argtypes = [C.c_void_p, D, D]
+        L.mpgpu_avprod_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.mpgpu_get_errors.argtypes = [C.c_void_p, D, D]
         L.mpgpu_get_monitor.argtypes = [C.c_void_p, I32, I32, C.POINTER(C.c_float), I64]
         L.mpgpu_run_steering.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(RunStats)]
