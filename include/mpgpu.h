@@ -87,10 +87,9 @@ typedef struct mpgpu_options {
     int32_t resolve_redundancy_cons; /* 'resolveredundancycons' (irslvrc): a constraint */
                              /* block whose product matrix is rank deficient is reduced */
                              /* to its independent constraints (src/pede.f90:1028-1050) */
-    float scale_errors[2];   /* 'scaleerrors f1 f2': errorScaleFactor, measurement      */
+    double scale_errors[2];  /* 'scaleerrors f1 f2': errorScaleFactor, measurement      */
                              /* errors with (1) / without (2) global derivatives are     */
                              /* multiplied by it (src/mppar.f90:312-316); default 1, 1   */
-    int32_t reserved[2];
     double refine_tol;       /* FP32+IR: stop when ||r|| <= refine_tol ||x|| ||A||_F     */
                              /* (0: eps sqrt(n), the test of LAPACK's DSPOSV)            */
 } mpgpu_options;

@@ -17,8 +17,7 @@ MODULE mpgpu_mod
         INTEGER(c_int32_t) :: monitor, monitor_bins
         INTEGER(c_int32_t) :: lapack_errors
         INTEGER(c_int32_t) :: pair_entries, skip_empty_cons, resolve_redundancy_cons
-        REAL(c_float)      :: scale_errors(2)
-        INTEGER(c_int32_t) :: reserved(2)
+        REAL(c_double)     :: scale_errors(2)
         REAL(c_double)     :: refine_tol
     END TYPE mpgpu_options
 

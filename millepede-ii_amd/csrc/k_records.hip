@@ -421,7 +421,8 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 double rmeas = lwv(sl);
                 for (int j = 1; j <= ng; j++) rmeas = rmeas - lwv(sg + j) * gp[li[sg + j]];
                 rm[i] = rmeas;
-                re[i] = lwv(sg);
+                // 'scaleerrors' (prepareRecords, src/mppar.f90:312-316)
+                re[i] = lwv(sg) * (ng > 0 ? o.esc[0] : o.esc[1]);
                 lc[i] = 0.0;
                 for (int j = 1; j <= nl; j++) nalc_t = max(nalc_t, li[sl + j]);
             }

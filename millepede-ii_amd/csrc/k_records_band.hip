@@ -241,7 +241,8 @@ __global__ __launch_bounds__(BNT) void k_records_band(DevRecords R, DevModel M, 
             double rmeas = bval<T>(R, sl);
             for (int j = 1; j <= ng; j++) rmeas = rmeas - bval<T>(R, sg + j) * gp[R.widx[sg + j]];
             rm[i] = rmeas;
-            re[i] = bval<T>(R, sg);
+            // 'scaleerrors' (prepareRecords, src/mppar.f90:312-316)
+            re[i] = bval<T>(R, sg) * (ng > 0 ? o.esc[0] : o.esc[1]);
             lc[i] = 0.0;
             int mx = 0;
             for (int j = 1; j <= nl; j++) mx = max(mx, R.widx[sl + j]);

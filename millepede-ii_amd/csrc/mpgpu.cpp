@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <functional>
 #include <limits>
 #include <numeric>
 #include <omp.h>
@@ -176,8 +177,8 @@ extern "C" void mpgpu_default_options(mpgpu_options *o)
     o->pair_entries = 1;
     o->skip_empty_cons = 0;
     o->resolve_redundancy_cons = 0;
-    o->scale_errors[0] = 1.0f;
-    o->scale_errors[1] = 1.0f;
+    o->scale_errors[0] = 1.0;
+    o->scale_errors[1] = 1.0;
     o->refine_tol = 0.0;
 }
 
@@ -586,10 +587,13 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     for (auto &x : cnt) x.store(0, std::memory_order_relaxed);
     std::vector<std::atomic<uint8_t>> bnd(ntot + 1);
     for (auto &x : bnd) x.store(0, std::memory_order_relaxed);
+    // group boundaries of one sorted list (pargrp, src/pede.f90:2311-2404): at its first element,
+    // after its last, and at both sides of every step that is not +1 -- a label repeated within
+    // one equation (pede sorts equation lists without removing duplicates, 2259-2270) is such a
+    // step, so it closes the group before it and starts its own
     auto mark_list = [&](std::vector<int32_t> &lst) {
         if (lst.empty()) return;
         std::sort(lst.begin(), lst.end());
-        lst.erase(std::unique(lst.begin(), lst.end()), lst.end());
         bnd[lst[0]].store(1, std::memory_order_relaxed);
         for (size_t q = 1; q < lst.size(); q++)
             if (lst[q] != lst[q - 1] + 1) { bnd[lst[q - 1] + 1].store(1, std::memory_order_relaxed); bnd[lst[q]].store(1, std::memory_order_relaxed); }
@@ -665,7 +669,9 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
                 const int64_t sg = eq_sl[e] + nl + 1;
                 const int32_t t = tix_of(I[sg + 1]);
                 mcnt[t]++;
-                msum[t] = msum[t] + val(sg);
+                // 'scaleerrors' applies to the errors pede reads here as well (prepareRecords
+                // before loop2, src/pede.f90:6638)
+                msum[t] = msum[t] + val(sg) * c->opt.scale_errors[0];
             }
         mon_grp.assign(ntot, -1);
         for (int32_t t = 0; t < ntot; t++)
@@ -685,12 +691,36 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->par[t] = c->sp_value[k];
         c->presig_tot[t] = c->sp_presig[k];
     }
+    // constraint groups (grpcon, src/pede.f90:777-1055): constraints joined by common parameters.
+    // A group with as many constraints as parameters fixes those parameters completely; with
+    // 'resolveredundancycons' its parameters become fixed (-4) and its constraints are dropped.
+    std::vector<char> redundant(ntot, 0), con_dropped(c->cons.size(), 0);
+    if (c->opt.resolve_redundancy_cons && !c->cons.empty()) {
+        const size_t nc = c->cons.size();
+        std::vector<size_t> up(nc);
+        std::iota(up.begin(), up.end(), 0);
+        std::function<size_t(size_t)> root = [&](size_t a) { while (up[a] != a) a = up[a] = up[up[a]]; return a; };
+        std::vector<int64_t> owner(ntot, -1);
+        for (size_t j = 0; j < nc; j++)
+            for (int32_t lab : c->cons[j].labels) {
+                const int32_t t = tix_of(lab);
+                if (owner[t] < 0) owner[t] = (int64_t)j;
+                else up[root(j)] = root((size_t)owner[t]);
+            }
+        std::vector<int64_t> ncon_g(nc, 0), npar_g(nc, 0);
+        for (size_t j = 0; j < nc; j++) ncon_g[root(j)]++;
+        for (int32_t t = 0; t < ntot; t++) if (owner[t] >= 0) npar_g[root((size_t)owner[t])]++;
+        for (int32_t t = 0; t < ntot; t++)
+            if (owner[t] >= 0) { const size_t g = root((size_t)owner[t]); if (ncon_g[g] == npar_g[g]) redundant[t] = 1; }
+        for (size_t j = 0; j < nc; j++) { const size_t g = root(j); if (ncon_g[g] == npar_g[g]) con_dropped[j] = 1; }
+    }
     c->var_index.assign(ntot, 0);
     c->var_to_tot.clear();
     int32_t nv = 0;
     for (int32_t t = 0; t < ntot; t++) {
         if (c->presig_tot[t] < 0.0) c->var_index[t] = -1;
         else if (c->counters[t] < c->opt.mreqenf) c->var_index[t] = -2;
+        else if (redundant[t]) c->var_index[t] = -4;
         else { c->var_index[t] = ++nv; c->var_to_tot.push_back(t); }
     }
     c->nvar = nv;
@@ -738,13 +768,18 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->cval.clear();
     c->con_tot.clear();
     c->crhs.clear();
-    for (auto &b : c->cons) {
+    int64_t nempty = 0;
+    for (size_t jc = 0; jc < c->cons.size(); jc++) {
+        const auto &b = c->cons[jc];
+        if (con_dropped[jc]) continue;
         std::vector<std::pair<int32_t, double>> tl;
         for (size_t q = 0; q < b.labels.size(); q++) tl.push_back({tix_of(b.labels[q]), b.factors[q]});
-        // rows with no variable parameter are skipped (skipemptycons semantics)
+        // a constraint without any variable parameter is empty: skipped with 'skipemptycons';
+        // without it pede checks the input only and ends without a solution (exit 5,
+        // src/pede.f90:1160-1190)
         bool any = false;
         for (auto &p : tl) if (c->var_index[p.first] > 0) any = true;
-        if (!any) continue;
+        if (!any) { nempty++; continue; }
         c->con_tot.push_back(tl);
         c->crhs.push_back(b.rhs);
         // accumulate factors per variable (duplicates add, as repeated mupdat calls do)
@@ -761,6 +796,9 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         }
         c->cptr.push_back((int64_t)c->cidx.size());
     }
+    if (nempty > 0 && !c->opt.skip_empty_cons)
+        throw MpError(5, std::to_string(nempty) + " empty constraints detected (use option 'skipemptycons' to skip "
+                         "those): ended without solution");
     // Without 'withmultipliers' the MINRES methods eliminate the constraints (pede's default,
     // src/pede.f90:6492-6498); inversion and decomposition keep Lagrange multipliers (the same
     // constrained solution; decomposition augments the parameter block, see mpgpu_solve)
@@ -1370,6 +1408,8 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
     o.chhuge = c->opt.chhuge;
     o.dwcut = c->opt.dwcut;
     o.cndlmx = c->opt.cndlmx;
+    o.esc[0] = c->opt.scale_errors[0];
+    o.esc[1] = c->opt.scale_errors[1];
     RecOut out{c->d_chi2.p, c->d_ndf.p, c->d_status.p};
     DevModel M = model_of(c);
     DevRecords R = records_of(c);

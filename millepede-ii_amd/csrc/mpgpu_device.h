@@ -52,6 +52,7 @@ struct DevModel {
 struct PassOpts {
     int32_t icalcm, nloopn, lhuber, lfitbb;
     float chicut, chhuge, dwcut, cndlmx;
+    double esc[2];   // errorScaleFactor: measurement errors with / without global derivatives
 };
 
 // residual monitoring (pede's measHists): per total index the measurement group of a

@@ -31,7 +31,10 @@ struct Steering {
     std::vector<ParLine> params;
     std::vector<EqLine> cons, meas;
     mpgpu_options opt;
-    int mitera = 1, matrit = 0, lsearch = 2, ipcntr = 0, igcorr = 0, isubit = 0, nofeas = 0;
+    int mitera = 1, matrit = 0, lsearch = 2, ipcntr = 0, igcorr = 0, isubit = 0, nofeas = 0, iforce = 0;
+    std::vector<std::string> unknown;      // keywords pede does not know (exit 13)
+    std::vector<std::string> missing;      // file names (with an extension) that do not exist (exit 15/16)
+    std::vector<std::string> unsupported;  // pede keywords that would change the result but are not built (exit 40)
     std::vector<int32_t> errlabels;   // 'errlabels': MINRES error estimates (at most 100 labels)
     int imonit = 0;                    // 'monitorresiduals'/'monitorpulls': bit 0 first, bit 1 last pass, < 0 all
     float dflim = 0.0f;
@@ -126,6 +129,7 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
         if (key_match(k, "method")) {
             std::string m = t.size() > 1 ? lower(t[1]) : "";
             S.method_given = true;
+            S.err.clear();          // the last method line decides (CI/mp2str.txt lists several)
             if (m == "inversion") { S.opt.metsol = MPGPU_INVERSION; S.opt.matsto = MPGPU_FULL; }
             else if (m == "decomposition" || m == "cholesky") { S.opt.metsol = MPGPU_DECOMPOSITION; S.opt.matsto = MPGPU_FULL; }
             else if (m == "fullminres") { S.opt.metsol = MPGPU_MINRES; S.opt.matsto = MPGPU_FULL; }
@@ -195,7 +199,21 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
             continue;
         }
         if (key_match(k, "subito")) { S.isubit = 1; continue; }
-        if (key_match(k, "nofeasibilitycheck")) { S.nofeas = 1; continue; }
+        if (key_match(k, "nofeasiblestart")) { S.nofeas = 1; continue; }
+        if (key_match(k, "force")) { S.iforce = 1; continue; }
+        if (key_match(k, "scaleerrors")) {
+            // src/pede.f90:10697-10704: one factor for both kinds, or one each
+            if (nums.size() > 0) S.opt.scale_errors[0] = S.opt.scale_errors[1] = nums[0];
+            if (nums.size() > 1) S.opt.scale_errors[1] = nums[1];
+            continue;
+        }
+        if (key_match(k, "skipemptycons")) { S.opt.skip_empty_cons = 1; continue; }
+        if (key_match(k, "resolveredundancycons")) { S.opt.resolve_redundancy_cons = 1; continue; }
+        if (key_match(k, "pairentries")) {
+            // mreqpe > 1 drops rarely filled matrix blocks (src/pede.f90:10762-10772): not built
+            if (!nums.empty() && nums[0] > 1.5) S.unsupported.push_back("pairentries");
+            continue;
+        }
         if (key_match(k, "withmultipliers")) { S.opt.withmultipliers = 1; continue; }
         if (key_match(k, "withelimination")) { S.opt.withmultipliers = 0; continue; }
         if (key_match(k, "countrecords")) { S.opt.mcount = 1; continue; }
@@ -228,8 +246,28 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
                 }
                 continue;
             }
+            if (f.find('.') != std::string::npos && t.size() == 1) { S.missing.push_back(f); continue; }
         }
-        // other keywords (threads, cache, compress, monitoring, ...) do not change the path
+        {
+            // pede keywords that change the results but are not part of this path
+            static const char *changes[] = {"weightedcons", "iterateentries", "checkinput", "postprocessing",
+                                            "maxrecord"};
+            // pede keywords without effect on the build and solve (threads, caches, printout,
+            // monitoring of progress, PARDISO tuning; 'wolfe' sets only the c1 constant, which
+            // the line search does not use, src/pede.f90:7389-7406)
+            static const char *neutral[] = {"threads", "cache", "compress", "print", "debug", "printrecord",
+                                            "closeandreopen", "matmoni", "outlierfracwarnthreshold", "pullrange",
+                                            "memorydebug", "checkparametergroups", "monitorprogress", "histprint",
+                                            "readerroraseof", "debugpardiso", "blocksizepardiso",
+                                            "withlapackelimination", "wolfe"};
+            bool known = false;
+            for (const char *w : changes)
+                if (key_match(k, w)) { S.unsupported.push_back(w); known = true; break; }
+            if (!known)
+                for (const char *w : neutral)
+                    if (key_match(k, w)) { known = true; break; }
+            if (!known) S.unknown.push_back(k);
+        }
     }
     return true;
 }
@@ -711,6 +749,22 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     int ftype = 1;
     if (!parse_text(S, sf, dir, true, ftype)) { std::fprintf(stderr, "%s\n", S.err.c_str()); return finish(10); }
     if (S.opt.metsol < 0) { std::fprintf(stderr, "%s\n", S.err.c_str()); return finish(36); }
+    for (auto &f : S.missing) {
+        // open errors for text (16) or binary (15) files (src/pede.f90:9700-9842)
+        std::string tail = lower(f.size() >= 3 ? f.substr(f.size() - 3) : f);
+        std::fprintf(stderr, " Open error for file %s\n", f.c_str());
+        return finish(tail == "txt" ? 16 : 15);
+    }
+    if (!S.unknown.empty()) {
+        // pede: 'Unknown keyword(s)' for each, then exit 13 (src/pede.f90:10886-10896, 10015-10020)
+        for (auto &u : S.unknown) std::fprintf(stderr, " Unknown keyword(s): %s\n", u.c_str());
+        return finish(13);
+    }
+    if (!S.unsupported.empty()) {
+        std::fprintf(stderr, "pede(gpu): steering keyword '%s' changes the result and is not supported\n",
+                     S.unsupported[0].c_str());
+        return finish(40);
+    }
     if (S.binfiles.empty()) return finish(11);
     mpgpu_ctx *c = nullptr;
     int rc = mpgpu_create(&S.opt, device, &c);
@@ -914,6 +968,16 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
             mpgpu_solve_stats sst{};
             if ((rc = mpgpu_solve(c, b.data(), x.data(), &sst))) return bail(rc);
             tsol += now_ms() - t;
+            if (metsol == MPGPU_DECOMPOSITION && sst.nrank < nall) {
+                // mchdec: a rank defect of the decomposition enforces SUBITO unless 'force'
+                // (src/pede.f90:8026-8037)
+                std::printf(" Warning: the rank defect of the symmetric %d-by-%d matrix is %d (should be zero).\n", nall,
+                            nall, nall - sst.nrank);
+                if (S.iforce == 0 && S.isubit == 0) {
+                    S.isubit = 1;
+                    std::printf("          --> enforcing SUBITO mode\n");
+                }
+            }
             nloopsol = nloopn;
             if (metsol == MPGPU_MINRES || metsol == MPGPU_MINRES_QLP) { iitera = sst.itn; istopa = sst.istop; }
             if (metsol == MPGPU_INVERSION ||

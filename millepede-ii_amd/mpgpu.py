@@ -43,7 +43,7 @@ class Options(C.Structure):
                 ("default_presigma", C.c_double), ("mrestl", C.c_double), ("mrtcnd", C.c_double),
                 ("nregul", C.c_int32), ("refine_steps", C.c_int32), ("monitor", C.c_int32), ("monitor_bins", C.c_int32),
                 ("lapack_errors", C.c_int32), ("pair_entries", C.c_int32), ("skip_empty_cons", C.c_int32),
-                ("resolve_redundancy_cons", C.c_int32), ("scale_errors", C.c_float * 2), ("reserved", C.c_int32 * 2),
+                ("resolve_redundancy_cons", C.c_int32), ("scale_errors", C.c_double * 2),
                 ("refine_tol", C.c_double)]
 
 

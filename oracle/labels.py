@@ -3,12 +3,14 @@ src/pede.f90:5777-6146; decodeNextMeasurement 2428-2468; updateParGroups/pargrp 
 
 TEST INFRASTRUCTURE ONLY (checker for the library's index maps).
 
-Group partition: pede builds parameter groups incrementally (pargrp) while reading; the
-partition it reaches is "a boundary between adjacent labels k, k+1 (in ascending label
-order) wherever some equation (or record, with countrecords) holds exactly one of the two".
-That rule is what this module computes; it is pinned by brute force below (every
-equation's label set is a union of whole groups, and no two adjacent groups could be
-merged without breaking that).
+Group partition: pede builds parameter groups incrementally (pargrp) while reading; for each
+sorted list (an equation's labels, duplicates kept, or with countrecords a record's distinct
+labels) the partition gets a boundary at the list's first element, after its last, and at
+both sides of every step that is not +1 (a repeated label is such a step).  That rule is what
+this module computes.  tests/test_oracle_labels.py pins it against a step-by-step restatement of
+pargrp's incremental procedure on random lists with gaps and repeats, and checks maximality by
+brute force (for repeat-free lists: every list is a union of whole groups, and no two adjacent
+groups can be merged without breaking that).
 """
 import numpy as np
 
@@ -61,7 +63,7 @@ def label_maps(rec_ptr, inder, glder, steer_labels=(), presigma=None, mreqenf=25
     def mark(lst):
         if not lst:
             return
-        lst = sorted(set(lst))
+        lst = sorted(lst)
         bnd[lst[0]] = True
         for a, b in zip(lst[:-1], lst[1:]):
             if b != a + 1:

@@ -74,3 +74,19 @@ END PROGRAM t
         r = subprocess.run([os.path.join(d, "t")], capture_output=True, text=True, timeout=60)
         assert r.returncode == 0, r.stderr
         assert r.stdout.split() == ["2000", "25", "1", "50.00"]
+
+
+def test_steering_keyword_exit_codes(tmp_path):
+    """The steering parser returns pede's exit codes before any device work: 13 for a keyword
+    pede does not know (src/pede.f90:10886-10896, 10015-10020), 40 for a pede keyword that would
+    change the result and is not part of this path, 15 for a missing binary file."""
+    import mpgpu
+    cases = {"unknown": ("notakeyword 3\nend\n", 13), "unsupported": ("weightedcons\nend\n", 40),
+             "pairs": ("pairentries 5\nend\n", 40), "missing": ("nosuchfile.bin\nend\n", 15)}
+    for name, (text, code) in cases.items():
+        f = tmp_path / f"{name}.txt"
+        f.write_text(text)
+        with pytest.raises(mpgpu.MpgpuError) as e:
+            mpgpu.run_steering(str(f), str(tmp_path))
+        assert e.value.code == code, (name, e.value)
+        assert (tmp_path / "millepede.end").read_text().split()[0] == str(code)

@@ -551,3 +551,122 @@ def test_mupdat_and_errors_codes():
     with pytest.raises(mpgpu.MpgpuError) as e:
         P.mupdat([P.nall + 5], [1], [1.0])
     assert e.value.code == 23
+
+
+def _duplicate_first_global(rp, ii, gg, every=3):
+    """Copy of the records where the first global word of the first measurement of every
+    `every`-th record appears twice (the same label and derivative repeated in one equation)."""
+    recs = LB.decode(rp, ii, gg)
+    out_i, out_g, ptr = [], [], [0]
+    for r, eqs in enumerate(recs):
+        w0, w1 = int(rp[r]), int(rp[r + 1])
+        I, G = list(ii[w0:w1]), list(gg[w0:w1])
+        if r % every == 0 and eqs:
+            sL, sG, lG = eqs[0]
+            if lG > sG:
+                k = sG + 1 - w0
+                I.insert(lG + 1 - w0, I[k])
+                G.insert(lG + 1 - w0, G[k])
+        out_i += I
+        out_g += G
+        ptr.append(len(out_i))
+    return np.array(ptr, np.int64), np.array(out_i, np.int32), np.array(out_g, np.float32)
+
+
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_label_maps_repeated_labels(orc, matsto):
+    """Counting per equation (pede's default, mcount 0) with a label repeated inside an equation:
+    pede keeps the repeat in the sorted equation list (src/pede.f90:2259-2270), which counts
+    both entries and splits the parameter group at that label (pargrp, 2357-2386).  Index maps
+    against the checker (itself pinned to pargrp's procedure), and the data pass against the
+    oracle."""
+    rp, ii, gg, _ = mpgpu.pixel_records(31, 4, 5, 3, 3000, 3, 6)
+    rp, ii, gg = _duplicate_first_global(rp, ii, gg)
+    P = _pede(rp, ii, gg, matsto=matsto, mreqenf=1, mcount=0)
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=1, mcount=0)
+    gsl = P.labels()[3]
+    assert len(set(gsl.tolist())) > len(set((lab // 10).tolist()))   # some module groups split
+    # The pass: chi^2, ndf, rejects, counters and vector equal the oracle's.  The matrix differs
+    # on the diagonal element of a repeated label: pede adds w d_j d_k over the pairs k <= j of
+    # the equation's entries (src/pede.f90:4196-4207), so two entries of one label contribute
+    # w (d1^2 + d1 d2 + d2^2); the device sums the entries of a label first, w (d1 + d2)^2.  The
+    # difference, w d1 d2 per such equation, is checked exactly.
+    nv = P.nvar
+    st = P.loopn()
+    mat, vec, cnt, so = orc.loopn(rp, ii, gg.astype(np.float64), lab, var, P.parameters(), nv)
+    assert st["nrej"] == so["nrej"] and st["ndf"] == so["ndf"]
+    assert abs(st["chi2"] - so["chi2"]) <= 1e-11 * abs(so["chi2"])
+    v, c = P.vector()
+    assert np.array_equal(c, cnt) and rel_err(v[:nv], vec) < 1e-10
+    A = packed_to_dense(P.matrix_packed(), P.nall)[:nv, :nv]
+    B = packed_to_dense(mat, nv)
+    pos = {int(l): i for i, l in enumerate(lab)}
+    extra = np.zeros(nv)
+    for eqs in LB.decode(rp, ii, gg):
+        for (sL, sG, lG) in eqs:
+            seen = {}
+            for w in range(sG + 1, lG + 1):
+                seen.setdefault(int(ii[w]), []).append(float(gg[w]))
+            for l, ds in seen.items():
+                for j in range(len(ds)):
+                    for k in range(j):
+                        extra[var[pos[l]] - 1] += ds[j] * ds[k] / float(gg[sG]) ** 2
+    assert np.count_nonzero(extra) > 0
+    assert rel_err(A, B + np.diag(extra)) < 1e-10
+
+
+def test_scaleerrors(orc):
+    """'scaleerrors f1 f2' multiplies the error of every measurement with global derivatives by
+    f1 and of every other one by f2 in FP64 when the records are prepared (src/mppar.f90:312-316):
+    the pass equals the oracle's pass over records whose error words were scaled the same way."""
+    rp, ii, gg = mpgpu.brokenlines_records(33, 6, 4, 3, 800, 4, 6)
+    f1, f2 = 1.7, 0.6
+    P = _pede(rp, ii, gg, matsto=mpgpu.SPARSE, mreqenf=1, mcount=1, scale_errors=(f1, f2))
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=1, mcount=1)
+    gs = gg.astype(np.float64)
+    for eqs in LB.decode(rp, ii, gg):
+        for (sL, sG, lG) in eqs:
+            gs[sG] *= f1 if lG > sG else f2
+    st = P.loopn()
+    mat, vec, cnt, so = orc.loopn(rp, ii, gs, lab, var, P.parameters(), P.nvar)
+    assert st["nrej"] == so["nrej"] and st["ndf"] == so["ndf"]
+    assert abs(st["chi2"] - so["chi2"]) <= 1e-11 * abs(so["chi2"])
+    v, _ = P.vector()
+    assert rel_err(v[:P.nvar], vec) < 1e-10
+    A = packed_to_dense(P.matrix_packed(), P.nall)[:P.nvar, :P.nvar]
+    assert rel_err(A, packed_to_dense(mat, P.nvar)) < 1e-10
+
+
+def test_empty_and_redundant_constraints():
+    """A constraint on parameters that are all fixed is empty: without 'skipemptycons' pede ends
+    without a solution (exit 5, src/pede.f90:1177-1190), with it the constraint is dropped.  With
+    'resolveredundancycons' a group of constraints with as many constraints as parameters fixes
+    those parameters (index -4) and drops the constraints (src/pede.f90:1015-1045, 5965-5966)."""
+    rp, ii, gg, _ = mpgpu.pixel_records(34, 4, 5, 3, 3000, 3, 6)
+    fixed = [100000 + 10 + k for k in range(1, 7)]               # layer 1, module 1: fixed
+    empty = [(0.0, 0.0, fixed[:2], [1.0, 1.0])]
+    red = [(0.0, 0.0, [200011], [1.0]), (0.0, 0.0, [200012, 200013], [1.0, 1.0]),
+           (0.0, 0.0, [200012, 200013], [1.0, -1.0])]           # 3 constraints on 3 parameters
+    ok = [(0.0, 0.0, [300011, 300021], [1.0, 1.0])]
+
+    def make(cons, **kw):
+        P = mpgpu.Pede(matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=1, mcount=1, withmultipliers=1, **kw)
+        P.add_records(rp, ii, gg)
+        P.set_steering_parameters(fixed, np.zeros(6), -np.ones(6))
+        P.add_constraints(cons)
+        return P
+    with pytest.raises(mpgpu.MpgpuError) as e:
+        make(empty + ok).build_structure()
+    assert e.value.code == 5
+    P = make(empty + ok, skip_empty_cons=1)
+    P.build_structure()
+    assert P.ncon == 1
+    Q = make(red + ok, resolve_redundancy_cons=1)
+    Q.build_structure()
+    lab, var = Q.labels()[:2]
+    pos = {int(l): i for i, l in enumerate(lab)}
+    assert [var[pos[l]] for l in (200011, 200012, 200013)] == [-4, -4, -4]
+    assert var[pos[200014]] > 0 and Q.ncon == 1
+    R = make(red + ok)
+    R.build_structure()
+    assert R.ncon == 4

@@ -207,6 +207,48 @@ def test_ci_wirechamber_minres_elimination(method):
 
 
 @pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
+def test_ci_wirechamber_verbatim_steering():
+    """CI/mp2str.txt exactly as the reference's CI job runs it (tests/golden/ci_mp2str.txt): six
+    method lines of which the last, 'fullLAPACK 3 0.8', applies; 'bandwidth 6 1',
+    'maxlocalcond 11', 'skipemptycons', 'resolveredundancycons', 'printcounts 2' and 'threads 24'
+    included.  All four goldens: millepede.res (parameters to 5 digits, counts exactly),
+    millepede.mon line for line, millepede.end, the iteration table of millepede.log, and the
+    final chi^2 of CI/refFiles/millepede.log:153 to 1e-9 relative."""
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([GEN, "100000"], cwd=d, check=True, capture_output=True)
+        steer = os.path.join(d, "mp2str.txt")
+        with open(os.path.join(ROOT, "tests", "golden", "ci_mp2str.txt")) as src, open(steer, "w") as dst:
+            dst.write(src.read())
+        st = mpgpu.run_steering(steer, d)
+        assert st["exit_code"] == 0
+        assert st["nrec"] == 100000 and st["ntot"] == 200 and st["ncon"] == 2
+        got = parse_res(os.path.join(d, "millepede.res"))
+        mon = open(os.path.join(d, "millepede.mon")).read().splitlines()
+        log = open(os.path.join(d, "millepede.log")).read().splitlines()
+        # 'LOOP1: 200 is total number nTotalParGroups of label/parameter groups'
+        # (CI/refFiles/millepede.log:17): every wire-chamber parameter is its own group
+        P = mpgpu.Pede(matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=50, mcount=1)
+        assert P.read_mille(os.path.join(d, "mp2tst.bin"), fortran=True) == 100000
+        P.build_structure()
+        assert len(set(P.labels()[3].tolist())) == 200
+        P.close()
+        assert open(os.path.join(d, "millepede.end")).read() == open(GOLD_END).read()
+    gold = parse_res(GOLD)
+    assert sorted(got) == sorted(gold)
+    bad = [(lab, vals[0], got[lab][0][0]) for lab, (vals, cnt) in gold.items()
+           if abs(got[lab][0][0] - vals[0]) > 1.2e-4 * max(abs(vals[0]), 1e-6)]
+    assert not bad, bad[:10]
+    assert all(got[lab][1] == cnt for lab, (vals, cnt) in gold.items())
+    gold_mon = open(GOLD_MON).read().splitlines()
+    diff = [(a, b) for a, b in zip(mon, gold_mon) if a != b]
+    assert len(mon) == len(gold_mon) and not diff, diff[:5]
+    compare_log(log, open(GOLD_LOG).read().splitlines())
+    rel = abs(st["final_chi2"] - 8130954.4875403922) / 8130954.4875403922
+    print(f"final chi^2 {st['final_chi2']!r}: relative difference to the CI log {rel:.2e}")
+    assert rel <= 1e-9
+
+
+@pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
 def test_ci_wirechamber_fulllapack():
     """The CI's own method, 'method fullLAPACK 3 0.8' (mdptrf, src/mplapack.f90:41-179), here
     with 'LAPACKwitherrors': parameters and counts of CI/refFiles/millepede.res, the iteration
