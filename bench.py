@@ -53,25 +53,23 @@ def committed_traffic():
     return None
 
 
-def cpu_baseline(lab_var, rp, ii, gg, target_s=12.0):
-    """The oracle (single-threaded C restatement of the record loop) on a bounded sample of
-    the same records, accumulating into a hash table (pede's sparse layout is not rebuilt)."""
+def cpu_baseline(lab_var, blocks, rp, ii, gg, frac=0.1):
+    """The oracle's record pass (C restatement of loopbf) on all host threads (OMP_NUM_THREADS,
+    else every core), accumulating into the device's group-block matrix layout with atomic
+    additions, over the first `frac` of the records of the same workload."""
     import oracle
     orc = oracle.Oracle()
     lab, var, par, nvar = lab_var
-
-    def run(n):
-        sl = slice(0, int(rp[n]))
-        t = time.perf_counter()
-        orc.loopn_hash(rp[: n + 1], ii[sl], gg[sl].astype(np.float64), lab, var, par, nvar, cap_log2=27)
-        return time.perf_counter() - t
-
-    t_cal = run(400)
-    rate = 400 / max(t_cal, 1e-6)
-    n = int(min(max(rate * target_s, 400), 15000))
-    dt = run(n)
-    return dict(value=n / dt, unit="records/s", cores=1, kind="port",
-                sample=f"first {n} records of the same workload, one thread, FP64, matrix into a hash table")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    n = max(1, int(frac * (len(rp) - 1)))
+    sl = slice(0, int(rp[n]))
+    t = time.perf_counter()
+    blk, vec, cnt, st = orc.loopn_blocks(rp[: n + 1], ii[sl], gg[sl].astype(np.float64), lab, var, par, nvar, blocks,
+                                         threads)
+    dt = time.perf_counter() - t
+    return dict(value=n / dt, unit="records/s", cores=threads, kind="port",
+                sample=f"first {n} of {len(rp) - 1} records ({100 * frac:.0f}%) of the same workload, {threads} OpenMP "
+                       "threads, FP64, matrix accumulated into the same group-block layout (atomic adds)")
 
 
 def serving_run(njobs, rate=200.0, seed=7):
@@ -310,17 +308,36 @@ def main():
                        "traffic_source": traffic["source"] if traffic else None}
     lab, var, cnt, gsl = P.labels()
     lab_var = (lab, var, P.parameters(), P.nvar)
+    blocks = P.block_structure()
     out["pass_chi2"] = st["chi2"]
     out["pass_ndf"] = st["ndf"]
 
     if not args.no_solve:
-        t = time.perf_counter()
-        x, sst = P.solve()
-        out["solve_s"] = time.perf_counter() - t
-        out["solve_rel_residual"] = sst["rel_residual"]
-        out["minres_itn"] = sst["itn"]
-        out["minres_istop"] = sst["istop"]
-        log(f"solve: {out['solve_s']:.3f}s itn={sst['itn']} istop={sst['istop']} rel.res={sst['rel_residual']:.2e}")
+        # Each timed solve starts from a changed matrix (a zero added by mupdat), so that it
+        # includes building the band preconditioner, as the first solve after a pass does.
+        def timed_solve(mrestl):
+            P.set_options(mrestl=mrestl)
+            P.mupdat([1], [1], [0.0])
+            t = time.perf_counter()
+            x, sst = P.solve()
+            dt = time.perf_counter() - t
+            log(f"solve mrestol {mrestl:.0e}: {dt:.3f}s itn={sst['itn']} istop={sst['istop']} "
+                f"rel.res={sst['rel_residual']:.2e}")
+            return dict(mrestol=mrestl, s=dt, itn=sst["itn"], istop=sst["istop"], rel_residual=sst["rel_residual"])
+        P.solve()                                   # first solve of the process: code loading
+        pede = timed_solve(1e-10)
+        out["solve_pede_mrestol_1e-10"] = pede     # pede's MINRES stopping test at 'mrestol 1e-10'
+        # the metric: ||A x - b|| / ||b|| <= 1e-10, reached by tightening MINRES's tolerance
+        hist = [pede]
+        for rt in (1e-12, 1e-13, 1e-14, 1e-15, 1e-16):
+            if hist[-1]["rel_residual"] <= 1e-10:
+                break
+            hist.append(timed_solve(rt))
+        out["solve_history"] = hist
+        hit = [h for h in hist if h["rel_residual"] <= 1e-10]
+        out["solve_s"] = hit[0]["s"] if hit else None
+        out["solve_rel_residual"] = hit[0]["rel_residual"] if hit else hist[-1]["rel_residual"]
+        out["minres_itn"] = hit[0]["itn"] if hit else hist[-1]["itn"]
 
     P.close()
     if args.serving_jobs > 0:
@@ -337,7 +354,7 @@ def main():
             out["dense"] = {"error": str(e)}
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(lab_var, rp, ii, gg)
+            out["cpu_baseline"] = cpu_baseline(lab_var, blocks, rp, ii, gg)
             log(f"cpu baseline: {out['cpu_baseline']['value']:.1f} records/s")
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": str(e)}

@@ -163,6 +163,14 @@ int mpgpu_get_labels(const mpgpu_ctx *ctx, int32_t *labels, int32_t *var_index, 
 /* stored elements of the global matrix (matrix words), number of group blocks */
 int mpgpu_matrix_info(const mpgpu_ctx *ctx, int64_t *stored_words, int64_t *nblocks, int32_t *ngroups);
 
+/* group-block storage of the matrix (sparse storage; the information of pede's
+ * sparseMatrixOffsets / sparseMatrixColumns, src/mpbits.f90:1220-1350, in contiguous blocks):
+ * gfirst[ngroups+1] first variable index (1-based) of each parameter group, brow[ngroups+1]
+ * first block of each block row, bcol[nblocks] column group of each block (ascending within a
+ * row, column group <= row group), boff[nblocks] offset of the row-major block in the stored
+ * matrix words; a diagonal block uses its lower triangle.  Sizes from mpgpu_matrix_info. */
+int mpgpu_get_block_structure(const mpgpu_ctx *ctx, int32_t *gfirst, int64_t *brow, int32_t *bcol, int64_t *boff);
+
 /* ---- state ------------------------------------------------------------------------------- */
 int mpgpu_set_parameters(mpgpu_ctx *ctx, const double *par_by_total_index);   /* globalParameter */
 int mpgpu_get_parameters(const mpgpu_ctx *ctx, double *par_by_total_index);

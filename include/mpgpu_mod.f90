@@ -186,6 +186,14 @@ MODULE mpgpu_mod
             IMPORT :: c_int, c_ptr
             TYPE(c_ptr), VALUE :: ctx, x_dev, y_dev
         END FUNCTION
+        ! group-block structure of the sparse matrix (sizes from mpgpu_matrix_info)
+        INTEGER(c_int) FUNCTION mpgpu_get_block_structure(ctx, gfirst, brow, bcol, boff) &
+                BIND(C, name='mpgpu_get_block_structure')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_int64_t
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int32_t), INTENT(OUT) :: gfirst(*), bcol(*)
+            INTEGER(c_int64_t), INTENT(OUT) :: brow(*), boff(*)
+        END FUNCTION
         INTEGER(c_int) FUNCTION mpgpu_get_errors(ctx, err, gcor) BIND(C, name='mpgpu_get_errors')
             IMPORT :: c_int, c_ptr, c_double
             TYPE(c_ptr), VALUE :: ctx

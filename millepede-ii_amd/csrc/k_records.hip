@@ -304,7 +304,7 @@ __host__ __device__ inline RecLds rec_lds(const LdsPlan &P, int tsz)
     return L;
 }
 
-template <typename T, bool MON>
+template <typename T, int MODE>
 __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_records(DevRecords R, DevModel M, PassOpts o, RecOut out, LdsPlan P,
                                                 int64_t rec_begin, int64_t rec_end, MonCtx mon)
 {
@@ -421,8 +421,10 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 double rmeas = lwv(sl);
                 for (int j = 1; j <= ng; j++) rmeas = rmeas - lwv(sg + j) * gp[li[sg + j]];
                 rm[i] = rmeas;
-                // 'scaleerrors' (prepareRecords, src/mppar.f90:312-316)
-                re[i] = lwv(sg) * (ng > 0 ? o.esc[0] : o.esc[1]);
+                // 'scaleerrors' (prepareRecords, src/mppar.f90:312-316): a separate instantiation,
+                // since even this product slows the common case down
+                if constexpr ((MODE & 2) != 0) re[i] = lwv(sg) * (ng > 0 ? o.esc[0] : o.esc[1]);
+                else re[i] = lwv(sg);
                 lc[i] = 0.0;
                 for (int j = 1; j <= nl; j++) nalc_t = max(nalc_t, li[sl + j]);
             }
@@ -646,7 +648,7 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     for (int j = 1; j <= nl; j++) rmloc = rmloc + lwv(sl + j) * bl[li[sl + j] - 1];
                     lc[i] = rmloc;
                     double rmeas = rm[i] - rmloc;
-                    if constexpr (MON) {
+                    if constexpr ((MODE & 1) != 0) {
                         // residual monitoring, first local-fit iteration (3938-3980): variance of
                         // the fitted value g^T C g from the local inverse (element (j, k) addressed
                         // as pede does, by the order of the local words), pull, and the bin of the
@@ -1108,18 +1110,18 @@ __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int6
 }
 
 // host launcher
-template <typename T, bool MON>
+template <typename T, int MODE>
 static hipError_t launch_records_t(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out,
                                    const LdsPlan &P, int64_t rec_begin, int64_t rec_end, int grid, hipStream_t s,
                                    const MonCtx &mon)
 {
     if (P.bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_records<T, MON>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes);
+        hipError_t e = hipFuncSetAttribute((const void *)k_records<T, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, P.bytes);
         if (e != hipSuccess) return e;
     }
     // as many workgroups as fit on the device at once; each loops over the records
     int per_cu = 0, dev = 0, ncu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_records<T, MON>, RT, P.bytes);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_records<T, MODE>, RT, P.bytes);
     if (e != hipSuccess) return e;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
@@ -1127,7 +1129,7 @@ static hipError_t launch_records_t(const DevRecords &R, const DevModel &M, const
     if (std::getenv("MPGPU_VERBOSE"))
         std::fprintf(stderr, "k_records: %d threads x %d workgroups, LDS %d bytes, %d workgroups per CU\n", RT, grid,
                      P.bytes, per_cu);
-    hipLaunchKernelGGL((k_records<T, MON>), dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end, mon);
+    hipLaunchKernelGGL((k_records<T, MODE>), dim3(grid), dim3(RT), P.bytes, s, R, M, o, out, P, rec_begin, rec_end, mon);
     return hipGetLastError();
 }
 
@@ -1136,11 +1138,17 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
                           const MonCtx &mon)
 {
     if (rec_end <= rec_begin) return hipSuccess;
-    const bool m = mon.mode != 0 && mon.hist != nullptr;
-    if (R.vf) return m ? launch_records_t<float, true>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon)
-                       : launch_records_t<float, false>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon);
-    return m ? launch_records_t<double, true>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon)
-             : launch_records_t<double, false>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon);
+    // instantiation: bit 0 residual monitoring, bit 1 scaled measurement errors
+    const int mode = (mon.mode != 0 && mon.hist != nullptr ? 1 : 0) | (o.esc[0] != 1.0 || o.esc[1] != 1.0 ? 2 : 0);
+#define MPGPU_REC_CASE(T_, M_) \
+    case M_: return launch_records_t<T_, M_>(R, M, o, out, P, rec_begin, rec_end, grid, s, mon);
+    if (R.vf) {
+        switch (mode) { MPGPU_REC_CASE(float, 0) MPGPU_REC_CASE(float, 1) MPGPU_REC_CASE(float, 2) MPGPU_REC_CASE(float, 3) }
+    } else {
+        switch (mode) { MPGPU_REC_CASE(double, 0) MPGPU_REC_CASE(double, 1) MPGPU_REC_CASE(double, 2) MPGPU_REC_CASE(double, 3) }
+    }
+#undef MPGPU_REC_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg, const int32_t *bcol,

@@ -1298,6 +1298,19 @@ extern "C" int mpgpu_matrix_info(const mpgpu_ctx *c, int64_t *stored, int64_t *n
     return 0;
 }
 
+extern "C" int mpgpu_get_block_structure(const mpgpu_ctx *c0, int32_t *gfirst, int64_t *brow, int32_t *bcol,
+                                         int64_t *boff)
+{
+    mpgpu_ctx *c = const_cast<mpgpu_ctx *>(c0);
+    if (!c || !c->built) return 30;
+    if (c->opt.matsto != MPGPU_SPARSE) return fail(c, 24, "block structure exists for sparse storage only");
+    if (gfirst) std::copy(c->gfirst.begin(), c->gfirst.end(), gfirst);
+    if (brow) std::copy(c->brow.begin(), c->brow.end(), brow);
+    if (bcol) std::copy(c->bcol.begin(), c->bcol.end(), bcol);
+    if (boff) std::copy(c->boff.begin(), c->boff.begin() + c->nblock, boff);
+    return 0;
+}
+
 extern "C" int mpgpu_set_parameters(mpgpu_ctx *c, const double *p)
 {
     if (!c || !c->built) return 30;

@@ -27,7 +27,7 @@ FULL, SPARSE = 1, 2
 EXPORTS = [
     "mpgpu_default_options", "mpgpu_create", "mpgpu_destroy", "mpgpu_last_error", "mpgpu_set_options",
     "mpgpu_add_records", "mpgpu_read_mille_file", "mpgpu_set_steering_parameters", "mpgpu_add_equations",
-    "mpgpu_build_structure", "mpgpu_get_labels", "mpgpu_matrix_info", "mpgpu_set_parameters",
+    "mpgpu_build_structure", "mpgpu_get_labels", "mpgpu_matrix_info", "mpgpu_get_block_structure", "mpgpu_set_parameters",
     "mpgpu_get_parameters", "mpgpu_loopn", "mpgpu_get_vector", "mpgpu_get_matrix_packed", "mpgpu_mupdat",
     "mpgpu_solve", "mpgpu_avprod", "mpgpu_avprod_noalloc", "mpgpu_get_errors", "mpgpu_get_monitor", "mpgpu_run_steering",
     "mpgpu_server_create", "mpgpu_server_submit", "mpgpu_server_wait", "mpgpu_server_destroy",
@@ -104,6 +104,7 @@ def lib():
         L.mpgpu_build_structure.argtypes = [C.c_void_p, I32, I32, I32]
         L.mpgpu_get_labels.argtypes = [C.c_void_p, I32, I32, I64, I32]
         L.mpgpu_matrix_info.argtypes = [C.c_void_p, I64, I64, I32]
+        L.mpgpu_get_block_structure.argtypes = [C.c_void_p, I32, I64, I32, I64]
         L.mpgpu_set_parameters.argtypes = [C.c_void_p, D]
         L.mpgpu_get_parameters.argtypes = [C.c_void_p, D]
         L.mpgpu_loopn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.POINTER(PassStats)]
@@ -230,6 +231,23 @@ class Pede:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int32()
         self._chk(self.L.mpgpu_matrix_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return dict(stored=a.value, nblocks=b.value, ngroups=c.value)
+
+    def block_structure(self):
+        """(gfirst, brow, bcol, boff, nstored) of the group-block storage (sparse storage)"""
+        info = self.matrix_info()
+        ng, nb = info["ngroups"], info["nblocks"]
+        gfirst = np.zeros(ng + 1, np.int32)
+        brow = np.zeros(ng + 1, np.int64)
+        bcol = np.zeros(max(nb, 1), np.int32)
+        boff = np.zeros(max(nb, 1), np.int64)
+        self._chk(self.L.mpgpu_get_block_structure(self.h, _p(gfirst, I32), _p(brow, I64), _p(bcol, I32), _p(boff, I64)))
+        return gfirst, brow, bcol[:nb], boff[:nb], info["stored"]
+
+    def set_options(self, **kw):
+        """change options of this context (those the structure build used are fixed, code 30)"""
+        for k, v in kw.items():
+            setattr(self.opt, k, v)
+        self._chk(self.L.mpgpu_set_options(self.h, C.byref(self.opt)))
 
     def set_parameters(self, par):
         p = np.ascontiguousarray(par, dtype=np.float64)

@@ -162,6 +162,8 @@ class Oracle:
         L.orc_packed_matvec.argtypes = [C.c_int, D, D, D]
         L.orc_loopn.argtypes = [C.POINTER(_RecStruct), C.POINTER(_PassOpts), D, D, D, I64, C.POINTER(_PassStats)]
         L.orc_loopn.restype = C.c_int
+        L.orc_loopn_blocks.argtypes = [C.POINTER(_RecStruct), C.POINTER(_PassOpts), C.c_int, C.c_int32, I32, I64, I32,
+                                        I64, C.c_int64, D, D, I64, C.POINTER(_PassStats)]
         L.orc_loopn_hash.argtypes = [C.POINTER(_RecStruct), C.POINTER(_PassOpts), C.POINTER(C.c_uint64), D,
                                      C.c_int64, D, I64, C.POINTER(_PassStats), I64]
         L.orc_loopn_hash.restype = C.c_int
@@ -367,6 +369,38 @@ class Oracle:
         m = keys != 0
         stats = dict(chi2=S.chi2, ndf=S.ndf, nrej=list(S.nrej), naccept=S.naccept, nband=S.nbandrec, used=used.value)
         return keys[m], vals[m], vec, cnt, stats
+
+    def loopn_blocks(self, rec_ptr, inder, glder, labels, var_index, par, nvar, blocks, nthreads, **kw):
+        """Record pass on nthreads threads into the device's group-block layout (blocks: gfirst,
+        brow, bcol, boff, nstored as Pede.block_structure returns them); returns (blk, vec, cnt,
+        stats).  The CPU baseline of the timing."""
+        rec_ptr = np.ascontiguousarray(rec_ptr, dtype=np.int64)
+        inder = np.ascontiguousarray(inder, dtype=np.int32)
+        glder = np.ascontiguousarray(glder, dtype=np.float64)
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        var_index = np.ascontiguousarray(var_index, dtype=np.int32)
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        R = _RecStruct(len(rec_ptr) - 1, _p(rec_ptr, I64), _p(inder, I32), _p(glder, D), len(labels),
+                       _p(labels, I32), _p(var_index, I32), _p(par, D), nvar)
+        O = _PassOpts(kw.get("icalcm", 1), kw.get("nloopn", 1), kw.get("lhuber", 0), kw.get("lfitbb", 1),
+                      kw.get("chicut", 0.0), kw.get("chhuge", 50.0), kw.get("dwcut", 0.0), kw.get("cndlmx", 100.0),
+                      nthreads)
+        gfirst, brow, bcol, boff, nstored = blocks
+        gfirst = np.ascontiguousarray(gfirst, dtype=np.int32)
+        brow = np.ascontiguousarray(brow, dtype=np.int64)
+        bcol = np.ascontiguousarray(bcol, dtype=np.int32)
+        boff = np.ascontiguousarray(boff, dtype=np.int64)
+        blk = np.empty(max(nstored, 1))
+        vec = np.zeros(nvar)
+        cnt = np.zeros(nvar, np.int64)
+        S = _PassStats()
+        rc = self.lib.orc_loopn_blocks(C.byref(R), C.byref(O), nthreads, len(gfirst) - 1, _p(gfirst, I32), _p(brow, I64),
+                                       _p(bcol, I32), _p(boff, I64), nstored, _p(blk, D), _p(vec, D), _p(cnt, I64),
+                                       C.byref(S))
+        if rc != 0:
+            raise RuntimeError(f"oracle loopn_blocks failed with code {rc}")
+        stats = dict(chi2=S.chi2, ndf=S.ndf, nrej=list(S.nrej), naccept=S.naccept, nband=S.nbandrec)
+        return blk, vec, cnt, stats
 
     def read_mille(self, path, fortran=False):
         rp, ii, gg = I64(), I32(), D()

@@ -1247,11 +1247,32 @@ typedef struct { double chi_frac; int64_t chi_int; int64_t ndf; double wndf; } o
 
 /* matrix sink: packed full matrix, or an open-addressing hash of (i,j) -> value used to
  * accumulate very large sparse systems on the CPU (keys are i<<32|j, 0 = empty slot) */
-typedef struct { double *mat; uint64_t *keys; double *vals; int64_t cap; int64_t used; int overflow; } orc_sink;
+typedef struct {
+    double *mat; uint64_t *keys; double *vals; int64_t cap; int64_t used; int overflow;
+    /* group-block storage (the device layout): groups of variables (gfirst 1-based, ngroup+1),
+     * per variable its group, block rows over groups with ascending column groups and the
+     * offset of each row-major block; updates added atomically (several threads share it) */
+    const int32_t *gfirst, *group_of, *bcol; const int64_t *brow, *boff; double *blk;
+} orc_sink;
 
 static void sink_add(orc_sink *S, int va, int vb, double v)
 {
     if (S->mat) { S->mat[IDX(va, vb) - 1] += v; return; }
+    if (S->blk) {
+        if (va < vb) { int t = va; va = vb; vb = t; }
+        const int ga = S->group_of[va], gb = S->group_of[vb];
+        int64_t lo = S->brow[ga], hi = S->brow[ga + 1] - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (S->bcol[mid] < gb) lo = mid + 1; else hi = mid;
+        }
+        if (lo > hi || S->bcol[lo] != gb) { S->overflow = 1; return; }
+        const int nc = S->gfirst[gb + 1] - S->gfirst[gb];
+        double *p = S->blk + S->boff[lo] + (int64_t)(va - S->gfirst[ga]) * nc + (vb - S->gfirst[gb]);
+#pragma omp atomic
+        *p += v;
+        return;
+    }
     uint64_t key = ((uint64_t)(uint32_t)va << 32) | (uint32_t)vb;
     uint64_t h = key * 0x9e3779b97f4a7c15ull;
     int64_t mask = S->cap - 1;
@@ -1568,7 +1589,7 @@ int orc_loopn(const orc_records *R, const orc_pass_opts *o, const double *pre_we
     memset(st, 0, sizeof(*st));
     int *back = (int *)calloc(nvar + 2, sizeof(int));
     int rc = 0;
-    orc_sink S = {mat, NULL, NULL, 0, 0, 0};
+    orc_sink S = {mat, NULL, NULL, 0, 0, 0, NULL, NULL, NULL, NULL, NULL, NULL};
     for (int64_t r = 0; r < R->nrec; r++) {
         rc = loop_record(R, r, o, &S, vec, counter, &sums, st->nrej, &st->naccept, &st->nbandrec, back);
         if (rc) break;
@@ -1594,7 +1615,7 @@ int orc_loopn_hash(const orc_records *R, const orc_pass_opts *o, uint64_t *keys,
     memset(st, 0, sizeof(*st));
     int *back = (int *)calloc(nvar + 2, sizeof(int));
     int rc = 0;
-    orc_sink S = {NULL, keys, vals, cap, 0, 0};
+    orc_sink S = {NULL, keys, vals, cap, 0, 0, NULL, NULL, NULL, NULL, NULL, NULL};
     for (int64_t r = 0; r < R->nrec; r++) {
         rc = loop_record(R, r, o, &S, vec, counter, &sums, st->nrej, &st->naccept, &st->nbandrec, back);
         if (rc) break;
@@ -1606,6 +1627,61 @@ int orc_loopn_hash(const orc_records *R, const orc_pass_opts *o, uint64_t *keys,
     st->wndf = sums.wndf;
     *used = S.used;
     return rc;
+}
+
+/* The same pass on nthreads OpenMP threads (records in dynamic chunks, as pede's loopbf loop),
+ * the matrix accumulated into the group-block storage of the device (blk: nstored words,
+ * zeroed here; gfirst/brow/bcol/boff as mpgpu_get_block_structure returns them), vector,
+ * counters and sums per thread then added.  This is the CPU baseline of the timing, not a parity
+ * reference: the order of the atomic additions varies between runs.  Returns 31 when an update
+ * falls outside the block structure. */
+int orc_loopn_blocks(const orc_records *R, const orc_pass_opts *o, int nthreads, int32_t ngroup, const int32_t *gfirst,
+                     const int64_t *brow, const int32_t *bcol, const int64_t *boff, int64_t nstored, double *blk,
+                     double *vec, int64_t *counter, orc_pass_stats *st)
+{
+    const int nvar = R->nvar;
+    int32_t *group_of = (int32_t *)malloc(sizeof(int32_t) * (nvar + 2));
+    for (int g = 0; g < ngroup; g++)
+        for (int v = gfirst[g]; v < gfirst[g + 1]; v++) group_of[v] = g;
+    memset(blk, 0, sizeof(double) * nstored);
+    for (int i = 0; i < nvar; i++) { vec[i] = 0.0; counter[i] = 0; }
+    memset(st, 0, sizeof(*st));
+    int rc = 0, bad = 0;
+    orc_sums tot = {0.0, 0, 0, 0.0};
+#pragma omp parallel num_threads(nthreads)
+    {
+        double *tv = (double *)calloc(nvar + 1, sizeof(double));
+        int64_t *tc = (int64_t *)calloc(nvar + 1, sizeof(int64_t));
+        int *back = (int *)calloc(nvar + 2, sizeof(int));
+        orc_sums sums = {0.0, 0, 0, 0.0};
+        int64_t nrej[6] = {0, 0, 0, 0, 0, 0}, nacc = 0, nband = 0;
+        orc_sink S = {NULL, NULL, NULL, 0, 0, 0, gfirst, group_of, bcol, brow, boff, blk};
+        int lrc = 0;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t r = 0; r < R->nrec; r++) {
+            if (lrc) continue;
+            lrc = loop_record(R, r, o, &S, tv, tc, &sums, nrej, &nacc, &nband, back);
+        }
+#pragma omp critical
+        {
+            for (int i = 0; i < nvar; i++) { vec[i] += tv[i]; counter[i] += tc[i]; }
+            for (int k = 0; k < 6; k++) st->nrej[k] += nrej[k];
+            st->naccept += nacc;
+            st->nbandrec += nband;
+            tot.chi_frac += sums.chi_frac;
+            tot.chi_int += sums.chi_int;
+            tot.ndf += sums.ndf;
+            tot.wndf += sums.wndf;
+            if (lrc) rc = lrc;
+            if (S.overflow) bad = 1;
+        }
+        free(tv); free(tc); free(back);
+    }
+    free(group_of);
+    st->chi2 = tot.chi_frac + (double)tot.chi_int;
+    st->ndf = tot.ndf;
+    st->wndf = tot.wndf;
+    return rc ? rc : (bad ? 31 : 0);
 }
 
 /* ------------------------------------------------------------------------------------------ */

@@ -209,3 +209,76 @@ def test_band_and_full_local_fit_agree(orc):
     assert np.allclose(m1, m2, rtol=1e-7, atol=1e-7 * np.abs(m2).max())
     assert np.allclose(v1, v2, rtol=1e-7, atol=1e-7 * np.abs(v2).max())
     assert abs(s1["chi2"] - s2["chi2"]) < 1e-8 * s2["chi2"]
+
+
+def test_loopn_blocks_equals_packed(orc):
+    """The multi-threaded pass into a group-block layout (the CPU baseline of the timing) adds
+    the same updates as the single-threaded packed pass: pixel records, groups of the 6
+    parameters of each module, a block for every pair of groups."""
+    rp, ii, gg, _ = mpgpu.pixel_records(5, 4, 3, 2, 1500, 3, 6)
+    lab, var, cnt, gsl = LB.label_maps(rp, ii, gg.astype(float), mreqenf=1, mcount=1)
+    nv = int(var.max())
+    par = np.zeros(len(lab))
+    mat, vec, c, so = orc.loopn(rp, ii, gg.astype(np.float64), lab, var, par, nv)
+    ng = nv // 6
+    gfirst = np.arange(ng + 1, dtype=np.int32) * 6 + 1
+    brow = np.zeros(ng + 1, np.int64)
+    bcol, boff, off = [], [], 0
+    for a in range(ng):
+        for b in range(a + 1):
+            bcol.append(b)
+            boff.append(off)
+            off += 36
+        brow[a + 1] = len(bcol)
+    blk, v2, c2, s2 = orc.loopn_blocks(rp, ii, gg.astype(np.float64), lab, var, par, nv,
+                                       (gfirst, brow, np.array(bcol), np.array(boff), off), nthreads=4)
+    assert s2["nrej"] == so["nrej"] and s2["ndf"] == so["ndf"] and abs(s2["chi2"] - so["chi2"]) < 1e-9 * so["chi2"]
+    assert np.array_equal(c2, c) and np.allclose(v2, vec, rtol=1e-12, atol=0)
+    A = packed_to_dense(mat, nv)
+    B = np.zeros((nv, nv))
+    k = 0
+    for a in range(ng):
+        for b in range(a + 1):
+            B[6 * a:6 * a + 6, 6 * b:6 * b + 6] = blk[boff[k]:boff[k] + 36].reshape(6, 6)
+            k += 1
+    L = np.tril(B)
+    assert np.abs(np.tril(A) - L).max() <= 1e-12 * np.abs(A).max()
+
+
+def cauchy_records(nrec=5, M=5.0, r=0.03, sigma=0.01, label=11):
+    """Records with one local parameter (an offset) and two measurements M +- r of error sigma;
+    the first measurement also depends on one global parameter (derivative 1).  By symmetry
+    the local fit leaves residuals +-r whatever the weights."""
+    I = [0, 0, 1, 0, label, 0, 1, 0]
+    G = [0.0, M + r, 1.0, sigma, 1.0, M - r, 1.0, sigma]
+    rp = np.arange(nrec + 1, dtype=np.int64) * len(I)
+    return rp, np.tile(np.array(I, np.int32), nrec), np.tile(np.array(G, np.float32), nrec)
+
+
+def cauchy_expected(nrec=5, r=0.03, sigma=0.01):
+    """Closed form of the last (Cauchy) outlier iteration (lhuber > 3, src/pede.f90:3988-3998,
+    4106-4131): each measurement gets weight (1/sigma^2) / (1 + (r/sigma/c)^2) with c = 2.3849
+    and chi^2 contribution c^2 log(1 + (r/sigma/c)^2); the matrix element of the global is
+    w - w^2 / (2 w) = w / 2 per record, its vector element w r (Schur complement of the offset)."""
+    c = 2.3849
+    t = (r / sigma / c) ** 2
+    w = (1.0 / sigma ** 2) / (1.0 + t)
+    # chi^2 sum in float rounding of pede's REAL(4) r/sigma is not involved: all in FP64
+    return dict(chi2=nrec * 2 * c * c * np.log1p(t), mat=nrec * w / 2, vec=nrec * w * r)
+
+
+@pytest.mark.parametrize("lhuber", [4, 6])
+def test_cauchy_downweighting_closed_form(orc, lhuber):
+    rp, ii, gg = cauchy_records()
+    lab, var, cnt, gsl = LB.label_maps(rp, ii, gg.astype(float), mreqenf=1)
+    mat, vec, c, so = orc.loopn(rp, ii, gg.astype(np.float64), lab, var, np.zeros(1), 1, nloopn=2, lhuber=lhuber)
+    # the records are FP32: r and sigma are the FP32 values of the file
+    r32 = float(np.float32(5.03) - np.float32(5.0))
+    ex = cauchy_expected(r=abs(r32), sigma=float(np.float32(0.01)))
+    assert so["ndf"] == 5 and so["nrej"] == [0] * 6
+    assert abs(so["chi2"] - ex["chi2"]) < 1e-9 * ex["chi2"]
+    assert abs(mat[0] - ex["mat"]) < 1e-9 * ex["mat"]
+    assert abs(abs(vec[0]) - ex["vec"]) < 1e-9 * ex["vec"]
+    # with lhuber = 3 the last iteration is a Huber one: a different chi^2
+    _, _, _, sh = orc.loopn(rp, ii, gg.astype(np.float64), lab, var, np.zeros(1), 1, nloopn=2, lhuber=3)
+    assert abs(sh["chi2"] - ex["chi2"]) > 1e-3 * ex["chi2"]
