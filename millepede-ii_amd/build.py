@@ -21,7 +21,7 @@ HIP_FLAGS = [
     "-fopenmp", "-Wno-unused-result",
 ]
 
-HIP_SOURCES = ["k_records.hip", "k_records_band.hip", "k_solve.hip", "k_chol.hip", "k_qlelim.hip", "mpgpu.cpp",
+HIP_SOURCES = ["k_records.hip", "k_records_band.hip", "k_solve.hip", "k_chol.hip", "k_qlelim.hip", "k_struct.hip", "mpgpu.cpp",
                "mpgpu_qldec.cpp", "mpgpu_pede.cpp", "mpgpu_serve.cpp"]
 
 

@@ -48,7 +48,7 @@ struct mpgpu_ctx {
     std::vector<int32_t> gfirst, group_of; // variable groups
     std::vector<int64_t> brow, boff;
     std::vector<int32_t> bcol, brg;
-    std::vector<int64_t> ccol_ptr, ccol_blk; // off-diagonal blocks by column group (symmetric product)
+    bool blocks_on_host = true;           // brow/bcol/brg/boff copied from the device (on first use)
     int64_t nblock = 0, nstored = 0;
     // constraints (CSR over variable indices) and residual bookkeeping
     std::vector<int64_t> cptr;
@@ -383,46 +383,6 @@ extern "C" int mpgpu_add_equations(mpgpu_ctx *c, int32_t kind, int32_t nblock, c
 // ---------------------------------------------------------------------------------------------
 // structure build
 // ---------------------------------------------------------------------------------------------
-namespace {
-
-struct Decoded {
-    // per equation: sl (absolute word), nl, ng
-    std::vector<int64_t> sl;
-    std::vector<uint32_t> n;
-};
-
-// decodeNextMeasurement over one record (src/pede.f90:2428-2468); returns false on bad structure
-inline bool decode_record(const int32_t *I, const float *F, const double *D, int64_t w0, int64_t w1,
-                          std::vector<int64_t> &sl, std::vector<uint32_t> &nn)
-{
-    // words are 1-based within the record: word k at w0 + k - 1
-    auto val = [&](int64_t k) { return D ? D[w0 + k - 1] : (double)F[w0 + k - 1]; };
-    auto idx = [&](int64_t k) { return I[w0 + k - 1]; };
-    int64_t nw = w1 - w0;
-    int64_t last = 1, end = nw;
-    for (;;) {
-        if (last >= end) break;
-        do { last++; if (last > nw) return false; } while (idx(last) != 0);
-        int64_t sL = last;
-        do { last++; if (last > nw) return false; } while (idx(last) != 0);
-        int64_t sG = last;
-        if (sL + 1 == sG && val(sG) < 0.0) {
-            last = last + (int64_t)std::llround(-val(sG));
-            continue;
-        }
-        while (last < end && idx(last + 1) != 0) last++;
-        int64_t nl = sG - sL - 1, ng = last - sG;
-        for (int64_t k = sL + 1; k < sG; k++)
-            if (idx(k) < 1) return false;          // local parameter indices are 1-based
-        if (nl > 0xffff || ng > 0xffff) return false;
-        sl.push_back(w0 + sL - 1);
-        nn.push_back((uint32_t)nl | ((uint32_t)ng << 16));
-    }
-    return true;
-}
-
-} // namespace
-
 // Bordered-band structure of one record's local normal matrix: the band-width histogram and
 // cost model of loopbf (src/pede.f90:3815-3874).  Returns 0 for the full (sqminv) path, else
 // (mbnd+1) | mbdr << 10 | mside << 20.
@@ -480,178 +440,71 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         t_phase = t;
     };
 
-    // ---- decode all records (equation tables) ----
-    std::vector<int64_t> rec_eq(nrec + 1, 0);
-    std::vector<std::vector<int64_t>> t_sl(omp_get_max_threads());
-    std::vector<std::vector<uint32_t>> t_n(omp_get_max_threads());
-    std::vector<int64_t> neq_rec(nrec, 0);
-    std::atomic<int64_t> bad{-1};
-    // decode in record order per chunk so that concatenation keeps record order
-    const int64_t CH = 4096;
-    const int64_t nch = (nrec + CH - 1) / CH;
-    std::vector<std::vector<int64_t>> ch_sl(nch);
-    std::vector<std::vector<uint32_t>> ch_n(nch);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t ch = 0; ch < nch; ch++) {
-        for (int64_t r = ch * CH; r < std::min(nrec, (ch + 1) * CH); r++) {
-            size_t before = ch_sl[ch].size();
-            if (!decode_record(I, F, D, c->rec_ptr[r], c->rec_ptr[r + 1], ch_sl[ch], ch_n[ch])) {
-                int64_t exp = -1;
-                bad.compare_exchange_strong(exp, r);
-            }
-            neq_rec[r] = (int64_t)(ch_sl[ch].size() - before);
-        }
-    }
-    if (bad.load() >= 0) throw MpError(20, "bad record structure in record " + std::to_string(bad.load() + 1));
-    for (int64_t r = 0; r < nrec; r++) rec_eq[r + 1] = rec_eq[r] + neq_rec[r];
-    const int64_t neq = rec_eq[nrec];
-    std::vector<int64_t> eq_sl(neq);
-    std::vector<uint32_t> eq_n(neq);
-    {
-        std::vector<int64_t> choff(nch + 1, 0);
-        for (int64_t ch = 0; ch < nch; ch++) choff[ch + 1] = choff[ch] + (int64_t)ch_sl[ch].size();
-#pragma omp parallel for schedule(static)
-        for (int64_t ch = 0; ch < nch; ch++) {
-            std::copy(ch_sl[ch].begin(), ch_sl[ch].end(), eq_sl.begin() + choff[ch]);
-            std::copy(ch_n[ch].begin(), ch_n[ch].end(), eq_n.begin() + choff[ch]);
-            std::vector<int64_t>().swap(ch_sl[ch]);
-            std::vector<uint32_t>().swap(ch_n[ch]);
-        }
-    }
-
+    hipStream_t s = c->stream;
+    // ---- records to the device: words and values; the words as read (labels) are kept apart
+    // from the words the record kernels use (global words replaced by their position in the
+    // record's list, see below) ----
+    c->d_rec_w.upload(c->rec_ptr, s);
+    if (c->dbl) c->d_vd.upload(c->vd, s); else c->d_vf.upload(c->vf, s);
+    DBuf<int32_t> d_lab;
+    d_lab.upload(c->inder, s);
+    c->d_widx.alloc(std::max<size_t>(c->inder.size(), 1));
+    if (!c->inder.empty())
+        HIPCHK(hipMemcpyAsync(c->d_widx.p, d_lab.p, sizeof(int32_t) * c->inder.size(), hipMemcpyDeviceToDevice, s));
+    StructIn sin;
+    sin.nrec = nrec;
+    sin.rec_w = c->d_rec_w.p;
+    sin.lab = d_lab.p;
+    sin.vf = c->dbl ? nullptr : c->d_vf.p;
+    sin.vd = c->dbl ? c->d_vd.p : nullptr;
+    phase("records to the device");
+    // ---- decode all records (equation tables) on the device ----
+    const int64_t neq = st_decode(sin, c->d_rec_eq, c->d_eq_sl, c->d_eq_n, s);
+    std::vector<int64_t> rec_eq(nrec + 1);
+    c->d_rec_eq.download(rec_eq.data(), nrec + 1, s);
+    HIPCHK(hipStreamSynchronize(s));
+    // host copies of the equation tables, only for the host passes that need them
+    std::vector<int64_t> eq_sl;
+    std::vector<uint32_t> eq_n;
+    auto host_eq = [&]() {
+        if ((int64_t)eq_sl.size() == neq) return;
+        eq_sl.resize(neq);
+        eq_n.resize(neq);
+        c->d_eq_sl.download(eq_sl.data(), neq, s);
+        c->d_eq_n.download(eq_n.data(), neq, s);
+        HIPCHK(hipStreamSynchronize(s));
+    };
     phase("decode");
-    // ---- label table: all labels of global words, steering parameters and equations ----
-    int32_t lmin = INT32_MAX, lmax = 0;
-#pragma omp parallel for reduction(min : lmin) reduction(max : lmax) schedule(static)
-    for (int64_t e = 0; e < neq; e++) {
-        int64_t sg = eq_sl[e] + (eq_n[e] & 0xffff) + 1;
-        int ng = eq_n[e] >> 16;
-        for (int j = 1; j <= ng; j++) {
-            int32_t l = I[sg + j];
-            if (l < lmin) lmin = l;
-            if (l > lmax) lmax = l;
-        }
-    }
-    auto upd_mm = [&](int32_t l) { lmin = std::min(lmin, l); lmax = std::max(lmax, l); };
-    for (int32_t l : c->sp_label) upd_mm(l);
-    for (auto &b : c->cons) for (int32_t l : b.labels) upd_mm(l);
-    for (auto &b : c->meas) for (int32_t l : b.labels) upd_mm(l);
-    if (lmax == 0) throw MpError(22, "no global parameters");
-    if (lmin <= 0) throw MpError(20, "label <= 0 in input");
-    const int64_t span = (int64_t)lmax - lmin + 1;
-    const int64_t nwbits = (span + 63) / 64;
-    std::vector<std::atomic<uint64_t>> bits(nwbits);
-    for (auto &b : bits) b.store(0, std::memory_order_relaxed);
-    auto setbit = [&](int32_t l) {
-        int64_t o = (int64_t)l - lmin;
-        bits[o >> 6].fetch_or(1ull << (o & 63), std::memory_order_relaxed);
-    };
-#pragma omp parallel for schedule(static)
-    for (int64_t e = 0; e < neq; e++) {
-        int64_t sg = eq_sl[e] + (eq_n[e] & 0xffff) + 1;
-        int ng = eq_n[e] >> 16;
-        for (int j = 1; j <= ng; j++) setbit(I[sg + j]);
-    }
-    for (int32_t l : c->sp_label) setbit(l);
-    for (auto &b : c->cons) for (int32_t l : b.labels) setbit(l);
-    for (auto &b : c->meas) for (int32_t l : b.labels) setbit(l);
-    std::vector<int32_t> rank_base(nwbits + 1, 0);
-    for (int64_t w = 0; w < nwbits; w++) rank_base[w + 1] = rank_base[w] + __builtin_popcountll(bits[w].load());
-    const int32_t ntot = rank_base[nwbits];
-    c->ntot = ntot;
-    c->labels.assign(ntot, 0);
-#pragma omp parallel for schedule(static)
-    for (int64_t w = 0; w < nwbits; w++) {
-        uint64_t b = bits[w].load();
-        int32_t k = rank_base[w];
-        while (b) {
-            int t = __builtin_ctzll(b);
-            c->labels[k++] = (int32_t)(lmin + w * 64 + t);
-            b &= b - 1;
-        }
-    }
-    auto tix_of = [&](int32_t l) -> int32_t { // 0-based total index
-        int64_t o = (int64_t)l - lmin;
-        uint64_t b = bits[o >> 6].load(std::memory_order_relaxed);
-        return rank_base[o >> 6] + __builtin_popcountll(b & ((1ull << (o & 63)) - 1ull));
-    };
 
-    phase("label table");
-    // ---- per record: sorted distinct labels (all global words) -> gl lists, widx ----
-    std::vector<int64_t> rec_gl(nrec + 1, 0);
-    std::vector<int32_t> ngl_rec(nrec, 0);
-    std::vector<int32_t> widx(c->inder.size());
-    std::copy(c->inder.begin(), c->inder.end(), widx.begin());
-    // counters and group boundaries (counting per equation or per record, nonzero derivatives)
-    std::vector<std::atomic<int64_t>> cnt(ntot);
-    for (auto &x : cnt) x.store(0, std::memory_order_relaxed);
-    std::vector<std::atomic<uint8_t>> bnd(ntot + 1);
-    for (auto &x : bnd) x.store(0, std::memory_order_relaxed);
-    // group boundaries of one sorted list (pargrp, src/pede.f90:2311-2404): at its first element,
-    // after its last, and at both sides of every step that is not +1 -- a label repeated within
-    // one equation (pede sorts equation lists without removing duplicates, 2259-2270) is such a
-    // step, so it closes the group before it and starts its own
-    auto mark_list = [&](std::vector<int32_t> &lst) {
-        if (lst.empty()) return;
-        std::sort(lst.begin(), lst.end());
-        bnd[lst[0]].store(1, std::memory_order_relaxed);
-        for (size_t q = 1; q < lst.size(); q++)
-            if (lst[q] != lst[q - 1] + 1) { bnd[lst[q - 1] + 1].store(1, std::memory_order_relaxed); bnd[lst[q]].store(1, std::memory_order_relaxed); }
-        bnd[lst.back() + 1].store(1, std::memory_order_relaxed);
-    };
-    std::vector<std::vector<int32_t>> rec_lists(nrec);
-#pragma omp parallel
+    // ---- label table: all labels of global words, steering parameters and equations; the
+    // rank of a label among them is its total index ----
+    std::vector<int32_t> ngl_rec;
+    std::vector<uint8_t> bnd;
     {
-        std::vector<int32_t> all, eql;
-#pragma omp for schedule(dynamic, 256)
-        for (int64_t r = 0; r < nrec; r++) {
-            all.clear();
-            std::vector<int32_t> recnz;
-            for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
-                int64_t sg = eq_sl[e] + (eq_n[e] & 0xffff) + 1;
-                int ng = eq_n[e] >> 16;
-                eql.clear();
-                for (int j = 1; j <= ng; j++) {
-                    int32_t t = tix_of(I[sg + j]);
-                    all.push_back(t);
-                    if (val(sg + j) != 0.0) {
-                        if (c->opt.mcount == 0) cnt[t].fetch_add(1, std::memory_order_relaxed);
-                        eql.push_back(t);
-                        recnz.push_back(t);
-                    }
-                }
-                if (c->opt.mcount == 0) mark_list(eql);
-            }
-            if (c->opt.mcount != 0) {
-                std::sort(recnz.begin(), recnz.end());
-                recnz.erase(std::unique(recnz.begin(), recnz.end()), recnz.end());
-                for (int32_t t : recnz) cnt[t].fetch_add(1, std::memory_order_relaxed);
-                mark_list(recnz);
-            }
-            std::sort(all.begin(), all.end());
-            all.erase(std::unique(all.begin(), all.end()), all.end());
-            ngl_rec[r] = (int32_t)all.size();
-            for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
-                int64_t sg = eq_sl[e] + (eq_n[e] & 0xffff) + 1;
-                int ng = eq_n[e] >> 16;
-                for (int j = 1; j <= ng; j++) {
-                    int32_t t = tix_of(I[sg + j]);
-                    widx[sg + j] = (int32_t)(std::lower_bound(all.begin(), all.end(), t) - all.begin());
-                }
-            }
-            rec_lists[r] = all;
-        }
+        std::vector<int32_t> extra(c->sp_label.begin(), c->sp_label.end());
+        for (auto &b : c->cons) extra.insert(extra.end(), b.labels.begin(), b.labels.end());
+        for (auto &b : c->meas) extra.insert(extra.end(), b.labels.begin(), b.labels.end());
+        LabelMap lmap;
+        st_label_table(sin, extra, lmap, c->labels, s);
+        c->ntot = (int32_t)c->labels.size();
+        phase("label table");
+        // ---- per record: sorted distinct total indices of its global words (gl lists), the
+        // position of each global word in them (widx), entry counters and the group boundaries
+        // of pargrp (src/pede.f90:2311-2404): at the first element of every sorted list, after
+        // its last, and at both sides of every step that is not +1 -- a label repeated within one
+        // equation (pede sorts equation lists without removing duplicates, 2259-2270) is such a
+        // step, so it closes the group before it and starts its own.  Counting and lists are per
+        // equation, or per record with 'countrecords'; derivatives of zero do not count.
+        st_record_lists(sin, lmap, c->ntot, c->opt.mcount != 0, c->d_widx, c->d_rec_gl, c->d_gl_tix, ngl_rec, c->counters,
+                        bnd, s);
+        d_lab.release();
+        sin.lab = nullptr;
     }
-    for (int64_t r = 0; r < nrec; r++) rec_gl[r + 1] = rec_gl[r] + ngl_rec[r];
-    std::vector<int32_t> gl_tix(rec_gl[nrec]);
-#pragma omp parallel for schedule(static)
-    for (int64_t r = 0; r < nrec; r++) {
-        std::copy(rec_lists[r].begin(), rec_lists[r].end(), gl_tix.begin() + rec_gl[r]);
-        std::vector<int32_t>().swap(rec_lists[r]);
-    }
-    bnd[0].store(1);
-    c->counters.assign(ntot, 0);
-    for (int32_t t = 0; t < ntot; t++) c->counters[t] = cnt[t].load();
+    const int32_t ntot = c->ntot;
+    bnd[0] = 1;
+    auto tix_of = [&](int32_t l) -> int32_t {   // 0-based total index of a label in use
+        return (int32_t)(std::lower_bound(c->labels.begin(), c->labels.end(), l) - c->labels.begin());
+    };
 
     // residual monitoring: every measurement with local and global derivatives belongs to the
     // group of its first global parameter; the group's mean error is summed in record order
@@ -660,6 +513,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->mon_res.clear();
     std::vector<int32_t> mon_grp;
     if (c->opt.monitor != 0) {
+        host_eq();
         std::vector<int64_t> mcnt(ntot, 0);
         std::vector<double> msum(ntot, 0.0);
         for (int64_t r = 0; r < nrec; r++)
@@ -730,7 +584,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     {
         int32_t start = 0;
         for (int32_t t = 0; t < ntot; t++) {
-            if (bnd[t].load()) start = t;
+            if (bnd[t]) start = t;
             c->group_start_label[t] = c->labels[start];
         }
     }
@@ -817,170 +671,92 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     phase("constraints");
     // ---- matrix structure ----
     const bool sparse = c->opt.matsto == MPGPU_SPARSE;
+    c->d_group_of.upload(c->group_of, s);
+    c->d_gfirst.upload(c->gfirst, s);
+    c->d_var.upload(c->var_index, s);
+    // per record on the device: segments (parameter groups of its variables), local-fit sizes,
+    // update size, first group; sparse storage: the group pairs it couples
+    std::vector<int32_t> rst;
+    std::vector<int64_t> cnt_r;
+    DBuf<int32_t> d_rst;
+    PairMap pm;
+    int32_t max_ngl_all = 1;
+    for (int32_t x : ngl_rec) max_ngl_all = std::max(max_ngl_all, x);
+    st_record_stats(sin, c->d_widx, c->d_rec_gl, c->d_gl_tix, c->d_var.p, c->d_group_of.p, c->ngroup, max_ngl_all, sparse,
+                    pm, rst, cnt_r, d_rst, s);
+    const int32_t *st_nv = rst.data(), *st_nseg = rst.data() + nrec, *st_nalc = rst.data() + 2 * nrec,
+                  *st_sup = rst.data() + 3 * nrec, *st_qe = rst.data() + 4 * nrec;
+    phase("record stats and group pairs");
     int32_t max_gl = 1, max_eq = 1, max_nalc = 1, max_var = 1, max_seg = 1;
     int32_t bmax_gl = 1, bmax_eq = 1, bmax_nalc = 1, bmax_var = 1, bmax_seg = 1, bmax_mp1 = 1, bmax_bdr = 0, bmax_sup = 1;
     int32_t max_qe = 1;   // most equations any record variable occurs in (full local fits)
-    std::vector<int64_t> rec_pb(nrec + 1, 0);
-    std::vector<int32_t> pair_blk;
     std::vector<int32_t> binfo(nrec, 0);
     const int lfitbb = c->opt.lfitbb;
-    {
-        // per-record maxima and (sparse) group pairs
-        std::vector<int32_t> nseg_rec(nrec, 0);
-        const int64_t ng = c->ngroup;
-        const int64_t npairs = ng * (ng + 1) / 2;
-        std::vector<std::atomic<uint64_t>> pbits(sparse ? (npairs + 63) / 64 : 0);
-        for (auto &b : pbits) b.store(0, std::memory_order_relaxed);
-#pragma omp parallel for schedule(dynamic, 256) reduction(max : max_gl, max_eq, max_nalc, max_var, max_seg, bmax_gl, \
+    if (lfitbb > 0)
+        for (int64_t r = 0; r < nrec; r++)
+            if (st_nalc[r] > 5) { host_eq(); break; }
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : max_gl, max_eq, max_nalc, max_var, max_seg, bmax_gl, \
     bmax_eq, bmax_nalc, bmax_var, bmax_seg, bmax_mp1, bmax_bdr, bmax_sup, max_qe)
-        for (int64_t r = 0; r < nrec; r++) {
-            thread_local std::vector<int> ibh;
-            int32_t ngl = ngl_rec[r];
-            const int32_t neq_r = (int32_t)(rec_eq[r + 1] - rec_eq[r]);
-            int nalc = 0;
-            int64_t sup = 0;
-            for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
-                int nl = eq_n[e] & 0xffff;
-                for (int j = 1; j <= nl; j++) nalc = std::max(nalc, c->inder[eq_sl[e] + j]);
-                sup += (int64_t)nl * (eq_n[e] >> 16);
-            }
-            int nv_r = 0, nseg = 0, lastg = -1;
-            for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
-                int v = c->var_index[gl_tix[q]];
-                if (v > 0) {
-                    nv_r++;
-                    int g = c->group_of[v];
-                    if (g != lastg) { nseg++; lastg = g; }
-                }
-            }
-            // bordered-band local fit (decided once, as pede does in the first iteration)
-            int32_t bi = 0;
-            if (nalc > 5 && lfitbb > 0) bi = band_info(c->inder.data(), eq_sl.data(), eq_n.data(), rec_eq[r], rec_eq[r + 1], nalc, ibh);
-            binfo[r] = bi;
-            if (bi) {
-                bmax_gl = std::max(bmax_gl, ngl);
-                bmax_eq = std::max(bmax_eq, neq_r);
-                bmax_nalc = std::max(bmax_nalc, nalc);
-                bmax_var = std::max(bmax_var, nv_r);
-                bmax_seg = std::max(bmax_seg, nseg);
-                bmax_mp1 = std::max(bmax_mp1, bi & 0x3ff);
-                bmax_bdr = std::max(bmax_bdr, (bi >> 10) & 0x3ff);
-                bmax_sup = std::max(bmax_sup, (int32_t)std::min<int64_t>(sup, INT32_MAX));
-            } else {
-                max_gl = std::max(max_gl, ngl);
-                max_eq = std::max(max_eq, neq_r);
-                max_nalc = std::max(max_nalc, nalc);
-                max_var = std::max(max_var, nv_r);
-                // equations per record variable (non-zero derivative), as the record kernel lists them
-                thread_local std::vector<int32_t> qcnt;
-                qcnt.assign(ngl, 0);
-                for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
-                    const int nl = eq_n[e] & 0xffff, ng = eq_n[e] >> 16;
-                    const int64_t sg = eq_sl[e] + nl + 1;
-                    for (int j = 1; j <= ng; j++) {
-                        const double gj = c->dbl ? c->vd[sg + j] : (double)c->vf[sg + j];
-                        const int pos = widx[sg + j];
-                        if (gj != 0.0 && c->var_index[gl_tix[rec_gl[r] + pos]] > 0) qcnt[pos]++;
-                    }
-                }
-                for (int32_t q : qcnt) max_qe = std::max(max_qe, q);
-            }
-            max_seg = std::max(max_seg, nseg);
-            nseg_rec[r] = nseg;
-            if (sparse) {
-                std::vector<int32_t> segs;
-                lastg = -1;
-                for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
-                    int v = c->var_index[gl_tix[q]];
-                    if (v > 0 && c->group_of[v] != lastg) { lastg = c->group_of[v]; segs.push_back(lastg); }
-                }
-                for (size_t a = 0; a < segs.size(); a++)
-                    for (size_t b = 0; b <= a; b++) {
-                        int64_t ga = segs[a], gb = segs[b];
-                        int64_t o = ga * (ga + 1) / 2 + gb;
-                        pbits[o >> 6].fetch_or(1ull << (o & 63), std::memory_order_relaxed);
-                    }
-            }
-        }
-        if (sparse) {
-            // external measurements couple their parameters too
-            for (auto &b : c->meas) {
-                std::vector<int32_t> gs;
-                for (int32_t l : b.labels) { int v = c->var_index[tix_of(l)]; if (v > 0) gs.push_back(c->group_of[v]); }
-                for (int32_t a : gs) for (int32_t b2 : gs) if (a >= b2) {
-                    int64_t o = (int64_t)a * (a + 1) / 2 + b2;
-                    pbits[o >> 6].fetch_or(1ull << (o & 63));
-                }
-            }
-            // every diagonal block exists (pre-sigma, preconditioner)
-            for (int64_t g = 0; g < ng; g++) { int64_t o = g * (g + 1) / 2 + g; pbits[o >> 6].fetch_or(1ull << (o & 63)); }
-            // rows -> blocks
-            c->brow.assign(ng + 1, 0);
-            std::vector<int64_t> rowcnt(ng, 0);
-#pragma omp parallel for schedule(dynamic, 64)
-            for (int64_t ga = 0; ga < ng; ga++) {
-                int64_t cntb = 0;
-                for (int64_t gb = 0; gb <= ga; gb++) {
-                    int64_t o = ga * (ga + 1) / 2 + gb;
-                    if (pbits[o >> 6].load(std::memory_order_relaxed) >> (o & 63) & 1) cntb++;
-                }
-                rowcnt[ga] = cntb;
-            }
-            for (int64_t ga = 0; ga < ng; ga++) c->brow[ga + 1] = c->brow[ga] + rowcnt[ga];
-            c->nblock = c->brow[ng];
-            c->bcol.assign(c->nblock, 0);
-            c->brg.assign(c->nblock, 0);
-            c->boff.assign(c->nblock, 0);
-#pragma omp parallel for schedule(dynamic, 64)
-            for (int64_t ga = 0; ga < ng; ga++) {
-                int64_t k = c->brow[ga];
-                for (int64_t gb = 0; gb <= ga; gb++) {
-                    int64_t o = ga * (ga + 1) / 2 + gb;
-                    if (pbits[o >> 6].load(std::memory_order_relaxed) >> (o & 63) & 1) { c->bcol[k] = (int32_t)gb; c->brg[k] = (int32_t)ga; k++; }
-                }
-            }
-            int64_t off = 0;
-            for (int64_t b = 0; b < c->nblock; b++) {
-                int ga = c->brg[b], gb = c->bcol[b];
-                int64_t nr = c->gfirst[ga + 1] - c->gfirst[ga], nc = c->gfirst[gb + 1] - c->gfirst[gb];
-                c->boff[b] = off;
-                off += nr * nc;
-            }
-            c->nstored = off;
-            // column lists of the off-diagonal blocks (ascending block row within a column)
-            c->ccol_ptr.assign(ng + 1, 0);
-            for (int64_t b = 0; b < c->nblock; b++) if (c->brg[b] != c->bcol[b]) c->ccol_ptr[c->bcol[b] + 1]++;
-            for (int64_t g = 0; g < ng; g++) c->ccol_ptr[g + 1] += c->ccol_ptr[g];
-            c->ccol_blk.assign(c->ccol_ptr[ng], 0);
-            {
-                std::vector<int64_t> fill(c->ccol_ptr.begin(), c->ccol_ptr.end() - 1);
-                for (int64_t b = 0; b < c->nblock; b++)
-                    if (c->brg[b] != c->bcol[b]) c->ccol_blk[fill[c->bcol[b]]++] = b;
-            }
-            // per record: block id of each segment pair
-            for (int64_t r = 0; r < nrec; r++) rec_pb[r + 1] = rec_pb[r] + (int64_t)nseg_rec[r] * (nseg_rec[r] + 1) / 2;
-            pair_blk.assign(rec_pb[nrec], -1);
-#pragma omp parallel for schedule(dynamic, 256)
-            for (int64_t r = 0; r < nrec; r++) {
-                std::vector<int32_t> segs;
-                int lastg = -1;
-                for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
-                    int v = c->var_index[gl_tix[q]];
-                    if (v > 0 && c->group_of[v] != lastg) { lastg = c->group_of[v]; segs.push_back(lastg); }
-                }
-                int64_t p = rec_pb[r];
-                for (size_t a = 0; a < segs.size(); a++)
-                    for (size_t b = 0; b <= a; b++) {
-                        int32_t ga = segs[a], gb = segs[b];
-                        auto it = std::lower_bound(c->bcol.begin() + c->brow[ga], c->bcol.begin() + c->brow[ga + 1], gb);
-                        pair_blk[p++] = (int32_t)(it - c->bcol.begin());
-                    }
-            }
+    for (int64_t r = 0; r < nrec; r++) {
+        thread_local std::vector<int> ibh;
+        const int32_t ngl = ngl_rec[r], neq_r = (int32_t)(rec_eq[r + 1] - rec_eq[r]);
+        const int32_t nalc = st_nalc[r], nv_r = st_nv[r], nseg = st_nseg[r];
+        // bordered-band local fit (decided once, as pede does in the first iteration)
+        int32_t bi = 0;
+        if (nalc > 5 && lfitbb > 0) bi = band_info(c->inder.data(), eq_sl.data(), eq_n.data(), rec_eq[r], rec_eq[r + 1], nalc, ibh);
+        binfo[r] = bi;
+        if (bi) {
+            bmax_gl = std::max(bmax_gl, ngl);
+            bmax_eq = std::max(bmax_eq, neq_r);
+            bmax_nalc = std::max(bmax_nalc, nalc);
+            bmax_var = std::max(bmax_var, nv_r);
+            bmax_seg = std::max(bmax_seg, nseg);
+            bmax_mp1 = std::max(bmax_mp1, bi & 0x3ff);
+            bmax_bdr = std::max(bmax_bdr, (bi >> 10) & 0x3ff);
+            bmax_sup = std::max(bmax_sup, st_sup[r]);
         } else {
-            int64_t n = c->nall;
-            c->nstored = n * (n + 1) / 2;
-            c->nblock = 0;
+            max_gl = std::max(max_gl, ngl);
+            max_eq = std::max(max_eq, neq_r);
+            max_nalc = std::max(max_nalc, nalc);
+            max_var = std::max(max_var, nv_r);
+            max_qe = std::max(max_qe, st_qe[r]);
         }
+        max_seg = std::max(max_seg, nseg);
+    }
+    BlockDev blk;
+    if (sparse) {
+        // external measurements couple their parameters too; every diagonal block exists
+        // (pre-sigma, preconditioner)
+        std::vector<int64_t> xp;
+        for (auto &b : c->meas) {
+            std::vector<int32_t> gs;
+            for (int32_t l : b.labels) { int v = c->var_index[tix_of(l)]; if (v > 0) gs.push_back(c->group_of[v]); }
+            for (int32_t a : gs) for (int32_t b2 : gs) if (a >= b2) xp.push_back((int64_t)a * (a + 1) / 2 + b2);
+        }
+        st_blocks(pm, c->ngroup, c->d_gfirst.p, xp, blk, s);
+        c->nblock = blk.nblock;
+        c->nstored = blk.nstored;
+        c->blocks_on_host = false;
+        c->d_brow.swap(blk.brow);
+        c->d_bcol.swap(blk.bcol);
+        c->d_brg.swap(blk.brg);
+        c->d_boff.swap(blk.boff);
+        c->d_bcf.swap(blk.bcf);
+        c->d_ccol_ptr.swap(blk.ccol_ptr);
+        c->d_cc_off.swap(blk.cc_off);
+        c->d_cc_fn.swap(blk.cc_fn);
+        // per record: block id of each segment pair
+        st_pair_blocks(sin, c->d_rec_gl, c->d_gl_tix, c->d_var.p, c->d_group_of.p, max_seg, d_rst, pm, c->d_rec_pb,
+                       c->d_pair_blk, s);
+    } else {
+        int64_t n = c->nall;
+        c->nstored = n * (n + 1) / 2;
+        c->nblock = 0;
+        c->blocks_on_host = true;
+        c->brow.clear();
+        c->bcol.clear();
+        c->brg.clear();
+        c->boff.clear();
     }
     c->max_pair = (int64_t)max_seg * (max_seg + 1) / 2;
     c->plan.max_gl = max_gl;
@@ -1007,7 +783,6 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     if (c->plan.bytes > 160 * 1024)
         throw MpError(30, "record too large for the record kernel (LDS " + std::to_string(c->plan.bytes) + " bytes)");
     // bordered-band records: their own kernel, LDS plan and per-workgroup inverse scratch
-    std::vector<int64_t> band_ids;
     c->nband = 0;
     for (int64_t r = 0; r < nrec; r++) if (binfo[r]) c->nband++;
     c->band_lfitbb = lfitbb;
@@ -1035,49 +810,20 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     // ---- processing order: records sorted by the first parameter group they touch, so that the
     // records in flight at the same time update a compact set of matrix blocks (cache reuse of
     // the FP64 atomics); results do not depend on the order beyond FP64 summation order
-    std::vector<int64_t> order(nrec);
-    {
-        std::vector<int32_t> key(nrec, INT32_MAX);
-#pragma omp parallel for schedule(static)
-        for (int64_t r = 0; r < nrec; r++)
-            for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
-                int v = c->var_index[gl_tix[q]];
-                if (v > 0) { key[r] = c->group_of[v]; break; }
-            }
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
-    }
+    std::vector<int64_t> order;
+    st_order(nrec, c->ngroup, d_rst, c->d_order, order, s);
     phase("processing order");
     // ---- two-phase build index (group-block storage) ----
     // Each record writes its update of every segment pair (r-major block, lower triangle for the
     // diagonal pairs) contiguously into a buffer; the block gather sums, for every stored block,
     // the values of the records that touch it, in processing order.  Records are taken in
     // batches whose updates fit the buffer budget.
-    std::vector<int64_t> rec_cb(nrec, -1);
-    std::vector<int32_t> rec_cn(nrec, 0);
-    std::vector<int64_t> inc_ptr_all, inc_src_all;
     c->batches.clear();
     if (sparse) {
-        std::vector<int64_t> cnt_r(nrec, 0);
-#pragma omp parallel for schedule(dynamic, 256)
-        for (int64_t r = 0; r < nrec; r++) {
-            if (binfo[r]) continue;                 // bordered-band records update the matrix directly
-            std::vector<int32_t> sz;
-            int lastg = -1;
-            for (int64_t q = rec_gl[r]; q < rec_gl[r + 1]; q++) {
-                int v = c->var_index[gl_tix[q]];
-                if (v <= 0) continue;
-                if (c->group_of[v] != lastg) { lastg = c->group_of[v]; sz.push_back(0); }
-                sz.back()++;
-            }
-            // segment sa stores the columns of all variables up to its end, na values each
-            int64_t n = 0, upto = 0;
-            for (size_t a = 0; a < sz.size(); a++) {
-                upto += sz[a];
-                n += (int64_t)sz[a] * upto;
-            }
-            cnt_r[r] = n;
-        }
+        std::vector<int64_t> rec_cb(nrec, -1);
+        std::vector<int32_t> rec_cn(nrec, 0);
+        for (int64_t r = 0; r < nrec; r++)
+            if (binfo[r]) cnt_r[r] = 0;             // bordered-band records update the matrix directly
         // update-buffer cap: MPGPU_UPDATE_BUFFER_GB, else 45 % of the free device memory up to
         // 128 GB (a whole pass of the large workloads in one batch: the matrix is read and
         // written once per pass, one record-kernel launch)
@@ -1090,91 +836,65 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
             else (void)hipGetLastError();
         }
         const int64_t budget = std::max<int64_t>(1 << 20, (int64_t)(gb * (1ull << 30) / 8.0));
-        int64_t q0 = 0, acc = 0;
+        std::vector<int64_t> bat_inc;                   // incidences of every batch
+        int64_t q0 = 0, acc = 0, ninc = 0;
         for (int64_t q = 0; q < nrec; q++) {
             const int64_t r = order[q];
             if (acc + cnt_r[r] > budget && q > q0) {
                 c->batches.push_back({q0, q, 0, acc, 0});
+                bat_inc.push_back(ninc);
                 q0 = q;
                 acc = 0;
+                ninc = 0;
             }
             rec_cb[r] = cnt_r[r] > 0 ? acc : -1;
             rec_cn[r] = (int32_t)cnt_r[r];
             acc += cnt_r[r];
+            if (cnt_r[r] > 0) ninc += (int64_t)st_nseg[r] * (st_nseg[r] + 1) / 2;
         }
-        if (nrec > q0 || c->batches.empty()) c->batches.push_back({q0, nrec, 0, acc, 0});
-        // incidences: per batch, per block, buffer offsets of the (record, pair) values
-        const int64_t nb = c->nblock, nbat = (int64_t)c->batches.size();
-        inc_ptr_all.assign((size_t)nbat * (nb + 1), 0);
-        int64_t inc_total = 0;
-        for (int64_t bt = 0; bt < nbat; bt++) {
-            auto &B = c->batches[bt];
-            int64_t *ptr = inc_ptr_all.data() + (size_t)bt * (nb + 1);
-            std::vector<std::atomic<int64_t>> cntb(nb);
-            for (auto &x : cntb) x.store(0, std::memory_order_relaxed);
-#pragma omp parallel for schedule(dynamic, 256)
-            for (int64_t q = B.q0; q < B.q1; q++) {
-                const int64_t r = order[q];
-                if (rec_cb[r] < 0) continue;
-                for (int64_t p = rec_pb[r]; p < rec_pb[r + 1]; p++) cntb[pair_blk[p]].fetch_add(1, std::memory_order_relaxed);
-            }
-            ptr[0] = 0;
-            for (int64_t k = 0; k < nb; k++) ptr[k + 1] = ptr[k] + cntb[k].load(std::memory_order_relaxed);
-            B.inc0 = inc_total;
-            inc_src_all.resize((size_t)(inc_total + ptr[nb]));
-            int64_t *src = inc_src_all.data() + inc_total;
-            for (int64_t k = 0; k < nb; k++) cntb[k].store(ptr[k], std::memory_order_relaxed);
-#pragma omp parallel for schedule(dynamic, 256)
-            for (int64_t q = B.q0; q < B.q1; q++) {
-                const int64_t r = order[q];
-                if (rec_cb[r] < 0) continue;
-                // segment sizes again, to place every pair's values
-                std::vector<int32_t> sz;
-                int lastg = -1;
-                for (int64_t g = rec_gl[r]; g < rec_gl[r + 1]; g++) {
-                    int v = c->var_index[gl_tix[g]];
-                    if (v <= 0) continue;
-                    if (c->group_of[v] != lastg) { lastg = c->group_of[v]; sz.push_back(0); }
-                    sz.back()++;
-                }
-                int64_t region = rec_cb[r], upto = 0;
-                int64_t p = rec_pb[r];
-                for (size_t a = 0; a < sz.size(); a++) {
-                    int64_t col = 0;                       // first column of segment b
-                    for (size_t b = 0; b <= a; b++) {
-                        src[cntb[pair_blk[p]].fetch_add(1, std::memory_order_relaxed)] = region + col * sz[a];
-                        col += sz[b];
-                        p++;
-                    }
-                    upto += sz[a];
-                    region += (int64_t)sz[a] * upto;
-                }
-            }
-            // processing order within every block (deterministic summation order)
-#pragma omp parallel for schedule(dynamic, 4096)
-            for (int64_t k = 0; k < nb; k++) std::sort(src + ptr[k], src + ptr[k + 1]);
-            inc_total += ptr[nb];
+        if (nrec > q0 || c->batches.empty()) {
+            c->batches.push_back({q0, nrec, 0, acc, 0});
+            bat_inc.push_back(ninc);
         }
-    }
-    phase("two-phase index");
-    // ---- device upload ----
-    hipStream_t s = c->stream;
-    c->d_order.upload(order, s);
-    if (sparse) {
         c->d_rec_cb.upload(rec_cb, s);
         c->d_rec_cn.upload(rec_cn, s);
-        c->d_inc_ptr.upload(inc_ptr_all, s);
-        c->d_inc_src.upload(inc_src_all, s);
+        // incidences: per batch, per block, buffer offsets of the (record, pair) values
+        const int64_t nb = c->nblock, nbat = (int64_t)c->batches.size();
+        int64_t inc_total = 0;
+        for (int64_t bt = 0; bt < nbat; bt++) {
+            c->batches[bt].inc0 = inc_total;
+            inc_total += bat_inc[bt];
+        }
+        c->d_inc_ptr.alloc((size_t)nbat * (nb + 1));
+        c->d_inc_src.alloc(std::max<int64_t>(inc_total, 1));
+        for (int64_t bt = 0; bt < nbat; bt++) {
+            const auto &B = c->batches[bt];
+            st_incidence(sin, c->d_rec_gl, c->d_gl_tix, c->d_var.p, c->d_group_of.p, max_seg, c->d_order, c->d_rec_pb,
+                         c->d_pair_blk, c->d_rec_cb, B.q0, B.q1, nb, c->d_inc_ptr.p + (size_t)bt * (nb + 1),
+                         c->d_inc_src.p + B.inc0, bat_inc[bt], s);
+        }
         int64_t mx = 1;
         for (auto &B : c->batches) mx = std::max(mx, B.len);
         c->d_cbuf.alloc((size_t)mx);
+        // the transposed products of the off-diagonal blocks (small groups): column-list blocks
+        if (c->plan.max_grp <= 8) {
+            c->d_cc_blk.swap(blk.cc_blk);
+            c->d_tb.alloc((size_t)8 * std::max<int64_t>(1, c->nblock));
+        } else {
+            c->d_cc_blk.release();
+            c->d_tb.release();
+        }
     } else {
         c->d_rec_cb.release();
         c->d_rec_cn.release();
         c->d_inc_ptr.release();
         c->d_inc_src.release();
         c->d_cbuf.release();
+        c->d_rec_pb.release();
+        c->d_pair_blk.release();
     }
+    phase("two-phase index");
+    // ---- remaining device state ----
     if (c->nband > 0) {
         std::vector<int64_t> ids;
         ids.reserve(c->nband);
@@ -1187,54 +907,6 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->d_band_info.release();
         c->d_band_scratch.release();
     }
-    c->d_rec_eq.upload(rec_eq, s);
-    c->d_rec_w.upload(c->rec_ptr, s);
-    c->d_rec_gl.upload(rec_gl, s);
-    c->d_eq_sl.upload(eq_sl, s);
-    c->d_eq_n.upload(eq_n, s);
-    if (c->dbl) c->d_vd.upload(c->vd, s); else c->d_vf.upload(c->vf, s);
-    c->d_widx.upload(widx, s);
-    c->d_gl_tix.upload(gl_tix, s);
-    if (sparse) {
-        c->d_rec_pb.upload(rec_pb, s);
-        c->d_pair_blk.upload(pair_blk, s);
-        c->d_brow.upload(c->brow, s);
-        c->d_bcol.upload(c->bcol, s);
-        c->d_brg.upload(c->brg, s);
-        c->d_boff.upload(c->boff, s);
-        c->d_ccol_ptr.upload(c->ccol_ptr, s);
-        {
-            // the product kernels read a block's column group (rows pass) or row group (columns
-            // pass) as (first index, size) next to its offset
-            std::vector<int32_t> bcf(2 * (size_t)c->nblock), ccfn(2 * c->ccol_blk.size());
-            std::vector<int64_t> ccoff(c->ccol_blk.size());
-            for (int64_t b = 0; b < c->nblock; b++) {
-                bcf[2 * b] = c->gfirst[c->bcol[b]] - 1;
-                bcf[2 * b + 1] = c->gfirst[c->bcol[b] + 1] - c->gfirst[c->bcol[b]];
-            }
-            for (size_t q = 0; q < c->ccol_blk.size(); q++) {
-                const int64_t b = c->ccol_blk[q];
-                ccoff[q] = c->boff[b];
-                ccfn[2 * q] = c->gfirst[c->brg[b]] - 1;
-                ccfn[2 * q + 1] = c->gfirst[c->brg[b] + 1] - c->gfirst[c->brg[b]];
-            }
-            c->d_bcf.upload(bcf, s);
-            c->d_cc_off.upload(ccoff, s);
-            c->d_cc_fn.upload(ccfn, s);
-            if (c->plan.max_grp <= 8 && c->nblock < ((int64_t)1 << 31)) {
-                std::vector<int32_t> ccb(c->ccol_blk.begin(), c->ccol_blk.end());
-                c->d_cc_blk.upload(ccb.empty() ? std::vector<int32_t>(1, 0) : ccb, s);
-                c->d_tb.alloc((size_t)8 * std::max<int64_t>(1, c->nblock));
-            } else {
-                c->d_cc_blk.release();
-                c->d_tb.release();
-            }
-            HIPCHK(hipStreamSynchronize(s));
-        }
-    }
-    c->d_group_of.upload(c->group_of, s);
-    c->d_gfirst.upload(c->gfirst, s);
-    c->d_var.upload(c->var_index, s);
     c->d_par.upload(c->par, s);
     c->d_pre_weight.upload(c->pre_weight, s);
     c->d_mat.alloc(std::max<int64_t>(1, c->nstored));
@@ -1298,17 +970,37 @@ extern "C" int mpgpu_matrix_info(const mpgpu_ctx *c, int64_t *stored, int64_t *n
     return 0;
 }
 
+// host copy of the block structure (built on the device)
+static void host_blocks(mpgpu_ctx *c)
+{
+    if (c->blocks_on_host) return;
+    const int64_t nb = c->nblock;
+    c->brow.resize((size_t)c->ngroup + 1);
+    c->bcol.resize(nb);
+    c->brg.resize(nb);
+    c->boff.resize(nb);
+    c->d_brow.download(c->brow.data(), c->brow.size(), c->stream);
+    c->d_bcol.download(c->bcol.data(), nb, c->stream);
+    c->d_brg.download(c->brg.data(), nb, c->stream);
+    c->d_boff.download(c->boff.data(), nb, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->blocks_on_host = true;
+}
+
 extern "C" int mpgpu_get_block_structure(const mpgpu_ctx *c0, int32_t *gfirst, int64_t *brow, int32_t *bcol,
                                          int64_t *boff)
 {
     mpgpu_ctx *c = const_cast<mpgpu_ctx *>(c0);
     if (!c || !c->built) return 30;
     if (c->opt.matsto != MPGPU_SPARSE) return fail(c, 24, "block structure exists for sparse storage only");
+    API_TRY
+    host_blocks(c);
     if (gfirst) std::copy(c->gfirst.begin(), c->gfirst.end(), gfirst);
     if (brow) std::copy(c->brow.begin(), c->brow.end(), brow);
     if (bcol) std::copy(c->bcol.begin(), c->bcol.end(), bcol);
     if (boff) std::copy(c->boff.begin(), c->boff.begin() + c->nblock, boff);
     return 0;
+    API_CATCH(c)
 }
 
 extern "C" int mpgpu_set_parameters(mpgpu_ctx *c, const double *p)
@@ -1592,6 +1284,7 @@ extern "C" int mpgpu_get_matrix_packed(const mpgpu_ctx *c0, double *packed, int6
         HIPCHK(hipStreamSynchronize(c->stream));
         return 0;
     }
+    host_blocks(c);
     std::vector<double> vals(c->nstored);
     c->d_mat.download(vals.data(), c->nstored, c->stream);
     HIPCHK(hipStreamSynchronize(c->stream));

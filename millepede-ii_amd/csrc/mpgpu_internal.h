@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include <cstring>
+#include <utility>
 
 namespace mpgpu {
 
@@ -56,6 +57,11 @@ template <typename T> struct DBuf {
     void zero(hipStream_t s)
     {
         if (n) HIPCHK(hipMemsetAsync(p, 0, sizeof(T) * n, s));
+    }
+    void swap(DBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
     }
 };
 
@@ -216,6 +222,69 @@ struct DenseFactor {
         l_scale_cols(n, sc.p, v, nrhs, n, s);
     }
 };
+
+// ---- index maps and matrix structure on the device (k_struct.hip) ----
+// the compiled records on the device, as the structure kernels read them
+struct StructIn {
+    int64_t nrec = 0, neq = 0;
+    const int64_t *rec_eq = nullptr, *rec_w = nullptr, *eq_sl = nullptr;   // rec_w: word range of each record
+    const uint32_t *eq_n = nullptr;
+    const int32_t *lab = nullptr;       // the record words as read (global words hold labels)
+    const float *vf = nullptr;
+    const double *vd = nullptr;
+};
+// labels in use: bitmap over [lmin, lmin + 32 nw) and the rank of every 32-bit word
+struct LabelMap {
+    int32_t lmin = 0;
+    int64_t nw = 0;
+    DBuf<uint32_t> bits;
+    DBuf<int64_t> wrank;
+};
+// group pairs in use: bitmap over the lower triangle (a, b <= a) at bit a (a + 1) / 2 + b
+struct PairMap {
+    int64_t nw = 0;
+    DBuf<uint32_t> bits;
+    DBuf<int64_t> wrank;
+};
+// group-block storage: blocks in row-major order of the lower block triangle
+struct BlockDev {
+    int64_t nblock = 0, nstored = 0, ncc = 0;
+    DBuf<int64_t> brow, boff, ccol_ptr, cc_off;
+    DBuf<int32_t> brg, bcol, bcf, cc_fn, cc_blk;
+};
+// equation tables of the records (decodeNextMeasurement): fills in.neq, in.rec_eq, in.eq_sl,
+// in.eq_n; error 20 names the first record that breaks the format
+int64_t st_decode(StructIn &in, DBuf<int64_t> &rec_eq, DBuf<int64_t> &eq_sl, DBuf<uint32_t> &eq_n, hipStream_t s);
+// labels -> total indices; c.labels (sorted) on the host; extra: labels of steering parameters,
+// constraints and external measurements
+void st_label_table(const StructIn &in, const std::vector<int32_t> &extra, LabelMap &map, std::vector<int32_t> &labels,
+                    hipStream_t s);
+// per record: sorted distinct total indices (gl lists), the position of every global word in its
+// record's list (widx), the entry counters and the group-boundary flags (per record or per
+// equation counting)
+void st_record_lists(const StructIn &in, const LabelMap &map, int32_t ntot, bool per_record, DBuf<int32_t> &widx,
+                     DBuf<int64_t> &rec_gl, DBuf<int32_t> &gl_tix, std::vector<int32_t> &ngl_rec,
+                     std::vector<int64_t> &counters, std::vector<uint8_t> &bnd, hipStream_t s);
+// per record: st[k * nrec + r], k = variables, segments, largest local index, local x global
+// words, equations per variable, first group; cnt_r: values of its matrix update; sparse: the
+// group pairs it couples into pm
+void st_record_stats(const StructIn &in, const DBuf<int32_t> &widx, const DBuf<int64_t> &rec_gl,
+                     const DBuf<int32_t> &gl_tix, const int32_t *var, const int32_t *group_of, int32_t ngroup,
+                     int32_t max_ngl, bool sparse, PairMap &pm, std::vector<int32_t> &st, std::vector<int64_t> &cnt_r,
+                     DBuf<int32_t> &d_st, hipStream_t s);
+void st_blocks(PairMap &pm, int32_t ngroup, const int32_t *gfirst, const std::vector<int64_t> &extra_pairs, BlockDev &B,
+               hipStream_t s);
+void st_pair_blocks(const StructIn &in, const DBuf<int64_t> &rec_gl, const DBuf<int32_t> &gl_tix, const int32_t *var,
+                    const int32_t *group_of, int32_t max_nseg, const DBuf<int32_t> &d_st, const PairMap &pm,
+                    DBuf<int64_t> &rec_pb, DBuf<int32_t> &pair_blk, hipStream_t s);
+// processing order: records stably sorted by their first parameter group
+void st_order(int64_t nrec, int32_t ngroup, const DBuf<int32_t> &d_st, DBuf<int64_t> &order, std::vector<int64_t> &h_order,
+              hipStream_t s);
+// incidence lists of the batch order[q0, q1): ptr[nblock + 1], src[nsrc]
+void st_incidence(const StructIn &in, const DBuf<int64_t> &rec_gl, const DBuf<int32_t> &gl_tix, const int32_t *var,
+                  const int32_t *group_of, int32_t max_nseg, const DBuf<int64_t> &order, const DBuf<int64_t> &rec_pb,
+                  const DBuf<int32_t> &pair_blk, const DBuf<int64_t> &rec_cb, int64_t q0, int64_t q1, int64_t nblock,
+                  int64_t *ptr, int64_t *src, int64_t nsrc, hipStream_t s);
 
 // ---- constraint elimination (mpgpu_qldec.cpp, k_qlelim.hip) ----
 // device view of the reflectors: rows over parameter ranges, the transpose per parameter, the WY

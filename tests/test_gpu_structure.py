@@ -1,0 +1,131 @@
+"""GPU: the index maps and the group-block structure built on the device, record decoding
+(special data, FP64 payloads, malformed records).
+
+The block structure is compared exactly with a direct restatement: the blocks are the group
+pairs (a >= b) that the variables of some record couple, plus every diagonal block, in
+row-major order of the lower block triangle, and the offsets are the running sum of the block
+sizes (loop2's pair marking, src/pede.f90:6889-6908, in group-block form).  Record decoding
+follows decodeNextMeasurement (src/pede.f90:2428-2468): special data (src/mille.f90:150-191)
+is skipped, and a record that breaks the format ends the build with exit code 20."""
+import numpy as np
+import pytest
+
+import labels as LB
+import mpgpu
+from helpers import packed_to_dense, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(rp, ii, gg, **opt):
+    P = mpgpu.Pede(**opt)
+    P.add_records(rp, ii, gg)
+    P.build_structure()
+    return P
+
+
+@pytest.mark.parametrize("mcount", [0, 1])
+def test_block_structure_matches_restatement(mcount):
+    rp, ii, gg, _ = mpgpu.pixel_records(17, 5, 6, 4, 4000, 3, 7)
+    P = _build(rp, ii, gg, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=3, mcount=mcount)
+    lab, var, cnt, gsl = P.labels()
+    gfirst, brow, bcol, boff, nstored = P.block_structure()
+    ng = len(gfirst) - 1
+    assert ng > 1 and gfirst[0] == 1 and gfirst[-1] == P.nvar + 1
+    pos = {int(l): t for t, l in enumerate(lab)}
+    pairs = {(g, g) for g in range(ng)}
+    for eqs in LB.decode(rp, ii, gg):
+        vs = {int(var[pos[int(ii[w])]]) for (sL, sG, lG) in eqs for w in range(sG + 1, lG + 1)}
+        gs = sorted({int(np.searchsorted(gfirst, v, side="right")) - 1 for v in vs if v > 0})
+        pairs.update((a, b) for i, a in enumerate(gs) for b in gs[:i + 1])
+    exp = sorted(pairs)
+    size = np.diff(gfirst)
+    exp_brow = np.searchsorted(np.array([a for a, _ in exp]), np.arange(ng + 1), side="left")
+    exp_boff = np.concatenate([[0], np.cumsum([size[a] * size[b] for a, b in exp])])
+    assert len(bcol) == len(exp)
+    assert np.array_equal(bcol, [b for _, b in exp])
+    assert np.array_equal(brow, exp_brow)
+    assert np.array_equal(boff, exp_boff[:-1]) and nstored == exp_boff[-1]
+
+
+def _with_special(rp, ii, gg, every=3, seed=5):
+    """Insert a special-data block (0, 0.0), (0, -nsp), nsp words after the start word of every
+    `every`-th record; the special integers include 0, negative and huge values."""
+    rng = np.random.default_rng(seed)
+    out_p, out_i, out_g = [0], [], []
+    for r in range(len(rp) - 1):
+        w0, w1 = int(rp[r]), int(rp[r + 1])
+        ri, rg = list(ii[w0:w1]), list(gg[w0:w1])
+        if r % every == 0:
+            nsp = int(rng.integers(1, 6))
+            sp_i = [int(x) for x in rng.choice([0, -7, 3, 2 ** 30, 123456789], nsp)]
+            sp_g = list(rng.normal(size=nsp))
+            ri = ri[:1] + [0, 0] + sp_i + ri[1:]
+            rg = rg[:1] + [0.0, -float(nsp)] + sp_g + rg[1:]
+        out_i += ri
+        out_g += rg
+        out_p.append(len(out_i))
+    return np.array(out_p, np.int64), np.array(out_i, np.int32), np.array(out_g, gg.dtype)
+
+
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_special_data_words_are_skipped(matsto):
+    rp, ii, gg, _ = mpgpu.pixel_records(23, 4, 5, 3, 2000, 3, 6)
+    sp, si, sg = _with_special(rp, ii, gg)
+    assert len(si) > len(ii)
+    opt = dict(matsto=matsto, metsol=mpgpu.MINRES, mreqenf=2, mcount=1)
+    P, Q = _build(rp, ii, gg, **opt), _build(sp, si, sg, **opt)
+    for a, b in zip(P.labels(), Q.labels()):
+        assert np.array_equal(a, b)
+    s1, s2 = P.loopn(), Q.loopn()
+    assert s1["chi2"] == s2["chi2"] and s1["ndf"] == s2["ndf"] and s1["nrej"] == s2["nrej"]
+    assert rel_err(P.vector()[0], Q.vector()[0]) < 1e-12
+    assert rel_err(P.matrix_packed(), Q.matrix_packed()) < 1e-12
+
+
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_fp64_payload_matches_fp32(orc, matsto):
+    """The same records as FP64 payload (the values are FP32 numbers, so both payloads carry the
+    same values): index maps equal, and the pass equals the FP32 pass and the oracle's."""
+    rp, ii, gg, _ = mpgpu.pixel_records(29, 4, 5, 3, 2000, 3, 6)
+    opt = dict(matsto=matsto, metsol=mpgpu.MINRES, mreqenf=2, mcount=1)
+    P, Q = _build(rp, ii, gg, **opt), _build(rp, ii, gg.astype(np.float64), **opt)
+    for a, b in zip(P.labels(), Q.labels()):
+        assert np.array_equal(a, b)
+    s1, s2 = P.loopn(), Q.loopn()
+    assert s1["ndf"] == s2["ndf"] and s1["nrej"] == s2["nrej"]
+    assert abs(s1["chi2"] - s2["chi2"]) <= 1e-12 * abs(s1["chi2"])
+    n = P.nall
+    assert rel_err(packed_to_dense(P.matrix_packed(), n), packed_to_dense(Q.matrix_packed(), n)) < 1e-12
+    lab, var = Q.labels()[:2]
+    mat, vec, cnt, so = orc.loopn(rp, ii, gg.astype(np.float64), lab, var, Q.parameters(), Q.nvar)
+    assert abs(s2["chi2"] - so["chi2"]) <= 1e-11 * abs(so["chi2"])
+    assert rel_err(packed_to_dense(Q.matrix_packed(), n)[:Q.nvar, :Q.nvar], packed_to_dense(mat, Q.nvar)) < 1e-10
+
+
+def test_malformed_records_report_exit_code_20():
+    rp, ii, gg, _ = mpgpu.pixel_records(3, 3, 4, 3, 300, 3, 5)
+    opt = dict(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES)
+    # record 41 (1-based) cut after its first measurement word: no error word follows
+    r = 40
+    w0, w1 = int(rp[r]), int(rp[r + 1])
+    keep = np.r_[0:w0 + 2, w1:len(ii)]
+    rp3 = rp.copy()
+    rp3[r + 1:] -= w1 - (w0 + 2)
+    with pytest.raises(mpgpu.MpgpuError) as e:
+        _build(rp3, ii[keep], gg[keep], **opt)
+    assert e.value.code == 20 and "record 41" in str(e.value)
+    # record 8: a local index below 1
+    sL, sG, lG = LB.decode(rp, ii, gg)[7][0]
+    assert sG > sL + 1
+    ii4 = ii.copy()
+    ii4[sL + 1] = -1
+    with pytest.raises(mpgpu.MpgpuError) as e:
+        _build(rp, ii4, gg, **opt)
+    assert e.value.code == 20 and "record 8" in str(e.value)
+    # a global label <= 0
+    ii5 = ii.copy()
+    ii5[lG] = -3
+    with pytest.raises(mpgpu.MpgpuError) as e:
+        _build(rp, ii5, gg, **opt)
+    assert e.value.code == 20
