@@ -499,7 +499,8 @@ __device__ __forceinline__ void bsymv_ld8(double (&bv)[8], const double *B, int 
 // the symmetric element (t, s) times x[fa+s].  Rows are summed over the eight lanes of a slot by a
 // halving butterfly (lane t ends with row t) and then over the slots.  With tb, the transposed
 // product of every off-diagonal block (column t: sum_r B[r][t] x[fa+r]) goes to tb[8 * block + t]
-// from the same loads, for the columns pass to sum without reading the matrix again.
+// from the same loads (the matrix is then read once; k_bsymv_rows_st does this with coalesced
+// loads and is the one used).
 __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64_t *brow, const int32_t *bcf,
                                                      const int64_t *boff, const int32_t *gfirst, const double *a,
                                                      const double *x, double *y, double *tb)
@@ -604,11 +605,260 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
     v += __shfl_xor(v, 32, 64);
     if (slot == 0 && t < nr) y[fa + t] = v;
 }
+// Rows pass for groups of at most 8 parameters with the matrix read once, fully coalesced.  A
+// wavefront takes one block row.  Per step it stages the row's next eight off-diagonal blocks
+// (one contiguous range of the matrix, consecutive lanes reading consecutive doubles) into LDS,
+// with the x values of their column groups beside them.  Lane (k, j) = (lane / 8, lane % 8) then
+// forms row j of block k times x_b (summed over the blocks of the row in the lane) and column j of
+// block k times x_a, the block's transposed product, which goes to tb[tbw * block + j] for the
+// columns pass.  Block descriptors come in windows of 64 blocks (one per lane), the next window
+// loaded one window ahead, so that a step's matrix loads depend on registers only; the next
+// step's loads are in flight while a step computes.  The diagonal block (the last of the row,
+// lower triangle stored) is done by the whole wave as lane (k, j) = element (j, k); the row sums
+// are then added over k.
+#define RS_WAVES 4
+__global__ __launch_bounds__(64 * RS_WAVES) void k_bsymv_rows_st(int32_t ngroup, const int64_t *brow, const int32_t *bcf,
+                                                                const int64_t *boff, const int32_t *gfirst,
+                                                                const double *a, const double *x, double *y, double *tb,
+                                                                int tbw)
+{
+    __shared__ double st[RS_WAVES][512];
+    __shared__ double sx[RS_WAVES][64];
+    __shared__ int sd[RS_WAVES][16];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ga = blockIdx.x * RS_WAVES + w;
+    if (ga >= ngroup) return;                        // whole wavefronts; no workgroup barrier below
+    const int k = lane >> 3, j = lane & 7;
+    const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
+    const int64_t b0 = brow[ga], b1 = brow[ga + 1];
+    int64_t be = b1;
+    double accy = 0.0;
+    if (b1 > b0 && bcf[2 * (b1 - 1)] == fa) {
+        be = b1 - 1;
+        const int tr = min(j, nr - 1), sr = min(k, nr - 1);
+        const double dv = a[boff[b1 - 1] + (tr >= sr ? tr * nr + sr : sr * nr + tr)];
+        const double dx = x[fa + sr];
+        if (j < nr && k < nr) accy = dv * dx;
+    }
+    double xa[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const double v = x[fa + min(r, nr - 1)];
+        xa[r] = r < nr ? v : 0.0;
+    }
+    const int64_t rend = boff[be];                   // end of the row's off-diagonal blocks
+    const int64_t nsteps = (be - b0 + 7) / 8;
+    // window descriptors: block wb + lane's first column, columns and offset (past the end of
+    // the row: no columns, offset rend)
+    // (every load below is issued unconditionally, at a clamped index, so that the compiler's
+    // wait counts stay exact and the loads of the step in flight are not waited for)
+    auto win = [&](int64_t wb, int &f, int &n, int64_t &o) {
+        const int64_t q = wb + lane, qc = min(q, be - 1);
+        const int2 d = *(const int2 *)(bcf + 2 * qc);
+        const int64_t oq = boff[qc];
+        f = q < be ? d.x : 0;
+        n = q < be ? d.y : 0;
+        o = q < be ? oq : rend;
+    };
+    int wf = 0, wn = 0, nf = 0, nn = 0;
+    int64_t wo = rend, no = rend, cw = 0;
+    if (nsteps > 0) win(b0, wf, wn, wo);
+    if (nsteps > 8) win(b0 + 64, nf, nn, no);
+    // two steps in flight: the registers of step t + 2 are loaded while step t computes
+    struct StepRegs {
+        double v[8], x;
+        int o, n, len;
+    };
+    auto load = [&](int64_t t, StepRegs &R) {
+        if (t / 8 != cw) {                           // next window; prefetch the one after it
+            wf = nf;
+            wn = nn;
+            wo = no;
+            cw = t / 8;
+            if ((cw + 1) * 64 < be - b0) win(b0 + (cw + 1) * 64, nf, nn, no);
+        }
+        const int i0 = (int)(t % 8) * 8;
+        const int kn = (int)min((int64_t)8, be - (b0 + 8 * t));
+        const int64_t s0 = __shfl(wo, i0, 64);
+        const int64_t s1 = __shfl(wo + (int64_t)nr * wn, i0 + 7, 64);
+        R.len = (int)(s1 - s0);
+#pragma unroll
+        for (int i = 0; i < 8; i++) R.v[i] = a[s0 + min(lane + 64 * i, R.len - 1)];
+        const int64_t ol = __shfl(wo, i0 + j, 64);
+        const int nl = __shfl(wn, i0 + j, 64);
+        const int fk = __shfl(wf, i0 + k, 64), nk = __shfl(wn, i0 + k, 64);
+        R.o = (int)(ol - s0);
+        R.n = nl;
+        const bool xv = k < kn && j < nk;
+        const double xl = x[xv ? fk + j : 0];
+        R.x = xv ? xl : 0.0;
+    };
+    auto step = [&](int64_t t, StepRegs &R) {
+        const int64_t g0 = b0 + 8 * t;
+        const int kn = (int)min((int64_t)8, be - g0);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int idx = lane + 64 * i;
+            if (64 * i < R.len && idx < R.len) st[w][idx] = R.v[i];
+        }
+        sx[w][lane] = R.x;
+        if (lane < 8) {
+            sd[w][lane] = R.o;
+            sd[w][8 + lane] = R.n;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (t + 2 < nsteps) load(t + 2, R);
+        if (k < kn) {
+            const int off = sd[w][k], nc = sd[w][8 + k];
+            const double *B = &st[w][off];
+            if (j < nr) {
+                double sv = 0.0;
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                    if (c < nc) sv += B[j * nc + c] * sx[w][8 * k + c];
+                accy += sv;
+            }
+            if (j < nc) {
+                double sv = 0.0;
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    if (r < nr) sv += B[r * nc + j] * xa[r];
+                tb[(g0 + k) * tbw + j] = sv;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    StepRegs RA, RB;
+    if (nsteps > 0) load(0, RA);
+    if (nsteps > 1) load(1, RB);
+    for (int64_t t = 0; t < nsteps; t += 2) {
+        step(t, RA);
+        if (t + 1 < nsteps) step(t + 1, RB);
+    }
+    accy += __shfl_xor(accy, 8, 64);
+    accy += __shfl_xor(accy, 16, 64);
+    accy += __shfl_xor(accy, 32, 64);
+    if (k == 0 && j < nr) y[fa + j] = accy;
+}
+// Rows pass when every parameter group has the same size G <= 8 (blocks G x G; group g's first
+// index is g G).  As k_bsymv_rows_st, with the block layout known: the row's off-diagonal blocks
+// are one contiguous range, so a step's matrix loads need no descriptor, and the x values of a
+// block's column group need only its column group (bcol).  Per step the wave issues, in this
+// order, the column groups of step t + 3, the x values of step t + 2 and the matrix values of
+// step t + 2, all unconditionally (clamped indices; the transposed products of lanes without a
+// block go to a spare slot past the end of tb), so that every wait is for the step it needs.
+template <int G>
+__global__ __launch_bounds__(64 * RS_WAVES) void k_bsymv_rows_u(int32_t ngroup, const int64_t *__restrict__ brow,
+                                                               const int32_t *__restrict__ bcol,
+                                                               const int64_t *__restrict__ boff,
+                                                               const double *__restrict__ a, const double *__restrict__ x,
+                                                               double *__restrict__ y, double *__restrict__ tb,
+                                                               int64_t tb_spare)
+{
+    constexpr int S = G * G;                   // values per block
+    __shared__ double st[RS_WAVES][8 * S];
+    __shared__ double sx[RS_WAVES][64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int ga = blockIdx.x * RS_WAVES + w;
+    if (ga >= ngroup) return;                  // whole wavefronts; no workgroup barrier below
+    const int k = lane >> 3, j = lane & 7;
+    const int fa = ga * G;
+    const int64_t b0 = brow[ga], b1 = brow[ga + 1];
+    const int64_t be = b1 - 1;                 // the last block of a row is its diagonal block
+    double accy = 0.0;
+    {
+        const int tr = min(j, G - 1), sr = min(k, G - 1);
+        const double dv = a[boff[be] + (tr >= sr ? tr * G + sr : sr * G + tr)];
+        const double dx = x[fa + sr];
+        if (j < G && k < G) accy = dv * dx;
+    }
+    double xa[G];
+#pragma unroll
+    for (int r = 0; r < G; r++) xa[r] = x[fa + r];
+    const int64_t s_row = boff[b0];
+    const int64_t nsteps = (be - b0 + 7) / 8;
+    constexpr int NV = (8 * S + 63) / 64;      // matrix values per lane and step
+    struct StepRegs {
+        double v[NV], x;
+        int col;                                // column group of the lane's block k (step + 1 ahead)
+    };
+    // column group of block k of step t (clamped to the row's last off-diagonal block)
+    auto ld_col = [&](int64_t t) -> int { return bcol[min(b0 + 8 * t + k, be - 1)]; };
+    auto ld_x = [&](int64_t t, int col) -> double { return x[col * G + min(j, G - 1)]; };
+    auto ld_v = [&](int64_t t, StepRegs &R) {
+        const int len = (int)(min((int64_t)8, be - (b0 + 8 * t)) * S);
+        const int64_t s0 = s_row + 8 * t * S;
+#pragma unroll
+        for (int i = 0; i < NV; i++) R.v[i] = a[s0 + min(lane + 64 * i, len - 1)];
+    };
+    if (nsteps == 0) {                          // a row with its diagonal block only
+        accy += __shfl_xor(accy, 8, 64);
+        accy += __shfl_xor(accy, 16, 64);
+        accy += __shfl_xor(accy, 32, 64);
+        if (k == 0 && j < G) y[fa + j] = accy;
+        return;
+    }
+    // steps past the last are loaded at the last step's indices and computed as empty steps
+    StepRegs RA, RB;
+    RA.col = ld_col(0);
+    RA.x = ld_x(0, RA.col);
+    ld_v(0, RA);
+    RB.col = ld_col(min((int64_t)1, nsteps - 1));
+    RB.x = ld_x(1, RB.col);
+    ld_v(min((int64_t)1, nsteps - 1), RB);
+    int col2 = ld_col(min((int64_t)2, nsteps - 1)), col_next = 0;   // column groups of step t + 2
+    auto step = [&](int64_t t, StepRegs &R) {
+        const int64_t g0 = b0 + 8 * t;
+        const int kn = (int)min((int64_t)8, be - g0);
+        const int len = kn * S;
+        // (no branch in the loop body: lanes past the step's values write the value of its last
+        // entry to that entry again, which the loads gave them)
+#pragma unroll
+        for (int i = 0; i < NV; i++) st[w][min(lane + 64 * i, max(len, 1) - 1)] = R.v[i];
+        sx[w][lane] = R.x;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        {
+            const int64_t t2 = min(t + 2, nsteps - 1), t3 = min(t + 3, nsteps - 1);
+            col_next = ld_col(t3);
+            R.col = col2;
+            R.x = ld_x(t2, col2);
+            ld_v(t2, R);
+            col2 = col_next;
+        }
+        const double *B = &st[w][max(min(k, kn - 1), 0) * S];
+        double sv = 0.0;
+#pragma unroll
+        for (int c = 0; c < G; c++) sv += B[min(j, G - 1) * G + c] * sx[w][8 * k + c];
+        accy += (k < kn && j < G) ? sv : 0.0;
+        double tv = 0.0;
+#pragma unroll
+        for (int r = 0; r < G; r++) tv += B[r * G + min(j, G - 1)] * xa[r];
+        const bool ok = k < kn && j < G;
+        tb[ok ? (g0 + k) * G + j : tb_spare + lane] = tv;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    for (int64_t t = 0; t < nsteps; t += 2) {
+        step(t, RA);
+        step(t + 1, RB);
+    }
+    accy += __shfl_xor(accy, 8, 64);
+    accy += __shfl_xor(accy, 16, 64);
+    accy += __shfl_xor(accy, 32, 64);
+    if (k == 0 && j < G) y[fa + j] = accy;
+}
 // Columns pass from the transposed block products of the rows pass: column group gb sums
-// tb[8 * block + t] over its off-diagonal blocks in column-list order (slot s takes entries
-// s, s+8, ...; one 64-byte read per block), then over the slots.
+// tb[tbw * block + t] over its off-diagonal blocks in column-list order (slot s takes entries
+// s, s+8, ...; one read of the block's values), then over the slots.
 __global__ __launch_bounds__(256) void k_bsymv_colst(int32_t ngroup, const int64_t *ccol_ptr, const int32_t *cc_blk,
-                                                     const int32_t *gfirst, const double *tb, double *y)
+                                                     const int32_t *gfirst, const double *tb, int tbw, double *y)
 {
     const int gb = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63, slot = lane >> 3, t = lane & 7;
@@ -618,12 +868,13 @@ __global__ __launch_bounds__(256) void k_bsymv_colst(int32_t ngroup, const int64
     if (q1 <= q0) return;
     double a0 = 0.0, a1 = 0.0;
     int64_t q = q0 + slot;
+    const int tt = min(t, nc - 1);
     for (; q + 8 < q1; q += 16) {
         const int64_t k0 = cc_blk[q], k1 = cc_blk[q + 8];
-        a0 += tb[8 * k0 + t];
-        a1 += tb[8 * k1 + t];
+        a0 += tb[tbw * k0 + tt];
+        a1 += tb[tbw * k1 + tt];
     }
-    if (q < q1) a0 += tb[8 * (int64_t)cc_blk[q] + t];
+    if (q < q1) a0 += tb[tbw * (int64_t)cc_blk[q] + tt];
     const double acc = sum_slots(a0 + a1);
     if (slot == 0 && t < nc) y[fb + t] += acc;
 }
@@ -1564,16 +1815,35 @@ __global__ void k_zero(int64_t n, double *y)
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
                   const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
                   const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, const int32_t *cc_blk,
-                  double *tb, hipStream_t s)
+                  double *tb, int32_t ugrp, const int32_t *bcol, int64_t nblock, hipStream_t s)
 {
     // rows beyond the parameter groups (Lagrange multipliers) start at zero
     if (nall > nvar) hipLaunchKernelGGL(k_zero, dim3(nblk(nall - nvar)), dim3(TPB), 0, s, nall - nvar, y + nvar);
-    if (maxgrp <= 8 && tb && cc_blk) {
-        // the matrix is read once: rows pass with the transposed block products, then their sums
-        hipLaunchKernelGGL(k_bsymv_rows8, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, brow, bcf, boff,
-                           gfirst, a, x, y, tb);
+    if (ugrp >= 1 && ugrp <= 8 && tb && cc_blk && bcol) {
+        // every group of size ugrp: the block layout is known, descriptors reduce to bcol
+        const dim3 gr((unsigned)((ngroup + RS_WAVES - 1) / RS_WAVES)), bl(64 * RS_WAVES);
+        const int64_t spare = (int64_t)ugrp * nblock;
+        switch (ugrp) {
+        case 1: hipLaunchKernelGGL(k_bsymv_rows_u<1>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+        case 2: hipLaunchKernelGGL(k_bsymv_rows_u<2>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+        case 3: hipLaunchKernelGGL(k_bsymv_rows_u<3>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+        case 4: hipLaunchKernelGGL(k_bsymv_rows_u<4>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+        case 5: hipLaunchKernelGGL(k_bsymv_rows_u<5>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+        case 6: hipLaunchKernelGGL(k_bsymv_rows_u<6>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+        case 7: hipLaunchKernelGGL(k_bsymv_rows_u<7>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+        default: hipLaunchKernelGGL(k_bsymv_rows_u<8>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+        }
         hipLaunchKernelGGL(k_bsymv_colst, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_blk,
-                           gfirst, tb, y);
+                           gfirst, tb, ugrp, y);
+        return;
+    }
+    if (maxgrp <= 8 && tb && cc_blk) {
+        // the matrix is read once: rows pass with the transposed block products (maxgrp values
+        // per block), then their sums
+        hipLaunchKernelGGL(k_bsymv_rows_st, dim3((unsigned)((ngroup + RS_WAVES - 1) / RS_WAVES)), dim3(64 * RS_WAVES), 0, s,
+                           ngroup, brow, bcf, boff, gfirst, a, x, y, tb, maxgrp);
+        hipLaunchKernelGGL(k_bsymv_colst, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_blk,
+                           gfirst, tb, maxgrp, y);
         return;
     }
     if (maxgrp <= 8)

@@ -48,6 +48,7 @@ struct mpgpu_ctx {
     std::vector<int32_t> gfirst, group_of; // variable groups
     std::vector<int64_t> brow, boff;
     std::vector<int32_t> bcol, brg;
+    int32_t uniform_grp = 0;              // size of every parameter group when they are all equal, else 0
     bool blocks_on_host = true;           // brow/bcol/brg/boff copied from the device (on first use)
     int64_t nblock = 0, nstored = 0;
     // constraints (CSR over variable indices) and residual bookkeeping
@@ -81,7 +82,7 @@ struct mpgpu_ctx {
     DBuf<int64_t> d_brow, d_boff, d_ccol_ptr, d_cc_off;
     DBuf<int32_t> d_bcf, d_cc_fn;   // per block / per column-list entry: first index and size of the other group
     DBuf<int32_t> d_cc_blk;         // per column-list entry: the block
-    DBuf<double> d_tb;              // per off-diagonal block: its transposed product (8 values, groups <= 8)
+    DBuf<double> d_tb;              // per block: its transposed product (max_grp values, groups <= 8)
     DBuf<double> d_psymv;           // full storage: partial sums of the packed product (deterministic)
     DBuf<unsigned long long> d_cnt;
     DBuf<double> d_chi2;
@@ -134,7 +135,7 @@ static int fail(mpgpu_ctx *c, int code, const std::string &m)
     return code;
 }
 
-#define API_TRY try {
+#define API_TRY try { AllocScope alloc_scope_(c ? c->stream : nullptr);
 #define API_CATCH(ctx)                                                                   \
     }                                                                                    \
     catch (const MpError &e) { return fail(ctx, e.code, e.msg); }                        \
@@ -195,6 +196,16 @@ extern "C" int mpgpu_create(const mpgpu_options *opt, int32_t device, mpgpu_ctx 
     c->device = device;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        // freed stream-ordered buffers stay in the device's pool up to 32 GB, so that the
+        // buffers of the next calls are made without returning to the driver
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+            uint64_t keep = (uint64_t)32 << 30;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        (void)hipGetLastError();
+    }
     *out = c;
     return 0;
     }
@@ -206,10 +217,14 @@ extern "C" int mpgpu_destroy(mpgpu_ctx *c)
 {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    c->stream = nullptr;
+    // the context's buffers are freed in the order of its stream: the stream goes last
+    hipStream_t s = c->stream;
+    if (s) (void)hipStreamSynchronize(s);
     delete c;
+    if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
     return 0;
 }
 
@@ -778,6 +793,10 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         // group-block storage: row descriptors of the update buffer
         c->plan.stage = sparse ? 1 : 0;
         c->plan.max_grp = maxgrp;
+        // every group of one size: the block SpMV knows the block layout
+        bool uni = true;
+        for (int32_t g = 0; g < c->ngroup; g++) uni = uni && c->gfirst[g + 1] - c->gfirst[g] == maxgrp;
+        c->uniform_grp = uni ? maxgrp : 0;
     }
     lds_bytes(c->plan);
     if (c->plan.bytes > 160 * 1024)
@@ -879,7 +898,8 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         // the transposed products of the off-diagonal blocks (small groups): column-list blocks
         if (c->plan.max_grp <= 8) {
             c->d_cc_blk.swap(blk.cc_blk);
-            c->d_tb.alloc((size_t)8 * std::max<int64_t>(1, c->nblock));
+            // transposed products of the blocks, and a spare slot per lane for the uniform kernel
+            c->d_tb.alloc((size_t)c->plan.max_grp * std::max<int64_t>(1, c->nblock) + 64);
         } else {
             c->d_cc_blk.release();
             c->d_tb.release();
@@ -1336,7 +1356,8 @@ void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
     hipStream_t s = c->stream;
     if (c->opt.matsto == MPGPU_SPARSE) {
         l_block_symv(c->ngroup, c->d_brow.p, c->d_bcf.p, c->d_boff.p, c->d_gfirst.p, c->d_ccol_ptr.p, c->d_cc_off.p,
-                     c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, c->plan.max_grp, c->d_cc_blk.p, c->d_tb.p, s);
+                     c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, c->plan.max_grp, c->d_cc_blk.p, c->d_tb.p,
+                     c->uniform_grp, c->d_bcol.p, c->nblock, s);
         l_cons_symv(c->ncon, c->nvar, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, x, y, s);
     } else {
         l_packed_symv(c->nall, c->d_mat.p, x, y, c->d_psymv.p, s);
@@ -2081,6 +2102,7 @@ static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_
         sc.alloc(4 * (size_t)nrhs);
         std::vector<double> bb(nrhs), rz(nrhs), pq(nrhs), rr(nrhs), xx(nrhs), coef(nrhs);
         std::vector<char> active(nrhs, 1), conv(nrhs, 0);
+        std::vector<double> hist;
         auto dots = [&](const double *A, const double *B, std::vector<double> &out) {
             l_col_dots(nv, nrhs, A, B, sc.p, s);
             sc.download(out.data(), nrhs, s);
@@ -2118,15 +2140,26 @@ static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_
             dots(R.p, R.p, rr);
             dots(X.p, X.p, xx);
             int left = 0;
+            double worst = 0.0;     // largest backward error of the columns still running
             for (int q = 0; q < nrhs; q++) {
                 if (!active[q]) continue;
+                const double be = std::sqrt(rr[q]) / (F.anorm * std::sqrt(xx[q]));
                 if (verbose && q == 0)
                     std::fprintf(stderr, "mpgpu: refinement step %d: ||r|| / (||A||_F ||x||) = %.3e (tol %.3e), ||r||/||b|| = %.3e\n",
-                                 it, std::sqrt(rr[q]) / (F.anorm * std::sqrt(xx[q])), tol, std::sqrt(rr[q] / bb[q]));
-                if (std::sqrt(rr[q]) <= tol * F.anorm * std::sqrt(xx[q])) { active[q] = 0; conv[q] = 1; }
-                else left++;
+                                 it, be, tol, std::sqrt(rr[q] / bb[q]));
+                if (be <= tol) { active[q] = 0; conv[q] = 1; }
+                else { left++; worst = std::max(worst, be); }
             }
             if (left == 0 || it == maxit) break;
+            // give up early when the rate of the last steps cannot reach tol within refine_steps
+            // (an ill-conditioned matrix: the FP64 factor is then cheaper than the steps left)
+            hist.push_back(worst);
+            const int span = 4;
+            if ((int)hist.size() > span) {
+                const double rate = std::pow(hist.back() / hist[hist.size() - 1 - span], 1.0 / span);
+                const double need = rate < 1.0 ? std::log(tol / hist.back()) / std::log(rate) : 1e30;
+                if (it + need > maxit) break;
+            }
             l_copy(len, R.p, Z.p, s);
             F.solve32(Z.p, nrhs, s);
             std::vector<double> rz2(nrhs);

@@ -22,26 +22,50 @@ struct MpError {
     MpError(int c, std::string m) : code(c), msg(std::move(m)) {}
 };
 
-// owning device buffer
+// Stream of the library call in progress on this thread.  Device buffers made during a call are
+// stream-ordered on its context's stream (hipMallocAsync / hipFreeAsync): hipFree synchronizes
+// the whole device, which would make concurrent contexts (serving workers) wait for each other's
+// kernels at every freed buffer.
+inline hipStream_t &alloc_stream()
+{
+    static thread_local hipStream_t s = nullptr;
+    return s;
+}
+struct AllocScope {
+    hipStream_t prev;
+    explicit AllocScope(hipStream_t s) : prev(alloc_stream()) { alloc_stream() = s; }
+    ~AllocScope() { alloc_stream() = prev; }
+    AllocScope(const AllocScope &) = delete;
+    AllocScope &operator=(const AllocScope &) = delete;
+};
+
+// owning device buffer; freed in the order of the stream it was made on (see alloc_stream)
 template <typename T> struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    hipStream_t as = nullptr;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (as) (void)hipFreeAsync(p, as);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        as = nullptr;
     }
     void alloc(size_t count)
     {
         if (count == n && p) return;
         release();
         if (count == 0) return;
-        HIPCHK(hipMalloc((void **)&p, sizeof(T) * count));
+        as = alloc_stream();
+        if (as) HIPCHK(hipMallocAsync((void **)&p, sizeof(T) * count, as));
+        else HIPCHK(hipMalloc((void **)&p, sizeof(T) * count));
         n = count;
     }
     void upload(const T *h, size_t count, hipStream_t s)
@@ -62,6 +86,7 @@ template <typename T> struct DBuf {
     {
         std::swap(p, o.p);
         std::swap(n, o.n);
+        std::swap(as, o.as);
     }
 };
 
@@ -125,7 +150,7 @@ void l_mr_update(int64_t n, const MrState *st, const double *v, double *w, doubl
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
                   const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
                   const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, const int32_t *cc_blk,
-                  double *tb, hipStream_t s);
+                  double *tb, int32_t ugrp, const int32_t *bcol, int64_t nblock, hipStream_t s);
 void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
                  double *y, hipStream_t s);
 void l_scale_copy(int64_t n, double sc, const double *a, double *o, hipStream_t s);

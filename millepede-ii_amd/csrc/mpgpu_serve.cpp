@@ -8,6 +8,8 @@
 #include "mpgpu_internal.h"
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -56,6 +58,7 @@ int64_t estimate_footprint(const Job &j)
     // distinct labels of the global words: after the second zero-index word of a measurement
     // (measurement, locals, sigma, globals; src/mille.f90:27-59)
     std::vector<int32_t> lab;
+    lab.reserve((size_t)nw / 2);
     int64_t upd = 0;
     for (int64_t r = 0; r < nrec; r++) {
         int zeros = 0;
@@ -68,8 +71,15 @@ int64_t estimate_footprint(const Job &j)
     }
     if (j.opt.matsto == MPGPU_FULL) upd = 0;
     upd = std::min<int64_t>(upd, (int64_t)128 << 30);
-    std::sort(lab.begin(), lab.end());
-    int64_t nlab = std::unique(lab.begin(), lab.end()) - lab.begin();
+    // number of distinct labels: a bitmap over their range (linear, on the submitting thread)
+    int64_t nlab = 0;
+    if (!lab.empty()) {
+        const auto mm = std::minmax_element(lab.begin(), lab.end());
+        const int64_t lo = *mm.first, span = (int64_t)*mm.second - lo + 1;
+        std::vector<uint64_t> bits((span + 63) / 64, 0);
+        for (int32_t l : lab) bits[(l - lo) >> 6] |= 1ull << ((l - lo) & 63);
+        for (uint64_t b : bits) nlab += __builtin_popcountll(b);
+    }
     int64_t nall = nlab + (int64_t)j.crhs.size();
     int64_t mat;
     if (j.opt.matsto == MPGPU_FULL) {
@@ -110,6 +120,15 @@ struct mpgpu_server {
     void run_job(Job &j, mpgpu_ctx *&ctx)
     {
         j.t_start = clk::now();
+        // MPGPU_VERBOSE: time of each step of the job on stderr
+        static const bool verbose = std::getenv("MPGPU_VERBOSE") != nullptr;
+        clk::time_point tp = j.t_start;
+        double ms[6] = {0, 0, 0, 0, 0, 0};
+        auto lap = [&](int k) {
+            const clk::time_point t = clk::now();
+            ms[k] = std::chrono::duration<double, std::milli>(t - tp).count();
+            tp = t;
+        };
         int rc = 0;
         mpgpu_ctx *c = nullptr;
         rc = mpgpu_create(&j.opt, device, &c);
@@ -120,12 +139,21 @@ struct mpgpu_server {
             std::vector<double> sig(j.crhs.size(), 0.0);
             rc = mpgpu_add_equations(c, 0, (int32_t)j.crhs.size(), j.cptr.data(), j.clab.data(), j.cfac.data(), j.crhs.data(), sig.data());
         }
+        lap(0);
         int32_t ntot = 0, nvar = 0, ncon = 0;
         if (!rc) rc = mpgpu_build_structure(c, &ntot, &nvar, &ncon);
+        lap(1);
         mpgpu_pass_stats ps{};
         if (!rc) rc = mpgpu_loopn(c, 1, 1, 0.0f, &ps);
+        lap(2);
         std::vector<double> x((size_t)nvar + ncon);
         if (!rc) rc = mpgpu_solve(c, nullptr, x.data(), &j.sst);
+        lap(3);
+        if (verbose)
+            std::fprintf(stderr, "job %lld: nvar %d nrec %lld: create+copy %.1f structure %.1f pass %.1f solve %.1f ms "
+                                 "(method %d, itn %d)\n",
+                         (long long)j.id, nvar, (long long)j.rec_ptr.size() - 1, ms[0], ms[1], ms[2], ms[3],
+                         j.sst.method, j.sst.itn);
         if (!rc) {
             std::vector<int32_t> lab(ntot), var(ntot), grp(ntot);
             std::vector<int64_t> cnt(ntot);
