@@ -1634,7 +1634,10 @@ void minres_dev(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_solve_st
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         if (!stopped() && k < itnlim) {
-            bool graphed = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            // MPGPU_NO_GRAPH: launch every iteration's kernels directly (for tools that do not
+            // follow graph launches; rocprofv3's kernel trace has crashed inside hipGraphLaunch)
+            static const bool no_graph = std::getenv("MPGPU_NO_GRAPH") != nullptr;
+            bool graphed = !no_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (graphed) {
                 for (int q = 1; q <= check; q++) iteration(k + q);
                 graphed = hipStreamEndCapture(s, &graph) == hipSuccess && graph != nullptr &&
