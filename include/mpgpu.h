@@ -79,14 +79,16 @@ typedef struct mpgpu_options {
                              /* the LAPACK methods (src/pede.f90:9017-9120, ilperr)     */
     int32_t pair_entries;    /* 'pairentries n': mreqpe, minimum number of entries of a */
                              /* pair of parameter groups for its matrix block to be      */
-                             /* stored; updates of rarer pairs are dropped (1 = all;     */
-                             /* src/pede.f90:10762-10772, 6564, 7036; src/mpbits.f90)    */
+                             /* stored (src/pede.f90:10762-10772, 6564, 7036;            */
+                             /* src/mpbits.f90); only 1 (every pair stored) is built:    */
+                             /* mpgpu_build_structure returns 40 for larger values       */
     int32_t skip_empty_cons; /* 'skipemptycons' (iskpec): drop constraints without any  */
                              /* variable parameter; without it such a constraint stops  */
                              /* the run (src/pede.f90:1164-1190)                         */
     int32_t resolve_redundancy_cons; /* 'resolveredundancycons' (irslvrc): a constraint */
-                             /* block whose product matrix is rank deficient is reduced */
-                             /* to its independent constraints (src/pede.f90:1028-1050) */
+                             /* group (grpcon's grouping) with as many constraints as    */
+                             /* parameters of its own fixes those parameters (-4) and is */
+                             /* dropped (src/pede.f90:1015-1050, 5965-5966)              */
     double scale_errors[2];  /* 'scaleerrors f1 f2': errorScaleFactor, measurement      */
                              /* errors with (1) / without (2) global derivatives are     */
                              /* multiplied by it (src/mppar.f90:312-316); default 1, 1   */

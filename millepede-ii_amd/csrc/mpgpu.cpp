@@ -439,6 +439,9 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     if (!c) return 30;
     API_TRY
     HIPCHK(hipSetDevice(c->device));
+    // 'pairentries' above 1 drops the matrix updates of rarely filled parameter-group pairs
+    // (src/pede.f90:6564, 7036, src/mpbits.f90): not built, refused as the steering parser does
+    if (c->opt.pair_entries > 1) throw MpError(40, "pairentries > 1 (dropping rare parameter pairs) is not supported");
     const int64_t nrec = (int64_t)c->rec_ptr.size() - 1;
     c->nrec = nrec;
     const int32_t *I = c->inder.data();
@@ -560,28 +563,71 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         c->par[t] = c->sp_value[k];
         c->presig_tot[t] = c->sp_presig[k];
     }
-    // constraint groups (grpcon, src/pede.f90:777-1055): constraints joined by common parameters.
-    // A group with as many constraints as parameters fixes those parameters completely; with
-    // 'resolveredundancycons' its parameters become fixed (-4) and its constraints are dropped.
+    // constraint groups (grpcon, src/pede.f90:854-1060).  The constraints are taken in the order
+    // of their (first, last) parameter; one that no earlier constraint has marked joins the group
+    // of the first related constraint (one sharing a parameter) that has a group, or opens a new
+    // group; it then sets the group of its parameters and marks every related constraint with it
+    // (constraints marked before keep their group unless re-marked, parameters keep the group of
+    // the last constraint that set them).  A group counts the parameters in its range whose group
+    // is that group; with as many constraints as such parameters and 'resolveredundancycons' the
+    // parameters become fixed (-4) and the group's constraints are dropped.  Constraints with the
+    // same (first, last) parameter share a parameter and end in one group whatever their order.
     std::vector<char> redundant(ntot, 0), con_dropped(c->cons.size(), 0);
     if (c->opt.resolve_redundancy_cons && !c->cons.empty()) {
-        const size_t nc = c->cons.size();
-        std::vector<size_t> up(nc);
-        std::iota(up.begin(), up.end(), 0);
-        std::function<size_t(size_t)> root = [&](size_t a) { while (up[a] != a) a = up[a] = up[up[a]]; return a; };
-        std::vector<int64_t> owner(ntot, -1);
-        for (size_t j = 0; j < nc; j++)
+        const int nc = (int)c->cons.size();
+        std::vector<std::vector<int32_t>> plist(nc);
+        std::vector<std::vector<int32_t>> parcons(ntot);
+        for (int j = 0; j < nc; j++) {
             for (int32_t lab : c->cons[j].labels) {
                 const int32_t t = tix_of(lab);
-                if (owner[t] < 0) owner[t] = (int64_t)j;
-                else up[root(j)] = root((size_t)owner[t]);
+                plist[j].push_back(t);
+                parcons[t].push_back(j);            // every occurrence, in input order
             }
-        std::vector<int64_t> ncon_g(nc, 0), npar_g(nc, 0);
-        for (size_t j = 0; j < nc; j++) ncon_g[root(j)]++;
-        for (int32_t t = 0; t < ntot; t++) if (owner[t] >= 0) npar_g[root((size_t)owner[t])]++;
-        for (int32_t t = 0; t < ntot; t++)
-            if (owner[t] >= 0) { const size_t g = root((size_t)owner[t]); if (ncon_g[g] == npar_g[g]) redundant[t] = 1; }
-        for (size_t j = 0; j < nc; j++) { const size_t g = root(j); if (ncon_g[g] == npar_g[g]) con_dropped[j] = 1; }
+            std::sort(plist[j].begin(), plist[j].end());
+            plist[j].erase(std::unique(plist[j].begin(), plist[j].end()), plist[j].end());
+        }
+        std::vector<int> ord;                       // constraints with parameters
+        for (int j = 0; j < nc; j++) if (!plist[j].empty()) ord.push_back(j);
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+            if (plist[x].front() != plist[y].front()) return plist[x].front() < plist[y].front();
+            return plist[x].back() < plist[y].back();
+        });
+        std::vector<int> grp(nc, 0), gpc(ntot, 0);
+        int ngrp = 0;
+        for (int jc : ord) {
+            int g = grp[jc];
+            if (g == 0) {
+                for (int32_t t : plist[jc]) {
+                    for (int j2 : parcons[t]) if ((g = grp[j2]) > 0) break;
+                    if (g > 0) break;
+                }
+                if (g == 0) g = ++ngrp;
+            }
+            for (int32_t t : plist[jc]) {
+                gpc[t] = g;
+                for (int j2 : parcons[t]) grp[j2] = g;
+            }
+        }
+        std::vector<int64_t> ncon_g(ngrp + 1, 0);
+        std::vector<int32_t> lo(ngrp + 1, INT32_MAX), hi(ngrp + 1, -1);
+        for (int j = 0; j < nc; j++) {
+            const int g = grp[j];
+            if (g == 0) continue;
+            ncon_g[g]++;
+            lo[g] = std::min(lo[g], plist[j].front());
+            hi[g] = std::max(hi[g], plist[j].back());
+        }
+        std::vector<char> red_g(ngrp + 1, 0);
+        for (int g = 1; g <= ngrp; g++) {
+            if (ncon_g[g] == 0) continue;
+            int64_t npar = 0;
+            for (int32_t t = lo[g]; t <= hi[g]; t++) npar += gpc[t] == g;
+            if (npar == ncon_g[g]) {
+                red_g[g] = 1;
+                for (int32_t t = lo[g]; t <= hi[g]; t++) if (gpc[t] == g) redundant[t] = 1;
+            }
+        }
+        for (int j = 0; j < nc; j++) if (grp[j] > 0 && red_g[grp[j]]) con_dropped[j] = 1;
     }
     c->var_index.assign(ntot, 0);
     c->var_to_tot.clear();

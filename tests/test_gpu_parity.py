@@ -655,6 +655,39 @@ def test_empty_and_redundant_constraints():
     assert R.ncon == 4
 
 
+def test_redundant_constraint_groups_follow_grpcon():
+    """pede's constraint grouping (grpcon, src/pede.f90:938-1060) is greedy in (first, last)
+    parameter order: a constraint joins the group of its first related constraint with a group,
+    sets the group of its parameters and re-marks its related constraints.  With the tix ranks
+    p1 < p2 < p3 < p5 < p6 < p8 and d{p1,p5}, c{p2,p6}, c2{p2,p3}, c3{p5,p8}, c4{p6,p8} the
+    procedure ends with groups {d, c, c3, c4} and {c2}, and parameter groups p1, p5, p6, p8 -> 1,
+    p2, p3 -> 2 (p6 re-set by c4).  Group 1 counts 4 constraints and 4 parameters of its own in its
+    range: with 'resolveredundancycons' p1, p5, p6, p8 become fixed (-4) and the four constraints
+    are dropped; c2 stays.  (Connected components would give one group of 5 constraints on 6
+    parameters: nothing resolved.)"""
+    rp, ii, gg, _ = mpgpu.pixel_records(35, 4, 5, 3, 3000, 3, 6)
+    P0 = _pede(rp, ii, gg, matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=1, mcount=1)
+    lab = P0.labels()[0]
+    P0.close()
+    k0 = len(lab) // 2
+    p1, p2, p3, p5, p6, p8 = (int(lab[k0 + d]) for d in (0, 1, 2, 4, 5, 7))
+    cons = [(0.0, 0.0, [p1, p5], [1.0, 1.0]),     # d
+            (0.0, 0.0, [p2, p6], [1.0, -1.0]),    # c
+            (0.0, 0.0, [p2, p3], [1.0, 1.0]),     # c2
+            (0.0, 0.0, [p5, p8], [1.0, 2.0]),     # c3
+            (0.0, 0.0, [p6, p8], [2.0, 1.0])]     # c4
+    Q = mpgpu.Pede(matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=1, mcount=1, withmultipliers=1,
+                   resolve_redundancy_cons=1)
+    Q.add_records(rp, ii, gg)
+    Q.add_constraints(cons)
+    Q.build_structure()
+    lab, var = Q.labels()[:2]
+    pos = {int(l): i for i, l in enumerate(lab)}
+    assert [int(var[pos[p]]) for p in (p1, p5, p6, p8)] == [-4] * 4
+    assert var[pos[p2]] > 0 and var[pos[p3]] > 0 and Q.ncon == 1
+    Q.close()
+
+
 @pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
 def test_cauchy_downweighting_closed_form(matsto):
     """The device's last outlier iteration with lhuber > 3 (Cauchy weights and chi^2,
