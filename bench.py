@@ -264,6 +264,7 @@ def main():
     info = P.matrix_info()
     log(f"structure: ntot={ntot} nvar={nvar} ncon={ncon} stored={info['stored']} blocks={info['nblocks']} "
         f"in {t_struct:.1f}s")
+    log(f"block product autotuner: {P.tuning()}")
 
     for _ in range(args.warmup):
         st = P.loopn()
@@ -309,6 +310,7 @@ def main():
                        "algorithmic_bytes_per_launch": per_launch, "mean_nalg": mean_nalg,
                        "gather_ms_per_launch": sum(gath_ms) / max(sum(gath_launches), 1),
                        "traffic_source": traffic["source"] if traffic else None}
+    out["spmv_autotuner"] = P.tuning()
     lab, var, cnt, gsl = P.labels()
     lab_var = (lab, var, P.parameters(), P.nvar)
     blocks = P.block_structure()

@@ -172,6 +172,10 @@ int mpgpu_matrix_info(const mpgpu_ctx *ctx, int64_t *stored_words, int64_t *nblo
  * row, column group <= row group), boff[nblocks] offset of the row-major block in the stored
  * matrix words; a diagonal block uses its lower triangle.  Sizes from mpgpu_matrix_info. */
 int mpgpu_get_block_structure(const mpgpu_ctx *ctx, int32_t *gfirst, int64_t *brow, int32_t *bcol, int64_t *boff);
+/* the choices of the library's autotuner for this structure (no counterpart in the reference:
+ * pede has no tunable kernels): the group-block product variants timed at the structure build,
+ * with their median times, and the one chosen; NUL-terminated text of at most len bytes */
+int mpgpu_get_tuning(const mpgpu_ctx *ctx, char *text, int32_t len);
 
 /* ---- state ------------------------------------------------------------------------------- */
 int mpgpu_set_parameters(mpgpu_ctx *ctx, const double *par_by_total_index);   /* globalParameter */

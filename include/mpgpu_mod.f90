@@ -132,6 +132,12 @@ MODULE mpgpu_mod
             INTEGER(c_int64_t), INTENT(OUT) :: stored, nblocks
             INTEGER(c_int32_t), INTENT(OUT) :: ngroups
         END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_get_tuning(ctx, text, len) BIND(C, name='mpgpu_get_tuning')
+            IMPORT :: c_int, c_ptr, c_int32_t, c_char
+            TYPE(c_ptr), VALUE :: ctx
+            CHARACTER(KIND=c_char), INTENT(OUT) :: text(*)
+            INTEGER(c_int32_t), VALUE :: len
+        END FUNCTION
         INTEGER(c_int) FUNCTION mpgpu_set_parameters(ctx, par) BIND(C, name='mpgpu_set_parameters')
             IMPORT :: c_int, c_ptr, c_double
             TYPE(c_ptr), VALUE :: ctx

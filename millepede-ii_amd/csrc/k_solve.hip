@@ -616,17 +616,17 @@ __global__ __launch_bounds__(256) void k_bsymv_rows8(int32_t ngroup, const int64
 // step's loads are in flight while a step computes.  The diagonal block (the last of the row,
 // lower triangle stored) is done by the whole wave as lane (k, j) = element (j, k); the row sums
 // are then added over k.
-#define RS_WAVES 4
-__global__ __launch_bounds__(64 * RS_WAVES) void k_bsymv_rows_st(int32_t ngroup, const int64_t *brow, const int32_t *bcf,
-                                                                const int64_t *boff, const int32_t *gfirst,
-                                                                const double *a, const double *x, double *y, double *tb,
-                                                                int tbw)
+// (W: wavefronts per workgroup, chosen per matrix by the autotuner of the product)
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_bsymv_rows_st(int32_t ngroup, const int64_t *brow, const int32_t *bcf,
+                                                         const int64_t *boff, const int32_t *gfirst, const double *a,
+                                                         const double *x, double *y, double *tb, int tbw)
 {
-    __shared__ double st[RS_WAVES][512];
-    __shared__ double sx[RS_WAVES][64];
-    __shared__ int sd[RS_WAVES][16];
+    __shared__ double st[W][512];
+    __shared__ double sx[W][64];
+    __shared__ int sd[W][16];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ga = blockIdx.x * RS_WAVES + w;
+    const int ga = blockIdx.x * W + w;
     if (ga >= ngroup) return;                        // whole wavefronts; no workgroup barrier below
     const int k = lane >> 3, j = lane & 7;
     const int fa = gfirst[ga] - 1, nr = gfirst[ga + 1] - 1 - fa;
@@ -751,8 +751,8 @@ __global__ __launch_bounds__(64 * RS_WAVES) void k_bsymv_rows_st(int32_t ngroup,
 // order, the column groups of step t + 3, the x values of step t + 2 and the matrix values of
 // step t + 2, all unconditionally (clamped indices; the transposed products of lanes without a
 // block go to a spare slot past the end of tb), so that every wait is for the step it needs.
-template <int G>
-__global__ __launch_bounds__(64 * RS_WAVES) void k_bsymv_rows_u(int32_t ngroup, const int64_t *__restrict__ brow,
+template <int G, int W>
+__global__ __launch_bounds__(64 * W) void k_bsymv_rows_u(int32_t ngroup, const int64_t *__restrict__ brow,
                                                                const int32_t *__restrict__ bcol,
                                                                const int64_t *__restrict__ boff,
                                                                const double *__restrict__ a, const double *__restrict__ x,
@@ -760,10 +760,10 @@ __global__ __launch_bounds__(64 * RS_WAVES) void k_bsymv_rows_u(int32_t ngroup, 
                                                                int64_t tb_spare)
 {
     constexpr int S = G * G;                   // values per block
-    __shared__ double st[RS_WAVES][8 * S];
-    __shared__ double sx[RS_WAVES][64];
+    __shared__ double st[W][8 * S];
+    __shared__ double sx[W][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int ga = blockIdx.x * RS_WAVES + w;
+    const int ga = blockIdx.x * W + w;
     if (ga >= ngroup) return;                  // whole wavefronts; no workgroup barrier below
     const int k = lane >> 3, j = lane & 7;
     const int fa = ga * G;
@@ -1812,38 +1812,75 @@ __global__ void k_zero(int64_t n, double *y)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = 0.0;
 }
+// Product variants with the same arithmetic (same summation order, bit-identical results), for
+// the autotuner: kind (0 the uniform-group rows pass, 1 the general staged rows pass) * 16 + W
+// wavefronts per workgroup
+int block_symv_variants(int32_t ugrp, int32_t maxgrp, int *out)
+{
+    int n = 0;
+    if (maxgrp > 8) return 0;
+    for (int w : {2, 4, 8}) {
+        if (ugrp >= 1 && ugrp <= 8) out[n++] = 0 * 16 + w;
+        out[n++] = 1 * 16 + w;
+    }
+    return n;
+}
+template <int G, int W>
+static void launch_rows_u(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const double *a,
+                          const double *x, double *y, double *tb, int64_t spare, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_bsymv_rows_u<G, W>), dim3((unsigned)((ngroup + W - 1) / W)), dim3(64 * W), 0, s, ngroup, brow,
+                       bcol, boff, a, x, y, tb, spare);
+}
+template <int W>
+static void launch_rows_u_g(int32_t g, int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff,
+                            const double *a, const double *x, double *y, double *tb, int64_t spare, hipStream_t s)
+{
+    switch (g) {
+    case 1: launch_rows_u<1, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 2: launch_rows_u<2, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 3: launch_rows_u<3, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 4: launch_rows_u<4, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 5: launch_rows_u<5, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 6: launch_rows_u<6, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 7: launch_rows_u<7, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    default: launch_rows_u<8, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    }
+}
+template <int W>
+static void launch_rows_st(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff,
+                           const int32_t *gfirst, const double *a, const double *x, double *y, double *tb, int tbw,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bsymv_rows_st<W>, dim3((unsigned)((ngroup + W - 1) / W)), dim3(64 * W), 0, s, ngroup, brow, bcf,
+                       boff, gfirst, a, x, y, tb, tbw);
+}
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
                   const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
                   const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, const int32_t *cc_blk,
-                  double *tb, int32_t ugrp, const int32_t *bcol, int64_t nblock, hipStream_t s)
+                  double *tb, int32_t ugrp, const int32_t *bcol, int64_t nblock, int variant, hipStream_t s)
 {
     // rows beyond the parameter groups (Lagrange multipliers) start at zero
     if (nall > nvar) hipLaunchKernelGGL(k_zero, dim3(nblk(nall - nvar)), dim3(TPB), 0, s, nall - nvar, y + nvar);
-    if (ugrp >= 1 && ugrp <= 8 && tb && cc_blk && bcol) {
-        // every group of size ugrp: the block layout is known, descriptors reduce to bcol
-        const dim3 gr((unsigned)((ngroup + RS_WAVES - 1) / RS_WAVES)), bl(64 * RS_WAVES);
-        const int64_t spare = (int64_t)ugrp * nblock;
-        switch (ugrp) {
-        case 1: hipLaunchKernelGGL(k_bsymv_rows_u<1>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
-        case 2: hipLaunchKernelGGL(k_bsymv_rows_u<2>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
-        case 3: hipLaunchKernelGGL(k_bsymv_rows_u<3>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
-        case 4: hipLaunchKernelGGL(k_bsymv_rows_u<4>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
-        case 5: hipLaunchKernelGGL(k_bsymv_rows_u<5>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
-        case 6: hipLaunchKernelGGL(k_bsymv_rows_u<6>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
-        case 7: hipLaunchKernelGGL(k_bsymv_rows_u<7>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
-        default: hipLaunchKernelGGL(k_bsymv_rows_u<8>, gr, bl, 0, s, ngroup, brow, bcol, boff, a, x, y, tb, spare); break;
+    if (maxgrp <= 8 && tb && cc_blk) {
+        // the matrix is read once: rows pass with the transposed block products (tbw values per
+        // block), then their sums over the column lists
+        const bool uni = ugrp >= 1 && ugrp <= 8 && bcol;
+        int kind = variant >= 0 ? variant / 16 : (uni ? 0 : 1), w = variant >= 0 ? variant % 16 : 4;
+        if (kind == 0 && !uni) kind = 1;
+        const int tbw = kind == 0 ? ugrp : maxgrp;
+        if (kind == 0) {
+            const int64_t spare = (int64_t)ugrp * nblock;
+            if (w == 2) launch_rows_u_g<2>(ugrp, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+            else if (w == 8) launch_rows_u_g<8>(ugrp, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+            else launch_rows_u_g<4>(ugrp, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        } else {
+            if (w == 2) launch_rows_st<2>(ngroup, brow, bcf, boff, gfirst, a, x, y, tb, tbw, s);
+            else if (w == 8) launch_rows_st<8>(ngroup, brow, bcf, boff, gfirst, a, x, y, tb, tbw, s);
+            else launch_rows_st<4>(ngroup, brow, bcf, boff, gfirst, a, x, y, tb, tbw, s);
         }
         hipLaunchKernelGGL(k_bsymv_colst, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_blk,
-                           gfirst, tb, ugrp, y);
-        return;
-    }
-    if (maxgrp <= 8 && tb && cc_blk) {
-        // the matrix is read once: rows pass with the transposed block products (maxgrp values
-        // per block), then their sums
-        hipLaunchKernelGGL(k_bsymv_rows_st, dim3((unsigned)((ngroup + RS_WAVES - 1) / RS_WAVES)), dim3(64 * RS_WAVES), 0, s,
-                           ngroup, brow, bcf, boff, gfirst, a, x, y, tb, maxgrp);
-        hipLaunchKernelGGL(k_bsymv_colst, dim3((unsigned)((ngroup + 3) / 4)), dim3(256), 0, s, ngroup, ccol_ptr, cc_blk,
-                           gfirst, tb, maxgrp, y);
+                           gfirst, tb, tbw, y);
         return;
     }
     if (maxgrp <= 8)

@@ -49,6 +49,8 @@ struct mpgpu_ctx {
     std::vector<int64_t> brow, boff;
     std::vector<int32_t> bcol, brg;
     int32_t uniform_grp = 0;              // size of every parameter group when they are all equal, else 0
+    int spmv_variant = -1;                // block product variant chosen by the autotuner (-1: default)
+    std::string tuning;                   // the autotuner's measurements
     bool blocks_on_host = true;           // brow/bcol/brg/boff copied from the device (on first use)
     int64_t nblock = 0, nstored = 0;
     // constraints (CSR over variable indices) and residual bookkeeping
@@ -432,6 +434,63 @@ static int32_t band_info(const int32_t *inder, const int64_t *eq_sl, const uint3
     }
     if (kx <= 0) return 0;
     return (kx - 1) | (kbdrx << 10) | (mside << 20);
+}
+
+// Autotuner of the group-block product: the variants with identical arithmetic (rows-pass kernel
+// and wavefronts per workgroup; bit-identical results, so the choice never changes a solution)
+// are timed on this structure, one warm-up and three timed products each (HIP events on the
+// context's stream; the time does not depend on the values), and the fastest median is kept.
+static void tune_block_symv(mpgpu_ctx *c)
+{
+    c->spmv_variant = -1;
+    c->tuning.clear();
+    int cand[16];
+    const int nc = block_symv_variants(c->uniform_grp, c->plan.max_grp, cand);
+    if (nc <= 1 || c->nblock == 0 || !c->d_tb.p || !c->d_cc_blk.p) return;
+    // MPGPU_SPMV_VARIANT=kind*16+w: a fixed variant instead of the timing (tests)
+    if (const char *fix = std::getenv("MPGPU_SPMV_VARIANT")) {
+        const int v = std::atoi(fix);
+        for (int q = 0; q < nc; q++)
+            if (cand[q] == v) {
+                c->spmv_variant = v;
+                c->tuning = "fixed variant " + std::to_string(v);
+            }
+        return;
+    }
+    hipStream_t s = c->stream;
+    DBuf<double> x, y;
+    x.alloc(c->nall);
+    y.alloc(c->nall);
+    l_zero(c->nall, x.p, s);
+    EventPool ev;
+    double best = 1e30;
+    char buf[96];
+    for (int q = 0; q < nc; q++) {
+        auto run = [&]() {
+            l_block_symv(c->ngroup, c->d_brow.p, c->d_bcf.p, c->d_boff.p, c->d_gfirst.p, c->d_ccol_ptr.p, c->d_cc_off.p,
+                         c->d_cc_fn.p, c->d_mat.p, x.p, y.p, c->nvar, c->nall, c->plan.max_grp, c->d_cc_blk.p,
+                         c->d_tb.p, c->uniform_grp, c->d_bcol.p, c->nblock, cand[q], s);
+        };
+        run();
+        float ms[3];
+        for (int k = 0; k < 3; k++) {
+            hipEvent_t e0 = ev.get(), e1 = ev.get();
+            HIPCHK(hipEventRecord(e0, s));
+            run();
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&ms[k], e0, e1));
+        }
+        std::sort(ms, ms + 3);
+        std::snprintf(buf, sizeof buf, "%s%s/w%d %.3f ms", q ? ", " : "", cand[q] / 16 == 0 ? "rows_u" : "rows_st",
+                      cand[q] % 16, ms[1]);
+        c->tuning += buf;
+        if (ms[1] < best) { best = ms[1]; c->spmv_variant = cand[q]; }
+    }
+    std::snprintf(buf, sizeof buf, "; chosen %s/w%d", c->spmv_variant / 16 == 0 ? "rows_u" : "rows_st",
+                  c->spmv_variant % 16);
+    c->tuning += buf;
+    if (std::getenv("MPGPU_VERBOSE")) std::fprintf(stderr, "block product autotuner: %s\n", c->tuning.c_str());
 }
 
 extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *nvar_out, int32_t *ncon_out)
@@ -1004,6 +1063,8 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     }
     HIPCHK(hipStreamSynchronize(s));
     phase("device upload");
+    if (sparse) tune_block_symv(c);
+    phase("product autotuner");
     c->built = true;
     c->pc.reset();
     c->dfac.reset();
@@ -1024,6 +1085,13 @@ extern "C" int mpgpu_get_labels(const mpgpu_ctx *c, int32_t *labels, int32_t *va
         if (counters) counters[t] = c->counters[t];
         if (group_first_label) group_first_label[t] = c->group_start_label[t];
     }
+    return 0;
+}
+
+extern "C" int mpgpu_get_tuning(const mpgpu_ctx *c, char *text, int32_t len)
+{
+    if (!c || !c->built || !text || len <= 0) return 30;
+    std::snprintf(text, (size_t)len, "%s", c->tuning.c_str());
     return 0;
 }
 
@@ -1403,7 +1471,7 @@ void avprod_dev(mpgpu_ctx *c, const double *x, double *y)
     if (c->opt.matsto == MPGPU_SPARSE) {
         l_block_symv(c->ngroup, c->d_brow.p, c->d_bcf.p, c->d_boff.p, c->d_gfirst.p, c->d_ccol_ptr.p, c->d_cc_off.p,
                      c->d_cc_fn.p, c->d_mat.p, x, y, c->nvar, c->nall, c->plan.max_grp, c->d_cc_blk.p, c->d_tb.p,
-                     c->uniform_grp, c->d_bcol.p, c->nblock, s);
+                     c->uniform_grp, c->d_bcol.p, c->nblock, c->spmv_variant, s);
         l_cons_symv(c->ncon, c->nvar, c->d_cptr.p, c->d_cidx.p, c->d_cval.p, x, y, s);
     } else {
         l_packed_symv(c->nall, c->d_mat.p, x, y, c->d_psymv.p, s);

@@ -150,7 +150,9 @@ void l_mr_update(int64_t n, const MrState *st, const double *v, double *w, doubl
 void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const int64_t *boff, const int32_t *gfirst,
                   const int64_t *ccol_ptr, const int64_t *cc_off, const int32_t *cc_fn, const double *a,
                   const double *x, double *y, int32_t nvar, int64_t nall, int32_t maxgrp, const int32_t *cc_blk,
-                  double *tb, int32_t ugrp, const int32_t *bcol, int64_t nblock, hipStream_t s);
+                  double *tb, int32_t ugrp, const int32_t *bcol, int64_t nblock, int variant, hipStream_t s);
+// variants of the product with identical arithmetic (for the autotuner); returns their number
+int block_symv_variants(int32_t ugrp, int32_t maxgrp, int *out);
 void l_cons_symv(int32_t ncon, int32_t nvar, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *x,
                  double *y, hipStream_t s);
 void l_scale_copy(int64_t n, double sc, const double *a, double *o, hipStream_t s);

@@ -27,7 +27,8 @@ FULL, SPARSE = 1, 2
 EXPORTS = [
     "mpgpu_default_options", "mpgpu_create", "mpgpu_destroy", "mpgpu_last_error", "mpgpu_set_options",
     "mpgpu_add_records", "mpgpu_read_mille_file", "mpgpu_set_steering_parameters", "mpgpu_add_equations",
-    "mpgpu_build_structure", "mpgpu_get_labels", "mpgpu_matrix_info", "mpgpu_get_block_structure", "mpgpu_set_parameters",
+    "mpgpu_build_structure", "mpgpu_get_labels", "mpgpu_matrix_info", "mpgpu_get_block_structure", "mpgpu_get_tuning",
+    "mpgpu_set_parameters",
     "mpgpu_get_parameters", "mpgpu_loopn", "mpgpu_get_vector", "mpgpu_get_matrix_packed", "mpgpu_mupdat",
     "mpgpu_solve", "mpgpu_avprod", "mpgpu_avprod_noalloc", "mpgpu_get_errors", "mpgpu_get_monitor", "mpgpu_run_steering",
     "mpgpu_server_create", "mpgpu_server_submit", "mpgpu_server_wait", "mpgpu_server_destroy",
@@ -105,6 +106,7 @@ def lib():
         L.mpgpu_get_labels.argtypes = [C.c_void_p, I32, I32, I64, I32]
         L.mpgpu_matrix_info.argtypes = [C.c_void_p, I64, I64, I32]
         L.mpgpu_get_block_structure.argtypes = [C.c_void_p, I32, I64, I32, I64]
+        L.mpgpu_get_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
         L.mpgpu_set_parameters.argtypes = [C.c_void_p, D]
         L.mpgpu_get_parameters.argtypes = [C.c_void_p, D]
         L.mpgpu_loopn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.POINTER(PassStats)]
@@ -231,6 +233,12 @@ class Pede:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int32()
         self._chk(self.L.mpgpu_matrix_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return dict(stored=a.value, nblocks=b.value, ngroups=c.value)
+
+    def tuning(self):
+        """the autotuner's measurements and choice for this structure (text)"""
+        buf = C.create_string_buffer(512)
+        self._chk(self.L.mpgpu_get_tuning(self.h, buf, 512))
+        return buf.value.decode()
 
     def block_structure(self):
         """(gfirst, brow, bcol, boff, nstored) of the group-block storage (sparse storage)"""

@@ -129,3 +129,31 @@ def test_malformed_records_report_exit_code_20():
     with pytest.raises(mpgpu.MpgpuError) as e:
         _build(rp, ii5, gg, **opt)
     assert e.value.code == 20
+
+
+def test_block_product_variants_identical(monkeypatch):
+    """The autotuner chooses among product variants with the same arithmetic: every variant
+    gives the bit-identical product (so the timing-dependent choice never changes a result),
+    equal to the dense product of the stored matrix."""
+    rp, ii, gg, _ = mpgpu.pixel_records(41, 4, 6, 3, 4000, 3, 7)
+    x = np.random.default_rng(3).normal(size=1)
+    ys = {}
+    for v in (2, 4, 8, 16 + 2, 16 + 4, 16 + 8, -1):
+        if v >= 0:
+            monkeypatch.setenv("MPGPU_SPMV_VARIANT", str(v))
+        else:
+            monkeypatch.delenv("MPGPU_SPMV_VARIANT", raising=False)
+        P = _build(rp, ii, gg, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=2, mcount=1)
+        P.loopn()
+        if x.size != P.nall:
+            x = np.random.default_rng(3).normal(size=P.nall)
+        ys[v] = P.avprod(x)
+        tun = P.tuning()
+        assert ("fixed variant" in tun) if v >= 0 else ("chosen" in tun)
+        if v == -1:
+            n = P.nall
+            A = packed_to_dense(P.matrix_packed(), n)
+            assert rel_err(ys[v], A @ x) < 1e-12
+        P.close()
+    for v, y in ys.items():
+        assert np.array_equal(y, ys[-1]), v
