@@ -16,6 +16,10 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# one hardware queue per serving worker's stream (8 workers; HIP's default is 4 queues, shared
+# round-robin, so a small job's kernels would wait behind another job's on the same queue);
+# set before the HIP runtime starts
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 sys.path.insert(0, os.path.join(ROOT, "millepede-ii_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 

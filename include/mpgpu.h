@@ -94,6 +94,10 @@ typedef struct mpgpu_options {
                              /* multiplied by it (src/mppar.f90:312-316); default 1, 1   */
     double refine_tol;       /* FP32+IR: stop when ||r|| <= refine_tol ||x|| ||A||_F     */
                              /* (0: eps sqrt(n), the test of LAPACK's DSPOSV)            */
+    int32_t fp32_min_n;      /* FP32+IR: parameter blocks with fewer parameters use the  */
+                             /* FP64 factor (reported as method 3, istop 4): below the   */
+                             /* size where the FP32 factor saves more than the refinement */
+                             /* costs (16384; 0: always try the FP32 factor)              */
 } mpgpu_options;
 
 /* Statistics of one data pass (loopn) */
@@ -118,7 +122,8 @@ typedef struct mpgpu_solve_stats {
     int32_t nrank;       /* rank found (inversion / decomposition: positive pivots)       */
     int32_t istop;       /* MINRES stop code (src/minresModule.f90:591-600); FP32+IR: 1   */
                          /* refinement converged, 3 not within refine_steps (the FP64   */
-                         /* factor then solved the system: method = 3)                  */
+                         /* factor then solved the system: method = 3), 4 block smaller  */
+                         /* than fp32_min_n (solved by the FP64 factor: method = 3)      */
     int32_t itn;         /* MINRES iterations / refinement steps                          */
     double anorm, acond, rnorm, arnorm, ynorm;
     double rel_residual; /* ||A x - b|| / ||b||, evaluated in FP64 on the device           */

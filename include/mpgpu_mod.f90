@@ -19,6 +19,7 @@ MODULE mpgpu_mod
         INTEGER(c_int32_t) :: pair_entries, skip_empty_cons, resolve_redundancy_cons
         REAL(c_double)     :: scale_errors(2)
         REAL(c_double)     :: refine_tol
+        INTEGER(c_int32_t) :: fp32_min_n
     END TYPE mpgpu_options
 
     TYPE, BIND(C) :: mpgpu_pass_stats

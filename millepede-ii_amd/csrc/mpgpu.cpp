@@ -183,6 +183,7 @@ extern "C" void mpgpu_default_options(mpgpu_options *o)
     o->scale_errors[0] = 1.0;
     o->scale_errors[1] = 1.0;
     o->refine_tol = 0.0;
+    o->fp32_min_n = 16384;
 }
 
 extern "C" int mpgpu_create(const mpgpu_options *opt, int32_t device, mpgpu_ctx **out)
@@ -447,6 +448,12 @@ static void tune_block_symv(mpgpu_ctx *c)
     int cand[16];
     const int nc = block_symv_variants(c->uniform_grp, c->plan.max_grp, cand);
     if (nc <= 1 || c->nblock == 0 || !c->d_tb.p || !c->d_cc_blk.p) return;
+    // small matrices (below 16 M stored words, 128 MB) keep the default: the timing would cost
+    // more than it can save (serving jobs)
+    if (c->nstored < ((int64_t)16 << 20) && !std::getenv("MPGPU_SPMV_VARIANT")) {
+        c->tuning = "default variant (matrix below 16 M words)";
+        return;
+    }
     // MPGPU_SPMV_VARIANT=kind*16+w: a fixed variant instead of the timing (tests)
     if (const char *fix = std::getenv("MPGPU_SPMV_VARIANT")) {
         const int v = std::atoi(fix);
@@ -2193,12 +2200,16 @@ static void dense_solve(mpgpu_ctx *c, const double *b_dev, double *x_dev, mpgpu_
     const bool want_cov = c->opt.lapack_errors && lapack;
     c->have_cov = false;
     std::vector<double> covcorr(want_cov ? nv : 0, 0.0);
+    // The FP32 factor with refinement pays only for large blocks: below fp32_min_n parameters the
+    // FP64 factor is the faster one on this GPU (measured: 3-16 ms against 20-140 ms for the
+    // FP32 factor and its refinement steps at 500-5,000 parameters; 1.16x faster at 30,000)
+    const bool by_size = c->opt.metsol == MPGPU_CHOLESKY_FP32_IR && nv < c->opt.fp32_min_n;
     if (!c->dfac || c->dfac->metsol != c->opt.metsol) {
         c->dfac.reset(new DenseFactor());
-        dense_factor(c, *c->dfac, c->opt.metsol == MPGPU_CHOLESKY_FP32_IR);
+        dense_factor(c, *c->dfac, c->opt.metsol == MPGPU_CHOLESKY_FP32_IR && !by_size);
     }
     DenseFactor &F = *c->dfac;
-    if (!F.fp32 && c->opt.metsol == MPGPU_CHOLESKY_FP32_IR) { loc.method = MPGPU_DECOMPOSITION; loc.istop = 3; }
+    if (!F.fp32 && c->opt.metsol == MPGPU_CHOLESKY_FP32_IR) { loc.method = MPGPU_DECOMPOSITION; loc.istop = by_size ? 4 : 3; }
     loc.nrank = F.nrank;
     loc.evmax = F.evmax;
     loc.evmin = F.evmin;

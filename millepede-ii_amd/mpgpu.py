@@ -45,7 +45,7 @@ class Options(C.Structure):
                 ("nregul", C.c_int32), ("refine_steps", C.c_int32), ("monitor", C.c_int32), ("monitor_bins", C.c_int32),
                 ("lapack_errors", C.c_int32), ("pair_entries", C.c_int32), ("skip_empty_cons", C.c_int32),
                 ("resolve_redundancy_cons", C.c_int32), ("scale_errors", C.c_double * 2),
-                ("refine_tol", C.c_double)]
+                ("refine_tol", C.c_double), ("fp32_min_n", C.c_int32)]
 
 
 class PassStats(C.Structure):

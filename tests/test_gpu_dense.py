@@ -82,7 +82,7 @@ def test_decomposition_rank_defect(orc):
 def test_fp32_refinement_runs(orc):
     """Method 11 on the dense problem: the FP32 factor on the matrix cores, FP64 CG refinement
     to the backward-error test of LAPACK's DSPOSV; max |dp|/sigma_p against chdec2/chslv2."""
-    P = _dense_problem(33, metsol=mpgpu.CHOLESKY_FP32_IR)
+    P = _dense_problem(33, metsol=mpgpu.CHOLESKY_FP32_IR, fp32_min_n=0)
     xo, nrank, _, _ = _oracle_chdec2(orc, P)
     x, st = P.solve()
     assert st["method"] == mpgpu.CHOLESKY_FP32_IR and st["istop"] == 1, st
@@ -98,12 +98,24 @@ def test_fp32_refinement_runs(orc):
 def test_fp32_refinement_nonconvergence_falls_back():
     """With one refinement step allowed and a tolerance far below what one step reaches, the
     refinement reports non-convergence (istop 3) and the FP64 factor solves the system."""
-    P = _dense_problem(34, metsol=mpgpu.CHOLESKY_FP32_IR, refine_steps=1, refine_tol=1e-30)
+    P = _dense_problem(34, metsol=mpgpu.CHOLESKY_FP32_IR, refine_steps=1, refine_tol=1e-30, fp32_min_n=0)
     x, st = P.solve()
     assert st["method"] == mpgpu.DECOMPOSITION and st["istop"] == 3 and st["itn"] == 1
     Q = _dense_problem(34, metsol=mpgpu.DECOMPOSITION)
     x64, _ = Q.solve()
     assert rel_err(x, x64) < 1e-8
+
+
+def test_fp32_size_rule():
+    """Below fp32_min_n parameters (default 16384) method 11 solves with the FP64 factor, which
+    is the faster one there; it reports method 3, istop 4, and the decomposition's solution."""
+    P = _dense_problem(36, metsol=mpgpu.CHOLESKY_FP32_IR)
+    assert P.nvar < 16384
+    x, st = P.solve()
+    assert st["method"] == mpgpu.DECOMPOSITION and st["istop"] == 4 and st["itn"] == 0, st
+    P.set_options(metsol=mpgpu.DECOMPOSITION)          # the same matrix, the FP64 method
+    x64, st64 = P.solve()
+    assert st64["method"] == mpgpu.DECOMPOSITION and np.array_equal(x, x64)
 
 
 @pytest.mark.parametrize("n_layers,nx,ny", [(3, 1, 1), (4, 11, 3)])

@@ -443,7 +443,8 @@ def test_inversion_with_multipliers():
 @pytest.mark.parametrize("metsol", [mpgpu.DECOMPOSITION, mpgpu.CHOLESKY_FP32_IR])
 def test_cholesky(metsol):
     rp, ii, gg, cons, gam = _pixel_cons_problem(seed=23, nl=6, nx=8, ny=4, ntr=20000)
-    P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.FULL, metsol=metsol, mreqenf=5, mcount=1)
+    P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.FULL, metsol=metsol, mreqenf=5, mcount=1,
+              fp32_min_n=0)
     P.loopn()
     xr, A, b = _kkt_reference(P)
     x, st = P.solve()
