@@ -98,6 +98,13 @@ typedef struct mpgpu_options {
                              /* FP64 factor (reported as method 3, istop 4): below the   */
                              /* size where the FP32 factor saves more than the refinement */
                              /* costs (16384; 0: always try the FP32 factor)              */
+    int32_t iteren;          /* iterated entries cut ('entries n1 n2 n3': n1*n3,        */
+                             /* 'iterateentries n'; loop1i, src/pede.f90:6164-6265):     */
+                             /* with counting per equation (mcount 0) and iteren above   */
+                             /* mreqenf, a variable parameter with fewer than iteren     */
+                             /* entries stays variable only with at least mreqenf        */
+                             /* entries in equations without parameters fixed by the     */
+                             /* entries cut, else it is fixed (-3); 0 = off              */
 } mpgpu_options;
 
 /* Statistics of one data pass (loopn) */
@@ -131,6 +138,26 @@ typedef struct mpgpu_solve_stats {
     double evmax, evmin; /* decomposition: largest / smallest positive pivot of LDL^T       */
                          /* (chdec2, src/mpnum.f90:917-925)                                */
 } mpgpu_solve_stats;
+
+/* Counts of the structure build, as pede's first and second data loops print them in
+ * millepede.log (loop1, src/pede.f90:6124-6141, counters of prepareRecords,
+ * src/mppar.f90:345-351; loop2, src/pede.f90:6683-6838 and 7271-7320) */
+typedef struct mpgpu_record_summary {
+    int64_t nrec;          /* NREC: records                                                  */
+    int64_t neq;           /* nbEqRead: equations (measurements)                             */
+    int64_t neq_glob;      /* nbEqWithGlobPar: equations with global derivatives             */
+    int64_t nglob_deriv;   /* nbGlobDeriv: global derivatives (words, zeros included)        */
+    int32_t fp64_payload;  /* 1 if any record carries FP64 values (pede's NRECD > 0)         */
+    int32_t ntot;          /* nTotalGlobalPar                                                */
+    int32_t nvar;          /* nVarGlobalPar                                                  */
+    int32_t ngroup_tot;    /* nTotalParGroups: groups over all parameters                    */
+    int32_t ngroup_var;    /* nVarParGroups: groups with variable parameters                 */
+    int32_t ncon;          /* nConstraints: constraints kept (empty / redundant ones dropped) */
+    int32_t max_glob_rec;  /* nGlobalMaxPerRec: most distinct variable parameters in a record */
+    int32_t max_loc_rec;   /* nLocalMaxPerRec: largest local parameter index in a record     */
+    int32_t max_eq_rec;    /* nEqMaxPerRec: most equations in a record                       */
+    int64_t nempty_rec;    /* nEmptyRecords: records without variable parameters             */
+} mpgpu_record_summary;
 
 /* ---- context ---------------------------------------------------------------------------- */
 void mpgpu_default_options(mpgpu_options *opt);
@@ -181,6 +208,8 @@ int mpgpu_get_block_structure(const mpgpu_ctx *ctx, int32_t *gfirst, int64_t *br
  * pede has no tunable kernels): the group-block product variants timed at the structure build,
  * with their median times, and the one chosen; NUL-terminated text of at most len bytes */
 int mpgpu_get_tuning(const mpgpu_ctx *ctx, char *text, int32_t len);
+/* the counts pede's loop1 / loop2 write to millepede.log (after mpgpu_build_structure) */
+int mpgpu_get_record_summary(const mpgpu_ctx *ctx, mpgpu_record_summary *out);
 
 /* ---- state ------------------------------------------------------------------------------- */
 int mpgpu_set_parameters(mpgpu_ctx *ctx, const double *par_by_total_index);   /* globalParameter */

@@ -19,7 +19,7 @@ MODULE mpgpu_mod
         INTEGER(c_int32_t) :: pair_entries, skip_empty_cons, resolve_redundancy_cons
         REAL(c_double)     :: scale_errors(2)
         REAL(c_double)     :: refine_tol
-        INTEGER(c_int32_t) :: fp32_min_n
+        INTEGER(c_int32_t) :: fp32_min_n, iteren
     END TYPE mpgpu_options
 
     TYPE, BIND(C) :: mpgpu_pass_stats
@@ -32,6 +32,13 @@ MODULE mpgpu_mod
         REAL(c_double)     :: ms_gather
         INTEGER(c_int64_t) :: n_gather
     END TYPE mpgpu_pass_stats
+
+    TYPE, BIND(C) :: mpgpu_record_summary
+        INTEGER(c_int64_t) :: nrec, neq, neq_glob, nglob_deriv
+        INTEGER(c_int32_t) :: fp64_payload, ntot, nvar, ngroup_tot, ngroup_var, ncon, &
+                              max_glob_rec, max_loc_rec, max_eq_rec
+        INTEGER(c_int64_t) :: nempty_rec
+    END TYPE mpgpu_record_summary
 
     TYPE, BIND(C) :: mpgpu_solve_stats
         INTEGER(c_int32_t) :: method, nrank, istop, itn
@@ -138,6 +145,11 @@ MODULE mpgpu_mod
             TYPE(c_ptr), VALUE :: ctx
             CHARACTER(KIND=c_char), INTENT(OUT) :: text(*)
             INTEGER(c_int32_t), VALUE :: len
+        END FUNCTION
+        INTEGER(c_int) FUNCTION mpgpu_get_record_summary(ctx, summary) BIND(C, name='mpgpu_get_record_summary')
+            IMPORT :: c_int, c_ptr, mpgpu_record_summary
+            TYPE(c_ptr), VALUE :: ctx
+            TYPE(mpgpu_record_summary), INTENT(OUT) :: summary
         END FUNCTION
         INTEGER(c_int) FUNCTION mpgpu_set_parameters(ctx, par) BIND(C, name='mpgpu_set_parameters')
             IMPORT :: c_int, c_ptr, c_double

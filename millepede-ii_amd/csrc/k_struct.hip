@@ -169,12 +169,18 @@ __global__ void k_decode_fill(int64_t nrec, const int64_t *rec_w, const int32_t 
 // ---------------------------------------------------------------------------------------------
 // label table
 // ---------------------------------------------------------------------------------------------
-__global__ void k_label_range(int64_t neq, const int64_t *eq_sl, const uint32_t *eq_n, const int32_t *I, int32_t *mm)
+// label range, and the equation counters of pede's first data loop (prepareRecords,
+// src/mppar.f90:345-351): equations with global derivatives and the number of global words
+__global__ void k_label_range(int64_t neq, const int64_t *eq_sl, const uint32_t *eq_n, const int32_t *I, int32_t *mm,
+                              unsigned long long *cnt)
 {
     int32_t lo = INT32_MAX, hi = INT32_MIN;
+    unsigned long long ne = 0, nd = 0;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < neq; e += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t n = eq_n[e];
         const int64_t sg = eq_sl[e] + (n & 0xffff) + 1;
+        ne += (n >> 16) > 0;
+        nd += n >> 16;
         for (int j = 1; j <= (int)(n >> 16); j++) {
             const int32_t l = I[sg + j];
             lo = min(lo, l);
@@ -183,9 +189,13 @@ __global__ void k_label_range(int64_t neq, const int64_t *eq_sl, const uint32_t 
     }
     lo = wmin(lo);
     hi = wmax(hi);
+    ne = wsum(ne);
+    nd = wsum(nd);
     if (lane_id() == 0) {
         atomicMin(&mm[0], lo);
         atomicMax(&mm[1], hi);
+        atomicAdd(&cnt[0], ne);
+        atomicAdd(&cnt[1], nd);
     }
 }
 __global__ void k_label_mark(int64_t neq, const int64_t *eq_sl, const uint32_t *eq_n, const int32_t *I, int32_t lmin,
@@ -747,14 +757,21 @@ void st_label_table(const StructIn &in, const std::vector<int32_t> &extra, Label
                     hipStream_t s)
 {
     DBuf<int32_t> mm;
+    DBuf<unsigned long long> cnt;
     const int32_t init[2] = {INT32_MAX, 0};
+    const unsigned long long zero2[2] = {0, 0};
     mm.upload(init, 2, s);
+    cnt.upload(zero2, 2, s);
     if (in.neq > 0)
         hipLaunchKernelGGL(k_label_range, dim3(grid_for(in.neq, 256)), dim3(256), 0, s, in.neq, in.eq_sl, in.eq_n, in.lab,
-                           mm.p);
+                           mm.p, cnt.p);
     int32_t h[2];
+    unsigned long long hc[2];
     mm.download(h, 2, s);
+    cnt.download(hc, 2, s);
     HIPCHK(hipStreamSynchronize(s));
+    map.neq_glob = (int64_t)hc[0];
+    map.nglob_words = (int64_t)hc[1];
     for (int32_t l : extra) {
         h[0] = std::min(h[0], l);
         h[1] = std::max(h[1], l);
@@ -791,6 +808,42 @@ void st_label_table(const StructIn &in, const std::vector<int32_t> &extra, Label
                        map.lmin, lab.p);
     labels.resize(ntot);
     lab.download(labels.data(), ntot, s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// Iterated entries cut (loop1i, src/pede.f90:6196-6255): the global words of every equation
+// none of whose global words belongs to a parameter fixed by the entries cut (low[t] = 1) are
+// counted again, zero derivatives included.  One thread per equation; integer atomics.
+__global__ void k_entries_recount(int64_t neq, const int64_t *eq_sl, const uint32_t *eq_n, const int32_t *I,
+                                  LabelMapD L, const uint8_t *low, unsigned long long *cnt)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < neq; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t n = eq_n[e];
+        const int ng = (int)(n >> 16);
+        const int64_t sg = eq_sl[e] + (n & 0xffff) + 1;
+        bool any_low = false;
+        for (int j = 1; j <= ng; j++) any_low |= low[tix_of(L, I[sg + j])] != 0;
+        if (any_low) continue;
+        for (int j = 1; j <= ng; j++) atomicAdd(&cnt[tix_of(L, I[sg + j])], 1ull);
+    }
+}
+
+void st_entries_recount(const StructIn &in, const LabelMap &map, const std::vector<uint8_t> &low,
+                        std::vector<int64_t> &counts, hipStream_t s)
+{
+    const int64_t ntot = (int64_t)low.size();
+    DBuf<uint8_t> d_low;
+    d_low.upload(low, s);
+    DBuf<unsigned long long> cnt;
+    cnt.alloc(std::max<int64_t>(ntot, 1));
+    cnt.zero(s);
+    const LabelMapD L{map.lmin, map.bits.p, map.wrank.p};
+    if (in.neq > 0)
+        hipLaunchKernelGGL(k_entries_recount, dim3(grid_for(in.neq, 256)), dim3(256), 0, s, in.neq, in.eq_sl, in.eq_n,
+                           in.lab, L, d_low.p, cnt.p);
+    HIPCHK(hipGetLastError());
+    counts.resize(ntot);
+    cnt.download((unsigned long long *)counts.data(), ntot, s);
     HIPCHK(hipStreamSynchronize(s));
 }
 

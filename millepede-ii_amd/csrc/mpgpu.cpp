@@ -51,6 +51,7 @@ struct mpgpu_ctx {
     int32_t uniform_grp = 0;              // size of every parameter group when they are all equal, else 0
     int spmv_variant = -1;                // block product variant chosen by the autotuner (-1: default)
     std::string tuning;                   // the autotuner's measurements
+    mpgpu_record_summary summary{};       // loop1 / loop2 counts for millepede.log
     bool blocks_on_host = true;           // brow/bcol/brg/boff copied from the device (on first use)
     int64_t nblock = 0, nstored = 0;
     // constraints (CSR over variable indices) and residual bookkeeping
@@ -251,6 +252,7 @@ extern "C" int mpgpu_set_options(mpgpu_ctx *c, const mpgpu_options *o)
         else if (o->default_presigma != b.default_presigma || o->regula != b.regula) which = "presigma/regula";
         else if (o->scale_errors[0] != b.scale_errors[0] || o->scale_errors[1] != b.scale_errors[1]) which = "scaleerrors";
         else if (o->pair_entries != b.pair_entries) which = "pairentries";
+        else if (o->iteren != b.iteren) which = "iteren";
         if (which)
             return fail(c, 30, std::string("option ") + which + " is fixed once the structure is built (rebuild the context to change it)");
     }
@@ -564,6 +566,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     // rank of a label among them is its total index ----
     std::vector<int32_t> ngl_rec;
     std::vector<uint8_t> bnd;
+    std::vector<int64_t> recount;   // iterated entries cut: entries in equations without low parameters
     {
         std::vector<int32_t> extra(c->sp_label.begin(), c->sp_label.end());
         for (auto &b : c->cons) extra.insert(extra.end(), b.labels.begin(), b.labels.end());
@@ -571,6 +574,8 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         LabelMap lmap;
         st_label_table(sin, extra, lmap, c->labels, s);
         c->ntot = (int32_t)c->labels.size();
+        c->summary.neq_glob = lmap.neq_glob;
+        c->summary.nglob_deriv = lmap.nglob_words;
         phase("label table");
         // ---- per record: sorted distinct total indices of its global words (gl lists), the
         // position of each global word in them (widx), entry counters and the group boundaries
@@ -581,6 +586,21 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         // equation, or per record with 'countrecords'; derivatives of zero do not count.
         st_record_lists(sin, lmap, c->ntot, c->opt.mcount != 0, c->d_widx, c->d_rec_gl, c->d_gl_tix, ngl_rec, c->counters,
                         bnd, s);
+        // iterated entries cut (loop1i, src/pede.f90:5975-5981, 6164-6265): only with counting
+        // per equation; the parameters fixed by the entries cut are those without a negative
+        // pre-sigma and with fewer than mreqenf entries
+        recount.clear();
+        if (c->opt.mcount == 0 && c->opt.iteren > c->opt.mreqenf) {
+            std::vector<uint8_t> low(c->ntot, 0);
+            std::vector<char> neg(c->ntot, 0);
+            for (size_t k = 0; k < c->sp_label.size(); k++) {
+                const int32_t t = (int32_t)(std::lower_bound(c->labels.begin(), c->labels.end(), c->sp_label[k]) -
+                                            c->labels.begin());
+                neg[t] = c->sp_presig[k] < 0.0;
+            }
+            for (int32_t t = 0; t < c->ntot; t++) low[t] = !neg[t] && c->counters[t] < c->opt.mreqenf;
+            st_entries_recount(sin, lmap, low, recount, s);
+        }
         d_lab.release();
         sin.lab = nullptr;
     }
@@ -702,6 +722,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         if (c->presig_tot[t] < 0.0) c->var_index[t] = -1;
         else if (c->counters[t] < c->opt.mreqenf) c->var_index[t] = -2;
         else if (redundant[t]) c->var_index[t] = -4;
+        else if (!recount.empty() && recount[t] < c->opt.mreqenf && c->counters[t] < c->opt.iteren) c->var_index[t] = -3;
         else { c->var_index[t] = ++nv; c->var_to_tot.push_back(t); }
     }
     c->nvar = nv;
@@ -814,6 +835,30 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     const int32_t *st_nv = rst.data(), *st_nseg = rst.data() + nrec, *st_nalc = rst.data() + 2 * nrec,
                   *st_sup = rst.data() + 3 * nrec, *st_qe = rst.data() + 4 * nrec;
     phase("record stats and group pairs");
+    {
+        // loop1 / loop2 counts for millepede.log.  A label belongs to a parameter group when it
+        // occurs with a non-zero derivative (zero derivatives are skipped for grouping,
+        // src/pede.f90:2219-2222), so the groups over all parameters are the group starts among
+        // labels with entries; labels only named in text files are in none (src/pede.f90:5986-5991).
+        mpgpu_record_summary &sm = c->summary;
+        sm.nrec = nrec;
+        sm.neq = neq;
+        sm.fp64_payload = c->dbl ? 1 : 0;
+        sm.ntot = ntot;
+        sm.nvar = c->nvar;
+        sm.ngroup_var = c->ngroup;
+        sm.ncon = (int32_t)c->con_tot.size();
+        sm.ngroup_tot = 0;
+        for (int32_t t = 0; t < ntot; t++) sm.ngroup_tot += bnd[t] && c->counters[t] > 0;
+        sm.max_glob_rec = sm.max_loc_rec = sm.max_eq_rec = 0;
+        sm.nempty_rec = 0;
+        for (int64_t r = 0; r < nrec; r++) {
+            sm.max_glob_rec = std::max(sm.max_glob_rec, st_nv[r]);
+            sm.max_loc_rec = std::max(sm.max_loc_rec, st_nalc[r]);
+            sm.max_eq_rec = std::max(sm.max_eq_rec, (int32_t)(rec_eq[r + 1] - rec_eq[r]));
+            sm.nempty_rec += st_nv[r] == 0;
+        }
+    }
     int32_t max_gl = 1, max_eq = 1, max_nalc = 1, max_var = 1, max_seg = 1;
     int32_t bmax_gl = 1, bmax_eq = 1, bmax_nalc = 1, bmax_var = 1, bmax_seg = 1, bmax_mp1 = 1, bmax_bdr = 0, bmax_sup = 1;
     int32_t max_qe = 1;   // most equations any record variable occurs in (full local fits)
@@ -1092,6 +1137,14 @@ extern "C" int mpgpu_get_labels(const mpgpu_ctx *c, int32_t *labels, int32_t *va
         if (counters) counters[t] = c->counters[t];
         if (group_first_label) group_first_label[t] = c->group_start_label[t];
     }
+    return 0;
+}
+
+extern "C" int mpgpu_get_record_summary(const mpgpu_ctx *c, mpgpu_record_summary *out)
+{
+    if (!c || !out) return 30;
+    if (!c->built) return 24;
+    *out = c->summary;
     return 0;
 }
 

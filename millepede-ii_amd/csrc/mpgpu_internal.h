@@ -264,6 +264,7 @@ struct StructIn {
 struct LabelMap {
     int32_t lmin = 0;
     int64_t nw = 0;
+    int64_t neq_glob = 0, nglob_words = 0;   // equations with global derivatives, global words
     DBuf<uint32_t> bits;
     DBuf<int64_t> wrank;
 };
@@ -295,6 +296,9 @@ void st_record_lists(const StructIn &in, const LabelMap &map, int32_t ntot, bool
 // per record: st[k * nrec + r], k = variables, segments, largest local index, local x global
 // words, equations per variable, first group; cnt_r: values of its matrix update; sparse: the
 // group pairs it couples into pm
+// iterated entries cut: recount the global words of the equations without a low parameter
+void st_entries_recount(const StructIn &in, const LabelMap &map, const std::vector<uint8_t> &low,
+                        std::vector<int64_t> &counts, hipStream_t s);
 void st_record_stats(const StructIn &in, const DBuf<int32_t> &widx, const DBuf<int64_t> &rec_gl,
                      const DBuf<int32_t> &gl_tix, const int32_t *var, const int32_t *group_of, int32_t ngroup,
                      int32_t max_ngl, bool sparse, PairMap &pm, std::vector<int32_t> &st, std::vector<int64_t> &cnt_r,

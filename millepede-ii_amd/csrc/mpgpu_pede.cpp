@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <ctime>
 #include <fstream>
 #include <functional>
 #include <map>
@@ -20,6 +21,7 @@
 #include <string>
 #include <vector>
 #include <sys/stat.h>
+#include <unistd.h>
 
 namespace {
 
@@ -28,6 +30,8 @@ struct EqLine { double rhs, sigma; std::vector<int32_t> labels; std::vector<doub
 
 struct Steering {
     std::vector<std::pair<std::string, int>> binfiles; // name, 1 = C binary, 3 = Fortran
+    std::vector<std::pair<std::string, int>> files;    // as named in the steering file: 1 C binary, 2 text, 3 Fortran
+    bool scale_given = false;                          // 'scaleerrors' present (scaleErrors = 1)
     std::vector<ParLine> params;
     std::vector<EqLine> cons, meas;
     mpgpu_options opt;
@@ -149,6 +153,7 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
         if (key_match(k, "entries")) {
             if (nums.size() > 0 && nums[0] > 0.5) S.opt.mreqenf = (int)std::llround(nums[0]);
             if (nums.size() > 1 && nums[1] > 0.5) S.opt.mreqena = (int)std::llround(nums[1]);
+            if (nums.size() > 2 && nums[2] > 0.5) S.opt.iteren = (int)std::llround(nums[0] * nums[2]);
             continue;
         }
         if (key_match(k, "chisqcut")) {
@@ -203,8 +208,14 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
         if (key_match(k, "force")) { S.iforce = 1; continue; }
         if (key_match(k, "scaleerrors")) {
             // src/pede.f90:10697-10704: one factor for both kinds, or one each
+            S.scale_given = true;
             if (nums.size() > 0) S.opt.scale_errors[0] = S.opt.scale_errors[1] = nums[0];
             if (nums.size() > 1) S.opt.scale_errors[1] = nums[1];
+            continue;
+        }
+        if (key_match(k, "iterateentries")) {
+            // src/pede.f90:10706-10712
+            S.opt.iteren = nums.empty() ? INT32_MAX : (int)std::llround(nums[0]);
             continue;
         }
         if (key_match(k, "skipemptycons")) { S.opt.skip_empty_cons = 1; continue; }
@@ -240,8 +251,10 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
                 std::string tail = lower(f.size() >= 2 ? f.substr(f.size() - 2) : f);
                 if (tail == "xt" || tail == "tx") {
                     int ft = cur_ftype;
+                    S.files.push_back({f, 2});
                     if (!parse_text(S, full, dir, false, ft)) return false;
                 } else {
+                    S.files.push_back({f, cur_ftype});
                     S.binfiles.push_back({full, cur_ftype});
                 }
                 continue;
@@ -250,8 +263,7 @@ bool parse_text(Steering &S, const std::string &path, const std::string &dir, bo
         }
         {
             // pede keywords that change the results but are not part of this path
-            static const char *changes[] = {"weightedcons", "iterateentries", "checkinput", "postprocessing",
-                                            "maxrecord"};
+            static const char *changes[] = {"weightedcons", "checkinput", "postprocessing", "maxrecord"};
             // pede keywords without effect on the build and solve (threads, caches, printout,
             // monitoring of progress, PARDISO tuning; 'wolfe' sets only the c1 constant, which
             // the line search does not use, src/pede.f90:7389-7406)
@@ -375,6 +387,39 @@ std::string ld_r4(float xf)
     return rjust(b, 17);
 }
 std::string ld_i(long long v, int w = 12) { return rjust(std::to_string(v), w); }
+
+// Fortran Aw output: a shorter string right-justified, a longer one cut to its first w characters
+std::string fa(const std::string &t, int w)
+{
+    return (int)t.size() >= w ? t.substr(0, w) : std::string(w - t.size(), ' ') + t;
+}
+// FORMAT(1X,a8,' =',i14,' = ',a) of the size tables (src/pede.f90:6143, 7527)
+std::string f101(const char *name, long long v, const char *text)
+{
+    return " " + fa(name, 8) + " =" + rjust(std::to_string(v), 14) + " = " + text;
+}
+// the Solution algorithm table, FORMATs 121-124 (src/pede.f90:8679-8682)
+std::string f121(const char *a, const char *b = "") { return " " + fa(a, 40) + "   " + b; }
+std::string f122(const char *a, long long v, const char *b = "") { return " " + fa(a, 40) + "   " + std::to_string(v) + b; }
+std::string f123(const char *a, double v) { return " " + fa(a, 40) + "  " + fmt_e(v, 9, 2); }
+
+// Chi^2/ndf cut for 2 or 3 standard deviations (chindl, src/mpnum.f90:2078-2115): the quantiles
+// of the chi^2 distribution at the 2 and 3 sigma probabilities, divided by ndf, up to ndf 30;
+// beyond, the normal approximation (z + sqrt(2 ndf - 1))^2 / (2 ndf)
+float chi2_ndf_cut(int nsig, int nd)
+{
+    static const float q2[30] = {4.0000f, 3.0900f, 2.6750f, 2.4290f, 2.2628f, 2.1415f, 2.0481f, 1.9736f, 1.9124f, 1.8610f,
+                                 1.8171f, 1.7791f, 1.7457f, 1.7161f, 1.6897f, 1.6658f, 1.6442f, 1.6246f, 1.6065f, 1.5899f,
+                                 1.5745f, 1.5603f, 1.5470f, 1.5346f, 1.5230f, 1.5120f, 1.5017f, 1.4920f, 1.4829f, 1.4742f};
+    static const float q3[30] = {9.0000f, 5.9146f, 4.7184f, 4.0628f, 3.6410f, 3.3436f, 3.1209f, 2.9468f, 2.8063f, 2.6902f,
+                                 2.5922f, 2.5082f, 2.4352f, 2.3711f, 2.3143f, 2.2635f, 2.2178f, 2.1764f, 2.1386f, 2.1040f,
+                                 2.0722f, 2.0428f, 2.0155f, 1.9901f, 1.9665f, 1.9443f, 1.9235f, 1.9040f, 1.8855f, 1.8681f};
+    if (nd < 1) return 0.0f;
+    if (nd <= 30) return nsig == 2 ? q2[nd - 1] : q3[nd - 1];
+    const float z = nsig == 2 ? 1.690140f : 2.782170f;
+    const float t = z + std::sqrt((float)(nd + nd - 1));
+    return t * t / (float)(nd + nd);
+}
 
 // ptline (src/linesrch.f90:89-231) with ptldef (232-257)
 struct LineSearch {
@@ -501,12 +546,14 @@ struct Feasibility {
     const std::vector<EqLine> *cons = nullptr;
     std::vector<int32_t> tot_of_con_term;   // total index of each constraint term (-1: unknown label)
     std::vector<int64_t> term_ptr;
+    int rank = 0;                            // rank of the product matrix A A^T (sum over groups)
 
     // symmetric Gauss-Jordan with diagonal pivoting on a dense n x n matrix (the sqminv
     // algorithm, src/mpnum.f90:97-213): rows without a pivot above 16 eps of their original
-    // diagonal are cleared, so dependent constraints get zero weight
-    static void invert(std::vector<double> &V, int n)
+    // diagonal are cleared, so dependent constraints get zero weight; returns the rank
+    static int invert(std::vector<double> &V, int n)
     {
+        int nrank = 0;
         std::vector<double> diag(n);
         std::vector<char> done(n, 0);
         for (int i = 0; i < n; i++) diag[i] = std::fabs(V[(size_t)i * n + i]);
@@ -521,6 +568,7 @@ struct Feasibility {
                 }
             if (k < 0) break;
             done[k] = 1;
+            nrank++;
             double piv = 1.0 / V[(size_t)k * n + k];
             for (int j = 0; j < n; j++)
                 for (int l = 0; l < n; l++)
@@ -531,6 +579,7 @@ struct Feasibility {
         }
         for (int j = 0; j < n; j++)
             for (int l = 0; l < n; l++) V[(size_t)j * n + l] = (done[j] && done[l]) ? -V[(size_t)j * n + l] : 0.0;
+        return nrank;
     }
 
     void setup(const std::vector<EqLine> &C, const std::vector<int32_t> &labels, const std::vector<int32_t> &var)
@@ -560,6 +609,7 @@ struct Feasibility {
             }
         std::map<int, int> gid;
         groups.clear();
+        rank = 0;
         for (int i = 0; i < nc; i++) {
             auto ins = gid.emplace(root(i), (int)groups.size());
             if (ins.second) groups.emplace_back();
@@ -583,7 +633,7 @@ struct Feasibility {
                 for (auto &x : f)
                     for (auto &y : f) g.inv[(size_t)x.first * n + y.first] += x.second * y.second;
             }
-            invert(g.inv, n);
+            rank += invert(g.inv, n);
         }
     }
 
@@ -737,6 +787,7 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
         return code;
     };
     std::string od = output_dir ? output_dir : ".";
+    const double t_start = now_ms();
     {
         // crash sentinel first, as pede does (src/pede.f90:134)
         FILE *e = std::fopen((od + "/millepede.end").c_str(), "w");
@@ -765,6 +816,39 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
                      S.unsupported[0].c_str());
         return finish(40);
     }
+    // millepede.log: header and file table (src/pede.f90:129-132, 9725-9735), the sizes of the
+    // two structure loops, the solution algorithm, the iteration table (ploopa-ploopd,
+    // src/pede.f90:2891-3040), the line search tables (ptlprt, src/linesrch.f90:294-343), the
+    // final chi^2 summary and the loop counts and times (src/pede.f90:455-480)
+    std::unique_ptr<FILE, int (*)(FILE *)> flog_keep(std::fopen((od + "/millepede.log").c_str(), "w"), std::fclose);
+    FILE *flog = flog_keep.get();
+    auto logl = [&](const std::string &l) {
+        if (flog) std::fprintf(flog, "%s\n", l.c_str());
+    };
+    auto both = [&](const std::string &l) {
+        std::printf("%s\n", l.c_str());
+        logl(l);
+    };
+    {
+        const std::time_t now = std::time(nullptr);
+        char date[32] = "", host[256] = "";
+        std::strftime(date, sizeof date, "%a %b %d %H:%M:%S %Y", std::localtime(&now));
+        if (gethostname(host, sizeof host - 1) != 0) host[0] = '\0';
+        std::string h = host;
+        if (h.size() < 24) h += std::string(24 - h.size(), ' ');
+        logl(" Using mpgpu: pede's global matrix build and global solve on HIP (gfx950)");
+        logl("  ");
+        logl(std::string(" Log-file Millepede II-P                        ") + date);
+        logl("                                                " + h);
+        logl("  ");
+        logl(" Text and data files:");
+        static const char *kind[4] = {"", "C_binary", "text", "Fortran_binary"};
+        for (size_t i = 0; i < S.files.size(); i++) {
+            std::string k = kind[S.files[i].second];
+            logl(ld_i((long long)i + 1, 3) + "  " + k + std::string(14 - k.size(), ' ') + "   " + S.files[i].first);
+        }
+    }
+    const double t_text = now_ms() - t_start;
     if (S.binfiles.empty()) return finish(11);
     mpgpu_ctx *c = nullptr;
     int rc = mpgpu_create(&S.opt, device, &c);
@@ -781,6 +865,7 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
         if (rc) return bail(rc);
         loc.nrec += n;
     }
+    const double t_read = now_ms() - t0;
     std::vector<int32_t> pl;
     std::vector<double> pv, ps;
     for (auto &p : S.params) { pl.push_back(p.label); pv.push_back(p.value); ps.push_back(p.presigma); }
@@ -802,13 +887,112 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
         if (rc) return bail(rc);
     }
     int32_t ntot, nvar, ncon;
+    const double t_build0 = now_ms();
     if ((rc = mpgpu_build_structure(c, &ntot, &nvar, &ncon))) return bail(rc);
     loc.ntot = ntot; loc.nvar = nvar; loc.ncon = ncon;
     loc.ms_structure = now_ms() - t0;
+    const double t_build = now_ms() - t_build0;
     const int nall = nvar + ncon;
     std::vector<int32_t> labels(ntot), var(ntot), grp(ntot);
     std::vector<int64_t> cnt_tot(ntot);
     mpgpu_get_labels(c, labels.data(), var.data(), cnt_tot.data(), grp.data());
+    mpgpu_record_summary sm{};
+    mpgpu_get_record_summary(c, &sm);
+    // pede's parameter counts: with elimination (its default, and always for decomposition,
+    // src/pede.f90:6476-6502) nConstraints fewer fit parameters, else nConstraints multipliers
+    const bool elim_count = !S.opt.withmultipliers || S.opt.metsol == MPGPU_DECOMPOSITION;
+    const int64_t nall_active = elim_count ? (int64_t)sm.nvar : (int64_t)sm.nvar + sm.ncon;
+    const int64_t nfitpar = elim_count ? (int64_t)sm.nvar - sm.ncon : (int64_t)sm.nvar + sm.ncon;
+    {
+        // loop1 (src/pede.f90:5802-6141): labels of the text files, parameter and group counts,
+        // record and equation counters
+        std::vector<int32_t> txt;
+        for (auto &p : S.params) txt.push_back(p.label);
+        for (auto &e : S.cons) txt.insert(txt.end(), e.labels.begin(), e.labels.end());
+        for (auto &e : S.meas) txt.insert(txt.end(), e.labels.begin(), e.labels.end());
+        std::sort(txt.begin(), txt.end());
+        txt.erase(std::unique(txt.begin(), txt.end()), txt.end());
+        long long npresg = 0, nfix3 = 0;
+        for (auto &p : S.params) npresg += p.presigma > 0.0;
+        for (int32_t t = 0; t < ntot; t++) nfix3 += var[t] == -3;
+        const bool iterate = S.opt.iteren > S.opt.mreqenf;
+        logl("  ");
+        logl(" LOOP1: starting");
+        if (!txt.empty()) logl(" LOOP1:" + ld_i((long long)txt.size()) + "  labels from txt data stored");
+        logl(" LOOP1: reading data files");
+        logl(" LOOP1:" + ld_i(sm.ntot) + "  is total number nTotalGlobalPar of labels/parameters");
+        logl(" LOOP1:" + ld_i(npresg) + "  is number of pre-sigmas");
+        logl(" LOOP1:" + ld_i(sm.nvar + nfix3) + "  is number nVarGlobalPar of variable parameters");
+        if (iterate && S.opt.mcount == 0) {
+            logl("  ");
+            logl(" LOOP1: iterating");
+            logl(" LOOP1:" + ld_i(sm.nvar) + "  is number nVarGlobalPar of variable parameters");
+        } else if (iterate) {
+            logl(" LOOP1: counting records, NO iteration of entries cut !");
+        }
+        logl(" LOOP1:" + ld_i(sm.ngroup_tot) + "  is total number nTotalParGroups of label/parameter groups");
+        logl("  ");
+        logl(f101("  NREC", sm.nrec, "number of records"));
+        if (sm.fp64_payload) logl(f101(" NRECD", sm.nrec, "number of records containing doubles"));
+        logl(f101("  nbEqRead", sm.neq, "number of equations (measurements)"));
+        logl(f101("  nbEqWithGlobPar", sm.neq_glob, "number of equations with global parameters"));
+        logl(f101("  nbGlobDeriv", sm.nglob_deriv, "number of global derivatives"));
+        logl(f101("MREQENF", S.opt.mreqenf, S.opt.mcount == 0 ? "required number of entries (eqns in binary files)"
+                                                               : "required number of entries (recs in binary files)"));
+        if (iterate && S.opt.mcount == 0)
+            logl(f101("ITEREN", S.opt.iteren, "iterate cut for parameters with less entries"));
+        logl(f101("MREQENA", S.opt.mreqena, "required number of entries (from accepted fits)"));
+        logl(" LOOP1: ending");
+        logl("  ");
+        // loop2 (src/pede.f90:6456-7493): sizes, solution method, chi^2 cut table
+        logl("  ");
+        logl(" LOOP2: starting");
+        logl(" LOOP2: start event reading");
+        logl(" LOOP2: event reading ended - end of data");
+        logl("  ");
+        logl(f101("nTotalGlobalPar", sm.ntot, "total number of parameters"));
+        logl(std::string(22, ' ') + "(all parameters, appearing in binary files)");
+        logl(f101("nVarGlobalPar", sm.nvar, "number of variable parameters"));
+        logl(std::string(22, ' ') + "(appearing in fit matrix/vectors)");
+        logl(f101("nAllActivePar", nall_active, "number of all parameters"));
+        logl(std::string(22, ' ') + "(including Lagrange multiplier or reduced)");
+        logl(f101("nTotalParGroups", sm.ngroup_tot, "total number of parameter groups"));
+        logl(f101("nVarParGroups", sm.ngroup_var, "number of variable parameter groups"));
+        logl(f101("nFitPar", nfitpar, "number of fit parameters"));
+        if (S.opt.metsol == MPGPU_MINRES || S.opt.metsol == MPGPU_MINRES_QLP) {
+            logl(f101("MBANDW", S.opt.mbandw, "band width of preconditioner matrix"));
+            logl(std::string(22, ' ') + "(if <0, no preconditioner matrix)");
+        }
+        const long long noff = nall_active * (nall_active - 1) / 2;
+        if (nall_active >= 65536) logl(f101("NOFF/K", noff / 1000, "max number of off-diagonal elements"));
+        else logl(f101("NOFF", noff, "max number of off-diagonal elements"));
+        logl(f101("nConstraints", sm.ncon, "number of constraints"));
+        logl(f101("nGlobalMaxPerRec", sm.max_glob_rec, "max number of global parameters in an event"));
+        logl(f101("nLocalMaxPerRec", sm.max_loc_rec, "max number of local parameters in an event"));
+        logl(f101("nEqMaxPerRec", sm.max_eq_rec, "max number of equations in an event"));
+        if (sm.nempty_rec > 0) logl(f101("nEmptyRecords", sm.nempty_rec, "number of records without variable parameters"));
+        logl("  ");
+        logl("  ");
+        logl(" Solution method and matrix-storage mode:");
+        const int ms = S.opt.metsol;
+        char rtol[64];
+        std::snprintf(rtol, sizeof rtol, "%s)", ld_r8(S.opt.mrestl).c_str());
+        if (ms == MPGPU_INVERSION) logl("      METSOL = 1:  matrix inversion");
+        else if (ms == MPGPU_DECOMPOSITION) logl("      METSOL = 3:  decomposition");
+        else if (ms == MPGPU_MINRES) logl(std::string("      METSOL = 4:  MINRES (rtol") + rtol);
+        else if (ms == MPGPU_MINRES_QLP) logl(std::string("      METSOL = 5:  MINRES-QLP (rtol") + rtol);
+        else if (ms == MPGPU_LAPACK_PACKED) logl("      METSOL = 7:  LAPACK factorization");
+        else if (ms == MPGPU_LAPACK_UNPACKED) logl("      METSOL = 8:  LAPACK factorization");
+        else if (ms == MPGPU_CHOLESKY_FP32_IR) logl("      METSOL = 11:  Cholesky, FP32 factor with FP64 refinement");
+        logl("                   with" + ld_i(S.mitera) + "  iterations");
+        if (ms == MPGPU_LAPACK_UNPACKED) logl("      MATSTO = 0:  unpacked symmetric matrix, n*n elements");
+        else if (S.opt.matsto == MPGPU_FULL) logl("      MATSTO = 1:  full symmetric matrix, (n*n+n)/2 elements");
+        else logl("      MATSTO = 2:  sparse matrix (custom)");
+        if (S.dflim != 0.0f) logl(" Convergence assumed, if expected dF <" + fmt_gd(S.dflim, 12, 4));
+        if (sm.ncon > 0)
+            logl(nfitpar < sm.nvar ? " Constraints handled by elimination" : " Constraints handled by Lagrange multipliers");
+    }
+    const double t_loop1 = t_read, t_loop2 = t_build;
     std::vector<double> par(ntot), start(ntot), presig(ntot, 0.0);
     mpgpu_get_parameters(c, par.data());
     start = par;
@@ -831,6 +1015,10 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     std::vector<double> b(nall), x(nall), xls(nvar), corr(nvar);
     std::vector<int64_t> cnt_var(nvar);
     double tp = 0, tsol = 0;
+    // times and counts of function passes, matrix passes with their solution, new solutions
+    // (the func. value / global matrix / new solution lines of millepede.log)
+    double t_cat[3] = {0, 0, 0};
+    long long n_cat[3] = {0, 0, 0}, mnrsit = 0;
     std::unique_ptr<FILE, int (*)(FILE *)> fmon_keep(nullptr, std::fclose);   // closed on every exit
     FILE *fmon = nullptr;
     if (S.imonit != 0) {
@@ -853,6 +1041,9 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     };
     auto pass = [&](int icalcm) -> int {
         nloopn++;
+        const double tp0 = now_ms();
+        struct Acc { double &t; long long &n; double t0; ~Acc() { t += now_ms() - t0; n++; } };
+        Acc acc{icalcm == 1 ? t_cat[1] : t_cat[0], icalcm == 1 ? n_cat[1] : n_cat[0], tp0};
         // cut factor of a new iteration (loopn, src/pede.f90:2580-2595)
         if (iterat != lastit && nloopn > 1) {
             if (iterat == 1) chicut = S.opt.chirem;
@@ -881,21 +1072,128 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     bool have_err = false;
     Feasibility FS;
     FS.setup(S.cons, labels, var);
+    if (sm.ncon > 0) {
+        // feasma (src/pede.f90:1494-1507): rank of the product matrix of the constraints; too
+        // few independent constraints enforce SUBITO unless 'force'
+        both(" Rank of product matrix of constraints is" + ld_i(FS.rank) + "  for" + ld_i(sm.ncon) +
+             "  constraint equations");
+        if (FS.rank < sm.ncon) {
+            both(" Warning: insufficient constraint equations!");
+            if (S.iforce == 0) {
+                S.isubit = 1;
+                both("          --> enforcing SUBITO mode");
+            }
+        }
+    }
+    {
+        // chi^2 cut table (src/pede.f90:7485-7510), LOOP2 end, outlier treatment (214-222) and
+        // the solution algorithm (xloopn, src/pede.f90:8571-8683)
+        logl("  ");
+        logl("    Cut values of Chi^2/Ndf and Chi2,");
+        logl("    corresponding to 2 and 3 standard deviations");
+        logl("    Ndf  Chi^2/Ndf(2)  Chi^2(2)     Chi^2/Ndf(3)  Chi^2(3)");
+        for (int ndf = 0;;) {
+            if (ndf > sm.max_eq_rec) break;
+            if (ndf < 10) ndf += 1;
+            else if (ndf < 20) ndf += 2;
+            else if (ndf < 100) ndf += 5;
+            else if (ndf < 200) ndf += 10;
+            else break;
+            const float c2 = chi2_ndf_cut(2, ndf), c3 = chi2_ndf_cut(3, ndf);
+            logl(ld_i(ndf, 6) + "   " + fmt_f(c2, 9, 3) + fmt_f(c2 * (float)ndf, 12, 1) + "      " + fmt_f(c3, 9, 3) +
+                 fmt_f(c3 * (float)ndf, 12, 1) + "   ");
+        }
+        logl(" LOOP2: ending");
+        logl("  ");
+        if (S.opt.chicut != 0.0f) {
+            logl(" Chi square cut equiv 3 st.dev applied ...");
+            logl("  in  first iteration with factor" + ld_r4(S.opt.chicut));
+            logl("  in second iteration with factor" + ld_r4(S.opt.chirem));
+            logl("  (reduced by sqrt in next iterations)");
+        }
+        if (S.opt.lhuber != 0) {
+            logl(" Down-weighting of outliers in" + ld_i(S.opt.lhuber) + "  iterations");
+            logl(" Cut on downweight fraction" + ld_r4(S.opt.dwcut));
+        }
+        const int ms = S.opt.metsol;
+        logl("  ");
+        logl(" Solution algorithm: ");
+        logl(f121("=================================================== "));
+        if (ms == MPGPU_INVERSION) logl(f121("solution method:", "matrix inversion"));
+        else if (ms == MPGPU_DECOMPOSITION) logl(f121("solution method:", "decomposition"));
+        else if (ms == MPGPU_MINRES) logl(f121("solution method:", "minres (Paige/Saunders)"));
+        else if (ms == MPGPU_MINRES_QLP) {
+            logl(f121("solution method:", "minres-qlp (Choi/Paige/Saunders)"));
+            if (S.opt.mrmode == 1) logl(f121(" ", "   using QR factorization"));
+            else if (S.opt.mrmode == 2) logl(f121(" ", "   using QLP factorization"));
+            else {
+                logl(f121(" ", "   using QR and QLP factorization"));
+                logl(f123("transition condition", S.opt.mrtcnd));
+            }
+        } else if (ms == MPGPU_LAPACK_PACKED || ms == MPGPU_LAPACK_UNPACKED) {
+            const bool indef = nall_active > sm.nvar;
+            logl(f121("solution method:", ms == MPGPU_LAPACK_PACKED
+                                              ? (indef ? "LAPACK factorization (DSPTRF)" : "LAPACK factorization (DPPTRF)")
+                                              : (indef ? "LAPACK factorization (DSYTRF)" : "LAPACK factorization (DPOTRF)")));
+            if (S.opt.lapack_errors) logl(f121(" ", "with error calculation (D??TRI)"));
+        } else if (ms == MPGPU_CHOLESKY_FP32_IR) {
+            logl(f121("solution method:", "Cholesky, FP32 factor, FP64 refinement"));
+        }
+        logl(f123("convergence limit at Delta F=", S.dflim));
+        logl(f122("maximum number of iterations=", S.mitera));
+        if (matrit > 1) logl(f122("matrix recalculation up to ", matrit, ". iteration"));
+        if (ms == MPGPU_MINRES || ms == MPGPU_MINRES_QLP) {
+            logl(f121("matrix storage:", S.opt.matsto == MPGPU_FULL ? "full" : "sparse"));
+            logl(f122("pre-con band-width parameter=", S.opt.mbandw));
+            if (S.opt.mbandw == 0) logl(f121("pre-conditioning:", "default"));
+            else if (S.opt.mbandw < 0) logl(f121("pre-conditioning:", "none!"));
+            else logl(f121("pre-conditioning=", S.opt.lprecm > 0 ? "skyline-matrix (rank preserving)" : "band-matrix"));
+        }
+        long long npresg = 0;
+        for (auto &p : S.params) npresg += p.presigma > 0.0;
+        if (S.opt.default_presigma == 0.0 && npresg == 0) {
+            logl(f121("using pre-sigmas:", "no"));
+        } else {
+            logl(" " + fa("pre-sigmas defined for", 40) + "   " +
+                 fmt_f((float)(100 * npresg) / (float)sm.nvar, 5, 1) + " % of variable parameters");
+            logl(f123("default pre-sigma=", S.opt.default_presigma));
+        }
+        if (S.opt.nregul == 0) logl(f121("regularization:", "no"));
+        else {
+            logl(f121("regularization:", "yes"));
+            logl(f123("regularization factor=", S.opt.regula));
+        }
+        if (S.opt.chicut != 0.0f) {
+            logl(f121("Chi square cut equiv 3 st.dev applied"));
+            logl(f123("... in first iteration with factor", S.opt.chicut));
+            logl(f123("... in second iteration with factor", S.opt.chirem));
+            logl(f121(" (reduced by sqrt in next iterations)"));
+        }
+        if (S.scale_given) {
+            logl(f121("Scaling of measurement errors applied"));
+            logl(f123("... factor for \"global\" measuements", S.opt.scale_errors[0]));
+            logl(f123("... factor for \"local\"  measuements", S.opt.scale_errors[1]));
+        }
+        if (S.opt.lhuber != 0) {
+            logl(f122("Down-weighting of outliers in", S.opt.lhuber, " iterations"));
+            logl(f123("Cut on downweight fraction", S.opt.dwcut));
+        }
+    }
     float concut = 1.0e-12f;                     // first check, then 1e-6 (src/pede.f90:8693-8694)
     const float concu2 = 1.0e-6f;
-    if (S.nofeas == 0 && FS.apply(concut, par, var)) {
-        std::printf(" Checking feasibility of parameters: parameters are made feasible\n");
-        if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
+    if (S.nofeas == 0) {
+        // feasib (src/pede.f90:8723-8741)
+        logl(" Checking feasibility of parameters:");
+        if (FS.apply(concut, par, var)) {
+            std::printf(" Checking feasibility of parameters: parameters are made feasible\n");
+            logl(" Call FEASIB with cut=" + fmt_gd(concut, 10, 3));
+            logl("    parameters are made feasible");
+            if ((rc = mpgpu_set_parameters(c, par.data()))) return bail(rc);
+        } else {
+            logl("    parameters are feasible  (i.e. satisfy constraints)");
+        }
     }
     concut = concu2;
-    // millepede.log: the iteration table (ploopa-ploopd, src/pede.f90:2891-3040), the line
-    // search tables (ptlprt, src/linesrch.f90:294-343) and the final chi^2 summary
-    std::unique_ptr<FILE, int (*)(FILE *)> flog_keep(std::fopen((od + "/millepede.log").c_str(), "w"), std::fclose);
-    FILE *flog = flog_keep.get();
-    auto both = [&](const std::string &l) {
-        std::printf("%s\n", l.c_str());
-        if (flog) std::fprintf(flog, "%s\n", l.c_str());
-    };
     int lsinfo = 0, iitera = 0, istopa = 0;
     float angras = 0.0f;
     double rstart = now_ms();
@@ -967,15 +1265,29 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
             double t = now_ms();
             mpgpu_solve_stats sst{};
             if ((rc = mpgpu_solve(c, b.data(), x.data(), &sst))) return bail(rc);
-            tsol += now_ms() - t;
+            const double dt_sol = now_ms() - t;
+            tsol += dt_sol;
+            if (icalcm == 1) t_cat[1] += dt_sol;
+            else { t_cat[2] += dt_sol; n_cat[2]++; }
+            if (metsol == MPGPU_MINRES || metsol == MPGPU_MINRES_QLP) mnrsit += sst.itn;
             if (metsol == MPGPU_DECOMPOSITION && sst.nrank < nall) {
                 // mchdec: a rank defect of the decomposition enforces SUBITO unless 'force'
                 // (src/pede.f90:8026-8037)
-                std::printf(" Warning: the rank defect of the symmetric %d-by-%d matrix is %d (should be zero).\n", nall,
-                            nall, nall - sst.nrank);
+                char w[160];
+                std::snprintf(w, sizeof w, " Warning: the rank defect of the symmetric%s-by-%s matrix is %s (should be zero).",
+                              ld_i(nall).c_str(), ld_i(nall).c_str(), ld_i(nall - sst.nrank).c_str());
+                both(w);
                 if (S.iforce == 0 && S.isubit == 0) {
                     S.isubit = 1;
-                    std::printf("          --> enforcing SUBITO mode\n");
+                    both("          --> enforcing SUBITO mode");
+                }
+            } else if (icalcm == 1 && metsol != MPGPU_MINRES && metsol != MPGPU_MINRES_QLP) {
+                // a factorization or inversion of full rank (src/pede.f90:7923-7928, 8031-8038;
+                // src/mplapack.f90:143-148)
+                logl(" No rank defect of the symmetric matrix");
+                if (metsol == MPGPU_DECOMPOSITION) {
+                    logl(" " + std::string("   largest  diagonal element (LDLt)") + ld_r8(sst.evmax));
+                    logl(" " + std::string("   smallest diagonal element (LDLt)") + ld_r8(sst.evmin));
                 }
             }
             nloopsol = nloopn;
@@ -1054,8 +1366,6 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     {
         // final chi^2 per degree of freedom (src/pede.f90:9124-9147); pede counts the fit
         // parameters as nvar - ncon with eliminated constraints, nvar + ncon with multipliers
-        const int ncon_steer = ncon > 0 ? ncon : (int)S.cons.size();
-        const int64_t nfitpar = S.opt.withmultipliers ? (int64_t)nvar + ncon : (int64_t)nvar - ncon_steer;
         const double dwmean = ps_st.ndf > 0 ? ps_st.wndf / (double)ps_st.ndf : 1.0;
         const double dratio = fvalue / dwmean / (double)(ps_st.ndf - nfitpar);
         float catio = (float)dratio;
@@ -1108,6 +1418,56 @@ extern "C" int mpgpu_run_steering(const char *steering_file, const char *output_
     fmon = nullptr;
     rc = write_res(od + "/millepede.res", labels, var, par, start, presig, cnt_var, cnt_tot, have_err ? &err_v : nullptr,
                    (have_err && S.igcorr) ? &gcor_v : nullptr, S.ipcntr, S.opt.mreqena);
+    {
+        // loop counts, times, peak device memory and rejects (src/pede.f90:455-489, prtrej 4448-4461)
+        logl("  ");
+        const bool iter3 = S.opt.iteren > S.opt.mreqenf && S.opt.mcount == 0;
+        logl(std::string(iter3 ? " In total 3 +" : " In total 2 +") + ld_i(nloopn) + "  loops through the data files");
+        if (mnrsit > 0) {
+            logl("  ");
+            logl(" In total    " + ld_i(mnrsit) + "  internal MINRES iterations");
+        }
+        auto sec = [](double ms) { return fmt_f(ms / 1000.0, 12, 3); };
+        auto cnt = [](long long n) { return rjust(std::to_string(n) + ".", 4); };   // F4.0
+        logl(" Times [in sec] for     text processing" + sec(t_text));
+        logl("                                  LOOP1" + sec(t_loop1));
+        logl("                                  LOOP2" + sec(t_loop2));
+        logl("   func. value                         " + sec(t_cat[0]) + " *" + cnt(n_cat[0]));
+        logl("   func. value, global matrix, solution" + sec(t_cat[1]) + " *" + cnt(n_cat[1]));
+        logl("                           new solution" + sec(t_cat[2]) + " *" + cnt(n_cat[2]));
+        logl("");
+        const double dt = (now_ms() - t_start) / 1000.0;
+        const long long ntsec = std::llround(dt);
+        const long long nhour = ntsec / 3600, minut = ntsec / 60 - 60 * nhour;
+        const long long nsec = std::llround(dt - 60.0 * (double)(minut + 60 * nhour));
+        logl(" Total time =" + ld_i(ntsec) + "  seconds =" + ld_i(nhour) + "  h" + ld_i(minut) + "  m" + ld_i(nsec) +
+             "  seconds");
+        const std::time_t now = std::time(nullptr);
+        char date[32] = "";
+        std::strftime(date, sizeof date, "%a %b %d %H:%M:%S %Y", std::localtime(&now));
+        logl(std::string(" end                                            ") + date);
+        // peak of the device memory taken from the stream-ordered pool during the run
+        hipMemPool_t pool = nullptr;
+        uint64_t high = 0;
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool &&
+            hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemHigh, &high) == hipSuccess) {
+            logl("  ");
+            logl("      Peak dynamic memory allocation: " + fmt_f(1.0e-9 * (double)high, 11, 6) + " GB");
+        }
+        int64_t nrejt = 0;
+        for (int k = 0; k < 6; k++) nrejt += ps_st.nrej[k];
+        if (nrejt != 0) {
+            static const char *why[6] = {" (local solution contains NaNs)", " (local matrix with rank deficit)",
+                                         " (local matrix with ill condition)", " (local fit with Ndf=0)",
+                                         " (local fit with huge Chi2(Ndf))", " (local fit with large Chi2(Ndf))"};
+            logl("  ");
+            logl(" Data records rejected in last iteration:   ");
+            for (int k = 0; k < 6; k++)
+                if (ps_st.nrej[k] > 0) logl(ld_i(ps_st.nrej[k], 21) + " " + why[k]);
+            logl("  ");
+        }
+    }
     mpgpu_destroy(c);
     if (rc) return finish(rc);
     return finish(0);

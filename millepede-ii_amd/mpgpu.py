@@ -28,7 +28,7 @@ EXPORTS = [
     "mpgpu_default_options", "mpgpu_create", "mpgpu_destroy", "mpgpu_last_error", "mpgpu_set_options",
     "mpgpu_add_records", "mpgpu_read_mille_file", "mpgpu_set_steering_parameters", "mpgpu_add_equations",
     "mpgpu_build_structure", "mpgpu_get_labels", "mpgpu_matrix_info", "mpgpu_get_block_structure", "mpgpu_get_tuning",
-    "mpgpu_set_parameters",
+    "mpgpu_get_record_summary", "mpgpu_set_parameters",
     "mpgpu_get_parameters", "mpgpu_loopn", "mpgpu_get_vector", "mpgpu_get_matrix_packed", "mpgpu_mupdat",
     "mpgpu_solve", "mpgpu_avprod", "mpgpu_avprod_noalloc", "mpgpu_get_errors", "mpgpu_get_monitor", "mpgpu_run_steering",
     "mpgpu_server_create", "mpgpu_server_submit", "mpgpu_server_wait", "mpgpu_server_destroy",
@@ -45,7 +45,7 @@ class Options(C.Structure):
                 ("nregul", C.c_int32), ("refine_steps", C.c_int32), ("monitor", C.c_int32), ("monitor_bins", C.c_int32),
                 ("lapack_errors", C.c_int32), ("pair_entries", C.c_int32), ("skip_empty_cons", C.c_int32),
                 ("resolve_redundancy_cons", C.c_int32), ("scale_errors", C.c_double * 2),
-                ("refine_tol", C.c_double), ("fp32_min_n", C.c_int32)]
+                ("refine_tol", C.c_double), ("fp32_min_n", C.c_int32), ("iteren", C.c_int32)]
 
 
 class PassStats(C.Structure):
@@ -60,6 +60,13 @@ class SolveStats(C.Structure):
                 ("anorm", C.c_double), ("acond", C.c_double), ("rnorm", C.c_double), ("arnorm", C.c_double),
                 ("ynorm", C.c_double), ("rel_residual", C.c_double), ("ms", C.c_double), ("evmax", C.c_double),
                 ("evmin", C.c_double)]
+
+
+class RecordSummary(C.Structure):
+    _fields_ = [("nrec", C.c_int64), ("neq", C.c_int64), ("neq_glob", C.c_int64), ("nglob_deriv", C.c_int64),
+                ("fp64_payload", C.c_int32), ("ntot", C.c_int32), ("nvar", C.c_int32), ("ngroup_tot", C.c_int32),
+                ("ngroup_var", C.c_int32), ("ncon", C.c_int32), ("max_glob_rec", C.c_int32),
+                ("max_loc_rec", C.c_int32), ("max_eq_rec", C.c_int32), ("nempty_rec", C.c_int64)]
 
 
 class RunStats(C.Structure):
@@ -107,6 +114,7 @@ def lib():
         L.mpgpu_matrix_info.argtypes = [C.c_void_p, I64, I64, I32]
         L.mpgpu_get_block_structure.argtypes = [C.c_void_p, I32, I64, I32, I64]
         L.mpgpu_get_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+        L.mpgpu_get_record_summary.argtypes = [C.c_void_p, C.POINTER(RecordSummary)]
         L.mpgpu_set_parameters.argtypes = [C.c_void_p, D]
         L.mpgpu_get_parameters.argtypes = [C.c_void_p, D]
         L.mpgpu_loopn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.POINTER(PassStats)]
@@ -239,6 +247,12 @@ class Pede:
         buf = C.create_string_buffer(512)
         self._chk(self.L.mpgpu_get_tuning(self.h, buf, 512))
         return buf.value.decode()
+
+    def record_summary(self):
+        """the counts of pede's loop1 / loop2 (millepede.log) for the built structure"""
+        r = RecordSummary()
+        self._chk(self.L.mpgpu_get_record_summary(self.h, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in RecordSummary._fields_}
 
     def block_structure(self):
         """(gfirst, brow, bcol, boff, nstored) of the group-block storage (sparse storage)"""

@@ -44,9 +44,15 @@ def decode(rec_ptr, inder, glder):
     return recs
 
 
-def label_maps(rec_ptr, inder, glder, steer_labels=(), presigma=None, mreqenf=25, mcount=0, extra_labels=()):
-    """labels (sorted), var_index (>0 variable, -1 fixed by pre-sigma, -2 entries cut),
-    counters, group start label per label."""
+def label_maps(rec_ptr, inder, glder, steer_labels=(), presigma=None, mreqenf=25, mcount=0, extra_labels=(),
+               iteren=0):
+    """labels (sorted), var_index (>0 variable, -1 fixed by pre-sigma, -2 entries cut, -3
+    iterated entries cut), counters, group start label per label.
+
+    Iterated entries cut (loop1i, src/pede.f90:6164-6265; only with counting per equation and
+    iteren > mreqenf): the global words of every equation without a parameter fixed by the
+    entries cut are counted again (zero derivatives included); a variable parameter with fewer
+    than iteren entries and fewer than mreqenf such counts is fixed (-3)."""
     recs = decode(rec_ptr, inder, glder)
     labs = set(int(x) for x in steer_labels) | set(int(x) for x in extra_labels)
     for eqs in recs:
@@ -99,6 +105,23 @@ def label_maps(rec_ptr, inder, glder, steer_labels=(), presigma=None, mreqenf=25
         else:
             k += 1
             var[t] = k
+    if mcount == 0 and iteren > mreqenf:
+        low = var == -2
+        newc = np.zeros(n, np.int64)
+        for eqs in recs:
+            for (sL, sG, lG) in eqs:
+                words = [pos[int(inder[w])] for w in range(sG + 1, lG + 1)]
+                if not any(low[t] for t in words):
+                    for t in words:
+                        newc[t] += 1
+        k = 0
+        for t in range(n):
+            if var[t] > 0:
+                if newc[t] >= mreqenf or cnt[t] >= iteren:
+                    k += 1
+                    var[t] = k
+                else:
+                    var[t] = -3
     gsl = np.zeros(n, np.int32)
     start = 0
     for t in range(n):
@@ -106,3 +129,28 @@ def label_maps(rec_ptr, inder, glder, steer_labels=(), presigma=None, mreqenf=25
             start = t
         gsl[t] = labels[start]
     return labels, var, cnt, gsl
+
+
+def record_summary(rec_ptr, inder, glder, labels, var):
+    """The counts pede's first and second data loops write to millepede.log: equations, those
+    with global derivatives and their global words (prepareRecords, src/mppar.f90:345-351), and
+    per record the equations, the largest local index and the distinct variable parameters of
+    all global words (loop2, src/pede.f90:6683-6838)."""
+    recs = decode(rec_ptr, inder, glder)
+    pos = {int(l): i for i, l in enumerate(labels)}
+    out = dict(neq=0, neq_glob=0, nglob_deriv=0, max_eq_rec=0, max_loc_rec=0, max_glob_rec=0, nempty_rec=0)
+    for eqs in recs:
+        out["neq"] += len(eqs)
+        out["max_eq_rec"] = max(out["max_eq_rec"], len(eqs))
+        vs = set()
+        for (sL, sG, lG) in eqs:
+            if lG > sG:
+                out["neq_glob"] += 1
+                out["nglob_deriv"] += lG - sG
+            for w in range(sL + 1, sG):
+                out["max_loc_rec"] = max(out["max_loc_rec"], int(inder[w]))
+            vs.update(int(var[pos[int(inder[w])]]) for w in range(sG + 1, lG + 1))
+        nv = len([v for v in vs if v > 0])
+        out["max_glob_rec"] = max(out["max_glob_rec"], nv)
+        out["nempty_rec"] += nv == 0
+    return out

@@ -31,7 +31,7 @@ GEN = os.path.join(ROOT, "oracle", "_ref", "mptest1_gen")
 GOLD = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.res")
 GOLD_MON = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.mon")
 GOLD_END = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede.end")
-GOLD_LOG = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede_iterations.txt")
+GOLD_LOG = os.path.join(ROOT, "tests", "golden", "ci_wirechamber_millepede_log.txt")
 
 STEER = """fortranfiles
 mp2con.txt
@@ -113,6 +113,25 @@ def compare_log(log, gold):
             elif table:
                 tol += 1e-6
             assert abs(float(x) - fy) <= tol, (x, y, la, lb)
+
+
+def compare_log_sizes(log, gold):
+    """The structure and setup sections of millepede.log against the CI golden, line for line
+    (trailing blanks aside): loop1's counts of labels, parameters, groups, records, equations
+    and global derivatives; loop2's size table, solution method and storage, the rank of the
+    constraints, the chi^2 cut table; the outlier and solution-algorithm tables; the feasibility
+    check and the rank of the first factorization -- everything from 'LOOP1: starting' up to the
+    iteration table except pede's read cache (NCACHE, cache splitting), which this path does not
+    have.  Then the loop count after the chi^2 summary."""
+    def section(lines):
+        i = next(k for k, l in enumerate(lines) if l.startswith(" LOOP1: starting"))
+        j = next(k for k, l in enumerate(lines) if l.startswith(" it fc   fcn_value"))
+        return [l.rstrip() for l in lines[i:j] if not l.startswith("   NCACHE") and "cache splitting" not in l]
+    a, b = section(log), section(gold)
+    diff = [(x, y) for x, y in zip(a, b) if x != y]
+    assert len(a) == len(b) and not diff, (len(a), len(b), diff[:5])
+    loops = [l for l in log if l.startswith(" In total 2 +")]
+    assert loops == [l for l in gold if l.startswith(" In total 2 +")]
 
 
 @pytest.mark.skipif(not os.path.exists(GEN), reason="reference generator not built (oracle/_ref)")
@@ -243,6 +262,7 @@ def test_ci_wirechamber_verbatim_steering():
     diff = [(a, b) for a, b in zip(mon, gold_mon) if a != b]
     assert len(mon) == len(gold_mon) and not diff, diff[:5]
     compare_log(log, open(GOLD_LOG).read().splitlines())
+    compare_log_sizes(log, open(GOLD_LOG).read().splitlines())
     rel = abs(st["final_chi2"] - 8130954.4875403922) / 8130954.4875403922
     print(f"final chi^2 {st['final_chi2']!r}: relative difference to the CI log {rel:.2e}")
     assert rel <= 1e-9

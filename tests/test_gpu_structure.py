@@ -157,3 +157,48 @@ def test_block_product_variants_identical(monkeypatch):
         P.close()
     for v, y in ys.items():
         assert np.array_equal(y, ys[-1]), v
+
+
+@pytest.mark.parametrize("mcount", [0, 1])
+def test_record_summary_matches_restatement(mcount):
+    """The loop1 / loop2 counts of millepede.log (equations, equations with globals, global
+    words, most equations / largest local index / most variable parameters per record, records
+    without variable parameters, groups over all and over variable parameters) against the
+    checker's restatement, with special data and a zero derivative in the records."""
+    rp, ii, gg, _ = mpgpu.pixel_records(31, 4, 5, 3, 3000, 3, 7)
+    sp, si, sg = _with_special(rp, ii, gg)
+    sg = sg.copy()
+    eq0 = LB.decode(sp, si, sg)[4][0]
+    sg[eq0[2]] = 0.0                          # a zero global derivative (still a global word)
+    P = _build(sp, si, sg, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=5, mcount=mcount)
+    lab, var, cnt, gsl = P.labels()
+    got = P.record_summary()
+    exp = LB.record_summary(sp, si, sg, lab, var)
+    for k, v in exp.items():
+        assert got[k] == v, (k, got[k], v)
+    assert got["nrec"] == len(sp) - 1 and got["ntot"] == len(lab) and got["nvar"] == P.nvar
+    assert got["fp64_payload"] == 0
+    assert got["ngroup_tot"] == len(set(gsl[cnt > 0].tolist()))
+    assert got["ngroup_var"] == len(set(gsl[var > 0].tolist())) == len(P.block_structure()[0]) - 1
+    P.close()
+
+
+def test_iterated_entries_cut():
+    """'entries n1 n2 n3' with counting per equation (loop1i, src/pede.f90:6164-6265): the case
+    whose outcome is fixed by construction (tests/test_oracle_labels.py), then a pixel detector
+    with a cut that fixes modules: every flag equals the checker's restatement."""
+    import test_oracle_labels as TL
+    rp, ii, gg = TL._eq_records(TL.ITEREN_EQS)
+    P = _build(rp, ii, gg.astype(np.float32), matsto=mpgpu.FULL, metsol=mpgpu.INVERSION, mreqenf=2, mcount=0, iteren=3)
+    lab, var = P.labels()[:2]
+    assert {int(l): (1 if v > 0 else int(v)) for l, v in zip(lab, var)} == TL.ITEREN_EXPECT
+    P.close()
+    rp, ii, gg, _ = mpgpu.pixel_records(37, 4, 6, 4, 1500, 3, 6)
+    for mreqenf, iteren in ((40, 80), (60, 1000)):
+        P = _build(rp, ii, gg, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mreqenf=mreqenf, mcount=0, iteren=iteren)
+        lab, var, cnt, gsl = P.labels()
+        elab, evar, ecnt, egsl = LB.label_maps(rp, ii, gg, mreqenf=mreqenf, mcount=0, iteren=iteren)
+        assert np.array_equal(lab, elab) and np.array_equal(cnt, ecnt)
+        assert np.array_equal(var, evar), np.nonzero(var != evar)
+        assert (var == -3).any() and (var == -2).any() and (var > 0).any(), (mreqenf, iteren)
+        P.close()

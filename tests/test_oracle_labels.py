@@ -157,3 +157,39 @@ def test_partition_is_coarsest_union_of_groups():
         for a in range(len(parts) - 1):     # merging any two neighbours breaks a list
             merged = parts[:a] + [parts[a] + parts[a + 1]] + parts[a + 2:]
             assert not unions(merged), (lists, parts, a)
+
+
+def _eq_records(eqs):
+    """One record per equation: measurement 1.0, local 1 (1.0), sigma 1.0, then the global
+    (label, derivative) pairs."""
+    rec_ptr, I, G = [0], [], []
+    for glob in eqs:
+        I += [0, 0, 1, 0] + [l for l, _ in glob]
+        G += [0.0, 1.0, 1.0, 1.0] + [d for _, d in glob]
+        rec_ptr.append(len(I))
+    return np.array(rec_ptr), np.array(I, np.int32), np.array(G)
+
+
+# Equations built so that the iterated entries cut (loop1i, src/pede.f90:6164-6265) has a known
+# outcome with mreqenf 2, iteren 3 (labels A=1 B=2 C=3 D=4 F=6 K=11, L1..L5=21..25):
+#   B and L1..L5 have one entry: fixed by the entries cut (-2);
+#   A has 2 entries, one of them next to B: recounted once, 1 < 2 and 2 < iteren -> -3;
+#   F has 3 entries, all next to a fixed label: recounted 0 times, but 3 >= iteren -> variable;
+#   K has 2 nonzero entries next to fixed labels and 2 zero derivatives in clean equations:
+#   zero derivatives are not entries but are recounted (2 >= 2) -> variable.
+ITEREN_EQS = [[(1, 1.0), (2, 1.0)], [(1, 1.0), (3, 1.0)], [(3, 1.0), (4, 1.0)], [(3, 1.0), (4, 1.0)],
+              [(6, 1.0), (21, 1.0)], [(6, 1.0), (22, 1.0)], [(6, 1.0), (23, 1.0)], [(11, 1.0), (24, 1.0)],
+              [(11, 1.0), (25, 1.0)], [(11, 0.0), (3, 1.0)], [(11, 0.0), (4, 1.0)]]
+ITEREN_EXPECT = {1: -3, 2: -2, 3: 1, 4: 1, 6: 1, 11: 1, 21: -2, 22: -2, 23: -2, 24: -2, 25: -2}
+
+
+def test_iterated_entries_cut_known_case():
+    rp, ii, gg = _eq_records(ITEREN_EQS)
+    labels, var, cnt, _ = LB.label_maps(rp, ii, gg, mreqenf=2, mcount=0, iteren=3)
+    got = {int(l): (1 if v > 0 else int(v)) for l, v in zip(labels, var)}
+    assert got == ITEREN_EXPECT
+    assert [int(c) for c in cnt] == [2, 1, 4, 3, 3, 2, 1, 1, 1, 1, 1]
+    # no iteration with counting per record, or with iteren <= mreqenf: A stays variable
+    for kw in (dict(mcount=1, iteren=3), dict(mcount=0, iteren=2)):
+        labels, var, _, _ = LB.label_maps(rp, ii, gg, mreqenf=2, **kw)
+        assert var[list(labels).index(1)] > 0
