@@ -389,7 +389,7 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             gp[k] = M.par[t];
             const int v = M.var[t];
             gv[k] = v;
-            gg[k] = v > 0 ? M.group_of[v] : -1;
+            gg[k] = M.grp_tot[t];
             const unsigned long long m = __ballot(v > 0);
             if (lane == 0) wcnt[k >> 6] = __popcll(m);
         }
@@ -995,12 +995,13 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     }
                 }
                 __syncthreads();
-                // row descriptors; the upper part of the diagonal blocks is zero
+                // row descriptors (update-buffer offset of the row, column stride, segment, first
+                // row of the segment); the upper part of the diagonal blocks is zero
                 const bool anycommon = sh[11] != 0;
                 for (int a = tid; a < nalg; a += RT) {
                     const int q = segof[a], a0 = segst[q], na = segst[q + 1] - a0;
                     const int base = segro[q] + (a - a0);
-                    rdesc[a] = make_int4(base, na, q, 0);
+                    rdesc[a] = make_int4(base, na, q, a0);
                     vsegm[a] = segm[q];
                     for (int b = a + 1; b < a0 + na; b++) dst[base + (int64_t)b * na] = 0.0;
                 }
@@ -1023,11 +1024,9 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 // the product is formed transposed (rows of G as the first operand): lane l holds
                 // row a = 16 ti + l%16 and four columns b, so the sixteen lanes of one store write
                 // a column run of consecutive rows, contiguous within a segment
+                int ti = 0, tj = wv;                  // tile tt = ti (ti + 1) / 2 + tj, tj <= ti
+                while (tj > ti) { tj -= ti + 1; ti++; }
                 for (int tt = wv; tt < nt * (nt + 1) / 2; tt += RW) {
-                    int ti = (int)((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-                    while (ti * (ti + 1) / 2 > tt) ti--;
-                    while ((ti + 1) * (ti + 2) / 2 <= tt) ti++;
-                    const int tj = tt - ti * (ti + 1) / 2;
                     const int a = 16 * ti + lr, rb = 16 * tj + lr;
                     const bool arow = a < nalg;
                     const int4 da = rdesc[arow ? a : nalg - 1];
@@ -1042,14 +1041,15 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     for (int e = 0; e < 4; e++) {
                         const int b = 16 * tj + lk + 4 * e;   // result element e: (row a, column b)
                         const int bc = b < nalg ? b : nalg - 1;
-                        const int sb = segof[bc];
-                        if (arow && b <= a && sb != da.z) {   // diagonal blocks: above
+                        if (arow && b < da.w) {               // columns before a's segment (diagonal blocks: above)
                             double c = 0.0;
                             if (anycommon && (vsegm[arow ? a : nalg - 1] & vsegm[bc]) != 0ull)
                                 c = dc1<T>(a, b, perq, QE, qn, qe, qg, lw, dw1, les, len, li, lv, neq, gpos);
                             dst[da.x + (int64_t)b * da.y] = c - acc[e];
                         }
                     }
+                    tj += RW;
+                    while (tj > ti) { tj -= ti + 1; ti++; }
                 }
             }
         }

@@ -82,6 +82,7 @@ struct mpgpu_ctx {
     DBuf<int32_t> d_widx, d_gl_tix, d_pair_blk;
     DBuf<double> d_par, d_mat, d_vec, d_pre_weight;
     DBuf<int32_t> d_var, d_group_of, d_gfirst, d_bcol, d_brg;
+    DBuf<int32_t> d_grp_tot;        // total index -> group of the variable (-1: fixed)
     DBuf<int64_t> d_brow, d_boff, d_ccol_ptr, d_cc_off;
     DBuf<int32_t> d_bcf, d_cc_fn;   // per block / per column-list entry: first index and size of the other group
     DBuf<int32_t> d_cc_blk;         // per column-list entry: the block
@@ -822,6 +823,12 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->d_group_of.upload(c->group_of, s);
     c->d_gfirst.upload(c->gfirst, s);
     c->d_var.upload(c->var_index, s);
+    {
+        std::vector<int32_t> gt(ntot, -1);
+        for (int32_t t = 0; t < ntot; t++)
+            if (c->var_index[t] > 0) gt[t] = c->group_of[c->var_index[t]];
+        c->d_grp_tot.upload(gt, s);
+    }
     // per record on the device: segments (parameter groups of its variables), local-fit sizes,
     // update size, first group; sparse storage: the group pairs it couples
     std::vector<int32_t> rst;
@@ -1224,6 +1231,7 @@ static DevModel model_of(mpgpu_ctx *c)
     M.sparse = c->opt.matsto == MPGPU_SPARSE;
     M.mat = c->d_mat.p;
     M.group_of = c->d_group_of.p;
+    M.grp_tot = c->d_grp_tot.p;
     M.gfirst = c->d_gfirst.p;
     M.brow = c->d_brow.p;
     M.bcol = c->d_bcol.p;

@@ -41,6 +41,7 @@ struct DevModel {
     int32_t sparse;           // 0: packed full, 1: group-block sparse
     double *mat;              // packed (nall(nall+1)/2) or block values
     const int32_t *group_of;  // [nvar+1] variable index (1-based) -> group (0-based)
+    const int32_t *grp_tot;   // [ntot] total index -> group of the variable (-1: fixed)
     const int32_t *gfirst;    // [ngroup+1] first variable index (1-based) of each group
     const int64_t *brow;      // [ngroup+1] block-row pointers (lower triangle incl. diagonal)
     const int32_t *bcol;      // [nblock] column group of each block (ascending in a row)
