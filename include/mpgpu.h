@@ -171,11 +171,14 @@ int mpgpu_set_options(mpgpu_ctx *ctx, const mpgpu_options *opt);
  * Records are appended in order.  Record r occupies words [rec_ptr[r], rec_ptr[r+1]) of
  * inder/glder exactly as they are stored in a Mille record (word 1 = (0.0, error count),
  * src/mille.f90:27-59).  Exactly one of glder_f (FP32 payload) / glder_d (FP64) is non-NULL.
- * Labels in global words stay raw labels. */
+ * Labels in global words stay raw labels.  The records are in device memory when the call
+ * returns (large inputs through pinned staging buffers); the caller's arrays may be reused. */
 int mpgpu_add_records(mpgpu_ctx *ctx, int64_t nrec, const int64_t *rec_ptr, const int32_t *inder,
                       const float *glder_f, const double *glder_d);
-/* Read a whole Mille binary file (C binary, or Fortran sequential when fortran != 0; gzip
- * and ROOT files are not supported) and append its records. */
+/* Read a whole Mille binary file (C binary, or Fortran sequential when fortran != 0; plain or
+ * gzip-compressed; ROOT files are not supported) and append its records: each record is read
+ * into a pinned staging buffer, and a full buffer is copied to the device while the next one
+ * fills.  On an error (18) the file's records are dropped and earlier ones stay. */
 int mpgpu_read_mille_file(mpgpu_ctx *ctx, const char *path, int32_t fortran, int64_t *nrec_out);
 
 /* ---- parameters from steering (Parameter lists, src/pede.f90:5925-5950) --------------------

@@ -811,6 +811,53 @@ void st_label_table(const StructIn &in, const std::vector<int32_t> &extra, Label
     HIPCHK(hipStreamSynchronize(s));
 }
 
+// FP32 record values widened to FP64 on the device (an input that mixes FP32 and FP64 records)
+__global__ void k_widen(int64_t n, const float *src, double *dst)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (double)src[i];
+}
+void st_widen(const float *src, double *dst, int64_t n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_widen, dim3(grid_for(n, 256)), dim3(256), 0, s, n, src, dst);
+    HIPCHK(hipGetLastError());
+}
+
+// Residual monitoring: per equation the label of its first global word (0: none) and its
+// measurement error, in equation order
+template <typename T>
+__global__ void k_eq_first(int64_t neq, const int64_t *eq_sl, const uint32_t *eq_n, const int32_t *I, const T *V,
+                           int32_t *lab, double *err)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < neq; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t n = eq_n[e];
+        const int64_t sg = eq_sl[e] + (n & 0xffff) + 1;
+        lab[e] = (n >> 16) > 0 ? I[sg + 1] : 0;
+        err[e] = (double)V[sg];
+    }
+}
+void st_eq_first(const StructIn &in, std::vector<int32_t> &lab, std::vector<double> &err, hipStream_t s)
+{
+    DBuf<int32_t> dl;
+    DBuf<double> de;
+    dl.alloc(std::max<int64_t>(in.neq, 1));
+    de.alloc(std::max<int64_t>(in.neq, 1));
+    if (in.neq > 0) {
+        if (in.vd)
+            hipLaunchKernelGGL(k_eq_first<double>, dim3(grid_for(in.neq, 256)), dim3(256), 0, s, in.neq, in.eq_sl, in.eq_n,
+                               in.lab, in.vd, dl.p, de.p);
+        else
+            hipLaunchKernelGGL(k_eq_first<float>, dim3(grid_for(in.neq, 256)), dim3(256), 0, s, in.neq, in.eq_sl, in.eq_n,
+                               in.lab, in.vf, dl.p, de.p);
+    }
+    HIPCHK(hipGetLastError());
+    lab.resize(in.neq);
+    err.resize(in.neq);
+    dl.download(lab.data(), in.neq, s);
+    de.download(err.data(), in.neq, s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 // Iterated entries cut (loop1i, src/pede.f90:6196-6255): the global words of every equation
 // none of whose global words belongs to a parameter fixed by the entries cut (low[t] = 1) are
 // counted again, zero derivatives included.  One thread per equation; integer atomics.

@@ -14,11 +14,42 @@
 #include <numeric>
 #include <omp.h>
 #include <zlib.h>
+#include <sys/stat.h>
 
 using namespace mpgpu;
 
 namespace {
 struct Precond;
+
+// Two pinned host buffers through which records travel to the device: the reader fills one
+// while the copy of the other runs.  Each holds CHUNK words and their values (FP64-sized).
+struct Staging {
+    static constexpr int64_t CHUNK = (int64_t)4 << 20;
+    char *pin[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool inflight[2] = {false, false};
+    int k = 0;
+    int64_t n = 0;     // words in buffer k
+    void init()
+    {
+        for (int i = 0; i < 2; i++) {
+            HIPCHK(hipHostMalloc((void **)&pin[i], (size_t)CHUNK * 12, hipHostMallocDefault));
+            HIPCHK(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+        }
+    }
+    ~Staging()
+    {
+        for (int i = 0; i < 2; i++) {
+            if (done[i]) {
+                if (inflight[i]) (void)hipEventSynchronize(done[i]);
+                (void)hipEventDestroy(done[i]);
+            }
+            if (pin[i]) (void)hipHostFree(pin[i]);
+        }
+    }
+    int32_t *words() { return (int32_t *)pin[k]; }
+    char *vals() { return pin[k] + CHUNK * 4; }
+};
 }
 
 struct mpgpu_ctx {
@@ -27,12 +58,14 @@ struct mpgpu_ctx {
     hipStream_t stream = nullptr;
     std::string err;
 
-    // ---- host copy of the input records ----
+    // ---- input records: record pointers on the host; words and values go to the device as
+    // they are added (d_in, and d_vf or d_vd below), through pinned staging buffers for files
+    // and large inputs ----
     std::vector<int64_t> rec_ptr{0};
-    std::vector<int32_t> inder;
-    std::vector<float> vf;
-    std::vector<double> vd;
-    bool dbl = false;
+    DBuf<int32_t> d_in;        // words as read (global words hold labels); capacity d_in.n
+    int64_t nwords = 0;        // words on the device or in flight (staged ones not included)
+    bool dbl = false;          // values held as FP64
+    std::unique_ptr<Staging> stage;
 
     // ---- steering ----
     std::vector<int32_t> sp_label;
@@ -261,34 +294,129 @@ extern "C" int mpgpu_set_options(mpgpu_ctx *c, const mpgpu_options *o)
     return 0;
 }
 
+// ---- record ingest ----------------------------------------------------------------------------
+// Device arrays that grow keep their contents (stream-ordered copy; the old buffer is freed after
+// it on the same stream).
+template <typename T> static void grow_dev(DBuf<T> &b, int64_t used, int64_t cap, hipStream_t s)
+{
+    DBuf<T> nb;
+    nb.alloc((size_t)cap);
+    if (used > 0) HIPCHK(hipMemcpyAsync(nb.p, b.p, sizeof(T) * used, hipMemcpyDeviceToDevice, s));
+    b.swap(nb);
+}
+// room on the device for `need` words and their values
+static void dev_reserve(mpgpu_ctx *c, int64_t need)
+{
+    if ((int64_t)c->d_in.n >= need) return;
+    const int64_t cap = std::max<int64_t>({need, (int64_t)c->d_in.n + (int64_t)c->d_in.n / 2, (int64_t)1 << 20});
+    grow_dev(c->d_in, c->nwords, cap, c->stream);
+    if (c->dbl) grow_dev(c->d_vd, c->nwords, cap, c->stream);
+    else grow_dev(c->d_vf, c->nwords, cap, c->stream);
+}
+// the values held so far become FP64 (the first FP64 record or payload after FP32 ones)
+static void widen_values(mpgpu_ctx *c)
+{
+    if (c->dbl) return;
+    DBuf<double> vd;
+    vd.alloc(std::max<size_t>(c->d_in.n, 1));
+    st_widen(c->d_vf.p, vd.p, c->nwords, c->stream);
+    c->d_vd.swap(vd);
+    c->d_vf.release();
+    c->dbl = true;
+}
+// send the staged words of the current buffer; the next buffer is free to fill on return
+static void stage_flush(mpgpu_ctx *c)
+{
+    Staging &S = *c->stage;
+    if (S.n == 0) return;
+    hipStream_t s = c->stream;
+    dev_reserve(c, c->nwords + S.n);
+    HIPCHK(hipMemcpyAsync(c->d_in.p + c->nwords, S.words(), sizeof(int32_t) * S.n, hipMemcpyHostToDevice, s));
+    if (c->dbl) HIPCHK(hipMemcpyAsync(c->d_vd.p + c->nwords, S.vals(), sizeof(double) * S.n, hipMemcpyHostToDevice, s));
+    else HIPCHK(hipMemcpyAsync(c->d_vf.p + c->nwords, S.vals(), sizeof(float) * S.n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(S.done[S.k], s));
+    S.inflight[S.k] = true;
+    c->nwords += S.n;
+    S.n = 0;
+    S.k ^= 1;
+    if (S.inflight[S.k]) {
+        HIPCHK(hipEventSynchronize(S.done[S.k]));
+        S.inflight[S.k] = false;
+    }
+}
+// room for a record of nw words in the staging buffer (FP64 values: widen what came before)
+static void stage_room(mpgpu_ctx *c, int64_t nw, bool fp64)
+{
+    if (!c->stage) {
+        c->stage = std::make_unique<Staging>();
+        c->stage->init();
+    }
+    if (nw > Staging::CHUNK) throw MpError(18, "record of " + std::to_string(nw) + " words exceeds the reader's buffer");
+    if (fp64 && !c->dbl) {
+        stage_flush(c);
+        widen_values(c);
+    }
+    if (c->stage->n + nw > Staging::CHUNK) stage_flush(c);
+}
+// all records on the device: the last staged words are sent and the staging buffers freed
+static void stage_finish(mpgpu_ctx *c)
+{
+    if (!c->stage) return;
+    stage_flush(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->stage.reset();
+}
+
 extern "C" int mpgpu_add_records(mpgpu_ctx *c, int64_t nrec, const int64_t *rec_ptr, const int32_t *inder,
                                  const float *gf, const double *gd)
 {
     if (!c) return 30;
     API_TRY
+    HIPCHK(hipSetDevice(c->device));
     if (c->built) throw MpError(24, "records cannot be added after the structure was built");
     if (nrec < 0 || (nrec > 0 && (!rec_ptr || !inder || (!gf && !gd)))) throw MpError(24, "bad record arrays");
     if (gf && gd) throw MpError(24, "give either FP32 or FP64 payload");
+    if (nrec == 0) return 0;
     int64_t w0 = rec_ptr[0], w1 = rec_ptr[nrec];
     if (w1 < w0) throw MpError(24, "record pointers not ascending");
     for (int64_t r = 0; r < nrec; r++)
         if (rec_ptr[r + 1] < rec_ptr[r]) throw MpError(24, "record pointers not ascending");
-    int64_t base = (int64_t)c->inder.size();
-    bool was_dbl = c->dbl;
-    if (gd && !was_dbl) {
-        // promote existing FP32 payload
-        c->vd.resize(c->vf.size());
-        for (size_t i = 0; i < c->vf.size(); i++) c->vd[i] = c->vf[i];
-        c->vf.clear();
-        c->vf.shrink_to_fit();
-        c->dbl = true;
-    }
-    c->inder.insert(c->inder.end(), inder + w0, inder + w1);
-    if (c->dbl) {
-        if (gd) c->vd.insert(c->vd.end(), gd + w0, gd + w1);
-        else for (int64_t i = w0; i < w1; i++) c->vd.push_back((double)gf[i]);
+    hipStream_t s = c->stream;
+    const int64_t total = w1 - w0;
+    const int64_t base = c->nwords + (c->stage ? c->stage->n : 0);
+    if (!c->stage && total < Staging::CHUNK) {
+        // a small input goes straight from the caller's arrays (the copies end before return)
+        if (gd) widen_values(c);
+        dev_reserve(c, c->nwords + total);
+        HIPCHK(hipMemcpyAsync(c->d_in.p + c->nwords, inder + w0, sizeof(int32_t) * total, hipMemcpyHostToDevice, s));
+        std::vector<double> wide;
+        if (c->dbl && gd) {
+            HIPCHK(hipMemcpyAsync(c->d_vd.p + c->nwords, gd + w0, sizeof(double) * total, hipMemcpyHostToDevice, s));
+        } else if (c->dbl) {
+            wide.assign(gf + w0, gf + w1);
+            HIPCHK(hipMemcpyAsync(c->d_vd.p + c->nwords, wide.data(), sizeof(double) * total, hipMemcpyHostToDevice, s));
+        } else {
+            HIPCHK(hipMemcpyAsync(c->d_vf.p + c->nwords, gf + w0, sizeof(float) * total, hipMemcpyHostToDevice, s));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        c->nwords += total;
     } else {
-        c->vf.insert(c->vf.end(), gf + w0, gf + w1);
+        // through the staging buffers, in pieces: filling one overlaps the copy of the other
+        for (int64_t w = w0; w < w1;) {
+            const int64_t piece = std::min<int64_t>(w1 - w, Staging::CHUNK);
+            stage_room(c, piece, gd != nullptr);
+            Staging &S = *c->stage;
+            std::memcpy(S.words() + S.n, inder + w, sizeof(int32_t) * piece);
+            if (c->dbl) {
+                double *v = (double *)S.vals() + S.n;
+                if (gd) std::memcpy(v, gd + w, sizeof(double) * piece);
+                else for (int64_t i = 0; i < piece; i++) v[i] = (double)gf[w + i];
+            } else {
+                std::memcpy((float *)S.vals() + S.n, gf + w, sizeof(float) * piece);
+            }
+            S.n += piece;
+            w += piece;
+        }
     }
     for (int64_t r = 0; r < nrec; r++) c->rec_ptr.push_back(base + rec_ptr[r + 1] - w0);
     return 0;
@@ -297,76 +425,92 @@ extern "C" int mpgpu_add_records(mpgpu_ctx *c, int64_t nrec, const int64_t *rec_
 
 // Mille binary files (src/mille.f90:208-219; C files as written by the Mille library, which
 // pede reads plain or gzip-compressed, src/pede.f90:1689): zlib reads both, so one reader covers
-// plain and .gz files of either flavour.
+// plain and .gz files of either flavour.  Records are read straight into the pinned staging
+// buffers; each full buffer is sent to the device while the next one fills.
 extern "C" int mpgpu_read_mille_file(mpgpu_ctx *c, const char *path, int32_t fortran, int64_t *nrec_out)
 {
     if (!c) return 30;
-    API_TRY
-    gzFile f = gzopen(path, "rb");
-    if (!f) throw MpError(18, std::string("cannot open binary file ") + path);
-    struct Closer { gzFile f; ~Closer() { gzclose(f); } } closer{f};
-    gzbuffer(f, 1 << 20);
-    auto get = [&](void *dst, size_t size, size_t count) -> size_t {
-        const size_t want = size * count;
-        size_t got = 0;
-        while (got < want) {
-            const unsigned chunk = (unsigned)std::min<size_t>(want - got, 1u << 30);
-            const int r = gzread(f, (char *)dst + got, chunk);
-            if (r <= 0) break;
-            got += (size_t)r;
+    const size_t rp_saved = c->rec_ptr.size();
+    const int64_t total_saved = c->nwords + (c->stage ? c->stage->n : 0);
+    try {
+        AllocScope alloc_scope_(c->stream);
+        HIPCHK(hipSetDevice(c->device));
+        if (c->built) throw MpError(24, "records cannot be added after the structure was built");
+        gzFile f = gzopen(path, "rb");
+        if (!f) throw MpError(18, std::string("cannot open binary file ") + path);
+        struct Closer { gzFile f; ~Closer() { gzclose(f); } } closer{f};
+        gzbuffer(f, 1 << 20);
+        {
+            // a plain file holds at most size / 8 words (FP32 value and index per word): the
+            // device arrays are sized once for it instead of growing
+            struct stat sb;
+            if (gzdirect(f) && stat(path, &sb) == 0 && sb.st_size > 0)
+                dev_reserve(c, c->nwords + (c->stage ? c->stage->n : 0) + (int64_t)sb.st_size / 8 + 1);
         }
-        return got / size;
-    };
-    std::vector<int64_t> rp{0};
-    std::vector<int32_t> in;
-    std::vector<float> ff;
-    std::vector<double> fd;
-    bool any_dbl = false;
-    int64_t nrec = 0;
-    std::vector<float> tmpf;
-    for (;;) {
-        int32_t marker = 0, nr = 0;
-        if (fortran) {
-            if (get(&marker, 4, 1) != 1) break;
-            if (get(&nr, 4, 1) != 1) throw MpError(18, "truncated Fortran record");
-        } else {
-            if (get(&nr, 4, 1) != 1) break;
+        auto get = [&](void *dst, size_t size, size_t count) -> size_t {
+            const size_t want = size * count;
+            size_t got = 0;
+            while (got < want) {
+                const unsigned chunk = (unsigned)std::min<size_t>(want - got, 1u << 30);
+                const int r = gzread(f, (char *)dst + got, chunk);
+                if (r <= 0) break;
+                got += (size_t)r;
+            }
+            return got / size;
+        };
+        int64_t nrec = 0;
+        std::vector<float> tmpf;
+        for (;;) {
+            int32_t marker = 0, nr = 0;
+            if (fortran) {
+                if (get(&marker, 4, 1) != 1) break;
+                if (get(&nr, 4, 1) != 1) throw MpError(18, "truncated Fortran record");
+            } else {
+                if (get(&nr, 4, 1) != 1) break;
+            }
+            const bool fp64 = nr < 0;
+            const int64_t nw = (fp64 ? -(int64_t)nr : (int64_t)nr) / 2;
+            stage_room(c, nw, fp64);
+            Staging &S = *c->stage;
+            if (fp64) {
+                if (get((double *)S.vals() + S.n, 8, nw) != (size_t)nw) throw MpError(18, "truncated record");
+            } else if (c->dbl) {
+                tmpf.resize(nw);
+                if (get(tmpf.data(), 4, nw) != (size_t)nw) throw MpError(18, "truncated record");
+                double *v = (double *)S.vals() + S.n;
+                for (int64_t i = 0; i < nw; i++) v[i] = tmpf[i];
+            } else {
+                if (get((float *)S.vals() + S.n, 4, nw) != (size_t)nw) throw MpError(18, "truncated record");
+            }
+            if (get(S.words() + S.n, 4, nw) != (size_t)nw) throw MpError(18, "truncated record");
+            if (fortran) {
+                int32_t m2 = 0;
+                if (get(&m2, 4, 1) != 1 || m2 != marker) throw MpError(18, "bad Fortran record marker");
+            }
+            S.n += nw;
+            c->rec_ptr.push_back(c->nwords + S.n);
+            nrec++;
         }
-        bool dbl = nr < 0;
-        int32_t nw = (dbl ? -nr : nr) / 2;
-        if (dbl && !any_dbl) {
-            any_dbl = true;
-            fd.resize(ff.size());
-            for (size_t i = 0; i < ff.size(); i++) fd[i] = ff[i];
-            ff.clear();
+        int err = 0;
+        gzerror(f, &err);
+        if (err != Z_OK && err != Z_STREAM_END) throw MpError(18, std::string("corrupt binary file ") + path);
+        if (nrec_out) *nrec_out = nrec;
+        return 0;
+    } catch (const MpError &e) {
+        // the records of this file are dropped (those of earlier calls stay)
+        c->rec_ptr.resize(rp_saved);
+        if (c->nwords >= total_saved) {
+            c->nwords = total_saved;
+            if (c->stage) c->stage->n = 0;
+        } else if (c->stage) {
+            c->stage->n = total_saved - c->nwords;
         }
-        size_t off = in.size();
-        in.resize(off + nw);
-        if (dbl) {
-            fd.resize(off + nw);
-            if (get(fd.data() + off, 8, nw) != (size_t)nw) throw MpError(18, "truncated record");
-        } else {
-            tmpf.resize(nw);
-            if (get(tmpf.data(), 4, nw) != (size_t)nw) throw MpError(18, "truncated record");
-            if (any_dbl) { fd.resize(off + nw); for (int i = 0; i < nw; i++) fd[off + i] = tmpf[i]; }
-            else ff.insert(ff.end(), tmpf.begin(), tmpf.end());
-        }
-        if (get(in.data() + off, 4, nw) != (size_t)nw) throw MpError(18, "truncated record");
-        if (fortran) {
-            int32_t m2 = 0;
-            if (get(&m2, 4, 1) != 1 || m2 != marker) throw MpError(18, "bad Fortran record marker");
-        }
-        rp.push_back((int64_t)in.size());
-        nrec++;
+        return fail(c, e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(c, 30, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(c, 30, e.what());
     }
-    int err = 0;
-    gzerror(f, &err);
-    if (err != Z_OK && err != Z_STREAM_END) throw MpError(18, std::string("corrupt binary file ") + path);
-    int rc = mpgpu_add_records(c, nrec, rp.data(), in.data(), any_dbl ? nullptr : ff.data(), any_dbl ? fd.data() : nullptr);
-    if (rc) return rc;
-    if (nrec_out) *nrec_out = nrec;
-    return 0;
-    API_CATCH(c)
 }
 
 extern "C" int mpgpu_set_steering_parameters(mpgpu_ctx *c, int32_t n, const int32_t *labels, const double *values,
@@ -511,12 +655,11 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     // 'pairentries' above 1 drops the matrix updates of rarely filled parameter-group pairs
     // (src/pede.f90:6564, 7036, src/mpbits.f90): not built, refused as the steering parser does
     if (c->opt.pair_entries > 1) throw MpError(40, "pairentries > 1 (dropping rare parameter pairs) is not supported");
+    if (c->built) throw MpError(24, "the structure is already built");
+    stage_finish(c);
     const int64_t nrec = (int64_t)c->rec_ptr.size() - 1;
+    const int64_t nwords = c->nwords;
     c->nrec = nrec;
-    const int32_t *I = c->inder.data();
-    const float *F = c->dbl ? nullptr : c->vf.data();
-    const double *D = c->dbl ? c->vd.data() : nullptr;
-    auto val = [&](int64_t w) { return D ? D[w] : (double)F[w]; };
     // MPGPU_VERBOSE: host time of each phase on stderr
     const bool verbose = std::getenv("MPGPU_VERBOSE") != nullptr;
     double t_phase = now_ms();
@@ -528,16 +671,20 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     };
 
     hipStream_t s = c->stream;
-    // ---- records to the device: words and values; the words as read (labels) are kept apart
-    // from the words the record kernels use (global words replaced by their position in the
-    // record's list, see below) ----
+    // ---- the records are on the device since they were added; the words as read (labels) are
+    // kept apart from the words the record kernels use (global words replaced by their position
+    // in the record's list, see below) ----
     c->d_rec_w.upload(c->rec_ptr, s);
-    if (c->dbl) c->d_vd.upload(c->vd, s); else c->d_vf.upload(c->vf, s);
     DBuf<int32_t> d_lab;
-    d_lab.upload(c->inder, s);
-    c->d_widx.alloc(std::max<size_t>(c->inder.size(), 1));
-    if (!c->inder.empty())
-        HIPCHK(hipMemcpyAsync(c->d_widx.p, d_lab.p, sizeof(int32_t) * c->inder.size(), hipMemcpyDeviceToDevice, s));
+    d_lab.swap(c->d_in);
+    if ((int64_t)d_lab.n > nwords + nwords / 16 + 1024) {
+        // growth slack of a file read: the value array is kept for the passes, at its size
+        if (c->dbl) grow_dev(c->d_vd, nwords, std::max<int64_t>(nwords, 1), s);
+        else grow_dev(c->d_vf, nwords, std::max<int64_t>(nwords, 1), s);
+    }
+    c->d_widx.alloc(std::max<int64_t>(nwords, 1));
+    if (nwords > 0)
+        HIPCHK(hipMemcpyAsync(c->d_widx.p, d_lab.p, sizeof(int32_t) * nwords, hipMemcpyDeviceToDevice, s));
     StructIn sin;
     sin.nrec = nrec;
     sin.rec_w = c->d_rec_w.p;
@@ -568,6 +715,8 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     std::vector<int32_t> ngl_rec;
     std::vector<uint8_t> bnd;
     std::vector<int64_t> recount;   // iterated entries cut: entries in equations without low parameters
+    std::vector<int32_t> mon_lab;   // residual monitoring: first global label and error of each equation
+    std::vector<double> mon_err;
     {
         std::vector<int32_t> extra(c->sp_label.begin(), c->sp_label.end());
         for (auto &b : c->cons) extra.insert(extra.end(), b.labels.begin(), b.labels.end());
@@ -587,6 +736,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         // equation, or per record with 'countrecords'; derivatives of zero do not count.
         st_record_lists(sin, lmap, c->ntot, c->opt.mcount != 0, c->d_widx, c->d_rec_gl, c->d_gl_tix, ngl_rec, c->counters,
                         bnd, s);
+        if (c->opt.monitor != 0) st_eq_first(sin, mon_lab, mon_err, s);
         // iterated entries cut (loop1i, src/pede.f90:5975-5981, 6164-6265): only with counting
         // per equation; the parameters fixed by the entries cut are those without a negative
         // pre-sigma and with fewer than mreqenf entries
@@ -618,20 +768,16 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->mon_res.clear();
     std::vector<int32_t> mon_grp;
     if (c->opt.monitor != 0) {
-        host_eq();
         std::vector<int64_t> mcnt(ntot, 0);
         std::vector<double> msum(ntot, 0.0);
-        for (int64_t r = 0; r < nrec; r++)
-            for (int64_t e = rec_eq[r]; e < rec_eq[r + 1]; e++) {
-                const int nl = eq_n[e] & 0xffff, ng = eq_n[e] >> 16;
-                if (ng <= 0) continue;
-                const int64_t sg = eq_sl[e] + nl + 1;
-                const int32_t t = tix_of(I[sg + 1]);
-                mcnt[t]++;
-                // 'scaleerrors' applies to the errors pede reads here as well (prepareRecords
-                // before loop2, src/pede.f90:6638)
-                msum[t] = msum[t] + val(sg) * c->opt.scale_errors[0];
-            }
+        for (int64_t e = 0; e < neq; e++) {
+            if (mon_lab[e] == 0) continue;
+            const int32_t t = tix_of(mon_lab[e]);
+            mcnt[t]++;
+            // 'scaleerrors' applies to the errors pede reads here as well (prepareRecords
+            // before loop2, src/pede.f90:6638)
+            msum[t] = msum[t] + mon_err[e] * c->opt.scale_errors[0];
+        }
         mon_grp.assign(ntot, -1);
         for (int32_t t = 0; t < ntot; t++)
             if (mcnt[t] > 0) {
@@ -871,9 +1017,17 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     int32_t max_qe = 1;   // most equations any record variable occurs in (full local fits)
     std::vector<int32_t> binfo(nrec, 0);
     const int lfitbb = c->opt.lfitbb;
+    // the record words with local indices (band detection; global words hold their list position)
+    std::vector<int32_t> words;
     if (lfitbb > 0)
         for (int64_t r = 0; r < nrec; r++)
-            if (st_nalc[r] > 5) { host_eq(); break; }
+            if (st_nalc[r] > 5) {
+                host_eq();
+                words.resize(std::max<int64_t>(nwords, 1));
+                c->d_widx.download(words.data(), nwords, s);
+                HIPCHK(hipStreamSynchronize(s));
+                break;
+            }
 #pragma omp parallel for schedule(dynamic, 4096) reduction(max : max_gl, max_eq, max_nalc, max_var, max_seg, bmax_gl, \
     bmax_eq, bmax_nalc, bmax_var, bmax_seg, bmax_mp1, bmax_bdr, bmax_sup, max_qe)
     for (int64_t r = 0; r < nrec; r++) {
@@ -882,7 +1036,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
         const int32_t nalc = st_nalc[r], nv_r = st_nv[r], nseg = st_nseg[r];
         // bordered-band local fit (decided once, as pede does in the first iteration)
         int32_t bi = 0;
-        if (nalc > 5 && lfitbb > 0) bi = band_info(c->inder.data(), eq_sl.data(), eq_n.data(), rec_eq[r], rec_eq[r + 1], nalc, ibh);
+        if (nalc > 5 && lfitbb > 0) bi = band_info(words.data(), eq_sl.data(), eq_n.data(), rec_eq[r], rec_eq[r + 1], nalc, ibh);
         binfo[r] = bi;
         if (bi) {
             bmax_gl = std::max(bmax_gl, ngl);

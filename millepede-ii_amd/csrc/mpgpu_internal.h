@@ -296,6 +296,10 @@ void st_record_lists(const StructIn &in, const LabelMap &map, int32_t ntot, bool
 // per record: st[k * nrec + r], k = variables, segments, largest local index, local x global
 // words, equations per variable, first group; cnt_r: values of its matrix update; sparse: the
 // group pairs it couples into pm
+// FP32 values widened to FP64 (device to device)
+void st_widen(const float *src, double *dst, int64_t n, hipStream_t s);
+// residual monitoring: label of the first global word (0: none) and error of every equation
+void st_eq_first(const StructIn &in, std::vector<int32_t> &lab, std::vector<double> &err, hipStream_t s);
 // iterated entries cut: recount the global words of the equations without a low parameter
 void st_entries_recount(const StructIn &in, const LabelMap &map, const std::vector<uint8_t> &low,
                         std::vector<int64_t> &counts, hipStream_t s);
