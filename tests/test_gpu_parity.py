@@ -128,6 +128,68 @@ def test_brokenlines_band_pass(orc, matsto):
     assert s_gpu["naccept"] > 1400
 
 
+def _band_side(recs, ii, r):
+    """bordered-band choice of one record (loopbf, src/pede.f90:3815-3874): 0 full, 1 border
+    first, 2 border last"""
+    nalc = max(int(ii[w]) for (sL, sG, lG) in recs[r] for w in range(sL + 1, sG))
+    if nalc <= 5:
+        return 0
+    ibh = [0] * (2 * nalc + 2)
+    for (sL, sG, lG) in recs[r]:
+        loc = [int(ii[w]) for w in range(sL + 1, sG)]
+        for j in range(len(loc)):
+            for k in range(j + 1):
+                a, b = loc[j], loc[k]
+                d = abs(a - b) + 1
+                ibh[d] = max(ibh[d], min(a, b))
+                ibh[nalc + d] = max(ibh[nalc + d], min(nalc + 1 - a, nalc + 1 - b))
+    icmn, side = nalc ** 3, 0
+    for sd in (1, 2):
+        if side:
+            break
+        kbdr = 0
+        for k in range(nalc, 1, -1):
+            kbdr = max(kbdr, ibh[k if sd == 1 else k + nalc])
+            t = k - 2 + kbdr + 1
+            icost = 6 * (nalc - kbdr) * t * t + 2 * kbdr ** 3
+            if icost < icmn:
+                icmn, side = icost, sd
+    return side
+
+
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_brokenlines_band_border_last(orc, matsto):
+    """The same broken-lines records with their local parameters numbered backwards: the border
+    is then at the end of the local system and pede's cost model picks the border-last
+    factorization (sqmibb2, src/mpnum.f90:3390-3628).  The GPU band kernel's pass against the
+    oracle, and equal to the pass of the records as generated (a renumbering of the local
+    parameters does not change the fit)."""
+    rp, ii, gg = mpgpu.brokenlines_records(33, 24, 3, 3, 1000, 8, 24)
+    recs = LB.decode(rp, ii, gg)
+    # local index j -> nalc + 1 - j, the local words of each equation kept ascending (pede adds
+    # them into the packed lower triangle assuming that order, src/pede.f90:3818-3824)
+    rev, gr = ii.copy(), gg.copy()
+    for eqs in recs:
+        nalc = max(int(ii[w]) for (sL, sG, lG) in eqs for w in range(sL + 1, sG))
+        for (sL, sG, lG) in eqs:
+            loc = sorted((nalc + 1 - int(ii[w]), gg[w]) for w in range(sL + 1, sG))
+            for q, (j, d) in enumerate(loc):
+                rev[sL + 1 + q], gr[sL + 1 + q] = j, d
+    sides = [_band_side(recs, rev, r) for r in range(len(recs))]
+    assert sides.count(2) > 900, sides.count(2)
+    assert all(_band_side(recs, ii, r) == 1 for r in range(0, len(recs), 50))
+    P = _pede(rp, rev, gr, matsto=matsto, mreqenf=1, mcount=1)
+    lab, var = _check_maps(P, rp, rev, gr, mreqenf=1, mcount=1)
+    s_rev, mat_rev, vec_rev = _compare_pass(P, orc, rp, rev, gr, lab, var)
+    Q = _pede(rp, ii, gg, matsto=matsto, mreqenf=1, mcount=1)
+    s_fwd, mat_fwd, vec_fwd = _compare_pass(Q, orc, rp, ii, gg, lab, var)
+    assert s_rev["ndf"] == s_fwd["ndf"] and s_rev["naccept"] == s_fwd["naccept"] > 900
+    assert abs(s_rev["chi2"] - s_fwd["chi2"]) <= 1e-9 * abs(s_fwd["chi2"])
+    assert rel_err(vec_rev, vec_fwd) < 1e-9
+    nv = P.nvar
+    assert rel_err(packed_to_dense(mat_rev, nv), packed_to_dense(mat_fwd, nv)) < 1e-9
+
+
 def test_brokenlines_band_downweighting(orc):
     """Band-path records with outliers: down-weighting iterations (lhuber 3), chi^2 cut and
     parameters away from zero, against the oracle."""
