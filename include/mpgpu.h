@@ -172,7 +172,7 @@ int mpgpu_set_options(mpgpu_ctx *ctx, const mpgpu_options *opt);
  * inder/glder exactly as they are stored in a Mille record (word 1 = (0.0, error count),
  * src/mille.f90:27-59).  Exactly one of glder_f (FP32 payload) / glder_d (FP64) is non-NULL.
  * Labels in global words stay raw labels.  The records are in device memory when the call
- * returns (large inputs through pinned staging buffers); the caller's arrays may be reused. */
+ * returns; the caller's arrays may be reused. */
 int mpgpu_add_records(mpgpu_ctx *ctx, int64_t nrec, const int64_t *rec_ptr, const int32_t *inder,
                       const float *glder_f, const double *glder_d);
 /* Read a whole Mille binary file (C binary, or Fortran sequential when fortran != 0; plain or
@@ -273,7 +273,10 @@ int mpgpu_run_steering(const char *steering_file, const char *output_dir, int32_
  * Jobs are admitted by their estimated device-memory footprint under budget_bytes and run by
  * nworkers workers, each with its own context and HIP stream, so the kernels of different
  * jobs overlap on the device.  A job is: options, FP32 Mille records, optional Parameter
- * lines (label, start value, pre-sigma) and Lagrange constraints (CSR: cptr[ncon+1]). */
+ * lines (label, start value, pre-sigma) and Lagrange constraints (CSR: cptr[ncon+1]).
+ * mpgpu_server_submit only queues the job: its arrays are not copied and must stay valid and
+ * unchanged until mpgpu_server_wait has returned for it; the worker that takes the job sizes
+ * it (status 30 when it exceeds the whole budget) and starts it when it fits. */
 typedef struct mpgpu_server mpgpu_server;
 typedef struct mpgpu_job_stats {
     int32_t status, nvar;

@@ -56,6 +56,7 @@ struct mpgpu_ctx {
     mpgpu_options opt{};
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;       // false: a stream the caller keeps (serving workers)
     std::string err;
 
     // ---- input records: record pointers on the host; words and values go to the device as
@@ -223,6 +224,13 @@ extern "C" void mpgpu_default_options(mpgpu_options *o)
 
 extern "C" int mpgpu_create(const mpgpu_options *opt, int32_t device, mpgpu_ctx **out)
 {
+    return mpgpu::create_context(opt, device, nullptr, out);
+}
+
+// A context on a stream of the caller's (kept when the context is destroyed), or on a stream of
+// its own (stream null)
+int mpgpu::create_context(const mpgpu_options *opt, int32_t device, hipStream_t stream, mpgpu_ctx **out)
+{
     if (!out) return 30;
     mpgpu_ctx *c = nullptr;
     API_TRY
@@ -233,7 +241,12 @@ extern "C" int mpgpu_create(const mpgpu_options *opt, int32_t device, mpgpu_ctx 
     if (device < 0 || device >= ndev) throw MpError(30, "bad device ordinal");
     c->device = device;
     HIPCHK(hipSetDevice(device));
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (stream) {
+        c->stream = stream;
+        c->own_stream = false;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    }
     {
         // freed stream-ordered buffers stay in the device's pool up to 32 GB, so that the
         // buffers of the next calls are made without returning to the driver
@@ -257,11 +270,12 @@ extern "C" int mpgpu_destroy(mpgpu_ctx *c)
     (void)hipSetDevice(c->device);
     // the context's buffers are freed in the order of its stream: the stream goes last
     hipStream_t s = c->stream;
+    const bool own = c->own_stream;
     if (s) (void)hipStreamSynchronize(s);
     delete c;
     if (s) {
         (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
+        if (own) (void)hipStreamDestroy(s);
     }
     return 0;
 }
@@ -383,41 +397,25 @@ extern "C" int mpgpu_add_records(mpgpu_ctx *c, int64_t nrec, const int64_t *rec_
         if (rec_ptr[r + 1] < rec_ptr[r]) throw MpError(24, "record pointers not ascending");
     hipStream_t s = c->stream;
     const int64_t total = w1 - w0;
-    const int64_t base = c->nwords + (c->stage ? c->stage->n : 0);
-    if (!c->stage && total < Staging::CHUNK) {
-        // a small input goes straight from the caller's arrays (the copies end before return)
-        if (gd) widen_values(c);
-        dev_reserve(c, c->nwords + total);
-        HIPCHK(hipMemcpyAsync(c->d_in.p + c->nwords, inder + w0, sizeof(int32_t) * total, hipMemcpyHostToDevice, s));
-        std::vector<double> wide;
-        if (c->dbl && gd) {
-            HIPCHK(hipMemcpyAsync(c->d_vd.p + c->nwords, gd + w0, sizeof(double) * total, hipMemcpyHostToDevice, s));
-        } else if (c->dbl) {
-            wide.assign(gf + w0, gf + w1);
-            HIPCHK(hipMemcpyAsync(c->d_vd.p + c->nwords, wide.data(), sizeof(double) * total, hipMemcpyHostToDevice, s));
-        } else {
-            HIPCHK(hipMemcpyAsync(c->d_vf.p + c->nwords, gf + w0, sizeof(float) * total, hipMemcpyHostToDevice, s));
-        }
-        HIPCHK(hipStreamSynchronize(s));
-        c->nwords += total;
+    // records a file read has staged go first; then the arrays are copied straight from the
+    // caller's memory (the runtime stages pageable memory itself; the copies end before return,
+    // so the caller may reuse its arrays)
+    if (c->stage) stage_flush(c);
+    const int64_t base = c->nwords;
+    if (gd) widen_values(c);
+    dev_reserve(c, c->nwords + total);
+    HIPCHK(hipMemcpyAsync(c->d_in.p + c->nwords, inder + w0, sizeof(int32_t) * total, hipMemcpyHostToDevice, s));
+    std::vector<double> wide;
+    if (c->dbl && gd) {
+        HIPCHK(hipMemcpyAsync(c->d_vd.p + c->nwords, gd + w0, sizeof(double) * total, hipMemcpyHostToDevice, s));
+    } else if (c->dbl) {
+        wide.assign(gf + w0, gf + w1);
+        HIPCHK(hipMemcpyAsync(c->d_vd.p + c->nwords, wide.data(), sizeof(double) * total, hipMemcpyHostToDevice, s));
     } else {
-        // through the staging buffers, in pieces: filling one overlaps the copy of the other
-        for (int64_t w = w0; w < w1;) {
-            const int64_t piece = std::min<int64_t>(w1 - w, Staging::CHUNK);
-            stage_room(c, piece, gd != nullptr);
-            Staging &S = *c->stage;
-            std::memcpy(S.words() + S.n, inder + w, sizeof(int32_t) * piece);
-            if (c->dbl) {
-                double *v = (double *)S.vals() + S.n;
-                if (gd) std::memcpy(v, gd + w, sizeof(double) * piece);
-                else for (int64_t i = 0; i < piece; i++) v[i] = (double)gf[w + i];
-            } else {
-                std::memcpy((float *)S.vals() + S.n, gf + w, sizeof(float) * piece);
-            }
-            S.n += piece;
-            w += piece;
-        }
+        HIPCHK(hipMemcpyAsync(c->d_vf.p + c->nwords, gf + w0, sizeof(float) * total, hipMemcpyHostToDevice, s));
     }
+    HIPCHK(hipStreamSynchronize(s));
+    c->nwords += total;
     for (int64_t r = 0; r < nrec; r++) c->rec_ptr.push_back(base + rec_ptr[r + 1] - w0);
     return 0;
     API_CATCH(c)

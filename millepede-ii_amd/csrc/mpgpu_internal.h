@@ -296,6 +296,9 @@ void st_record_lists(const StructIn &in, const LabelMap &map, int32_t ntot, bool
 // per record: st[k * nrec + r], k = variables, segments, largest local index, local x global
 // words, equations per variable, first group; cnt_r: values of its matrix update; sparse: the
 // group pairs it couples into pm
+// a context on a stream the caller keeps (serving workers reuse one stream per worker), or on
+// a stream of its own when stream is null
+int create_context(const mpgpu_options *opt, int32_t device, hipStream_t stream, mpgpu_ctx **out);
 // FP32 values widened to FP64 (device to device)
 void st_widen(const float *src, double *dst, int64_t n, hipStream_t s);
 // residual monitoring: label of the first global word (0: none) and error of every equation

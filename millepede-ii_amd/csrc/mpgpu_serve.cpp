@@ -1,9 +1,9 @@
 // Serving loop: alignment/calibration jobs of different sizes arrive over time, are admitted
 // by their device-memory footprint under a budget, run concurrently on one GPU (one context
-// and HIP stream per worker, so the kernels of different jobs overlap on the device) and
-// report their latency from arrival to result.  Small jobs (full storage) are grouped: a
-// worker that takes a small job also takes the small jobs queued behind it (up to 8, while
-// they fit the budget) and runs them back to back on its stream.
+// per job on its worker's HIP stream, so the kernels of different jobs overlap on the device)
+// and report their latency from arrival to result.  Submitting a job only queues it: the job
+// refers to the caller's arrays (kept valid until the job is waited for), and the worker that
+// takes it sizes it and waits until it fits the budget.
 // No pede counterpart: pede runs one job per process (src/pede.f90:33-528).
 #include "mpgpu_internal.h"
 #include <algorithm>
@@ -25,15 +25,19 @@ using clk = std::chrono::steady_clock;
 struct Job {
     int64_t id = 0;
     mpgpu_options opt{};
-    std::vector<int64_t> rec_ptr;
-    std::vector<int32_t> inder;
-    std::vector<float> glder;
-    std::vector<int32_t> plab;
-    std::vector<double> pval, psig;
-    std::vector<int64_t> cptr;
-    std::vector<int32_t> clab;
-    std::vector<double> cfac, crhs;
-    int64_t footprint = 0;
+    // the caller's arrays (valid until the job has been waited for)
+    int64_t nrec = 0;
+    const int64_t *rec_ptr = nullptr;
+    const int32_t *inder = nullptr;
+    const float *glder = nullptr;
+    int32_t npar = 0;
+    const int32_t *plab = nullptr;
+    const double *pval = nullptr, *psig = nullptr;
+    int32_t ncon = 0;
+    const int64_t *cptr = nullptr;
+    const int32_t *clab = nullptr;
+    const double *cfac = nullptr, *crhs = nullptr;
+    int64_t footprint = -1;   // device bytes, sized by the worker that takes the job
     clk::time_point t_arrival, t_start, t_end;
     // results
     int status = -1;
@@ -52,8 +56,8 @@ struct Job {
 // vectors and solver work space
 int64_t estimate_footprint(const Job &j)
 {
-    int64_t nw = j.rec_ptr.empty() ? 0 : j.rec_ptr.back();
-    int64_t nrec = (int64_t)j.rec_ptr.size() - 1;
+    const int64_t nrec = j.nrec;
+    const int64_t nw = nrec > 0 ? j.rec_ptr[nrec] - j.rec_ptr[0] : 0;
     int64_t rec = 12 * nw + 64 * nrec;
     // distinct labels of the global words: after the second zero-index word of a measurement
     // (measurement, locals, sigma, globals; src/mille.f90:27-59)
@@ -80,7 +84,7 @@ int64_t estimate_footprint(const Job &j)
         for (int32_t l : lab) bits[(l - lo) >> 6] |= 1ull << ((l - lo) & 63);
         for (uint64_t b : bits) nlab += __builtin_popcountll(b);
     }
-    int64_t nall = nlab + (int64_t)j.crhs.size();
+    int64_t nall = nlab + (int64_t)j.ncon;
     int64_t mat;
     if (j.opt.matsto == MPGPU_FULL) {
         // packed matrix, plus what the solve allocates beside it: the inversion keeps a copy
@@ -108,16 +112,16 @@ int64_t estimate_footprint(const Job &j)
 struct mpgpu_server {
     int device = 0;
     int64_t budget = 0, in_use = 0;
-    int batch_max_params = 5000;
+    int batch_max_params = 5000;   // (kept for the interface; jobs are run one per worker)
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_budget;
     std::deque<std::shared_ptr<Job>> queue;
     std::map<int64_t, std::shared_ptr<Job>> jobs;
     std::vector<std::thread> workers;
     bool stop = false;
     int64_t next_id = 1;
 
-    void run_job(Job &j, mpgpu_ctx *&ctx)
+    void run_job(Job &j, hipStream_t wstream)
     {
         j.t_start = clk::now();
         // MPGPU_VERBOSE: time of each step of the job on stderr
@@ -131,13 +135,12 @@ struct mpgpu_server {
         };
         int rc = 0;
         mpgpu_ctx *c = nullptr;
-        rc = mpgpu_create(&j.opt, device, &c);
-        if (!rc) rc = mpgpu_add_records(c, (int64_t)j.rec_ptr.size() - 1, j.rec_ptr.data(), j.inder.data(), j.glder.data(), nullptr);
-        if (!rc && !j.plab.empty())
-            rc = mpgpu_set_steering_parameters(c, (int32_t)j.plab.size(), j.plab.data(), j.pval.data(), j.psig.data());
-        if (!rc && !j.crhs.empty()) {
-            std::vector<double> sig(j.crhs.size(), 0.0);
-            rc = mpgpu_add_equations(c, 0, (int32_t)j.crhs.size(), j.cptr.data(), j.clab.data(), j.cfac.data(), j.crhs.data(), sig.data());
+        rc = mpgpu::create_context(&j.opt, device, wstream, &c);
+        if (!rc) rc = mpgpu_add_records(c, j.nrec, j.rec_ptr, j.inder, j.glder, nullptr);
+        if (!rc && j.npar > 0) rc = mpgpu_set_steering_parameters(c, j.npar, j.plab, j.pval, j.psig);
+        if (!rc && j.ncon > 0) {
+            std::vector<double> sig(j.ncon, 0.0);
+            rc = mpgpu_add_equations(c, 0, j.ncon, j.cptr, j.clab, j.cfac, j.crhs, sig.data());
         }
         lap(0);
         int32_t ntot = 0, nvar = 0, ncon = 0;
@@ -152,7 +155,7 @@ struct mpgpu_server {
         if (verbose)
             std::fprintf(stderr, "job %lld: nvar %d nrec %lld: create+copy %.1f structure %.1f pass %.1f solve %.1f ms "
                                  "(method %d, itn %d)\n",
-                         (long long)j.id, nvar, (long long)j.rec_ptr.size() - 1, ms[0], ms[1], ms[2], ms[3],
+                         (long long)j.id, nvar, (long long)j.nrec, ms[0], ms[1], ms[2], ms[3],
                          j.sst.method, j.sst.itn);
         if (!rc) {
             std::vector<int32_t> lab(ntot), var(ntot), grp(ntot);
@@ -168,7 +171,6 @@ struct mpgpu_server {
             j.err = mpgpu_last_error(c);
         }
         if (c) mpgpu_destroy(c);
-        (void)ctx;
         j.status = rc;
         j.t_end = clk::now();
     }
@@ -177,41 +179,49 @@ struct mpgpu_server {
 
     void worker()
     {
-        mpgpu_ctx *ctx = nullptr;
         (void)hipSetDevice(device);
         // the host parts of the workers' jobs share the host's cores instead of each taking all
         omp_set_num_threads(omp_threads);
+        // one stream per worker for all its jobs (a stream made and destroyed per job costs
+        // milliseconds); a job's context orders its work and frees its buffers on it
+        hipStream_t wstream = nullptr;
+        if (hipStreamCreateWithFlags(&wstream, hipStreamNonBlocking) != hipSuccess) wstream = nullptr;
+        struct StreamGuard { hipStream_t s; ~StreamGuard() { if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } } guard{wstream};
         for (;;) {
-            std::vector<std::shared_ptr<Job>> batch;
+            std::shared_ptr<Job> j;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] {
-                    if (stop) return true;
-                    return !queue.empty() && (in_use + queue.front()->footprint <= budget || in_use == 0);
-                });
-                if (stop && queue.empty()) return;
-                if (queue.empty()) continue;
-                // admit the head job; group further small jobs behind it while they fit
-                auto head = queue.front();
+                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+                if (queue.empty()) return;   // stop, and nothing left
+                j = queue.front();
                 queue.pop_front();
-                in_use += head->footprint;
-                batch.push_back(head);
-                bool small = head->opt.matsto == MPGPU_FULL;
-                while (small && !queue.empty() && batch.size() < 8 && queue.front()->opt.matsto == MPGPU_FULL &&
-                       in_use + queue.front()->footprint <= budget) {
-                    auto j = queue.front();
-                    queue.pop_front();
+            }
+            // size the job (a scan of its words), then wait until it fits the budget; a job
+            // alone may use more than the budget's remainder when nothing else runs
+            j->footprint = estimate_footprint(*j);
+            bool admitted = false;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                if (j->footprint <= budget) {
+                    cv_budget.wait(lk, [&] { return in_use + j->footprint <= budget || in_use == 0; });
                     in_use += j->footprint;
-                    batch.push_back(j);
+                    admitted = true;
                 }
             }
-            for (auto &j : batch) run_job(*j, ctx);
+            if (admitted) {
+                run_job(*j, wstream);
+            } else {
+                j->t_start = j->t_end = clk::now();
+                j->status = 30;
+                j->err = "job larger than the memory budget";
+            }
             {
                 std::lock_guard<std::mutex> lk(mu);
-                for (auto &j : batch) { in_use -= j->footprint; j->done = true; }
+                if (admitted) in_use -= j->footprint;
+                j->done = true;
             }
             cv_done.notify_all();
-            cv_work.notify_all();
+            cv_budget.notify_all();
         }
     }
 };
@@ -237,31 +247,31 @@ int mpgpu_server_submit(mpgpu_server *s, const mpgpu_options *opt, int64_t nrec,
                         int64_t *job_id)
 {
     if (!s || !opt || nrec < 0) return 30;
+    if (nrec > 0 && (!rec_ptr || !inder || !glder)) return 30;
+    if (ncon > 0 && (!cptr || !clab || !cfac || !crhs)) return 30;
     auto j = std::make_shared<Job>();
-    // the job has arrived when the caller hands it over, before it is copied and sized
     j->t_arrival = clk::now();
     j->opt = *opt;
-    int64_t w0 = rec_ptr[0], w1 = rec_ptr[nrec];
-    j->rec_ptr.resize(nrec + 1);
-    for (int64_t r = 0; r <= nrec; r++) j->rec_ptr[r] = rec_ptr[r] - w0;
-    j->inder.assign(inder + w0, inder + w1);
-    j->glder.assign(glder + w0, glder + w1);
+    j->nrec = nrec;
+    j->rec_ptr = rec_ptr;
+    j->inder = inder;
+    j->glder = glder;
     if (npar > 0) {
-        j->plab.assign(plab, plab + npar);
-        j->pval.assign(pval, pval + npar);
-        j->psig.assign(psig, psig + npar);
+        j->npar = npar;
+        j->plab = plab;
+        j->pval = pval;
+        j->psig = psig;
     }
     if (ncon > 0) {
-        j->cptr.assign(cptr, cptr + ncon + 1);
-        j->clab.assign(clab + cptr[0], clab + cptr[ncon]);
-        j->cfac.assign(cfac + cptr[0], cfac + cptr[ncon]);
-        for (auto &p : j->cptr) p -= cptr[0];
-        j->crhs.assign(crhs, crhs + ncon);
+        j->ncon = ncon;
+        j->cptr = cptr;
+        j->clab = clab;
+        j->cfac = cfac;
+        j->crhs = crhs;
     }
-    j->footprint = estimate_footprint(*j);
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (j->footprint > s->budget) return 30;
+        if (s->stop) return 30;
         j->id = s->next_id++;
         s->jobs[j->id] = j;
         s->queue.push_back(j);

@@ -353,6 +353,7 @@ class Server:
         if rc:
             raise MpgpuError(rc, "server create failed")
         self.h = h
+        self._keep = {}
 
     def submit(self, records, params=None, constraints=None, **options):
         rp, ii, gg = records
@@ -386,7 +387,9 @@ class Server:
                                         _p(pl, I32), _p(pv, D), _p(ps, D), ncon, _p(cp, I64), _p(cl, I32), _p(cf, D),
                                         _p(cr, D), C.byref(jid))
         if rc:
-            raise MpgpuError(rc, "submit failed (job larger than the memory budget?)")
+            raise MpgpuError(rc, "submit failed")
+        # the server reads these arrays while the job runs: keep them until it is waited for
+        self._keep[jid.value] = (rp, ii, gg, pl, pv, ps, cp, cl, cf, cr)
         return jid.value
 
     def wait(self, job_id, capacity=0):
@@ -395,6 +398,7 @@ class Server:
         cor = np.zeros(max(capacity, 1))
         rc = self.L.mpgpu_server_wait(self.h, job_id, capacity, _p(lab, I32) if capacity else None,
                                       _p(cor, D) if capacity else None, C.byref(st))
+        self._keep.pop(job_id, None)
         out = {f: getattr(st, f) for f, _ in JobStats._fields_}
         n = min(capacity, st.nvar)
         return rc, out, lab[:n], cor[:n]

@@ -1,9 +1,8 @@
-"""GPU: record ingest.  Records go to the device as they are added: small inputs straight from
-the caller's arrays, files and large inputs through two pinned staging buffers (4 Mi words each)
-whose copies overlap the reading of the next chunk (Mille files, src/pede.f90:1714-2075,
-read through zlib).  Every route must give the same device records, so the structure and the
-pass agree exactly: one large staged call against many small direct ones, a file read against
-the in-memory records, FP32 and FP64 records mixed in one file (the values held so far are
+"""GPU: record ingest.  Records go to the device as they are added: arrays straight from the
+caller's memory, Mille files (src/pede.f90:1714-2075, read through zlib) through two pinned
+staging buffers (4 Mi words each) whose copies overlap the reading of the next chunk.  Every
+route must give the same device records, so the structure and the pass agree exactly: one large
+call against many small ones, a file read of several staging chunks against the in-memory records, FP32 and FP64 records mixed in one file (the values held so far are
 widened when the first FP64 record arrives), and a truncated file whose records are dropped
 while the earlier ones stay."""
 import numpy as np
@@ -56,12 +55,12 @@ def test_staged_ingest_equals_direct(tmp_path):
     rp, ii, gg, _ = mpgpu.pixel_records(61, 6, 8, 6, 60000, 8, 30)
     assert rp[-1] > 2 * CHUNK                      # three staging chunks at least
     P = mpgpu.Pede(**_opts())
-    P.add_records(rp, ii, gg)                      # one call: staged
+    P.add_records(rp, ii, gg)                      # one call
     a = _pass(P)
     P.close()
     Q = mpgpu.Pede(**_opts())
     step = 2000
-    for r0 in range(0, len(rp) - 1, step):         # many small calls: direct copies
+    for r0 in range(0, len(rp) - 1, step):         # many small calls (the device arrays grow)
         r1 = min(r0 + step, len(rp) - 1)
         w0, w1 = int(rp[r0]), int(rp[r1])
         Q.add_records(rp[r0:r1 + 1] - w0, ii[w0:w1], gg[w0:w1])

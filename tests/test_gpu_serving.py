@@ -43,8 +43,16 @@ def test_alone_vs_in_the_mix():
 
 
 def test_budget_rejects_oversized_job():
+    """A job larger than the whole budget ends with status 30 (the worker that takes it sizes
+    it); the server keeps serving the next ones."""
     rec, par, cons = _job(2, 4, 3, 3, 500)
     S = mpgpu.Server(nworkers=1, budget_bytes=1 << 20)
-    with pytest.raises(mpgpu.MpgpuError):
-        S.submit(rec, par, cons, matsto=mpgpu.FULL, metsol=mpgpu.DECOMPOSITION, mreqenf=5, mcount=1)
+    j = S.submit(rec, par, cons, matsto=mpgpu.FULL, metsol=mpgpu.DECOMPOSITION, mreqenf=5, mcount=1)
+    rc, st, _, _ = S.wait(j)
+    assert rc == 30 and st["status"] == 30
+    S.close()
+    S = mpgpu.Server(nworkers=2, budget_bytes=8 << 30)
+    j = S.submit(rec, par, cons, matsto=mpgpu.FULL, metsol=mpgpu.DECOMPOSITION, mreqenf=5, mcount=1)
+    rc, st, lab, x = S.wait(j, capacity=10000)
+    assert rc == 0 and st["nvar"] == len(lab) > 0
     S.close()
