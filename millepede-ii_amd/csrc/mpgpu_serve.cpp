@@ -8,6 +8,7 @@
 #include "mpgpu_internal.h"
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <condition_variable>
@@ -38,6 +39,7 @@ struct Job {
     const int32_t *clab = nullptr;
     const double *cfac = nullptr, *crhs = nullptr;
     int64_t footprint = -1;   // device bytes, sized by the worker that takes the job
+    double size_ms = 0;       // time of the sizing
     clk::time_point t_arrival, t_start, t_end;
     // results
     int status = -1;
@@ -60,28 +62,47 @@ int64_t estimate_footprint(const Job &j)
     const int64_t nw = nrec > 0 ? j.rec_ptr[nrec] - j.rec_ptr[0] : 0;
     int64_t rec = 12 * nw + 64 * nrec;
     // distinct labels of the global words: after the second zero-index word of a measurement
-    // (measurement, locals, sigma, globals; src/mille.f90:27-59)
-    std::vector<int32_t> lab;
-    lab.reserve((size_t)nw / 2);
+    // (measurement, locals, sigma, globals; src/mille.f90:27-59).  Two passes over the records
+    // (the worker's OpenMP threads): their global words and label range, then a bitmap
+    const int64_t *rp = j.rec_ptr;
+    const int32_t *I = j.inder;
     int64_t upd = 0;
+    int32_t lmin = INT32_MAX, lmax = INT32_MIN;
+#pragma omp parallel for schedule(static, 256) reduction(+ : upd) reduction(min : lmin) reduction(max : lmax)
     for (int64_t r = 0; r < nrec; r++) {
         int zeros = 0;
         int64_t g = 0;
-        for (int64_t w = j.rec_ptr[r] + 1; w < j.rec_ptr[r + 1]; w++) {
-            if (j.inder[w] == 0) { zeros++; continue; }
-            if (zeros > 0 && zeros % 2 == 0) { lab.push_back(j.inder[w]); g++; }
+        for (int64_t w = rp[r] + 1; w < rp[r + 1]; w++) {
+            const int32_t l = I[w];
+            if (l == 0) { zeros++; continue; }
+            if (zeros > 0 && zeros % 2 == 0) {
+                g++;
+                lmin = std::min(lmin, l);
+                lmax = std::max(lmax, l);
+            }
         }
         upd += 8 * g * (g + 1) / 2;
     }
     if (j.opt.matsto == MPGPU_FULL) upd = 0;
     upd = std::min<int64_t>(upd, (int64_t)128 << 30);
-    // number of distinct labels: a bitmap over their range (linear, on the submitting thread)
     int64_t nlab = 0;
-    if (!lab.empty()) {
-        const auto mm = std::minmax_element(lab.begin(), lab.end());
-        const int64_t lo = *mm.first, span = (int64_t)*mm.second - lo + 1;
+    if (lmax >= lmin) {
+        const int64_t span = (int64_t)lmax - lmin + 1;
         std::vector<uint64_t> bits((span + 63) / 64, 0);
-        for (int32_t l : lab) bits[(l - lo) >> 6] |= 1ull << ((l - lo) & 63);
+        uint64_t *B = bits.data();
+#pragma omp parallel for schedule(static, 256)
+        for (int64_t r = 0; r < nrec; r++) {
+            int zeros = 0;
+            for (int64_t w = rp[r] + 1; w < rp[r + 1]; w++) {
+                const int32_t l = I[w];
+                if (l == 0) { zeros++; continue; }
+                if (zeros > 0 && zeros % 2 == 0) {
+                    const int64_t o = (int64_t)l - lmin;
+                    const uint64_t m = 1ull << (o & 63);
+                    if ((__atomic_load_n(&B[o >> 6], __ATOMIC_RELAXED) & m) == 0) __atomic_fetch_or(&B[o >> 6], m, __ATOMIC_RELAXED);
+                }
+            }
+        }
         for (uint64_t b : bits) nlab += __builtin_popcountll(b);
     }
     int64_t nall = nlab + (int64_t)j.ncon;
@@ -153,9 +174,9 @@ struct mpgpu_server {
         if (!rc) rc = mpgpu_solve(c, nullptr, x.data(), &j.sst);
         lap(3);
         if (verbose)
-            std::fprintf(stderr, "job %lld: nvar %d nrec %lld: create+copy %.1f structure %.1f pass %.1f solve %.1f ms "
+            std::fprintf(stderr, "job %lld: nvar %d nrec %lld: size %.1f create+copy %.1f structure %.1f pass %.1f solve %.1f ms "
                                  "(method %d, itn %d)\n",
-                         (long long)j.id, nvar, (long long)j.nrec, ms[0], ms[1], ms[2], ms[3],
+                         (long long)j.id, nvar, (long long)j.nrec, j.size_ms, ms[0], ms[1], ms[2], ms[3],
                          j.sst.method, j.sst.itn);
         if (!rc) {
             std::vector<int32_t> lab(ntot), var(ntot), grp(ntot);
@@ -198,7 +219,9 @@ struct mpgpu_server {
             }
             // size the job (a scan of its words), then wait until it fits the budget; a job
             // alone may use more than the budget's remainder when nothing else runs
+            const clk::time_point t_size = clk::now();
             j->footprint = estimate_footprint(*j);
+            j->size_ms = std::chrono::duration<double, std::milli>(clk::now() - t_size).count();
             bool admitted = false;
             {
                 std::unique_lock<std::mutex> lk(mu);
