@@ -751,7 +751,7 @@ __global__ __launch_bounds__(64 * W) void k_bsymv_rows_st(int32_t ngroup, const 
 // order, the column groups of step t + 3, the x values of step t + 2 and the matrix values of
 // step t + 2, all unconditionally (clamped indices; the transposed products of lanes without a
 // block go to a spare slot past the end of tb), so that every wait is for the step it needs.
-template <int G, int W>
+template <int G, int W, bool V2>
 __global__ __launch_bounds__(64 * W) void k_bsymv_rows_u(int32_t ngroup, const int64_t *__restrict__ brow,
                                                                const int32_t *__restrict__ bcol,
                                                                const int64_t *__restrict__ boff,
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(64 * W) void k_bsymv_rows_u(int32_t ngroup, const i
                                                                int64_t tb_spare)
 {
     constexpr int S = G * G;                   // values per block
-    __shared__ double st[W][8 * S];
+    __shared__ __align__(16) double st[W][8 * S];
     __shared__ double sx[W][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ga = blockIdx.x * W + w;
@@ -781,7 +781,10 @@ __global__ __launch_bounds__(64 * W) void k_bsymv_rows_u(int32_t ngroup, const i
     for (int r = 0; r < G; r++) xa[r] = x[fa + r];
     const int64_t s_row = boff[b0];
     const int64_t nsteps = (be - b0 + 7) / 8;
-    constexpr int NV = (8 * S + 63) / 64;      // matrix values per lane and step
+    // matrix values per lane and step; with V2 (even block size, so every step starts on 16
+    // bytes) they are loaded two at a time
+    constexpr int NV2 = (4 * S + 63) / 64;
+    constexpr int NV = V2 ? 2 * NV2 : (8 * S + 63) / 64;
     struct StepRegs {
         double v[NV], x;
         int col;                                // column group of the lane's block k (step + 1 ahead)
@@ -792,8 +795,18 @@ __global__ __launch_bounds__(64 * W) void k_bsymv_rows_u(int32_t ngroup, const i
     auto ld_v = [&](int64_t t, StepRegs &R) {
         const int len = (int)(min((int64_t)8, be - (b0 + 8 * t)) * S);
         const int64_t s0 = s_row + 8 * t * S;
+        if constexpr (V2) {
+            const double2 *a2 = reinterpret_cast<const double2 *>(a + s0);
 #pragma unroll
-        for (int i = 0; i < NV; i++) R.v[i] = a[s0 + min(lane + 64 * i, len - 1)];
+            for (int i = 0; i < NV2; i++) {
+                const double2 u = a2[min(lane + 64 * i, len / 2 - 1)];
+                R.v[2 * i] = u.x;
+                R.v[2 * i + 1] = u.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NV; i++) R.v[i] = a[s0 + min(lane + 64 * i, len - 1)];
+        }
     };
     if (nsteps == 0) {                          // a row with its diagonal block only
         accy += __shfl_xor(accy, 8, 64);
@@ -817,8 +830,16 @@ __global__ __launch_bounds__(64 * W) void k_bsymv_rows_u(int32_t ngroup, const i
         const int len = kn * S;
         // (no branch in the loop body: lanes past the step's values write the value of its last
         // entry to that entry again, which the loads gave them)
+        if constexpr (V2) {
 #pragma unroll
-        for (int i = 0; i < NV; i++) st[w][min(lane + 64 * i, max(len, 1) - 1)] = R.v[i];
+            for (int i = 0; i < NV2; i++) {
+                const int q = min(lane + 64 * i, max(len / 2, 1) - 1);
+                *reinterpret_cast<double2 *>(&st[w][2 * q]) = make_double2(R.v[2 * i], R.v[2 * i + 1]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NV; i++) st[w][min(lane + 64 * i, max(len, 1) - 1)] = R.v[i];
+        }
         sx[w][lane] = R.x;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1813,38 +1834,53 @@ __global__ void k_zero(int64_t n, double *y)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = 0.0;
 }
 // Product variants with the same arithmetic (same summation order, bit-identical results), for
-// the autotuner: kind (0 the uniform-group rows pass, 1 the general staged rows pass) * 16 + W
-// wavefronts per workgroup
+// the autotuner: kind (0 the uniform-group rows pass, 1 the general staged rows pass, 2 the
+// uniform-group rows pass with 16-byte loads, for even group sizes) * 16 + W wavefronts per
+// workgroup
 int block_symv_variants(int32_t ugrp, int32_t maxgrp, int *out)
 {
     int n = 0;
     if (maxgrp > 8) return 0;
     for (int w : {2, 4, 8}) {
         if (ugrp >= 1 && ugrp <= 8) out[n++] = 0 * 16 + w;
+        if (ugrp >= 2 && ugrp <= 8 && ugrp % 2 == 0) out[n++] = 2 * 16 + w;
         out[n++] = 1 * 16 + w;
     }
     return n;
 }
-template <int G, int W>
+template <int G, int W, bool V2>
 static void launch_rows_u(int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff, const double *a,
                           const double *x, double *y, double *tb, int64_t spare, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_bsymv_rows_u<G, W>), dim3((unsigned)((ngroup + W - 1) / W)), dim3(64 * W), 0, s, ngroup, brow,
-                       bcol, boff, a, x, y, tb, spare);
+    hipLaunchKernelGGL((k_bsymv_rows_u<G, W, V2>), dim3((unsigned)((ngroup + W - 1) / W)), dim3(64 * W), 0, s, ngroup,
+                       brow, bcol, boff, a, x, y, tb, spare);
 }
 template <int W>
-static void launch_rows_u_g(int32_t g, int32_t ngroup, const int64_t *brow, const int32_t *bcol, const int64_t *boff,
-                            const double *a, const double *x, double *y, double *tb, int64_t spare, hipStream_t s)
+static void launch_rows_u_g(int32_t g, bool v2, int32_t ngroup, const int64_t *brow, const int32_t *bcol,
+                            const int64_t *boff, const double *a, const double *x, double *y, double *tb, int64_t spare,
+                            hipStream_t s)
 {
     switch (g) {
-    case 1: launch_rows_u<1, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
-    case 2: launch_rows_u<2, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
-    case 3: launch_rows_u<3, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
-    case 4: launch_rows_u<4, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
-    case 5: launch_rows_u<5, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
-    case 6: launch_rows_u<6, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
-    case 7: launch_rows_u<7, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
-    default: launch_rows_u<8, W>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 1: launch_rows_u<1, W, false>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 2:
+        if (v2) launch_rows_u<2, W, true>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        else launch_rows_u<2, W, false>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        break;
+    case 3: launch_rows_u<3, W, false>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 4:
+        if (v2) launch_rows_u<4, W, true>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        else launch_rows_u<4, W, false>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        break;
+    case 5: launch_rows_u<5, W, false>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    case 6:
+        if (v2) launch_rows_u<6, W, true>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        else launch_rows_u<6, W, false>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        break;
+    case 7: launch_rows_u<7, W, false>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s); break;
+    default:
+        if (v2) launch_rows_u<8, W, true>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        else launch_rows_u<8, W, false>(ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+        break;
     }
 }
 template <int W>
@@ -1867,13 +1903,15 @@ void l_block_symv(int32_t ngroup, const int64_t *brow, const int32_t *bcf, const
         // block), then their sums over the column lists
         const bool uni = ugrp >= 1 && ugrp <= 8 && bcol;
         int kind = variant >= 0 ? variant / 16 : (uni ? 0 : 1), w = variant >= 0 ? variant % 16 : 4;
+        if (kind == 2 && (!uni || ugrp % 2 != 0)) kind = 0;
         if (kind == 0 && !uni) kind = 1;
-        const int tbw = kind == 0 ? ugrp : maxgrp;
-        if (kind == 0) {
+        const int tbw = kind != 1 ? ugrp : maxgrp;
+        if (kind != 1) {
             const int64_t spare = (int64_t)ugrp * nblock;
-            if (w == 2) launch_rows_u_g<2>(ugrp, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
-            else if (w == 8) launch_rows_u_g<8>(ugrp, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
-            else launch_rows_u_g<4>(ugrp, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+            const bool v2 = kind == 2;
+            if (w == 2) launch_rows_u_g<2>(ugrp, v2, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+            else if (w == 8) launch_rows_u_g<8>(ugrp, v2, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
+            else launch_rows_u_g<4>(ugrp, v2, ngroup, brow, bcol, boff, a, x, y, tb, spare, s);
         } else {
             if (w == 2) launch_rows_st<2>(ngroup, brow, bcf, boff, gfirst, a, x, y, tb, tbw, s);
             else if (w == 8) launch_rows_st<8>(ngroup, brow, bcf, boff, gfirst, a, x, y, tb, tbw, s);

@@ -586,6 +586,7 @@ static int32_t band_info(const int32_t *inder, const int64_t *eq_sl, const uint3
 // and wavefronts per workgroup; bit-identical results, so the choice never changes a solution)
 // are timed on this structure, one warm-up and three timed products each (HIP events on the
 // context's stream; the time does not depend on the values), and the fastest median is kept.
+static const char *kind_name(int kind) { return kind == 0 ? "rows_u" : kind == 2 ? "rows_u16" : "rows_st"; }
 static void tune_block_symv(mpgpu_ctx *c)
 {
     c->spmv_variant = -1;
@@ -634,12 +635,12 @@ static void tune_block_symv(mpgpu_ctx *c)
             HIPCHK(hipEventElapsedTime(&ms[k], e0, e1));
         }
         std::sort(ms, ms + 3);
-        std::snprintf(buf, sizeof buf, "%s%s/w%d %.3f ms", q ? ", " : "", cand[q] / 16 == 0 ? "rows_u" : "rows_st",
+        std::snprintf(buf, sizeof buf, "%s%s/w%d %.3f ms", q ? ", " : "", kind_name(cand[q] / 16),
                       cand[q] % 16, ms[1]);
         c->tuning += buf;
         if (ms[1] < best) { best = ms[1]; c->spmv_variant = cand[q]; }
     }
-    std::snprintf(buf, sizeof buf, "; chosen %s/w%d", c->spmv_variant / 16 == 0 ? "rows_u" : "rows_st",
+    std::snprintf(buf, sizeof buf, "; chosen %s/w%d", kind_name(c->spmv_variant / 16),
                   c->spmv_variant % 16);
     c->tuning += buf;
     if (std::getenv("MPGPU_VERBOSE")) std::fprintf(stderr, "block product autotuner: %s\n", c->tuning.c_str());

@@ -138,7 +138,7 @@ def test_block_product_variants_identical(monkeypatch):
     rp, ii, gg, _ = mpgpu.pixel_records(41, 4, 6, 3, 4000, 3, 7)
     x = np.random.default_rng(3).normal(size=1)
     ys = {}
-    for v in (2, 4, 8, 16 + 2, 16 + 4, 16 + 8, -1):
+    for v in (2, 4, 8, 16 + 2, 16 + 4, 16 + 8, 32 + 2, 32 + 4, 32 + 8, -1):
         if v >= 0:
             monkeypatch.setenv("MPGPU_SPMV_VARIANT", str(v))
         else:
