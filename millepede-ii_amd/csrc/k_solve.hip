@@ -2,6 +2,8 @@
 // matrix-vector products for packed and group-block storage, MINRES vector algebra and the
 // MINRES preconditioners.
 #include "mpgpu_device.h"
+#include "mpgpu_internal.h"
+#include <climits>
 #include <mutex>
 #include <float.h>
 
@@ -1549,6 +1551,56 @@ __global__ __launch_bounds__(256) void k_lltdec_w(int n, double *c, const int *i
 }
 
 
+// One row of the band factorization (lltdec, src/mpnum.f90:2155-2230): row k (slot s = (k-1) % MW,
+// first column mk) from its band values r (element (k, k-MW+1+o) at r[o], zero before mk) and the
+// previous MW-1 factor rows in ring (row j in slot (j-1) % MW).  Returns 1 when the pivot is not
+// above eps times the diagonal and the row is dropped (a rank defect), else 0.  The sequential and
+// the chunk-parallel kernels both use it, so their results agree bit for bit.
+template <int MW>
+__device__ __forceinline__ int llt_row(int k, int mk, int s, double (&r)[MW], const double (&ring)[MW][MW], int iopt)
+{
+    const double eps = 16.0 * DBL_EPSILON;
+    if (k == 1) {
+        double v = r[MW - 1];
+        int bad = 0;
+        if (v > 0.0) v = 1.0 / sqrt(v);
+        else { v = 0.0; bad = 1; }
+        r[MW - 1] = v;
+        return bad;
+    }
+#pragma unroll
+    for (int jo = 0; jo < MW - 1; jo++) {
+        const int sj = (s + 1 + jo) % MW;
+        const bool valid = k - MW + 1 + jo >= mk;
+        double ckj = r[jo];
+#pragma unroll
+        for (int io = 0; io < jo; io++) ckj = ckj - r[io] * (valid ? ring[sj][MW - 1 - (jo - io)] : 0.0);
+        r[jo] = ckj * (valid ? ring[sj][MW - 1] : 0.0);
+    }
+    const double diag = r[MW - 1];
+    double ckk = r[MW - 1];
+#pragma unroll
+    for (int io = 0; io < MW - 1; io++) ckk = ckk - r[io] * r[io];
+    if (ckk > eps * diag) {
+        r[MW - 1] = 1.0 / sqrt(ckk);
+        return 0;
+    }
+#pragma unroll
+    for (int o = 0; o < MW; o++) r[o] = 0.0;
+    if (iopt > 0 && diag > 0.0) {
+        r[MW - 1] = 1.0 / sqrt(diag);
+        return 0;
+    }
+    return 1;
+}
+// rank-defect code of lltdec from the dropped rows: 0 none, -k for one (row k), else their number
+__device__ __forceinline__ int llt_nr(int nr, int k)
+{
+    if (nr == 0) return -k;
+    if (nr < 0) nr = 1;
+    return nr + 1;
+}
+
 // The same recurrence with the tile's rows staged densely: the lanes copy row k's band (MW values,
 // element (k, k-MW+1+o) at d[(k-k0)*MW + o], zero before the row's first column) and its first
 // column into LDS, lane 0 reads and writes rows at fixed offsets, and the lanes scatter the
@@ -1561,7 +1613,6 @@ __global__ __launch_bounds__(64) void k_lltdec_d(int n, double *c, const int *in
     __shared__ double d[ROWS * MW];
     __shared__ int mkr[ROWS];
     const int tid = threadIdx.x;
-    const double eps = 16.0 * DBL_EPSILON;
     double ring[MW][MW];                  // slot s: row j with (j-1) % MW == s, element (j, j-MW+1+o) at [s][o]
 #pragma unroll
     for (int q = 0; q < MW; q++)
@@ -1590,36 +1641,7 @@ __global__ __launch_bounds__(64) void k_lltdec_d(int n, double *c, const int *in
                     double r[MW];
 #pragma unroll
                     for (int o = 0; o < MW; o++) r[o] = Pk[o];
-                    if (k == 1) {
-                        double v = r[MW - 1];
-                        if (v > 0.0) v = 1.0 / sqrt(v);
-                        else { v = 0.0; nr = -1; }
-                        r[MW - 1] = v;
-                    } else {
-#pragma unroll
-                        for (int jo = 0; jo < MW - 1; jo++) {
-                            const int sj = (s + 1 + jo) % MW;
-                            const bool valid = k - MW + 1 + jo >= mk;
-                            double ckj = r[jo];
-#pragma unroll
-                            for (int io = 0; io < jo; io++) ckj = ckj - r[io] * (valid ? ring[sj][MW - 1 - (jo - io)] : 0.0);
-                            r[jo] = ckj * (valid ? ring[sj][MW - 1] : 0.0);
-                        }
-                        const double diag = r[MW - 1];
-                        double ckk = r[MW - 1];
-#pragma unroll
-                        for (int io = 0; io < MW - 1; io++) ckk = ckk - r[io] * r[io];
-                        if (ckk > eps * diag) r[MW - 1] = 1.0 / sqrt(ckk);
-                        else {
-#pragma unroll
-                            for (int o = 0; o < MW; o++) r[o] = 0.0;
-                            if (iopt > 0 && diag > 0.0) r[MW - 1] = 1.0 / sqrt(diag);
-                            else {
-                                if (nr == 0) nr = -k;
-                                else { if (nr < 0) nr = 1; nr++; }
-                            }
-                        }
-                    }
+                    if (llt_row<MW>(k, mk, s, r, ring, iopt)) nr = llt_nr(nr, k);
 #pragma unroll
                     for (int o = 0; o < MW; o++) {
                         Pk[o] = r[o];
@@ -1980,6 +2002,126 @@ void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, cons
     hipLaunchKernelGGL(k_presols_b, dim3(nblk(n)), dim3(TPB), 0, s, (int64_t)n, cu, x);
 }
 
+// Chunk-parallel band factorization with the result of the sequential one, bit for bit.  The
+// rows are cut into chunks of ch rows, one thread each.  Phase 1 factors every chunk from zero
+// rows before it (exact for the first chunk); the error this makes in the later chunks decays
+// along them, since each row depends on the matrix and the MW-1 rows before it only.  Phase 2
+// redoes every later chunk from the phase-1 rows before it and stops at the first MW-1
+// consecutive rows equal bit for bit to phase 1 (and of the same dropped-or-not status): from
+// there the phase-1 rows are the sequential ones.  A chunk that ends before that sets *unconv,
+// and the sequential kernel runs instead.  L: n x MW dense rows, bad: the dropped rows.
+template <int MW>
+__global__ __launch_bounds__(64) void k_lltdec_chunks(int n, int ch, const double *__restrict__ c,
+                                                      const int *__restrict__ india, int iopt, double *L, int8_t *bad,
+                                                      int phase, int *unconv)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k0l = (int64_t)p * ch + 1;
+    if (k0l > n || (phase == 2 && p == 0)) return;
+    const int k0 = (int)k0l, k1 = (int)min((int64_t)n, k0l + ch - 1);
+    double ring[MW][MW];
+#pragma unroll
+    for (int q = 0; q < MW; q++)
+#pragma unroll
+        for (int o = 0; o < MW; o++) ring[q][o] = 0.0;
+    if (phase == 2)
+        for (int j = max(1, k0 - MW + 1); j < k0; j++) {
+            const int sj = (j - 1) % MW;
+#pragma unroll
+            for (int o = 0; o < MW; o++) ring[sj][o] = L[(int64_t)(j - 1) * MW + o];
+        }
+    int match = 0;
+    for (int k = k0; k <= k1; k++) {
+        const int indk = india[k - 1], indk1 = k > 1 ? india[k - 2] : 0;
+        const int mk = k - indk + indk1 + 1;
+        double r[MW];
+#pragma unroll
+        for (int o = 0; o < MW; o++) {
+            const int col = k - MW + 1 + o;
+            r[o] = col >= mk ? c[indk - k + col - 1] : 0.0;
+        }
+        const int s = (k - 1) % MW;
+        const int b = llt_row<MW>(k, mk, s, r, ring, iopt);
+#pragma unroll
+        for (int o = 0; o < MW; o++) ring[s][o] = r[o];
+        double *Lk = L + (int64_t)(k - 1) * MW;
+        if (phase == 2) {
+            bool same = bad[k - 1] == b;
+#pragma unroll
+            for (int o = 0; o < MW; o++) same = same && __double_as_longlong(Lk[o]) == __double_as_longlong(r[o]);
+            if (same) {
+                if (++match >= MW - 1) return;
+                continue;
+            }
+            match = 0;
+        }
+#pragma unroll
+        for (int o = 0; o < MW; o++) Lk[o] = r[o];
+        bad[k - 1] = (int8_t)b;
+    }
+    if (phase == 2 && k1 < n) atomicOr(unconv, 1);
+}
+// the factor rows back into pede's band storage; the dropped rows counted (aux[1]) and the first
+// of them (aux[2])
+template <int MW>
+__global__ __launch_bounds__(256) void k_lltdec_store(int n, double *c, const int *india, const double *L,
+                                                      const int8_t *bad, int *aux)
+{
+    int mc = 0, mf = INT_MAX;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x + 1; k <= n; k += gridDim.x * blockDim.x) {
+        const int indk = india[k - 1], indk1 = k > 1 ? india[k - 2] : 0;
+        const int mk = k - indk + indk1 + 1;
+#pragma unroll
+        for (int o = 0; o < MW; o++) {
+            const int col = k - MW + 1 + o;
+            if (col >= mk) c[indk - k + col - 1] = L[(int64_t)(k - 1) * MW + o];
+        }
+        if (bad[k - 1]) { mc++; mf = min(mf, k); }
+    }
+    if (mc) { atomicAdd(&aux[1], mc); atomicMin(&aux[2], mf); }
+}
+// the rank-defect code of lltdec: 0 none, -k for one dropped row k, else their number
+__global__ void k_lltdec_nr(const int *aux, int *nrkd)
+{
+    if (threadIdx.x == 0) *nrkd = aux[1] == 0 ? 0 : (aux[1] == 1 ? -aux[2] : aux[1]);
+}
+
+template <int MW>
+static void lltdec_chunked(int n, double *c, const int *india, int *nrkd, int iopt, double *L, int8_t *bad, int *flag,
+                           hipStream_t s)
+{
+    const int ch = 256;
+    const int nch = (n + ch - 1) / ch;
+    const int init[3] = {0, 0, INT_MAX};   // not-caught-up flag, dropped rows, first dropped row
+    int host_flag = 0;
+    HIPCHK(hipMemcpyAsync(flag, init, sizeof(init), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_lltdec_chunks<MW>, dim3((nch + 63) / 64), dim3(64), 0, s, n, ch, c, india, iopt, L, bad, 1, flag);
+    hipLaunchKernelGGL(k_lltdec_chunks<MW>, dim3((nch + 63) / 64), dim3(64), 0, s, n, ch, c, india, iopt, L, bad, 2, flag);
+    HIPCHK(hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (host_flag == 0) {
+        hipLaunchKernelGGL(k_lltdec_store<MW>, dim3((unsigned)std::min(1024, (n + 255) / 256)), dim3(256), 0, s, n, c, india,
+                           L, bad, flag);
+        hipLaunchKernelGGL(k_lltdec_nr, dim3(1), dim3(64), 0, s, flag, nrkd);
+    } else {
+        hipLaunchKernelGGL(k_lltdec_d<MW>, dim3(1), dim3(64), 0, s, n, c, india, nrkd, iopt);
+    }
+}
+bool lltdec_parallel(int n, int mbw) { return mbw >= 1 && mbw <= 8 && n >= 4096; }
+void l_lltdec_par(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, double *L, int8_t *bad, int *flag,
+                  hipStream_t s)
+{
+    switch (mbw) {
+    case 1: lltdec_chunked<1>(n, c, india, nrkd, iopt, L, bad, flag, s); break;
+    case 2: lltdec_chunked<2>(n, c, india, nrkd, iopt, L, bad, flag, s); break;
+    case 3: lltdec_chunked<3>(n, c, india, nrkd, iopt, L, bad, flag, s); break;
+    case 4: lltdec_chunked<4>(n, c, india, nrkd, iopt, L, bad, flag, s); break;
+    case 5: lltdec_chunked<5>(n, c, india, nrkd, iopt, L, bad, flag, s); break;
+    case 6: lltdec_chunked<6>(n, c, india, nrkd, iopt, L, bad, flag, s); break;
+    case 7: lltdec_chunked<7>(n, c, india, nrkd, iopt, L, bad, flag, s); break;
+    default: lltdec_chunked<8>(n, c, india, nrkd, iopt, L, bad, flag, s); break;
+    }
+}
 void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, hipStream_t s)
 {
     switch (mbw) {

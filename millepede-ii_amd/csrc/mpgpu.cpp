@@ -1804,7 +1804,17 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
     l_get_band(M, P.n, mb, P.india.p, P.band.p, s);
     if (c->elim) elim_band(c, mb, P.india.p, P.band.p);
     P.nrkd.alloc(2);
-    l_lltdec(P.n, P.band.p, P.india.p, P.nrkd.p, c->opt.lprecm, mb, s);
+    if (lltdec_parallel(P.n, mb) && !std::getenv("MPGPU_SEQUENTIAL_LLT")) {
+        DBuf<double> L;
+        DBuf<int8_t> bad;
+        DBuf<int> flag;
+        L.alloc((size_t)P.n * mb);
+        bad.alloc(P.n);
+        flag.alloc(3);
+        l_lltdec_par(P.n, P.band.p, P.india.p, P.nrkd.p, c->opt.lprecm, mb, L.p, bad.p, flag.p, s);
+    } else {
+        l_lltdec(P.n, P.band.p, P.india.p, P.nrkd.p, c->opt.lprecm, mb, s);
+    }
     if (P.m > 0) {
         // constraint blocks: merge overlapping variable ranges of constraints
         std::vector<int> first(P.m), last(P.m);

@@ -171,6 +171,12 @@ void l_precons(int32_t n, int32_t p, const double *diag, double *cu, int *nrkd, 
 void l_presols(int32_t n, int32_t p, const double *cu, const int64_t *cptr, const int32_t *cidx, const double *cval,
                const double *sm, const double *y, double *x, hipStream_t s);
 void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, hipStream_t s);
+// the band factorization by chunks of rows in parallel, with the sequential result bit for bit
+// (falls back to the sequential kernel when a chunk does not catch up); L: n * mbw doubles,
+// bad: n bytes, flag: three ints of device scratch
+bool lltdec_parallel(int n, int mbw);
+void l_lltdec_par(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, double *L, int8_t *bad, int *flag,
+                  hipStream_t s);
 void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
               hipStream_t s);
 void l_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h, const int *hind, int *nrkd2,

@@ -302,6 +302,36 @@ def test_minres_repeatable(mbandw):
     assert runs[0][1] <= 203
 
 
+@pytest.mark.parametrize("lprecm", [0, 1])
+def test_band_factor_chunked_equals_sequential(monkeypatch, lprecm):
+    """The band preconditioner's factorization (lltdec, src/mpnum.f90:2155-2230) by chunks of
+    rows in parallel gives the sequential factor bit for bit: with the same matrix and
+    right-hand side, MINRES returns bitwise the same solution, iteration count and stop code
+    whichever factorization built its preconditioner (4,320 parameters: above the size where
+    the chunked one is used)."""
+    nl, nx, ny = 8, 10, 9
+    rp, ii, gg, _ = mpgpu.pixel_records(27, nl, nx, ny, 20000, 3, 6)
+    cons = mpgpu.pixel_constraints(nl, nx, ny)
+    gam = [(l + 1) * 100000 + (m + 1) * 10 + k for l in range(nl) for m in range(nx * ny) for k in (3, 4, 5, 6)]
+    runs, b = [], None
+    for seq in (False, True):
+        if seq:
+            monkeypatch.setenv("MPGPU_SEQUENTIAL_LLT", "1")
+        else:
+            monkeypatch.delenv("MPGPU_SEQUENTIAL_LLT", raising=False)
+        P = _pede(rp, ii, gg, gam, mpgpu.pixel_presigma(gam), cons, matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES,
+                  mreqenf=5, mcount=1, mbandw=6, lprecm=lprecm, mrestl=1e-12, withmultipliers=1)
+        assert P.nvar >= 4096
+        P.loopn()
+        if b is None:
+            b = P.vector()[0]
+        x, st = P.solve(rhs=b)
+        runs.append((x, st["itn"], st["istop"]))
+        P.close()
+    assert runs[0][1] > 10
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1:] == runs[1][1:]
+
+
 @pytest.mark.parametrize("mbandw,mrmode", [(-1, 0), (-1, 2), (6, 0), (6, 2)])
 def test_minresqlp_sparse_constraints(orc, mbandw, mrmode):
     """MINRES-QLP (mresmode 0: QR steps, QLP once ill-conditioned; 2: QLP only) on the
