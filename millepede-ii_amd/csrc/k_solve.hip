@@ -1717,12 +1717,83 @@ __global__ void k_band_k(int32_t ncon, int n, const double *c, const int *india,
     }
 }
 
-// H = K K^T restricted to overlapping ranges (equdecs products), packed m(m+1)/2, plus lltdec
+// The same forward substitutions with the last MW-1 solution values of a column kept in registers
+// (the same operations in the same order): a row then waits for its own arithmetic only, not for
+// the store and reload of the value before it.
+template <int MW>
+__global__ void k_band_k_r(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff,
+                           double *kmat)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ncon) return;
+    int i0 = cb[2 * j], ns = cb[2 * j + 1];
+    double *X = kmat + koff[j] - 1;
+    const double *C = c - 1;
+    const int *IND = india - 1;
+    int kmax = min(ns, n - i0);
+    double xr[MW];                        // X[jj] at slot (jj - 1) % MW
+#pragma unroll
+    for (int q = 0; q < MW; q++) xr[q] = 0.0;
+    // row k's operands (its band row of the factor, from column k - MW + 1 on, and X[k]) are
+    // loaded one row ahead, so that the loads do not wait for the arithmetic
+    auto load_row = [&](int k, double (&cr)[MW], double &xk0, int &j0) {
+        const int l = k + i0;
+        const int il = IND[l], il1 = l > 1 ? IND[l - 1] : 0;
+        j0 = l > 1 ? max(1, k - il + il1 + 1) : k;
+#pragma unroll
+        for (int d = 0; d < MW; d++) {
+            const int jj = k - (MW - 1) + d;                 // d = MW - 1: the diagonal
+            cr[d] = (jj >= j0 || d == MW - 1) ? C[il - k + jj] : 0.0;
+        }
+        xk0 = X[k];
+    };
+    double cn[MW], xn = 0.0;
+    int jn = 1;
+    if (kmax >= 1) load_row(1, cn, xn, jn);
+    for (int k = 1; k <= kmax; k++) {
+        double cr[MW];
+#pragma unroll
+        for (int d = 0; d < MW; d++) cr[d] = cn[d];
+        double xk = xn;
+        const int j0 = jn;
+        if (k < kmax) load_row(k + 1, cn, xn, jn);
+#pragma unroll
+        for (int d = MW - 1; d >= 1; d--) {                  // jj = k - d, ascending
+            const int jj = k - d;
+            if (jj >= j0) xk = xk - cr[MW - 1 - d] * xr[(jj - 1) % MW];
+        }
+        xk = xk * cr[MW - 1];
+        X[k] = xk;
+        xr[(k - 1) % MW] = xk;
+    }
+}
+
+// H = K K^T restricted to overlapping ranges (equdecs products), packed m(m+1)/2: one thread per
+// pair (j >= k), its sum in index order
+__global__ void k_band_hprod(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h)
+{
+    const int np = (m * m + m) / 2;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < np; t += gridDim.x * blockDim.x) {
+        int j = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((j * j + j) / 2 > t) j--;
+        while (((j + 1) * (j + 1) + (j + 1)) / 2 <= t) j++;
+        int k = t - (j * j + j) / 2;
+        int ij0 = cb[2 * j], ijn = cb[2 * j + 1];
+        int ik0 = cb[2 * k], ikn = cb[2 * k + 1];
+        int lo = max(ij0, ik0), hi = min(ij0 + ijn, ik0 + ikn);
+        double acc = 0.0;
+        for (int q = lo; q < hi; q++) acc += kmat[koff[j] + (q - ij0)] * kmat[koff[k] + (q - ik0)];
+        h[t] = acc;
+    }
+}
+
+// H = K K^T restricted to overlapping ranges (equdecs products), packed m(m+1)/2, plus lltdec;
+// with products, the pairs are summed here (one workgroup), else by k_band_hprod before
 __global__ void k_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h, const int *hind,
-                         int *nrkd2)
+                         int *nrkd2, int products)
 {
     // one workgroup; pairs (j>=k) by threads
-    const int np = (m * m + m) / 2;
+    const int np = products ? (m * m + m) / 2 : 0;
     for (int t = threadIdx.x; t < np; t += blockDim.x) {
         int j = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
         while ((j * j + j) / 2 > t) j--;
@@ -1736,9 +1807,16 @@ __global__ void k_band_h(int32_t m, const int *cb, const int64_t *koff, const do
         h[t] = acc;
     }
     __syncthreads();
+    // the factorization below is one thread's recurrence: H staged in LDS when it fits
+    extern __shared__ double hs[];
+    const int nh = (m * m + m) / 2;
+    const bool lds = nh <= 6144;
+    if (lds)
+        for (int t = threadIdx.x; t < nh; t += blockDim.x) hs[t] = h[t];
+    __syncthreads();
     if (threadIdx.x == 0) {
         // lltdec of the dense m x m lower-packed H (india = i(i+1)/2)
-        double *C = h - 1;
+        double *C = (lds ? hs : h) - 1;
         const int *IND = hind - 1;
         const double eps = 16.0 * DBL_EPSILON;
         int nr = 0;
@@ -1764,6 +1842,9 @@ __global__ void k_band_h(int32_t m, const int *cb, const int64_t *koff, const do
         }
         *nrkd2 = nr;
     }
+    __syncthreads();
+    if (lds)
+        for (int t = threadIdx.x; t < nh; t += blockDim.x) h[t] = hs[t];
 }
 
 // equslvs apply (src/mpnum.f90:2613-2678) as one single-lane kernel: the band recurrences are
@@ -2137,11 +2218,31 @@ void l_lltdec(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, 
     }
 }
 void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
-              hipStream_t s)
-{ if (ncon > 0) hipLaunchKernelGGL(k_band_k, dim3((ncon + 63) / 64), dim3(64), 0, s, ncon, n, c, india, cb, koff, kmat); }
+              int mbw, hipStream_t s)
+{
+    if (ncon <= 0) return;
+    const dim3 g((ncon + 63) / 64), b(64);
+    switch (mbw) {
+    case 1: hipLaunchKernelGGL(k_band_k_r<1>, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    case 2: hipLaunchKernelGGL(k_band_k_r<2>, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    case 3: hipLaunchKernelGGL(k_band_k_r<3>, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    case 4: hipLaunchKernelGGL(k_band_k_r<4>, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    case 5: hipLaunchKernelGGL(k_band_k_r<5>, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    case 6: hipLaunchKernelGGL(k_band_k_r<6>, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    case 7: hipLaunchKernelGGL(k_band_k_r<7>, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    case 8: hipLaunchKernelGGL(k_band_k_r<8>, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    default: hipLaunchKernelGGL(k_band_k, g, b, 0, s, ncon, n, c, india, cb, koff, kmat); break;
+    }
+}
 void l_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h, const int *hind, int *nrkd2,
               hipStream_t s)
-{ if (m > 0) hipLaunchKernelGGL(k_band_h, dim3(1), dim3(TPB), 0, s, m, cb, koff, kmat, h, hind, nrkd2); }
+{
+    if (m <= 0) return;
+    const int np = (m * m + m) / 2;
+    hipLaunchKernelGGL(k_band_hprod, dim3((unsigned)std::min(1024, (np + 63) / 64)), dim3(64), 0, s, m, cb, koff, kmat, h);
+    const size_t lds = np <= 6144 ? sizeof(double) * np : 0;
+    hipLaunchKernelGGL(k_band_h, dim3(1), dim3(TPB), lds, s, m, cb, koff, kmat, h, hind, nrkd2, 0);
+}
 void l_equslvs(int n, int m, const double *c, const int *india, const int *cb, const int64_t *koff, const double *kmat,
                const double *h, const int *hind, double *x, hipStream_t s)
 { hipLaunchKernelGGL(k_equslvs, dim3(1), dim3(64), 0, s, n, m, c, india, cb, koff, kmat, h, hind, x); }

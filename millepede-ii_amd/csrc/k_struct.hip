@@ -762,9 +762,11 @@ void st_label_table(const StructIn &in, const std::vector<int32_t> &extra, Label
     const unsigned long long zero2[2] = {0, 0};
     mm.upload(init, 2, s);
     cnt.upload(zero2, 2, s);
+    // a few thousand workgroups stride over the equations: every wavefront ends with four
+    // atomics on the same words, which one wavefront per 256 equations would queue up
     if (in.neq > 0)
-        hipLaunchKernelGGL(k_label_range, dim3(grid_for(in.neq, 256)), dim3(256), 0, s, in.neq, in.eq_sl, in.eq_n, in.lab,
-                           mm.p, cnt.p);
+        hipLaunchKernelGGL(k_label_range, dim3(std::min<unsigned>(grid_for(in.neq, 256), 4096)), dim3(256), 0, s, in.neq,
+                           in.eq_sl, in.eq_n, in.lab, mm.p, cnt.p);
     int32_t h[2];
     unsigned long long hc[2];
     mm.download(h, 2, s);

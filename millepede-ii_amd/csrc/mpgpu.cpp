@@ -1843,7 +1843,7 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
         for (int i = 1; i <= P.m; i++) hind[i - 1] = (i * i + i) / 2;
         P.hind.upload(hind, s);
         P.h.alloc((P.m * P.m + P.m) / 2);
-        l_band_k(P.m, P.n, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, s);
+        l_band_k(P.m, P.n, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, mb, s);
         l_band_h(P.m, P.cb.p, P.koff.p, P.kmat.p, P.h.p, P.hind.p, P.nrkd.p + 1, s);
     }
     // chunk-parallel band solves (response matrices of 128-row chunks)

@@ -178,7 +178,7 @@ bool lltdec_parallel(int n, int mbw);
 void l_lltdec_par(int n, double *c, const int *india, int *nrkd, int iopt, int mbw, double *L, int8_t *bad, int *flag,
                   hipStream_t s);
 void l_band_k(int32_t ncon, int n, const double *c, const int *india, const int *cb, const int64_t *koff, double *kmat,
-              hipStream_t s);
+              int mbw, hipStream_t s);
 void l_band_h(int32_t m, const int *cb, const int64_t *koff, const double *kmat, double *h, const int *hind, int *nrkd2,
               hipStream_t s);
 void l_equslvs(int n, int m, const double *c, const int *india, const int *cb, const int64_t *koff, const double *kmat,
