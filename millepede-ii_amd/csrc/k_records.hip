@@ -913,10 +913,29 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         int j = t / nl + 1, k = t % nl + 1;
                         int pj = li[sg + j];
                         int a = gpos[pj];
-                        if (a >= 0) {
-                            int ik = li[sl + k];
-                            G[a * nalc + ik - 1] = G[a * nalc + ik - 1] + dw2 * wght * lwv(sg + j) * lwv(sl + k);
+                        if (a < 0) continue;
+                        const int ik = li[sl + k];
+                        // word pairs of one equation that share the element (a label or a local
+                        // index repeated in the equation) are summed by the thread of the first
+                        // such pair, in the reference's (j, k) order
+                        bool own = true, rep = false;
+                        for (int x = 1; x <= ng; x++)
+                            if (x != j && li[sg + x] == pj) { rep = true; own = own && x > j; }
+                        for (int x = 1; x <= nl; x++)
+                            if (x != k && li[sl + x] == ik) { rep = true; own = own && x > k; }
+                        if (!own) continue;
+                        double *ge = &G[a * nalc + ik - 1];
+                        if (!rep) {
+                            *ge = *ge + dw2 * wght * lwv(sg + j) * lwv(sl + k);
+                            continue;
                         }
+                        double v = *ge;
+                        for (int j2 = j; j2 <= ng; j2++) {
+                            if (li[sg + j2] != pj) continue;
+                            for (int k2 = k; k2 <= nl; k2++)
+                                if (li[sl + k2] == ik) v = v + dw2 * wght * lwv(sg + j2) * lwv(sl + k2);
+                        }
+                        *ge = v;
                     }
                     __syncthreads();
                 }

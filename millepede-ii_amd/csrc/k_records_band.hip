@@ -661,14 +661,41 @@ __global__ __launch_bounds__(BNT) void k_records_band(DevRecords R, DevModel M, 
                 const double wght = lw[i];
                 for (int t = tid; t < ng * nl; t += BNT) {
                     int j = t / nl + 1, k = t % nl + 1;
-                    int a = gpos[R.widx[sg + j]];
+                    const int pj = R.widx[sg + j];
+                    int a = gpos[pj];
+                    if (a < 0) continue;
+                    const int ik = R.widx[sl + k];
+                    // word pairs of one equation that share the element (a label or a local
+                    // index repeated in the equation) are summed by the thread of the first
+                    // such pair, in the reference's (j, k) order
+                    bool own = true, rep = false;
+                    for (int x = 1; x <= ng; x++)
+                        if (x != j && R.widx[sg + x] == pj) { rep = true; own = own && x > j; }
+                    for (int x = 1; x <= nl; x++)
+                        if (x != k && R.widx[sl + x] == ik) { rep = true; own = own && x > k; }
                     // zero derivatives add nothing and are not part of the support masks
-                    if (a < 0 || bval<T>(R, sg + j) == 0.0) continue;
-                    int kk = R.widx[sl + k] - 1;
+                    if (!own || (!rep && bval<T>(R, sg + j) == 0.0)) continue;
+                    const int kk = ik - 1;
                     int pos = 0;
                     for (int w = 0; w < (kk >> 6); w++) pos += __popcll(mask[a * NWM + w]);
                     pos += __popcll(mask[a * NWM + (kk >> 6)] & ((1ull << (kk & 63)) - 1ull));
-                    gval[gptr[a] + pos] = gval[gptr[a] + pos] + dw2 * wght * bval<T>(R, sg + j) * bval<T>(R, sl + k);
+                    double *ge = &gval[gptr[a] + pos];
+                    if (!rep) {
+                        *ge = *ge + dw2 * wght * bval<T>(R, sg + j) * bval<T>(R, sl + k);
+                        continue;
+                    }
+                    // the element exists once some pair of the run has a non-zero derivative
+                    double v = 0.0;
+                    bool any = false;
+                    for (int j2 = j; j2 <= ng; j2++) {
+                        if (R.widx[sg + j2] != pj || bval<T>(R, sg + j2) == 0.0) continue;
+                        for (int k2 = k; k2 <= nl; k2++) {
+                            if (R.widx[sl + k2] != ik) continue;
+                            if (!any) { v = *ge; any = true; }
+                            v = v + dw2 * wght * bval<T>(R, sg + j2) * bval<T>(R, sl + k2);
+                        }
+                    }
+                    if (any) *ge = v;
                 }
                 __syncthreads();
             }

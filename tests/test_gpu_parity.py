@@ -672,23 +672,45 @@ def test_label_maps_repeated_labels(orc, matsto):
     pede keeps the repeat in the sorted equation list (src/pede.f90:2259-2270), which counts
     both entries and splits the parameter group at that label (pargrp, 2357-2386).  Index maps
     against the checker (itself pinned to pargrp's procedure), and the data pass against the
-    oracle."""
+    oracle: matrix, vector, chi^2, ndf, rejects and counters."""
     rp, ii, gg, _ = mpgpu.pixel_records(31, 4, 5, 3, 3000, 3, 6)
     rp, ii, gg = _duplicate_first_global(rp, ii, gg)
     P = _pede(rp, ii, gg, matsto=matsto, mreqenf=1, mcount=0)
     lab, var = _check_maps(P, rp, ii, gg, mreqenf=1, mcount=0)
     gsl = P.labels()[3]
     assert len(set(gsl.tolist())) > len(set((lab // 10).tolist()))   # some module groups split
-    # The pass: chi^2, ndf, rejects, counters and vector equal the oracle's.  The matrix of such
-    # records is not compared: the device's Schur update for a label repeated inside one
-    # equation does not reproduce pede's (parity unpinned for that input).
-    nv = P.nvar
-    st = P.loopn()
-    mat, vec, cnt, so = orc.loopn(rp, ii, gg.astype(np.float64), lab, var, P.parameters(), nv)
-    assert st["nrej"] == so["nrej"] and st["ndf"] == so["ndf"]
-    assert abs(st["chi2"] - so["chi2"]) <= 1e-11 * abs(so["chi2"])
-    v, c = P.vector()
-    assert np.array_equal(c, cnt) and rel_err(v[:nv], vec) < 1e-10
+    _, mat, _ = _compare_pass(P, orc, rp, ii, gg, lab, var)
+    # The diagonal element of a repeated label: pede adds w d_j d_k over the word pairs k <= j
+    # of the equation (src/pede.f90:4196-4207), w (d1^2 + d1 d2 + d2^2) for two entries, where
+    # the summed derivative gives w (d1 + d2)^2.  The difference is far above the tolerance, so
+    # the comparison above tells the two apart (and catches a lost G contribution as well).
+    pos = {int(l): i for i, l in enumerate(lab)}
+    extra = np.zeros(P.nvar)
+    for eqs in LB.decode(rp, ii, gg):
+        for (sL, sG, lG) in eqs:
+            seen = {}
+            for w in range(sG + 1, lG + 1):
+                seen.setdefault(int(ii[w]), []).append(float(gg[w]))
+            for l, ds in seen.items():
+                if var[pos[l]] > 0:
+                    for j in range(len(ds)):
+                        for k in range(j):
+                            extra[var[pos[l]] - 1] += ds[j] * ds[k] / float(gg[sG]) ** 2
+    assert np.abs(extra).max() > 1e-4 * np.abs(packed_to_dense(mat, P.nvar)).max()
+
+
+@pytest.mark.parametrize("matsto", [mpgpu.FULL, mpgpu.SPARSE])
+def test_repeated_labels_band_pass(orc, matsto):
+    """Bordered-band local fits (sqmibb) over records with a label repeated inside an equation:
+    the band kernel's G rows sum both entries (the word pairs of one element summed by one
+    thread in the reference's order); the pass against the oracle."""
+    rp, ii, gg = mpgpu.brokenlines_records(35, 24, 3, 3, 900, 8, 24)
+    rp, ii, gg = _duplicate_first_global(rp, ii, gg, every=2)
+    recs = LB.decode(rp, ii, gg)
+    assert sum(_band_side(recs, ii, r) > 0 for r in range(len(recs))) > len(recs) // 2
+    P = _pede(rp, ii, gg, matsto=matsto, mreqenf=1, mcount=0)
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=1, mcount=0)
+    _compare_pass(P, orc, rp, ii, gg, lab, var)
 
 
 def test_scaleerrors(orc):
