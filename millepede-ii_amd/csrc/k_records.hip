@@ -857,6 +857,11 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         __syncthreads();
         const bool perq = sh[5] != 0;
+        // the two-phase build stores each variable's vector value in the update buffer (formed
+        // again where the row descriptors are written), for the block gather to sum in
+        // processing order; otherwise, and on the per-equation path, it is added to the vector
+        // directly
+        const bool vgather = o.icalcm == 1 && twophase;
         if (!perq) {
             for (int a = tid; a < nalg; a += RT) {
                 const int n = qn[a];
@@ -865,7 +870,7 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const int i = qe[a * QE + x];
                     v = v + dw1 * lw[i] * rm[i] * qg[a * QE + x];
                 }
-                atomicAdd(&M.vec[gvl[a] - 1], v);
+                if (!vgather) atomicAdd(&M.vec[gvl[a] - 1], v);
                 atomicAdd(&M.cnt[gvl[a] - 1], (unsigned long long)n);
             }
         } else {
@@ -968,8 +973,10 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 // Group-block storage.  Layout of a record's update: for every segment sa (in
                 // order) the columns b = 0 .. end of sa (all record variables up to and including
                 // segment sa), each column holding the na values of the segment's rows
-                // (value (ra, b) at b * na + ra); the diagonal block is kept square, its upper
-                // part is zero.  The block of segment pair (sa, sb) is thus contiguous
+                // (value (ra, b) at b * na + ra), then one column with the rows' vector values
+                // (padded to a multiple of four values, which keeps every region 32-byte aligned
+                // when the columns are); the diagonal block is kept square, its upper part is
+                // zero.  The block of segment pair (sa, sb) is thus contiguous
                 // (columns of sb), which the block gather reads.  The values
                 // C1(a,b) - sum_k Gh[a,k] G[b,k] (b <= a) of different segments come from 16 x 16
                 // tiles of the record's lower triangle, one tile per wavefront step, the products
@@ -1003,7 +1010,8 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     int run = 0;
                     for (int base = 0; base < nseg; base += WAVE) {
                         const int q = base + lane;
-                        int v = q < nseg ? segst[q + 1] * (segst[q + 1] - segst[q]) : 0;
+                        const int na = q < nseg ? segst[q + 1] - segst[q] : 0;
+                        int v = q < nseg ? segst[q + 1] * na + ((na + 3) & ~3) : 0;
                         int x = v;
                         for (int d = 1; d < WAVE; d <<= 1) {
                             const int y = __shfl_up(x, d, WAVE);
@@ -1023,6 +1031,18 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     rdesc[a] = make_int4(base, na, q, a0);
                     vsegm[a] = segm[q];
                     for (int b = a + 1; b < a0 + na; b++) dst[base + (int64_t)b * na] = 0.0;
+                    {
+                        // the segment's vector column: this variable's value (the per-equation
+                        // path added its values directly and leaves a zero; a per-variable loop
+                        // for it here costs the whole kernel registers)
+                        double v = 0.0;
+                        if (!perq)
+                            for (int x = 0; x < qn[a]; x++) {
+                                const int i = qe[a * QE + x];
+                                v = v + dw1 * lw[i] * rm[i] * qg[a * QE + x];
+                            }
+                        dst[base + (int64_t)(a0 + na) * na] = v;
+                    }
                 }
                 __syncthreads();
                 // diagonal blocks (a, b <= a in one segment), one element per thread: g g^T of the
@@ -1083,7 +1103,7 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // updates); diagonal blocks use their lower triangle.
 __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg,
                                                       const int32_t *bcol, const int32_t *gfirst, const int64_t *ptr,
-                                                      const int64_t *src, const double *cbuf, double *mat)
+                                                      const int64_t *src, const double *cbuf, double *mat, double *vec)
 {
     const int64_t k = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -1094,10 +1114,14 @@ __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int6
     const int nr = gfirst[ga + 1] - gfirst[ga], nc = gfirst[gb + 1] - gfirst[gb];
     const bool diag = ga == gb;
     double *B = mat + boff[k];
-    for (int e0 = 0; e0 < nr * nc; e0 += 64) {
+    // a diagonal block also sums the vector column that follows it in the records' updates
+    // (elements nr*nc .. nr*nc + nr - 1, column nc)
+    const int ne = nr * nc + (diag ? nr : 0);
+    for (int e0 = 0; e0 < ne; e0 += 64) {
         const int e = e0 + lane;
-        const bool act = e < nr * nc;
-        const int rr = act ? e / nc : 0, cc = act ? e - rr * nc : 0;
+        const bool act = e < ne, inm = e < nr * nc;
+        const int rr = !act ? 0 : inm ? e / nc : e - nr * nc;
+        const int cc = !act ? 0 : inm ? e - rr * nc : nc;
         const int se = cc * nr + rr;        // records store their blocks column by column
         // the contributions' offsets, 64 at a time, one per lane (a coalesced read), then eight
         // independent partial sums over them, each offset broadcast from its lane
@@ -1123,8 +1147,9 @@ __global__ __launch_bounds__(256) void k_block_gather(int64_t nblock, const int6
                 acc[0] += cbuf[o + se];
             }
         }
-        if (act && !(diag && cc > rr))
-            B[e] += ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        const double sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        if (act && inm && !(diag && cc > rr)) B[e] += sum;
+        if (act && !inm) vec[gfirst[ga] - 1 + rr] += sum;
     }
 }
 
@@ -1172,11 +1197,11 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
 
 hipError_t launch_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg, const int32_t *bcol,
                                const int32_t *gfirst, const int64_t *ptr, const int64_t *src, const double *cbuf,
-                               double *mat, hipStream_t s)
+                               double *mat, double *vec, hipStream_t s)
 {
     if (nblock <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_block_gather, dim3((unsigned)((nblock + 3) / 4)), dim3(256), 0, s, nblock, boff, brg, bcol, gfirst,
-                       ptr, src, cbuf, mat);
+                       ptr, src, cbuf, mat, vec);
     return hipGetLastError();
 }
 

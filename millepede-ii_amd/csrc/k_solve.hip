@@ -2329,16 +2329,58 @@ __global__ __launch_bounds__(64) void k_spk_local(int n, int nch, const double *
 {
     __shared__ double sc[SPK_CH * (W + 1)];   // row r = k - k0: element (k, k-W+o) at [r*(W+1) + o]
     __shared__ double sx[SPK_CH];
+    __shared__ int si[SPK_CH + 1];            // india of rows k0-1 .. k1 (0 for row 0)
     const int ch = blockIdx.x, lane = threadIdx.x;
     if (ch >= nch) return;
     const int k0 = ch * SPK_CH + 1, k1 = min(n, k0 + SPK_CH - 1), L = k1 - k0 + 1;
-    for (int t = lane; t < L * (W + 1); t += 64) {
-        const int r = t / (W + 1), o = t - r * (W + 1);
-        const int k = k0 + r, j = k - W + o;
-        const int ik = india[k - 1], len = ik - (k > 1 ? india[k - 2] : 0);
-        sc[t] = (j >= k - len + 1 && j >= k0) ? c[ik - k + j - 1] : 0.0;
+    // every lane issues all of its loads of a stage before it waits on any of them
+    {
+        constexpr int NI = (SPK_CH + 1 + 63) / 64, NX = (SPK_CH + 63) / 64;
+        int iv[NI];
+        double xv[NX];
+#pragma unroll
+        for (int u = 0; u < NI; u++) {
+            const int r = lane + 64 * u, row = k0 - 1 + r;
+            const bool ok = r <= L && row >= 1;
+            const int ia = india[ok ? row - 1 : k0 - 1];
+            iv[u] = ok ? ia : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int r = lane + 64 * u;
+            const double xa = x[k0 - 1 + (r < L ? r : 0)];
+            xv[u] = r < L ? xa : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NI; u++)
+            if (lane + 64 * u <= L) si[lane + 64 * u] = iv[u];
+#pragma unroll
+        for (int u = 0; u < NX; u++)
+            if (lane + 64 * u < L) sx[lane + 64 * u] = xv[u];
     }
-    for (int r = lane; r < L; r += 64) sx[r] = x[k0 + r - 1];
+    __syncthreads();
+    constexpr int NB = 8;
+    for (int t0 = 0; t0 < L * (W + 1); t0 += 64 * NB) {
+        double v[NB];
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            // unconditional load at a clamped index (the row's diagonal when outside the band)
+            const int t = t0 + lane + 64 * u;
+            const bool tv = t < L * (W + 1);
+            const int tt = tv ? t : 0;
+            const int r = tt / (W + 1), o = tt - r * (W + 1);
+            const int k = k0 + r, j = k - W + o;
+            const int ik = si[r + 1], len = ik - si[r];
+            const bool in = tv && j >= k - len + 1 && j >= k0;
+            const double cv = c[in ? ik - k + j - 1 : ik - 1];
+            v[u] = in ? cv : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int t = t0 + lane + 64 * u;
+            if (t < L * (W + 1)) sc[t] = v[u];
+        }
+    }
     __syncthreads();
     if (lane == 0) {
         double xw[W + 1];
@@ -2550,8 +2592,22 @@ __global__ void k_eqs_cdot(int n, const int *cb, const int64_t *koff, const doub
     int j = blockIdx.x;
     __shared__ double red[TPB];
     int i0 = cb[2 * j], in = cb[2 * j + 1];
+    const double *kj = kmat + koff[j];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < in; i += TPB) if (i0 + i < n) acc += kmat[koff[j] + i] * x[i0 + i];
+    // eight terms' loads issued together, the terms added in the same order as one at a time
+    for (int i = threadIdx.x; i < in; i += TPB * 8) {
+        double kv[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int iu = i + u * TPB;
+            const bool ok = iu < in && i0 + iu < n;
+            kv[u] = kj[ok ? iu : 0];                 // clamped: loads unconditional
+            xv[u] = x[ok ? i0 + iu : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (i + u * TPB < in && i0 + i + u * TPB < n) acc += kv[u] * xv[u];
+    }
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int o = TPB / 2; o > 0; o >>= 1) {
@@ -2570,7 +2626,15 @@ __global__ __launch_bounds__(64) void k_eqs_small(int n, int m, const double *h,
     const int nh = m > 0 ? hind[m - 1] : 0;
     const bool fit = nh + m <= EQS_LDS && m <= EQS_LDS / 2;
     if (fit) {
-        for (int t = threadIdx.x; t < nh; t += 64) sh_h[t] = h[t];
+        // loads in batches of 16 per lane, all issued before the first is used
+        for (int t0 = threadIdx.x; t0 < nh; t0 += 64 * 16) {
+            double v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) v[u] = h[t0 + 64 * u < nh ? t0 + 64 * u : nh - 1];
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                if (t0 + 64 * u < nh) sh_h[t0 + 64 * u] = v[u];
+        }
         for (int t = threadIdx.x; t < m; t += 64) {
             sh_h[nh + t] = d[t];
             sh_i[t] = hind[t];
@@ -2617,16 +2681,44 @@ __global__ __launch_bounds__(64) void k_eqs_small(int n, int m, const double *h,
 }
 // x_i -= K_ji lambda_j over the constraints covering i in ascending j (the order of equslvs'
 // sequential loop), one thread per parameter: deterministic, no atomics
-__global__ void k_eqs_upd(int n, int m, const int *cb, const int64_t *koff, const double *kmat, double *x)
+__global__ __launch_bounds__(TPB) void k_eqs_upd(int n, int m, const int *cb, const int64_t *koff, const double *kmat,
+                                                  double *x)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double xi = x[i];
-        for (int j = 0; j < m; j++) {
-            const int i0 = cb[2 * j], in = cb[2 * j + 1];
-            if (i >= i0 && i < i0 + in) xi = xi - kmat[koff[j] + (i - i0)] * x[n + j];
+    // the constraints' ranges, offsets and multipliers are staged in LDS, TPB at a time; only the
+    // constraints whose range meets the workgroup's parameters are kept
+    __shared__ int s0[TPB], sn[TPB];
+    __shared__ int64_t so[TPB];
+    __shared__ double sl[TPB];
+    __shared__ int cnt, wcnt[TPB / 64];
+    const int ib = blockIdx.x * TPB, i = ib + threadIdx.x;
+    double xi = i < n ? x[i] : 0.0;
+    for (int j0 = 0; j0 < m; j0 += TPB) {
+        __syncthreads();
+        const int j = j0 + threadIdx.x;
+        int a = 0, b = 0;
+        if (j < m) { a = cb[2 * j]; b = cb[2 * j + 1]; }
+        const bool meets = j < m && a < ib + TPB && a + b > ib;
+        // keep ascending constraint order: position = number of meeting constraints before j
+        const unsigned long long bal = __ballot(meets);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0) wcnt[wv] = __popcll(bal);
+        __syncthreads();
+        int pos = __popcll(bal & ((1ull << lane) - 1ull));
+        for (int w = 0; w < wv; w++) pos += wcnt[w];
+        if (meets) {
+            s0[pos] = a;
+            sn[pos] = b;
+            so[pos] = koff[j];
+            sl[pos] = x[n + j];
         }
-        x[i] = xi;
+        if (threadIdx.x == TPB - 1) cnt = pos + (meets ? 1 : 0);
+        __syncthreads();
+        const int nc = cnt;
+        if (i < n)
+            for (int c = 0; c < nc; c++)
+                if (i >= s0[c] && i < s0[c] + sn[c]) xi = xi - kmat[so[c] + (i - s0[c])] * sl[c];
     }
+    if (i < n) x[i] = xi;
 }
 
 namespace {

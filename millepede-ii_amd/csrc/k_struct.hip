@@ -483,10 +483,13 @@ __global__ __launch_bounds__(SW *STRUCT_WAVES) void k_rec_stats(int64_t nrec, in
         for (int k = lane; k < ngl; k += SW) qe = max(qe, qc[k]);
         qe = wmax(qe);
         if (lane == 0) {
+            // segment a's region: the columns of all variables up to its end, then the column of
+            // its rows' vector values, padded to a multiple of four values (the regions keep the
+            // 32-byte alignment that the columns alone give them)
             int64_t n = 0, upto = 0;
             for (int a = 0; a < nseg; a++) {
                 upto += s_z[a];
-                n += (int64_t)s_z[a] * upto;
+                n += (int64_t)s_z[a] * upto + ((s_z[a] + 3) & ~3);
             }
             cnt_r[r] = n;
             st[r] = nv;
@@ -634,7 +637,9 @@ __global__ void k_batch_npair(int64_t q0, int64_t q1, const int64_t *order, cons
 }
 // (block, buffer offset) of every segment pair of the batch's records: the record's update lies
 // at rec_cb[r]; segment a's region holds the columns of all variables up to its end (sz[a]
-// values each), so pair (a, b) starts at region(a) + (first column of b) * sz[a].  LDS per
+// values each) and then the column of its vector values (padded to a multiple of four), so
+// pair (a, b) starts at region(a) + (first column of b) * sz[a] and the vector column follows
+// the diagonal pair (a, a).  LDS per
 // wavefront: s_reg [cap] i64, s_g, s_z, s_col [cap] i32.
 __global__ __launch_bounds__(SW *STRUCT_WAVES) void k_incidence(int64_t q0, int64_t q1, int cap, const int64_t *order,
                                                                const int64_t *rec_gl, const int32_t *gl_tix,
@@ -663,7 +668,7 @@ __global__ __launch_bounds__(SW *STRUCT_WAVES) void k_incidence(int64_t q0, int6
                 s_col[a] = col;
                 col += s_z[a];
                 upto += s_z[a];
-                reg += (int64_t)s_z[a] * upto;
+                reg += (int64_t)s_z[a] * upto + ((s_z[a] + 3) & ~3);
             }
         }
         wsync();

@@ -1495,7 +1495,7 @@ extern "C" int mpgpu_loopn(mpgpu_ctx *c, int32_t icalcm, int32_t nloopn, float c
                 HIPCHK(hipEventRecord(tg.first, s));
                 HIPCHK(launch_block_gather(c->nblock, c->d_boff.p, c->d_brg.p, c->d_bcol.p, c->d_gfirst.p,
                                            c->d_inc_ptr.p + bt * (c->nblock + 1), c->d_inc_src.p + B.inc0, c->d_cbuf.p,
-                                           c->d_mat.p, s));
+                                           c->d_mat.p, c->d_vec.p, s));
                 HIPCHK(hipEventRecord(tg.second, s));
             }
         } else {

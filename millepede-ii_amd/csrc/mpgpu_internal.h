@@ -122,7 +122,7 @@ hipError_t launch_records(const DevRecords &R, const DevModel &M, const PassOpts
 int lds_bytes(LdsPlan &P);
 hipError_t launch_block_gather(int64_t nblock, const int64_t *boff, const int32_t *brg, const int32_t *bcol,
                                const int32_t *gfirst, const int64_t *ptr, const int64_t *src, const double *cbuf,
-                               double *mat, hipStream_t s);
+                               double *mat, double *vec, hipStream_t s);
 hipError_t launch_records_band(const DevRecords &R, const DevModel &M, const PassOpts &o, const RecOut &out,
                                const BandPlan &P, const int64_t *ids, int64_t nb, const int32_t *binfo,
                                double *scratch, int grid, hipStream_t s);

@@ -42,6 +42,33 @@ def test_alone_vs_in_the_mix():
     S.close()
 
 
+def test_group_block_job_bitwise_alone_vs_in_the_mix():
+    """Group-block storage has no FP64 atomics on the path of a pixel job: the matrix and the
+    vector are summed per block in processing order (the vector values ride in the records'
+    updates), the product, the preconditioner and the MINRES reductions use fixed orders, and
+    the product variants the autotuner picks between are bit-identical.  The same job alone and
+    among concurrent jobs gives the bit-identical solution."""
+    rec, par, cons = _job(3, 6, 4, 3, 3000)
+    opt = dict(matsto=mpgpu.SPARSE, metsol=mpgpu.MINRES, mbandw=6, mreqenf=5, mcount=1, mrestl=1e-12)
+    S = mpgpu.Server(nworkers=4, budget_bytes=8 << 30)
+    rc, st, lab0, x0 = S.wait(S.submit(rec, par, cons, **opt), capacity=100000)
+    assert rc == 0 and st["nvar"] > 0
+    others = [_job(20 + k, 5 + k % 3, 3 + k % 2, 3, 1500 + 300 * k) for k in range(6)]
+    ids = []
+    for k, (r2, p2, c2) in enumerate(others):
+        ids.append(S.submit(r2, p2, c2, **opt))
+        if k == 2:
+            jm = S.submit(rec, par, cons, **opt)
+    for i in ids:
+        rc2, st2, _, _ = S.wait(i)
+        assert rc2 == 0, st2
+    rc, st, lab1, x1 = S.wait(jm, capacity=100000)
+    assert rc == 0
+    assert np.array_equal(lab0, lab1)
+    assert np.array_equal(x0, x1)
+    S.close()
+
+
 def test_budget_rejects_oversized_job():
     """A job larger than the whole budget ends with status 30 (the worker that takes it sizes
     it); the server keeps serving the next ones."""
