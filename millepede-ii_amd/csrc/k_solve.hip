@@ -2259,7 +2259,7 @@ void l_equslvs(int n, int m, const double *c, const int *india, const int *cb, c
 // Layout: row k (1-based) of the factor holds columns k-len+1..k at c[india(k)-k+j-1],
 // len = india(k)-india(k-1) (india(0)=0).
 // ---------------------------------------------------------------------------------------------
-#define SPK_CH 128
+#define SPK_CH 256
 #define SPK_WMAX 16
 
 __device__ __forceinline__ int spk_len(const int *india, int k) { return india[k - 1] - (k > 1 ? india[k - 2] : 0); }
@@ -2779,6 +2779,7 @@ void l_spk_build(int n, int w, const double *c, const int *india, double *Sf, do
     int nch = (n + SPK_CH - 1) / SPK_CH;
     hipLaunchKernelGGL(k_spk_build, dim3(g1((int64_t)nch * w, 64)), dim3(64), 0, s, n, w, nch, c, india, Sf, Sb, scr);
 }
+int spk_chunk_rows() { return SPK_CH; }
 // scan levels of the carry (nlev = ceil(log2 nch)); A holds nlev * nch * w * w values per direction
 int spk_scan_levels(int n, int w)
 {

@@ -1846,17 +1846,17 @@ void setup_precond(mpgpu_ctx *c, Precond &P)
         l_band_k(P.m, P.n, P.band.p, P.india.p, P.cb.p, P.koff.p, P.kmat.p, mb, s);
         l_band_h(P.m, P.cb.p, P.koff.p, P.kmat.p, P.h.p, P.hind.p, P.nrkd.p + 1, s);
     }
-    // chunk-parallel band solves (response matrices of 128-row chunks)
+    // chunk-parallel band solves (response matrices of chunks of rows)
     P.w = mb - 1;
     P.spike = P.w >= 1 && P.w <= 16;
     if (P.spike) {
-        const int nch = (P.n + 127) / 128;
+        const int ch = spk_chunk_rows(), nch = (P.n + ch - 1) / ch;
         P.Sf.alloc((size_t)P.n * P.w);
         P.Sb.alloc((size_t)P.n * P.w);
         P.tbuf.alloc((size_t)nch * P.w);
         P.dtmp.alloc(std::max(1, P.m));
         DBuf<double> scr;
-        scr.alloc((size_t)nch * P.w * 128);
+        scr.alloc((size_t)nch * P.w * ch);
         l_spk_build(P.n, P.w, P.band.p, P.india.p, P.Sf.p, P.Sb.p, scr.p, s);
         P.nlev = spk_scan_levels(P.n, P.w);
         if (P.nlev > 0) {

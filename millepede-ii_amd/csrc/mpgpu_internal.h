@@ -188,6 +188,7 @@ void l_spk_build(int n, int w, const double *c, const int *india, double *Sf, do
 void l_equslvs_spike(int n, int m, int w, const double *c, const int *india, const double *Sf, const double *Sb,
                      double *tbuf, const int *cb, const int64_t *koff, const double *kmat, const double *h, const int *hind,
                      double *dtmp, double *x, const double *Af, const double *Ab, int nlev, hipStream_t s);
+int spk_chunk_rows();   // rows per chunk of the chunk-parallel band solves
 int spk_scan_levels(int n, int w);
 void l_spk_scan_build(int n, int w, const double *Sf, const double *Sb, double *Af, double *Ab, int nlev, hipStream_t s);
 
