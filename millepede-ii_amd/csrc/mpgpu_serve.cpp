@@ -258,7 +258,9 @@ int mpgpu_server_create(int32_t device, int32_t nworkers, int64_t budget_bytes, 
     s->device = device;
     s->budget = budget_bytes;
     s->batch_max_params = batch_max_params;
-    s->omp_threads = std::max(1, omp_get_num_procs() / nworkers);
+    // the host threads the process may use (OMP_NUM_THREADS when set: the cores a process on a
+    // shared host is given, which may be far fewer than the processors it sees), shared out
+    s->omp_threads = std::max(1, std::min(omp_get_num_procs(), omp_get_max_threads()) / nworkers);
     for (int i = 0; i < nworkers; i++) s->workers.emplace_back([s] { s->worker(); });
     *out = s;
     return 0;
