@@ -2386,19 +2386,29 @@ __global__ __launch_bounds__(64) void k_spk_local(int n, int nch, const double *
         double xw[W + 1];
 #pragma unroll
         for (int q = 0; q <= W; q++) xw[q] = 0.0;
+        // the next row's band values (and right-hand side) are read from LDS one row ahead
+        double rw[W + 1], nw[W + 1];
         if (!backward) {
             // at row k: xw[q+1] = x_{k-W+q}
-#pragma unroll 2
-            for (int r = 0; r < L; r++) {
-                const double *row = sc + r * (W + 1);
-                double xk = sx[r];
 #pragma unroll
-                for (int q = 0; q < W; q++) xk = xk - row[q] * xw[q + 1];
-                xk = xk * row[W];
+            for (int q = 0; q <= W; q++) rw[q] = sc[q];
+            double xs = sx[0];
+            for (int r = 0; r < L; r++) {
+                const int rn = r + 1 < L ? r + 1 : r;
+#pragma unroll
+                for (int q = 0; q <= W; q++) nw[q] = sc[rn * (W + 1) + q];
+                const double nxs = sx[rn];
+                double xk = xs;
+#pragma unroll
+                for (int q = 0; q < W; q++) xk = xk - rw[q] * xw[q + 1];
+                xk = xk * rw[W];
                 sx[r] = xk;
 #pragma unroll
                 for (int q = 0; q < W; q++) xw[q] = xw[q + 1];
                 xw[W] = xk;
+#pragma unroll
+                for (int q = 0; q <= W; q++) rw[q] = nw[q];
+                xs = nxs;
             }
         } else {
             // at row k: xw[q] = pending x_{k-W+q} (q < W), xw[W] = pending x_k
@@ -2407,17 +2417,23 @@ __global__ __launch_bounds__(64) void k_spk_local(int n, int nch, const double *
                 const int r = L - 1 - W + q;
                 xw[q] = r >= 0 ? sx[r] : 0.0;
             }
-#pragma unroll 2
+#pragma unroll
+            for (int q = 0; q <= W; q++) rw[q] = sc[(L - 1) * (W + 1) + q];
             for (int r = L - 1; r >= 0; r--) {
-                const double *row = sc + r * (W + 1);
-                const double xk = xw[W] * row[W];
+                const int rp = r > 0 ? r - 1 : 0;
+#pragma unroll
+                for (int q = 0; q <= W; q++) nw[q] = sc[rp * (W + 1) + q];
+                const int rn = r - 1 - W;
+                const double pend = rn >= 0 ? sx[rn] : 0.0;
+                const double xk = xw[W] * rw[W];
                 sx[r] = xk;
 #pragma unroll
-                for (int q = 0; q < W; q++) xw[q] = xw[q] - row[q] * xk;
+                for (int q = 0; q < W; q++) xw[q] = xw[q] - rw[q] * xk;
 #pragma unroll
                 for (int q = W; q > 0; q--) xw[q] = xw[q - 1];
-                const int rn = r - 1 - W;
-                xw[0] = rn >= 0 ? sx[rn] : 0.0;
+                xw[0] = pend;
+#pragma unroll
+                for (int q = 0; q <= W; q++) rw[q] = nw[q];
             }
         }
     }

@@ -73,6 +73,28 @@ def test_pixel_pass(orc, matsto):
     _compare_pass(P, orc, rp, ii, gg, lab, var)
 
 
+def test_two_phase_build_in_batches(orc, monkeypatch):
+    """Group-block storage with an update buffer far smaller than the pass (8 MB): the records
+    are taken in several batches, each followed by its block gather, which adds the batch's sums
+    of matrix and vector values to the blocks.  Matrix, vector, chi^2, ndf, rejects and counters
+    against the oracle; the batched pass equals the one-batch pass within FP64 rounding."""
+    rp, ii, gg, _ = mpgpu.pixel_records(19, 4, 6, 3, 3000, 3, 6)
+    opt = dict(matsto=mpgpu.SPARSE, mreqenf=5, mcount=1, metsol=4)
+    P1 = _pede(rp, ii, gg, **opt)
+    s1 = P1.loopn()
+    assert s1["n_gather"] == 1
+    monkeypatch.setenv("MPGPU_UPDATE_BUFFER_GB", "0.0078125")
+    P = _pede(rp, ii, gg, **opt)
+    lab, var = _check_maps(P, rp, ii, gg, mreqenf=5, mcount=1)
+    s, _, _ = _compare_pass(P, orc, rp, ii, gg, lab, var)
+    assert s["n_gather"] >= 2
+    n = P.nall
+    assert rel_err(packed_to_dense(P.matrix_packed(), n), packed_to_dense(P1.matrix_packed(), n)) < 1e-13
+    assert rel_err(P.vector()[0], P1.vector()[0]) < 1e-13
+    P.close()
+    P1.close()
+
+
 def test_downweighting_pass(orc):
     rp, ii, gg, _ = mpgpu.toy_records(5, 4000)
     gg = gg.copy()
