@@ -276,7 +276,8 @@ int mpgpu_run_steering(const char *steering_file, const char *output_dir, int32_
  * lines (label, start value, pre-sigma) and Lagrange constraints (CSR: cptr[ncon+1]).
  * mpgpu_server_submit only queues the job: its arrays are not copied and must stay valid and
  * unchanged until mpgpu_server_wait has returned for it; the worker that takes the job sizes
- * it (status 30 when it exceeds the whole budget) and starts it when it fits. */
+ * it (status 30 when it exceeds the whole budget) and starts it when it fits.  Every job is
+ * solved in its own context: batch_max_params is accepted and not used. */
 typedef struct mpgpu_server mpgpu_server;
 typedef struct mpgpu_job_stats {
     int32_t status, nvar;
