@@ -113,6 +113,46 @@ __global__ void k_final_sum(int n, const double *part, double *out)
     if (threadIdx.x == 0) *out = s[0];
 }
 
+// dot product in one launch: each workgroup's partial sum as k_partial_dot, then the last
+// workgroup to finish (a counter in part[DOT_CNT], zero between calls) adds the partial sums in
+// k_final_sum's order, so the result is bit-identical to the two-launch form
+#define DOT_CNT 2047
+__global__ __launch_bounds__(TPB) void k_dot_fused(int64_t n, const double *a, const double *b, double *part, double *out)
+{
+    __shared__ double s[TPB];
+    __shared__ int last;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+        acc += a[i] * (b ? b[i] : 1.0);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    unsigned int *cnt = (unsigned int *)(part + DOT_CNT);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&part[blockIdx.x], s[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned int done = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    double acc2 = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += TPB)
+        acc2 += __hip_atomic_load(&part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s[threadIdx.x] = acc2;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out = s[0];
+        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // pass statistics: chi2 (double), ndf (int64), status histogram [0..20]
 __global__ void k_pass_stats(int64_t n, const double *chi2, const int32_t *ndf, const int8_t *status, double *part,
                              long long *ndf_sum, unsigned long long *hist)
@@ -1885,14 +1925,13 @@ void l_get_diag(const DevModel &M, int32_t n, double *d, hipStream_t s)
 void l_get_band(const DevModel &M, int32_t n, int32_t mbandw, const int32_t *india, double *c, hipStream_t s)
 { hipLaunchKernelGGL(k_get_band, dim3(nblk(n)), dim3(TPB), 0, s, M, n, mbandw, india, c); }
 
-// dot product into *out (device), deterministic; part: >= 1024 doubles
+// dot product into *out (device), deterministic; part: >= 2048 doubles, part[2047] zero between calls
 void l_dot(int64_t n, const double *a, const double *b, double *part, double *out, hipStream_t s)
 {
     int g = (int)((n + TPB - 1) / TPB);
     if (g > 1024) g = 1024;
     if (g < 1) g = 1;
-    hipLaunchKernelGGL(k_partial_dot, dim3(g), dim3(TPB), 0, s, n, a, b, part);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, g, part, out);
+    hipLaunchKernelGGL(k_dot_fused, dim3(g), dim3(TPB), 0, s, n, a, b, part, out);
 }
 void l_pass_stats(int64_t n, const double *chi2, const int32_t *ndf, const int8_t *st, double *part, long long *ndfsum,
                   unsigned long long *hist, double *chi_out, hipStream_t s)

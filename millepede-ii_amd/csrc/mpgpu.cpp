@@ -1253,6 +1253,7 @@ extern "C" int mpgpu_build_structure(mpgpu_ctx *c, int32_t *ntot_out, int32_t *n
     c->d_ndf.alloc(std::max<int64_t>(1, nrec));
     c->d_status.alloc(std::max<int64_t>(1, nrec));
     c->d_part.alloc(2048);
+    l_zero(1, c->d_part.p + 2047, s);       // the completion counter of the one-launch dot product
     c->d_scal.alloc(16);
     if (!sparse) c->d_psymv.alloc(packed_symv_scratch(c->nall));
     c->d_ndfsum.alloc(1);
